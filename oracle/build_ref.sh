@@ -1,0 +1,81 @@
+#!/usr/bin/env bash
+# Builds oracle/_ref/mom_ref: the reference Fortran (jmakitalo/for90-mom2,
+# read from /root/reference) linked with our own driver ref_driver.f90,
+# which dumps golden vectors (mesh numbering, computeD/H/K, sysmat_pmchwt,
+# srcvec, cs_prtsrf, rcs) at full precision.
+#
+# TEST INFRASTRUCTURE ONLY.  The reference source never enters the
+# repository: this script compiles it from a scratch copy outside the repo
+# and deletes the copy; only the linked binary lands in oracle/_ref/ (which
+# .gitignore lists).
+#
+# The reference is written for ifort; the image's only Fortran compiler is
+# AMD flang, which rejects two ifort extensions.  The scratch copy gets these
+# mechanical dialect fixes, nothing else:
+#   1. LOGICAL compared with ==, /= against .TRUE./.FALSE. -> .EQV./.NEQV.
+#   2. rank-1 array constructors initialising rank-2 PARAMETERs get RESHAPE
+#      (symmetry.f90:16-18 id33r, mesh.f90:976 cmp_triplets indices).
+# No LAPACK is linked: the driver never calls the reference's solve routines
+# (the LU is LAPACK, a third-party library absent from the image), so their
+# six LAPACK references (zgetrf zgetrs zgetri zgeev zggev zgemm) are left
+# unresolved at link time; nothing on the driver's path reaches them.
+set -euo pipefail
+REF=${REF:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+FC=${FC:-/opt/rocm/llvm/bin/flang}
+if [ ! -d "$REF" ]; then echo "reference not present: $REF" >&2; exit 2; fi
+mkdir -p "$OUT"
+WORK=$(mktemp -d /tmp/momref.XXXXXX)
+trap 'rm -rf "$WORK"' EXIT
+
+MODS="constants linalg aux mesh quad rwgf green greenprd symmetry int srcint nfields dipole bessel source bc nlsurf nlbulk common sysmat ffields cs"
+# Fix 1 wraps the primary left of the operator in parentheses, so that the
+# operator keeps the precedence of == (a relational operator binds tighter
+# than .AND., .EQV. does not): `A .AND. P(x)==.FALSE.` becomes
+# `A .AND. (P(x) .EQV. .FALSE.)`.
+for m in $MODS; do
+  python3 - "$REF/$m.f90" "$WORK/$m.f90" <<'PYEOF'
+import re, sys
+src = open(sys.argv[1]).read()
+pat = re.compile(r'(==|/=)\s*\.(TRUE|FALSE)\.', re.I)
+out, pos = [], 0
+for mt in pat.finditer(src):
+    i = mt.start()
+    j = i
+    while j > 0 and src[j - 1] == ' ':
+        j -= 1
+    depth = 0
+    while j > 0:
+        c = src[j - 1]
+        if c == ')':
+            depth += 1
+        elif c == '(':
+            if depth == 0:
+                break
+            depth -= 1
+        elif depth == 0 and not (c.isalnum() or c in '_%'):
+            break
+        j -= 1
+    op = '.EQV.' if mt.group(1) == '==' else '.NEQV.'
+    out.append(src[pos:j])
+    out.append('(' + src[j:i].rstrip() + ' ' + op + ' .' + mt.group(2).upper() + '.)')
+    pos = mt.end()
+out.append(src[pos:])
+open(sys.argv[2], 'w').write(''.join(out))
+PYEOF
+done
+# rank-2 PARAMETER constructors
+sed -i -E '16s/id33r = \(\//id33r = RESHAPE((\//; 18s/1\.0_dp\/\)/1.0_dp\/),(\/3,3\/))/' "$WORK/symmetry.f90"
+sed -i -E '976s/indices = \((\/.*\/)\)/indices = RESHAPE((\1),(\/3,6\/))/' "$WORK/mesh.f90"
+
+FFLAGS="-cpp -O2 -fopenmp"
+cd "$WORK"
+OBJS=""
+for m in $MODS; do
+  $FC $FFLAGS -c "$m.f90" -o "$m.o"
+  OBJS="$OBJS $m.o"
+done
+$FC $FFLAGS -c "$HERE/ref_driver.f90" -o ref_driver.o
+$FC -fopenmp -o "$OUT/mom_ref" ref_driver.o $OBJS -Wl,--unresolved-symbols=ignore-in-object-files -Wl,-z,execstack
+echo "built $OUT/mom_ref"

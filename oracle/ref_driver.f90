@@ -1,0 +1,182 @@
+! ref_driver.f90 -- golden-vector driver linked against the reference's own
+! modules (built by oracle/build_ref.sh).  TEST INFRASTRUCTURE ONLY.
+!
+! It performs the set-up of the `mesh`, `symm 1 id`, `sdom`, `fdom` commands
+! (interface.f90:44-123, 160-210, 357-366) and then calls the reference
+! routines on the hot path, dumping their outputs as raw little-endian
+! stream files:
+!   mesh_<d>.bin   set-up arrays of the master mesh (d=0) and each domain
+!   op_<d>_<X>.bin computeD/H/K (X = D, H, K) of domain d, E x E complex
+!   A.bin          sysmat_pmchwt, 2N x 2N complex (column-major)
+!   q.bin          srcvec of the plane wave, 2N complex
+!   post.bin       csca, cabs (cs_prtsrf) and rcs(ntheta,nphi) for the
+!                  solution read from x.bin (if present)
+!   times.txt      omp_get_wtime seconds of sysmat_pmchwt and srcvec
+!
+! Input on stdin, one item per line:
+!   mesh file / scale / ndom / per domain: "nsurf id1 id2 ..." /
+!   wavelength [m] / per domain: "re im" refractive index /
+!   "theta phi psi" [deg] / "dump_ops dump_sysmat" (0/1) /
+!   "ntheta nphi" / output directory
+PROGRAM ref_driver
+  USE sysmat
+  USE cs
+  USE ffields
+  USE omp_lib
+  IMPLICIT NONE
+
+  TYPE(batch) :: b
+  CHARACTER(LEN=512) :: mshfile, outdir, ln
+  REAL(KIND=dp) :: scale, wl, omega, th, ph, ps, t0, tfill, tsrc, csca, cabs, re, im
+  INTEGER :: ndom, d, nsurf, i, ids(16), dump_ops, dump_sys, nth, nph, N, u, ios
+  TYPE(srcdata) :: src
+  COMPLEX(KIND=dp), DIMENSION(:,:,:), ALLOCATABLE :: A
+  COMPLEX(KIND=dp), DIMENSION(:,:), ALLOCATABLE :: q, M
+  COMPLEX(KIND=dp), DIMENSION(:), ALLOCATABLE :: x
+  REAL(KIND=dp), DIMENSION(:,:), ALLOCATABLE :: rcsdata
+  INTEGER, DIMENSION(:), POINTER :: vol_ids
+  TYPE(prdnfo), POINTER :: prd
+  LOGICAL :: have_x
+  CHARACTER(LEN=1), DIMENSION(3) :: opn = (/'D','H','K'/)
+
+  READ(*,'(A)') mshfile
+  READ(*,*) scale
+  READ(*,*) ndom
+
+  CALL batch_defaults(b)
+  CALL group_id(b%ga)
+
+  b%mesh = load_mesh(TRIM(mshfile))
+  CALL build_mesh(b%mesh, scale)
+
+  ALLOCATE(b%domains(ndom))
+  vol_ids => NULL()
+  DO d=1,ndom
+     READ(*,'(A)') ln
+     READ(ln,*) nsurf
+     READ(ln,*) nsurf, ids(1:nsurf)
+     b%domains(d)%medium_index = d
+     b%domains(d)%gf_index = -1
+     b%domains(d)%mesh = extract_submesh(b%mesh, ABS(ids(1:nsurf)), vol_ids)
+     DO i=1,nsurf
+        IF(ids(i)<0) CALL invert_faces(b%domains(d)%mesh, ABS(ids(i)))
+     END DO
+     CALL build_mesh(b%domains(d)%mesh, 1.0_dp)
+  END DO
+  CALL determine_edge_couples(b%mesh, 1D-12)
+  CALL submesh_edge_connectivity(b%mesh, b%domains(:)%mesh)
+  CALL orient_basis(b%mesh, b%domains(:)%mesh)
+
+  READ(*,*) wl
+  b%nwl = 1
+  ALLOCATE(b%sols(1))
+  b%sols(1)%wl = wl
+  ALLOCATE(b%media(ndom))
+  DO d=1,ndom
+     READ(*,*) re, im
+     b%media(d)%type = mtype_linear
+     ALLOCATE(b%media(d)%prop(1))
+     b%media(d)%prop(1)%ri = CMPLX(re, im, KIND=dp)
+  END DO
+  READ(*,*) th, ph, ps
+  src%type = src_pw
+  src%pos(:) = 0.0_dp
+  src%theta = th*degtorad
+  src%phi = ph*degtorad
+  src%psi = ps*degtorad
+  READ(*,*) dump_ops, dump_sys
+  READ(*,*) nth, nph
+  READ(*,'(A)') outdir
+
+  N = b%mesh%nedges
+  omega = 2.0_dp*pi*c0/wl
+  prd => NULL()
+
+  CALL dump_mesh(b%mesh, TRIM(outdir)//'/mesh_0.bin')
+  DO d=1,ndom
+     WRITE(ln,'(A,A,I0,A)') TRIM(outdir), '/mesh_', d, '.bin'
+     CALL dump_mesh(b%domains(d)%mesh, TRIM(ln))
+  END DO
+
+  IF(dump_ops==1) THEN
+     DO d=1,ndom
+        ALLOCATE(M(b%domains(d)%mesh%nedges, b%domains(d)%mesh%nedges))
+        DO i=1,3
+           IF(i==1) CALL computeD(omega, b%media(d)%prop(1)%ri, b%domains(d)%mesh, b%ga(1), prd, b%qd_tri, M)
+           IF(i==2) CALL computeH(omega, b%media(d)%prop(1)%ri, b%domains(d)%mesh, b%ga(1), prd, b%qd_tri, M)
+           IF(i==3) CALL computeK(omega, b%media(d)%prop(1)%ri, b%domains(d)%mesh, b%ga(1), prd, b%qd_tri, M)
+           WRITE(ln,'(A,A,I0,A,A,A)') TRIM(outdir), '/op_', d, '_', opn(i), '.bin'
+           OPEN(NEWUNIT=u, FILE=TRIM(ln), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+           WRITE(u) M
+           CLOSE(u)
+        END DO
+        DEALLOCATE(M)
+     END DO
+  END IF
+
+  ALLOCATE(q(2*N,1))
+  t0 = omp_get_wtime()
+  CALL srcvec(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, src, b%qd_tri, q)
+  tsrc = omp_get_wtime() - t0
+  OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/q.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+  WRITE(u) q(:,1)
+  CLOSE(u)
+
+  tfill = -1.0_dp
+  IF(dump_sys==1) THEN
+     ALLOCATE(A(2*N,2*N,1))
+     t0 = omp_get_wtime()
+     CALL sysmat_pmchwt(b, 1, A)
+     tfill = omp_get_wtime() - t0
+     OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/A.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+     WRITE(u) A(:,:,1)
+     CLOSE(u)
+     DEALLOCATE(A)
+  END IF
+
+  OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/times.txt', STATUS='REPLACE')
+  WRITE(u,'(A,ES25.17)') 'sysmat_pmchwt ', tfill
+  WRITE(u,'(A,ES25.17)') 'srcvec ', tsrc
+  WRITE(u,'(A,I0)') 'threads ', omp_get_max_threads()
+  CLOSE(u)
+
+  INQUIRE(FILE=TRIM(outdir)//'/x.bin', EXIST=have_x)
+  IF(have_x) THEN
+     ALLOCATE(x(2*N))
+     OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/x.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD', IOSTAT=ios)
+     READ(u) x
+     CLOSE(u)
+     CALL cs_prtsrf(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, &
+          RESHAPE(x, (/2*N,1/)), src, b%qd_tri, csca, cabs)
+     ALLOCATE(rcsdata(nth,nph))
+     CALL rcs(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, &
+          RESHAPE(x, (/2*N,1/)), nth, nph, b%qd_tri, rcsdata)
+     OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/post.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+     WRITE(u) csca, cabs, rcsdata
+     CLOSE(u)
+  END IF
+
+CONTAINS
+  SUBROUTINE dump_mesh(mesh, fname)
+    TYPE(mesh_container), INTENT(IN) :: mesh
+    CHARACTER(LEN=*), INTENT(IN) :: fname
+    INTEGER :: uu, k
+    OPEN(NEWUNIT=uu, FILE=fname, ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(uu) mesh%nnodes, mesh%nfaces, mesh%nedges
+    WRITE(uu) (mesh%faces(k)%node_indices, k=1,mesh%nfaces)
+    WRITE(uu) (mesh%faces(k)%edge_indices, k=1,mesh%nfaces)
+    WRITE(uu) (mesh%edges(k)%face_indices, k=1,mesh%nedges)
+    WRITE(uu) (mesh%edges(k)%bnode_indices, k=1,mesh%nedges)
+    WRITE(uu) (mesh%edges(k)%node_indices, k=1,mesh%nedges)
+    WRITE(uu) (mesh%edges(k)%parent_index, k=1,mesh%nedges)
+    WRITE(uu) (mesh%nodes(k)%p, k=1,mesh%nnodes)
+    WRITE(uu) (mesh%faces(k)%n, k=1,mesh%nfaces)
+    WRITE(uu) (mesh%faces(k)%cp, k=1,mesh%nfaces)
+    WRITE(uu) (mesh%faces(k)%s, k=1,mesh%nfaces)
+    WRITE(uu) (mesh%faces(k)%m, k=1,mesh%nfaces)
+    WRITE(uu) (mesh%faces(k)%area, k=1,mesh%nfaces)
+    WRITE(uu) (mesh%edges(k)%length, k=1,mesh%nedges)
+    WRITE(uu) mesh%avelen
+    CLOSE(uu)
+  END SUBROUTINE dump_mesh
+END PROGRAM ref_driver

@@ -1,0 +1,93 @@
+"""Generate the golden vectors under tests/golden/ from the reference Fortran.
+
+Runs oracle/_ref/mom_ref (the reference's own modules, built by
+oracle/build_ref.sh) on small synthetic problems and stores inputs and
+outputs as .npz fixtures.  The solution vector fed to the reference's
+post-processing (cs_prtsrf, rcs) is computed here with LAPACK (scipy), which
+is what the reference's solve_multi_linsys calls (linalg.f90:159-180).
+
+Fixtures (all complex128 / float64 at full precision):
+  sphere_s1.npz  icosphere s=1, R=50 nm, eps_in=-9.4+1.5i, 600 nm:
+                 full A (240x240), q, x, csca, cabs, rcs 37x72, mesh arrays
+  sphere_s2.npz  icosphere s=2: sampled A entries, block norms, q, x, csca,
+                 cabs, rcs 37x72, mesh index arrays
+  dimer_s1.npz   two s=1 spheres (ids 15, 16), 3 domains, eps_in=-40+2.5i at
+                 1060 nm: sampled A entries, q, x, csca, cabs, mesh arrays
+Usage: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+import scipy.linalg as sl
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(ROOT, "for90-mom2_amd"))
+
+import refrun  # noqa: E402
+from mom2amd import meshgen  # noqa: E402
+
+PI_F32 = float(np.float32(3.141592653589))
+C0 = 2.997924580003e8
+
+
+def mesh_fields(m, full):
+    keys = ["fnodes", "fedges", "efaces", "ebn", "enodes", "eparent"]
+    if full:
+        keys += ["p", "fn", "fcp", "fs", "fm", "farea", "elen"]
+    out = {k: m[k] for k in keys}
+    out["avelen"] = np.array(m["avelen"])
+    return out
+
+
+def case(name, mshfile, domains, wl, ri, angles, full_A, nsample=3000, seed=1):
+    R = refrun.run(mshfile, 1e-9, domains, wl, ri, angles, dump_ops=False, dump_sys=True)
+    A, q = R["A"], R["q"]
+    x = sl.solve(A, q)
+    P = refrun.run(mshfile, 1e-9, domains, wl, ri, angles, dump_ops=False, dump_sys=False, x=x,
+                   ntheta=37, nphi=72)
+    N = A.shape[0]
+    data = dict(wl=np.array(wl), ri=np.array(ri, dtype=np.complex128), angles_deg=np.array(angles, float),
+                domains=np.array([d + [0] * (4 - len(d)) for d in domains]), q=q, x=x,
+                csca=np.array(P["csca"]), cabs=np.array(P["cabs"]), rcs=P["rcs"])
+    for d, m in enumerate(R["mesh"]):
+        for k, v in mesh_fields(m, full_A).items():
+            data["mesh%d_%s" % (d, k)] = v
+    E = N // 2
+    data["block_fro"] = np.array([[np.linalg.norm(A[i * E:(i + 1) * E, j * E:(j + 1) * E]) for j in range(2)]
+                                  for i in range(2)])
+    if full_A:
+        data["A"] = A
+    else:
+        rng = np.random.default_rng(seed)
+        ii = rng.integers(0, N, nsample)
+        jj = rng.integers(0, N, nsample)
+        ii = np.concatenate([ii, np.arange(N)])
+        jj = np.concatenate([jj, np.arange(N)])
+        data["A_i"], data["A_j"], data["A_v"] = ii.astype(np.int32), jj.astype(np.int32), A[ii, jj]
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **data)
+    print(name, "N=%d" % N, "csca=%.6e cabs=%.6e" % (P["csca"], P["cabs"]))
+
+
+def main():
+    if not refrun.available():
+        raise SystemExit("oracle/_ref/mom_ref is not built (run oracle/build_ref.sh)")
+    tmp = os.path.join(HERE, "_msh")
+    os.makedirs(tmp, exist_ok=True)
+    n_in = np.sqrt(-9.4 + 1.5j)
+    for s, full in ((1, True), (2, False)):
+        nodes, faces = meshgen.icosphere(s, 50.0)
+        fn = os.path.join(tmp, "ico%d.msh" % s)
+        meshgen.write_gmsh(fn, nodes, faces, 15)
+        case("sphere_s%d" % s, fn, [[15], [-15]], 600e-9, [1.0 + 0j, n_in], (0.0, 90.0, 0.0), full)
+    nodes, faces, ids = meshgen.dimer(1, 40.0, 10.0)
+    fn = os.path.join(tmp, "dimer1.msh")
+    meshgen.write_gmsh(fn, nodes, faces, ids)
+    case("dimer_s1", fn, [[15, 16], [-15], [-16]], 1060e-9, [1.0 + 0j, np.sqrt(-40 + 2.5j), np.sqrt(-40 + 2.5j)],
+         (0.0, 90.0, 0.0), False)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+"""Run the reference golden-vector driver (oracle/_ref/mom_ref) and read its dumps.
+
+TEST INFRASTRUCTURE ONLY (used by tests/golden/make_golden.py and the
+reference-vs-oracle tests, which are skipped where oracle/_ref is not built).
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "mom_ref")
+
+
+def available():
+    return os.path.exists(REF_BIN)
+
+
+def read_mesh(path):
+    with open(path, "rb") as fh:
+        V, F, E = np.fromfile(fh, np.int32, 3)
+        r = lambda n, dt: np.fromfile(fh, dt, n)
+        out = dict(fnodes=r(3 * F, np.int32).reshape(F, 3) - 1,
+                   fedges=r(3 * F, np.int32).reshape(F, 3) - 1,
+                   efaces=r(2 * E, np.int32).reshape(E, 2) - 1,
+                   ebn=r(2 * E, np.int32).reshape(E, 2) - 1,
+                   enodes=r(2 * E, np.int32).reshape(E, 2) - 1,
+                   eparent=r(E, np.int32) - 1,
+                   p=r(3 * V, np.float64).reshape(V, 3),
+                   fn=r(3 * F, np.float64).reshape(F, 3),
+                   fcp=r(3 * F, np.float64).reshape(F, 3),
+                   fs=r(9 * F, np.float64).reshape(F, 3, 3),
+                   fm=r(9 * F, np.float64).reshape(F, 3, 3),
+                   farea=r(F, np.float64), elen=r(E, np.float64),
+                   avelen=float(r(1, np.float64)[0]))
+    return out
+
+
+def run(mshfile, scale, domains, wl, ri, angles_deg, dump_ops=True, dump_sys=True, x=None,
+        ntheta=5, nphi=7, outdir=None, threads=None):
+    """Run the reference set-up + hot path; returns a dict of arrays."""
+    outdir = outdir or tempfile.mkdtemp(prefix="momref_")
+    if x is not None:
+        np.ascontiguousarray(x, dtype=np.complex128).tofile(os.path.join(outdir, "x.bin"))
+    lines = [mshfile, repr(float(scale)), str(len(domains))]
+    lines += [" ".join(str(v) for v in [len(d)] + list(d)) for d in domains]
+    lines += [repr(float(wl))]
+    lines += ["%r %r" % (complex(r).real, complex(r).imag) for r in ri]
+    lines += ["%r %r %r" % tuple(float(a) for a in angles_deg)]
+    lines += ["%d %d" % (int(dump_ops), int(dump_sys)), "%d %d" % (ntheta, nphi), outdir]
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    subprocess.run([REF_BIN], input="\n".join(lines) + "\n", text=True, check=True,
+                   stdout=subprocess.DEVNULL, env=env)
+    res = {"mesh": [read_mesh(os.path.join(outdir, "mesh_%d.bin" % d)) for d in range(len(domains) + 1)]}
+    N2 = 2 * res["mesh"][0]["enodes"].shape[0]
+    res["q"] = np.fromfile(os.path.join(outdir, "q.bin"), np.complex128)
+    if dump_sys:
+        res["A"] = np.fromfile(os.path.join(outdir, "A.bin"), np.complex128).reshape(N2, N2, order="F")
+    if dump_ops:
+        for d in range(len(domains)):
+            E = res["mesh"][d + 1]["enodes"].shape[0]
+            for op in "DHK":
+                res["op_%d_%s" % (d, op)] = np.fromfile(os.path.join(outdir, "op_%d_%s.bin" % (d + 1, op)),
+                                                        np.complex128).reshape(E, E, order="F")
+    if x is not None:
+        post = np.fromfile(os.path.join(outdir, "post.bin"), np.float64)
+        res["csca"], res["cabs"] = post[0], post[1]
+        res["rcs"] = post[2:].reshape(ntheta, nphi, order="F")
+    with open(os.path.join(outdir, "times.txt")) as fh:
+        for ln in fh:
+            k, v = ln.split()
+            res["time_" + k] = float(v)
+    return res
