@@ -1,0 +1,366 @@
+// C-ABI entry points of libmomgpu: contexts, meshes, fill, RHS, post.
+// Factorisation and solve entry points live in lu.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+#include "mom_internal.hpp"
+
+namespace mom {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+int DevBuf::alloc(size_t n) {
+  release();
+  if (n == 0) n = 16;
+  hipError_t e = hipMalloc(&ptr, n);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    (void)hipGetLastError();
+    return fail(MOM_ERR_NOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed: " +
+                                   hipGetErrorString(e));
+  }
+  bytes = n;
+  return 0;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+template <class T>
+static int upload(DevBuf &b, const T *src, size_t count, hipStream_t s) {
+  int rc = b.alloc(count * sizeof(T));
+  if (rc) return rc;
+  if (count) MOM_HIP(hipMemcpyAsync(b.ptr, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+int make_rule(int32_t nq, const double *qw, const double *qnodes, QuadRule *r) {
+  if (nq < 1 || nq > kMaxQ || !qw || !qnodes)
+    return fail(MOM_ERR_ARG, "triangle rule must have 1.." + std::to_string(kMaxQ) + " points");
+  r->Q = nq;
+  for (int i = 0; i < kMaxQ; i++) r->w[i] = i < nq ? qw[i] : 0.0;
+  for (int i = 0; i < 3 * kMaxQ; i++) r->nodes[i] = i < 3 * nq ? qnodes[i] : 0.0;
+  return 0;
+}
+
+// Near face pairs: centroid distance^2 < (3*avelen)^2 (sysmat.f90:233-248),
+// evaluated exactly as the reference does, found with a uniform grid.
+static void near_pairs(const std::vector<double> &cp, int64_t F, double avelen,
+                       std::vector<int32_t> &out) {
+  const double thr = avelen * 3;
+  const double thr2 = thr * thr;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t f = 0; f < F; f++)
+    for (int c = 0; c < 3; c++) {
+      lo[c] = std::min(lo[c], cp[3 * f + c]);
+      hi[c] = std::max(hi[c], cp[3 * f + c]);
+    }
+  const double cell = thr > 0 ? thr : 1.0;
+  int64_t dim[3];
+  for (int c = 0; c < 3; c++) dim[c] = std::max<int64_t>(1, (int64_t)((hi[c] - lo[c]) / cell) + 1);
+  auto key = [&](int64_t i, int64_t j, int64_t k) { return (i * dim[1] + j) * dim[2] + k; };
+  std::unordered_map<int64_t, std::vector<int32_t>> grid;
+  std::vector<int64_t> cell_of(3 * F);
+  for (int64_t f = 0; f < F; f++) {
+    int64_t ic[3];
+    for (int c = 0; c < 3; c++) ic[c] = std::min(dim[c] - 1, (int64_t)((cp[3 * f + c] - lo[c]) / cell));
+    for (int c = 0; c < 3; c++) cell_of[3 * f + c] = ic[c];
+    grid[key(ic[0], ic[1], ic[2])].push_back((int32_t)f);
+  }
+  std::vector<int32_t> nb;
+  for (int64_t m = 0; m < F; m++) {
+    nb.clear();
+    for (int64_t a = -1; a <= 1; a++)
+      for (int64_t b = -1; b <= 1; b++)
+        for (int64_t c = -1; c <= 1; c++) {
+          int64_t i = cell_of[3 * m] + a, j = cell_of[3 * m + 1] + b, k = cell_of[3 * m + 2] + c;
+          if (i < 0 || j < 0 || k < 0 || i >= dim[0] || j >= dim[1] || k >= dim[2]) continue;
+          auto it = grid.find(key(i, j, k));
+          if (it == grid.end()) continue;
+          for (int32_t n : it->second) {
+            double d0 = cp[3 * m] - cp[3 * n], d1 = cp[3 * m + 1] - cp[3 * n + 1],
+                   d2 = cp[3 * m + 2] - cp[3 * n + 2];
+            double ds = d0 * d0 + d1 * d1 + d2 * d2;
+            if (ds < thr2) nb.push_back(n);
+          }
+        }
+    std::sort(nb.begin(), nb.end());
+    for (int32_t n : nb) {
+      out.push_back((int32_t)m);
+      out.push_back(n);
+    }
+  }
+}
+
+}  // namespace mom
+
+using namespace mom;
+
+extern "C" {
+
+const char *mom_last_error(void) { return g_err.c_str(); }
+
+int mom_ctx_create(int32_t device, mom_ctx **ctx) {
+  if (!ctx) return fail(MOM_ERR_ARG, "ctx is NULL");
+  int ndev = 0;
+  MOM_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(MOM_ERR_ARG, "no such device");
+  MOM_HIP(hipSetDevice(device));
+  mom_ctx *c = new mom_ctx;
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(MOM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *ctx = c;
+  return 0;
+}
+
+int mom_ctx_destroy(mom_ctx *ctx) {
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->qp.release();
+  ctx->scratch.release();
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nfaces,
+                    const int32_t *fnodes, const int32_t *fedges, const double *fn,
+                    const double *fcp, const double *fs, const double *fm, const double *farea,
+                    int64_t nedges, const int32_t *efaces, const int32_t *ebn, const double *elen,
+                    const int32_t *eparent, double avelen, mom_mesh **mesh) {
+  if (!ctx || !mesh || !p || !fnodes || !fedges || !fn || !fcp || !fs || !fm || !farea ||
+      !efaces || !ebn || !elen || !eparent)
+    return fail(MOM_ERR_ARG, "NULL argument to mom_mesh_create");
+  if (nnodes < 3 || nfaces < 1 || nedges < 1 || nfaces > (1 << 28) || nedges > (1 << 29))
+    return fail(MOM_ERR_ARG, "mesh sizes out of range");
+  if (!(avelen > 0) || !std::isfinite(avelen)) return fail(MOM_ERR_ARG, "avelen must be > 0");
+  const int64_t F = nfaces, E = nedges;
+  for (int64_t i = 0; i < 3 * F; i++) {
+    if (fnodes[i] < 0 || fnodes[i] >= nnodes) return fail(MOM_ERR_ARG, "face node index out of range");
+    if (fedges[i] < 0 || fedges[i] >= E) return fail(MOM_ERR_ARG, "face edge index out of range");
+  }
+  // Per face side: sign (get_face_sign, mesh.f90:1326-1338), coefficient
+  // L/(2A), divergence sign*L/A (rwgf.f90:34-78) and free vertex
+  // (get_face_bnode, mesh.f90:1358-1369).
+  std::vector<int32_t> sg(3 * F), sl(3 * F);
+  std::vector<double> scoef(3 * F), sdiv(3 * F), sv(9 * F);
+  for (int64_t f = 0; f < F; f++)
+    for (int e = 0; e < 3; e++) {
+      const int32_t ge = fedges[3 * f + e];
+      double sign;
+      int32_t bn;
+      if (efaces[2 * ge] == f) {
+        sign = 1.0;
+        bn = ebn[2 * ge];
+      } else if (efaces[2 * ge + 1] == f) {
+        sign = -1.0;
+        bn = ebn[2 * ge + 1];
+      } else {
+        return fail(MOM_ERR_ARG, "corrupted mesh connectivity: face " + std::to_string(f) +
+                                     " is neither face of its edge " + std::to_string(ge));
+      }
+      if (bn < 0 || bn >= nnodes) return fail(MOM_ERR_ARG, "free vertex index out of range");
+      if (eparent[ge] < 0) return fail(MOM_ERR_ARG, "edge without parent index");
+      const double L = elen[ge], A = farea[f];
+      sg[3 * f + e] = eparent[ge];
+      sl[3 * f + e] = ge;
+      scoef[3 * f + e] = sign * (L / (2.0 * A));
+      sdiv[3 * f + e] = sign * L / A;
+      for (int c = 0; c < 3; c++) sv[3 * (3 * f + e) + c] = p[3 * bn + c];
+    }
+  mom_mesh *m = new mom_mesh;
+  m->ctx = ctx;
+  for (int64_t e = 0; e < E; e++) m->max_parent = std::max<int64_t>(m->max_parent, eparent[e]);
+  m->V = nnodes;
+  m->F = F;
+  m->E = E;
+  m->avelen = avelen;
+  m->h_fcp.assign(fcp, fcp + 3 * F);
+  std::vector<int32_t> np;
+  near_pairs(m->h_fcp, F, avelen, np);
+  m->nnear = (int64_t)np.size() / 2;
+  hipStream_t s = ctx->stream;
+  int rc = 0;
+  (void)hipSetDevice(ctx->device);
+  if ((rc = upload(m->p, p, 3 * nnodes, s)) || (rc = upload(m->fnodes, fnodes, 3 * F, s)) ||
+      (rc = upload(m->fn, fn, 3 * F, s)) || (rc = upload(m->fm, fm, 9 * F, s)) ||
+      (rc = upload(m->fs, fs, 9 * F, s)) || (rc = upload(m->farea, farea, F, s)) ||
+      (rc = upload(m->fcp, fcp, 3 * F, s)) || (rc = upload(m->s_g, sg.data(), 3 * F, s)) ||
+      (rc = upload(m->s_l, sl.data(), 3 * F, s)) || (rc = upload(m->s_coef, scoef.data(), 3 * F, s)) ||
+      (rc = upload(m->s_div, sdiv.data(), 3 * F, s)) || (rc = upload(m->s_v, sv.data(), 9 * F, s)) ||
+      (rc = upload(m->near, np.data(), np.size(), s))) {
+    delete m;
+    return rc;
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete m;
+    return fail(MOM_ERR_HIP, std::string("mesh upload: ") + hipGetErrorString(e));
+  }
+  *mesh = m;
+  return 0;
+}
+
+int mom_mesh_destroy(mom_mesh *mesh) {
+  if (!mesh) return 0;
+  (void)hipSetDevice(mesh->ctx->device);
+  (void)hipStreamSynchronize(mesh->ctx->stream);
+  delete mesh;
+  return 0;
+}
+
+int64_t mom_mesh_near_pairs(const mom_mesh *mesh) { return mesh ? mesh->nnear : -1; }
+
+int mom_compute_operator(mom_ctx *ctx, const mom_mesh *mesh, char op, double omega, double ri_re,
+                         double ri_im, int32_t nq, const double *qw, const double *qnodes,
+                         double *M, int64_t ldm) {
+  if (!ctx || !mesh || !M) return fail(MOM_ERR_ARG, "NULL argument");
+  int mode = op == 'D' ? 1 : op == 'H' ? 2 : op == 'K' ? 3 : 0;
+  if (!mode) return fail(MOM_ERR_ARG, "op must be 'D', 'H' or 'K'");
+  if (ldm < mesh->E) return fail(MOM_ERR_ARG, "ldm < nedges");
+  if (!(omega > 0)) return fail(MOM_ERR_ARG, "omega must be > 0");
+  QuadRule rule;
+  int rc = make_rule(nq, qw, qnodes, &rule);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const int64_t E = mesh->E;
+  DevBuf dM;
+  if ((rc = dM.alloc(sizeof(double2) * E * E))) return rc;
+  MOM_HIP(hipMemsetAsync(dM.ptr, 0, sizeof(double2) * E * E, ctx->stream));
+  FillTarget t;
+  t.Z = dM.as<double2>();
+  t.ld = E;
+  t.mode = mode;
+  if ((rc = fill_domain(ctx, mesh, rule, omega, cd(ri_re, ri_im), t))) return rc;
+  MOM_HIP(hipMemcpy2DAsync(M, ldm * sizeof(double2), dM.ptr, E * sizeof(double2), E * sizeof(double2),
+                           E, hipMemcpyDeviceToHost, ctx->stream));
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mom_srcvec_pw(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega,
+                  double ri_re, double ri_im, int64_t nsrc, const double *angles, int32_t nq,
+                  const double *qw, const double *qnodes, double *q, int64_t ldq) {
+  if (!ctx || !mesh || !angles || !q) return fail(MOM_ERR_ARG, "NULL argument");
+  if (nsrc < 1 || nedges_total < 1 || ldq < 2 * nedges_total) return fail(MOM_ERR_ARG, "bad sizes");
+  QuadRule rule;
+  int rc = make_rule(nq, qw, qnodes, &rule);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const int64_t N2 = 2 * nedges_total;
+  DevBuf dq;
+  if ((rc = dq.alloc(sizeof(double2) * N2 * nsrc))) return rc;
+  MOM_HIP(hipMemsetAsync(dq.ptr, 0, sizeof(double2) * N2 * nsrc, ctx->stream));
+  if ((rc = srcvec_pw(ctx, mesh, rule, omega, cd(ri_re, ri_im), nsrc, angles, nedges_total,
+                      dq.as<double2>(), N2)))
+    return rc;
+  MOM_HIP(hipMemcpy2DAsync(q, ldq * sizeof(double2), dq.ptr, N2 * sizeof(double2), N2 * sizeof(double2),
+                           nsrc, hipMemcpyDeviceToHost, ctx->stream));
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int mom_cs_prtsrf(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega,
+                  double ri_re, double ri_im, const double *x, double theta, double phi, double psi,
+                  int32_t nq, const double *qw, const double *qnodes, double *csca, double *cabs) {
+  if (!ctx || !mesh || !x || !csca || !cabs) return fail(MOM_ERR_ARG, "NULL argument");
+  if (!(ri_re > 0)) return fail(MOM_ERR_ARG, "Re(ri) must be > 0");
+  QuadRule rule;
+  int rc = make_rule(nq, qw, qnodes, &rule);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const int64_t N2 = 2 * nedges_total;
+  DevBuf dx, dpart;
+  if ((rc = dx.alloc(sizeof(double2) * N2)) || (rc = dpart.alloc(sizeof(double) * 2 * mesh->F))) return rc;
+  MOM_HIP(hipMemcpyAsync(dx.ptr, x, sizeof(double2) * N2, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = cs_partials(ctx, mesh, rule, omega, cd(ri_re, ri_im), dx.as<double2>(), nedges_total, theta,
+                        phi, psi, dpart.as<double>())))
+    return rc;
+  std::vector<double> part(2 * mesh->F);
+  MOM_HIP(hipMemcpyAsync(part.data(), dpart.ptr, sizeof(double) * 2 * mesh->F, hipMemcpyDeviceToHost,
+                         ctx->stream));
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  // cs.f90:38, 97-102: sums in face order, then c = 2/(eps0 c0 Re ri)
+  double sa = 0, ss = 0;
+  for (int64_t f = 0; f < mesh->F; f++) {
+    sa += part[2 * f];
+    ss += part[2 * f + 1];
+  }
+  const double c = 2.0 / (kEps0 * kC0 * ri_re);
+  *cabs = -c * sa;
+  *csca = c * ss;
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" int mom_sysmat_pmchwt(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms,
+                                 const double *ri, int64_t nedges_total, double omega, int32_t nq,
+                                 const double *qw, const double *qnodes, double *A, int64_t lda) {
+  if (!ctx || !doms || !ri || !A) return fail(MOM_ERR_ARG, "NULL argument");
+  const int64_t N2 = 2 * nedges_total;
+  if (ndom < 1 || nedges_total < 1 || lda < N2) return fail(MOM_ERR_ARG, "bad sizes");
+  (void)hipSetDevice(ctx->device);
+  DevBuf dA;
+  int rc = dA.alloc(sizeof(double2) * N2 * N2);
+  if (rc) return rc;
+  if ((rc = mom::fill_system(ctx, ndom, doms, ri, nedges_total, omega, nq, qw, qnodes,
+                             dA.as<double2>(), N2)))
+    return rc;
+  MOM_HIP(hipMemcpy2DAsync(A, lda * sizeof(double2), dA.ptr, N2 * sizeof(double2), N2 * sizeof(double2),
+                           N2, hipMemcpyDeviceToHost, ctx->stream));
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+namespace mom {
+// sysmat_pmchwt (sysmat.f90:18-105) into a device matrix Z (2N x 2N, ldz):
+// zeroed, then each domain adds D+H to the J-J block, Zsi (D+H) to the M-M
+// block, K to the M-J block and -K to the J-M block in global numbering.
+int fill_system(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *ri,
+                int64_t nedges_total, double omega, int32_t nq, const double *qw,
+                const double *qnodes, double2 *Z, int64_t ldz) {
+  if (!(omega > 0)) return fail(MOM_ERR_ARG, "omega must be > 0");
+  QuadRule rule;
+  int rc = make_rule(nq, qw, qnodes, &rule);
+  if (rc) return rc;
+  for (int d = 0; d < ndom; d++) {
+    if (!doms[d]) return fail(MOM_ERR_ARG, "NULL domain mesh");
+    if (doms[d]->ctx != ctx) return fail(MOM_ERR_ARG, "domain mesh belongs to another context");
+    if (doms[d]->max_parent >= nedges_total)
+      return fail(MOM_ERR_ARG, "domain edge parent index >= nedges_total");
+  }
+  const int64_t N2 = 2 * nedges_total;
+  MOM_HIP(hipMemset2DAsync(Z, ldz * sizeof(double2), 0, N2 * sizeof(double2), N2, ctx->stream));
+  for (int d = 0; d < ndom; d++) {
+    const cd r(ri[2 * d], ri[2 * d + 1]);
+    FillTarget t;
+    t.Z = Z;
+    t.ld = ldz;
+    t.Noff = nedges_total;
+    t.mode = 0;
+    const cd zsi = (r * r) / (kEta0 * kEta0);
+    t.zsi = make_double2(zsi.real(), zsi.imag());
+    if ((rc = fill_domain(ctx, doms[d], rule, omega, r, t))) return rc;
+  }
+  return 0;
+}
+}  // namespace mom

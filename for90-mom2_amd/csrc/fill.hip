@@ -1,0 +1,840 @@
+// PMCHWT system fill on gfx950: far/smooth pair kernel, singular-correction
+// kernel, plane-wave right-hand sides and surface cross-sections.
+//
+// Reference algorithm (for90-mom2): sysmat.f90:18-105 (assembly),
+// 272-519 (computeD/H/K), srcint.f90:57-448 (intK1/2/4), green.f90:25-151
+// (G, grad G, smoothed kernels), int.f90 (analytic singular integrals),
+// quad.f90:352-367 (quadrature points), rwgf.f90:34-78 (RWG functions),
+// source.f90:63-247 (plane-wave RHS), cs.f90:14-104 (cross-sections).
+//
+// Design.  The reference evaluates, for every test face m, test point r_i
+// and source face n, the Green's function at the 13 source points three
+// times (D and H each call G, K calls grad G) and then contracts with the RWG
+// functions edge by edge.  Here one thread owns one face pair (m, n) and
+// evaluates exp(ikR) once per point pair, feeding both kernels G and
+// g = G (1/R^2 - ik/R) (grad' G = (r - r') g).  Because an RWG function is
+// affine on its face, f(r) = coef (rho - nu) in the face's local coordinates
+// (rho = r - centroid, nu = free vertex - centroid), every D and K entry of
+// the 3x3 edge block of the pair is a fixed combination of 18 complex
+// moments sum_ij w_i w_j {G, g} x {1, rho_i, rho'_j, rho_i.rho'_j,
+// rho_i x rho'_j}; H needs only sum_ij w_i w_j G.  Local coordinates keep
+// the moments free of cancellation between face-sized vectors.
+//
+// Work layout: a workgroup of 4 waves owns 64 source faces (one per lane)
+// and 16 test faces (4 per wave, wave-uniform).  Source points sit in LDS
+// as [point][xyz][lane] (conflict-free), test points are broadcast.  Near
+// pairs (centroid distance < 3*avelen, sysmat.f90:233-260) use the smoothed
+// kernels; their analytic singular parts are added by a second kernel that
+// runs one thread per near pair over the pair list built on the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mom_internal.hpp"
+
+namespace mom {
+
+constexpr double kOofpi = 1.0 / (4.0 * MOM_PI);
+
+struct FaceSide {
+  const int *g;
+  const double *coef, *div, *v;
+};
+
+struct FarArgs {
+  int F, Q;
+  double w[kMaxQ];
+  const double *qp;   // [F][Q][3] absolute quadrature points
+  const double *cp;   // [F][3]
+  const double *area; // [F]
+  FaceSide side;      // per face side (3F)
+  double kr, ki;      // wavenumber
+  double thr2;        // near threshold^2
+  double2 c1, c2;     // D and H prefactors
+  FillTarget out;
+};
+
+__device__ __forceinline__ void cadd_atomic(double2 *p, double re, double im) {
+  unsafeAtomicAdd(&p->x, re);
+  unsafeAtomicAdd(&p->y, im);
+}
+
+// near_faces (sysmat.f90:233-260) without FMA contraction, so the
+// classification of borderline pairs follows the reference's arithmetic.
+__device__ bool near_pair(const double *cpm, const double *cpn, double thr2) {
+#pragma clang fp contract(off)
+  double d0 = cpm[0] - cpn[0], d1 = cpm[1] - cpn[1], d2 = cpm[2] - cpn[2];
+  double ds = d0 * d0 + d1 * d1 + d2 * d2;
+  return ds < thr2;
+}
+
+// Writes the 3x3 edge block of one face pair.  D, H, K are complex values
+// of the pair's contribution to computeD/H/K entries.
+__device__ __forceinline__ void emit(const FillTarget &o, int gi, int gj, double Dr, double Di,
+                                     double Hr, double Hi, double Kr, double Ki) {
+  if (o.mode == 0) {
+    const int64_t N = o.Noff, ld = o.ld;
+    double Sr = Dr + Hr, Si = Di + Hi;
+    cadd_atomic(o.Z + gi + (int64_t)gj * ld, Sr, Si);
+    double Mr = o.zsi.x * Sr - o.zsi.y * Si, Mi = o.zsi.x * Si + o.zsi.y * Sr;
+    cadd_atomic(o.Z + (gi + N) + (int64_t)(gj + N) * ld, Mr, Mi);
+    cadd_atomic(o.Z + (gi + N) + (int64_t)gj * ld, Kr, Ki);
+    cadd_atomic(o.Z + gi + (int64_t)(gj + N) * ld, -Kr, -Ki);
+  } else {
+    double vr = o.mode == 1 ? Dr : (o.mode == 2 ? Hr : Kr);
+    double vi = o.mode == 1 ? Di : (o.mode == 2 ? Hi : Ki);
+    cadd_atomic(o.Z + gi + (int64_t)gj * o.ld, vr, vi);
+  }
+}
+
+// quad_tri_points (quad.f90:352-367): r = node(t,3) p1 + node(t,1) p2 + node(t,2) p3
+__global__ void quad_points_kernel(int F, int Q, const double *__restrict__ p,
+                                   const int *__restrict__ fnodes, QuadRule rule,
+                                   double *__restrict__ qp) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= F * Q) return;
+  int f = idx / Q, t = idx - f * Q;
+  const double *p1 = p + 3 * fnodes[3 * f], *p2 = p + 3 * fnodes[3 * f + 1],
+               *p3 = p + 3 * fnodes[3 * f + 2];
+  double a = rule.nodes[2 * Q + t], b = rule.nodes[t], c = rule.nodes[Q + t];
+  for (int k = 0; k < 3; k++) {
+#pragma clang fp contract(off)
+    qp[3 * idx + k] = a * p1[k] + b * p2[k] + c * p3[k];
+  }
+}
+
+constexpr int kWaves = 4;
+constexpr int kTestPerWave = 4;
+constexpr int kTestPerBlock = kWaves * kTestPerWave;
+
+// Far / smooth kernel: one thread per (test face, source face) pair.
+template <bool SMOOTH>
+__device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *__restrict__ tpts,
+                                                const double *__restrict__ spts, int lane,
+                                                double dx, double dy, double dz, double acc[36]) {
+  // acc layout (complex pairs): 0 S0, 1-3 Srho, 4-6 Srho', 7 Srr',
+  //                             8 M0, 9-11 Mrho, 12-14 Mrho', 15-17 Mx
+  const int Q = a.Q;
+  const double kr = a.kr, ki = a.ki;
+  const double k2r = kr * kr - ki * ki, k2i = 2.0 * kr * ki;
+  for (int i = 0; i < Q; i++) {
+    const double rx = tpts[3 * i], ry = tpts[3 * i + 1], rz = tpts[3 * i + 2];
+    const double bx = dx + rx, by = dy + ry, bz = dz + rz;
+    double T0r = 0, T0i = 0, T1r[3] = {0, 0, 0}, T1i[3] = {0, 0, 0};
+    double U0r = 0, U0i = 0, U1r[3] = {0, 0, 0}, U1i[3] = {0, 0, 0};
+    for (int j = 0; j < Q; j++) {
+      const double sx = spts[(3 * j) * 64 + lane], sy = spts[(3 * j + 1) * 64 + lane],
+                   sz = spts[(3 * j + 2) * 64 + lane];
+      const double x = bx - sx, y = by - sy, z = bz - sz;
+      const double R = sqrt(x * x + y * y + z * z);
+      double Gr, Gi, gr, gi;
+      if (!SMOOTH) {
+        const double iR = 1.0 / R;
+        const double ex = (ki == 0.0) ? 1.0 : exp(-ki * R);
+        double sn, cs;
+        sincos(kr * R, &sn, &cs);
+        const double e = ex * iR * kOofpi;
+        Gr = e * cs;
+        Gi = e * sn;
+        // g = G * ((1/R^2 + ki/R) - i kr/R)
+        const double ar = iR * iR + ki * iR, ai = -kr * iR;
+        gr = Gr * ar - Gi * ai;
+        gi = Gr * ai + Gi * ar;
+      } else if (R == 0.0) {
+        // green.f90:33-34 (Gs = ik/4pi) and 88-91 (grad Gs = 0)
+        Gr = -ki * kOofpi;
+        Gi = kr * kOofpi;
+        gr = 0.0;
+        gi = 0.0;
+      } else {
+        const double iR = 1.0 / R;
+        const double ex = (ki == 0.0) ? 1.0 : exp(-ki * R);
+        double sn, cs;
+        sincos(kr * R, &sn, &cs);
+        const double Er = ex * cs, Ei = ex * sn;
+        // Gs = [(e^{ikR} - 1)/R + k^2 R/2] / 4pi
+        Gr = ((Er - 1.0) * iR + 0.5 * k2r * R) * kOofpi;
+        Gi = (Ei * iR + 0.5 * k2i * R) * kOofpi;
+        // grad Gs factor = [e^{ikR}(1/R^3 - ik/R^2) - 1/R^3 - k^2/(2R)] / 4pi
+        const double iR2 = iR * iR, iR3 = iR2 * iR;
+        const double ar = iR3 + ki * iR2, ai = -kr * iR2;
+        gr = ((Er * ar - Ei * ai) - iR3 - 0.5 * k2r * iR) * kOofpi;
+        gi = ((Er * ai + Ei * ar) - 0.5 * k2i * iR) * kOofpi;
+      }
+      const double wj = a.w[j];
+      Gr *= wj;
+      Gi *= wj;
+      gr *= wj;
+      gi *= wj;
+      T0r += Gr;
+      T0i += Gi;
+      U0r += gr;
+      U0i += gi;
+      T1r[0] += Gr * sx; T1i[0] += Gi * sx;
+      T1r[1] += Gr * sy; T1i[1] += Gi * sy;
+      T1r[2] += Gr * sz; T1i[2] += Gi * sz;
+      U1r[0] += gr * sx; U1i[0] += gi * sx;
+      U1r[1] += gr * sy; U1i[1] += gi * sy;
+      U1r[2] += gr * sz; U1i[2] += gi * sz;
+    }
+    const double wi = a.w[i];
+    const double r3[3] = {rx, ry, rz};
+    acc[0] += wi * T0r;
+    acc[1] += wi * T0i;
+    acc[16] += wi * U0r;
+    acc[17] += wi * U0i;
+    double dr = 0, di = 0;
+    for (int c = 0; c < 3; c++) {
+      acc[2 + 2 * c] += wi * r3[c] * T0r;
+      acc[3 + 2 * c] += wi * r3[c] * T0i;
+      acc[8 + 2 * c] += wi * T1r[c];
+      acc[9 + 2 * c] += wi * T1i[c];
+      dr += r3[c] * T1r[c];
+      di += r3[c] * T1i[c];
+      acc[18 + 2 * c] += wi * r3[c] * U0r;
+      acc[19 + 2 * c] += wi * r3[c] * U0i;
+      acc[24 + 2 * c] += wi * U1r[c];
+      acc[25 + 2 * c] += wi * U1i[c];
+    }
+    acc[14] += wi * dr;
+    acc[15] += wi * di;
+    // rho_i x U1
+    const double xr[3] = {ry * U1r[2] - rz * U1r[1], rz * U1r[0] - rx * U1r[2], rx * U1r[1] - ry * U1r[0]};
+    const double xi[3] = {ry * U1i[2] - rz * U1i[1], rz * U1i[0] - rx * U1i[2], rx * U1i[1] - ry * U1i[0]};
+    for (int c = 0; c < 3; c++) {
+      acc[30 + 2 * c] += wi * xr[c];
+      acc[31 + 2 * c] += wi * xi[c];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void far_fill_kernel(FarArgs a) {
+  __shared__ double spts[3 * kMaxQ * 64];
+  __shared__ double tpts[kTestPerBlock][3 * kMaxQ];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int Q = a.Q, F = a.F;
+  const int n0 = blockIdx.x * 64, m0 = blockIdx.y * kTestPerBlock;
+  // stage source points of 64 faces, relative to their centroids
+  for (int idx = threadIdx.x; idx < 64 * Q * 3; idx += blockDim.x) {
+    int l = idx % 64, jc = idx / 64;  // jc = 3*j + c
+    int n = n0 + l;
+    double v = 0.0;
+    if (n < F) v = a.qp[(int64_t)n * 3 * Q + jc] - a.cp[3 * n + jc % 3];
+    spts[jc * 64 + l] = v;
+  }
+  for (int idx = threadIdx.x; idx < kTestPerBlock * Q * 3; idx += blockDim.x) {
+    int t = idx / (3 * Q), ic = idx - t * 3 * Q;
+    int m = m0 + t;
+    double v = 0.0;
+    if (m < F) v = a.qp[(int64_t)m * 3 * Q + ic] - a.cp[3 * m + ic % 3];
+    tpts[t][ic] = v;
+  }
+  __syncthreads();
+
+  const int n = n0 + lane;
+  const bool valid = n < F;
+  const int nn = valid ? n : 0;
+  const double cnx = a.cp[3 * nn], cny = a.cp[3 * nn + 1], cnz = a.cp[3 * nn + 2];
+  const double An = a.area[nn];
+  double coefn[3], divn[3], nun[3][3];
+  int gn[3];
+  for (int q = 0; q < 3; q++) {
+    coefn[q] = a.side.coef[3 * nn + q];
+    divn[q] = a.side.div[3 * nn + q];
+    gn[q] = a.side.g[3 * nn + q];
+    nun[q][0] = a.side.v[3 * (3 * nn + q)] - cnx;
+    nun[q][1] = a.side.v[3 * (3 * nn + q) + 1] - cny;
+    nun[q][2] = a.side.v[3 * (3 * nn + q) + 2] - cnz;
+  }
+
+  for (int t = 0; t < kTestPerWave; t++) {
+    const int tl = wave * kTestPerWave + t;
+    const int m = m0 + tl;
+    if (m >= F) break;
+    const double *cpm = a.cp + 3 * m;
+    const double dx = cpm[0] - cnx, dy = cpm[1] - cny, dz = cpm[2] - cnz;
+    const bool near = near_pair(cpm, a.cp + 3 * nn, a.thr2);
+    double acc[36];
+    for (int k = 0; k < 36; k++) acc[k] = 0.0;
+    if (near)
+      pair_point_loop<true>(a, tpts[tl], spts, lane, dx, dy, dz, acc);
+    else
+      pair_point_loop<false>(a, tpts[tl], spts, lane, dx, dy, dz, acc);
+    if (!valid) continue;
+
+    const double Am = a.area[m];
+    const bool self = (m == n);
+    double coefm[3], divm[3], num[3][3];
+    int gm[3];
+    for (int p = 0; p < 3; p++) {
+      coefm[p] = a.side.coef[3 * m + p];
+      divm[p] = a.side.div[3 * m + p];
+      gm[p] = a.side.g[3 * m + p];
+      for (int c = 0; c < 3; c++) num[p][c] = a.side.v[3 * (3 * m + p) + c] - cpm[c];
+    }
+    const double AA = Am * An;
+    const double D3[3] = {dx, dy, dz};
+    for (int p = 0; p < 3; p++) {
+      for (int q = 0; q < 3; q++) {
+        const double *vm = num[p], *vn = nun[q];
+        // D: Srr' - nu_n.Srho - nu_m.Srho' + (nu_m.nu_n) S0
+        double vmn = vm[0] * vn[0] + vm[1] * vn[1] + vm[2] * vn[2];
+        double sr = acc[14] + vmn * acc[0], si = acc[15] + vmn * acc[1];
+        for (int c = 0; c < 3; c++) {
+          sr -= vn[c] * acc[2 + 2 * c] + vm[c] * acc[8 + 2 * c];
+          si -= vn[c] * acc[3 + 2 * c] + vm[c] * acc[9 + 2 * c];
+        }
+        const double fD = AA * coefm[p] * coefn[q];
+        const double Dr = fD * (a.c1.x * sr - a.c1.y * si);
+        const double Di = fD * (a.c1.x * si + a.c1.y * sr);
+        const double fH = AA * divm[p] * divn[q];
+        const double Hr = fH * (a.c2.x * acc[0] - a.c2.y * acc[1]);
+        const double Hi = fH * (a.c2.x * acc[1] + a.c2.y * acc[0]);
+        double Kr = 0, Ki = 0;
+        if (!self) {
+          // det[rho - nu_m, Delta + rho - rho', rho' - nu_n] summed:
+          //  -(Delta + nu_m - nu_n).Mx - [(Delta x nu_n) - (nu_n x nu_m)].Mrho
+          //  - [(nu_m x Delta) + (nu_n x nu_m)].Mrho' + nu_m.(Delta x nu_n) M0
+          double av[3], dn[3], nm[3], md[3];
+          for (int c = 0; c < 3; c++) av[c] = D3[c] + vm[c] - vn[c];
+          dn[0] = D3[1] * vn[2] - D3[2] * vn[1];
+          dn[1] = D3[2] * vn[0] - D3[0] * vn[2];
+          dn[2] = D3[0] * vn[1] - D3[1] * vn[0];
+          nm[0] = vn[1] * vm[2] - vn[2] * vm[1];
+          nm[1] = vn[2] * vm[0] - vn[0] * vm[2];
+          nm[2] = vn[0] * vm[1] - vn[1] * vm[0];
+          md[0] = vm[1] * D3[2] - vm[2] * D3[1];
+          md[1] = vm[2] * D3[0] - vm[0] * D3[2];
+          md[2] = vm[0] * D3[1] - vm[1] * D3[0];
+          const double c0 = vm[0] * dn[0] + vm[1] * dn[1] + vm[2] * dn[2];
+          double kr = c0 * acc[16], ki = c0 * acc[17];
+          for (int c = 0; c < 3; c++) {
+            const double cr = dn[c] - nm[c], cs = md[c] + nm[c];
+            kr -= av[c] * acc[30 + 2 * c] + cr * acc[18 + 2 * c] + cs * acc[24 + 2 * c];
+            ki -= av[c] * acc[31 + 2 * c] + cr * acc[19 + 2 * c] + cs * acc[25 + 2 * c];
+          }
+          Kr = fD * kr;
+          Ki = fD * ki;
+        }
+        emit(a.out, gm[p], gn[q], Dr, Di, Hr, Hi, Kr, Ki);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Analytic singular integrals (int.f90, Hanninen 2006), FP64, evaluated with
+// FMA contraction off so that exact-zero branch tests and rounding follow the
+// reference's arithmetic.
+struct NearFace {
+  double p[3][3];  // nodes
+  double n[3];     // normal
+  double m[3][3];  // in-plane edge normals m(:,e)
+  double s[3][3];  // edge tangents s(:,e)
+};
+
+__device__ inline double dot3(const double *a, const double *b) {
+#pragma clang fp contract(off)
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ inline double norm3(const double *a) {
+#pragma clang fp contract(off)
+  return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+}
+__device__ inline void cross3(const double *a, const double *b, double *r) {
+#pragma clang fp contract(off)
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// int.f90:15-53 intLm1 for side e (nodes e, e+1)
+__device__ double d_intLm1(const double *r, const NearFace &f, int e) {
+#pragma clang fp contract(off)
+  const double *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
+  double d2[3] = {r[0] - p2[0], r[1] - p2[1], r[2] - p2[2]};
+  double d1[3] = {r[0] - p1[0], r[1] - p1[1], r[2] - p1[2]};
+  double Rp = norm3(d2), Rm = norm3(d1);
+  double q2[3] = {p2[0] - r[0], p2[1] - r[1], p2[2] - r[2]};
+  double q1[3] = {p1[0] - r[0], p1[1] - r[1], p1[2] - r[2]};
+  double sp = dot3(q2, s), sm = dot3(q1, s);
+  double div1 = Rp - sp, div2 = Rm + sm;
+  if (fabs(div1) > fabs(div2)) return log((Rm - sm) / div1);
+  return log((Rp + sp) / div2);
+}
+
+// int.f90:55-135 intL1 and intL3 of side e (they share R0^2, s+-, R+-)
+__device__ void d_intL13(const double *r, const NearFace &f, int e, double lm1, double *L1,
+                         double *L3) {
+#pragma clang fp contract(off)
+  const double *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
+  double p[3], rp[3];
+  for (int k = 0; k < 3; k++) p[k] = 0.5 * (p1[k] + p2[k]);
+  for (int k = 0; k < 3; k++) rp[k] = r[k] - p[k];
+  double h = dot3(f.n, rp), t = dot3(f.m[e], rp);
+  double R02 = t * t + h * h;
+  double d2[3] = {r[0] - p2[0], r[1] - p2[1], r[2] - p2[2]};
+  double d1[3] = {r[0] - p1[0], r[1] - p1[1], r[2] - p1[2]};
+  double Rp = norm3(d2), Rm = norm3(d1);
+  double q2[3] = {p2[0] - r[0], p2[1] - r[1], p2[2] - r[2]};
+  double q1[3] = {p1[0] - r[0], p1[1] - r[1], p1[2] - r[2]};
+  double sp = dot3(q2, s), sm = dot3(q1, s);
+  double l1;
+  if (R02 == 0)
+    l1 = 0.5 * (sp * Rp - sm * Rm);
+  else
+    l1 = 0.5 * R02 * lm1 + 0.5 * (sp * Rp - sm * Rm);
+  *L1 = l1;
+  *L3 = 0.75 * R02 * l1 + 0.25 * (sp * (Rp * Rp * Rp) - sm * (Rm * Rm * Rm));
+}
+
+// int.f90:137-162 intSm3h (signed solid angle)
+__device__ double d_intSm3h(const double *r, const NearFace &f) {
+#pragma clang fp contract(off)
+  double a[3][3];
+  for (int v = 0; v < 3; v++) {
+    double d[3] = {f.p[v][0] - r[0], f.p[v][1] - r[1], f.p[v][2] - r[2]};
+    double l = norm3(d);
+    for (int k = 0; k < 3; k++) a[v][k] = d[k] / l;
+  }
+  double x = 1.0 + dot3(a[0], a[1]) + dot3(a[0], a[2]) + dot3(a[1], a[2]);
+  double c[3];
+  cross3(a[1], a[2], c);
+  double y = fabs(dot3(a[0], c));
+  double om = 2.0 * atan2(y, x);
+  double rp1[3] = {r[0] - f.p[0][0], r[1] - f.p[0][1], r[2] - f.p[0][2]};
+  return dot3(f.n, rp1) > 0.0 ? fabs(om) : -fabs(om);
+}
+
+struct PointInts {
+  double h;          // n.(r - p1)
+  double lm1[3], l1[3], l3[3];
+  double sm3h, sm1, s1;
+};
+
+__device__ void d_point_ints(const double *r, const NearFace &f, PointInts &o) {
+#pragma clang fp contract(off)
+  double t[3];
+  for (int e = 0; e < 3; e++) {
+    double d[3] = {r[0] - f.p[e][0], r[1] - f.p[e][1], r[2] - f.p[e][2]};
+    t[e] = dot3(f.m[e], d);
+  }
+  double d0[3] = {r[0] - f.p[0][0], r[1] - f.p[0][1], r[2] - f.p[0][2]};
+  const double h = dot3(f.n, d0);
+  o.h = h;
+  for (int e = 0; e < 3; e++) {
+    o.lm1[e] = d_intLm1(r, f, e);
+    d_intL13(r, f, e, o.lm1[e], &o.l1[e], &o.l3[e]);
+  }
+  o.sm3h = d_intSm3h(r, f);
+  // int.f90:181-226 intSm1
+  if (h == 0) {
+    double e1 = t[0] == 0 ? 0.0 : o.lm1[0];
+    double e2 = t[1] == 0 ? 0.0 : o.lm1[1];
+    double e3 = t[2] == 0 ? 0.0 : o.lm1[2];
+    o.sm1 = -(t[0] * e1 + t[1] * e2 + t[2] * e3);
+  } else {
+    o.sm1 = -h * o.sm3h - (t[0] * o.lm1[0] + t[1] * o.lm1[1] + t[2] * o.lm1[2]);
+  }
+  // int.f90:228-246 intS1
+  o.s1 = (h * h) / 3.0 * o.sm1 - 1.0 / 3.0 * (t[0] * o.l1[0] + t[1] * o.l1[1] + t[2] * o.l1[2]);
+}
+
+struct NearArgs {
+  int Q;
+  double w[kMaxQ];
+  const int *pairs;  // (m, n)
+  int64_t npairs;
+  const double *qp, *p, *fn, *fm, *fs, *area;
+  const int *fnodes;
+  FaceSide side;
+  double2 kk4;  // 1/(4pi) (real, kept complex for symmetry)
+  double2 kk8;  // k^2/(8pi)
+  double2 c1, c2;
+  FillTarget out;
+};
+
+__global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id >= a.npairs) return;
+  const int m = a.pairs[2 * id], n = a.pairs[2 * id + 1];
+  const bool self = (m == n);
+  NearFace f;
+  for (int v = 0; v < 3; v++) {
+    int node = a.fnodes[3 * n + v];
+    for (int k = 0; k < 3; k++) f.p[v][k] = a.p[3 * node + k];
+  }
+  for (int k = 0; k < 3; k++) f.n[k] = a.fn[3 * n + k];
+  for (int e = 0; e < 3; e++)
+    for (int k = 0; k < 3; k++) {
+      f.m[e][k] = a.fm[9 * n + 3 * e + k];
+      f.s[e][k] = a.fs[9 * n + 3 * e + k];
+    }
+  double coefn[3], divn[3], vn[3][3];
+  for (int q = 0; q < 3; q++) {
+    coefn[q] = a.side.coef[3 * n + q];
+    divn[q] = a.side.div[3 * n + q];
+    for (int k = 0; k < 3; k++) vn[q][k] = a.side.v[3 * (3 * n + q) + k];
+  }
+  double coefm[3], vm[3][3];
+  for (int p = 0; p < 3; p++) {
+    coefm[p] = a.side.coef[3 * m + p];
+    for (int k = 0; k < 3; k++) vm[p][k] = a.side.v[3 * (3 * m + p) + k];
+  }
+  double accD[3][3][2], accK[3][3][2], accH[3][2];
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) accD[p][q][0] = accD[p][q][1] = accK[p][q][0] = accK[p][q][1] = 0.0;
+  for (int q = 0; q < 3; q++) accH[q][0] = accH[q][1] = 0.0;
+
+  for (int i = 0; i < a.Q; i++) {
+    const double *r = a.qp + ((int64_t)m * a.Q + i) * 3;
+    PointInts P;
+    d_point_ints(r, f, P);
+    // intK31 / intK3m1 (int.f90:330-374), shared by the three edges
+    double k31[3], k3m1[3];
+    for (int c = 0; c < 3; c++) {
+      double s = f.m[0][c] * P.l1[0] + f.m[1][c] * P.l1[1] + f.m[2][c] * P.l1[2];
+      k31[c] = (P.h == 0) ? s : s - P.h * f.n[c] * P.sm1;
+      k3m1[c] = f.m[0][c] * P.lm1[0] + f.m[1][c] * P.lm1[1] + f.m[2][c] * P.lm1[2] + f.n[c] * P.sm3h;
+    }
+    double rho[3];
+    for (int c = 0; c < 3; c++) rho[c] = r[c] - P.h * f.n[c];
+    const double wi = a.w[i];
+    double fm_[3][3];
+    for (int p = 0; p < 3; p++)
+      for (int c = 0; c < 3; c++) fm_[p][c] = coefm[p] * (r[c] - vm[p][c]);
+    for (int q = 0; q < 3; q++) {
+      // intK1m1 / intK11 (int.f90:248-274): sign L/A * {Sm1, S1}
+      const double h1 = divn[q] * P.sm1, h0 = divn[q] * P.s1;
+      const double aHr = a.kk4.x * h1 - (a.kk8.x * h0), aHi = -(a.kk8.y * h0);
+      accH[q][0] += wi * aHr;
+      accH[q][1] += wi * aHi;
+      // intK2m1 / intK21 (int.f90:276-328)
+      double k2m1[3], k21[3];
+      for (int c = 0; c < 3; c++) {
+        k2m1[c] = coefn[q] * (f.m[0][c] * P.l1[0] + f.m[1][c] * P.l1[1] + f.m[2][c] * P.l1[2] +
+                              (rho[c] - vn[q][c]) * P.sm1);
+        k21[c] = coefn[q] * (1.0 / 3.0 * (f.m[0][c] * P.l3[0] + f.m[1][c] * P.l3[1] + f.m[2][c] * P.l3[2]) +
+                             (rho[c] - vn[q][c]) * P.s1);
+      }
+      double aD[3][2];
+      for (int c = 0; c < 3; c++) {
+        aD[c][0] = a.kk4.x * k2m1[c] - a.kk8.x * k21[c];
+        aD[c][1] = -(a.kk8.y * k21[c]);
+      }
+      double aK[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+      if (!self) {
+        // intK4m1 / intK41 (int.f90:376-406): -sign L/(2A) (r - p) x {K3m1, K31}
+        double d[3] = {r[0] - vn[q][0], r[1] - vn[q][1], r[2] - vn[q][2]};
+        double c1v[3], c0v[3];
+        cross3(d, k3m1, c1v);
+        cross3(d, k31, c0v);
+        const double sc = -coefn[q];
+        for (int c = 0; c < 3; c++) {
+          const double k4m1 = sc * c1v[c], k41 = sc * c0v[c];
+          aK[c][0] = a.kk4.x * k4m1 - a.kk8.x * k41;
+          aK[c][1] = -(a.kk8.y * k41);
+        }
+      }
+      for (int p = 0; p < 3; p++) {
+        double dr = 0, di = 0, kr = 0, ki = 0;
+        for (int c = 0; c < 3; c++) {
+          dr += fm_[p][c] * aD[c][0];
+          di += fm_[p][c] * aD[c][1];
+          kr += fm_[p][c] * aK[c][0];
+          ki += fm_[p][c] * aK[c][1];
+        }
+        accD[p][q][0] += wi * dr;
+        accD[p][q][1] += wi * di;
+        accK[p][q][0] += wi * kr;
+        accK[p][q][1] += wi * ki;
+      }
+    }
+  }
+  const double Am = a.area[m];
+  for (int p = 0; p < 3; p++) {
+    const double divm = a.side.div[3 * m + p];
+    const int gi = a.side.g[3 * m + p];
+    for (int q = 0; q < 3; q++) {
+      const int gj = a.side.g[3 * n + q];
+      const double dr = accD[p][q][0] * Am, di = accD[p][q][1] * Am;
+      const double Dr = a.c1.x * dr - a.c1.y * di, Di = a.c1.x * di + a.c1.y * dr;
+      const double hr = accH[q][0] * Am * divm, hi = accH[q][1] * Am * divm;
+      const double Hr = a.c2.x * hr - a.c2.y * hi, Hi = a.c2.x * hi + a.c2.y * hr;
+      emit(a.out, gi, gj, Dr, Di, Hr, Hi, accK[p][q][0] * Am, accK[p][q][1] * Am);
+    }
+  }
+}
+
+int compute_quad_points(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule) {
+  size_t need = (size_t)m->F * rule.Q * 3 * sizeof(double);
+  if (ctx->qp.bytes < need) {
+    int rc = ctx->qp.alloc(need);
+    if (rc) return rc;
+  }
+  int total = (int)(m->F * rule.Q);
+  hipLaunchKernelGGL(quad_points_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
+                     (int)m->F, rule.Q, m->p.as<double>(), m->fnodes.as<int>(), rule,
+                     ctx->qp.as<double>());
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
+static FaceSide sides(const mom_mesh *m, bool global) {
+  FaceSide s;
+  s.g = global ? m->s_g.as<int>() : m->s_l.as<int>();
+  s.coef = m->s_coef.as<double>();
+  s.div = m->s_div.as<double>();
+  s.v = m->s_v.as<double>();
+  return s;
+}
+
+int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
+                const FillTarget &t) {
+  int rc = compute_quad_points(ctx, m, rule);
+  if (rc) return rc;
+  const cd I(0.0, 1.0);
+  const cd k = ri * omega / kC0;
+  // sysmat.f90:301 and 388: c1 = -i omega mu0, c2 = -1/(i omega eps0 ri^2)
+  const cd c1 = -(I * omega) * mu0();
+  const cd c2 = -1.0 / ((I * omega) * kEps0 * (ri * ri));
+  FarArgs a;
+  a.F = (int)m->F;
+  a.Q = rule.Q;
+  for (int i = 0; i < kMaxQ; i++) a.w[i] = i < rule.Q ? rule.w[i] : 0.0;
+  a.qp = ctx->qp.as<double>();
+  a.cp = m->fcp.as<double>();
+  a.area = m->farea.as<double>();
+  a.side = sides(m, t.mode == 0);
+  a.kr = k.real();
+  a.ki = k.imag();
+  const double thr = m->avelen * 3;
+  a.thr2 = thr * thr;
+  a.c1 = make_double2(c1.real(), c1.imag());
+  a.c2 = make_double2(c2.real(), c2.imag());
+  a.out = t;
+  dim3 grid((unsigned)((m->F + 63) / 64), (unsigned)((m->F + kTestPerBlock - 1) / kTestPerBlock));
+  hipLaunchKernelGGL(far_fill_kernel, grid, dim3(256), 0, ctx->stream, a);
+  MOM_HIP(hipGetLastError());
+
+  if (m->nnear > 0) {
+    NearArgs b;
+    b.Q = rule.Q;
+    for (int i = 0; i < kMaxQ; i++) b.w[i] = a.w[i];
+    b.pairs = m->near.as<int>();
+    b.npairs = m->nnear;
+    b.qp = a.qp;
+    b.p = m->p.as<double>();
+    b.fn = m->fn.as<double>();
+    b.fm = m->fm.as<double>();
+    b.fs = m->fs.as<double>();
+    b.area = a.area;
+    b.fnodes = m->fnodes.as<int>();
+    b.side = a.side;
+    b.kk4 = make_double2(1.0 / (4.0 * MOM_PI), 0.0);
+    const cd kk8 = (k * k) / (8.0 * MOM_PI);
+    b.kk8 = make_double2(kk8.real(), kk8.imag());
+    b.c1 = a.c1;
+    b.c2 = a.c2;
+    b.out = t;
+    hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((m->nnear + 127) / 128)), dim3(128), 0,
+                       ctx->stream, b);
+    MOM_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Plane-wave right-hand sides (source.f90:136-247 with pw_fields 84-101):
+// one thread per (face, source); each face adds <f_p, E_inc> and
+// <f_p, H_inc> of its three sides.
+struct PwSrc {
+  double dir[3], pol[3];
+};
+
+__global__ void srcvec_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
+                              const double *__restrict__ area, FaceSide side, int nsrc,
+                              const PwSrc *__restrict__ src, double kr, double ki, double2 hfac,
+                              int64_t Nt, double2 *q, int64_t ldq) {
+  int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id >= (int64_t)F * nsrc) return;
+  int f = (int)(id % F), s = (int)(id / F);
+  const PwSrc S = src[s];
+  double Ie[3][2] = {{0, 0}, {0, 0}, {0, 0}}, Ih[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int t = 0; t < Q; t++) {
+    const double *r = qp + ((int64_t)f * Q + t) * 3;
+    double kd = S.dir[0] * r[0] + S.dir[1] * r[1] + S.dir[2] * r[2];
+    // exp(i k d.r)
+    double ex = exp(-ki * kd), sn, cs;
+    sincos(kr * kd, &sn, &cs);
+    double er = ex * cs, ei = ex * sn;
+    double E[3][2], H[3][2];
+    for (int c = 0; c < 3; c++) {
+      E[c][0] = S.pol[c] * er;
+      E[c][1] = S.pol[c] * ei;
+    }
+    // H = k/(omega mu0) dir x E
+    for (int c = 0; c < 3; c++) {
+      int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      double xr = S.dir[c1] * E[c2][0] - S.dir[c2] * E[c1][0];
+      double xi = S.dir[c1] * E[c2][1] - S.dir[c2] * E[c1][1];
+      H[c][0] = hfac.x * xr - hfac.y * xi;
+      H[c][1] = hfac.x * xi + hfac.y * xr;
+    }
+    for (int p = 0; p < 3; p++) {
+      const double cf = side.coef[3 * f + p];
+      const double *v = side.v + 3 * (3 * f + p);
+      double fv[3] = {cf * (r[0] - v[0]), cf * (r[1] - v[1]), cf * (r[2] - v[2])};
+      double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+      for (int c = 0; c < 3; c++) {
+        a0 += fv[c] * E[c][0];
+        a1 += fv[c] * E[c][1];
+        b0 += fv[c] * H[c][0];
+        b1 += fv[c] * H[c][1];
+      }
+      Ie[p][0] += rule.w[t] * a0;
+      Ie[p][1] += rule.w[t] * a1;
+      Ih[p][0] += rule.w[t] * b0;
+      Ih[p][1] += rule.w[t] * b1;
+    }
+  }
+  const double Am = area[f];
+  double2 *qs = q + (int64_t)s * ldq;
+  for (int p = 0; p < 3; p++) {
+    const int g = side.g[3 * f + p];
+    cadd_atomic(qs + g, Ie[p][0] * Am, Ie[p][1] * Am);
+    cadd_atomic(qs + g + Nt, Ih[p][0] * Am, Ih[p][1] * Am);
+  }
+}
+
+int srcvec_pw(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
+              int64_t nsrc, const double *angles, int64_t Nt, double2 *q, int64_t ldq) {
+  int rc = compute_quad_points(ctx, m, rule);
+  if (rc) return rc;
+  std::vector<PwSrc> hs(nsrc);
+  for (int64_t s = 0; s < nsrc; s++) {
+    double th = angles[3 * s], ph = angles[3 * s + 1], ps = angles[3 * s + 2];
+    // aux.f90:245-261 get_dir / get_pol
+    hs[s].dir[0] = sin(th) * cos(ph);
+    hs[s].dir[1] = sin(th) * sin(ph);
+    hs[s].dir[2] = cos(th);
+    hs[s].pol[0] = cos(ps) * sin(ph) - sin(ps) * cos(th) * cos(ph);
+    hs[s].pol[1] = -cos(ps) * cos(ph) - sin(ps) * cos(th) * sin(ph);
+    hs[s].pol[2] = sin(ps) * sin(th);
+  }
+  DevBuf dsrc;
+  if ((rc = dsrc.alloc(sizeof(PwSrc) * nsrc))) return rc;
+  MOM_HIP(hipMemcpyAsync(dsrc.ptr, hs.data(), sizeof(PwSrc) * nsrc, hipMemcpyHostToDevice, ctx->stream));
+  const cd k = ri * omega / kC0;
+  const cd hfac = k / (omega * mu0());
+  int64_t total = m->F * nsrc;
+  hipLaunchKernelGGL(srcvec_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, ctx->stream,
+                     (int)m->F, rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(),
+                     sides(m, true), (int)nsrc, dsrc.as<PwSrc>(), k.real(), k.imag(),
+                     make_double2(hfac.real(), hfac.imag()), Nt, q, ldq);
+  MOM_HIP(hipGetLastError());
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// cs_prtsrf (cs.f90:14-104), identity group: per face, the quadrature sums
+// of the total (absorption) and scattered Poynting flux; the host adds the
+// per-face values in face order as the reference does.
+__global__ void cs_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
+                          const double *__restrict__ area, const double *__restrict__ fnrm,
+                          FaceSide side, const double2 *__restrict__ x, int64_t Nt, PwSrc S,
+                          double kr, double ki, double2 hfac, double *out) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const double *nor = fnrm + 3 * f;
+  double int1 = 0, int2 = 0;
+  for (int t = 0; t < Q; t++) {
+    const double *r = qp + ((int64_t)f * Q + t) * 3;
+    double Et[3][2] = {{0, 0}, {0, 0}, {0, 0}}, Ht[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int p = 0; p < 3; p++) {
+      const double cf = side.coef[3 * f + p];
+      const double *v = side.v + 3 * (3 * f + p);
+      double fv[3] = {cf * (r[0] - v[0]), cf * (r[1] - v[1]), cf * (r[2] - v[2])};
+      double fn[3] = {nor[1] * fv[2] - nor[2] * fv[1], nor[2] * fv[0] - nor[0] * fv[2],
+                      nor[0] * fv[1] - nor[1] * fv[0]};
+      const int g = side.g[3 * f + p];
+      const double2 xj = x[g], xm = x[g + Nt];
+      for (int c = 0; c < 3; c++) {
+        Ht[c][0] -= xj.x * fn[c];
+        Ht[c][1] -= xj.y * fn[c];
+        Et[c][0] += xm.x * fn[c];
+        Et[c][1] += xm.y * fn[c];
+      }
+    }
+    double kd = S.dir[0] * r[0] + S.dir[1] * r[1] + S.dir[2] * r[2];
+    double ex = exp(-ki * kd), sn, cs;
+    sincos(kr * kd, &sn, &cs);
+    double er = ex * cs, ei = ex * sn;
+    double Ei[3][2], Hi[3][2];
+    for (int c = 0; c < 3; c++) {
+      Ei[c][0] = S.pol[c] * er;
+      Ei[c][1] = S.pol[c] * ei;
+    }
+    for (int c = 0; c < 3; c++) {
+      int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      double xr = S.dir[c1] * Ei[c2][0] - S.dir[c2] * Ei[c1][0];
+      double xi = S.dir[c1] * Ei[c2][1] - S.dir[c2] * Ei[c1][1];
+      Hi[c][0] = hfac.x * xr - hfac.y * xi;
+      Hi[c][1] = hfac.x * xi + hfac.y * xr;
+    }
+    // tangential incident fields (n x F) x n = F - (n.F) n
+    double nE[2] = {0, 0}, nH[2] = {0, 0};
+    for (int c = 0; c < 3; c++) {
+      nE[0] += nor[c] * Ei[c][0];
+      nE[1] += nor[c] * Ei[c][1];
+      nH[0] += nor[c] * Hi[c][0];
+      nH[1] += nor[c] * Hi[c][1];
+    }
+    double Es[3][2], Hs[3][2];
+    for (int c = 0; c < 3; c++) {
+      Es[c][0] = Et[c][0] - (Ei[c][0] - nE[0] * nor[c]);
+      Es[c][1] = Et[c][1] - (Ei[c][1] - nE[1] * nor[c]);
+      Hs[c][0] = Ht[c][0] - (Hi[c][0] - nH[0] * nor[c]);
+      Hs[c][1] = Ht[c][1] - (Hi[c][1] - nH[1] * nor[c]);
+    }
+    // 0.5 Re[(E x conj H) . n]
+    double s1 = 0, s2 = 0;
+    for (int c = 0; c < 3; c++) {
+      int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      double re1 = (Et[c1][0] * Ht[c2][0] + Et[c1][1] * Ht[c2][1]) -
+                   (Et[c2][0] * Ht[c1][0] + Et[c2][1] * Ht[c1][1]);
+      double re2 = (Es[c1][0] * Hs[c2][0] + Es[c1][1] * Hs[c2][1]) -
+                   (Es[c2][0] * Hs[c1][0] + Es[c2][1] * Hs[c1][1]);
+      s1 += re1 * nor[c];
+      s2 += re2 * nor[c];
+    }
+    int1 += rule.w[t] * 0.5 * s1;
+    int2 += rule.w[t] * 0.5 * s2;
+  }
+  out[2 * f] = int1 * area[f];
+  out[2 * f + 1] = int2 * area[f];
+}
+
+int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
+                const double2 *x, int64_t Nt, double th, double ph, double ps, double *partial) {
+  int rc = compute_quad_points(ctx, m, rule);
+  if (rc) return rc;
+  PwSrc S;
+  S.dir[0] = sin(th) * cos(ph);
+  S.dir[1] = sin(th) * sin(ph);
+  S.dir[2] = cos(th);
+  S.pol[0] = cos(ps) * sin(ph) - sin(ps) * cos(th) * cos(ph);
+  S.pol[1] = -cos(ps) * cos(ph) - sin(ps) * cos(th) * sin(ph);
+  S.pol[2] = sin(ps) * sin(th);
+  const cd k = ri * omega / kC0;
+  const cd hfac = k / (omega * mu0());
+  hipLaunchKernelGGL(cs_kernel, dim3((unsigned)((m->F + 127) / 128)), dim3(128), 0, ctx->stream,
+                     (int)m->F, rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(),
+                     m->fn.as<double>(), sides(m, true), x, Nt, S, k.real(), k.imag(),
+                     make_double2(hfac.real(), hfac.imag()), partial);
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mom

@@ -1,0 +1,1334 @@
+// Dense complex LU with partial pivoting, triangular solves and
+// mixed-precision GMRES iterative refinement on gfx950.
+//
+// Replaces the LAPACK calls of solve_multi_linsys (linalg.f90:159-180,
+// ZGETRF + ZGETRS) behind solve_systems (solver.f90:325-357).
+//
+// Factorisation: blocked right-looking LU (outer block NB = 256) whose
+// trailing update A22 -= L21 U12 runs on the matrix cores
+// (v_mfma_f64_16x16x4f64 for complex128, v_mfma_f32_16x16x4f32 for
+// complex64; a complex product is four real MFMAs).  The panel is factored
+// in inner blocks of IB = 32 columns: per column one kernel finds the pivot
+// (|re|+|im| with the lowest index on ties, as LAPACK's IZAMAX) and swaps
+// the two rows across the panel, a second scales the column and applies
+// the rank-1 update to the rest of the inner block; the remaining panel
+// columns then get a small triangular solve and an MFMA update.  Row
+// interchanges reach the columns left and right of the panel through a
+// laswp kernel, so the factors are in ZGETRF layout (unit L below the
+// diagonal, U on and above, 1-based ipiv).
+//
+// Solves: the row permutation is applied as one gather; forward and back
+// substitution work block by block with precomputed inverses of the NB x NB
+// diagonal blocks (x_b = inv(T_bb) r_b, then a GEMV-style update of the
+// rest), so every step is parallel.
+//
+// Mixed precision: the complex128 matrix Z stays resident; a copy scaled by
+// power-of-two row and column factors (exact in binary) is rounded to
+// complex64 and factored.  GMRES on the FP64 system, preconditioned by the
+// FP32 factors, refines the solution until the complex128 backward error
+// ||b - Zx||_inf / (||Z||_inf ||x||_inf + ||b||_inf) reaches the tolerance.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mom_internal.hpp"
+
+namespace mom {
+
+template <typename T> struct Cx;
+// acc_row(q, v): row (0..15) of the 16x16 output held by accumulator v of a
+// lane in row group q = lane/16 (column = lane%16).  The two instructions
+// differ: f32 16x16x4 keeps 4 consecutive rows per lane, f64 16x16x4
+// interleaves them (checked by mom_test_mfma_layout).
+template <> struct Cx<float> {
+  using type = float2;
+  typedef float acc4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ int acc_row(int q, int v) { return 4 * q + v; }
+};
+template <> struct Cx<double> {
+  using type = double2;
+  typedef double acc4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ int acc_row(int q, int v) { return q + 4 * v; }
+};
+
+template <typename T> __device__ __forceinline__ typename Cx<T>::acc4 mfma4(T a, T b, typename Cx<T>::acc4 c);
+template <> __device__ __forceinline__ Cx<float>::acc4 mfma4<float>(float a, float b, Cx<float>::acc4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <> __device__ __forceinline__ Cx<double>::acc4 mfma4<double>(double a, double b, Cx<double>::acc4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+constexpr int NB = 256;  // outer block (trailing update depth)
+constexpr int IB = 32;   // inner panel block
+
+// ---------------------------------------------------------------------------
+// Trailing update C -= A B (all complex, column-major), MFMA tiles.
+// Wave tile (16 WM) x (16 WN) complex, 4 waves in a 2 x 2 arrangement.
+template <typename T, int WM, int WN>
+__global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::type *__restrict__ A, int64_t lda,
+                                                       const typename Cx<T>::type *__restrict__ B, int64_t ldb,
+                                                       typename Cx<T>::type *__restrict__ Cm, int64_t ldc, int M,
+                                                       int N, int K) {
+  using C2 = typename Cx<T>::type;
+  using acc4 = typename Cx<T>::acc4;
+  constexpr int BM = 32 * WM, BN = 32 * WN, BK = 16;
+  constexpr int PA = BM + 16 / (int)(sizeof(T) / 4), PB = BN + 16 / (int)(sizeof(T) / 4);
+  __shared__ T As[2][BK][PA];
+  __shared__ T Bs[2][BK][PB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int wr = (wave & 1) * 16 * WM, wc = (wave >> 1) * 16 * WN;
+  const int64_t row0 = (int64_t)blockIdx.x * BM, col0 = (int64_t)blockIdx.y * BN;
+  acc4 accr[WM][WN], acci[WM][WN];
+  for (int i = 0; i < WM; i++)
+    for (int j = 0; j < WN; j++)
+      for (int v = 0; v < 4; v++) {
+        accr[i][j][v] = 0;
+        acci[i][j][v] = 0;
+      }
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    for (int idx = tid; idx < BM * BK; idx += 256) {
+      const int r = idx % BM, k = idx / BM;
+      const int64_t gr = row0 + r;
+      const int gk = k0 + k;
+      C2 v;
+      v.x = 0;
+      v.y = 0;
+      if (gr < M && gk < K) v = A[gr + (int64_t)gk * lda];
+      As[0][k][r] = v.x;
+      As[1][k][r] = v.y;
+    }
+    for (int idx = tid; idx < BN * BK; idx += 256) {
+      const int k = idx % BK, c = idx / BK;
+      const int64_t gc = col0 + c;
+      const int gk = k0 + k;
+      C2 v;
+      v.x = 0;
+      v.y = 0;
+      if (gc < N && gk < K) v = B[gk + gc * ldb];
+      Bs[0][k][c] = v.x;
+      Bs[1][k][c] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T ar[WM], ai[WM], br[WN], bi[WN];
+#pragma unroll
+      for (int i = 0; i < WM; i++) {
+        ar[i] = As[0][kk + kq][wr + 16 * i + l16];
+        ai[i] = As[1][kk + kq][wr + 16 * i + l16];
+      }
+#pragma unroll
+      for (int j = 0; j < WN; j++) {
+        br[j] = Bs[0][kk + kq][wc + 16 * j + l16];
+        bi[j] = Bs[1][kk + kq][wc + 16 * j + l16];
+      }
+#pragma unroll
+      for (int i = 0; i < WM; i++) {
+        const T nai = -ai[i];
+#pragma unroll
+        for (int j = 0; j < WN; j++) {
+          accr[i][j] = mfma4<T>(ar[i], br[j], accr[i][j]);
+          accr[i][j] = mfma4<T>(nai, bi[j], accr[i][j]);
+          acci[i][j] = mfma4<T>(ar[i], bi[j], acci[i][j]);
+          acci[i][j] = mfma4<T>(ai[i], br[j], acci[i][j]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < WM; i++)
+#pragma unroll
+    for (int j = 0; j < WN; j++) {
+      const int64_t col = col0 + wc + 16 * j + l16;
+      if (col >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int64_t row = row0 + wr + 16 * i + Cx<T>::acc_row(kq, v);
+        if (row < M) {
+          C2 *p = Cm + row + col * ldc;
+          C2 c = *p;
+          c.x -= accr[i][j][v];
+          c.y -= acci[i][j][v];
+          *p = c;
+        }
+      }
+    }
+}
+
+template <typename T> struct GemmCfg;
+template <> struct GemmCfg<float> { static constexpr int WM = 2, WN = 2; };
+template <> struct GemmCfg<double> { static constexpr int WM = 1, WN = 2; };
+
+// Event-pair timing of the trailing-update kernel launches (enabled per
+// system); events are read back after the factorisation, so timing does
+// not stall the launch queue.
+struct GemmStats {
+  bool enabled = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<double> flops_of;  // per recorded launch
+  double seconds = 0, launches = 0, flops = 0;
+  void reset() {
+    used = 0;
+    flops_of.clear();
+    seconds = launches = flops = 0;
+  }
+  int begin(hipStream_t s) {
+    if (used + 2 > pool.size()) {
+      for (int i = 0; i < 512; i++) {
+        hipEvent_t e;
+        MOM_HIP(hipEventCreate(&e));
+        pool.push_back(e);
+      }
+    }
+    MOM_HIP(hipEventRecord(pool[used], s));
+    return 0;
+  }
+  int end(hipStream_t s, double fl) {
+    MOM_HIP(hipEventRecord(pool[used + 1], s));
+    used += 2;
+    flops_of.push_back(fl);
+    return 0;
+  }
+  // call after the stream is synchronised
+  int collect() {
+    for (size_t i = 0; i < flops_of.size(); i++) {
+      float ms = 0;
+      MOM_HIP(hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]));
+      seconds += ms * 1e-3;
+      launches += 1;
+      flops += flops_of[i];
+    }
+    flops_of.clear();
+    used = 0;
+    return 0;
+  }
+  ~GemmStats() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+template <typename T>
+static int gemm_sub(hipStream_t s, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
+                    int64_t ldb, typename Cx<T>::type *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
+                    GemmStats *st = nullptr) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
+  dim3 grid((unsigned)((M + 32 * WM - 1) / (32 * WM)), (unsigned)((N + 32 * WN - 1) / (32 * WN)));
+  const bool timed = st && st->enabled;
+  int rc;
+  if (timed && (rc = st->begin(s))) return rc;
+  hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M,
+                     (int)N, (int)K);
+  MOM_HIP(hipGetLastError());
+  if (timed && (rc = st->end(s, 8.0 * (double)M * (double)N * (double)K))) return rc;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Panel kernels.
+struct PivotWork {
+  double *part_val;  // per block
+  int *part_idx;
+  int *counter;
+  int *info;         // first zero pivot (1-based), 0 if none
+};
+
+constexpr int kPivBlocks = 64;
+
+template <typename T>
+__device__ __forceinline__ T cabs1(typename Cx<T>::type v) {
+  return fabs(v.x) + fabs(v.y);
+}
+
+// Pivot search on column j over rows j..n-1 and row swap across the panel
+// columns [pc0, pc1).  The last block to finish reduces the block results.
+template <typename T>
+__global__ __launch_bounds__(256) void pivot_kernel(typename Cx<T>::type *A, int64_t ld, int j, int n, int pc0,
+                                                     int pc1, int *ipiv, PivotWork w) {
+  using C2 = typename Cx<T>::type;
+  __shared__ double sval[256];
+  __shared__ int sidx[256];
+  __shared__ bool last;
+  const int tid = threadIdx.x;
+  double best = -1.0;
+  int bidx = n;
+  const C2 *col = A + (int64_t)j * ld;
+  for (int i = j + blockIdx.x * 256 + tid; i < n; i += gridDim.x * 256) {
+    const double v = (double)cabs1<T>(col[i]);
+    if (v > best) {
+      best = v;
+      bidx = i;
+    }
+  }
+  sval[tid] = best;
+  sidx[tid] = bidx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double v2 = sval[tid + s];
+      const int i2 = sidx[tid + s];
+      if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
+        sval[tid] = v2;
+        sidx[tid] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    w.part_val[blockIdx.x] = sval[0];
+    w.part_idx[blockIdx.x] = sidx[0];
+    __threadfence();
+    const int t = atomicAdd(w.counter, 1);
+    last = (t == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (tid < (int)gridDim.x) {
+    sval[tid] = __hip_atomic_load(&w.part_val[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sidx[tid] = __hip_atomic_load(&w.part_idx[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    sval[tid] = -1.0;
+    sidx[tid] = n;
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double v2 = sval[tid + s];
+      const int i2 = sidx[tid + s];
+      if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
+        sval[tid] = v2;
+        sidx[tid] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  const int p = (sidx[0] < n) ? sidx[0] : j;
+  const bool zero = !(sval[0] > 0.0);
+  if (tid == 0) {
+    ipiv[j] = p + 1;
+    *w.counter = 0;
+    if (zero) atomicCAS(w.info, 0, j + 1);
+  }
+  if (p != j) {
+    for (int c = pc0 + tid; c < pc1; c += 256) {
+      C2 *cc = A + (int64_t)c * ld;
+      const C2 a = cc[j], b = cc[p];
+      cc[j] = b;
+      cc[p] = a;
+    }
+  }
+}
+
+// Scale column j below the diagonal by 1/pivot and apply the rank-1 update
+// to columns j+1..jend-1 of the inner block.
+template <typename T>
+__global__ __launch_bounds__(256) void column_update_kernel(typename Cx<T>::type *A, int64_t ld, int j, int n,
+                                                             int jend) {
+  using C2 = typename Cx<T>::type;
+  const int64_t i = (int64_t)j + 1 + blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  C2 *colj = A + (int64_t)j * ld;
+  const C2 piv = colj[j];
+  C2 l = colj[i];
+  if (piv.x != 0 || piv.y != 0) {
+    // l / piv (Smith's formula keeps the scaling safe)
+    T a = piv.x, b = piv.y;
+    if (fabs(a) >= fabs(b)) {
+      const T r = b / a, den = a + b * r;
+      C2 q;
+      q.x = (l.x + l.y * r) / den;
+      q.y = (l.y - l.x * r) / den;
+      l = q;
+    } else {
+      const T r = a / b, den = a * r + b;
+      C2 q;
+      q.x = (l.x * r + l.y) / den;
+      q.y = (l.y * r - l.x) / den;
+      l = q;
+    }
+    colj[i] = l;
+  }
+  for (int c = j + 1; c < jend; c++) {
+    C2 *cc = A + (int64_t)c * ld;
+    const C2 u = cc[j];
+    C2 v = cc[i];
+    v.x -= l.x * u.x - l.y * u.y;
+    v.y -= l.x * u.y + l.y * u.x;
+    cc[i] = v;
+  }
+}
+
+// Row interchanges k1..k2-1 (ipiv 1-based) on columns [c0, c1) and [c2, c3).
+template <typename T>
+__global__ void laswp_kernel(typename Cx<T>::type *A, int64_t ld, int k1, int k2, const int *ipiv, int c0, int c1,
+                             int c2, int c3) {
+  using C2 = typename Cx<T>::type;
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t n1 = c1 - c0;
+  int64_t c;
+  if (t < n1)
+    c = c0 + t;
+  else if (t < n1 + (c3 - c2))
+    c = c2 + (t - n1);
+  else
+    return;
+  C2 *col = A + c * ld;
+  for (int k = k1; k < k2; k++) {
+    const int p = ipiv[k] - 1;
+    if (p != k) {
+      const C2 a = col[k], b = col[p];
+      col[k] = b;
+      col[p] = a;
+    }
+  }
+}
+
+// X := inv(L) X for the unit lower IB x IB block at (r0, r0); one thread per
+// column of X (columns [c0, c0 + ncols)).
+template <typename T, int W>
+__global__ __launch_bounds__(128) void trsm_unit_lower_kernel(typename Cx<T>::type *A, int64_t ld, int64_t r0,
+                                                               int w, int64_t c0, int64_t ncols) {
+  using C2 = typename Cx<T>::type;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ncols) return;
+  C2 *col = A + (c0 + t) * ld + r0;
+  C2 x[W];
+#pragma unroll
+  for (int i = 0; i < W; i++) {
+    x[i].x = 0;
+    x[i].y = 0;
+    if (i < w) x[i] = col[i];
+  }
+#pragma unroll
+  for (int k = 0; k < W; k++) {
+    const C2 xk = x[k];
+    const C2 *lk = A + (r0 + k) * ld + r0;
+#pragma unroll
+    for (int i = k + 1; i < W; i++) {
+      if (i < w) {
+        const C2 l = lk[i];
+        x[i].x -= l.x * xk.x - l.y * xk.y;
+        x[i].y -= l.x * xk.y + l.y * xk.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < W; i++)
+    if (i < w) col[i] = x[i];
+}
+
+// X := inv(L) X with L the unit lower w x w block at (r0, r0), w <= NB,
+// processed in IB-row slices: small triangle then MFMA update of the rows
+// below within the block.
+template <typename T>
+static int trsm_block(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t r0, int64_t w, int64_t c0,
+                      int64_t ncols, GemmStats *st) {
+  if (ncols <= 0) return 0;
+  for (int64_t b = 0; b < w; b += IB) {
+    const int bw = (int)std::min<int64_t>(IB, w - b);
+    hipLaunchKernelGGL((trsm_unit_lower_kernel<T, IB>), dim3((unsigned)((ncols + 127) / 128)), dim3(128), 0, s,
+                       A, ld, r0 + b, bw, c0, ncols);
+    MOM_HIP(hipGetLastError());
+    const int64_t below = w - b - bw;
+    if (below > 0) {
+      int rc = gemm_sub<T>(s, A + (r0 + b) * ld + r0 + b + bw, ld, A + c0 * ld + r0 + b, ld,
+                           A + c0 * ld + r0 + b + bw, ld, below, ncols, bw, st);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+// LU with partial pivoting of the n x n matrix A (ld), in place.
+template <typename T>
+static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, int *ipiv, PivotWork w,
+                 GemmStats *st) {
+  for (int64_t j0 = 0; j0 < n; j0 += NB) {
+    const int64_t jb = std::min<int64_t>(NB, n - j0);
+    for (int64_t i0 = j0; i0 < j0 + jb; i0 += IB) {
+      const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
+      for (int64_t j = i0; j < i0 + ib; j++) {
+        const int64_t rows = n - j;
+        const int nblk = (int)std::min<int64_t>(kPivBlocks, (rows + 255) / 256);
+        hipLaunchKernelGGL(pivot_kernel<T>, dim3(nblk), dim3(256), 0, s, A, ld, (int)j, (int)n, (int)j0,
+                           (int)(j0 + jb), ipiv, w);
+        MOM_HIP(hipGetLastError());
+        if (rows > 1) {
+          hipLaunchKernelGGL(column_update_kernel<T>, dim3((unsigned)((rows - 1 + 255) / 256)), dim3(256), 0, s,
+                             A, ld, (int)j, (int)n, (int)(i0 + ib));
+          MOM_HIP(hipGetLastError());
+        }
+      }
+      const int64_t rest = j0 + jb - (i0 + ib);
+      if (rest > 0) {
+        // U block of the remaining panel columns, then their update
+        hipLaunchKernelGGL((trsm_unit_lower_kernel<T, IB>), dim3((unsigned)((rest + 127) / 128)), dim3(128), 0, s,
+                           A, ld, i0, (int)ib, i0 + ib, rest);
+        MOM_HIP(hipGetLastError());
+        int rc = gemm_sub<T>(s, A + i0 * ld + i0 + ib, ld, A + (i0 + ib) * ld + i0, ld,
+                             A + (i0 + ib) * ld + i0 + ib, ld, n - i0 - ib, rest, ib, st);
+        if (rc) return rc;
+      }
+    }
+    // row interchanges of this panel on the columns left and right of it
+    const int64_t left = j0, right = n - (j0 + jb);
+    if (left + right > 0) {
+      hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)((left + right + 255) / 256)), dim3(256), 0, s, A, ld,
+                         (int)j0, (int)(j0 + jb), ipiv, 0, (int)j0, (int)(j0 + jb), (int)n);
+      MOM_HIP(hipGetLastError());
+    }
+    if (right > 0) {
+      int rc = trsm_block<T>(s, A, ld, j0, jb, j0 + jb, right, st);
+      if (rc) return rc;
+      rc = gemm_sub<T>(s, A + j0 * ld + j0 + jb, ld, A + (j0 + jb) * ld + j0, ld, A + (j0 + jb) * ld + j0 + jb, ld,
+                       n - j0 - jb, right, jb, st);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Inverses of the NB x NB diagonal blocks of L (unit lower) and U (upper),
+// stored as nblk consecutive NB x NB column-major blocks.  One workgroup
+// per block and per factor; column c of the inverse is solved by thread c.
+template <typename T>
+__global__ __launch_bounds__(256) void diag_inverse_kernel(const typename Cx<T>::type *__restrict__ A, int64_t ld,
+                                                            int64_t n, typename Cx<T>::type *Linv,
+                                                            typename Cx<T>::type *Uinv) {
+  using C2 = typename Cx<T>::type;
+  const int b = blockIdx.x;
+  const bool upper = blockIdx.y == 1;
+  const int64_t r0 = (int64_t)b * NB;
+  const int w = (int)std::min<int64_t>(NB, n - r0);
+  C2 *out = (upper ? Uinv : Linv) + (int64_t)b * NB * NB;
+  const int c = threadIdx.x;
+  if (c >= NB) return;
+  // solve T x = e_c by substitution, writing x into out(:, c)
+  C2 *x = out + (int64_t)c * NB;
+  for (int i = 0; i < NB; i++) {
+    x[i].x = 0;
+    x[i].y = 0;
+  }
+  if (c >= w) {
+    x[c].x = 1;  // identity padding outside the matrix
+    return;
+  }
+  if (!upper) {
+    x[c].x = 1;
+    for (int k = c; k < w; k++) {
+      const C2 xk = x[k];
+      const C2 *lk = A + (r0 + k) * ld + r0;
+      for (int i = k + 1; i < w; i++) {
+        const C2 l = lk[i];
+        x[i].x -= l.x * xk.x - l.y * xk.y;
+        x[i].y -= l.x * xk.y + l.y * xk.x;
+      }
+    }
+  } else {
+    for (int i = c; i >= 0; i--) {
+      // x_i = (delta_ic - sum_{k>i} U_ik x_k) / U_ii
+      C2 acc;
+      acc.x = (i == c) ? 1 : 0;
+      acc.y = 0;
+      for (int k = i + 1; k <= c; k++) {
+        const C2 u = A[(r0 + k) * ld + r0 + i];
+        acc.x -= u.x * x[k].x - u.y * x[k].y;
+        acc.y -= u.x * x[k].y + u.y * x[k].x;
+      }
+      const C2 d = A[(r0 + i) * ld + r0 + i];
+      const T den = d.x * d.x + d.y * d.y;
+      C2 q;
+      q.x = (acc.x * d.x + acc.y * d.y) / den;
+      q.y = (acc.y * d.x - acc.x * d.y) / den;
+      x[i] = q;
+    }
+  }
+}
+
+// y[r] = sum_c M[r, c] x[c] for an NB x NB block M (column-major), one
+// thread per row, nrhs columns of x / y (leading dims ldx, ldy).
+template <typename T>
+__global__ __launch_bounds__(256) void block_apply_kernel(const typename Cx<T>::type *__restrict__ M,
+                                                           const typename Cx<T>::type *__restrict__ x, int64_t ldx,
+                                                           typename Cx<T>::type *__restrict__ y, int64_t ldy,
+                                                           int w) {
+  using C2 = typename Cx<T>::type;
+  const int r = threadIdx.x;
+  const int64_t rhs = blockIdx.x;
+  __shared__ C2 xs[NB];
+  xs[r] = (r < w) ? x[rhs * ldx + r] : C2{0, 0};
+  __syncthreads();
+  if (r >= w) return;
+  C2 acc{0, 0};
+  for (int c = 0; c < w; c++) {
+    const C2 m = M[(int64_t)c * NB + r];
+    const C2 v = xs[c];
+    acc.x += m.x * v.x - m.y * v.y;
+    acc.y += m.x * v.y + m.y * v.x;
+  }
+  y[rhs * ldy + r] = acc;
+}
+
+// b[rows] -= A[rows, c0:c0+w] x_blk, rows [r0, r1); column-major A.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_gemv_sub_kernel(const typename Cx<T>::type *__restrict__ A, int64_t ld,
+                                                              int64_t r0, int64_t r1, int64_t c0, int w,
+                                                              const typename Cx<T>::type *__restrict__ xb,
+                                                              typename Cx<T>::type *b, int64_t ldb) {
+  using C2 = typename Cx<T>::type;
+  __shared__ C2 xs[NB];
+  const int64_t rhs = blockIdx.y;
+  for (int i = threadIdx.x; i < w; i += 256) xs[i] = xb[rhs * NB + i];
+  __syncthreads();
+  const int64_t r = r0 + blockIdx.x * 256 + threadIdx.x;
+  if (r >= r1) return;
+  C2 acc{0, 0};
+  const C2 *a = A + c0 * ld + r;
+  for (int c = 0; c < w; c++) {
+    const C2 m = a[(int64_t)c * ld];
+    const C2 v = xs[c];
+    acc.x += m.x * v.x - m.y * v.y;
+    acc.y += m.x * v.y + m.y * v.x;
+  }
+  C2 *p = b + rhs * ldb + r;
+  C2 o = *p;
+  o.x -= acc.x;
+  o.y -= acc.y;
+  *p = o;
+}
+
+// Solve L U x = P b in place on B (n x nrhs, ldb); perm is the row gather
+// (B_perm[i] = B[perm[i]]).  xb is NB x nrhs scratch, tmp n x nrhs scratch.
+template <typename T>
+static int getrs(hipStream_t s, const typename Cx<T>::type *LU, int64_t ld, int64_t n, const int *perm,
+                 const typename Cx<T>::type *Linv, const typename Cx<T>::type *Uinv, typename Cx<T>::type *B,
+                 int64_t ldb, int64_t nrhs, typename Cx<T>::type *tmp, typename Cx<T>::type *xb);
+
+template <typename T>
+__global__ void gather_copy_kernel(const typename Cx<T>::type *__restrict__ src, int64_t lds,
+                                   typename Cx<T>::type *__restrict__ dst, int64_t ldd, int w) {
+  const int r = threadIdx.x;
+  if (r < w) dst[blockIdx.x * ldd + r] = src[blockIdx.x * lds + r];
+}
+
+template <typename T>
+__global__ void gather_rows_kernel(const typename Cx<T>::type *__restrict__ src, int64_t lds,
+                                   typename Cx<T>::type *__restrict__ dst, int64_t ldd, const int *__restrict__ perm,
+                                   int64_t n, int64_t nrhs) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * nrhs) return;
+  const int64_t i = t % n, r = t / n;
+  dst[r * ldd + i] = src[r * lds + perm[i]];
+}
+
+template <typename T>
+static int getrs(hipStream_t s, const typename Cx<T>::type *LU, int64_t ld, int64_t n, const int *perm,
+                 const typename Cx<T>::type *Linv, const typename Cx<T>::type *Uinv, typename Cx<T>::type *B,
+                 int64_t ldb, int64_t nrhs, typename Cx<T>::type *tmp, typename Cx<T>::type *xb) {
+  // tmp = P B
+  hipLaunchKernelGGL(gather_rows_kernel<T>, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, B, ldb, tmp,
+                     n, perm, n, nrhs);
+  MOM_HIP(hipGetLastError());
+  const int64_t nblk = (n + NB - 1) / NB;
+  // forward: tmp_b := Linv_bb tmp_b ; tmp_below -= L_below,b tmp_b
+  for (int64_t b = 0; b < nblk; b++) {
+    const int64_t r0 = b * NB;
+    const int w = (int)std::min<int64_t>(NB, n - r0);
+    hipLaunchKernelGGL(block_apply_kernel<T>, dim3((unsigned)nrhs), dim3(NB), 0, s, Linv + b * NB * NB, tmp + r0,
+                       n, xb, (int64_t)NB, w);
+    MOM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gather_copy_kernel<T>, dim3((unsigned)nrhs), dim3(NB), 0, s, xb, (int64_t)NB, tmp + r0, n, w);
+    MOM_HIP(hipGetLastError());
+    if (r0 + w < n) {
+      hipLaunchKernelGGL(panel_gemv_sub_kernel<T>, dim3((unsigned)((n - r0 - w + 255) / 256), (unsigned)nrhs),
+                         dim3(256), 0, s, LU, ld, r0 + w, n, r0, w, xb, tmp, n);
+      MOM_HIP(hipGetLastError());
+    }
+  }
+  // backward: tmp_b := Uinv_bb tmp_b ; tmp_above -= U_above,b tmp_b
+  for (int64_t b = nblk - 1; b >= 0; b--) {
+    const int64_t r0 = b * NB;
+    const int w = (int)std::min<int64_t>(NB, n - r0);
+    hipLaunchKernelGGL(block_apply_kernel<T>, dim3((unsigned)nrhs), dim3(NB), 0, s, Uinv + b * NB * NB, tmp + r0,
+                       n, xb, (int64_t)NB, w);
+    MOM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gather_copy_kernel<T>, dim3((unsigned)nrhs), dim3(NB), 0, s, xb, (int64_t)NB, tmp + r0, n, w);
+    MOM_HIP(hipGetLastError());
+    if (r0 > 0) {
+      hipLaunchKernelGGL(panel_gemv_sub_kernel<T>, dim3((unsigned)((r0 + 255) / 256), (unsigned)nrhs), dim3(256), 0,
+                         s, LU, ld, (int64_t)0, r0, r0, w, xb, tmp, n);
+      MOM_HIP(hipGetLastError());
+    }
+  }
+  // B = tmp
+  MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(typename Cx<T>::type), tmp, n * sizeof(typename Cx<T>::type),
+                           n * sizeof(typename Cx<T>::type), nrhs, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+}  // namespace mom
+
+// ---------------------------------------------------------------------------
+// Complex128 matrix-vector kernels for the residual and the scaling.
+namespace mom {
+
+constexpr int kChunks = 32;  // column chunks of the chunked row kernels
+
+// part[chunk][r] = sum_{c in chunk} Z[r, c] x[c]
+__global__ __launch_bounds__(256) void zgemv_part_kernel(const double2 *__restrict__ Z, int64_t ld, int64_t n,
+                                                          const double2 *__restrict__ x, double2 *part) {
+  __shared__ double2 xs[2048];
+  const int64_t cw = (n + kChunks - 1) / kChunks;
+  const int64_t c0 = blockIdx.y * cw, c1 = std::min<int64_t>(n, c0 + cw);
+  for (int64_t c = c0 + threadIdx.x; c < c1; c += 256) xs[c - c0] = x[c];
+  __syncthreads();
+  const int64_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  double ar = 0, ai = 0;
+  const double2 *z = Z + c0 * ld + r;
+  for (int64_t c = c0; c < c1; c++, z += ld) {
+    const double2 m = *z, v = xs[c - c0];
+    ar += m.x * v.x - m.y * v.y;
+    ai += m.x * v.y + m.y * v.x;
+  }
+  part[blockIdx.y * n + r] = make_double2(ar, ai);
+}
+
+// y = b - sum_chunks part (b may be NULL: y = -Z x)
+__global__ void zgemv_reduce_kernel(const double2 *__restrict__ part, int64_t n, const double2 *__restrict__ b,
+                                    double2 *y) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double ar = 0, ai = 0;
+  for (int k = 0; k < kChunks; k++) {
+    ar += part[k * n + r].x;
+    ai += part[k * n + r].y;
+  }
+  double2 o = b ? b[r] : make_double2(0, 0);
+  y[r] = make_double2(o.x - ar, o.y - ai);
+}
+
+// part[chunk][r] = (max_c |Z[r,c]|, sum_c |Z[r,c]|) over the chunk
+__global__ __launch_bounds__(256) void row_stats_part_kernel(const double2 *__restrict__ Z, int64_t ld, int64_t n,
+                                                              double2 *part) {
+  const int64_t cw = (n + kChunks - 1) / kChunks;
+  const int64_t c0 = blockIdx.y * cw, c1 = std::min<int64_t>(n, c0 + cw);
+  const int64_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  double mx = 0, sm = 0;
+  const double2 *z = Z + c0 * ld + r;
+  for (int64_t c = c0; c < c1; c++, z += ld) {
+    const double2 m = *z;
+    const double a = sqrt(m.x * m.x + m.y * m.y);
+    mx = fmax(mx, a);
+    sm += a;
+  }
+  part[blockIdx.y * n + r] = make_double2(mx, sm);
+}
+
+// rscale[r] = 2^-ilogb(rowmax); rowsum[r] = sum_c |Z[r,c]|
+__global__ void row_stats_reduce_kernel(const double2 *__restrict__ part, int64_t n, double *rscale,
+                                        double *rowsum) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double mx = 0, sm = 0;
+  for (int k = 0; k < kChunks; k++) {
+    mx = fmax(mx, part[k * n + r].x);
+    sm += part[k * n + r].y;
+  }
+  rscale[r] = (mx > 0 && isfinite(mx)) ? ldexp(1.0, -ilogb(mx)) : 1.0;
+  rowsum[r] = sm;
+}
+
+// cscale[c] = 2^-ilogb(max_r rscale[r] |Z[r,c]|), one workgroup per column
+__global__ __launch_bounds__(256) void col_scale_kernel(const double2 *__restrict__ Z, int64_t ld, int64_t n,
+                                                         const double *__restrict__ rscale, double *cscale) {
+  __shared__ double sm[256];
+  const int64_t c = blockIdx.x;
+  double mx = 0;
+  const double2 *z = Z + c * ld;
+  for (int64_t r = threadIdx.x; r < n; r += 256) {
+    const double2 m = z[r];
+    mx = fmax(mx, rscale[r] * sqrt(m.x * m.x + m.y * m.y));
+  }
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cscale[c] = (sm[0] > 0 && isfinite(sm[0])) ? ldexp(1.0, -ilogb(sm[0])) : 1.0;
+}
+
+// Z32[r,c] = fl32(rscale[r] Z[r,c] cscale[c])
+__global__ __launch_bounds__(256) void to_fp32_kernel(const double2 *__restrict__ Z, int64_t ld, int64_t n,
+                                                       const double *__restrict__ rscale,
+                                                       const double *__restrict__ cscale, float2 *Z32) {
+  const int64_t c = blockIdx.y;
+  const double cs = cscale[c];
+  const double2 *z = Z + c * ld;
+  float2 *o = Z32 + c * n;
+  for (int64_t r = blockIdx.x * 256 + threadIdx.x; r < n; r += (int64_t)gridDim.x * 256) {
+    const double s = rscale[r] * cs;
+    const double2 m = z[r];
+    o[r] = make_float2((float)(m.x * s), (float)(m.y * s));
+  }
+}
+
+// t32 = fl32(rscale .* v)
+__global__ void scale_down_kernel(const double2 *__restrict__ v, const double *__restrict__ rscale, int64_t n,
+                                  float2 *t) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  t[r] = make_float2((float)(v[r].x * rscale[r]), (float)(v[r].y * rscale[r]));
+}
+
+// y = cscale .* t32
+__global__ void scale_up_kernel(const float2 *__restrict__ t, const double *__restrict__ cscale, int64_t n,
+                                double2 *y) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  y[r] = make_double2((double)t[r].x * cscale[r], (double)t[r].y * cscale[r]);
+}
+
+template <typename T>
+static int diag_inverses(hipStream_t s, const typename Cx<T>::type *LU, int64_t ld, int64_t n,
+                         typename Cx<T>::type *Linv, typename Cx<T>::type *Uinv) {
+  const int64_t nblk = (n + NB - 1) / NB;
+  hipLaunchKernelGGL(diag_inverse_kernel<T>, dim3((unsigned)nblk, 2), dim3(NB), 0, s, LU, ld, n, Linv, Uinv);
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
+static double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace mom
+
+struct mom_system {
+  mom_ctx *ctx = nullptr;
+  int64_t N = 0;  // nedges_total
+  int64_t n = 0;  // 2N
+  int precision = MOM_PREC_FP64;
+  bool filled = false, factored = false;
+  mom::DevBuf Z64, Z32, ipiv, Linv, Uinv, rscale, cscale, rowsum, part, pivwork;
+  std::vector<int> perm;
+  double znorm_inf = 0;
+  double t_fill = 0, t_factor = 0, t_solve = 0;
+  mom::GemmStats gemm;
+  int info = 0;
+};
+
+namespace mom {
+
+static int ensure_pivwork(mom_system *S, PivotWork *w) {
+  const size_t bytes = kPivBlocks * (sizeof(double) + sizeof(int)) + 64;
+  if (S->pivwork.bytes < bytes) {
+    int rc = S->pivwork.alloc(bytes);
+    if (rc) return rc;
+  }
+  char *base = S->pivwork.as<char>();
+  w->part_val = (double *)base;
+  w->part_idx = (int *)(base + kPivBlocks * sizeof(double));
+  w->counter = (int *)(base + kPivBlocks * (sizeof(double) + sizeof(int)));
+  w->info = w->counter + 1;
+  MOM_HIP(hipMemsetAsync(w->counter, 0, 2 * sizeof(int), S->ctx->stream));
+  return 0;
+}
+
+// Factor the n x n matrix held by the system (complex128 in place, or the
+// scaled complex64 copy), build the permutation and diagonal-block inverses.
+template <typename T>
+static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  PivotWork w;
+  int rc;
+  if ((rc = ensure_pivwork(S, &w))) return rc;
+  if (S->gemm.enabled) S->gemm.reset();
+  if ((rc = getrf<T>(s, A, n, n, S->ipiv.as<int>(), w, &S->gemm))) return rc;
+  const int64_t nblk = (n + NB - 1) / NB;
+  const size_t inv_bytes = sizeof(typename Cx<T>::type) * nblk * NB * NB;
+  if (S->Linv.bytes < inv_bytes && ((rc = S->Linv.alloc(inv_bytes)) || (rc = S->Uinv.alloc(inv_bytes)))) return rc;
+  if ((rc = diag_inverses<T>(s, A, n, n, S->Linv.as<typename Cx<T>::type>(), S->Uinv.as<typename Cx<T>::type>())))
+    return rc;
+  std::vector<int> ip(n);
+  int info = 0;
+  MOM_HIP(hipMemcpyAsync(ip.data(), S->ipiv.ptr, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  MOM_HIP(hipMemcpyAsync(&info, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
+  MOM_HIP(hipStreamSynchronize(s));
+  if (S->gemm.enabled && (rc = S->gemm.collect())) return rc;
+  S->perm.resize(n);
+  for (int64_t i = 0; i < n; i++) S->perm[i] = (int)i;
+  for (int64_t k = 0; k < n; k++) std::swap(S->perm[k], S->perm[ip[k] - 1]);
+  S->info = info;
+  return 0;
+}
+
+static int upload_perm(mom_system *S, DevBuf &dperm) {
+  int rc = dperm.alloc(sizeof(int) * S->n);
+  if (rc) return rc;
+  MOM_HIP(hipMemcpyAsync(dperm.ptr, S->perm.data(), sizeof(int) * S->n, hipMemcpyHostToDevice, S->ctx->stream));
+  return 0;
+}
+
+// Mixed precision solve of one right-hand side b (host, complex128) into x.
+struct IrWork {
+  DevBuf dperm, v64, w64, t32, tmp32, xb32, b64, x64;
+};
+
+static int apply_minv(mom_system *S, IrWork &W, const std::vector<cd> &v, std::vector<cd> &y) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  MOM_HIP(hipMemcpyAsync(W.v64.ptr, v.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+  const unsigned g = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(scale_down_kernel, dim3(g), dim3(256), 0, s, W.v64.as<double2>(), S->rscale.as<double>(), n,
+                     W.t32.as<float2>());
+  int rc = getrs<float>(s, S->Z32.as<float2>(), n, n, W.dperm.as<int>(), S->Linv.as<float2>(), S->Uinv.as<float2>(),
+                        W.t32.as<float2>(), n, 1, W.tmp32.as<float2>(), W.xb32.as<float2>());
+  if (rc) return rc;
+  hipLaunchKernelGGL(scale_up_kernel, dim3(g), dim3(256), 0, s, W.t32.as<float2>(), S->cscale.as<double>(), n,
+                     W.w64.as<double2>());
+  y.resize(n);
+  MOM_HIP(hipMemcpyAsync(y.data(), W.w64.ptr, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+  MOM_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// y = b - Z x (b may be NULL: y = -Z x); x, y on the host
+static int residual(mom_system *S, IrWork &W, const std::vector<cd> *b, const std::vector<cd> &x,
+                    std::vector<cd> &y) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  MOM_HIP(hipMemcpyAsync(W.x64.ptr, x.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+  if (b) MOM_HIP(hipMemcpyAsync(W.b64.ptr, b->data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(zgemv_part_kernel, dim3((unsigned)((n + 255) / 256), kChunks), dim3(256), 0, s,
+                     S->Z64.as<double2>(), n, n, W.x64.as<double2>(), S->part.as<double2>());
+  hipLaunchKernelGGL(zgemv_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                     S->part.as<double2>(), n, b ? W.b64.as<double2>() : nullptr, W.w64.as<double2>());
+  MOM_HIP(hipGetLastError());
+  y.resize(n);
+  MOM_HIP(hipMemcpyAsync(y.data(), W.w64.ptr, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+  MOM_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+static double inf_norm(const std::vector<cd> &v) {
+  double m = 0;
+  for (const cd &c : v) m = std::max(m, std::abs(c));
+  return m;
+}
+
+static cd dotc(const std::vector<cd> &a, const std::vector<cd> &b) {
+  cd s = 0;
+  for (size_t i = 0; i < a.size(); i++) s += std::conj(a[i]) * b[i];
+  return s;
+}
+
+// GMRES-IR for one right-hand side.  Outer loop: r = b - Z x in FP64; inner:
+// restarted GMRES on M^-1 Z d = M^-1 r with the FP32 factors as M.
+static int gmres_ir(mom_system *S, IrWork &W, const std::vector<cd> &b, std::vector<cd> &x, double tol, int maxit,
+                    int *iters, double *berr) {
+  const int64_t n = S->n;
+  int rc;
+  if ((rc = apply_minv(S, W, b, x))) return rc;
+  const double bn = inf_norm(b);
+  const int m = 20;
+  std::vector<std::vector<cd>> V(m + 1);
+  std::vector<cd> r, z, w, H((m + 1) * m), g(m + 1), cs(m), sn(m);
+  double be = 0;
+  int it = 0;
+  for (;; it++) {
+    if ((rc = residual(S, W, &b, x, r))) return rc;
+    be = inf_norm(r) / (S->znorm_inf * inf_norm(x) + bn);
+    if (be <= tol || it >= maxit) break;
+    if ((rc = apply_minv(S, W, r, z))) return rc;
+    double beta = std::sqrt(std::real(dotc(z, z)));
+    if (beta == 0) break;
+    V[0].resize(n);
+    for (int64_t i = 0; i < n; i++) V[0][i] = z[i] / beta;
+    std::fill(g.begin(), g.end(), cd(0));
+    g[0] = beta;
+    int k = 0;
+    for (; k < m; k++) {
+      // w = M^-1 Z v_k  (residual() returns -Z v)
+      if ((rc = residual(S, W, nullptr, V[k], w))) return rc;
+      for (auto &c : w) c = -c;
+      if ((rc = apply_minv(S, W, w, z))) return rc;
+      for (int i = 0; i <= k; i++) {
+        cd h = dotc(V[i], z);
+        H[i * m + k] = h;
+        for (int64_t j = 0; j < n; j++) z[j] -= h * V[i][j];
+      }
+      double hn = std::sqrt(std::real(dotc(z, z)));
+      H[(k + 1) * m + k] = hn;
+      V[k + 1].resize(n);
+      for (int64_t j = 0; j < n; j++) V[k + 1][j] = hn > 0 ? z[j] / hn : cd(0);
+      for (int i = 0; i < k; i++) {
+        cd a = H[i * m + k], c2 = H[(i + 1) * m + k];
+        H[i * m + k] = std::conj(cs[i]) * a + std::conj(sn[i]) * c2;
+        H[(i + 1) * m + k] = -sn[i] * a + cs[i] * c2;
+      }
+      cd a = H[k * m + k], c2 = H[(k + 1) * m + k];
+      double den = std::sqrt(std::norm(a) + std::norm(c2));
+      cs[k] = den > 0 ? a / den : cd(1);
+      sn[k] = den > 0 ? c2 / den : cd(0);
+      H[k * m + k] = std::conj(cs[k]) * a + std::conj(sn[k]) * c2;
+      H[(k + 1) * m + k] = 0;
+      cd gk = g[k];
+      g[k] = std::conj(cs[k]) * gk;
+      g[k + 1] = -sn[k] * gk;
+      if (std::abs(g[k + 1]) <= 1e-6 * beta || hn == 0) {
+        k++;
+        break;
+      }
+    }
+    // back substitution H y = g, x += V y
+    std::vector<cd> y(k);
+    for (int i = k - 1; i >= 0; i--) {
+      cd acc = g[i];
+      for (int j = i + 1; j < k; j++) acc -= H[i * m + j] * y[j];
+      y[i] = acc / H[i * m + i];
+    }
+    for (int i = 0; i < k; i++)
+      for (int64_t j = 0; j < n; j++) x[j] += y[i] * V[i][j];
+  }
+  if (iters) *iters = it;
+  if (berr) *berr = be;
+  return be <= tol ? 0 : fail(MOM_ERR_NOCONV, "GMRES-IR did not reach the backward-error tolerance");
+}
+
+static int ir_alloc(mom_system *S, IrWork &W) {
+  const int64_t n = S->n;
+  int rc;
+  if ((rc = upload_perm(S, W.dperm)) || (rc = W.v64.alloc(sizeof(double2) * n)) ||
+      (rc = W.w64.alloc(sizeof(double2) * n)) || (rc = W.b64.alloc(sizeof(double2) * n)) ||
+      (rc = W.x64.alloc(sizeof(double2) * n)) || (rc = W.t32.alloc(sizeof(float2) * n)) ||
+      (rc = W.tmp32.alloc(sizeof(float2) * n)) || (rc = W.xb32.alloc(sizeof(float2) * NB)))
+    return rc;
+  return 0;
+}
+
+}  // namespace mom
+
+using namespace mom;
+
+extern "C" {
+
+int mom_system_create(mom_ctx *ctx, int64_t nedges_total, int32_t precision, mom_system **sys) {
+  if (!ctx || !sys) return fail(MOM_ERR_ARG, "NULL argument");
+  if (nedges_total < 1 || nedges_total > (1 << 29)) return fail(MOM_ERR_ARG, "nedges_total out of range");
+  if (precision != MOM_PREC_FP64 && precision != MOM_PREC_MIXED)
+    return fail(MOM_ERR_ARG, "precision must be MOM_PREC_FP64 or MOM_PREC_MIXED");
+  (void)hipSetDevice(ctx->device);
+  mom_system *S = new mom_system;
+  S->ctx = ctx;
+  S->N = nedges_total;
+  S->n = 2 * nedges_total;
+  S->precision = precision;
+  const int64_t n = S->n;
+  int rc;
+  if ((rc = S->Z64.alloc(sizeof(double2) * n * n)) || (rc = S->ipiv.alloc(sizeof(int) * n)) ||
+      (rc = S->part.alloc(sizeof(double2) * n * kChunks)) || (rc = S->rscale.alloc(sizeof(double) * n)) ||
+      (rc = S->cscale.alloc(sizeof(double) * n)) || (rc = S->rowsum.alloc(sizeof(double) * n)) ||
+      (precision == MOM_PREC_MIXED && (rc = S->Z32.alloc(sizeof(float2) * n * n)))) {
+    delete S;
+    return rc;
+  }
+  *sys = S;
+  return 0;
+}
+
+int mom_system_destroy(mom_system *sys) {
+  if (!sys) return 0;
+  (void)hipSetDevice(sys->ctx->device);
+  (void)hipStreamSynchronize(sys->ctx->stream);
+  delete sys;
+  return 0;
+}
+
+int mom_system_profile(mom_system *sys, int32_t enable) {
+  if (!sys) return fail(MOM_ERR_ARG, "NULL argument");
+  sys->gemm.enabled = enable != 0;
+  return 0;
+}
+
+int mom_system_fill(mom_system *S, int32_t ndom, const mom_mesh *const *doms, const double *ri, double omega,
+                    int32_t nq, const double *qw, const double *qnodes) {
+  if (!S) return fail(MOM_ERR_ARG, "NULL system");
+  (void)hipSetDevice(S->ctx->device);
+  const double t0 = now_s();
+  int rc = fill_system(S->ctx, ndom, doms, ri, S->N, omega, nq, qw, qnodes, S->Z64.as<double2>(), S->n);
+  if (rc) return rc;
+  MOM_HIP(hipStreamSynchronize(S->ctx->stream));
+  S->t_fill = now_s() - t0;
+  S->filled = true;
+  S->factored = false;
+  return 0;
+}
+
+int mom_system_get_matrix(mom_system *S, double *A, int64_t lda) {
+  if (!S || !A) return fail(MOM_ERR_ARG, "NULL argument");
+  if (lda < S->n) return fail(MOM_ERR_ARG, "lda < 2N");
+  if (!S->filled || (S->factored && S->precision == MOM_PREC_FP64))
+    return fail(MOM_ERR_STATE, "no unfactored matrix on the device");
+  const int64_t n = S->n;
+  MOM_HIP(hipMemcpy2DAsync(A, lda * sizeof(double2), S->Z64.ptr, n * sizeof(double2), n * sizeof(double2), n,
+                           hipMemcpyDeviceToHost, S->ctx->stream));
+  MOM_HIP(hipStreamSynchronize(S->ctx->stream));
+  return 0;
+}
+
+int mom_system_set_matrix(mom_system *S, const double *A, int64_t lda) {
+  if (!S || !A) return fail(MOM_ERR_ARG, "NULL argument");
+  if (lda < S->n) return fail(MOM_ERR_ARG, "lda < 2N");
+  const int64_t n = S->n;
+  MOM_HIP(hipMemcpy2DAsync(S->Z64.ptr, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n,
+                           hipMemcpyHostToDevice, S->ctx->stream));
+  MOM_HIP(hipStreamSynchronize(S->ctx->stream));
+  S->filled = true;
+  S->factored = false;
+  return 0;
+}
+
+int mom_system_factor(mom_system *S) {
+  if (!S) return fail(MOM_ERR_ARG, "NULL system");
+  if (!S->filled) return fail(MOM_ERR_STATE, "factor before fill");
+  if (S->factored) return fail(MOM_ERR_STATE, "already factored");
+  (void)hipSetDevice(S->ctx->device);
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  const double t0 = now_s();
+  int rc;
+  if (S->precision == MOM_PREC_FP64) {
+    rc = factor_impl<double>(S, S->Z64.as<double2>());
+  } else {
+    hipLaunchKernelGGL(row_stats_part_kernel, dim3((unsigned)((n + 255) / 256), kChunks), dim3(256), 0, s,
+                       S->Z64.as<double2>(), n, n, S->part.as<double2>());
+    hipLaunchKernelGGL(row_stats_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       S->part.as<double2>(), n, S->rscale.as<double>(), S->rowsum.as<double>());
+    hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)n), dim3(256), 0, s, S->Z64.as<double2>(), n, n,
+                       S->rscale.as<double>(), S->cscale.as<double>());
+    hipLaunchKernelGGL(to_fp32_kernel, dim3((unsigned)std::min<int64_t>(64, (n + 255) / 256), (unsigned)n),
+                       dim3(256), 0, s, S->Z64.as<double2>(), n, n, S->rscale.as<double>(), S->cscale.as<double>(),
+                       S->Z32.as<float2>());
+    MOM_HIP(hipGetLastError());
+    std::vector<double> rs(n);
+    MOM_HIP(hipMemcpyAsync(rs.data(), S->rowsum.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    MOM_HIP(hipStreamSynchronize(s));
+    S->znorm_inf = *std::max_element(rs.begin(), rs.end());
+    rc = factor_impl<float>(S, S->Z32.as<float2>());
+  }
+  if (rc) return rc;
+  S->t_factor = now_s() - t0;
+  S->factored = true;
+  return S->info;
+}
+
+int mom_system_solve(mom_system *S, int64_t nrhs, double *B, int64_t ldb, double tol, int32_t maxit,
+                     int32_t *iters, double *berr) {
+  if (!S || !B) return fail(MOM_ERR_ARG, "NULL argument");
+  if (!S->factored) return fail(MOM_ERR_STATE, "solve before factor");
+  if (nrhs < 1 || ldb < S->n) return fail(MOM_ERR_ARG, "bad sizes");
+  (void)hipSetDevice(S->ctx->device);
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  const double t0 = now_s();
+  int rc;
+  if (S->precision == MOM_PREC_FP64) {
+    DevBuf dB, tmp, xb, dperm;
+    if ((rc = dB.alloc(sizeof(double2) * n * nrhs)) || (rc = tmp.alloc(sizeof(double2) * n * nrhs)) ||
+        (rc = xb.alloc(sizeof(double2) * NB * nrhs)) || (rc = upload_perm(S, dperm)))
+      return rc;
+    MOM_HIP(hipMemcpy2DAsync(dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs,
+                             hipMemcpyHostToDevice, s));
+    if ((rc = getrs<double>(s, S->Z64.as<double2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
+                            S->Uinv.as<double2>(), dB.as<double2>(), n, nrhs, tmp.as<double2>(), xb.as<double2>())))
+      return rc;
+    MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs,
+                             hipMemcpyDeviceToHost, s));
+    MOM_HIP(hipStreamSynchronize(s));
+    if (iters) *iters = 0;
+    if (berr) *berr = -1.0;
+  } else {
+    IrWork W;
+    if ((rc = ir_alloc(S, W))) return rc;
+    int itmax = 0;
+    double bemax = 0;
+    for (int64_t r = 0; r < nrhs; r++) {
+      std::vector<cd> b(n), x;
+      std::memcpy(b.data(), B + 2 * r * ldb, sizeof(double2) * n);
+      int it = 0;
+      double be = 0;
+      rc = gmres_ir(S, W, b, x, tol, maxit, &it, &be);
+      itmax = std::max(itmax, it);
+      bemax = std::max(bemax, be);
+      std::memcpy(B + 2 * r * ldb, x.data(), sizeof(double2) * n);
+      if (rc && rc != MOM_ERR_NOCONV) return rc;
+    }
+    if (iters) *iters = itmax;
+    if (berr) *berr = bemax;
+    if (bemax > tol) return fail(MOM_ERR_NOCONV, "GMRES-IR did not reach the backward-error tolerance");
+  }
+  S->t_solve = now_s() - t0;
+  return 0;
+}
+
+int mom_system_solve_pw(mom_system *S, const mom_mesh *mesh, double omega, double ri_re, double ri_im, int64_t nsrc,
+                        const double *angles, int32_t nq, const double *qw, const double *qnodes, double tol,
+                        int32_t maxit, double *X, int64_t ldx, int32_t *iters, double *berr) {
+  if (!S || !mesh || !X) return fail(MOM_ERR_ARG, "NULL argument");
+  if (ldx < S->n || nsrc < 1) return fail(MOM_ERR_ARG, "bad sizes");
+  int rc = mom_srcvec_pw(S->ctx, mesh, S->N, omega, ri_re, ri_im, nsrc, angles, nq, qw, qnodes, X, ldx);
+  if (rc) return rc;
+  return mom_system_solve(S, nsrc, X, ldx, tol, maxit, iters, berr);
+}
+
+int mom_system_stats(const mom_system *S, double *out) {
+  if (!S || !out) return fail(MOM_ERR_ARG, "NULL argument");
+  out[0] = S->t_fill;
+  out[1] = S->t_factor;
+  out[2] = S->t_solve;
+  out[3] = S->gemm.seconds;
+  out[4] = S->gemm.launches;
+  out[5] = S->gemm.flops;
+  return 0;
+}
+
+int mom_solve_multi_linsys(mom_ctx *ctx, int64_t n, double *A, int64_t lda, int64_t nrhs, double *B, int64_t ldb,
+                           int32_t *ipiv_out) {
+  if (!ctx || !A || !B) return fail(MOM_ERR_ARG, "NULL argument");
+  if (n < 1 || lda < n || ldb < n || nrhs < 1) return fail(MOM_ERR_ARG, "bad sizes");
+  (void)hipSetDevice(ctx->device);
+  hipStream_t s = ctx->stream;
+  mom_system S;
+  S.ctx = ctx;
+  S.n = n;
+  S.N = (n + 1) / 2;
+  int rc;
+  if ((rc = S.Z64.alloc(sizeof(double2) * n * n)) || (rc = S.ipiv.alloc(sizeof(int) * n))) return rc;
+  MOM_HIP(hipMemcpy2DAsync(S.Z64.ptr, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n,
+                           hipMemcpyHostToDevice, s));
+  if ((rc = factor_impl<double>(&S, S.Z64.as<double2>()))) return rc;
+  S.factored = true;
+  S.precision = MOM_PREC_FP64;
+  int info = S.info;
+  DevBuf dB, tmp, xb, dperm;
+  if ((rc = dB.alloc(sizeof(double2) * n * nrhs)) || (rc = tmp.alloc(sizeof(double2) * n * nrhs)) ||
+      (rc = xb.alloc(sizeof(double2) * NB * nrhs)) || (rc = upload_perm(&S, dperm)))
+    return rc;
+  if (info == 0) {
+    MOM_HIP(hipMemcpy2DAsync(dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs,
+                             hipMemcpyHostToDevice, s));
+    if ((rc = getrs<double>(s, S.Z64.as<double2>(), n, n, dperm.as<int>(), S.Linv.as<double2>(), S.Uinv.as<double2>(),
+                            dB.as<double2>(), n, nrhs, tmp.as<double2>(), xb.as<double2>())))
+      return rc;
+    MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs,
+                             hipMemcpyDeviceToHost, s));
+  }
+  MOM_HIP(hipMemcpy2DAsync(A, lda * sizeof(double2), S.Z64.ptr, n * sizeof(double2), n * sizeof(double2), n,
+                           hipMemcpyDeviceToHost, s));
+  if (ipiv_out) MOM_HIP(hipMemcpyAsync(ipiv_out, S.ipiv.ptr, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  MOM_HIP(hipStreamSynchronize(s));
+  return info;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Test hooks (include/mom_gpu_testing.h).
+namespace mom {
+template <typename T>
+__global__ void mfma_probe_kernel(double *out) {
+  using acc4 = typename Cx<T>::acc4;
+  const int L0 = blockIdx.x / 64, L1 = blockIdx.x % 64, lane = threadIdx.x;
+  const T a = lane == L0 ? (T)1 : (T)0, b = lane == L1 ? (T)1 : (T)0;
+  acc4 c;
+  for (int v = 0; v < 4; v++) c[v] = 0;
+  c = mfma4<T>(a, b, c);
+  for (int v = 0; v < 4; v++) out[((int64_t)blockIdx.x * 64 + lane) * 4 + v] = (double)c[v];
+}
+
+__global__ void widen_kernel(const float2 *in, double2 *out, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_double2(in[i].x, in[i].y);
+}
+__global__ void narrow_kernel(const double2 *in, float2 *out, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+}  // namespace mom
+
+extern "C" int mom_test_mfma_layout(mom_ctx *ctx, int32_t precision, double *out) {
+  if (!ctx || !out) return fail(MOM_ERR_ARG, "NULL argument");
+  DevBuf d;
+  const size_t cnt = 64 * 64 * 64 * 4;
+  int rc = d.alloc(cnt * sizeof(double));
+  if (rc) return rc;
+  if (precision == 64)
+    hipLaunchKernelGGL(mfma_probe_kernel<double>, dim3(4096), dim3(64), 0, ctx->stream, d.as<double>());
+  else
+    hipLaunchKernelGGL(mfma_probe_kernel<float>, dim3(4096), dim3(64), 0, ctx->stream, d.as<double>());
+  MOM_HIP(hipGetLastError());
+  MOM_HIP(hipMemcpyAsync(out, d.ptr, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int64_t N, int64_t K, const double *A,
+                                 int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc) {
+  if (!ctx || !A || !B || !C) return fail(MOM_ERR_ARG, "NULL argument");
+  if (M < 1 || N < 1 || K < 1 || lda < M || ldb < K || ldc < M) return fail(MOM_ERR_ARG, "bad sizes");
+  hipStream_t s = ctx->stream;
+  DevBuf dA, dB, dC;
+  int rc;
+  if ((rc = dA.alloc(sizeof(double2) * lda * K)) || (rc = dB.alloc(sizeof(double2) * ldb * N)) ||
+      (rc = dC.alloc(sizeof(double2) * ldc * N)))
+    return rc;
+  MOM_HIP(hipMemcpyAsync(dA.ptr, A, sizeof(double2) * lda * K, hipMemcpyHostToDevice, s));
+  MOM_HIP(hipMemcpyAsync(dB.ptr, B, sizeof(double2) * ldb * N, hipMemcpyHostToDevice, s));
+  MOM_HIP(hipMemcpyAsync(dC.ptr, C, sizeof(double2) * ldc * N, hipMemcpyHostToDevice, s));
+  if (precision == 64) {
+    if ((rc = gemm_sub<double>(s, dA.as<double2>(), lda, dB.as<double2>(), ldb, dC.as<double2>(), ldc, M, N, K)))
+      return rc;
+  } else {
+    DevBuf fA, fB, fC;
+    if ((rc = fA.alloc(sizeof(float2) * lda * K)) || (rc = fB.alloc(sizeof(float2) * ldb * N)) ||
+        (rc = fC.alloc(sizeof(float2) * ldc * N)))
+      return rc;
+    auto nar = [&](DevBuf &in, DevBuf &out, int64_t cnt) {
+      hipLaunchKernelGGL(narrow_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, in.as<double2>(),
+                         out.as<float2>(), cnt);
+    };
+    nar(dA, fA, lda * K);
+    nar(dB, fB, ldb * N);
+    nar(dC, fC, ldc * N);
+    if ((rc = gemm_sub<float>(s, fA.as<float2>(), lda, fB.as<float2>(), ldb, fC.as<float2>(), ldc, M, N, K)))
+      return rc;
+    hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((ldc * N + 255) / 256)), dim3(256), 0, s, fC.as<float2>(),
+                       dC.as<double2>(), ldc * N);
+  }
+  MOM_HIP(hipGetLastError());
+  MOM_HIP(hipMemcpyAsync(C, dC.ptr, sizeof(double2) * ldc * N, hipMemcpyDeviceToHost, s));
+  MOM_HIP(hipStreamSynchronize(s));
+  return 0;
+}

@@ -1,0 +1,107 @@
+// Internal declarations of libmomgpu (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mom_gpu.h"
+#include "../../include/mom_gpu_testing.h"
+
+namespace mom {
+
+// constants.f90:8-16.  pi is a default-REAL literal in the reference, so its
+// value is the float32 rounding of 3.141592653589 widened to double.
+#define MOM_PI ((double)3.141592653589f)
+constexpr double kEps0 = 8.8541878176e-12;
+constexpr double kC0 = 2.997924580003e8;
+constexpr double kEta0 = 3.767303134622e2;
+inline double mu0() { return 4 * MOM_PI * 1e-7; }
+
+constexpr int kMaxQ = 16;  // largest triangle rule supported by the kernels
+
+using cd = std::complex<double>;
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define MOM_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return ::mom::fail(MOM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Device buffer with RAII.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  int alloc(size_t n);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+}  // namespace mom
+
+struct mom_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  mom::DevBuf qp;       // quadrature points of the last mesh/rule (scratch)
+  mom::DevBuf scratch;  // general scratch
+};
+
+// Device copy of one domain mesh, with per-face-side RWG data precomputed.
+struct mom_mesh {
+  mom_ctx *ctx = nullptr;
+  int64_t V = 0, F = 0, E = 0;
+  double avelen = 0;
+  int64_t nnear = 0;
+  int64_t max_parent = -1;  // largest global index among the mesh's edges
+  std::vector<double> h_fcp;  // host centroids (near list)
+  mom::DevBuf p, fnodes, fn, fm, fs, farea, fcp;
+  // per face side s = 3*f + e:
+  mom::DevBuf s_g;     // int32 parent (global) index of the edge
+  mom::DevBuf s_l;     // int32 local edge index
+  mom::DevBuf s_coef;  // sign * L/(2A): f(r) = coef * (r - v)
+  mom::DevBuf s_div;   // sign * L / A (surface divergence)
+  mom::DevBuf s_v;     // free vertex (3 doubles)
+  mom::DevBuf near;    // int32 pairs (test face, source face), sorted
+};
+
+namespace mom {
+
+struct QuadRule {
+  int Q;
+  double w[kMaxQ];
+  double nodes[3 * kMaxQ];  // (Q,3) column-major, as quad_data%nodes
+};
+int make_rule(int32_t nq, const double *qw, const double *qnodes, QuadRule *r);
+
+// Output of the fill kernels.
+struct FillTarget {
+  double2 *Z = nullptr;  // column-major complex matrix
+  int64_t ld = 0;
+  int64_t Noff = 0;   // offset of the M rows/columns (nedges_total)
+  int mode = 0;       // 0: PMCHWT blocks with global indices; 1/2/3: D/H/K, local indices
+  double2 zsi;        // Zsi = ri^2 / eta0^2 (mode 0)
+};
+
+// fill.hip
+int compute_quad_points(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule);
+int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
+                const FillTarget &t);
+int srcvec_pw(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
+              int64_t nsrc, const double *angles, int64_t Nt, double2 *q, int64_t ldq);
+int fill_system(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *ri,
+                int64_t nedges_total, double omega, int32_t nq, const double *qw,
+                const double *qnodes, double2 *Z, int64_t ldz);
+int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
+                const double2 *x, int64_t Nt, double th, double ph, double ps, double *partial);
+
+}  // namespace mom

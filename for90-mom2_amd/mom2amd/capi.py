@@ -1,0 +1,89 @@
+"""ctypes binding of the C-ABI in include/mom_gpu.h (lib/libmomgpu.so).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when the shared object is missing, and every call raises MomError on a
+non-zero status with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libmomgpu.so")
+
+MOM_PREC_FP64 = 64
+MOM_PREC_MIXED = 32
+
+# every symbol declared in include/mom_gpu.h
+EXPORTS = [
+    "mom_last_error", "mom_ctx_create", "mom_ctx_destroy", "mom_mesh_create", "mom_mesh_destroy",
+    "mom_mesh_near_pairs", "mom_compute_operator", "mom_sysmat_pmchwt", "mom_srcvec_pw",
+    "mom_solve_multi_linsys", "mom_system_create", "mom_system_destroy", "mom_system_fill",
+    "mom_system_get_matrix", "mom_system_set_matrix", "mom_system_factor", "mom_system_solve",
+    "mom_system_solve_pw", "mom_system_profile", "mom_system_stats", "mom_cs_prtsrf",
+]
+# test hooks declared in include/mom_gpu_testing.h
+TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub"]
+
+
+class MomError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("status %d: %s" % (status, msg))
+        self.status = status
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+_d = C.c_double
+
+
+def lib():
+    """Load libmomgpu.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmomgpu.so is not built: %s (run __graft_entry__.build())" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    L.mom_last_error.restype = C.c_char_p
+    L.mom_ctx_create.argtypes = [_i32, C.POINTER(_vp)]
+    L.mom_ctx_destroy.argtypes = [_vp]
+    L.mom_mesh_create.argtypes = [_vp, _i64, _dp, _i64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _i64, _ip, _ip,
+                                  _dp, _ip, _d, C.POINTER(_vp)]
+    L.mom_mesh_destroy.argtypes = [_vp]
+    L.mom_mesh_near_pairs.restype = _i64
+    L.mom_mesh_near_pairs.argtypes = [_vp]
+    L.mom_compute_operator.argtypes = [_vp, _vp, C.c_char, _d, _d, _d, _i32, _dp, _dp, _dp, _i64]
+    L.mom_sysmat_pmchwt.argtypes = [_vp, _i32, C.POINTER(_vp), _dp, _i64, _d, _i32, _dp, _dp, _dp, _i64]
+    L.mom_srcvec_pw.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i64, _dp, _i32, _dp, _dp, _dp, _i64]
+    L.mom_solve_multi_linsys.argtypes = [_vp, _i64, _dp, _i64, _i64, _dp, _i64, C.c_void_p]
+    L.mom_system_create.argtypes = [_vp, _i64, _i32, C.POINTER(_vp)]
+    L.mom_system_destroy.argtypes = [_vp]
+    L.mom_system_fill.argtypes = [_vp, _i32, C.POINTER(_vp), _dp, _d, _i32, _dp, _dp]
+    L.mom_system_get_matrix.argtypes = [_vp, _dp, _i64]
+    L.mom_system_set_matrix.argtypes = [_vp, _dp, _i64]
+    L.mom_system_factor.argtypes = [_vp]
+    L.mom_system_solve.argtypes = [_vp, _i64, _dp, _i64, _d, _i32, C.POINTER(_i32), C.POINTER(_d)]
+    L.mom_system_solve_pw.argtypes = [_vp, _vp, _d, _d, _d, _i64, _dp, _i32, _dp, _dp, _d, _i32, _dp, _i64,
+                                      C.POINTER(_i32), C.POINTER(_d)]
+    L.mom_system_stats.argtypes = [_vp, _dp]
+    L.mom_system_profile.argtypes = [_vp, _i32]
+    L.mom_test_mfma_layout.argtypes = [_vp, _i32, _dp]
+    L.mom_test_gemm_sub.argtypes = [_vp, _i32, _i64, _i64, _i64, _dp, _i64, _dp, _i64, _dp, _i64]
+    L.mom_cs_prtsrf.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _d, _d, _i32, _dp, _dp, C.POINTER(_d),
+                                C.POINTER(_d)]
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != 0:
+        raise MomError(status, lib().mom_last_error().decode(errors="replace"))
+    return status

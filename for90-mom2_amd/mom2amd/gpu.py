@@ -1,0 +1,243 @@
+"""Reference-named entry points of the hot path, running in libmomgpu.
+
+  sysmat_pmchwt        sysmat.f90:18-105      PMCHWT matrix (host copy)
+  computeD/H/K         sysmat.f90:272-519     operator matrices of one domain
+  srcvec               source.f90:136-247     plane-wave right-hand sides
+  solve_multi_linsys   linalg.f90:159-180     LU + solve (complex128)
+  cs_prtsrf            cs.f90:14-104          cross-sections
+  System               solver.f90:109-136     device-resident fill/factor/solve
+Errors raise capi.MomError, the counterpart of the reference's WRITE + STOP.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+
+import numpy as np
+
+from . import capi
+from .constants import tri_quad_data
+
+
+def _quad(quad):
+    w, nodes = tri_quad_data() if quad is None else quad
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    # quad_data%nodes is (Q,3) column-major
+    n = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).T).ravel()
+    return len(w), w, n
+
+
+class Context:
+    """One HIP stream on one device (mom_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        self._children = weakref.WeakSet()
+        capi.check(capi.lib().mom_ctx_create(device, C.byref(self.h)))
+
+    def _adopt(self, child):
+        self._children.add(child)
+
+    def close(self):
+        """Destroy the context; meshes and systems created on it are
+        destroyed first (they hold device memory and the context's stream)."""
+        if self.h:
+            for child in list(self._children):
+                child.close()
+            capi.lib().mom_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMesh:
+    """A domain mesh uploaded to the device (mom_mesh)."""
+
+    def __init__(self, ctx: Context, dm):
+        self.ctx = ctx
+        self.nedges = dm.nedges
+        self.nfaces = dm.nfaces
+        self.h = C.c_void_p()
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32).ravel()
+        capi.check(capi.lib().mom_mesh_create(
+            ctx.h, dm.nnodes, f64(dm.p), dm.nfaces, i32(dm.fnodes), i32(dm.fedges), f64(dm.fn), f64(dm.fcp),
+            f64(dm.fs), f64(dm.fm), f64(dm.farea), dm.nedges, i32(dm.efaces), i32(dm.ebn), f64(dm.elen),
+            i32(dm.eparent), float(dm.avelen), C.byref(self.h)))
+        ctx._adopt(self)
+
+    @property
+    def near_pairs(self):
+        return int(capi.lib().mom_mesh_near_pairs(self.h))
+
+    def close(self):
+        if self.h:
+            capi.lib().mom_mesh_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _operator(ctx, dmesh, op, omega, ri, quad):
+    Q, w, n = _quad(quad)
+    E = dmesh.nedges
+    M = np.zeros(2 * E * E)
+    capi.check(capi.lib().mom_compute_operator(ctx.h, dmesh.h, op.encode(), omega, complex(ri).real,
+                                               complex(ri).imag, Q, w, n, M, E))
+    return M.view(np.complex128).reshape(E, E, order="F")
+
+
+def computeD(ctx, dmesh, omega, ri, quad=None):
+    return _operator(ctx, dmesh, "D", omega, ri, quad)
+
+
+def computeH(ctx, dmesh, omega, ri, quad=None):
+    return _operator(ctx, dmesh, "H", omega, ri, quad)
+
+
+def computeK(ctx, dmesh, omega, ri, quad=None):
+    return _operator(ctx, dmesh, "K", omega, ri, quad)
+
+
+def _domain_handles(dmeshes):
+    arr = (C.c_void_p * len(dmeshes))(*[d.h.value for d in dmeshes])
+    return C.cast(arr, C.POINTER(C.c_void_p)), arr
+
+
+def _ri_array(ri_list):
+    return np.ascontiguousarray(np.array([[complex(r).real, complex(r).imag] for r in ri_list]).ravel())
+
+
+def sysmat_pmchwt(ctx, dmeshes, ri_list, nedges_total, omega, quad=None):
+    """PMCHWT matrix A (2N x 2N complex128) of the domains (sysmat.f90:18-105)."""
+    Q, w, n = _quad(quad)
+    N2 = 2 * nedges_total
+    A = np.zeros(2 * N2 * N2)
+    ptr, keep = _domain_handles(dmeshes)
+    capi.check(capi.lib().mom_sysmat_pmchwt(ctx.h, len(dmeshes), ptr, _ri_array(ri_list), nedges_total,
+                                            omega, Q, w, n, A, N2))
+    return A.view(np.complex128).reshape(N2, N2, order="F")
+
+
+def srcvec(ctx, dmesh, nedges_total, omega, ri, angles, quad=None):
+    """Plane-wave excitation vectors q (2N x nsrc) (source.f90:136-247).
+    angles: (nsrc, 3) array of (theta, phi, psi) in radians."""
+    Q, w, n = _quad(quad)
+    ang = np.ascontiguousarray(np.atleast_2d(np.asarray(angles, dtype=np.float64))).ravel()
+    nsrc = ang.size // 3
+    N2 = 2 * nedges_total
+    q = np.zeros(2 * N2 * nsrc)
+    capi.check(capi.lib().mom_srcvec_pw(ctx.h, dmesh.h, nedges_total, omega, complex(ri).real, complex(ri).imag,
+                                        nsrc, ang, Q, w, n, q, N2))
+    return q.view(np.complex128).reshape(N2, nsrc, order="F")
+
+
+def solve_multi_linsys(ctx, A, B):
+    """LU-factor A in place and solve A X = B (linalg.f90:159-180).
+    Returns (LU, X, ipiv) with ipiv 1-based as LAPACK's."""
+    A = np.asfortranarray(A, dtype=np.complex128).copy(order="F")
+    B2 = np.asfortranarray(np.atleast_2d(B.T).T, dtype=np.complex128).copy(order="F")
+    n = A.shape[0]
+    nrhs = B2.shape[1]
+    ipiv = np.zeros(n, dtype=np.int32)
+    Av = A.reshape(-1, order="F").view(np.float64)
+    Bv = B2.reshape(-1, order="F").view(np.float64)
+    st = capi.lib().mom_solve_multi_linsys(ctx.h, n, Av, n, nrhs, Bv, n, ipiv.ctypes.data)
+    if st > 0:
+        raise capi.MomError(st, "Linear system solution failed (singular matrix, INFO=%d)" % st)
+    capi.check(st)
+    LU = Av.view(np.complex128).reshape(n, n, order="F")
+    X = Bv.view(np.complex128).reshape(n, nrhs, order="F")
+    return LU, X, ipiv
+
+
+def cs_prtsrf(ctx, dmesh, nedges_total, omega, ri, x, theta, phi, psi, quad=None):
+    """(csca, cabs) of solution x for a plane wave (cs.f90:14-104)."""
+    Q, w, n = _quad(quad)
+    xv = np.ascontiguousarray(x, dtype=np.complex128).view(np.float64)
+    cs, ca = C.c_double(), C.c_double()
+    capi.check(capi.lib().mom_cs_prtsrf(ctx.h, dmesh.h, nedges_total, omega, complex(ri).real, complex(ri).imag,
+                                        xv, theta, phi, psi, Q, w, n, C.byref(cs), C.byref(ca)))
+    return cs.value, ca.value
+
+
+class System:
+    """Device-resident PMCHWT system (mom_system): fill, factor, solve."""
+
+    def __init__(self, ctx, nedges_total, precision=capi.MOM_PREC_MIXED):
+        self.ctx = ctx
+        self.N = nedges_total
+        self.h = C.c_void_p()
+        capi.check(capi.lib().mom_system_create(ctx.h, nedges_total, precision, C.byref(self.h)))
+        ctx._adopt(self)
+
+    def fill(self, dmeshes, ri_list, omega, quad=None):
+        Q, w, n = _quad(quad)
+        ptr, keep = _domain_handles(dmeshes)
+        capi.check(capi.lib().mom_system_fill(self.h, len(dmeshes), ptr, _ri_array(ri_list), omega, Q, w, n))
+
+    def get_matrix(self):
+        N2 = 2 * self.N
+        A = np.zeros(2 * N2 * N2)
+        capi.check(capi.lib().mom_system_get_matrix(self.h, A, N2))
+        return A.view(np.complex128).reshape(N2, N2, order="F")
+
+    def set_matrix(self, A):
+        N2 = 2 * self.N
+        Av = np.asfortranarray(A, dtype=np.complex128).reshape(-1, order="F").view(np.float64)
+        capi.check(capi.lib().mom_system_set_matrix(self.h, np.ascontiguousarray(Av), N2))
+
+    def factor(self):
+        st = capi.lib().mom_system_factor(self.h)
+        if st > 0:
+            raise capi.MomError(st, "zero pivot in column %d" % st)
+        capi.check(st)
+
+    def solve(self, B, tol=1e-12, maxit=30):
+        N2 = 2 * self.N
+        B2 = np.asfortranarray(np.atleast_2d(np.asarray(B).T).T, dtype=np.complex128).copy(order="F")
+        nrhs = B2.shape[1]
+        Bv = np.ascontiguousarray(B2.reshape(-1, order="F").view(np.float64))
+        it, be = C.c_int32(), C.c_double()
+        st = capi.lib().mom_system_solve(self.h, nrhs, Bv, N2, tol, maxit, C.byref(it), C.byref(be))
+        self.iters, self.berr = it.value, be.value
+        capi.check(st)
+        return Bv.view(np.complex128).reshape(N2, nrhs, order="F")
+
+    def solve_pw(self, dmesh, omega, ri, angles, tol=1e-12, maxit=30, quad=None):
+        Q, w, n = _quad(quad)
+        ang = np.ascontiguousarray(np.atleast_2d(np.asarray(angles, dtype=np.float64))).ravel()
+        nsrc = ang.size // 3
+        N2 = 2 * self.N
+        X = np.zeros(2 * N2 * nsrc)
+        it, be = C.c_int32(), C.c_double()
+        st = capi.lib().mom_system_solve_pw(self.h, dmesh.h, omega, complex(ri).real, complex(ri).imag, nsrc, ang,
+                                            Q, w, n, tol, maxit, X, N2, C.byref(it), C.byref(be))
+        self.iters, self.berr = it.value, be.value
+        capi.check(st)
+        return X.view(np.complex128).reshape(N2, nsrc, order="F")
+
+    def stats(self):
+        out = np.zeros(8)
+        capi.check(capi.lib().mom_system_stats(self.h, out))
+        return dict(fill_s=out[0], factor_s=out[1], solve_s=out[2], gemm_s=out[3], gemm_launches=out[4],
+                    gemm_flops=out[5])
+
+    def close(self):
+        if self.h:
+            capi.lib().mom_system_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

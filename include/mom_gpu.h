@@ -1,0 +1,173 @@
+/*
+ * mom_gpu.h -- C-ABI of the MI355X PMCHWT hot path (libmomgpu.so).
+ *
+ * Drop-in boundary for the for90-mom2 routines that fill the dense PMCHWT
+ * system, build the plane-wave right-hand sides and factor/solve the
+ * system.  Every entry point is a plain C function over pointers and
+ * 64-bit sizes; a Fortran host binds them with ISO_C_BINDING.
+ *
+ * Conventions (those of the reference):
+ *  - complex numbers are interleaved (re, im) doubles, i.e. Fortran
+ *    COMPLEX(KIND=dp) / C double _Complex; matrices are column-major with
+ *    a leading dimension;
+ *  - index arrays in mom_mesh_create are 0-based (the Fortran shim
+ *    subtracts 1), -1 marks "none";
+ *  - all calls are synchronous at return; work runs on the context's HIP
+ *    stream on one GPU.
+ * Status: 0 = success, > 0 = LAPACK-style INFO (a zero pivot in column
+ * INFO, 1-based), < 0 = one of the MOM_ERR_* codes below.  The message of
+ * the last failure on the calling thread is returned by mom_last_error().
+ */
+#ifndef MOM_GPU_H
+#define MOM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOM_OK 0
+#define MOM_ERR_ARG (-1)         /* invalid argument */
+#define MOM_ERR_UNSUPPORTED (-2) /* request outside the implemented path */
+#define MOM_ERR_HIP (-3)         /* HIP runtime failure */
+#define MOM_ERR_NOMEM (-4)       /* device allocation failed */
+#define MOM_ERR_NOCONV (-5)      /* iterative refinement did not converge */
+#define MOM_ERR_STATE (-6)       /* call out of order (e.g. solve before factor) */
+
+typedef struct mom_ctx mom_ctx;
+typedef struct mom_mesh mom_mesh;
+typedef struct mom_system mom_system;
+
+const char *mom_last_error(void);
+
+/* One context per host thread and device; owns the HIP stream. */
+int mom_ctx_create(int32_t device, mom_ctx **ctx);
+int mom_ctx_destroy(mom_ctx *ctx);
+
+/*
+ * Upload one domain mesh (a flattened mesh_container, mesh.f90:75-83, as
+ * built by build_mesh, extract_submesh and orient_basis):
+ *   p[3*nnodes]          node coordinates (x,y,z per node) [m]
+ *   fnodes[3*nfaces]     face node indices
+ *   fedges[3*nfaces]     face edge indices (local edge numbering)
+ *   fn, fcp[3*nfaces]    unit normal, centroid
+ *   fs, fm[9*nfaces]     s(:,e), m(:,e) of each face side e, e = 0..2
+ *   farea[nfaces]
+ *   efaces, ebn[2*nedges] "+"/"-" face and free vertex of each edge
+ *   elen[nedges]          edge length
+ *   eparent[nedges]       global RWG index (parent_index) of each edge
+ *   avelen                average edge length (near-pair threshold 3*avelen)
+ * Geometry is taken as given, so the kernels see the host's exact bits.
+ */
+int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nfaces,
+                    const int32_t *fnodes, const int32_t *fedges, const double *fn,
+                    const double *fcp, const double *fs, const double *fm, const double *farea,
+                    int64_t nedges, const int32_t *efaces, const int32_t *ebn, const double *elen,
+                    const int32_t *eparent, double avelen, mom_mesh **mesh);
+int mom_mesh_destroy(mom_mesh *mesh);
+/* Number of near (singular-subtraction) face pairs of the mesh. */
+int64_t mom_mesh_near_pairs(const mom_mesh *mesh);
+
+/*
+ * computeD / computeH / computeK (sysmat.f90:272-347, 359-434, 446-519),
+ * identity group, free-space Green's function:
+ *   op = 'D', 'H' or 'K'; k = ri*omega/c0; triangle rule of nq points with
+ *   weights qw[nq] and barycentric nodes qnodes[nq*3] (column-major, as
+ *   quad_data%nodes); M[E x E] (ldm >= E) is overwritten.
+ */
+int mom_compute_operator(mom_ctx *ctx, const mom_mesh *mesh, char op, double omega, double ri_re,
+                         double ri_im, int32_t nq, const double *qw, const double *qnodes,
+                         double *M, int64_t ldm);
+
+/*
+ * sysmat_pmchwt (sysmat.f90:18-105), identity group, free space:
+ * A[2N x 2N] (lda >= 2N, N = nedges_total) is overwritten with the PMCHWT
+ * matrix of the ndom domains; ri[2*ndom] holds each domain's refractive
+ * index (re, im).  Domain meshes map their edges to global indices through
+ * eparent.
+ */
+int mom_sysmat_pmchwt(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *ri,
+                      int64_t nedges_total, double omega, int32_t nq, const double *qw,
+                      const double *qnodes, double *A, int64_t lda);
+
+/*
+ * srcvec (source.f90:136-247) for nsrc plane waves (type 'pw'), identity
+ * group: angles[3*nsrc] = (theta, phi, psi) in radians per source;
+ * q[2N x nsrc] (ldq >= 2N) is overwritten: rows 0..N-1 <f, E_inc>,
+ * rows N..2N-1 <f, H_inc>.
+ */
+int mom_srcvec_pw(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega,
+                  double ri_re, double ri_im, int64_t nsrc, const double *angles, int32_t nq,
+                  const double *qw, const double *qnodes, double *q, int64_t ldq);
+
+/*
+ * solve_multi_linsys (linalg.f90:159-180): LU with partial pivoting of
+ * A[n x n] (overwritten by L\U in LAPACK ZGETRF layout, ipiv[n] 1-based row
+ * interchanges, may be NULL) and solve of B[n x nrhs] (overwritten by X).
+ * Runs in complex128.  Returns INFO > 0 for an exactly zero pivot.
+ */
+int mom_solve_multi_linsys(mom_ctx *ctx, int64_t n, double *A, int64_t lda, int64_t nrhs,
+                           double *B, int64_t ldb, int32_t *ipiv);
+
+/*
+ * Device-resident PMCHWT system: the matrix stays in HBM between fill,
+ * factorisation and solves (solver.f90:109-136 without host round trips).
+ *   precision MOM_PREC_FP64:  complex128 matrix, complex128 LU.
+ *   precision MOM_PREC_MIXED: complex128 matrix kept for the residual, an
+ *     equilibrated complex64 copy factorised on the FP32 matrix cores, and
+ *     GMRES-based iterative refinement to a complex128 backward error.
+ */
+#define MOM_PREC_FP64 64
+#define MOM_PREC_MIXED 32
+
+int mom_system_create(mom_ctx *ctx, int64_t nedges_total, int32_t precision, mom_system **sys);
+int mom_system_destroy(mom_system *sys);
+/* Fill (same arguments as mom_sysmat_pmchwt, result kept on the device). */
+int mom_system_fill(mom_system *sys, int32_t ndom, const mom_mesh *const *doms, const double *ri,
+                    double omega, int32_t nq, const double *qw, const double *qnodes);
+/* Copy the complex128 system matrix to the host (before factorisation). */
+int mom_system_get_matrix(mom_system *sys, double *A, int64_t lda);
+/* Upload a complex128 matrix instead of filling it. */
+int mom_system_set_matrix(mom_system *sys, const double *A, int64_t lda);
+/* Factorise (returns INFO > 0 on a zero pivot). */
+int mom_system_factor(mom_system *sys);
+/*
+ * Solve for B[2N x nrhs] (host, overwritten by X).  In MOM_PREC_MIXED, the
+ * refinement stops at backward error ||b - Zx|| / (||Z|| ||x|| + ||b||)
+ * (infinity norms) <= tol or after maxit refinement steps; iters and berr
+ * (may be NULL) return the steps taken and the final backward error of the
+ * worst column.  MOM_ERR_NOCONV if tol is not met.
+ */
+int mom_system_solve(mom_system *sys, int64_t nrhs, double *B, int64_t ldb, double tol,
+                     int32_t maxit, int32_t *iters, double *berr);
+/* Build the plane-wave right-hand sides on the device and solve in place of
+ * mom_srcvec_pw + mom_system_solve; X[2N x nsrc] receives the solutions. */
+int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, double ri_re,
+                        double ri_im, int64_t nsrc, const double *angles, int32_t nq,
+                        const double *qw, const double *qnodes, double tol, int32_t maxit,
+                        double *X, int64_t ldx, int32_t *iters, double *berr);
+/* Enable (1) or disable (0) event timing of the trailing-update kernel
+ * launches during mom_system_factor (reported by mom_system_stats). */
+int mom_system_profile(mom_system *sys, int32_t enable);
+/*
+ * Wall times of the last fill / factor / solve [s] and kernel statistics:
+ * out[0] fill, out[1] factor, out[2] solve, out[3] trailing-update GEMM
+ * kernel time summed over the last factorisation [s], out[4] number of
+ * those GEMM launches, out[5] their algorithmic flops.
+ */
+int mom_system_stats(const mom_system *sys, double *out);
+
+/*
+ * cs_prtsrf (cs.f90:14-104) for a plane-wave source: scattering and
+ * absorption cross-sections of the solution x[2N] over the domain mesh
+ * (normally domain 1), medium index ri.
+ */
+int mom_cs_prtsrf(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega,
+                  double ri_re, double ri_im, const double *x, double theta, double phi, double psi,
+                  int32_t nq, const double *qw, const double *qnodes, double *csca, double *cabs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOM_GPU_H */

@@ -1,0 +1,40 @@
+/*
+ * mom_gpu_testing.h -- test hooks of libmomgpu.so.
+ *
+ * These entry points have no counterpart in the reference; they exist so
+ * that the test suite can check single kernels of the hot path (matrix-core
+ * lane layout, the trailing-update GEMM) in isolation through the C-ABI.
+ */
+#ifndef MOM_GPU_TESTING_H
+#define MOM_GPU_TESTING_H
+
+#include <stdint.h>
+
+#include "mom_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * Lane layout probe of the 16x16x4 matrix-core instruction used by the LU
+ * (precision 64: v_mfma_f64_16x16x4f64, 32: v_mfma_f32_16x16x4f32): for
+ * every pair (L0, L1) of lanes one wave multiplies A = e(L0), B = e(L1)
+ * (lane L0 / L1 holds 1, all other lanes 0) and writes its 64 x 4
+ * accumulators to out[((L0*64 + L1)*64 + lane)*4 + v] (1048576 doubles).
+ */
+int mom_test_mfma_layout(mom_ctx *ctx, int32_t precision, double *out);
+
+/*
+ * C -= A B with the trailing-update kernel of the LU, complex column-major:
+ * A[M x K] (lda), B[K x N] (ldb), C[M x N] (ldc), interleaved (re, im),
+ * in complex128 (precision 64) or complex64 (precision 32, values rounded
+ * on upload and widened on return).
+ */
+int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int64_t N, int64_t K, const double *A,
+                      int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

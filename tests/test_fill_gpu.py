@@ -1,0 +1,85 @@
+"""GPU parity of the fill, RHS and cross-section kernels against the oracle
+and the reference's golden vectors.  Tolerance: matrix entries within
+1e-10 relative to the largest entry of their PMCHWT block (FP64 path)."""
+import numpy as np
+import pytest
+
+import oracle
+from cases import blockwise_relerr, dimer, sphere
+from conftest import golden
+from mom2amd import DEGTORAD, DeviceMesh, computeD, computeH, computeK, cs_prtsrf, srcvec, sysmat_pmchwt
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _dev(ctx, case):
+    return [DeviceMesh(ctx, d) for d in case["problem"].domains]
+
+
+@pytest.mark.parametrize("s", [1, 2])
+def test_operators_match_oracle(gpu_ctx, msh_dir, s):
+    case = sphere(msh_dir, s)
+    op = oracle.OracleProblem(case["path"], 1e-9, case["domains"])
+    dms = _dev(gpu_ctx, case)
+    for d in range(2):
+        for name, fn in (("D", computeD), ("H", computeH), ("K", computeK)):
+            M = fn(gpu_ctx, dms[d], case["omega"], case["ri"][d])
+            R = op.operator(d, name, case["omega"], case["ri"][d])
+            err = np.abs(M - R).max() / np.abs(R).max()
+            print("s=%d dom=%d %s err=%.3e" % (s, d, name, err), flush=True)
+            assert err < TOL, (s, d, name, err)
+
+
+def test_sysmat_golden_sphere_s1(gpu_ctx, msh_dir):
+    g = golden("sphere_s1")
+    case = sphere(msh_dir, 1)
+    dms = _dev(gpu_ctx, case)
+    A = sysmat_pmchwt(gpu_ctx, dms, case["ri"], case["problem"].nedges, case["omega"])
+    err = blockwise_relerr(A, g["A"])
+    print("sphere s=1 sysmat vs reference: %.3e" % err, flush=True)
+    assert err < TOL
+
+
+@pytest.mark.parametrize("name", ["sphere_s2", "dimer_s1"])
+def test_sysmat_golden_sampled(gpu_ctx, msh_dir, name):
+    g = golden(name)
+    case = sphere(msh_dir, 2) if name == "sphere_s2" else dimer(msh_dir, 1)
+    dms = _dev(gpu_ctx, case)
+    A = sysmat_pmchwt(gpu_ctx, dms, case["ri"], case["problem"].nedges, case["omega"])
+    N = A.shape[0] // 2
+    # block norms pin the whole matrix, sampled entries pin positions
+    fro = np.array([[np.linalg.norm(A[i * N:(i + 1) * N, j * N:(j + 1) * N]) for j in range(2)] for i in range(2)])
+    assert np.abs(fro / g["block_fro"] - 1).max() < TOL
+    ref = g["A_v"]
+    got = A[g["A_i"], g["A_j"]]
+    scale = np.array([[np.abs(A[i * N:(i + 1) * N, j * N:(j + 1) * N]).max() for j in range(2)] for i in range(2)])
+    s = scale[g["A_i"] // N, g["A_j"] // N]
+    err = (np.abs(got - ref) / s).max()
+    print("%s sampled sysmat vs reference: %.3e" % (name, err), flush=True)
+    assert err < TOL
+
+
+@pytest.mark.parametrize("name", ["sphere_s1", "sphere_s2", "dimer_s1"])
+def test_srcvec_and_cs_golden(gpu_ctx, msh_dir, name):
+    g = golden(name)
+    case = {"sphere_s1": lambda: sphere(msh_dir, 1), "sphere_s2": lambda: sphere(msh_dir, 2),
+            "dimer_s1": lambda: dimer(msh_dir, 1)}[name]()
+    dm = DeviceMesh(gpu_ctx, case["problem"].domains[0])
+    ang = np.array(g["angles_deg"]) * DEGTORAD
+    q = srcvec(gpu_ctx, dm, case["problem"].nedges, case["omega"], case["ri"][0], ang)[:, 0]
+    err_q = np.abs(q - g["q"]).max() / np.abs(g["q"]).max()
+    cs, ca = cs_prtsrf(gpu_ctx, dm, case["problem"].nedges, case["omega"], case["ri"][0], g["x"], *ang)
+    print("%s q err %.3e csca %.3e cabs %.3e" % (name, err_q, cs / g["csca"] - 1, ca / g["cabs"] - 1), flush=True)
+    assert err_q < 1e-12
+    assert abs(cs / g["csca"] - 1) < 1e-12 and abs(ca / g["cabs"] - 1) < 1e-12
+
+
+def test_near_pair_count_matches_oracle_rule(gpu_ctx, msh_dir):
+    case = sphere(msh_dir, 2)
+    d = case["problem"].domains[0]
+    cp = d.fcp
+    diff = cp[:, None, :] - cp[None, :, :]
+    ds = diff[..., 0] * diff[..., 0] + diff[..., 1] * diff[..., 1] + diff[..., 2] * diff[..., 2]
+    thr = d.avelen * 3
+    assert DeviceMesh(gpu_ctx, d).near_pairs == int((ds < thr * thr).sum())
