@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the PMCHWT hot path on one MI355X.
+
+Workload (default): a sphere of radius 50 nm meshed as an icosahedron
+subdivided 5 times and projected to the sphere (20480 triangles, 30720 RWG
+edges, N = 61440 unknowns J and M), exterior index 1, interior
+eps = -9.4+1.5i, vacuum wavelength 600 nm, x-polarised plane wave along +z.
+
+One step = the whole solve on the device: PMCHWT fill (complex128), FP32
+matrix-core LU of the equilibrated complex64 copy, plane-wave RHS and
+GMRES iterative refinement against the complex128 matrix to backward error
+<= 1e-12.  `value` is the mean wall time of a step [s].  After the timed
+steps the all-FP64 GPU path (complex128 fill + LU + solve) is timed once on
+the same problem, and the cross-sections of both solutions are compared.
+
+Prints one JSON line.  --gpus > 1 is not supported (one GPU).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "for90-mom2_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "fill+LU+solve s @N=61440: FP32-MFMA+IR vs FP64 GPU path, σ rel err≤1e-8"
+PEAK_FP32_MFMA = 157.3  # TFLOP/s, MI355X dense FP32 matrix
+PEAK_FP64_MFMA = 78.6
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def build_case(subdiv):
+    from mom2amd import build_problem, meshgen, omega_of
+
+    nodes, faces = meshgen.icosphere(subdiv, 50.0)
+    d = tempfile.mkdtemp(prefix="mom_bench_")
+    path = os.path.join(d, "ico%d.msh" % subdiv)
+    meshgen.write_gmsh(path, nodes, faces, 15)
+    doms = [[15], [-15]]
+    prob = build_problem(*meshgen.read_gmsh(path), 1e-9, doms)
+    return dict(path=path, domains=doms, problem=prob, omega=omega_of(600e-9),
+                ri=[1.0 + 0j, complex(np.sqrt(-9.4 + 1.5j))])
+
+
+def run_path(ctx, dms, case, prec, profile=False, tol=1e-12):
+    from mom2amd import DEGTORAD, System, cs_prtsrf
+    from mom2amd import capi
+
+    N = case["problem"].nedges
+    ang = np.array([0.0, 90.0, 0.0]) * DEGTORAD
+    S = System(ctx, N, prec)
+    if profile:
+        capi.check(capi.lib().mom_system_profile(S.h, 1))
+    t0 = time.perf_counter()
+    S.fill(dms, case["ri"], case["omega"])
+    S.factor()
+    x = S.solve_pw(dms[0], case["omega"], case["ri"][0], ang, tol=tol, maxit=30)[:, 0]
+    wall = time.perf_counter() - t0
+    st = S.stats()
+    st.update(wall_s=wall, iters=S.iters, berr=S.berr)
+    cs = cs_prtsrf(ctx, dms[0], N, case["omega"], case["ri"][0], x, *ang)
+    S.close()
+    return st, cs
+
+
+def cpu_baseline(case, subdiv, threads):
+    """Oracle fill on a bounded set of test faces plus LAPACK LU on a smaller
+    matrix, both scaled to the full workload (fill ~ faces, LU ~ n^3)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import scipy.linalg as sl
+
+    op = oracle.OracleProblem(case["path"], 1e-9, case["domains"])
+    F = case["problem"].domains[0].nfaces
+    rows = 32
+    t_rows = op.fill_rows_seconds(case["omega"], case["ri"], 0, rows)
+    fill_est = t_rows * F / rows
+    n = 2 * case["problem"].nedges
+    ns = 4096
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((ns, ns)) + 1j * rng.standard_normal((ns, ns))
+    b = rng.standard_normal(ns) + 0j
+    t0 = time.perf_counter()
+    lu = sl.lu_factor(A, overwrite_a=True, check_finite=False)
+    sl.lu_solve(lu, b, check_finite=False)
+    t_lu = time.perf_counter() - t0
+    lu_est = t_lu * (n / ns) ** 3
+    sample = ("oracle C fill (OpenMP) over %d of %d test faces per domain (%.2f s) scaled by faces; "
+              "LAPACK zgetrf+zgetrs (scipy OpenBLAS) at n=%d (%.2f s) scaled by n^3 to n=%d"
+              % (rows, F, t_rows, ns, t_lu, n))
+    return dict(value=fill_est + lu_est, unit="s", cores=threads, kind="port", sample=sample,
+                fill_s=fill_est, lu_s=lu_est)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--subdiv", type=int, default=5, help="icosphere subdivision (5: N = 61440)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison path")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from mom2amd import Context, DeviceMesh
+    from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
+
+    t0 = time.perf_counter()
+    case = build_case(args.subdiv)
+    N = case["problem"].nedges
+    log("mesh s=%d: %d faces, N=%d unknowns (set-up %.1f s)" % (args.subdiv, case["problem"].domains[0].nfaces,
+                                                                 2 * N, time.perf_counter() - t0))
+    ctx = Context(0)
+    dms = [DeviceMesh(ctx, d) for d in case["problem"].domains]
+    for i in range(args.warmup):
+        st, _ = run_path(ctx, dms, case, MOM_PREC_MIXED)
+        log("warmup %d: %.2f s (fill %.2f factor %.2f solve %.2f)" % (i, st["wall_s"], st["fill_s"], st["factor_s"],
+                                                                     st["solve_s"]))
+    times, gemm_s, gemm_flops, gemm_launches, last = [], 0.0, 0.0, 0, None
+    for i in range(args.steps):
+        st, cs_mixed = run_path(ctx, dms, case, MOM_PREC_MIXED, profile=True)
+        times.append(st["wall_s"])
+        gemm_s += st["gemm_s"]
+        gemm_flops += st["gemm_flops"]
+        gemm_launches += int(st["gemm_launches"])
+        last = st
+        log("step %d: %.2f s (fill %.2f factor %.2f solve %.2f, IR iters %d, berr %.2e)" %
+            (i, st["wall_s"], st["fill_s"], st["factor_s"], st["solve_s"], st["iters"], st["berr"]))
+    mixed_s = float(np.mean(times))
+
+    fp64 = None
+    if not args.no_fp64:
+        st64, cs64 = run_path(ctx, dms, case, MOM_PREC_FP64, profile=True)
+        fp64 = st64
+        log("fp64 path: %.2f s (fill %.2f factor %.2f solve %.2f)" % (st64["wall_s"], st64["fill_s"],
+                                                                      st64["factor_s"], st64["solve_s"]))
+    achieved = gemm_flops / gemm_s / 1e12 if gemm_s > 0 else None
+    out = {
+        "metric": METRIC,
+        "value": round(mixed_s, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed_s * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "fp32-mfma LU + fp64 fill/residual",
+        "data": "synthetic icosphere mesh",
+        "config": {"workload": "PMCHWT sphere R=50nm icosphere s=%d (%d faces), N=%d unknowns, 600 nm, "
+                               "eps_in=-9.4+1.5i, 1 plane wave; fill+LU+solve, GMRES-IR to backward err 1e-12"
+                               % (args.subdiv, case["problem"].domains[0].nfaces, 2 * N),
+                   "n_unknowns": 2 * N},
+        "phases_s": {"fill": round(last["fill_s"], 4), "factor": round(last["factor_s"], 4),
+                     "solve_ir": round(last["solve_s"], 4), "ir_iters": last["iters"], "berr": last["berr"]},
+        "roofline": None if achieved is None else {
+            "kernel": "gemm_sub_kernel<float> (LU trailing update, complex64, v_mfma_f32_16x16x4f32)",
+            "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_MFMA, 4), "traffic": None,
+            "launches": gemm_launches, "avg_launch_ms": round(gemm_s / max(gemm_launches, 1) * 1e3, 4)},
+    }
+    if fp64 is not None:
+        ach64 = fp64["gemm_flops"] / fp64["gemm_s"] / 1e12 if fp64["gemm_s"] > 0 else None
+        out["fp64_path"] = {"value_s": round(fp64["wall_s"], 4), "fill": round(fp64["fill_s"], 4),
+                            "factor": round(fp64["factor_s"], 4), "solve": round(fp64["solve_s"], 4),
+                            "gemm_tflops": None if ach64 is None else round(ach64, 3),
+                            "gemm_frac_of_fp64_peak": None if ach64 is None else round(ach64 / PEAK_FP64_MFMA, 4)}
+        out["speedup_vs_fp64_path"] = round(fp64["wall_s"] / mixed_s, 3)
+        out["sigma_relerr_vs_fp64_path"] = {"sca": abs(cs_mixed[0] / cs64[0] - 1), "abs": abs(cs_mixed[1] / cs64[1] - 1)}
+        out["sigma"] = {"sca_m2": cs_mixed[0], "abs_m2": cs_mixed[1]}
+    ctx.close()
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        log("cpu baseline (bounded sample, %d threads)" % threads)
+        cb = cpu_baseline(case, args.subdiv, threads)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"]["value"] = round(cb["value"], 1)
+        out["speedup_vs_cpu"] = round(cb["value"] / mixed_s, 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

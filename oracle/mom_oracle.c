@@ -970,15 +970,24 @@ static int near_faces(const omesh *m, int n, int mm) {
 
 /* sysmat.f90:272-347 computeD, 359-434 computeH, 446-519 computeK.
  * op: 0 = D, 1 = H, 2 = K.  M is E x E column-major, overwritten. */
+static void compute_op_rows(const omesh *m, int op, double omega, cplx ri, const oquad *qd, cplx *M,
+                            int m0, int m1);
 static void compute_op(const omesh *m, int op, double omega, cplx ri, const oquad *qd, cplx *M) {
+  memset(M, 0, sizeof(cplx) * (size_t)m->nedges * m->nedges);
+  compute_op_rows(m, op, omega, ri, qd, M, 0, m->nfaces);
+}
+
+/* computeD/H/K restricted to test faces m0 <= m < m1 (the reference's outer
+ * loop, sysmat.f90:303/390/473), adding into M. */
+static void compute_op_rows(const omesh *m, int op, double omega, cplx ri, const oquad *qd, cplx *M,
+                            int m0, int m1) {
   int E = m->nedges, F = m->nfaces, Q = qd->Q;
-  memset(M, 0, sizeof(cplx) * (size_t)E * E);
   cplx k = ri * omega / orc_c0;
   cplx c1 = -I * omega * orc_mu0();
   cplx c2 = -1.0 / (I * omega * orc_eps0 * (ri * ri));
   int per = (op == 1) ? 3 : 9;
   cplx *intaux = xcalloc((size_t)per * Q * F, sizeof(cplx));
-  for (int mm = 0; mm < F; mm++) {
+  for (int mm = m0; mm < m1; mm++) {
     double qpm[96], fmv[3][96], fmDiv[3];
     quad_tri_points(qd, m, mm, qpm);
     double Am = m->farea[mm];
@@ -1250,6 +1259,25 @@ void orc_rcs(void *h, int dom, double omega, double ri_re, double ri_im, const d
       cplx ep = -prefix * (Lt - orc_eta0 / ri * Np);
       out[(size_t)b * ntheta + a] = cabs(et) * cabs(et) + cabs(ep) * cabs(ep);
     }
+}
+
+/* CPU timing sample of the fill: the three operators of every domain for
+ * test faces [m0, m1) of each domain mesh; returns seconds of wall time. */
+#include <omp.h>
+double orc_fill_rows_seconds(void *h, double omega, const double *ri, int Q, const double *w,
+                             const double *nodes, int m0, int m1) {
+  oproblem *P = h;
+  oquad qd;
+  mkquad(&qd, Q, w, nodes);
+  double t0 = omp_get_wtime();
+  for (int d = 0; d < P->ndom; d++) {
+    omesh *m = P->dom + d;
+    cplx r = ri[2 * d] + I * ri[2 * d + 1];
+    cplx *M = xcalloc((size_t)m->nedges * m->nedges, sizeof(cplx));
+    for (int op = 0; op < 3; op++) compute_op_rows(m, op, omega, r, &qd, M, m0, m1 < m->nfaces ? m1 : m->nfaces);
+    free(M);
+  }
+  return omp_get_wtime() - t0;
 }
 
 void orc_problem_free(void *h) {

@@ -42,6 +42,8 @@ def lib():
                              C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]
         L.orc_rcs.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _dp, C.c_int,
                               C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_fill_rows_seconds.restype = C.c_double
+        L.orc_fill_rows_seconds.argtypes = [C.c_void_p, C.c_double, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int]
         L.orc_int_scalar.restype = C.c_double
         L.orc_int_scalar.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_int_vector.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
@@ -138,6 +140,13 @@ class OracleProblem:
         Q, w, n = self._q(quad)
         lib().orc_rcs(self.h, dom, omega, ri.real, ri.imag, x, ntheta, nphi, Q, w, n, out)
         return out.reshape(ntheta, nphi, order="F")
+
+    def fill_rows_seconds(self, omega, ri_list, m0, m1, quad=None):
+        """Wall time of the oracle fill (3 operators, every domain) for test
+        faces [m0, m1) -- a bounded CPU timing sample."""
+        ri = np.array([[complex(r).real, complex(r).imag] for r in ri_list]).ravel()
+        Q, w, n = self._q(quad)
+        return lib().orc_fill_rows_seconds(self.h, omega, ri, Q, w, n, m0, m1)
 
     def int_scalar(self, dom, which, f, e, r):
         return lib().orc_int_scalar(self.h, dom, which, f, e, np.ascontiguousarray(r, dtype=np.float64))
