@@ -1,0 +1,172 @@
+! MODULE: mom_gpu
+! ISO_C_BINDING interface of libmomgpu.so (include/mom_gpu.h) for Fortran
+! hosts.  The drop-in bodies of sysmat_pmchwt, computeD/H/K, srcvec and
+! solve_multi_linsys (sysmat.f90:18, 272, 359, 446; source.f90:136;
+! linalg.f90:159) call these functions; see INTEGRATION.md for the bodies.
+!
+! Conventions: COMPLEX(C_DOUBLE_COMPLEX) arrays are passed by address,
+! column-major with an explicit leading dimension; mesh index arrays are
+! 0-based (subtract 1 from the reference's 1-based indices); a non-zero
+! status is turned into the reference's WRITE + STOP by mom_check.
+MODULE mom_gpu
+  USE ISO_C_BINDING
+  IMPLICIT NONE
+
+  INTEGER(C_INT32_T), PARAMETER :: MOM_PREC_FP64 = 64, MOM_PREC_MIXED = 32
+
+  INTERFACE
+     FUNCTION mom_last_error() BIND(C, NAME='mom_last_error') RESULT(p)
+       IMPORT :: C_PTR
+       TYPE(C_PTR) :: p
+     END FUNCTION mom_last_error
+
+     FUNCTION mom_ctx_create(device, ctx) BIND(C, NAME='mom_ctx_create') RESULT(st)
+       IMPORT :: C_INT32_T, C_INT, C_PTR
+       INTEGER(C_INT32_T), VALUE :: device
+       TYPE(C_PTR) :: ctx
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_ctx_create
+
+     FUNCTION mom_ctx_destroy(ctx) BIND(C, NAME='mom_ctx_destroy') RESULT(st)
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: ctx
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_ctx_destroy
+
+     FUNCTION mom_mesh_create(ctx, nnodes, p, nfaces, fnodes, fedges, fn, fcp, fs, fm, farea, &
+          nedges, efaces, ebn, elen, eparent, avelen, mesh) BIND(C, NAME='mom_mesh_create') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: ctx
+       INTEGER(C_INT64_T), VALUE :: nnodes, nfaces, nedges
+       REAL(C_DOUBLE), INTENT(IN) :: p(*), fn(*), fcp(*), fs(*), fm(*), farea(*), elen(*)
+       INTEGER(C_INT32_T), INTENT(IN) :: fnodes(*), fedges(*), efaces(*), ebn(*), eparent(*)
+       REAL(C_DOUBLE), VALUE :: avelen
+       TYPE(C_PTR) :: mesh
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_mesh_create
+
+     FUNCTION mom_mesh_destroy(mesh) BIND(C, NAME='mom_mesh_destroy') RESULT(st)
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: mesh
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_mesh_destroy
+
+     FUNCTION mom_compute_operator(ctx, mesh, op, omega, ri_re, ri_im, nq, qw, qnodes, M, ldm) &
+          BIND(C, NAME='mom_compute_operator') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR, C_CHAR
+       TYPE(C_PTR), VALUE :: ctx, mesh, M
+       CHARACTER(KIND=C_CHAR), VALUE :: op
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       INTEGER(C_INT32_T), VALUE :: nq
+       REAL(C_DOUBLE), INTENT(IN) :: qw(*), qnodes(*)
+       INTEGER(C_INT64_T), VALUE :: ldm
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_compute_operator
+
+     FUNCTION mom_sysmat_pmchwt(ctx, ndom, doms, ri, nedges_total, omega, nq, qw, qnodes, A, lda) &
+          BIND(C, NAME='mom_sysmat_pmchwt') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, A
+       INTEGER(C_INT32_T), VALUE :: ndom, nq
+       TYPE(C_PTR), INTENT(IN) :: doms(*)
+       REAL(C_DOUBLE), INTENT(IN) :: ri(*), qw(*), qnodes(*)
+       INTEGER(C_INT64_T), VALUE :: nedges_total, lda
+       REAL(C_DOUBLE), VALUE :: omega
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_sysmat_pmchwt
+
+     FUNCTION mom_srcvec_pw(ctx, mesh, nedges_total, omega, ri_re, ri_im, nsrc, angles, nq, qw, qnodes, q, ldq) &
+          BIND(C, NAME='mom_srcvec_pw') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, mesh, q
+       INTEGER(C_INT64_T), VALUE :: nedges_total, nsrc, ldq
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       REAL(C_DOUBLE), INTENT(IN) :: angles(*), qw(*), qnodes(*)
+       INTEGER(C_INT32_T), VALUE :: nq
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_srcvec_pw
+
+     FUNCTION mom_solve_multi_linsys(ctx, n, A, lda, nrhs, B, ldb, ipiv) &
+          BIND(C, NAME='mom_solve_multi_linsys') RESULT(st)
+       IMPORT :: C_INT, C_INT64_T, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, A, B, ipiv
+       INTEGER(C_INT64_T), VALUE :: n, lda, nrhs, ldb
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_solve_multi_linsys
+
+     FUNCTION mom_system_create(ctx, nedges_total, precision, sys) BIND(C, NAME='mom_system_create') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR
+       TYPE(C_PTR), VALUE :: ctx
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       INTEGER(C_INT32_T), VALUE :: precision
+       TYPE(C_PTR) :: sys
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_create
+
+     FUNCTION mom_system_destroy(sys) BIND(C, NAME='mom_system_destroy') RESULT(st)
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: sys
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_destroy
+
+     FUNCTION mom_system_fill(sys, ndom, doms, ri, omega, nq, qw, qnodes) BIND(C, NAME='mom_system_fill') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: sys
+       INTEGER(C_INT32_T), VALUE :: ndom, nq
+       TYPE(C_PTR), INTENT(IN) :: doms(*)
+       REAL(C_DOUBLE), INTENT(IN) :: ri(*), qw(*), qnodes(*)
+       REAL(C_DOUBLE), VALUE :: omega
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_fill
+
+     FUNCTION mom_system_factor(sys) BIND(C, NAME='mom_system_factor') RESULT(st)
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: sys
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_factor
+
+     FUNCTION mom_system_solve(sys, nrhs, B, ldb, tol, maxit, iters, berr) BIND(C, NAME='mom_system_solve') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: sys, B
+       INTEGER(C_INT64_T), VALUE :: nrhs, ldb
+       REAL(C_DOUBLE), VALUE :: tol
+       INTEGER(C_INT32_T), VALUE :: maxit
+       INTEGER(C_INT32_T) :: iters
+       REAL(C_DOUBLE) :: berr
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_solve
+
+     FUNCTION mom_cs_prtsrf(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, theta, phi, psi, nq, qw, qnodes, &
+          csca, cabs) BIND(C, NAME='mom_cs_prtsrf') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, mesh, x
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im, theta, phi, psi
+       INTEGER(C_INT32_T), VALUE :: nq
+       REAL(C_DOUBLE), INTENT(IN) :: qw(*), qnodes(*)
+       REAL(C_DOUBLE) :: csca, cabs
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_cs_prtsrf
+  END INTERFACE
+
+CONTAINS
+
+  ! The reference reports failures with WRITE + STOP (linalg.f90:170-179);
+  ! the drop-in keeps that behaviour on any non-zero status.
+  SUBROUTINE mom_check(st, what)
+    INTEGER(C_INT), INTENT(IN) :: st
+    CHARACTER(LEN=*), INTENT(IN) :: what
+    CHARACTER(KIND=C_CHAR), DIMENSION(:), POINTER :: msg
+    INTEGER :: n
+    IF(st==0) RETURN
+    CALL C_F_POINTER(mom_last_error(), msg, (/512/))
+    n = 0
+    DO WHILE(n < 512)
+       IF(msg(n+1)==C_NULL_CHAR) EXIT
+       n = n + 1
+    END DO
+    WRITE(*,*) what, ' failed (status ', st, '): ', msg(1:n)
+    STOP
+  END SUBROUTINE mom_check
+
+END MODULE mom_gpu
