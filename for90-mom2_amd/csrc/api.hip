@@ -194,6 +194,32 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
   m->E = E;
   m->avelen = avelen;
   m->h_fcp.assign(fcp, fcp + 3 * F);
+  // face-side codes of each edge and the slab edge lists of the fill
+  std::vector<int32_t> eside(2 * E, -1);
+  for (int64_t f = 0; f < F; f++)
+    for (int e = 0; e < 3; e++) {
+      const int32_t ge = fedges[3 * f + e];
+      const int slot = (efaces[2 * ge] == f) ? 0 : 1;
+      eside[2 * ge + slot] = (int32_t)(3 * f + e);
+    }
+  std::vector<int32_t> slab_edges;
+  m->h_slab_ptr.assign(1, 0);
+  {
+    std::vector<int64_t> seen(E, -1);
+    const int64_t nslab = (F + kSlab - 1) / kSlab;
+    for (int64_t k = 0; k < nslab; k++) {
+      for (int64_t f = k * kSlab; f < std::min<int64_t>(F, (k + 1) * kSlab); f++)
+        for (int e = 0; e < 3; e++) {
+          const int32_t ge = fedges[3 * f + e];
+          if (seen[ge] != k) {
+            seen[ge] = k;
+            slab_edges.push_back(ge);
+          }
+        }
+      m->h_slab_ptr.push_back((int)slab_edges.size());
+    }
+  }
+  m->h_eparent.assign(eparent, eparent + E);
   std::vector<int32_t> np;
   near_pairs(m->h_fcp, F, avelen, np);
   m->nnear = (int64_t)np.size() / 2;
@@ -206,7 +232,8 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
       (rc = upload(m->fcp, fcp, 3 * F, s)) || (rc = upload(m->s_g, sg.data(), 3 * F, s)) ||
       (rc = upload(m->s_l, sl.data(), 3 * F, s)) || (rc = upload(m->s_coef, scoef.data(), 3 * F, s)) ||
       (rc = upload(m->s_div, sdiv.data(), 3 * F, s)) || (rc = upload(m->s_v, sv.data(), 9 * F, s)) ||
-      (rc = upload(m->near, np.data(), np.size(), s))) {
+      (rc = upload(m->near, np.data(), np.size(), s)) || (rc = upload(m->eside, eside.data(), eside.size(), s)) ||
+      (rc = upload(m->slab_edges, slab_edges.data(), slab_edges.size(), s))) {
     delete m;
     return rc;
   }

@@ -108,13 +108,16 @@ constexpr int kTestPerWave = 4;
 constexpr int kTestPerBlock = kWaves * kTestPerWave;
 
 // Far / smooth kernel: one thread per (test face, source face) pair.
-template <bool SMOOTH>
+// QT > 0 fixes the rule size at compile time (full unrolling of the source
+// point loop, so the exp/sincos chains of different points interleave);
+// QT == 0 takes a.Q at run time.
+template <bool SMOOTH, int QT>
 __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *__restrict__ tpts,
                                                 const double *__restrict__ spts, int lane,
                                                 double dx, double dy, double dz, double acc[36]) {
   // acc layout (complex pairs): 0 S0, 1-3 Srho, 4-6 Srho', 7 Srr',
   //                             8 M0, 9-11 Mrho, 12-14 Mrho', 15-17 Mx
-  const int Q = a.Q;
+  const int Q = QT > 0 ? QT : a.Q;
   const double kr = a.kr, ki = a.ki;
   const double k2r = kr * kr - ki * ki, k2i = 2.0 * kr * ki;
   for (int i = 0; i < Q; i++) {
@@ -122,14 +125,17 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
     const double bx = dx + rx, by = dy + ry, bz = dz + rz;
     double T0r = 0, T0i = 0, T1r[3] = {0, 0, 0}, T1i[3] = {0, 0, 0};
     double U0r = 0, U0i = 0, U1r[3] = {0, 0, 0}, U1i[3] = {0, 0, 0};
-    for (int j = 0; j < Q; j++) {
+#pragma unroll 2
+    for (int j = 0; j < (QT > 0 ? QT : kMaxQ); j++) {
+      if (QT == 0 && j >= Q) break;
       const double sx = spts[(3 * j) * 64 + lane], sy = spts[(3 * j + 1) * 64 + lane],
                    sz = spts[(3 * j + 2) * 64 + lane];
       const double x = bx - sx, y = by - sy, z = bz - sz;
-      const double R = sqrt(x * x + y * y + z * z);
+      const double R2 = x * x + y * y + z * z;
       double Gr, Gi, gr, gi;
       if (!SMOOTH) {
-        const double iR = 1.0 / R;
+        const double iR = rsqrt(R2);
+        const double R = R2 * iR;
         const double ex = (ki == 0.0) ? 1.0 : exp(-ki * R);
         double sn, cs;
         sincos(kr * R, &sn, &cs);
@@ -140,14 +146,15 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
         const double ar = iR * iR + ki * iR, ai = -kr * iR;
         gr = Gr * ar - Gi * ai;
         gi = Gr * ai + Gi * ar;
-      } else if (R == 0.0) {
+      } else if (R2 == 0.0) {
         // green.f90:33-34 (Gs = ik/4pi) and 88-91 (grad Gs = 0)
         Gr = -ki * kOofpi;
         Gi = kr * kOofpi;
         gr = 0.0;
         gi = 0.0;
       } else {
-        const double iR = 1.0 / R;
+        const double iR = rsqrt(R2);
+        const double R = R2 * iR;
         const double ex = (ki == 0.0) ? 1.0 : exp(-ki * R);
         double sn, cs;
         sincos(kr * R, &sn, &cs);
@@ -208,90 +215,102 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
   }
 }
 
-__global__ __launch_bounds__(256) void far_fill_kernel(FarArgs a) {
-  __shared__ double spts[3 * kMaxQ * 64];
-  __shared__ double tpts[kTestPerBlock][3 * kMaxQ];
+// Pair kernel: one thread per (test face m, source face n) with the test
+// faces on the lanes (64 per wave) and four source faces per wave handled
+// in turn (wave-uniform).  The 3x3 edge blocks of D+H (or the single
+// operator asked for) and of K are written with plain, coalesced stores to
+// face-side planes B[type][3p+q][n - n0][m] of the current slab of source
+// faces; assemble_kernel then gathers them into the matrix.
+struct PlaneArgs {
+  FarArgs f;
+  double2 *B;     // [2][9][S][F]
+  int n0, S;      // slab of source faces [n0, n0 + S)
+  int mode;       // 0: D+H and K; 1: D; 2: H; 3: K
+};
+
+constexpr int kSrcPerWave = 4;
+constexpr int kSrcPerBlock = kWaves * kSrcPerWave;
+
+template <int QT>
+__global__ __launch_bounds__(256) void pair_plane_kernel(PlaneArgs pa) {
+  const FarArgs &a = pa.f;
+  __shared__ double tpts[3 * kMaxQ * 64];          // test points, [3j+c][lane]
+  __shared__ double spts[kSrcPerBlock][3 * kMaxQ];  // source points, broadcast
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int Q = a.Q, F = a.F;
-  const int n0 = blockIdx.x * 64, m0 = blockIdx.y * kTestPerBlock;
-  // stage source points of 64 faces, relative to their centroids
+  const int Q = QT > 0 ? QT : a.Q, F = a.F;
+  const int m0 = blockIdx.x * 64;
+  const int nb = pa.n0 + blockIdx.y * kSrcPerBlock;
   for (int idx = threadIdx.x; idx < 64 * Q * 3; idx += blockDim.x) {
-    int l = idx % 64, jc = idx / 64;  // jc = 3*j + c
-    int n = n0 + l;
+    const int l = idx % 64, jc = idx / 64;
+    const int m = m0 + l;
     double v = 0.0;
-    if (n < F) v = a.qp[(int64_t)n * 3 * Q + jc] - a.cp[3 * n + jc % 3];
-    spts[jc * 64 + l] = v;
+    if (m < F) v = a.qp[(int64_t)m * 3 * Q + jc] - a.cp[3 * m + jc % 3];
+    tpts[jc * 64 + l] = v;
   }
-  for (int idx = threadIdx.x; idx < kTestPerBlock * Q * 3; idx += blockDim.x) {
-    int t = idx / (3 * Q), ic = idx - t * 3 * Q;
-    int m = m0 + t;
+  for (int idx = threadIdx.x; idx < kSrcPerBlock * Q * 3; idx += blockDim.x) {
+    const int t = idx / (3 * Q), jc = idx - t * 3 * Q;
+    const int n = nb + t;
     double v = 0.0;
-    if (m < F) v = a.qp[(int64_t)m * 3 * Q + ic] - a.cp[3 * m + ic % 3];
-    tpts[t][ic] = v;
+    if (n < pa.n0 + pa.S && n < F) v = a.qp[(int64_t)n * 3 * Q + jc] - a.cp[3 * n + jc % 3];
+    spts[t][jc] = v;
   }
   __syncthreads();
 
-  const int n = n0 + lane;
-  const bool valid = n < F;
-  const int nn = valid ? n : 0;
-  const double cnx = a.cp[3 * nn], cny = a.cp[3 * nn + 1], cnz = a.cp[3 * nn + 2];
-  const double An = a.area[nn];
-  double coefn[3], divn[3], nun[3][3];
-  int gn[3];
-  for (int q = 0; q < 3; q++) {
-    coefn[q] = a.side.coef[3 * nn + q];
-    divn[q] = a.side.div[3 * nn + q];
-    gn[q] = a.side.g[3 * nn + q];
-    nun[q][0] = a.side.v[3 * (3 * nn + q)] - cnx;
-    nun[q][1] = a.side.v[3 * (3 * nn + q) + 1] - cny;
-    nun[q][2] = a.side.v[3 * (3 * nn + q) + 2] - cnz;
-  }
+  const int m = m0 + lane;
+  const bool valid = m < F;
+  const int mm = valid ? m : 0;
+  const double cmx = a.cp[3 * mm], cmy = a.cp[3 * mm + 1], cmz = a.cp[3 * mm + 2];
+  const int64_t plane = (int64_t)pa.S * F;
 
-  for (int t = 0; t < kTestPerWave; t++) {
-    const int tl = wave * kTestPerWave + t;
-    const int m = m0 + tl;
-    if (m >= F) break;
-    const double *cpm = a.cp + 3 * m;
-    const double dx = cpm[0] - cnx, dy = cpm[1] - cny, dz = cpm[2] - cnz;
-    const bool near = near_pair(cpm, a.cp + 3 * nn, a.thr2);
+  for (int t = 0; t < kSrcPerWave; t++) {
+    const int tl = wave * kSrcPerWave + t;
+    const int n = nb + tl;
+    if (n >= pa.n0 + pa.S || n >= F) break;
+    const double *cpn = a.cp + 3 * n;
+    // Delta = c_m - c_n.  The point loop runs over the source points in its
+    // outer ("i") index and the test points on the lanes ("j"), so it is
+    // given -Delta; its moments then come out with the roles exchanged:
+    // acc[2..7] = S_rho' (source), acc[8..13] = S_rho (test),
+    // acc[18..23] = M_rho', acc[24..29] = M_rho, acc[30..35] = -M_x.
+    const double dx = cmx - cpn[0], dy = cmy - cpn[1], dz = cmz - cpn[2];
+    const bool near = near_pair(a.cp + 3 * mm, cpn, a.thr2);
     double acc[36];
     for (int k = 0; k < 36; k++) acc[k] = 0.0;
     if (near)
-      pair_point_loop<true>(a, tpts[tl], spts, lane, dx, dy, dz, acc);
+      pair_point_loop<true, QT>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
     else
-      pair_point_loop<false>(a, tpts[tl], spts, lane, dx, dy, dz, acc);
+      pair_point_loop<false, QT>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
     if (!valid) continue;
 
-    const double Am = a.area[m];
-    const bool self = (m == n);
-    double coefm[3], divm[3], num[3][3];
-    int gm[3];
-    for (int p = 0; p < 3; p++) {
-      coefm[p] = a.side.coef[3 * m + p];
-      divm[p] = a.side.div[3 * m + p];
-      gm[p] = a.side.g[3 * m + p];
-      for (int c = 0; c < 3; c++) num[p][c] = a.side.v[3 * (3 * m + p) + c] - cpm[c];
-    }
-    const double AA = Am * An;
+    const double AA = a.area[mm] * a.area[n];
+    const bool self = (mm == n);
     const double D3[3] = {dx, dy, dz};
-    for (int p = 0; p < 3; p++) {
-      for (int q = 0; q < 3; q++) {
-        const double *vm = num[p], *vn = nun[q];
+    const int nl = n - pa.n0;
+#pragma unroll 1
+    for (int q = 0; q < 3; q++) {
+      const int sq = 3 * n + q;
+      const double coefn = a.side.coef[sq], divn = a.side.div[sq];
+      const double vn[3] = {a.side.v[3 * sq] - cpn[0], a.side.v[3 * sq + 1] - cpn[1], a.side.v[3 * sq + 2] - cpn[2]};
+#pragma unroll 1
+      for (int p = 0; p < 3; p++) {
+        const int sp = 3 * mm + p;
+        const double coefm = a.side.coef[sp], divm = a.side.div[sp];
+        const double vm[3] = {a.side.v[3 * sp] - cmx, a.side.v[3 * sp + 1] - cmy, a.side.v[3 * sp + 2] - cmz};
         // D: Srr' - nu_n.Srho - nu_m.Srho' + (nu_m.nu_n) S0
-        double vmn = vm[0] * vn[0] + vm[1] * vn[1] + vm[2] * vn[2];
+        const double vmn = vm[0] * vn[0] + vm[1] * vn[1] + vm[2] * vn[2];
         double sr = acc[14] + vmn * acc[0], si = acc[15] + vmn * acc[1];
         for (int c = 0; c < 3; c++) {
-          sr -= vn[c] * acc[2 + 2 * c] + vm[c] * acc[8 + 2 * c];
-          si -= vn[c] * acc[3 + 2 * c] + vm[c] * acc[9 + 2 * c];
+          sr -= vn[c] * acc[8 + 2 * c] + vm[c] * acc[2 + 2 * c];
+          si -= vn[c] * acc[9 + 2 * c] + vm[c] * acc[3 + 2 * c];
         }
-        const double fD = AA * coefm[p] * coefn[q];
+        const double fD = AA * coefm * coefn;
         const double Dr = fD * (a.c1.x * sr - a.c1.y * si);
         const double Di = fD * (a.c1.x * si + a.c1.y * sr);
-        const double fH = AA * divm[p] * divn[q];
+        const double fH = AA * divm * divn;
         const double Hr = fH * (a.c2.x * acc[0] - a.c2.y * acc[1]);
         const double Hi = fH * (a.c2.x * acc[1] + a.c2.y * acc[0]);
         double Kr = 0, Ki = 0;
-        if (!self) {
+        if (!self && (pa.mode == 0 || pa.mode == 3)) {
           // det[rho - nu_m, Delta + rho - rho', rho' - nu_n] summed:
           //  -(Delta + nu_m - nu_n).Mx - [(Delta x nu_n) - (nu_n x nu_m)].Mrho
           //  - [(nu_m x Delta) + (nu_n x nu_m)].Mrho' + nu_m.(Delta x nu_n) M0
@@ -310,15 +329,98 @@ __global__ __launch_bounds__(256) void far_fill_kernel(FarArgs a) {
           double kr = c0 * acc[16], ki = c0 * acc[17];
           for (int c = 0; c < 3; c++) {
             const double cr = dn[c] - nm[c], cs = md[c] + nm[c];
-            kr -= av[c] * acc[30 + 2 * c] + cr * acc[18 + 2 * c] + cs * acc[24 + 2 * c];
-            ki -= av[c] * acc[31 + 2 * c] + cr * acc[19 + 2 * c] + cs * acc[25 + 2 * c];
+            kr += av[c] * acc[30 + 2 * c] - cr * acc[24 + 2 * c] - cs * acc[18 + 2 * c];
+            ki += av[c] * acc[31 + 2 * c] - cr * acc[25 + 2 * c] - cs * acc[19 + 2 * c];
           }
           Kr = fD * kr;
           Ki = fD * ki;
         }
-        emit(a.out, gm[p], gn[q], Dr, Di, Hr, Hi, Kr, Ki);
+        double vr, vi;
+        if (pa.mode == 1) {
+          vr = Dr;
+          vi = Di;
+        } else if (pa.mode == 2) {
+          vr = Hr;
+          vi = Hi;
+        } else {
+          vr = Dr + Hr;
+          vi = Di + Hi;
+        }
+        const int64_t off = (int64_t)(3 * p + q) * plane + (int64_t)nl * F + mm;
+        pa.B[off] = make_double2(vr, vi);
+        pa.B[9 * plane + off] = make_double2(Kr, Ki);
       }
     }
+  }
+}
+
+// Gathers the face-side planes of one slab into the matrix: thread
+// (row r of the output numbering, source edge e2 of the slab) sums the
+// contributions of the (<= 2) test face sides of r's edge and the (<= 2)
+// source face sides of e2 that lie in the slab, then updates the entry with
+// one read-modify-write (each entry is owned by one thread per launch).
+struct AssembleArgs {
+  const double2 *B;
+  int F, S, n0;
+  const int *row_local;   // output row -> local edge of this mesh, -1 if none
+  int64_t nrows;
+  const int *slab_edges;  // local source edges of this slab
+  const int *eside;       // [E][2] face-side codes 3f+s of each local edge, -1 if none
+  const int *col_index;   // local edge -> output column
+  FillTarget out;
+};
+
+__global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs g) {
+  const int64_t r = blockIdx.x * 256 + (int64_t)threadIdx.x;
+  if (r >= g.nrows) return;
+  const int e1 = g.row_local[r];
+  if (e1 < 0) return;
+  const int e2 = g.slab_edges[blockIdx.y];
+  const int64_t plane = (int64_t)g.S * g.F;
+  const int s1[2] = {g.eside[2 * e1], g.eside[2 * e1 + 1]};
+  const int s2[2] = {g.eside[2 * e2], g.eside[2 * e2 + 1]};
+  double ar = 0, ai = 0, kr = 0, ki = 0;
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    if (s2[b] < 0) continue;
+    const int n = s2[b] / 3, q = s2[b] - 3 * n;
+    const int nl = n - g.n0;
+    if (nl < 0 || nl >= g.S) continue;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      if (s1[a] < 0) continue;
+      const int m = s1[a] / 3, p = s1[a] - 3 * m;
+      const int64_t off = (int64_t)(3 * p + q) * plane + (int64_t)nl * g.F + m;
+      const double2 v = g.B[off], k = g.B[9 * plane + off];
+      ar += v.x;
+      ai += v.y;
+      kr += k.x;
+      ki += k.y;
+    }
+  }
+  const FillTarget &o = g.out;
+  const int64_t gi = r, gj = g.col_index[e2];
+  if (o.mode == 0) {
+    const int64_t N = o.Noff, ld = o.ld;
+    double2 *z = o.Z + gi + gj * ld;
+    double2 v = *z;
+    *z = make_double2(v.x + ar, v.y + ai);
+    z = o.Z + (gi + N) + (gj + N) * ld;
+    v = *z;
+    *z = make_double2(v.x + (o.zsi.x * ar - o.zsi.y * ai), v.y + (o.zsi.x * ai + o.zsi.y * ar));
+    z = o.Z + (gi + N) + gj * ld;
+    v = *z;
+    *z = make_double2(v.x + kr, v.y + ki);
+    z = o.Z + gi + (gj + N) * ld;
+    v = *z;
+    *z = make_double2(v.x - kr, v.y - ki);
+  } else {
+    double2 *z = o.Z + gi + gj * o.ld;
+    double2 v = *z;
+    if (o.mode == 3)
+      *z = make_double2(v.x + kr, v.y + ki);
+    else
+      *z = make_double2(v.x + ar, v.y + ai);
   }
 }
 
@@ -613,9 +715,59 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   a.c1 = make_double2(c1.real(), c1.imag());
   a.c2 = make_double2(c2.real(), c2.imag());
   a.out = t;
-  dim3 grid((unsigned)((m->F + 63) / 64), (unsigned)((m->F + kTestPerBlock - 1) / kTestPerBlock));
-  hipLaunchKernelGGL(far_fill_kernel, grid, dim3(256), 0, ctx->stream, a);
-  MOM_HIP(hipGetLastError());
+  // far/smooth pairs, slab by slab of source faces, through the face-side
+  // planes (no atomics on the matrix)
+  const int S = (int)std::min<int64_t>(m->F, kSlab);
+  const size_t bbytes = sizeof(double2) * 18 * (size_t)S * m->F;
+  if (ctx->scratch.bytes < bbytes && (rc = ctx->scratch.alloc(bbytes))) return rc;
+  // output row -> local edge
+  const int64_t nrows = t.mode == 0 ? t.Noff : m->E;
+  std::vector<int> rl(nrows, -1);
+  std::vector<int> colidx(m->E);
+  for (int64_t e = 0; e < m->E; e++) {
+    const int g = t.mode == 0 ? m->h_eparent[e] : (int)e;
+    rl[g] = (int)e;
+    colidx[e] = g;
+  }
+  DevBuf drl, dcol;
+  if ((rc = drl.alloc(sizeof(int) * nrows)) || (rc = dcol.alloc(sizeof(int) * m->E))) return rc;
+  MOM_HIP(hipMemcpyAsync(drl.ptr, rl.data(), sizeof(int) * nrows, hipMemcpyHostToDevice, ctx->stream));
+  MOM_HIP(hipMemcpyAsync(dcol.ptr, colidx.data(), sizeof(int) * m->E, hipMemcpyHostToDevice, ctx->stream));
+  PlaneArgs pa;
+  pa.f = a;
+  pa.B = ctx->scratch.as<double2>();
+  pa.S = S;
+  pa.mode = t.mode;
+  AssembleArgs ga;
+  ga.B = pa.B;
+  ga.F = (int)m->F;
+  ga.S = S;
+  ga.row_local = drl.as<int>();
+  ga.nrows = nrows;
+  ga.eside = m->eside.as<int>();
+  ga.col_index = dcol.as<int>();
+  ga.out = t;
+  const int nslab = (int)((m->F + S - 1) / S);
+  for (int k = 0; k < nslab; k++) {
+    pa.n0 = k * S;
+    const int scur = (int)std::min<int64_t>(S, m->F - pa.n0);
+    dim3 grid((unsigned)((m->F + 63) / 64), (unsigned)((scur + kSrcPerBlock - 1) / kSrcPerBlock));
+    if (rule.Q == 13)
+      hipLaunchKernelGGL(pair_plane_kernel<13>, grid, dim3(256), 0, ctx->stream, pa);
+    else
+      hipLaunchKernelGGL(pair_plane_kernel<0>, grid, dim3(256), 0, ctx->stream, pa);
+    MOM_HIP(hipGetLastError());
+    const int ne = m->h_slab_ptr[k + 1] - m->h_slab_ptr[k];
+    if (ne > 0) {
+      ga.n0 = pa.n0;
+      ga.slab_edges = m->slab_edges.as<int>() + m->h_slab_ptr[k];
+      hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)((nrows + 255) / 256), (unsigned)ne), dim3(256), 0,
+                         ctx->stream, ga);
+      MOM_HIP(hipGetLastError());
+    }
+  }
+  // the row/column maps must outlive the queued kernels
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
 
   if (m->nnear > 0) {
     NearArgs b;

@@ -76,7 +76,11 @@ constexpr int IB = 32;   // inner panel block
 // ---------------------------------------------------------------------------
 // Trailing update C -= A B (all complex, column-major), MFMA tiles.
 // Wave tile (16 WM) x (16 WN) complex, 4 waves in a 2 x 2 arrangement.
-template <typename T, int WM, int WN>
+// M3 = false: four real products per complex product (complex128 path).
+// M3 = true:  three (Gauss/3M: T1 = Ar Br, T2 = Ai Bi, T3 = (Ar+Ai)(Br+Bi),
+//             Re = T1 - T2, Im = T3 - T1 - T2), used for the complex64
+//             preconditioner factorisation; (re+im) planes are staged in LDS.
+template <typename T, int WM, int WN, bool M3>
 __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::type *__restrict__ A, int64_t lda,
                                                        const typename Cx<T>::type *__restrict__ B, int64_t ldb,
                                                        typename Cx<T>::type *__restrict__ Cm, int64_t ldc, int M,
@@ -85,18 +89,20 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
   using acc4 = typename Cx<T>::acc4;
   constexpr int BM = 32 * WM, BN = 32 * WN, BK = 16;
   constexpr int PA = BM + 16 / (int)(sizeof(T) / 4), PB = BN + 16 / (int)(sizeof(T) / 4);
-  __shared__ T As[2][BK][PA];
-  __shared__ T Bs[2][BK][PB];
+  constexpr int NP = M3 ? 3 : 2;  // planes: re, im (, re+im)
+  __shared__ T As[NP][BK][PA];
+  __shared__ T Bs[NP][BK][PB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const int wr = (wave & 1) * 16 * WM, wc = (wave >> 1) * 16 * WN;
   const int64_t row0 = (int64_t)blockIdx.x * BM, col0 = (int64_t)blockIdx.y * BN;
-  acc4 accr[WM][WN], acci[WM][WN];
+  acc4 acc0[WM][WN], acc1[WM][WN], acc2[WM][WN];
   for (int i = 0; i < WM; i++)
     for (int j = 0; j < WN; j++)
       for (int v = 0; v < 4; v++) {
-        accr[i][j][v] = 0;
-        acci[i][j][v] = 0;
+        acc0[i][j][v] = 0;
+        acc1[i][j][v] = 0;
+        acc2[i][j][v] = 0;
       }
   for (int k0 = 0; k0 < K; k0 += BK) {
     for (int idx = tid; idx < BM * BK; idx += 256) {
@@ -109,6 +115,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
       if (gr < M && gk < K) v = A[gr + (int64_t)gk * lda];
       As[0][k][r] = v.x;
       As[1][k][r] = v.y;
+      if (M3) As[NP - 1][k][r] = v.x + v.y;
     }
     for (int idx = tid; idx < BN * BK; idx += 256) {
       const int k = idx % BK, c = idx / BK;
@@ -120,30 +127,39 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
       if (gc < N && gk < K) v = B[gk + gc * ldb];
       Bs[0][k][c] = v.x;
       Bs[1][k][c] = v.y;
+      if (M3) Bs[NP - 1][k][c] = v.x + v.y;
     }
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      T ar[WM], ai[WM], br[WN], bi[WN];
+      T ar[WM], ai[WM], as_[WM], br[WN], bi[WN], bs[WN];
 #pragma unroll
       for (int i = 0; i < WM; i++) {
         ar[i] = As[0][kk + kq][wr + 16 * i + l16];
         ai[i] = As[1][kk + kq][wr + 16 * i + l16];
+        if (M3) as_[i] = As[NP - 1][kk + kq][wr + 16 * i + l16];
       }
 #pragma unroll
       for (int j = 0; j < WN; j++) {
         br[j] = Bs[0][kk + kq][wc + 16 * j + l16];
         bi[j] = Bs[1][kk + kq][wc + 16 * j + l16];
+        if (M3) bs[j] = Bs[NP - 1][kk + kq][wc + 16 * j + l16];
       }
 #pragma unroll
       for (int i = 0; i < WM; i++) {
         const T nai = -ai[i];
 #pragma unroll
         for (int j = 0; j < WN; j++) {
-          accr[i][j] = mfma4<T>(ar[i], br[j], accr[i][j]);
-          accr[i][j] = mfma4<T>(nai, bi[j], accr[i][j]);
-          acci[i][j] = mfma4<T>(ar[i], bi[j], acci[i][j]);
-          acci[i][j] = mfma4<T>(ai[i], br[j], acci[i][j]);
+          if (M3) {
+            acc0[i][j] = mfma4<T>(ar[i], br[j], acc0[i][j]);   // T1
+            acc1[i][j] = mfma4<T>(ai[i], bi[j], acc1[i][j]);   // T2
+            acc2[i][j] = mfma4<T>(as_[i], bs[j], acc2[i][j]);  // T3
+          } else {
+            acc0[i][j] = mfma4<T>(ar[i], br[j], acc0[i][j]);   // Re
+            acc0[i][j] = mfma4<T>(nai, bi[j], acc0[i][j]);
+            acc1[i][j] = mfma4<T>(ar[i], bi[j], acc1[i][j]);   // Im
+            acc1[i][j] = mfma4<T>(ai[i], br[j], acc1[i][j]);
+          }
         }
       }
     }
@@ -161,8 +177,13 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
         if (row < M) {
           C2 *p = Cm + row + col * ldc;
           C2 c = *p;
-          c.x -= accr[i][j][v];
-          c.y -= acci[i][j][v];
+          if (M3) {
+            c.x -= acc0[i][j][v] - acc1[i][j][v];
+            c.y -= acc2[i][j][v] - acc0[i][j][v] - acc1[i][j][v];
+          } else {
+            c.x -= acc0[i][j][v];
+            c.y -= acc1[i][j][v];
+          }
           *p = c;
         }
       }
@@ -170,8 +191,8 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
 }
 
 template <typename T> struct GemmCfg;
-template <> struct GemmCfg<float> { static constexpr int WM = 2, WN = 2; };
-template <> struct GemmCfg<double> { static constexpr int WM = 1, WN = 2; };
+template <> struct GemmCfg<float> { static constexpr int WM = 2, WN = 2; static constexpr bool M3 = false; };
+template <> struct GemmCfg<double> { static constexpr int WM = 1, WN = 2; static constexpr bool M3 = false; };
 
 // Event-pair timing of the trailing-update kernel launches (enabled per
 // system); events are read back after the factorisation, so timing does
@@ -232,8 +253,8 @@ static int gemm_sub(hipStream_t s, const typename Cx<T>::type *A, int64_t lda, c
   const bool timed = st && st->enabled;
   int rc;
   if (timed && (rc = st->begin(s))) return rc;
-  hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M,
-                     (int)N, (int)K);
+  hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::M3>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc,
+                     (int)M, (int)N, (int)K);
   MOM_HIP(hipGetLastError());
   if (timed && (rc = st->end(s, 8.0 * (double)M * (double)N * (double)K))) return rc;
   return 0;
@@ -562,124 +583,127 @@ __global__ __launch_bounds__(256) void diag_inverse_kernel(const typename Cx<T>:
   }
 }
 
-// y[r] = sum_c M[r, c] x[c] for an NB x NB block M (column-major), one
-// thread per row, nrhs columns of x / y (leading dims ldx, ldy).
-template <typename T>
-__global__ __launch_bounds__(256) void block_apply_kernel(const typename Cx<T>::type *__restrict__ M,
-                                                           const typename Cx<T>::type *__restrict__ x, int64_t ldx,
-                                                           typename Cx<T>::type *__restrict__ y, int64_t ldy,
-                                                           int w) {
-  using C2 = typename Cx<T>::type;
-  const int r = threadIdx.x;
-  const int64_t rhs = blockIdx.x;
-  __shared__ C2 xs[NB];
-  xs[r] = (r < w) ? x[rhs * ldx + r] : C2{0, 0};
-  __syncthreads();
-  if (r >= w) return;
-  C2 acc{0, 0};
-  for (int c = 0; c < w; c++) {
-    const C2 m = M[(int64_t)c * NB + r];
-    const C2 v = xs[c];
-    acc.x += m.x * v.x - m.y * v.y;
-    acc.y += m.x * v.y + m.y * v.x;
-  }
-  y[rhs * ldy + r] = acc;
+// Solves with the LU factors.  The factors are stored in TS (complex64 or
+// complex128); vectors and arithmetic are in TC.  With TS = float, TC =
+// double the preconditioner of the refinement is an exact FP64 linear
+// operator built from the stored FP32 factors.
+template <typename TS, typename TC>
+__device__ __forceinline__ void cmac(typename Cx<TC>::type &acc, typename Cx<TS>::type m, typename Cx<TC>::type v) {
+  const TC mr = (TC)m.x, mi = (TC)m.y;
+  acc.x += mr * v.x - mi * v.y;
+  acc.y += mr * v.y + mi * v.x;
 }
 
-// b[rows] -= A[rows, c0:c0+w] x_blk, rows [r0, r1); column-major A.
-template <typename T>
-__global__ __launch_bounds__(256) void panel_gemv_sub_kernel(const typename Cx<T>::type *__restrict__ A, int64_t ld,
+// y[r] = sum_c M[r, c] x[c] for the NB x NB diagonal-block inverse M; the
+// result overwrites x (in tmp) and is kept in xb for the panel update.
+// Grid (NB/32, nrhs): 32 rows per workgroup, 8 column slices per row.
+template <typename TS, typename TC>
+__global__ __launch_bounds__(256) void block_apply_kernel(const typename Cx<TS>::type *__restrict__ M,
+                                                           typename Cx<TC>::type *x, int64_t ldx,
+                                                           typename Cx<TC>::type *__restrict__ xb, int w) {
+  using CV = typename Cx<TC>::type;
+  __shared__ CV xs[NB];
+  __shared__ CV part[8][32];
+  const int64_t rhs = blockIdx.y;
+  for (int i = threadIdx.x; i < NB; i += 256) xs[i] = (i < w) ? x[rhs * ldx + i] : CV{0, 0};
+  __syncthreads();
+  const int rl = threadIdx.x & 31, sl = threadIdx.x >> 5;
+  const int r = blockIdx.x * 32 + rl;
+  CV acc{0, 0};
+  if (r < w) {
+    for (int c = sl; c < w; c += 8) cmac<TS, TC>(acc, M[(int64_t)c * NB + r], xs[c]);
+  }
+  part[sl][rl] = acc;
+  __syncthreads();
+  if (sl == 0 && r < w) {
+    CV s = part[0][rl];
+    for (int k = 1; k < 8; k++) {
+      s.x += part[k][rl].x;
+      s.y += part[k][rl].y;
+    }
+    x[rhs * ldx + r] = s;
+    xb[rhs * NB + r] = s;
+  }
+}
+
+// b[r] -= sum_{c < w} A[r, c0 + c] xb[c] for rows [r0, r1); 64 rows x 4
+// column slices per workgroup.
+template <typename TS, typename TC>
+__global__ __launch_bounds__(256) void panel_gemv_sub_kernel(const typename Cx<TS>::type *__restrict__ A, int64_t ld,
                                                               int64_t r0, int64_t r1, int64_t c0, int w,
-                                                              const typename Cx<T>::type *__restrict__ xb,
-                                                              typename Cx<T>::type *b, int64_t ldb) {
-  using C2 = typename Cx<T>::type;
-  __shared__ C2 xs[NB];
+                                                              const typename Cx<TC>::type *__restrict__ xb,
+                                                              typename Cx<TC>::type *b, int64_t ldb) {
+  using CV = typename Cx<TC>::type;
+  __shared__ CV xs[NB];
+  __shared__ CV part[4][64];
   const int64_t rhs = blockIdx.y;
   for (int i = threadIdx.x; i < w; i += 256) xs[i] = xb[rhs * NB + i];
   __syncthreads();
-  const int64_t r = r0 + blockIdx.x * 256 + threadIdx.x;
-  if (r >= r1) return;
-  C2 acc{0, 0};
-  const C2 *a = A + c0 * ld + r;
-  for (int c = 0; c < w; c++) {
-    const C2 m = a[(int64_t)c * ld];
-    const C2 v = xs[c];
-    acc.x += m.x * v.x - m.y * v.y;
-    acc.y += m.x * v.y + m.y * v.x;
+  const int rl = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const int64_t r = r0 + blockIdx.x * 64 + rl;
+  CV acc{0, 0};
+  if (r < r1) {
+    const typename Cx<TS>::type *a = A + c0 * ld + r;
+#pragma unroll 4
+    for (int c = sl; c < w; c += 4) cmac<TS, TC>(acc, a[(int64_t)c * ld], xs[c]);
   }
-  C2 *p = b + rhs * ldb + r;
-  C2 o = *p;
-  o.x -= acc.x;
-  o.y -= acc.y;
-  *p = o;
+  part[sl][rl] = acc;
+  __syncthreads();
+  if (sl == 0 && r < r1) {
+    CV s = part[0][rl];
+    for (int k = 1; k < 4; k++) {
+      s.x += part[k][rl].x;
+      s.y += part[k][rl].y;
+    }
+    CV *p = b + rhs * ldb + r;
+    CV o = *p;
+    o.x -= s.x;
+    o.y -= s.y;
+    *p = o;
+  }
 }
 
-// Solve L U x = P b in place on B (n x nrhs, ldb); perm is the row gather
-// (B_perm[i] = B[perm[i]]).  xb is NB x nrhs scratch, tmp n x nrhs scratch.
-template <typename T>
-static int getrs(hipStream_t s, const typename Cx<T>::type *LU, int64_t ld, int64_t n, const int *perm,
-                 const typename Cx<T>::type *Linv, const typename Cx<T>::type *Uinv, typename Cx<T>::type *B,
-                 int64_t ldb, int64_t nrhs, typename Cx<T>::type *tmp, typename Cx<T>::type *xb);
-
-template <typename T>
-__global__ void gather_copy_kernel(const typename Cx<T>::type *__restrict__ src, int64_t lds,
-                                   typename Cx<T>::type *__restrict__ dst, int64_t ldd, int w) {
-  const int r = threadIdx.x;
-  if (r < w) dst[blockIdx.x * ldd + r] = src[blockIdx.x * lds + r];
-}
-
-template <typename T>
-__global__ void gather_rows_kernel(const typename Cx<T>::type *__restrict__ src, int64_t lds,
-                                   typename Cx<T>::type *__restrict__ dst, int64_t ldd, const int *__restrict__ perm,
-                                   int64_t n, int64_t nrhs) {
+template <typename TV>
+__global__ void gather_rows_kernel(const TV *__restrict__ src, int64_t lds, TV *__restrict__ dst, int64_t ldd,
+                                   const int *__restrict__ perm, int64_t n, int64_t nrhs) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n * nrhs) return;
   const int64_t i = t % n, r = t / n;
   dst[r * ldd + i] = src[r * lds + perm[i]];
 }
 
-template <typename T>
-static int getrs(hipStream_t s, const typename Cx<T>::type *LU, int64_t ld, int64_t n, const int *perm,
-                 const typename Cx<T>::type *Linv, const typename Cx<T>::type *Uinv, typename Cx<T>::type *B,
-                 int64_t ldb, int64_t nrhs, typename Cx<T>::type *tmp, typename Cx<T>::type *xb) {
-  // tmp = P B
-  hipLaunchKernelGGL(gather_rows_kernel<T>, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, B, ldb, tmp,
+// Solve L U x = P b in place on B (n x nrhs, ldb, TC); perm is the row
+// gather (B_perm[i] = B[perm[i]]); tmp is n x nrhs and xb NB x nrhs scratch.
+template <typename TS, typename TC>
+static int getrs(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, int64_t n, const int *perm,
+                 const typename Cx<TS>::type *Linv, const typename Cx<TS>::type *Uinv, typename Cx<TC>::type *B,
+                 int64_t ldb, int64_t nrhs, typename Cx<TC>::type *tmp, typename Cx<TC>::type *xb) {
+  using CV = typename Cx<TC>::type;
+  hipLaunchKernelGGL(gather_rows_kernel<CV>, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, B, ldb, tmp,
                      n, perm, n, nrhs);
   MOM_HIP(hipGetLastError());
   const int64_t nblk = (n + NB - 1) / NB;
-  // forward: tmp_b := Linv_bb tmp_b ; tmp_below -= L_below,b tmp_b
   for (int64_t b = 0; b < nblk; b++) {
     const int64_t r0 = b * NB;
     const int w = (int)std::min<int64_t>(NB, n - r0);
-    hipLaunchKernelGGL(block_apply_kernel<T>, dim3((unsigned)nrhs), dim3(NB), 0, s, Linv + b * NB * NB, tmp + r0,
-                       n, xb, (int64_t)NB, w);
-    MOM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gather_copy_kernel<T>, dim3((unsigned)nrhs), dim3(NB), 0, s, xb, (int64_t)NB, tmp + r0, n, w);
-    MOM_HIP(hipGetLastError());
-    if (r0 + w < n) {
-      hipLaunchKernelGGL(panel_gemv_sub_kernel<T>, dim3((unsigned)((n - r0 - w + 255) / 256), (unsigned)nrhs),
+    hipLaunchKernelGGL((block_apply_kernel<TS, TC>), dim3(NB / 32, (unsigned)nrhs), dim3(256), 0, s,
+                       Linv + b * NB * NB, tmp + r0, n, xb, w);
+    if (r0 + w < n)
+      hipLaunchKernelGGL((panel_gemv_sub_kernel<TS, TC>), dim3((unsigned)((n - r0 - w + 63) / 64), (unsigned)nrhs),
                          dim3(256), 0, s, LU, ld, r0 + w, n, r0, w, xb, tmp, n);
-      MOM_HIP(hipGetLastError());
-    }
   }
-  // backward: tmp_b := Uinv_bb tmp_b ; tmp_above -= U_above,b tmp_b
   for (int64_t b = nblk - 1; b >= 0; b--) {
     const int64_t r0 = b * NB;
     const int w = (int)std::min<int64_t>(NB, n - r0);
-    hipLaunchKernelGGL(block_apply_kernel<T>, dim3((unsigned)nrhs), dim3(NB), 0, s, Uinv + b * NB * NB, tmp + r0,
-                       n, xb, (int64_t)NB, w);
-    MOM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gather_copy_kernel<T>, dim3((unsigned)nrhs), dim3(NB), 0, s, xb, (int64_t)NB, tmp + r0, n, w);
-    MOM_HIP(hipGetLastError());
-    if (r0 > 0) {
-      hipLaunchKernelGGL(panel_gemv_sub_kernel<T>, dim3((unsigned)((r0 + 255) / 256), (unsigned)nrhs), dim3(256), 0,
-                         s, LU, ld, (int64_t)0, r0, r0, w, xb, tmp, n);
-      MOM_HIP(hipGetLastError());
-    }
+    hipLaunchKernelGGL((block_apply_kernel<TS, TC>), dim3(NB / 32, (unsigned)nrhs), dim3(256), 0, s,
+                       Uinv + b * NB * NB, tmp + r0, n, xb, w);
+    if (r0 > 0)
+      hipLaunchKernelGGL((panel_gemv_sub_kernel<TS, TC>), dim3((unsigned)((r0 + 63) / 64), (unsigned)nrhs), dim3(256),
+                         0, s, LU, ld, (int64_t)0, r0, r0, w, xb, tmp, n);
   }
-  // B = tmp
-  MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(typename Cx<T>::type), tmp, n * sizeof(typename Cx<T>::type),
-                           n * sizeof(typename Cx<T>::type), nrhs, hipMemcpyDeviceToDevice, s));
+  MOM_HIP(hipGetLastError());
+  MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(CV), tmp, n * sizeof(CV), n * sizeof(CV), nrhs, hipMemcpyDeviceToDevice,
+                           s));
   return 0;
 }
 
@@ -792,20 +816,11 @@ __global__ __launch_bounds__(256) void to_fp32_kernel(const double2 *__restrict_
   }
 }
 
-// t32 = fl32(rscale .* v)
-__global__ void scale_down_kernel(const double2 *__restrict__ v, const double *__restrict__ rscale, int64_t n,
-                                  float2 *t) {
+// y = d .* v (diagonal scaling, complex128)
+__global__ void dscale_kernel(const double2 *__restrict__ v, const double *__restrict__ d, int64_t n, double2 *y) {
   const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (r >= n) return;
-  t[r] = make_float2((float)(v[r].x * rscale[r]), (float)(v[r].y * rscale[r]));
-}
-
-// y = cscale .* t32
-__global__ void scale_up_kernel(const float2 *__restrict__ t, const double *__restrict__ cscale, int64_t n,
-                                double2 *y) {
-  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  y[r] = make_double2((double)t[r].x * cscale[r], (double)t[r].y * cscale[r]);
+  y[r] = make_double2(v[r].x * d[r], v[r].y * d[r]);
 }
 
 template <typename T>
@@ -893,21 +908,25 @@ static int upload_perm(mom_system *S, DevBuf &dperm) {
 
 // Mixed precision solve of one right-hand side b (host, complex128) into x.
 struct IrWork {
-  DevBuf dperm, v64, w64, t32, tmp32, xb32, b64, x64;
+  DevBuf dperm, v64, w64, t64, tmp64, xb64, b64, x64;
 };
 
 static int apply_minv(mom_system *S, IrWork &W, const std::vector<cd> &v, std::vector<cd> &y) {
+  // M^-1 v = C U^-1 L^-1 P R v with the complex64 factors of R Z C, applied
+  // in complex128 arithmetic
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
   MOM_HIP(hipMemcpyAsync(W.v64.ptr, v.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
   const unsigned g = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(scale_down_kernel, dim3(g), dim3(256), 0, s, W.v64.as<double2>(), S->rscale.as<double>(), n,
-                     W.t32.as<float2>());
-  int rc = getrs<float>(s, S->Z32.as<float2>(), n, n, W.dperm.as<int>(), S->Linv.as<float2>(), S->Uinv.as<float2>(),
-                        W.t32.as<float2>(), n, 1, W.tmp32.as<float2>(), W.xb32.as<float2>());
+  hipLaunchKernelGGL(dscale_kernel, dim3(g), dim3(256), 0, s, W.v64.as<double2>(), S->rscale.as<double>(), n,
+                     W.t64.as<double2>());
+  int rc = getrs<float, double>(s, S->Z32.as<float2>(), n, n, W.dperm.as<int>(), S->Linv.as<float2>(),
+                                S->Uinv.as<float2>(), W.t64.as<double2>(), n, 1, W.tmp64.as<double2>(),
+                                W.xb64.as<double2>());
   if (rc) return rc;
-  hipLaunchKernelGGL(scale_up_kernel, dim3(g), dim3(256), 0, s, W.t32.as<float2>(), S->cscale.as<double>(), n,
+  hipLaunchKernelGGL(dscale_kernel, dim3(g), dim3(256), 0, s, W.t64.as<double2>(), S->cscale.as<double>(), n,
                      W.w64.as<double2>());
+  MOM_HIP(hipGetLastError());
   y.resize(n);
   MOM_HIP(hipMemcpyAsync(y.data(), W.w64.ptr, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
   MOM_HIP(hipStreamSynchronize(s));
@@ -961,6 +980,8 @@ static int gmres_ir(mom_system *S, IrWork &W, const std::vector<cd> &b, std::vec
     if ((rc = residual(S, W, &b, x, r))) return rc;
     be = inf_norm(r) / (S->znorm_inf * inf_norm(x) + bn);
     if (be <= tol || it >= maxit) break;
+    // reduce the correction's residual just enough for the next outer step
+    const double inner_tol = std::min(1e-2, std::max(1e-12, 0.1 * tol / be));
     if ((rc = apply_minv(S, W, r, z))) return rc;
     double beta = std::sqrt(std::real(dotc(z, z)));
     if (beta == 0) break;
@@ -997,7 +1018,7 @@ static int gmres_ir(mom_system *S, IrWork &W, const std::vector<cd> &b, std::vec
       cd gk = g[k];
       g[k] = std::conj(cs[k]) * gk;
       g[k + 1] = -sn[k] * gk;
-      if (std::abs(g[k + 1]) <= 1e-6 * beta || hn == 0) {
+      if (std::abs(g[k + 1]) <= inner_tol * beta || hn == 0) {
         k++;
         break;
       }
@@ -1022,8 +1043,8 @@ static int ir_alloc(mom_system *S, IrWork &W) {
   int rc;
   if ((rc = upload_perm(S, W.dperm)) || (rc = W.v64.alloc(sizeof(double2) * n)) ||
       (rc = W.w64.alloc(sizeof(double2) * n)) || (rc = W.b64.alloc(sizeof(double2) * n)) ||
-      (rc = W.x64.alloc(sizeof(double2) * n)) || (rc = W.t32.alloc(sizeof(float2) * n)) ||
-      (rc = W.tmp32.alloc(sizeof(float2) * n)) || (rc = W.xb32.alloc(sizeof(float2) * NB)))
+      (rc = W.x64.alloc(sizeof(double2) * n)) || (rc = W.t64.alloc(sizeof(double2) * n)) ||
+      (rc = W.tmp64.alloc(sizeof(double2) * n)) || (rc = W.xb64.alloc(sizeof(double2) * NB)))
     return rc;
   return 0;
 }
@@ -1161,7 +1182,7 @@ int mom_system_solve(mom_system *S, int64_t nrhs, double *B, int64_t ldb, double
       return rc;
     MOM_HIP(hipMemcpy2DAsync(dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs,
                              hipMemcpyHostToDevice, s));
-    if ((rc = getrs<double>(s, S->Z64.as<double2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
+    if ((rc = getrs<double, double>(s, S->Z64.as<double2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
                             S->Uinv.as<double2>(), dB.as<double2>(), n, nrhs, tmp.as<double2>(), xb.as<double2>())))
       return rc;
     MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs,
@@ -1239,7 +1260,7 @@ int mom_solve_multi_linsys(mom_ctx *ctx, int64_t n, double *A, int64_t lda, int6
   if (info == 0) {
     MOM_HIP(hipMemcpy2DAsync(dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs,
                              hipMemcpyHostToDevice, s));
-    if ((rc = getrs<double>(s, S.Z64.as<double2>(), n, n, dperm.as<int>(), S.Linv.as<double2>(), S.Uinv.as<double2>(),
+    if ((rc = getrs<double, double>(s, S.Z64.as<double2>(), n, n, dperm.as<int>(), S.Linv.as<double2>(), S.Uinv.as<double2>(),
                             dB.as<double2>(), n, nrhs, tmp.as<double2>(), xb.as<double2>())))
       return rc;
     MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs,

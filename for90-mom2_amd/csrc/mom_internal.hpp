@@ -21,6 +21,7 @@ constexpr double kEta0 = 3.767303134622e2;
 inline double mu0() { return 4 * MOM_PI * 1e-7; }
 
 constexpr int kMaxQ = 16;  // largest triangle rule supported by the kernels
+constexpr int kSlab = 1024; // source faces per fill slab (face-side planes of 18*kSlab*F complex)
 
 using cd = std::complex<double>;
 
@@ -72,6 +73,11 @@ struct mom_mesh {
   mom::DevBuf s_div;   // sign * L / A (surface divergence)
   mom::DevBuf s_v;     // free vertex (3 doubles)
   mom::DevBuf near;    // int32 pairs (test face, source face), sorted
+  // fill assembly: face-side codes 3f+s of each edge's two faces (-1 none),
+  // and per slab of kSlab source faces the edges with a side in the slab
+  mom::DevBuf eside, slab_edges;
+  std::vector<int> h_slab_ptr;
+  std::vector<int> h_eparent;
 };
 
 namespace mom {
