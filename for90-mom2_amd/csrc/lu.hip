@@ -74,76 +74,118 @@ constexpr int NB = 256;  // outer block (trailing update depth)
 constexpr int IB = 32;   // inner panel block
 
 // ---------------------------------------------------------------------------
-// Trailing update C -= A B (all complex, column-major), MFMA tiles.
-// Wave tile (16 WM) x (16 WN) complex, 4 waves in a 2 x 2 arrangement.
-// M3 = false: four real products per complex product (complex128 path).
-// M3 = true:  three (Gauss/3M: T1 = Ar Br, T2 = Ai Bi, T3 = (Ar+Ai)(Br+Bi),
-//             Re = T1 - T2, Im = T3 - T1 - T2), used for the complex64
-//             preconditioner factorisation; (re+im) planes are staged in LDS.
-template <typename T, int WM, int WN, bool M3>
+// Trailing update C -= A B (all complex, column-major), on the matrix cores.
+//  - workgroup of 4 waves in a 2 x 2 arrangement; wave tile (16 WM) x (16 WN)
+//    complex, workgroup tile BM x BN = (32 WM) x (32 WN);
+//  - K is walked in chunks of BK complex: the next chunk is fetched into
+//    registers while the current one is multiplied out of LDS (two LDS
+//    buffers, planar re/im), one barrier per chunk;
+//  - LDS rows are padded by 16 elements so that the four k-rows a wave reads
+//    per MFMA operand fall into disjoint bank groups;
+//  - tiles are visited in groups of kGroupM row blocks so that concurrently
+//    running workgroups share A and B panels in L2;
+//  - M3 = false: four real products per complex product; M3 = true: three
+//    (Gauss/3M: T1 = Ar Br, T2 = Ai Bi, T3 = (Ar+Ai)(Br+Bi)), the sums formed
+//    in registers from the two staged planes.
+constexpr int kGroupM = 8;
+
+template <typename T, int WM, int WN, int BK, bool M3>
 __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::type *__restrict__ A, int64_t lda,
                                                        const typename Cx<T>::type *__restrict__ B, int64_t ldb,
                                                        typename Cx<T>::type *__restrict__ Cm, int64_t ldc, int M,
-                                                       int N, int K) {
+                                                       int N, int K, int gm, int gn) {
   using C2 = typename Cx<T>::type;
   using acc4 = typename Cx<T>::acc4;
-  constexpr int BM = 32 * WM, BN = 32 * WN, BK = 16;
-  constexpr int PA = BM + 16 / (int)(sizeof(T) / 4), PB = BN + 16 / (int)(sizeof(T) / 4);
-  constexpr int NP = M3 ? 3 : 2;  // planes: re, im (, re+im)
-  __shared__ T As[NP][BK][PA];
-  __shared__ T Bs[NP][BK][PB];
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int PA = BM + 16, PB = BN + 16;
+  constexpr int NP = 2;  // planes: re, im (3M forms re+im in registers)
+  constexpr int LA = BM * BK / 256, LB = BN * BK / 256;  // complex loads per thread
+  static_assert(LA * 256 == BM * BK && LB * 256 == BN * BK, "tile");
+  __shared__ T As[2][NP][BK][PA];
+  __shared__ T Bs[2][NP][BK][PB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, kq = lane >> 4;
   const int wr = (wave & 1) * 16 * WM, wc = (wave >> 1) * 16 * WN;
-  const int64_t row0 = (int64_t)blockIdx.x * BM, col0 = (int64_t)blockIdx.y * BN;
+  // grouped tile order
+  const int pid = blockIdx.x;
+  const int per_group = kGroupM * gn;
+  const int first_m = (pid / per_group) * kGroupM;
+  const int gsize = min(gm - first_m, kGroupM);
+  const int pm = first_m + (pid % per_group) % gsize;
+  const int pn = (pid % per_group) / gsize;
+  const int64_t row0 = (int64_t)pm * BM, col0 = (int64_t)pn * BN;
+
   acc4 acc0[WM][WN], acc1[WM][WN], acc2[WM][WN];
+#pragma unroll
   for (int i = 0; i < WM; i++)
+#pragma unroll
     for (int j = 0; j < WN; j++)
+#pragma unroll
       for (int v = 0; v < 4; v++) {
         acc0[i][j][v] = 0;
         acc1[i][j][v] = 0;
-        acc2[i][j][v] = 0;
+        if (M3) acc2[i][j][v] = 0;
       }
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    for (int idx = tid; idx < BM * BK; idx += 256) {
+  C2 ra[LA], rb[LB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < LA; u++) {
+      const int idx = tid + 256 * u;
       const int r = idx % BM, k = idx / BM;
       const int64_t gr = row0 + r;
       const int gk = k0 + k;
-      C2 v;
-      v.x = 0;
-      v.y = 0;
-      if (gr < M && gk < K) v = A[gr + (int64_t)gk * lda];
-      As[0][k][r] = v.x;
-      As[1][k][r] = v.y;
-      if (M3) As[NP - 1][k][r] = v.x + v.y;
+      ra[u].x = 0;
+      ra[u].y = 0;
+      if (gr < M && gk < K) ra[u] = A[gr + (int64_t)gk * lda];
     }
-    for (int idx = tid; idx < BN * BK; idx += 256) {
+#pragma unroll
+    for (int u = 0; u < LB; u++) {
+      const int idx = tid + 256 * u;
       const int k = idx % BK, c = idx / BK;
       const int64_t gc = col0 + c;
       const int gk = k0 + k;
-      C2 v;
-      v.x = 0;
-      v.y = 0;
-      if (gc < N && gk < K) v = B[gk + gc * ldb];
-      Bs[0][k][c] = v.x;
-      Bs[1][k][c] = v.y;
-      if (M3) Bs[NP - 1][k][c] = v.x + v.y;
+      rb[u].x = 0;
+      rb[u].y = 0;
+      if (gc < N && gk < K) rb[u] = B[gk + gc * ldb];
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < LA; u++) {
+      const int idx = tid + 256 * u;
+      const int r = idx % BM, k = idx / BM;
+      As[buf][0][k][r] = ra[u].x;
+      As[buf][1][k][r] = ra[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < LB; u++) {
+      const int idx = tid + 256 * u;
+      const int k = idx % BK, c = idx / BK;
+      Bs[buf][0][k][c] = rb[u].x;
+      Bs[buf][1][k][c] = rb[u].y;
+    }
+  };
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    const bool more = k0 + BK < K;
+    if (more) fetch(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T ar[WM], ai[WM], as_[WM], br[WN], bi[WN], bs[WN];
 #pragma unroll
       for (int i = 0; i < WM; i++) {
-        ar[i] = As[0][kk + kq][wr + 16 * i + l16];
-        ai[i] = As[1][kk + kq][wr + 16 * i + l16];
-        if (M3) as_[i] = As[NP - 1][kk + kq][wr + 16 * i + l16];
+        ar[i] = As[buf][0][kk + kq][wr + 16 * i + l16];
+        ai[i] = As[buf][1][kk + kq][wr + 16 * i + l16];
+        if (M3) as_[i] = ar[i] + ai[i];
       }
 #pragma unroll
       for (int j = 0; j < WN; j++) {
-        br[j] = Bs[0][kk + kq][wc + 16 * j + l16];
-        bi[j] = Bs[1][kk + kq][wc + 16 * j + l16];
-        if (M3) bs[j] = Bs[NP - 1][kk + kq][wc + 16 * j + l16];
+        br[j] = Bs[buf][0][kk + kq][wc + 16 * j + l16];
+        bi[j] = Bs[buf][1][kk + kq][wc + 16 * j + l16];
+        if (M3) bs[j] = br[j] + bi[j];
       }
 #pragma unroll
       for (int i = 0; i < WM; i++) {
@@ -151,19 +193,21 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
 #pragma unroll
         for (int j = 0; j < WN; j++) {
           if (M3) {
-            acc0[i][j] = mfma4<T>(ar[i], br[j], acc0[i][j]);   // T1
-            acc1[i][j] = mfma4<T>(ai[i], bi[j], acc1[i][j]);   // T2
-            acc2[i][j] = mfma4<T>(as_[i], bs[j], acc2[i][j]);  // T3
+            acc0[i][j] = mfma4<T>(ar[i], br[j], acc0[i][j]);
+            acc1[i][j] = mfma4<T>(ai[i], bi[j], acc1[i][j]);
+            acc2[i][j] = mfma4<T>(as_[i], bs[j], acc2[i][j]);
           } else {
-            acc0[i][j] = mfma4<T>(ar[i], br[j], acc0[i][j]);   // Re
+            acc0[i][j] = mfma4<T>(ar[i], br[j], acc0[i][j]);
             acc0[i][j] = mfma4<T>(nai, bi[j], acc0[i][j]);
-            acc1[i][j] = mfma4<T>(ar[i], bi[j], acc1[i][j]);   // Im
+            acc1[i][j] = mfma4<T>(ar[i], bi[j], acc1[i][j]);
             acc1[i][j] = mfma4<T>(ai[i], br[j], acc1[i][j]);
           }
         }
       }
     }
+    if (more) stash(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int i = 0; i < WM; i++)
@@ -190,9 +234,28 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
     }
 }
 
+// Tile configuration (wave tile 16 WM x 16 WN complex, K chunk BK); the
+// MOM_GEMM_* macros let a build choose another configuration.
+#ifndef MOM_GEMM_F32_WM
+#define MOM_GEMM_F32_WM 2
+#define MOM_GEMM_F32_WN 2
+#define MOM_GEMM_F32_BK 16
+#define MOM_GEMM_F32_M3 true
+#endif
+#ifndef MOM_GEMM_F64_WM
+#define MOM_GEMM_F64_WM 2
+#define MOM_GEMM_F64_WN 2
+#define MOM_GEMM_F64_BK 8
+#endif
 template <typename T> struct GemmCfg;
-template <> struct GemmCfg<float> { static constexpr int WM = 2, WN = 2; static constexpr bool M3 = false; };
-template <> struct GemmCfg<double> { static constexpr int WM = 1, WN = 2; static constexpr bool M3 = false; };
+template <> struct GemmCfg<float> {
+  static constexpr int WM = MOM_GEMM_F32_WM, WN = MOM_GEMM_F32_WN, BK = MOM_GEMM_F32_BK;
+  static constexpr bool M3 = MOM_GEMM_F32_M3;
+};
+template <> struct GemmCfg<double> {
+  static constexpr int WM = MOM_GEMM_F64_WM, WN = MOM_GEMM_F64_WN, BK = MOM_GEMM_F64_BK;
+  static constexpr bool M3 = false;
+};
 
 // Event-pair timing of the trailing-update kernel launches (enabled per
 // system); events are read back after the factorisation, so timing does
@@ -249,12 +312,12 @@ static int gemm_sub(hipStream_t s, const typename Cx<T>::type *A, int64_t lda, c
                     GemmStats *st = nullptr) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
-  dim3 grid((unsigned)((M + 32 * WM - 1) / (32 * WM)), (unsigned)((N + 32 * WN - 1) / (32 * WN)));
+  const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
   const bool timed = st && st->enabled;
   int rc;
   if (timed && (rc = st->begin(s))) return rc;
-  hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::M3>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc,
-                     (int)M, (int)N, (int)K);
+  hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn), dim3(256),
+                     0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn);
   MOM_HIP(hipGetLastError());
   if (timed && (rc = st->end(s, 8.0 * (double)M * (double)N * (double)K))) return rc;
   return 0;
@@ -395,6 +458,128 @@ __global__ __launch_bounds__(256) void column_update_kernel(typename Cx<T>::type
   }
 }
 
+// One column step of the inner-block panel factorisation, fused with the
+// pivot search of the next column: rows j+1..n-1 are scaled by 1/pivot
+// (column j) and get the rank-1 update of columns j+1..jend-1; while column
+// j+1 is updated each block records its largest |re|+|im| (lowest row on
+// ties).  With fuse_next, the last block to finish (atomic ticket, no
+// waiting) reduces the block results, records ipiv[j+1] and swaps rows j+1
+// and the pivot row across the panel columns [pc0, pc1).
+template <typename T>
+__global__ __launch_bounds__(256) void column_step_kernel(typename Cx<T>::type *A, int64_t ld, int j, int n,
+                                                           int jend, int pc0, int pc1, int *ipiv, PivotWork w,
+                                                           int fuse_next) {
+  using C2 = typename Cx<T>::type;
+  __shared__ double sval[256];
+  __shared__ int sidx[256];
+  __shared__ bool last;
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)j + 1 + blockIdx.x * 256 + tid;
+  double best = -1.0;
+  int bidx = n;
+  if (i < n) {
+    C2 *colj = A + (int64_t)j * ld;
+    const C2 piv = colj[j];
+    C2 l = colj[i];
+    if (piv.x != 0 || piv.y != 0) {
+      // l / piv (Smith's formula)
+      const T a = piv.x, b = piv.y;
+      C2 q;
+      if (fabs(a) >= fabs(b)) {
+        const T r = b / a, den = a + b * r;
+        q.x = (l.x + l.y * r) / den;
+        q.y = (l.y - l.x * r) / den;
+      } else {
+        const T r = a / b, den = a * r + b;
+        q.x = (l.x * r + l.y) / den;
+        q.y = (l.y * r - l.x) / den;
+      }
+      l = q;
+      colj[i] = l;
+    }
+    for (int c = j + 1; c < jend; c++) {
+      C2 *cc = A + (int64_t)c * ld;
+      const C2 u = cc[j];
+      C2 v = cc[i];
+      v.x -= l.x * u.x - l.y * u.y;
+      v.y -= l.x * u.y + l.y * u.x;
+      cc[i] = v;
+      if (c == j + 1) {
+        best = (double)cabs1<T>(v);
+        bidx = (int)i;
+      }
+    }
+  }
+  if (!fuse_next) return;
+  // every wave publishes its updated rows at device scope before the block
+  // takes its ticket: the last block swaps rows written by other blocks
+  __threadfence();
+  sval[tid] = best;
+  sidx[tid] = bidx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double v2 = sval[tid + s];
+      const int i2 = sidx[tid + s];
+      if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
+        sval[tid] = v2;
+        sidx[tid] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    w.part_val[blockIdx.x] = sval[0];
+    w.part_idx[blockIdx.x] = sidx[0];
+    __threadfence();
+    const int t = atomicAdd(w.counter, 1);
+    last = (t == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double bv = -1.0;
+  int bi = n;
+  for (int b = tid; b < (int)gridDim.x; b += 256) {
+    const double v2 = __hip_atomic_load(&w.part_val[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int i2 = __hip_atomic_load(&w.part_idx[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v2 > bv || (v2 == bv && i2 < bi)) {
+      bv = v2;
+      bi = i2;
+    }
+  }
+  sval[tid] = bv;
+  sidx[tid] = bi;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) {
+      const double v2 = sval[tid + s];
+      const int i2 = sidx[tid + s];
+      if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
+        sval[tid] = v2;
+        sidx[tid] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  const int jn = j + 1;
+  const int p = (sidx[0] < n) ? sidx[0] : jn;
+  const bool zero = !(sval[0] > 0.0);
+  if (tid == 0) {
+    ipiv[jn] = p + 1;
+    *w.counter = 0;
+    if (zero) atomicCAS(w.info, 0, jn + 1);
+  }
+  if (p != jn) {
+    for (int c = pc0 + tid; c < pc1; c += 256) {
+      C2 *cc = A + (int64_t)c * ld;
+      const C2 a = cc[jn], b = cc[p];
+      cc[jn] = b;
+      cc[p] = a;
+    }
+  }
+}
+
 // Row interchanges k1..k2-1 (ipiv 1-based) on columns [c0, c1) and [c2, c3).
 template <typename T>
 __global__ void laswp_kernel(typename Cx<T>::type *A, int64_t ld, int k1, int k2, const int *ipiv, int c0, int c1,
@@ -484,17 +669,21 @@ static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, 
     const int64_t jb = std::min<int64_t>(NB, n - j0);
     for (int64_t i0 = j0; i0 < j0 + jb; i0 += IB) {
       const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
-      for (int64_t j = i0; j < i0 + ib; j++) {
-        const int64_t rows = n - j;
+      // pivot of the block's first column, then one fused step per column
+      {
+        const int64_t rows = n - i0;
         const int nblk = (int)std::min<int64_t>(kPivBlocks, (rows + 255) / 256);
-        hipLaunchKernelGGL(pivot_kernel<T>, dim3(nblk), dim3(256), 0, s, A, ld, (int)j, (int)n, (int)j0,
+        hipLaunchKernelGGL(pivot_kernel<T>, dim3(nblk), dim3(256), 0, s, A, ld, (int)i0, (int)n, (int)j0,
                            (int)(j0 + jb), ipiv, w);
         MOM_HIP(hipGetLastError());
-        if (rows > 1) {
-          hipLaunchKernelGGL(column_update_kernel<T>, dim3((unsigned)((rows - 1 + 255) / 256)), dim3(256), 0, s,
-                             A, ld, (int)j, (int)n, (int)(i0 + ib));
-          MOM_HIP(hipGetLastError());
-        }
+      }
+      for (int64_t j = i0; j < i0 + ib; j++) {
+        const int64_t rows = n - j - 1;
+        if (rows <= 0) break;
+        const int fuse = (j + 1 < i0 + ib) ? 1 : 0;
+        hipLaunchKernelGGL(column_step_kernel<T>, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, A, ld,
+                           (int)j, (int)n, (int)(i0 + ib), (int)j0, (int)(j0 + jb), ipiv, w, fuse);
+        MOM_HIP(hipGetLastError());
       }
       const int64_t rest = j0 + jb - (i0 + ib);
       if (rest > 0) {
@@ -856,15 +1045,16 @@ struct mom_system {
 namespace mom {
 
 static int ensure_pivwork(mom_system *S, PivotWork *w) {
-  const size_t bytes = kPivBlocks * (sizeof(double) + sizeof(int)) + 64;
+  const int64_t nb = std::max<int64_t>(kPivBlocks, (S->n + 255) / 256);
+  const size_t bytes = nb * (sizeof(double) + sizeof(int)) + 64;
   if (S->pivwork.bytes < bytes) {
     int rc = S->pivwork.alloc(bytes);
     if (rc) return rc;
   }
   char *base = S->pivwork.as<char>();
   w->part_val = (double *)base;
-  w->part_idx = (int *)(base + kPivBlocks * sizeof(double));
-  w->counter = (int *)(base + kPivBlocks * (sizeof(double) + sizeof(int)));
+  w->part_idx = (int *)(base + nb * sizeof(double));
+  w->counter = (int *)(base + nb * (sizeof(double) + sizeof(int)));
   w->info = w->counter + 1;
   MOM_HIP(hipMemsetAsync(w->counter, 0, 2 * sizeof(int), S->ctx->stream));
   return 0;
