@@ -6,16 +6,18 @@
 //
 // Factorisation: blocked right-looking LU (outer block NB = 256) whose
 // trailing update A22 -= L21 U12 runs on the matrix cores
-// (v_mfma_f64_16x16x4f64 for complex128, v_mfma_f32_16x16x4f32 for
-// complex64; a complex product is four real MFMAs).  The panel is factored
-// in inner blocks of IB = 32 columns: per column one kernel finds the pivot
-// (|re|+|im| with the lowest index on ties, as LAPACK's IZAMAX) and swaps
-// the two rows across the panel, a second scales the column and applies
-// the rank-1 update to the rest of the inner block; the remaining panel
-// columns then get a small triangular solve and an MFMA update.  Row
-// interchanges reach the columns left and right of the panel through a
-// laswp kernel, so the factors are in ZGETRF layout (unit L below the
-// diagonal, U on and above, 1-based ipiv).
+// (v_mfma_f64_16x16x4f64 for complex128 with four real products per complex
+// product; v_mfma_f32_16x16x4f32 for complex64 with three, Gauss's form).
+// The panel is factored in inner blocks of IB = 32 columns with one launch
+// per column: every block reduces the previous launch's per-block pivot
+// candidates (|re|+|im|, lowest index on ties, as LAPACK's IZAMAX), the
+// block holding the pivot row swaps it with the diagonal row across the
+// panel, every row below is scaled and updated in registers, and each block
+// leaves its candidate for the next column.  The remaining panel columns
+// then get a small triangular solve and an MFMA update.  Row interchanges
+// reach the columns left and right of the panel through a laswp kernel, so
+// the factors are in ZGETRF layout (unit L below the diagonal, U on and
+// above, 1-based ipiv).
 //
 // Solves: the row permutation is applied as one gather; forward and back
 // substitution work block by block with precomputed inverses of the NB x NB
@@ -325,258 +327,199 @@ static int gemm_sub(hipStream_t s, const typename Cx<T>::type *A, int64_t lda, c
 
 // ---------------------------------------------------------------------------
 // Panel kernels.
+// Pivot candidates of one panel column, one per block of 256 rows: the
+// block's largest |re|+|im| (lowest row on ties) and that row's values in
+// the inner block's columns.  Two parities: the kernel for column j reads
+// parity j&1 and writes the candidates of column j+1 into the other one, so
+// blocks exchange pivot data only across kernel boundaries (no device-scope
+// fences inside a kernel).
 struct PivotWork {
-  double *part_val;  // per block
-  int *part_idx;
-  int *counter;
-  int *info;         // first zero pivot (1-based), 0 if none
+  double *val;   // [2][mb]
+  int *idx;      // [2][mb]
+  void *row;     // [2][mb][32] complex of the factor's type
+  int mb;        // slots per parity (>= blocks of the widest launch)
+  int *info;     // first zero pivot (1-based), 0 if none
 };
 
-constexpr int kPivBlocks = 64;
 
 template <typename T>
 __device__ __forceinline__ T cabs1(typename Cx<T>::type v) {
   return fabs(v.x) + fabs(v.y);
 }
 
-// Pivot search on column j over rows j..n-1 and row swap across the panel
-// columns [pc0, pc1).  The last block to finish reduces the block results.
+__device__ __forceinline__ void argmax_merge(double &v, int &i, int &w, double v2, int i2, int w2) {
+  if (v2 > v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
+    w = w2;
+  }
+}
+
+// Block-wide argmax of (value, row, tag) over 256 threads; every thread
+// receives the result.  The order (value descending, row ascending) is
+// total, so the result does not depend on the reduction tree.
+__device__ __forceinline__ void block_argmax(double &v, int &i, int &w, double *sv, int *si, int *sw) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double v2 = __shfl_xor(v, o);
+    const int i2 = __shfl_xor(i, o);
+    const int w2 = __shfl_xor(w, o);
+    argmax_merge(v, i, w, v2, i2, w2);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    sv[wv] = v;
+    si[wv] = i;
+    sw[wv] = w;
+  }
+  __syncthreads();
+  v = sv[0];
+  i = si[0];
+  w = sw[0];
+#pragma unroll
+  for (int k = 1; k < 4; k++) argmax_merge(v, i, w, sv[k], si[k], sw[k]);
+  __syncthreads();
+}
+
+// Candidates of column `col` over rows col..n-1 (block b: rows col+256b ..),
+// with the candidate row's values in columns [col, cend) stored at the
+// inner-block offsets c - i0.
 template <typename T>
-__global__ __launch_bounds__(256) void pivot_kernel(typename Cx<T>::type *A, int64_t ld, int j, int n, int pc0,
-                                                     int pc1, int *ipiv, PivotWork w) {
+__global__ __launch_bounds__(256) void pivot_candidates_kernel(const typename Cx<T>::type *A, int64_t ld, int col,
+                                                                int n, int i0, int cend, PivotWork w, int par) {
   using C2 = typename Cx<T>::type;
-  __shared__ double sval[256];
-  __shared__ int sidx[256];
-  __shared__ bool last;
+  __shared__ double sv[4];
+  __shared__ int si[4], sw[4];
   const int tid = threadIdx.x;
-  double best = -1.0;
-  int bidx = n;
-  const C2 *col = A + (int64_t)j * ld;
-  for (int i = j + blockIdx.x * 256 + tid; i < n; i += gridDim.x * 256) {
-    const double v = (double)cabs1<T>(col[i]);
-    if (v > best) {
-      best = v;
-      bidx = i;
-    }
+  const int64_t r = (int64_t)col + (int64_t)blockIdx.x * 256 + tid;
+  double v = -1.0;
+  int idx = n, win = tid;
+  if (r < n) {
+    v = (double)cabs1<T>(A[(int64_t)col * ld + r]);
+    idx = (int)r;
   }
-  sval[tid] = best;
-  sidx[tid] = bidx;
+  block_argmax(v, idx, win, sv, si, sw);
+  if (tid == win && idx < n) {
+    const size_t slot = (size_t)par * w.mb + blockIdx.x;
+    w.val[slot] = v;
+    w.idx[slot] = idx;
+    C2 *row = (C2 *)w.row + slot * IB;
+    for (int c = col; c < cend; c++) row[c - i0] = A[(int64_t)c * ld + r];
+  }
+}
+
+// One column step j of the inner block [i0, jend) of the panel [pc0, pc1):
+//  - every block reduces the candidates of column j (parity j&1) to the
+//    pivot row p and takes the pivot row's values from the candidate slot;
+//  - the block holding row p moves row j's old values into row p and swaps
+//    rows j and p in the panel columns outside the inner block, and writes
+//    the pivot row into row j;
+//  - every row i > j is scaled (l = a_ij / pivot) and gets the rank-1
+//    update of columns j+1..jend-1 in registers;
+//  - with fuse_next, each block stores its candidate for column j+1.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_column_kernel(typename Cx<T>::type *A, int64_t ld, int j, int n, int i0,
+                                                            int jend, int pc0, int pc1, int *ipiv, PivotWork w,
+                                                            int nb_in, int fuse_next) {
+  using C2 = typename Cx<T>::type;
+  __shared__ C2 u[IB];
+  __shared__ double sv[4];
+  __shared__ int si[4], sw[4];
+  const int tid = threadIdx.x, b = blockIdx.x;
+  const int64_t i = (int64_t)j + 1 + (int64_t)b * 256 + tid;
+  const bool own = i < n;
+  const int k = j - i0, kend = jend - i0;
+  C2 v[IB];
+#pragma unroll
+  for (int s = 0; s < IB; s++) {
+    v[s].x = 0;
+    v[s].y = 0;
+    if (own && s >= k && s < kend) v[s] = A[(int64_t)(i0 + s) * ld + i];
+  }
+  // pivot of column j from the block candidates
+  const int par = j & 1;
+  double bv = -1.0;
+  int bi = n, bw = 0;
+  for (int t = tid; t < nb_in; t += 256)
+    argmax_merge(bv, bi, bw, w.val[(size_t)par * w.mb + t], w.idx[(size_t)par * w.mb + t], t);
+  block_argmax(bv, bi, bw, sv, si, sw);
+  const int p = (bi < n) ? bi : j;
+  const bool zero = !(bv > 0.0);
+  const C2 *cand = (const C2 *)w.row + ((size_t)par * w.mb + bw) * IB;
+  if (tid < kend - k) u[k + tid] = (bi < n) ? cand[k + tid] : A[(int64_t)(j + tid) * ld + j];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) {
-      const double v2 = sval[tid + s];
-      const int i2 = sidx[tid + s];
-      if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
-        sval[tid] = v2;
-        sidx[tid] = i2;
-      }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    w.part_val[blockIdx.x] = sval[0];
-    w.part_idx[blockIdx.x] = sidx[0];
-    __threadfence();
-    const int t = atomicAdd(w.counter, 1);
-    last = (t == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (tid < (int)gridDim.x) {
-    sval[tid] = __hip_atomic_load(&w.part_val[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sidx[tid] = __hip_atomic_load(&w.part_idx[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    sval[tid] = -1.0;
-    sidx[tid] = n;
-  }
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) {
-      const double v2 = sval[tid + s];
-      const int i2 = sidx[tid + s];
-      if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
-        sval[tid] = v2;
-        sidx[tid] = i2;
-      }
-    }
-    __syncthreads();
-  }
-  const int p = (sidx[0] < n) ? sidx[0] : j;
-  const bool zero = !(sval[0] > 0.0);
-  if (tid == 0) {
+  if (b == 0 && tid == 0) {
     ipiv[j] = p + 1;
-    *w.counter = 0;
     if (zero) atomicCAS(w.info, 0, j + 1);
   }
-  if (p != j) {
+  if (p > j && b == (int)((p - j - 1) / 256)) {
+    if (i == p) {
+#pragma unroll
+      for (int s = 0; s < IB; s++)
+        if (s >= k && s < kend) v[s] = A[(int64_t)(i0 + s) * ld + j];
+    }
     for (int c = pc0 + tid; c < pc1; c += 256) {
+      if (c >= j && c < jend) continue;
       C2 *cc = A + (int64_t)c * ld;
-      const C2 a = cc[j], b = cc[p];
-      cc[j] = b;
-      cc[p] = a;
+      const C2 x = cc[j], y = cc[p];
+      cc[j] = y;
+      cc[p] = x;
     }
+    __syncthreads();  // row j read by its new owner before it is overwritten
+    if (tid < kend - k) A[(int64_t)(j + tid) * ld + j] = u[k + tid];
   }
-}
-
-// Scale column j below the diagonal by 1/pivot and apply the rank-1 update
-// to columns j+1..jend-1 of the inner block.
-template <typename T>
-__global__ __launch_bounds__(256) void column_update_kernel(typename Cx<T>::type *A, int64_t ld, int j, int n,
-                                                             int jend) {
-  using C2 = typename Cx<T>::type;
-  const int64_t i = (int64_t)j + 1 + blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  C2 *colj = A + (int64_t)j * ld;
-  const C2 piv = colj[j];
-  C2 l = colj[i];
-  if (piv.x != 0 || piv.y != 0) {
-    // l / piv (Smith's formula keeps the scaling safe)
-    T a = piv.x, b = piv.y;
-    if (fabs(a) >= fabs(b)) {
-      const T r = b / a, den = a + b * r;
-      C2 q;
-      q.x = (l.x + l.y * r) / den;
-      q.y = (l.y - l.x * r) / den;
-      l = q;
-    } else {
-      const T r = a / b, den = a * r + b;
-      C2 q;
-      q.x = (l.x * r + l.y) / den;
-      q.y = (l.y * r - l.x) / den;
-      l = q;
-    }
-    colj[i] = l;
-  }
-  for (int c = j + 1; c < jend; c++) {
-    C2 *cc = A + (int64_t)c * ld;
-    const C2 u = cc[j];
-    C2 v = cc[i];
-    v.x -= l.x * u.x - l.y * u.y;
-    v.y -= l.x * u.y + l.y * u.x;
-    cc[i] = v;
-  }
-}
-
-// One column step of the inner-block panel factorisation, fused with the
-// pivot search of the next column: rows j+1..n-1 are scaled by 1/pivot
-// (column j) and get the rank-1 update of columns j+1..jend-1; while column
-// j+1 is updated each block records its largest |re|+|im| (lowest row on
-// ties).  With fuse_next, the last block to finish (atomic ticket, no
-// waiting) reduces the block results, records ipiv[j+1] and swaps rows j+1
-// and the pivot row across the panel columns [pc0, pc1).
-template <typename T>
-__global__ __launch_bounds__(256) void column_step_kernel(typename Cx<T>::type *A, int64_t ld, int j, int n,
-                                                           int jend, int pc0, int pc1, int *ipiv, PivotWork w,
-                                                           int fuse_next) {
-  using C2 = typename Cx<T>::type;
-  __shared__ double sval[256];
-  __shared__ int sidx[256];
-  __shared__ bool last;
-  const int tid = threadIdx.x;
-  const int64_t i = (int64_t)j + 1 + blockIdx.x * 256 + tid;
-  double best = -1.0;
-  int bidx = n;
-  if (i < n) {
-    C2 *colj = A + (int64_t)j * ld;
-    const C2 piv = colj[j];
-    C2 l = colj[i];
+  if (own) {
+    C2 l;
+    l.x = 0;
+    l.y = 0;
+#pragma unroll
+    for (int s = 0; s < IB; s++)
+      if (s == k) l = v[s];
+    const C2 piv = u[k];
     if (piv.x != 0 || piv.y != 0) {
       // l / piv (Smith's formula)
-      const T a = piv.x, b = piv.y;
+      const T a = piv.x, bb = piv.y;
       C2 q;
-      if (fabs(a) >= fabs(b)) {
-        const T r = b / a, den = a + b * r;
+      if (fabs(a) >= fabs(bb)) {
+        const T r = bb / a, den = a + bb * r;
         q.x = (l.x + l.y * r) / den;
         q.y = (l.y - l.x * r) / den;
       } else {
-        const T r = a / b, den = a * r + b;
+        const T r = a / bb, den = a * r + bb;
         q.x = (l.x * r + l.y) / den;
         q.y = (l.y * r - l.x) / den;
       }
       l = q;
-      colj[i] = l;
     }
-    for (int c = j + 1; c < jend; c++) {
-      C2 *cc = A + (int64_t)c * ld;
-      const C2 u = cc[j];
-      C2 v = cc[i];
-      v.x -= l.x * u.x - l.y * u.y;
-      v.y -= l.x * u.y + l.y * u.x;
-      cc[i] = v;
-      if (c == j + 1) {
-        best = (double)cabs1<T>(v);
-        bidx = (int)i;
+#pragma unroll
+    for (int s = 0; s < IB; s++) {
+      if (s == k) v[s] = l;
+      if (s > k && s < kend) {
+        const C2 uu = u[s];
+        v[s].x -= l.x * uu.x - l.y * uu.y;
+        v[s].y -= l.x * uu.y + l.y * uu.x;
       }
+      if (s >= k && s < kend) A[(int64_t)(i0 + s) * ld + i] = v[s];
     }
   }
   if (!fuse_next) return;
-  // every wave publishes its updated rows at device scope before the block
-  // takes its ticket: the last block swaps rows written by other blocks
-  __threadfence();
-  sval[tid] = best;
-  sidx[tid] = bidx;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) {
-      const double v2 = sval[tid + s];
-      const int i2 = sidx[tid + s];
-      if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
-        sval[tid] = v2;
-        sidx[tid] = i2;
-      }
-    }
-    __syncthreads();
+  double cv = -1.0;
+  int ci = n, cw = tid;
+  if (own) {
+#pragma unroll
+    for (int s = 0; s < IB; s++)
+      if (s == k + 1) cv = (double)cabs1<T>(v[s]);
+    ci = (int)i;
   }
-  if (tid == 0) {
-    w.part_val[blockIdx.x] = sval[0];
-    w.part_idx[blockIdx.x] = sidx[0];
-    __threadfence();
-    const int t = atomicAdd(w.counter, 1);
-    last = (t == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double bv = -1.0;
-  int bi = n;
-  for (int b = tid; b < (int)gridDim.x; b += 256) {
-    const double v2 = __hip_atomic_load(&w.part_val[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int i2 = __hip_atomic_load(&w.part_idx[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (v2 > bv || (v2 == bv && i2 < bi)) {
-      bv = v2;
-      bi = i2;
-    }
-  }
-  sval[tid] = bv;
-  sidx[tid] = bi;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) {
-      const double v2 = sval[tid + s];
-      const int i2 = sidx[tid + s];
-      if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
-        sval[tid] = v2;
-        sidx[tid] = i2;
-      }
-    }
-    __syncthreads();
-  }
-  const int jn = j + 1;
-  const int p = (sidx[0] < n) ? sidx[0] : jn;
-  const bool zero = !(sval[0] > 0.0);
-  if (tid == 0) {
-    ipiv[jn] = p + 1;
-    *w.counter = 0;
-    if (zero) atomicCAS(w.info, 0, jn + 1);
-  }
-  if (p != jn) {
-    for (int c = pc0 + tid; c < pc1; c += 256) {
-      C2 *cc = A + (int64_t)c * ld;
-      const C2 a = cc[jn], b = cc[p];
-      cc[jn] = b;
-      cc[p] = a;
-    }
+  block_argmax(cv, ci, cw, sv, si, sw);
+  if (tid == cw && ci < n) {
+    const size_t slot = (size_t)(par ^ 1) * w.mb + b;
+    w.val[slot] = cv;
+    w.idx[slot] = ci;
+    C2 *row = (C2 *)w.row + slot * IB;
+#pragma unroll
+    for (int s = 0; s < IB; s++)
+      if (s > k && s < kend) row[s] = v[s];
   }
 }
 
@@ -669,21 +612,19 @@ static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, 
     const int64_t jb = std::min<int64_t>(NB, n - j0);
     for (int64_t i0 = j0; i0 < j0 + jb; i0 += IB) {
       const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
-      // pivot of the block's first column, then one fused step per column
-      {
-        const int64_t rows = n - i0;
-        const int nblk = (int)std::min<int64_t>(kPivBlocks, (rows + 255) / 256);
-        hipLaunchKernelGGL(pivot_kernel<T>, dim3(nblk), dim3(256), 0, s, A, ld, (int)i0, (int)n, (int)j0,
-                           (int)(j0 + jb), ipiv, w);
-        MOM_HIP(hipGetLastError());
-      }
+      // candidates of the block's first column, then one launch per column
+      hipLaunchKernelGGL(pivot_candidates_kernel<T>, dim3((unsigned)((n - i0 + 255) / 256)), dim3(256), 0, s, A, ld,
+                         (int)i0, (int)n, (int)i0, (int)(i0 + ib), w, (int)(i0 & 1));
+      MOM_HIP(hipGetLastError());
+      int nb_in = (int)((n - i0 + 255) / 256);
       for (int64_t j = i0; j < i0 + ib; j++) {
         const int64_t rows = n - j - 1;
-        if (rows <= 0) break;
+        const int nblk = (int)std::max<int64_t>(1, (rows + 255) / 256);
         const int fuse = (j + 1 < i0 + ib) ? 1 : 0;
-        hipLaunchKernelGGL(column_step_kernel<T>, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, A, ld,
-                           (int)j, (int)n, (int)(i0 + ib), (int)j0, (int)(j0 + jb), ipiv, w, fuse);
+        hipLaunchKernelGGL(panel_column_kernel<T>, dim3((unsigned)nblk), dim3(256), 0, s, A, ld, (int)j, (int)n,
+                           (int)i0, (int)(i0 + ib), (int)j0, (int)(j0 + jb), ipiv, w, nb_in, fuse);
         MOM_HIP(hipGetLastError());
+        nb_in = nblk;
       }
       const int64_t rest = j0 + jb - (i0 + ib);
       if (rest > 0) {
@@ -1045,18 +986,20 @@ struct mom_system {
 namespace mom {
 
 static int ensure_pivwork(mom_system *S, PivotWork *w) {
-  const int64_t nb = std::max<int64_t>(kPivBlocks, (S->n + 255) / 256);
-  const size_t bytes = nb * (sizeof(double) + sizeof(int)) + 64;
+  const int64_t mb = (S->n + 255) / 256 + 1;
+  const size_t rowbytes = 2 * mb * IB * sizeof(double2);
+  const size_t bytes = rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64;
   if (S->pivwork.bytes < bytes) {
     int rc = S->pivwork.alloc(bytes);
     if (rc) return rc;
   }
   char *base = S->pivwork.as<char>();
-  w->part_val = (double *)base;
-  w->part_idx = (int *)(base + nb * sizeof(double));
-  w->counter = (int *)(base + nb * (sizeof(double) + sizeof(int)));
-  w->info = w->counter + 1;
-  MOM_HIP(hipMemsetAsync(w->counter, 0, 2 * sizeof(int), S->ctx->stream));
+  w->row = base;
+  w->val = (double *)(base + rowbytes);
+  w->idx = (int *)(base + rowbytes + 2 * mb * sizeof(double));
+  w->info = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)));
+  w->mb = (int)mb;
+  MOM_HIP(hipMemsetAsync(w->info, 0, sizeof(int), S->ctx->stream));
   return 0;
 }
 
