@@ -39,12 +39,110 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+constexpr size_t kStageBytes = size_t(16) << 20;
+
+static int stage_init(mom_ctx *ctx) {
+  for (int k = 0; k < 2; k++) {
+    if (!ctx->stage[k]) MOM_HIP(hipHostMalloc(&ctx->stage[k], kStageBytes, hipHostMallocDefault));
+    if (!ctx->stage_ev[k]) MOM_HIP(hipEventCreateWithFlags(&ctx->stage_ev[k], hipEventDisableTiming));
+  }
+  return 0;
+}
+
+int copy_h2d_2d(mom_ctx *ctx, void *dev, size_t dpitch, const void *host, size_t hpitch, size_t width,
+                size_t rows) {
+  if (!width || !rows) return 0;
+  int rc = stage_init(ctx);
+  if (rc) return rc;
+  const char *h = static_cast<const char *>(host);
+  char *d = static_cast<char *>(dev);
+  if (width > kStageBytes) {  // long runs: one run at a time in stage-sized pieces
+    for (size_t r = 0; r < rows; r++)
+      if ((rc = copy_h2d(ctx, d + r * dpitch, h + r * hpitch, width))) return rc;
+    return 0;
+  }
+  const size_t per = kStageBytes / width;
+  int k = 0;
+  for (size_t r0 = 0; r0 < rows; r0 += per, k ^= 1) {
+    const size_t nr = std::min(per, rows - r0);
+    MOM_HIP(hipEventSynchronize(ctx->stage_ev[k]));
+    char *st = static_cast<char *>(ctx->stage[k]);
+    for (size_t r = 0; r < nr; r++) std::memcpy(st + r * width, h + (r0 + r) * hpitch, width);
+    MOM_HIP(hipMemcpy2DAsync(d + r0 * dpitch, dpitch, st, width, width, nr, hipMemcpyHostToDevice, ctx->stream));
+    MOM_HIP(hipEventRecord(ctx->stage_ev[k], ctx->stream));
+  }
+  return 0;
+}
+
+int copy_h2d(mom_ctx *ctx, void *dev, const void *host, size_t bytes) {
+  if (!bytes) return 0;
+  int rc = stage_init(ctx);
+  if (rc) return rc;
+  int k = 0;
+  for (size_t off = 0; off < bytes; off += kStageBytes, k ^= 1) {
+    const size_t len = std::min(kStageBytes, bytes - off);
+    MOM_HIP(hipEventSynchronize(ctx->stage_ev[k]));
+    std::memcpy(ctx->stage[k], static_cast<const char *>(host) + off, len);
+    MOM_HIP(hipMemcpyAsync(static_cast<char *>(dev) + off, ctx->stage[k], len, hipMemcpyHostToDevice, ctx->stream));
+    MOM_HIP(hipEventRecord(ctx->stage_ev[k], ctx->stream));
+  }
+  return 0;
+}
+
+int copy_d2h_2d(mom_ctx *ctx, void *host, size_t hpitch, const void *dev, size_t dpitch, size_t width,
+                size_t rows) {
+  if (!width || !rows) return 0;
+  int rc = stage_init(ctx);
+  if (rc) return rc;
+  char *h = static_cast<char *>(host);
+  const char *d = static_cast<const char *>(dev);
+  if (width > kStageBytes) {
+    for (size_t r = 0; r < rows; r++)
+      if ((rc = copy_d2h(ctx, h + r * hpitch, d + r * dpitch, width))) return rc;
+    return 0;
+  }
+  const size_t per = kStageBytes / width;
+  const size_t nchunk = (rows + per - 1) / per;
+  auto issue = [&](size_t c) -> int {
+    const int k = (int)(c & 1);
+    const size_t r0 = c * per, nr = std::min(per, rows - r0);
+    MOM_HIP(hipEventSynchronize(ctx->stage_ev[k]));
+    MOM_HIP(hipMemcpy2DAsync(ctx->stage[k], width, d + r0 * dpitch, dpitch, width, nr, hipMemcpyDeviceToHost,
+                             ctx->stream));
+    MOM_HIP(hipEventRecord(ctx->stage_ev[k], ctx->stream));
+    return 0;
+  };
+  if ((rc = issue(0))) return rc;
+  if (nchunk > 1 && (rc = issue(1))) return rc;
+  for (size_t c = 0; c < nchunk; c++) {
+    const int k = (int)(c & 1);
+    const size_t r0 = c * per, nr = std::min(per, rows - r0);
+    MOM_HIP(hipEventSynchronize(ctx->stage_ev[k]));
+    const char *st = static_cast<const char *>(ctx->stage[k]);
+    for (size_t r = 0; r < nr; r++) std::memcpy(h + (r0 + r) * hpitch, st + r * width, width);
+    if (c + 2 < nchunk && (rc = issue(c + 2))) return rc;
+  }
+  return 0;
+}
+
+int copy_d2h(mom_ctx *ctx, void *host, const void *dev, size_t bytes) {
+  if (!bytes) return 0;
+  const size_t rows = (bytes + kStageBytes - 1) / kStageBytes;
+  if (rows == 1) return copy_d2h_2d(ctx, host, bytes, dev, bytes, bytes, 1);
+  // whole stage-sized pieces, then the tail
+  const size_t whole = bytes / kStageBytes;
+  int rc = copy_d2h_2d(ctx, host, kStageBytes, dev, kStageBytes, kStageBytes, whole);
+  if (rc || bytes == whole * kStageBytes) return rc;
+  const size_t off = whole * kStageBytes;
+  return copy_d2h_2d(ctx, static_cast<char *>(host) + off, bytes - off, static_cast<const char *>(dev) + off,
+                     bytes - off, bytes - off, 1);
+}
+
 template <class T>
-static int upload(DevBuf &b, const T *src, size_t count, hipStream_t s) {
+static int upload(mom_ctx *ctx, DevBuf &b, const T *src, size_t count) {
   int rc = b.alloc(count * sizeof(T));
   if (rc) return rc;
-  if (count) MOM_HIP(hipMemcpyAsync(b.ptr, src, count * sizeof(T), hipMemcpyHostToDevice, s));
-  return 0;
+  return copy_h2d(ctx, b.ptr, src, count * sizeof(T));
 }
 
 int make_rule(int32_t nq, const double *qw, const double *qnodes, QuadRule *r) {
@@ -136,6 +234,10 @@ int mom_ctx_destroy(mom_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->qp.release();
   ctx->scratch.release();
+  for (int k = 0; k < 2; k++) {
+    if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
+    if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);
+  }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return 0;
@@ -226,14 +328,14 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
   hipStream_t s = ctx->stream;
   int rc = 0;
   (void)hipSetDevice(ctx->device);
-  if ((rc = upload(m->p, p, 3 * nnodes, s)) || (rc = upload(m->fnodes, fnodes, 3 * F, s)) ||
-      (rc = upload(m->fn, fn, 3 * F, s)) || (rc = upload(m->fm, fm, 9 * F, s)) ||
-      (rc = upload(m->fs, fs, 9 * F, s)) || (rc = upload(m->farea, farea, F, s)) ||
-      (rc = upload(m->fcp, fcp, 3 * F, s)) || (rc = upload(m->s_g, sg.data(), 3 * F, s)) ||
-      (rc = upload(m->s_l, sl.data(), 3 * F, s)) || (rc = upload(m->s_coef, scoef.data(), 3 * F, s)) ||
-      (rc = upload(m->s_div, sdiv.data(), 3 * F, s)) || (rc = upload(m->s_v, sv.data(), 9 * F, s)) ||
-      (rc = upload(m->near, np.data(), np.size(), s)) || (rc = upload(m->eside, eside.data(), eside.size(), s)) ||
-      (rc = upload(m->slab_edges, slab_edges.data(), slab_edges.size(), s))) {
+  if ((rc = upload(ctx, m->p, p, 3 * nnodes)) || (rc = upload(ctx, m->fnodes, fnodes, 3 * F)) ||
+      (rc = upload(ctx, m->fn, fn, 3 * F)) || (rc = upload(ctx, m->fm, fm, 9 * F)) ||
+      (rc = upload(ctx, m->fs, fs, 9 * F)) || (rc = upload(ctx, m->farea, farea, F)) ||
+      (rc = upload(ctx, m->fcp, fcp, 3 * F)) || (rc = upload(ctx, m->s_g, sg.data(), 3 * F)) ||
+      (rc = upload(ctx, m->s_l, sl.data(), 3 * F)) || (rc = upload(ctx, m->s_coef, scoef.data(), 3 * F)) ||
+      (rc = upload(ctx, m->s_div, sdiv.data(), 3 * F)) || (rc = upload(ctx, m->s_v, sv.data(), 9 * F)) ||
+      (rc = upload(ctx, m->near, np.data(), np.size())) || (rc = upload(ctx, m->eside, eside.data(), eside.size())) ||
+      (rc = upload(ctx, m->slab_edges, slab_edges.data(), slab_edges.size()))) {
     delete m;
     return rc;
   }
@@ -277,9 +379,7 @@ int mom_compute_operator(mom_ctx *ctx, const mom_mesh *mesh, char op, double ome
   t.ld = E;
   t.mode = mode;
   if ((rc = fill_domain(ctx, mesh, rule, omega, cd(ri_re, ri_im), t))) return rc;
-  MOM_HIP(hipMemcpy2DAsync(M, ldm * sizeof(double2), dM.ptr, E * sizeof(double2), E * sizeof(double2),
-                           E, hipMemcpyDeviceToHost, ctx->stream));
-  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  return copy_d2h_2d(ctx, M, ldm * sizeof(double2), dM.ptr, E * sizeof(double2), E * sizeof(double2), E);
   return 0;
 }
 
@@ -299,9 +399,7 @@ int mom_srcvec_pw(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, doub
   if ((rc = srcvec_pw(ctx, mesh, rule, omega, cd(ri_re, ri_im), nsrc, angles, nedges_total,
                       dq.as<double2>(), N2)))
     return rc;
-  MOM_HIP(hipMemcpy2DAsync(q, ldq * sizeof(double2), dq.ptr, N2 * sizeof(double2), N2 * sizeof(double2),
-                           nsrc, hipMemcpyDeviceToHost, ctx->stream));
-  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  return copy_d2h_2d(ctx, q, ldq * sizeof(double2), dq.ptr, N2 * sizeof(double2), N2 * sizeof(double2), nsrc);
   return 0;
 }
 
@@ -317,14 +415,12 @@ int mom_cs_prtsrf(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, doub
   const int64_t N2 = 2 * nedges_total;
   DevBuf dx, dpart;
   if ((rc = dx.alloc(sizeof(double2) * N2)) || (rc = dpart.alloc(sizeof(double) * 2 * mesh->F))) return rc;
-  MOM_HIP(hipMemcpyAsync(dx.ptr, x, sizeof(double2) * N2, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = copy_h2d(ctx, dx.ptr, x, sizeof(double2) * N2))) return rc;
   if ((rc = cs_partials(ctx, mesh, rule, omega, cd(ri_re, ri_im), dx.as<double2>(), nedges_total, theta,
                         phi, psi, dpart.as<double>())))
     return rc;
   std::vector<double> part(2 * mesh->F);
-  MOM_HIP(hipMemcpyAsync(part.data(), dpart.ptr, sizeof(double) * 2 * mesh->F, hipMemcpyDeviceToHost,
-                         ctx->stream));
-  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  if ((rc = copy_d2h(ctx, part.data(), dpart.ptr, sizeof(double) * 2 * mesh->F))) return rc;
   // cs.f90:38, 97-102: sums in face order, then c = 2/(eps0 c0 Re ri)
   double sa = 0, ss = 0;
   for (int64_t f = 0; f < mesh->F; f++) {
@@ -352,10 +448,7 @@ extern "C" int mom_sysmat_pmchwt(mom_ctx *ctx, int32_t ndom, const mom_mesh *con
   if ((rc = mom::fill_system(ctx, ndom, doms, ri, nedges_total, omega, nq, qw, qnodes,
                              dA.as<double2>(), N2)))
     return rc;
-  MOM_HIP(hipMemcpy2DAsync(A, lda * sizeof(double2), dA.ptr, N2 * sizeof(double2), N2 * sizeof(double2),
-                           N2, hipMemcpyDeviceToHost, ctx->stream));
-  MOM_HIP(hipStreamSynchronize(ctx->stream));
-  return 0;
+  return mom::copy_d2h_2d(ctx, A, lda * sizeof(double2), dA.ptr, N2 * sizeof(double2), N2 * sizeof(double2), N2);
 }
 
 namespace mom {

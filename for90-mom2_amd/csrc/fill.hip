@@ -731,8 +731,8 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   }
   DevBuf drl, dcol;
   if ((rc = drl.alloc(sizeof(int) * nrows)) || (rc = dcol.alloc(sizeof(int) * m->E))) return rc;
-  MOM_HIP(hipMemcpyAsync(drl.ptr, rl.data(), sizeof(int) * nrows, hipMemcpyHostToDevice, ctx->stream));
-  MOM_HIP(hipMemcpyAsync(dcol.ptr, colidx.data(), sizeof(int) * m->E, hipMemcpyHostToDevice, ctx->stream));
+  MOM_TRY(copy_h2d(ctx, drl.ptr, rl.data(), sizeof(int) * nrows));
+  MOM_TRY(copy_h2d(ctx, dcol.ptr, colidx.data(), sizeof(int) * m->E));
   PlaneArgs pa;
   pa.f = a;
   pa.B = ctx->scratch.as<double2>();
@@ -876,7 +876,7 @@ int srcvec_pw(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omeg
   }
   DevBuf dsrc;
   if ((rc = dsrc.alloc(sizeof(PwSrc) * nsrc))) return rc;
-  MOM_HIP(hipMemcpyAsync(dsrc.ptr, hs.data(), sizeof(PwSrc) * nsrc, hipMemcpyHostToDevice, ctx->stream));
+  MOM_TRY(copy_h2d(ctx, dsrc.ptr, hs.data(), sizeof(PwSrc) * nsrc));
   const cd k = ri * omega / kC0;
   const cd hfac = k / (omega * mu0());
   int64_t total = m->F * nsrc;

@@ -84,12 +84,26 @@ constexpr int IB = 32;   // inner panel block
 //    buffers, planar re/im), one barrier per chunk;
 //  - LDS rows are padded by 16 elements so that the four k-rows a wave reads
 //    per MFMA operand fall into disjoint bank groups;
-//  - tiles are visited in groups of kGroupM row blocks so that concurrently
-//    running workgroups share A and B panels in L2;
+//  - workgroups are dispatched to the 8 XCDs round-robin and each XCD has
+//    its own L2, so the launch index is first remapped to make each XCD's
+//    workgroups a contiguous range of tiles; within it tiles are visited in
+//    groups of kGroupM row blocks so that concurrently running workgroups
+//    of one XCD share A and B panels in their L2;
 //  - M3 = false: four real products per complex product; M3 = true: three
 //    (Gauss/3M: T1 = Ar Br, T2 = Ai Bi, T3 = (Ar+Ai)(Br+Bi)), the sums formed
 //    in registers from the two staged planes.
 constexpr int kGroupM = 8;
+constexpr int kXcds = 8;
+
+// Bijection launch index -> tile index under which XCD x (launch indices
+// x, x+8, ...) receives one contiguous range of tile indices.
+__device__ __forceinline__ int xcd_contiguous(int pid, int G) {
+  const int per = (G + kXcds - 1) / kXcds;
+  int tall = G % kXcds;
+  if (tall == 0) tall = kXcds;
+  const int x = pid % kXcds, l = pid / kXcds;
+  return x < tall ? x * per + l : tall * per + (x - tall) * (per - 1) + l;
+}
 
 template <typename T, int WM, int WN, int BK, bool M3>
 __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::type *__restrict__ A, int64_t lda,
@@ -109,7 +123,7 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
   const int l16 = lane & 15, kq = lane >> 4;
   const int wr = (wave & 1) * 16 * WM, wc = (wave >> 1) * 16 * WN;
   // grouped tile order
-  const int pid = blockIdx.x;
+  const int pid = xcd_contiguous(blockIdx.x, gridDim.x);
   const int per_group = kGroupM * gn;
   const int first_m = (pid / per_group) * kGroupM;
   const int gsize = min(gm - first_m, kGroupM);
@@ -339,6 +353,7 @@ struct PivotWork {
   void *row;     // [2][mb][32] complex of the factor's type
   int mb;        // slots per parity (>= blocks of the widest launch)
   int *info;     // first zero pivot (1-based), 0 if none
+  int *plan;     // laswp plan of one panel: 1 + 4 NB ints
 };
 
 
@@ -404,7 +419,13 @@ __global__ __launch_bounds__(256) void pivot_candidates_kernel(const typename Cx
     w.val[slot] = v;
     w.idx[slot] = idx;
     C2 *row = (C2 *)w.row + slot * IB;
-    for (int c = col; c < cend; c++) row[c - i0] = A[(int64_t)c * ld + r];
+    C2 x[IB];
+#pragma unroll
+    for (int q = 0; q < IB; q++)
+      if (i0 + q >= col && i0 + q < cend) x[q] = A[(int64_t)(i0 + q) * ld + r];
+#pragma unroll
+    for (int q = 0; q < IB; q++)
+      if (i0 + q >= col && i0 + q < cend) row[q] = x[q];
   }
 }
 
@@ -523,37 +544,114 @@ __global__ __launch_bounds__(256) void panel_column_kernel(typename Cx<T>::type 
   }
 }
 
-// Row interchanges k1..k2-1 (ipiv 1-based) on columns [c0, c1) and [c2, c3).
-template <typename T>
-__global__ void laswp_kernel(typename Cx<T>::type *A, int64_t ld, int k1, int k2, const int *ipiv, int c0, int c1,
-                             int c2, int c3) {
-  using C2 = typename Cx<T>::type;
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t n1 = c1 - c0;
-  int64_t c;
-  if (t < n1)
-    c = c0 + t;
-  else if (t < n1 + (c3 - c2))
-    c = c2 + (t - n1);
-  else
-    return;
-  C2 *col = A + c * ld;
-  for (int k = k1; k < k2; k++) {
-    const int p = ipiv[k] - 1;
-    if (p != k) {
-      const C2 a = col[k], b = col[p];
-      col[k] = b;
-      col[p] = a;
+// Row interchanges k1..k2-1 (ipiv 1-based, k2 - k1 <= NB) as one gather.
+// The plan kernel replays the interchanges on slot indices: slot s < jb is
+// row k1 + s, slot jb + k is row p_k for an interchange k whose pivot row
+// lies below k2 (one slot per distinct such row, at its first occurrence).
+// After the replay slot s holds the old content of slot idx[s]; the plan is
+// the list of (destination row, source row) pairs, -1 for unchanged slots:
+// plan[0] = 2 jb, plan[1 + e] = destination, plan[1 + 2 NB + e] = source.
+__global__ __launch_bounds__(256) void laswp_plan_kernel(const int *ipiv, int k1, int k2, int *plan) {
+  __shared__ int pv[NB], first[NB], idx[2 * NB];
+  const int tid = threadIdx.x, jb = k2 - k1;
+  if (tid < jb) pv[tid] = ipiv[k1 + tid] - 1;
+  for (int q = tid; q < 2 * NB; q += 256) idx[q] = q;
+  __syncthreads();
+  if (tid < jb) {
+    int f = tid;
+    if (pv[tid] >= k2)
+      for (int q = 0; q < tid; q++)
+        if (pv[q] == pv[tid]) {
+          f = q;
+          break;
+        }
+    first[tid] = f;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int k = 0; k < jb; k++) {
+      const int p = pv[k];
+      if (p == k1 + k) continue;
+      const int b = (p < k2) ? p - k1 : jb + first[k];
+      const int t = idx[k];
+      idx[k] = idx[b];
+      idx[b] = t;
     }
+    plan[0] = 2 * jb;
+  }
+  __syncthreads();
+  for (int e = tid; e < 2 * jb; e += 256) {
+    int dst = -1, src = -1;
+    const bool live = e < jb || (first[e - jb] == e - jb && pv[e - jb] >= k2);
+    if (live) {
+      const int se = idx[e];
+      dst = e < jb ? k1 + e : pv[e - jb];
+      src = se < jb ? k1 + se : pv[se - jb];
+      if (src == dst) dst = -1;
+    }
+    plan[1 + e] = dst;
+    plan[1 + 2 * NB + e] = src;
   }
 }
 
-// X := inv(L) X for the unit lower IB x IB block at (r0, r0); one thread per
-// column of X (columns [c0, c0 + ncols)).
+// Applies a laswp plan to columns [c0, c1) and [c2, c3): each block gathers
+// the source rows of kLaswpCols columns into LDS, then scatters them to the
+// destination rows (the barrier orders every read before any write).
+constexpr int kLaswpCols = 4;
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_apply_kernel(typename Cx<T>::type *A, int64_t ld, const int *plan,
+                                                          int c0, int c1, int c2, int c3) {
+  using C2 = typename Cx<T>::type;
+  __shared__ C2 buf[kLaswpCols][2 * NB];
+  const int tid = threadIdx.x;
+  const int m = plan[0];
+  int dst[2], src[2];
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int e = tid + 256 * u;
+    dst[u] = e < m ? plan[1 + e] : -1;
+    src[u] = e < m ? plan[1 + 2 * NB + e] : -1;
+  }
+  const int n1 = c1 - c0, ncols = n1 + (c3 - c2);
+  for (int cb = blockIdx.x * kLaswpCols; cb < ncols; cb += gridDim.x * kLaswpCols) {
+#pragma unroll
+    for (int q = 0; q < kLaswpCols; q++) {
+      const int t = cb + q;
+      if (t >= ncols) break;
+      const C2 *col = A + (int64_t)(t < n1 ? c0 + t : c2 + (t - n1)) * ld;
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+        if (dst[u] >= 0) buf[q][tid + 256 * u] = col[src[u]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kLaswpCols; q++) {
+      const int t = cb + q;
+      if (t >= ncols) break;
+      C2 *col = A + (int64_t)(t < n1 ? c0 + t : c2 + (t - n1)) * ld;
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+        if (dst[u] >= 0) col[dst[u]] = buf[q][tid + 256 * u];
+    }
+    __syncthreads();
+  }
+}
+
+// X := inv(L) X for the unit lower IB x IB block at (r0, r0); L is staged
+// in LDS, one thread per column of X (columns [c0, c0 + ncols)).
 template <typename T, int W>
 __global__ __launch_bounds__(128) void trsm_unit_lower_kernel(typename Cx<T>::type *A, int64_t ld, int64_t r0,
                                                                int w, int64_t c0, int64_t ncols) {
   using C2 = typename Cx<T>::type;
+  __shared__ C2 Ls[W][W];  // Ls[k][i] = L(r0 + i, r0 + k)
+  for (int e = threadIdx.x; e < W * W; e += 128) {
+    const int k = e / W, i = e % W;
+    C2 z;
+    z.x = 0;
+    z.y = 0;
+    Ls[k][i] = (i < w && k < w) ? A[(r0 + k) * ld + r0 + i] : z;
+  }
+  __syncthreads();
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ncols) return;
   C2 *col = A + (c0 + t) * ld + r0;
@@ -567,14 +665,11 @@ __global__ __launch_bounds__(128) void trsm_unit_lower_kernel(typename Cx<T>::ty
 #pragma unroll
   for (int k = 0; k < W; k++) {
     const C2 xk = x[k];
-    const C2 *lk = A + (r0 + k) * ld + r0;
 #pragma unroll
     for (int i = k + 1; i < W; i++) {
-      if (i < w) {
-        const C2 l = lk[i];
-        x[i].x -= l.x * xk.x - l.y * xk.y;
-        x[i].y -= l.x * xk.y + l.y * xk.x;
-      }
+      const C2 l = Ls[k][i];
+      x[i].x -= l.x * xk.x - l.y * xk.y;
+      x[i].y -= l.x * xk.y + l.y * xk.x;
     }
   }
 #pragma unroll
@@ -640,8 +735,11 @@ static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, 
     // row interchanges of this panel on the columns left and right of it
     const int64_t left = j0, right = n - (j0 + jb);
     if (left + right > 0) {
-      hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)((left + right + 255) / 256)), dim3(256), 0, s, A, ld,
-                         (int)j0, (int)(j0 + jb), ipiv, 0, (int)j0, (int)(j0 + jb), (int)n);
+      hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)j0, (int)(j0 + jb), w.plan);
+      MOM_HIP(hipGetLastError());
+      const int64_t groups = (left + right + kLaswpCols - 1) / kLaswpCols;
+      hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, s, A,
+                         ld, w.plan, 0, (int)j0, (int)(j0 + jb), (int)n);
       MOM_HIP(hipGetLastError());
     }
     if (right > 0) {
@@ -988,7 +1086,7 @@ namespace mom {
 static int ensure_pivwork(mom_system *S, PivotWork *w) {
   const int64_t mb = (S->n + 255) / 256 + 1;
   const size_t rowbytes = 2 * mb * IB * sizeof(double2);
-  const size_t bytes = rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64;
+  const size_t bytes = rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64 + (1 + 4 * NB) * sizeof(int);
   if (S->pivwork.bytes < bytes) {
     int rc = S->pivwork.alloc(bytes);
     if (rc) return rc;
@@ -998,6 +1096,7 @@ static int ensure_pivwork(mom_system *S, PivotWork *w) {
   w->val = (double *)(base + rowbytes);
   w->idx = (int *)(base + rowbytes + 2 * mb * sizeof(double));
   w->info = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)));
+  w->plan = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64);
   w->mb = (int)mb;
   MOM_HIP(hipMemsetAsync(w->info, 0, sizeof(int), S->ctx->stream));
   return 0;
@@ -1021,9 +1120,8 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
     return rc;
   std::vector<int> ip(n);
   int info = 0;
-  MOM_HIP(hipMemcpyAsync(ip.data(), S->ipiv.ptr, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-  MOM_HIP(hipMemcpyAsync(&info, w.info, sizeof(int), hipMemcpyDeviceToHost, s));
-  MOM_HIP(hipStreamSynchronize(s));
+  MOM_TRY(copy_d2h(S->ctx, ip.data(), S->ipiv.ptr, sizeof(int) * n));
+  MOM_TRY(copy_d2h(S->ctx, &info, w.info, sizeof(int)));
   if (S->gemm.enabled && (rc = S->gemm.collect())) return rc;
   S->perm.resize(n);
   for (int64_t i = 0; i < n; i++) S->perm[i] = (int)i;
@@ -1035,7 +1133,7 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
 static int upload_perm(mom_system *S, DevBuf &dperm) {
   int rc = dperm.alloc(sizeof(int) * S->n);
   if (rc) return rc;
-  MOM_HIP(hipMemcpyAsync(dperm.ptr, S->perm.data(), sizeof(int) * S->n, hipMemcpyHostToDevice, S->ctx->stream));
+  MOM_TRY(copy_h2d(S->ctx, dperm.ptr, S->perm.data(), sizeof(int) * S->n));
   return 0;
 }
 
@@ -1049,7 +1147,7 @@ static int apply_minv(mom_system *S, IrWork &W, const std::vector<cd> &v, std::v
   // in complex128 arithmetic
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
-  MOM_HIP(hipMemcpyAsync(W.v64.ptr, v.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+  MOM_TRY(copy_h2d(S->ctx, W.v64.ptr, v.data(), sizeof(double2) * n));
   const unsigned g = (unsigned)((n + 255) / 256);
   hipLaunchKernelGGL(dscale_kernel, dim3(g), dim3(256), 0, s, W.v64.as<double2>(), S->rscale.as<double>(), n,
                      W.t64.as<double2>());
@@ -1061,9 +1159,7 @@ static int apply_minv(mom_system *S, IrWork &W, const std::vector<cd> &v, std::v
                      W.w64.as<double2>());
   MOM_HIP(hipGetLastError());
   y.resize(n);
-  MOM_HIP(hipMemcpyAsync(y.data(), W.w64.ptr, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
-  MOM_HIP(hipStreamSynchronize(s));
-  return 0;
+  return copy_d2h(S->ctx, y.data(), W.w64.ptr, sizeof(double2) * n);
 }
 
 // y = b - Z x (b may be NULL: y = -Z x); x, y on the host
@@ -1071,17 +1167,15 @@ static int residual(mom_system *S, IrWork &W, const std::vector<cd> *b, const st
                     std::vector<cd> &y) {
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
-  MOM_HIP(hipMemcpyAsync(W.x64.ptr, x.data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
-  if (b) MOM_HIP(hipMemcpyAsync(W.b64.ptr, b->data(), sizeof(double2) * n, hipMemcpyHostToDevice, s));
+  MOM_TRY(copy_h2d(S->ctx, W.x64.ptr, x.data(), sizeof(double2) * n));
+  if (b) MOM_TRY(copy_h2d(S->ctx, W.b64.ptr, b->data(), sizeof(double2) * n));
   hipLaunchKernelGGL(zgemv_part_kernel, dim3((unsigned)((n + 255) / 256), kChunks), dim3(256), 0, s,
                      S->Z64.as<double2>(), n, n, W.x64.as<double2>(), S->part.as<double2>());
   hipLaunchKernelGGL(zgemv_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                      S->part.as<double2>(), n, b ? W.b64.as<double2>() : nullptr, W.w64.as<double2>());
   MOM_HIP(hipGetLastError());
   y.resize(n);
-  MOM_HIP(hipMemcpyAsync(y.data(), W.w64.ptr, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
-  MOM_HIP(hipStreamSynchronize(s));
-  return 0;
+  return copy_d2h(S->ctx, y.data(), W.w64.ptr, sizeof(double2) * n);
 }
 
 static double inf_norm(const std::vector<cd> &v) {
@@ -1246,9 +1340,7 @@ int mom_system_get_matrix(mom_system *S, double *A, int64_t lda) {
   if (!S->filled || (S->factored && S->precision == MOM_PREC_FP64))
     return fail(MOM_ERR_STATE, "no unfactored matrix on the device");
   const int64_t n = S->n;
-  MOM_HIP(hipMemcpy2DAsync(A, lda * sizeof(double2), S->Z64.ptr, n * sizeof(double2), n * sizeof(double2), n,
-                           hipMemcpyDeviceToHost, S->ctx->stream));
-  MOM_HIP(hipStreamSynchronize(S->ctx->stream));
+  MOM_TRY(copy_d2h_2d(S->ctx, A, lda * sizeof(double2), S->Z64.ptr, n * sizeof(double2), n * sizeof(double2), n));
   return 0;
 }
 
@@ -1256,8 +1348,7 @@ int mom_system_set_matrix(mom_system *S, const double *A, int64_t lda) {
   if (!S || !A) return fail(MOM_ERR_ARG, "NULL argument");
   if (lda < S->n) return fail(MOM_ERR_ARG, "lda < 2N");
   const int64_t n = S->n;
-  MOM_HIP(hipMemcpy2DAsync(S->Z64.ptr, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n,
-                           hipMemcpyHostToDevice, S->ctx->stream));
+  MOM_TRY(copy_h2d_2d(S->ctx, S->Z64.ptr, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n));
   MOM_HIP(hipStreamSynchronize(S->ctx->stream));
   S->filled = true;
   S->factored = false;
@@ -1287,8 +1378,7 @@ int mom_system_factor(mom_system *S) {
                        S->Z32.as<float2>());
     MOM_HIP(hipGetLastError());
     std::vector<double> rs(n);
-    MOM_HIP(hipMemcpyAsync(rs.data(), S->rowsum.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    MOM_HIP(hipStreamSynchronize(s));
+    MOM_TRY(copy_d2h(S->ctx, rs.data(), S->rowsum.ptr, sizeof(double) * n));
     S->znorm_inf = *std::max_element(rs.begin(), rs.end());
     rc = factor_impl<float>(S, S->Z32.as<float2>());
   }
@@ -1313,14 +1403,11 @@ int mom_system_solve(mom_system *S, int64_t nrhs, double *B, int64_t ldb, double
     if ((rc = dB.alloc(sizeof(double2) * n * nrhs)) || (rc = tmp.alloc(sizeof(double2) * n * nrhs)) ||
         (rc = xb.alloc(sizeof(double2) * NB * nrhs)) || (rc = upload_perm(S, dperm)))
       return rc;
-    MOM_HIP(hipMemcpy2DAsync(dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs,
-                             hipMemcpyHostToDevice, s));
+    MOM_TRY(copy_h2d_2d(S->ctx, dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs));
     if ((rc = getrs<double, double>(s, S->Z64.as<double2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
                             S->Uinv.as<double2>(), dB.as<double2>(), n, nrhs, tmp.as<double2>(), xb.as<double2>())))
       return rc;
-    MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs,
-                             hipMemcpyDeviceToHost, s));
-    MOM_HIP(hipStreamSynchronize(s));
+    MOM_TRY(copy_d2h_2d(S->ctx, B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs));
     if (iters) *iters = 0;
     if (berr) *berr = -1.0;
   } else {
@@ -1380,8 +1467,7 @@ int mom_solve_multi_linsys(mom_ctx *ctx, int64_t n, double *A, int64_t lda, int6
   S.N = (n + 1) / 2;
   int rc;
   if ((rc = S.Z64.alloc(sizeof(double2) * n * n)) || (rc = S.ipiv.alloc(sizeof(int) * n))) return rc;
-  MOM_HIP(hipMemcpy2DAsync(S.Z64.ptr, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n,
-                           hipMemcpyHostToDevice, s));
+  MOM_TRY(copy_h2d_2d(ctx, S.Z64.ptr, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n));
   if ((rc = factor_impl<double>(&S, S.Z64.as<double2>()))) return rc;
   S.factored = true;
   S.precision = MOM_PREC_FP64;
@@ -1391,18 +1477,14 @@ int mom_solve_multi_linsys(mom_ctx *ctx, int64_t n, double *A, int64_t lda, int6
       (rc = xb.alloc(sizeof(double2) * NB * nrhs)) || (rc = upload_perm(&S, dperm)))
     return rc;
   if (info == 0) {
-    MOM_HIP(hipMemcpy2DAsync(dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs,
-                             hipMemcpyHostToDevice, s));
+    MOM_TRY(copy_h2d_2d(ctx, dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs));
     if ((rc = getrs<double, double>(s, S.Z64.as<double2>(), n, n, dperm.as<int>(), S.Linv.as<double2>(), S.Uinv.as<double2>(),
                             dB.as<double2>(), n, nrhs, tmp.as<double2>(), xb.as<double2>())))
       return rc;
-    MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs,
-                             hipMemcpyDeviceToHost, s));
+    MOM_TRY(copy_d2h_2d(ctx, B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs));
   }
-  MOM_HIP(hipMemcpy2DAsync(A, lda * sizeof(double2), S.Z64.ptr, n * sizeof(double2), n * sizeof(double2), n,
-                           hipMemcpyDeviceToHost, s));
-  if (ipiv_out) MOM_HIP(hipMemcpyAsync(ipiv_out, S.ipiv.ptr, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-  MOM_HIP(hipStreamSynchronize(s));
+  MOM_TRY(copy_d2h_2d(ctx, A, lda * sizeof(double2), S.Z64.ptr, n * sizeof(double2), n * sizeof(double2), n));
+  if (ipiv_out) MOM_TRY(copy_d2h(ctx, ipiv_out, S.ipiv.ptr, sizeof(int) * n));
   return info;
 }
 
@@ -1443,8 +1525,7 @@ extern "C" int mom_test_mfma_layout(mom_ctx *ctx, int32_t precision, double *out
   else
     hipLaunchKernelGGL(mfma_probe_kernel<float>, dim3(4096), dim3(64), 0, ctx->stream, d.as<double>());
   MOM_HIP(hipGetLastError());
-  MOM_HIP(hipMemcpyAsync(out, d.ptr, cnt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  MOM_TRY(copy_d2h(ctx, out, d.ptr, cnt * sizeof(double)));
   return 0;
 }
 
@@ -1458,9 +1539,9 @@ extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int
   if ((rc = dA.alloc(sizeof(double2) * lda * K)) || (rc = dB.alloc(sizeof(double2) * ldb * N)) ||
       (rc = dC.alloc(sizeof(double2) * ldc * N)))
     return rc;
-  MOM_HIP(hipMemcpyAsync(dA.ptr, A, sizeof(double2) * lda * K, hipMemcpyHostToDevice, s));
-  MOM_HIP(hipMemcpyAsync(dB.ptr, B, sizeof(double2) * ldb * N, hipMemcpyHostToDevice, s));
-  MOM_HIP(hipMemcpyAsync(dC.ptr, C, sizeof(double2) * ldc * N, hipMemcpyHostToDevice, s));
+  MOM_TRY(copy_h2d(ctx, dA.ptr, A, sizeof(double2) * lda * K));
+  MOM_TRY(copy_h2d(ctx, dB.ptr, B, sizeof(double2) * ldb * N));
+  MOM_TRY(copy_h2d(ctx, dC.ptr, C, sizeof(double2) * ldc * N));
   if (precision == 64) {
     if ((rc = gemm_sub<double>(s, dA.as<double2>(), lda, dB.as<double2>(), ldb, dC.as<double2>(), ldc, M, N, K)))
       return rc;
@@ -1482,7 +1563,5 @@ extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int
                        dC.as<double2>(), ldc * N);
   }
   MOM_HIP(hipGetLastError());
-  MOM_HIP(hipMemcpyAsync(C, dC.ptr, sizeof(double2) * ldc * N, hipMemcpyDeviceToHost, s));
-  MOM_HIP(hipStreamSynchronize(s));
-  return 0;
+  return copy_d2h(ctx, C, dC.ptr, sizeof(double2) * ldc * N);
 }

@@ -35,6 +35,12 @@ int fail(int code, const std::string &msg);
       return ::mom::fail(MOM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+#define MOM_TRY(call)      \
+  do {                     \
+    int rc_ = (call);      \
+    if (rc_) return rc_;   \
+  } while (0)
+
 // Device buffer with RAII.
 struct DevBuf {
   void *ptr = nullptr;
@@ -48,6 +54,19 @@ struct DevBuf {
   template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// Host <-> device copies on the context's stream through two pinned staging
+// buffers: caller (pageable) memory is only read and written by the CPU, so
+// the runtime never has to pin it.  copy_h2d returns once the source has
+// been consumed (the device copy may still be in flight on the stream);
+// copy_d2h returns when the destination holds the data.  The 2-D forms copy
+// `rows` runs of `width` bytes with the given pitches.
+int copy_h2d(mom_ctx *ctx, void *dev, const void *host, size_t bytes);
+int copy_d2h(mom_ctx *ctx, void *host, const void *dev, size_t bytes);
+int copy_h2d_2d(mom_ctx *ctx, void *dev, size_t dpitch, const void *host, size_t hpitch, size_t width,
+                size_t rows);
+int copy_d2h_2d(mom_ctx *ctx, void *host, size_t hpitch, const void *dev, size_t dpitch, size_t width,
+                size_t rows);
+
 }  // namespace mom
 
 struct mom_ctx {
@@ -55,6 +74,8 @@ struct mom_ctx {
   hipStream_t stream = nullptr;
   mom::DevBuf qp;       // quadrature points of the last mesh/rule (scratch)
   mom::DevBuf scratch;  // general scratch
+  void *stage[2] = {nullptr, nullptr};   // pinned staging buffers
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 // Device copy of one domain mesh, with per-face-side RWG data precomputed.
