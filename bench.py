@@ -134,16 +134,18 @@ def main():
         gemm_flops += st["gemm_flops"]
         gemm_launches += int(st["gemm_launches"])
         last = st
-        log("step %d: %.2f s (fill %.2f factor %.2f solve %.2f, IR iters %d, berr %.2e)" %
-            (i, st["wall_s"], st["fill_s"], st["factor_s"], st["solve_s"], st["iters"], st["berr"]))
+        log("step %d: %.2f s (fill %.2f factor %.2f solve %.2f, IR iters %d, berr %.2e); factor: gemm %.2f panel %.2f "
+            "trsm %.2f laswp %.2f" % (i, st["wall_s"], st["fill_s"], st["factor_s"], st["solve_s"], st["iters"],
+                                      st["berr"], st["gemm_s"], st["panel_s"], st["trsm_s"], st["laswp_s"]))
     mixed_s = float(np.mean(times))
 
     fp64 = None
     if not args.no_fp64:
         st64, cs64 = run_path(ctx, dms, case, MOM_PREC_FP64, profile=True)
         fp64 = st64
-        log("fp64 path: %.2f s (fill %.2f factor %.2f solve %.2f)" % (st64["wall_s"], st64["fill_s"],
-                                                                      st64["factor_s"], st64["solve_s"]))
+        log("fp64 path: %.2f s (fill %.2f factor %.2f solve %.2f); factor: gemm %.2f panel %.2f trsm %.2f laswp %.2f"
+            % (st64["wall_s"], st64["fill_s"], st64["factor_s"], st64["solve_s"], st64["gemm_s"], st64["panel_s"],
+               st64["trsm_s"], st64["laswp_s"]))
     achieved = gemm_flops / gemm_s / 1e12 if gemm_s > 0 else None
     out = {
         "metric": METRIC,
@@ -164,16 +166,21 @@ def main():
                    "n_unknowns": 2 * N},
         "phases_s": {"fill": round(last["fill_s"], 4), "factor": round(last["factor_s"], 4),
                      "solve_ir": round(last["solve_s"], 4), "ir_iters": last["iters"], "berr": last["berr"]},
+        "factor_breakdown_s": {k: round(last[k + "_s"], 4) for k in ("gemm", "panel", "trsm", "laswp")},
         "roofline": None if achieved is None else {
-            "kernel": "gemm_sub_kernel<float> (LU trailing update, complex64, v_mfma_f32_16x16x4f32)",
+            "kernel": "gemm_sub_kernel<float,2,2,16,3M> (LU trailing update A22 -= L21 U12, complex64 as three real "
+                      "products on v_mfma_f32_16x16x4f32); achieved = 8MNK algorithmic flops / kernel time",
             "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_MFMA, 4), "traffic": None,
+            "mfma_issued_frac": round(achieved * 6 / 8 / PEAK_FP32_MFMA, 4),
             "launches": gemm_launches, "avg_launch_ms": round(gemm_s / max(gemm_launches, 1) * 1e3, 4)},
     }
     if fp64 is not None:
         ach64 = fp64["gemm_flops"] / fp64["gemm_s"] / 1e12 if fp64["gemm_s"] > 0 else None
         out["fp64_path"] = {"value_s": round(fp64["wall_s"], 4), "fill": round(fp64["fill_s"], 4),
                             "factor": round(fp64["factor_s"], 4), "solve": round(fp64["solve_s"], 4),
+                            "factor_breakdown_s": {k: round(fp64[k + "_s"], 4)
+                                                   for k in ("gemm", "panel", "trsm", "laswp")},
                             "gemm_tflops": None if ach64 is None else round(ach64, 3),
                             "gemm_frac_of_fp64_peak": None if ach64 is None else round(ach64 / PEAK_FP64_MFMA, 4)}
         out["speedup_vs_fp64_path"] = round(fp64["wall_s"] / mixed_s, 3)

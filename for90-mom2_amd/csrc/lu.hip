@@ -273,70 +273,92 @@ template <> struct GemmCfg<double> {
   static constexpr bool M3 = false;
 };
 
-// Event-pair timing of the trailing-update kernel launches (enabled per
-// system); events are read back after the factorisation, so timing does
-// not stall the launch queue.
-struct GemmStats {
+// Event-pair timing of the factorisation's kernel groups (enabled per
+// system): trailing-update GEMM launches (with their algorithmic flops),
+// the panel column kernels of each inner block, triangular solves and row
+// interchanges.  Pairs live in a fixed ring; when it is full the host waits
+// for the oldest pairs and folds them into the totals, so the number of
+// live events stays bounded.
+enum Phase { kPhGemm = 0, kPhPanel, kPhTrsm, kPhLaswp, kPhases };
+
+struct PhaseClock {
+  static constexpr int kRing = 1024;  // event pairs
   bool enabled = false;
   std::vector<hipEvent_t> pool;
-  size_t used = 0;
-  std::vector<double> flops_of;  // per recorded launch
-  double seconds = 0, launches = 0, flops = 0;
+  int used = 0;
+  int phase_of[kRing];
+  double flops_of[kRing];
+  double seconds[kPhases] = {}, launches[kPhases] = {};
+  double gemm_flops = 0;
   void reset() {
     used = 0;
-    flops_of.clear();
-    seconds = launches = flops = 0;
+    for (int p = 0; p < kPhases; p++) seconds[p] = launches[p] = 0;
+    gemm_flops = 0;
   }
   int begin(hipStream_t s) {
-    if (used + 2 > pool.size()) {
-      for (int i = 0; i < 512; i++) {
-        hipEvent_t e;
-        MOM_HIP(hipEventCreate(&e));
-        pool.push_back(e);
-      }
+    if (pool.empty()) {
+      pool.resize(2 * kRing);
+      for (auto &e : pool) MOM_HIP(hipEventCreate(&e));
     }
-    MOM_HIP(hipEventRecord(pool[used], s));
+    if (used == kRing) {
+      int rc = collect();
+      if (rc) return rc;
+    }
+    MOM_HIP(hipEventRecord(pool[2 * used], s));
     return 0;
   }
-  int end(hipStream_t s, double fl) {
-    MOM_HIP(hipEventRecord(pool[used + 1], s));
-    used += 2;
-    flops_of.push_back(fl);
+  int end(hipStream_t s, Phase ph, double fl = 0) {
+    MOM_HIP(hipEventRecord(pool[2 * used + 1], s));
+    phase_of[used] = ph;
+    flops_of[used] = fl;
+    used++;
     return 0;
   }
-  // call after the stream is synchronised
+  // waits for the recorded pairs and adds them to the totals
   int collect() {
-    for (size_t i = 0; i < flops_of.size(); i++) {
+    if (used == 0) return 0;
+    MOM_HIP(hipEventSynchronize(pool[2 * used - 1]));
+    for (int i = 0; i < used; i++) {
       float ms = 0;
       MOM_HIP(hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]));
-      seconds += ms * 1e-3;
-      launches += 1;
-      flops += flops_of[i];
+      seconds[phase_of[i]] += ms * 1e-3;
+      launches[phase_of[i]] += 1;
+      if (phase_of[i] == kPhGemm) gemm_flops += flops_of[i];
     }
-    flops_of.clear();
     used = 0;
     return 0;
   }
-  ~GemmStats() {
+  ~PhaseClock() {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
+};
+
+// Brackets a group of launches on stream s with an event pair when the
+// clock is enabled.
+struct Timed {
+  PhaseClock *c;
+  hipStream_t s;
+  Phase ph;
+  double fl;
+  int rc = 0;
+  Timed(PhaseClock *c_, hipStream_t s_, Phase ph_, double fl_ = 0) : c(c_ && c_->enabled ? c_ : nullptr), s(s_), ph(ph_), fl(fl_) {
+    if (c) rc = c->begin(s);
+  }
+  int done() { return (c && !rc) ? c->end(s, ph, fl) : rc; }
 };
 
 template <typename T>
 static int gemm_sub(hipStream_t s, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
                     int64_t ldb, typename Cx<T>::type *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
-                    GemmStats *st = nullptr) {
+                    PhaseClock *st = nullptr) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
   const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
-  const bool timed = st && st->enabled;
-  int rc;
-  if (timed && (rc = st->begin(s))) return rc;
+  Timed t(st, s, kPhGemm, 8.0 * (double)M * (double)N * (double)K);
   hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn), dim3(256),
                      0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn);
   MOM_HIP(hipGetLastError());
-  if (timed && (rc = st->end(s, 8.0 * (double)M * (double)N * (double)K))) return rc;
-  return 0;
+  return t.done();
 }
 
 // ---------------------------------------------------------------------------
@@ -637,44 +659,57 @@ __global__ __launch_bounds__(256) void laswp_apply_kernel(typename Cx<T>::type *
   }
 }
 
-// X := inv(L) X for the unit lower IB x IB block at (r0, r0); L is staged
-// in LDS, one thread per column of X (columns [c0, c0 + ncols)).
-template <typename T, int W>
-__global__ __launch_bounds__(128) void trsm_unit_lower_kernel(typename Cx<T>::type *A, int64_t ld, int64_t r0,
+// X := inv(L) X for the unit lower w x w block (w <= 32) at (r0, r0),
+// columns [c0, c0 + ncols) of X.  L is staged in LDS; each half-wave holds
+// one column of X (lane i = row i), and step k broadcasts x_k with a
+// shuffle and updates the rows below it.
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_unit_lower_kernel(typename Cx<T>::type *A, int64_t ld, int64_t r0,
                                                                int w, int64_t c0, int64_t ncols) {
   using C2 = typename Cx<T>::type;
-  __shared__ C2 Ls[W][W];  // Ls[k][i] = L(r0 + i, r0 + k)
-  for (int e = threadIdx.x; e < W * W; e += 128) {
-    const int k = e / W, i = e % W;
-    C2 z;
-    z.x = 0;
-    z.y = 0;
-    Ls[k][i] = (i < w && k < w) ? A[(r0 + k) * ld + r0 + i] : z;
+  __shared__ C2 Ls[IB][IB];  // Ls[k][i] = L(r0 + i, r0 + k), zero outside the w x w block
+  for (int e = threadIdx.x; e < IB * IB; e += 256) {
+    const int k = e / IB, i = e % IB;
+    T re = 0, im = 0;
+    if (i < w && k < w && i > k) {
+      const C2 l = A[(r0 + k) * ld + r0 + i];
+      re = l.x;
+      im = l.y;
+    }
+    Ls[k][i].x = re;
+    Ls[k][i].y = im;
   }
   __syncthreads();
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= ncols) return;
-  C2 *col = A + (c0 + t) * ld + r0;
-  C2 x[W];
-#pragma unroll
-  for (int i = 0; i < W; i++) {
-    x[i].x = 0;
-    x[i].y = 0;
-    if (i < w) x[i] = col[i];
-  }
-#pragma unroll
-  for (int k = 0; k < W; k++) {
-    const C2 xk = x[k];
-#pragma unroll
-    for (int i = k + 1; i < W; i++) {
-      const C2 l = Ls[k][i];
-      x[i].x -= l.x * xk.x - l.y * xk.y;
-      x[i].y -= l.x * xk.y + l.y * xk.x;
+  const int lane = threadIdx.x & 63, i = lane & 31, half = lane >> 5;
+  const int64_t pairs = (ncols + 1) / 2;
+  for (int64_t cp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); cp < pairs; cp += (int64_t)gridDim.x * 4) {
+    const int64_t c = 2 * cp + half;
+    const bool live = c < ncols && i < w;
+    C2 *x = A + (c0 + c) * ld + r0 + i;
+    T vr = 0, vi = 0;
+    if (live) {
+      const C2 v = *x;
+      vr = v.x;
+      vi = v.y;
+    }
+    for (int k = 0; k < w - 1; k++) {
+      const int src = (lane & 32) + k;
+      const T xr = __shfl(vr, src), xi = __shfl(vi, src);
+      const T lr = Ls[k][i].x, li = Ls[k][i].y;
+      vr -= lr * xr - li * xi;
+      vi -= lr * xi + li * xr;
+    }
+    if (live) {
+      C2 v;
+      v.x = vr;
+      v.y = vi;
+      *x = v;
     }
   }
-#pragma unroll
-  for (int i = 0; i < W; i++)
-    if (i < w) col[i] = x[i];
+}
+
+static unsigned trsm_grid(int64_t ncols) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(((ncols + 1) / 2 + 3) / 4, 2048));
 }
 
 // X := inv(L) X with L the unit lower w x w block at (r0, r0), w <= NB,
@@ -682,13 +717,17 @@ __global__ __launch_bounds__(128) void trsm_unit_lower_kernel(typename Cx<T>::ty
 // below within the block.
 template <typename T>
 static int trsm_block(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t r0, int64_t w, int64_t c0,
-                      int64_t ncols, GemmStats *st) {
+                      int64_t ncols, PhaseClock *st) {
   if (ncols <= 0) return 0;
   for (int64_t b = 0; b < w; b += IB) {
     const int bw = (int)std::min<int64_t>(IB, w - b);
-    hipLaunchKernelGGL((trsm_unit_lower_kernel<T, IB>), dim3((unsigned)((ncols + 127) / 128)), dim3(128), 0, s,
-                       A, ld, r0 + b, bw, c0, ncols);
-    MOM_HIP(hipGetLastError());
+    {
+      Timed t(st, s, kPhTrsm);
+      hipLaunchKernelGGL(trsm_unit_lower_kernel<T>, dim3(trsm_grid(ncols)), dim3(256), 0, s, A, ld, r0 + b, bw, c0,
+                         ncols);
+      MOM_HIP(hipGetLastError());
+      MOM_TRY(t.done());
+    }
     const int64_t below = w - b - bw;
     if (below > 0) {
       int rc = gemm_sub<T>(s, A + (r0 + b) * ld + r0 + b + bw, ld, A + c0 * ld + r0 + b, ld,
@@ -702,12 +741,13 @@ static int trsm_block(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_
 // LU with partial pivoting of the n x n matrix A (ld), in place.
 template <typename T>
 static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, int *ipiv, PivotWork w,
-                 GemmStats *st) {
+                 PhaseClock *st) {
   for (int64_t j0 = 0; j0 < n; j0 += NB) {
     const int64_t jb = std::min<int64_t>(NB, n - j0);
     for (int64_t i0 = j0; i0 < j0 + jb; i0 += IB) {
       const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
       // candidates of the block's first column, then one launch per column
+      Timed tp(st, s, kPhPanel);
       hipLaunchKernelGGL(pivot_candidates_kernel<T>, dim3((unsigned)((n - i0 + 255) / 256)), dim3(256), 0, s, A, ld,
                          (int)i0, (int)n, (int)i0, (int)(i0 + ib), w, (int)(i0 & 1));
       MOM_HIP(hipGetLastError());
@@ -721,12 +761,15 @@ static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, 
         MOM_HIP(hipGetLastError());
         nb_in = nblk;
       }
+      MOM_TRY(tp.done());
       const int64_t rest = j0 + jb - (i0 + ib);
       if (rest > 0) {
         // U block of the remaining panel columns, then their update
-        hipLaunchKernelGGL((trsm_unit_lower_kernel<T, IB>), dim3((unsigned)((rest + 127) / 128)), dim3(128), 0, s,
-                           A, ld, i0, (int)ib, i0 + ib, rest);
+        Timed tt(st, s, kPhTrsm);
+        hipLaunchKernelGGL(trsm_unit_lower_kernel<T>, dim3(trsm_grid(rest)), dim3(256), 0, s, A, ld, i0, (int)ib,
+                           i0 + ib, rest);
         MOM_HIP(hipGetLastError());
+        MOM_TRY(tt.done());
         int rc = gemm_sub<T>(s, A + i0 * ld + i0 + ib, ld, A + (i0 + ib) * ld + i0, ld,
                              A + (i0 + ib) * ld + i0 + ib, ld, n - i0 - ib, rest, ib, st);
         if (rc) return rc;
@@ -735,12 +778,14 @@ static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, 
     // row interchanges of this panel on the columns left and right of it
     const int64_t left = j0, right = n - (j0 + jb);
     if (left + right > 0) {
+      Timed tl(st, s, kPhLaswp);
       hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)j0, (int)(j0 + jb), w.plan);
       MOM_HIP(hipGetLastError());
       const int64_t groups = (left + right + kLaswpCols - 1) / kLaswpCols;
       hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, s, A,
                          ld, w.plan, 0, (int)j0, (int)(j0 + jb), (int)n);
       MOM_HIP(hipGetLastError());
+      MOM_TRY(tl.done());
     }
     if (right > 0) {
       int rc = trsm_block<T>(s, A, ld, j0, jb, j0 + jb, right, st);
@@ -1077,7 +1122,7 @@ struct mom_system {
   std::vector<int> perm;
   double znorm_inf = 0;
   double t_fill = 0, t_factor = 0, t_solve = 0;
-  mom::GemmStats gemm;
+  mom::PhaseClock clock;
   int info = 0;
 };
 
@@ -1111,8 +1156,8 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
   PivotWork w;
   int rc;
   if ((rc = ensure_pivwork(S, &w))) return rc;
-  if (S->gemm.enabled) S->gemm.reset();
-  if ((rc = getrf<T>(s, A, n, n, S->ipiv.as<int>(), w, &S->gemm))) return rc;
+  if (S->clock.enabled) S->clock.reset();
+  if ((rc = getrf<T>(s, A, n, n, S->ipiv.as<int>(), w, &S->clock))) return rc;
   const int64_t nblk = (n + NB - 1) / NB;
   const size_t inv_bytes = sizeof(typename Cx<T>::type) * nblk * NB * NB;
   if (S->Linv.bytes < inv_bytes && ((rc = S->Linv.alloc(inv_bytes)) || (rc = S->Uinv.alloc(inv_bytes)))) return rc;
@@ -1122,7 +1167,7 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
   int info = 0;
   MOM_TRY(copy_d2h(S->ctx, ip.data(), S->ipiv.ptr, sizeof(int) * n));
   MOM_TRY(copy_d2h(S->ctx, &info, w.info, sizeof(int)));
-  if (S->gemm.enabled && (rc = S->gemm.collect())) return rc;
+  if (S->clock.enabled && (rc = S->clock.collect())) return rc;
   S->perm.resize(n);
   for (int64_t i = 0; i < n; i++) S->perm[i] = (int)i;
   for (int64_t k = 0; k < n; k++) std::swap(S->perm[k], S->perm[ip[k] - 1]);
@@ -1316,7 +1361,7 @@ int mom_system_destroy(mom_system *sys) {
 
 int mom_system_profile(mom_system *sys, int32_t enable) {
   if (!sys) return fail(MOM_ERR_ARG, "NULL argument");
-  sys->gemm.enabled = enable != 0;
+  sys->clock.enabled = enable != 0;
   return 0;
 }
 
@@ -1449,9 +1494,12 @@ int mom_system_stats(const mom_system *S, double *out) {
   out[0] = S->t_fill;
   out[1] = S->t_factor;
   out[2] = S->t_solve;
-  out[3] = S->gemm.seconds;
-  out[4] = S->gemm.launches;
-  out[5] = S->gemm.flops;
+  out[3] = S->clock.seconds[mom::kPhGemm];
+  out[4] = S->clock.launches[mom::kPhGemm];
+  out[5] = S->clock.gemm_flops;
+  out[6] = S->clock.seconds[mom::kPhPanel];
+  out[7] = S->clock.seconds[mom::kPhTrsm];
+  out[8] = S->clock.seconds[mom::kPhLaswp];
   return 0;
 }
 

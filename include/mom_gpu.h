@@ -147,14 +147,17 @@ int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, dou
                         double ri_im, int64_t nsrc, const double *angles, int32_t nq,
                         const double *qw, const double *qnodes, double tol, int32_t maxit,
                         double *X, int64_t ldx, int32_t *iters, double *berr);
-/* Enable (1) or disable (0) event timing of the trailing-update kernel
- * launches during mom_system_factor (reported by mom_system_stats). */
+/* Enable (1) or disable (0) event timing of the kernel groups of
+ * mom_system_factor (reported by mom_system_stats). */
 int mom_system_profile(mom_system *sys, int32_t enable);
 /*
- * Wall times of the last fill / factor / solve [s] and kernel statistics:
+ * Wall times of the last fill / factor / solve [s] and, with profiling
+ * enabled, the factorisation's kernel time by group (9 doubles):
  * out[0] fill, out[1] factor, out[2] solve, out[3] trailing-update GEMM
- * kernel time summed over the last factorisation [s], out[4] number of
- * those GEMM launches, out[5] their algorithmic flops.
+ * kernel time [s], out[4] number of those GEMM launches, out[5] their
+ * algorithmic flops (8 M N K each), out[6] panel factorisation (pivot
+ * search, row swaps in the panel, column updates) [s], out[7] triangular
+ * solves of the U blocks [s], out[8] row interchanges outside the panels [s].
  */
 int mom_system_stats(const mom_system *sys, double *out);
 
