@@ -30,11 +30,13 @@
 // FP32 factors, refines the solution until the complex128 backward error
 // ||b - Zx||_inf / (||Z||_inf ||x||_inf + ||b||_inf) reaches the tolerance.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "mom_internal.hpp"
@@ -258,19 +260,10 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
 #define MOM_GEMM_F32_BK 16
 #define MOM_GEMM_F32_M3 true
 #endif
-#ifndef MOM_GEMM_F64_WM
-#define MOM_GEMM_F64_WM 2
-#define MOM_GEMM_F64_WN 2
-#define MOM_GEMM_F64_BK 8
-#endif
 template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> {
   static constexpr int WM = MOM_GEMM_F32_WM, WN = MOM_GEMM_F32_WN, BK = MOM_GEMM_F32_BK;
   static constexpr bool M3 = MOM_GEMM_F32_M3;
-};
-template <> struct GemmCfg<double> {
-  static constexpr int WM = MOM_GEMM_F64_WM, WN = MOM_GEMM_F64_WN, BK = MOM_GEMM_F64_BK;
-  static constexpr bool M3 = false;
 };
 
 // Event-pair timing of the factorisation's kernel groups (enabled per
@@ -347,17 +340,55 @@ struct Timed {
   int done() { return (c && !rc) ? c->end(s, ph, fl) : rc; }
 };
 
+// rocBLAS handle of a context, created on first use and bound to the
+// context's stream.
+static int blas_handle(mom_ctx *ctx, rocblas_handle *h) {
+  if (!ctx->blas) {
+    rocblas_handle b = nullptr;
+    if (rocblas_create_handle(&b) != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_create_handle failed");
+    if (rocblas_set_stream(b, ctx->stream) != rocblas_status_success) {
+      rocblas_destroy_handle(b);
+      return fail(MOM_ERR_HIP, "rocblas_set_stream failed");
+    }
+    ctx->blas = b;
+  }
+  *h = static_cast<rocblas_handle>(ctx->blas);
+  return 0;
+}
+
+void release_blas(mom_ctx *ctx) {
+  if (ctx->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
+  ctx->blas = nullptr;
+}
+
+// C -= A B on the context's stream.  complex64: the hand-written 3M
+// matrix-core kernel above.  complex128: rocBLAS ZGEMM (the plain library
+// GEMM; its FP64 matrix-core kernels are the strongest available baseline
+// for the all-FP64 path).
 template <typename T>
-static int gemm_sub(hipStream_t s, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
+static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
                     int64_t ldb, typename Cx<T>::type *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
                     PhaseClock *st = nullptr) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
-  constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
-  const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
+  hipStream_t s = ctx->stream;
   Timed t(st, s, kPhGemm, 8.0 * (double)M * (double)N * (double)K);
-  hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn), dim3(256),
-                     0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn);
-  MOM_HIP(hipGetLastError());
+  if constexpr (std::is_same<T, double>::value) {
+    rocblas_handle h;
+    MOM_TRY(blas_handle(ctx, &h));
+    const rocblas_double_complex alpha(-1.0, 0.0), beta(1.0, 0.0);
+    const rocblas_status rs = rocblas_zgemm(
+        h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &alpha,
+        reinterpret_cast<const rocblas_double_complex *>(A), (rocblas_int)lda,
+        reinterpret_cast<const rocblas_double_complex *>(B), (rocblas_int)ldb, &beta,
+        reinterpret_cast<rocblas_double_complex *>(C), (rocblas_int)ldc);
+    if (rs != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_zgemm failed: " + std::to_string((int)rs));
+  } else {
+    constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
+    const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
+    hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn),
+                       dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn);
+    MOM_HIP(hipGetLastError());
+  }
   return t.done();
 }
 
@@ -716,9 +747,10 @@ static unsigned trsm_grid(int64_t ncols) {
 // processed in IB-row slices: small triangle then MFMA update of the rows
 // below within the block.
 template <typename T>
-static int trsm_block(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t r0, int64_t w, int64_t c0,
+static int trsm_block(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t r0, int64_t w, int64_t c0,
                       int64_t ncols, PhaseClock *st) {
   if (ncols <= 0) return 0;
+  hipStream_t s = ctx->stream;
   for (int64_t b = 0; b < w; b += IB) {
     const int bw = (int)std::min<int64_t>(IB, w - b);
     {
@@ -730,7 +762,7 @@ static int trsm_block(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_
     }
     const int64_t below = w - b - bw;
     if (below > 0) {
-      int rc = gemm_sub<T>(s, A + (r0 + b) * ld + r0 + b + bw, ld, A + c0 * ld + r0 + b, ld,
+      int rc = gemm_sub<T>(ctx, A + (r0 + b) * ld + r0 + b + bw, ld, A + c0 * ld + r0 + b, ld,
                            A + c0 * ld + r0 + b + bw, ld, below, ncols, bw, st);
       if (rc) return rc;
     }
@@ -740,8 +772,9 @@ static int trsm_block(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_
 
 // LU with partial pivoting of the n x n matrix A (ld), in place.
 template <typename T>
-static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, int *ipiv, PivotWork w,
+static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, int *ipiv, PivotWork w,
                  PhaseClock *st) {
+  hipStream_t s = ctx->stream;
   for (int64_t j0 = 0; j0 < n; j0 += NB) {
     const int64_t jb = std::min<int64_t>(NB, n - j0);
     for (int64_t i0 = j0; i0 < j0 + jb; i0 += IB) {
@@ -770,7 +803,7 @@ static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, 
                            i0 + ib, rest);
         MOM_HIP(hipGetLastError());
         MOM_TRY(tt.done());
-        int rc = gemm_sub<T>(s, A + i0 * ld + i0 + ib, ld, A + (i0 + ib) * ld + i0, ld,
+        int rc = gemm_sub<T>(ctx, A + i0 * ld + i0 + ib, ld, A + (i0 + ib) * ld + i0, ld,
                              A + (i0 + ib) * ld + i0 + ib, ld, n - i0 - ib, rest, ib, st);
         if (rc) return rc;
       }
@@ -788,9 +821,9 @@ static int getrf(hipStream_t s, typename Cx<T>::type *A, int64_t ld, int64_t n, 
       MOM_TRY(tl.done());
     }
     if (right > 0) {
-      int rc = trsm_block<T>(s, A, ld, j0, jb, j0 + jb, right, st);
+      int rc = trsm_block<T>(ctx, A, ld, j0, jb, j0 + jb, right, st);
       if (rc) return rc;
-      rc = gemm_sub<T>(s, A + j0 * ld + j0 + jb, ld, A + (j0 + jb) * ld + j0, ld, A + (j0 + jb) * ld + j0 + jb, ld,
+      rc = gemm_sub<T>(ctx, A + j0 * ld + j0 + jb, ld, A + (j0 + jb) * ld + j0, ld, A + (j0 + jb) * ld + j0 + jb, ld,
                        n - j0 - jb, right, jb, st);
       if (rc) return rc;
     }
@@ -1157,7 +1190,7 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
   int rc;
   if ((rc = ensure_pivwork(S, &w))) return rc;
   if (S->clock.enabled) S->clock.reset();
-  if ((rc = getrf<T>(s, A, n, n, S->ipiv.as<int>(), w, &S->clock))) return rc;
+  if ((rc = getrf<T>(S->ctx, A, n, n, S->ipiv.as<int>(), w, &S->clock))) return rc;
   const int64_t nblk = (n + NB - 1) / NB;
   const size_t inv_bytes = sizeof(typename Cx<T>::type) * nblk * NB * NB;
   if (S->Linv.bytes < inv_bytes && ((rc = S->Linv.alloc(inv_bytes)) || (rc = S->Uinv.alloc(inv_bytes)))) return rc;
@@ -1591,7 +1624,7 @@ extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int
   MOM_TRY(copy_h2d(ctx, dB.ptr, B, sizeof(double2) * ldb * N));
   MOM_TRY(copy_h2d(ctx, dC.ptr, C, sizeof(double2) * ldc * N));
   if (precision == 64) {
-    if ((rc = gemm_sub<double>(s, dA.as<double2>(), lda, dB.as<double2>(), ldb, dC.as<double2>(), ldc, M, N, K)))
+    if ((rc = gemm_sub<double>(ctx, dA.as<double2>(), lda, dB.as<double2>(), ldb, dC.as<double2>(), ldc, M, N, K)))
       return rc;
   } else {
     DevBuf fA, fB, fC;
@@ -1605,7 +1638,7 @@ extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int
     nar(dA, fA, lda * K);
     nar(dB, fB, ldb * N);
     nar(dC, fC, ldc * N);
-    if ((rc = gemm_sub<float>(s, fA.as<float2>(), lda, fB.as<float2>(), ldb, fC.as<float2>(), ldc, M, N, K)))
+    if ((rc = gemm_sub<float>(ctx, fA.as<float2>(), lda, fB.as<float2>(), ldb, fC.as<float2>(), ldc, M, N, K)))
       return rc;
     hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((ldc * N + 255) / 256)), dim3(256), 0, s, fC.as<float2>(),
                        dC.as<double2>(), ldc * N);

@@ -66,6 +66,7 @@ int copy_h2d_2d(mom_ctx *ctx, void *dev, size_t dpitch, const void *host, size_t
                 size_t rows);
 int copy_d2h_2d(mom_ctx *ctx, void *host, size_t hpitch, const void *dev, size_t dpitch, size_t width,
                 size_t rows);
+void release_blas(mom_ctx *ctx);
 
 }  // namespace mom
 
@@ -75,6 +76,7 @@ struct mom_ctx {
   mom::DevBuf qp;       // quadrature points of the last mesh/rule (scratch)
   mom::DevBuf scratch;  // general scratch
   void *stage[2] = {nullptr, nullptr};   // pinned staging buffers
+  void *blas = nullptr;                  // rocBLAS handle on `stream` (lu.hip)
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
