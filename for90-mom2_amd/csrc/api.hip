@@ -234,6 +234,7 @@ int mom_ctx_destroy(mom_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->qp.release();
   ctx->scratch.release();
+  ctx->panel_t.release();
   release_blas(ctx);
   for (int k = 0; k < 2; k++) {
     if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);

@@ -33,6 +33,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -252,6 +253,178 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
     }
 }
 
+// ---------------------------------------------------------------------------
+// Trailing update C -= A B for complex64 with K = 32 or 256 (the inner-panel
+// and outer-block depths), fed by direct global->LDS transfers:
+//  - B (K x N, column-major) is first transposed into Bt[k][c] (row stride
+//    ldbt, zero-padded to a multiple of the tile width), so that k-rows of
+//    both operands are contiguous;
+//  - each stage holds BK k-rows of the A tile (128 rows) and the B tile
+//    (64 columns) as interleaved complex; every wave moves whole 1 KB
+//    pieces with global_load_lds_dwordx4 (16 bytes per lane), three stages
+//    in flight, one barrier per stage.  The K loop is fully unrolled, so
+//    every stage buffer is a distinct LDS array and the only waits on the
+//    transfers are the explicit vmcnt ones;
+//  - 4 waves in 2 x 2, wave tile 64 x 32 complex = two 32x32 blocks of
+//    v_mfma_f32_32x32x2f32 per product, three products per complex product
+//    (3M), fragments of the next k-step read while the current MFMAs run.
+// Rows of A beyond M are clamped to M - 2 (M even): they only feed output
+// rows that are not stored.
+constexpr int kDmaBK = 8;
+constexpr int kDmaBM = 128, kDmaBN = 64;
+
+__global__ __launch_bounds__(256) void transpose_panel_kernel(const float2 *__restrict__ B, int64_t ldb, int K,
+                                                              int N, float2 *__restrict__ Bt, int64_t ldbt) {
+  __shared__ float2 t[32][33];
+  const int c0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int c = c0 + r, k = k0 + tx;
+    t[r][tx] = (c < N && k < K) ? B[k + (int64_t)c * ldb] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int k = k0 + r, c = c0 + tx;
+    if (k < K && c < ldbt) Bt[(int64_t)k * ldbt + c] = t[tx][r];
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  // s_waitcnt vmcnt(N); expcnt and lgkmcnt left unconstrained
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+#define MOM_LDS(p) ((__attribute__((address_space(3))) void *)(p))
+#define MOM_COMPILER_FENCE() asm volatile("" ::: "memory")
+
+template <int NK>
+__global__ __launch_bounds__(256) void gemm3m_dma_kernel(const float2 *__restrict__ A, int64_t lda,
+                                                         const float2 *__restrict__ Bt, int64_t ldbt,
+                                                         float2 *__restrict__ Cm, int64_t ldc, int M, int N, int gm,
+                                                         int gn) {
+  constexpr int BK = kDmaBK, BM = kDmaBM, BN = kDmaBN, TM = 2, TN = 1;
+  constexpr int NA = BK * BM / 128, NB = BK * BN / 128;  // 1 KB transfers per stage
+  static_assert(NA % 4 == 0 && NB % 4 == 0, "transfers per stage must split over 4 waves");
+  constexpr int PER_WAVE = NA / 4 + NB / 4;
+  typedef float acc16 __attribute__((ext_vector_type(16)));
+  __shared__ float2 As0[BK][BM], As1[BK][BM], As2[BK][BM];
+  __shared__ float2 Bs0[BK][BN], Bs1[BK][BN], Bs2[BK][BN];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l32 = lane & 31, kh = lane >> 5;
+  const int wr = (wave & 1) * 32 * TM, wc = (wave >> 1) * 32 * TN;
+  const int pid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int per_group = kGroupM * gn;
+  const int first_m = (pid / per_group) * kGroupM;
+  const int gsize = min(gm - first_m, kGroupM);
+  const int pm = first_m + (pid % per_group) % gsize;
+  const int pn = (pid % per_group) / gsize;
+  const int64_t row0 = (int64_t)pm * BM, col0 = (int64_t)pn * BN;
+  // transfer t of this wave covers stage element e = 128 (wave + 4 t) + 2 lane
+  const float2 *pa[NA / 4];
+  const float2 *pb[NB / 4];
+#pragma unroll
+  for (int t = 0; t < NA / 4; t++) {
+    const int e = 128 * (wave + 4 * t) + 2 * lane, k = e / BM, r = e % BM;
+    int64_t gr = row0 + r;
+    if (gr > M - 2) gr = M - 2;
+    pa[t] = A + gr + (int64_t)k * lda;
+  }
+#pragma unroll
+  for (int t = 0; t < NB / 4; t++) {
+    const int e = 128 * (wave + 4 * t) + 2 * lane, k = e / BN, c = e % BN;
+    pb[t] = Bt + col0 + c + (int64_t)k * ldbt;
+  }
+  const int64_t astep = (int64_t)BK * lda, bstep = (int64_t)BK * ldbt;
+  auto issue = [&](int c, float2(*as)[BM], float2(*bs)[BN]) {
+#pragma unroll
+    for (int t = 0; t < NA / 4; t++)
+      __builtin_amdgcn_global_load_lds((const void *)(pa[t] + c * astep), MOM_LDS(&as[0][0] + 128 * (wave + 4 * t)),
+                                       16, 0, 0);
+#pragma unroll
+    for (int t = 0; t < NB / 4; t++)
+      __builtin_amdgcn_global_load_lds((const void *)(pb[t] + c * bstep), MOM_LDS(&bs[0][0] + 128 * (wave + 4 * t)),
+                                       16, 0, 0);
+  };
+  auto stageA = [&](int st) -> float2(*)[BM] { return st == 0 ? As0 : (st == 1 ? As1 : As2); };
+  auto stageB = [&](int st) -> float2(*)[BN] { return st == 0 ? Bs0 : (st == 1 ? Bs1 : Bs2); };
+  acc16 acc0[TM][TN], acc1[TM][TN], acc2[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        acc0[i][j][v] = 0.f;
+        acc1[i][j][v] = 0.f;
+        acc2[i][j][v] = 0.f;
+      }
+  issue(0, As0, Bs0);
+  if (NK > 1) issue(1, As1, Bs1);
+#pragma unroll
+  for (int it = 0; it < NK; it++) {
+    // this wave's transfers of stage `it` have landed; the barrier makes
+    // everyone's visible and retires all reads of the stage refilled next
+    if (it + 1 < NK)
+      wait_vmcnt<PER_WAVE>();
+    else
+      wait_vmcnt<0>();
+    MOM_COMPILER_FENCE();
+    __builtin_amdgcn_s_barrier();
+    MOM_COMPILER_FENCE();
+    if (it + 2 < NK) issue(it + 2, stageA((it + 2) % 3), stageB((it + 2) % 3));
+    float2(*as)[BM] = stageA(it % 3);
+    float2(*bs)[BN] = stageB(it % 3);
+    float2 fa[2][TM], fb[2][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++) fa[0][i] = as[kh][wr + 32 * i + l32];
+#pragma unroll
+    for (int j = 0; j < TN; j++) fb[0][j] = bs[kh][wc + 32 * j + l32];
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 2) {
+      const int cur = (kk / 2) & 1;
+      if (kk + 2 < BK) {
+#pragma unroll
+        for (int i = 0; i < TM; i++) fa[cur ^ 1][i] = as[kk + 2 + kh][wr + 32 * i + l32];
+#pragma unroll
+        for (int j = 0; j < TN; j++) fb[cur ^ 1][j] = bs[kk + 2 + kh][wc + 32 * j + l32];
+      }
+      float aS[TM], bS[TN];
+#pragma unroll
+      for (int i = 0; i < TM; i++) aS[i] = fa[cur][i].x + fa[cur][i].y;
+#pragma unroll
+      for (int j = 0; j < TN; j++) bS[j] = fb[cur][j].x + fb[cur][j].y;
+#pragma unroll
+      for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+          acc0[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[cur][i].x, fb[cur][j].x, acc0[i][j], 0, 0, 0);
+          acc1[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[cur][i].y, fb[cur][j].y, acc1[i][j], 0, 0, 0);
+          acc2[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(aS[i], bS[j], acc2[i][j], 0, 0, 0);
+        }
+    }
+  }
+  // 32x32x2 output: accumulator v of lane l is row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      const int64_t col = col0 + wc + 32 * j + l32;
+      if (col >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 16; v++) {
+        const int64_t row = row0 + wr + 32 * i + 8 * (v / 4) + 4 * kh + (v % 4);
+        if (row < M) {
+          float2 *p = Cm + row + col * ldc;
+          float2 c = *p;
+          c.x -= acc0[i][j][v] - acc1[i][j][v];
+          c.y -= acc2[i][j][v] - acc0[i][j][v] - acc1[i][j][v];
+          *p = c;
+        }
+      }
+    }
+}
+
 // Tile configuration (wave tile 16 WM x 16 WN complex, K chunk BK); the
 // MOM_GEMM_* macros let a build choose another configuration.
 #ifndef MOM_GEMM_F32_WM
@@ -362,7 +535,8 @@ void release_blas(mom_ctx *ctx) {
 }
 
 // C -= A B on the context's stream.  complex64: the hand-written 3M
-// matrix-core kernel above.  complex128: rocBLAS ZGEMM (the plain library
+// matrix-core kernels above (the direct-transfer kernel for K = 32 and 256,
+// the register-staged one otherwise).  complex128: rocBLAS ZGEMM (the plain library
 // GEMM; its FP64 matrix-core kernels are the strongest available baseline
 // for the all-FP64 path).
 template <typename T>
@@ -383,11 +557,30 @@ static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, co
         reinterpret_cast<rocblas_double_complex *>(C), (rocblas_int)ldc);
     if (rs != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_zgemm failed: " + std::to_string((int)rs));
   } else {
-    constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
-    const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
-    hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn),
-                       dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn);
-    MOM_HIP(hipGetLastError());
+    const bool dma = (K == 256 || K == 32) && M >= 2 && M % 2 == 0 && lda % 2 == 0 &&
+                     (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+    if (dma) {
+      const int gm = (int)((M + kDmaBM - 1) / kDmaBM), gn = (int)((N + kDmaBN - 1) / kDmaBN);
+      const int64_t ldbt = (int64_t)gn * kDmaBN;
+      const size_t bytes = sizeof(float2) * (size_t)K * ldbt;
+      if (ctx->panel_t.bytes < bytes) MOM_TRY(ctx->panel_t.alloc(bytes));
+      float2 *Bt = ctx->panel_t.as<float2>();
+      hipLaunchKernelGGL(transpose_panel_kernel, dim3((unsigned)(ldbt / 32), (unsigned)((K + 31) / 32)), dim3(256), 0,
+                         s, B, ldb, (int)K, (int)N, Bt, ldbt);
+      if (K == 256)
+        hipLaunchKernelGGL(gemm3m_dma_kernel<256 / kDmaBK>, dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt, ldbt,
+                           C, ldc, (int)M, (int)N, gm, gn);
+      else
+        hipLaunchKernelGGL(gemm3m_dma_kernel<32 / kDmaBK>, dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt, ldbt,
+                           C, ldc, (int)M, (int)N, gm, gn);
+      MOM_HIP(hipGetLastError());
+    } else {
+      constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
+      const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
+      hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn),
+                         dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn);
+      MOM_HIP(hipGetLastError());
+    }
   }
   return t.done();
 }

@@ -75,6 +75,7 @@ struct mom_ctx {
   hipStream_t stream = nullptr;
   mom::DevBuf qp;       // quadrature points of the last mesh/rule (scratch)
   mom::DevBuf scratch;  // general scratch
+  mom::DevBuf panel_t;  // transposed U12 panel of the complex64 trailing update (lu.hip)
   void *stage[2] = {nullptr, nullptr};   // pinned staging buffers
   void *blas = nullptr;                  // rocBLAS handle on `stream` (lu.hip)
   hipEvent_t stage_ev[2] = {nullptr, nullptr};

@@ -40,7 +40,8 @@ def test_mfma_lane_layout(gpu_ctx, prec):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (16, 64, 4), (100, 70, 45), (300, 257, 256), (513, 130, 32)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (16, 64, 4), (100, 70, 45), (300, 257, 256), (513, 130, 32),
+                                   (1000, 900, 256), (64, 700, 32), (2, 5, 256), (258, 1, 32)])
 def test_gemm_sub(gpu_ctx, prec, shape):
     M, N, K = shape
     rng = np.random.default_rng(M * 1000 + N + K)
