@@ -790,6 +790,251 @@ __global__ __launch_bounds__(256) void panel_column_kernel(typename Cx<T>::type 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Persistent factorisation of one inner block (ib <= IB columns starting at
+// i0) in a single cooperative launch.  Workgroup g owns the 256 row
+// positions i0 + 256 g .. and keeps their ib values in registers for the
+// whole block.  Per column the workgroups exchange data through tagged
+// 64-bit words, (round << 32) | 32-bit payload, written and polled with
+// relaxed device-scope atomics: a reader accepts a word only when it
+// carries the current round, so no fences are needed, and the two slot
+// parities keep round r+2 from overwriting data of round r before every
+// workgroup has finished with it.  Per column j:
+//   1. every workgroup publishes its candidate (largest |re|+|im| over its
+//      positions >= j, lowest row on ties) with that row's ib values, and
+//      the owner of position j publishes row j;
+//   2. every workgroup reads all candidates and picks the same pivot p;
+//   3. it reads the pivot row (and, in p's owner, old row j), swaps in
+//      registers and applies the scaling and rank-1 update to its rows > j.
+// Every wait is bounded: a poll that exceeds kSpinLimit records a timeout
+// in sync.err and the workgroup leaves together (checked by the host).
+typedef unsigned long long u64;
+constexpr int kRowWords = 4 * IB;                 // ib complex128 values as 32-bit words
+constexpr int kCandWords = 3 + kRowWords;         // magnitude (1 or 2 words), row index, row values
+constexpr long kSpinLimit = 1L << 22;
+constexpr int kPanelMaxGroups = 256;              // candidate slots (rows <= 256 * 256 per launch)
+
+struct PanelSync {
+  u64 *cand;  // [2][gmax][kCandWords]
+  u64 *diag;  // [2][kRowWords]
+  int gmax;
+  int *err;
+};
+
+__device__ __forceinline__ void put_word(u64 *w, unsigned tag, unsigned payload) {
+  __hip_atomic_store(w, ((u64)tag << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Polls up to NW words at once (null pointers are skipped) until each
+// carries `tag`: the loads of one pass are issued back to back, so a pass
+// costs one memory round trip.  False on timeout (after recording it).
+template <int NW>
+__device__ __forceinline__ bool poll_words(u64 *const (&w)[NW], unsigned tag, unsigned (&out)[NW], int *err) {
+  bool got[NW];
+#pragma unroll
+  for (int q = 0; q < NW; q++) got[q] = (w[q] == nullptr);
+  for (long spins = 0;; spins++) {
+    u64 x[NW];
+#pragma unroll
+    for (int q = 0; q < NW; q++)
+      if (!got[q]) x[q] = __hip_atomic_load(w[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool all = true;
+#pragma unroll
+    for (int q = 0; q < NW; q++) {
+      if (!got[q] && (unsigned)(x[q] >> 32) == tag) {
+        out[q] = (unsigned)x[q];
+        got[q] = true;
+      }
+      all = all && got[q];
+    }
+    if (all) return true;
+    if ((spins & 255) == 255 &&
+        (spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      atomicExch(err, 1);
+      return false;
+    }
+  }
+}
+
+// 32-bit words of one complex value of precision T
+template <typename T> struct Words;
+template <> struct Words<float> {
+  static constexpr int N = 2, V = 1;  // words per complex value, per pivot magnitude
+  static __device__ __forceinline__ unsigned mag(double m, int q) { return __float_as_uint((float)m); }
+  static __device__ __forceinline__ double mag_of(const unsigned *w) { return (double)__uint_as_float(w[0]); }
+  static __device__ __forceinline__ unsigned get(float2 v, int q) { return __float_as_uint(q ? v.y : v.x); }
+  static __device__ __forceinline__ float2 make(const unsigned *w) {
+    return make_float2(__uint_as_float(w[0]), __uint_as_float(w[1]));
+  }
+};
+template <> struct Words<double> {
+  static constexpr int N = 4, V = 2;
+  static __device__ __forceinline__ unsigned mag(double m, int q) {
+    return q ? (unsigned)__double2hiint(m) : (unsigned)__double2loint(m);
+  }
+  static __device__ __forceinline__ double mag_of(const unsigned *w) { return __hiloint2double((int)w[1], (int)w[0]); }
+  static __device__ __forceinline__ unsigned get(double2 v, int q) {
+    const double d = (q >> 1) ? v.y : v.x;
+    return (q & 1) ? (unsigned)__double2hiint(d) : (unsigned)__double2loint(d);
+  }
+  static __device__ __forceinline__ double2 make(const unsigned *w) {
+    return make_double2(__hiloint2double((int)w[1], (int)w[0]), __hiloint2double((int)w[3], (int)w[2]));
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *A, int64_t ld, int i0, int ib,
+                                                          int n, int *ipiv, int *info, PanelSync sync,
+                                                          unsigned tag0) {
+  using C2 = typename Cx<T>::type;
+  constexpr int WN = Words<T>::N, VW = Words<T>::V;
+  __shared__ unsigned uw[IB * WN], djw[IB * WN];    // pivot row / old row j as 32-bit words
+  __shared__ unsigned pubw[IB * WN], pdw[IB * WN];  // rows this workgroup publishes
+  __shared__ double sv[4];
+  __shared__ int si[4], sw[4];
+  __shared__ int fail;
+  const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+  const int r = i0 + 256 * g + tid;
+  const bool own = r < n;
+  C2 v[IB];
+#pragma unroll
+  for (int s = 0; s < IB; s++) {
+    v[s].x = 0;
+    v[s].y = 0;
+    if (own && s < ib) v[s] = A[(int64_t)(i0 + s) * ld + r];
+  }
+  if (tid == 0) fail = 0;
+  __syncthreads();
+  for (int k = 0; k < ib; k++) {
+    const int j = i0 + k;
+    const unsigned tag = tag0 + (unsigned)k;
+    const int par = tag & 1;
+    u64 *cand = sync.cand + (size_t)par * sync.gmax * kCandWords;
+    u64 *diag = sync.diag + (size_t)par * kRowWords;
+    // 1. local candidate and publication.  The magnitude of a complex64
+    //    entry is exact in float, so that path sends one word for it.
+    double val = -1.0;
+    int idx = n, win = tid;
+    if (own && r >= j) {
+#pragma unroll
+      for (int s = 0; s < IB; s++)
+        if (s == k) val = (double)cabs1<T>(v[s]);
+      idx = r;
+    }
+    block_argmax(val, idx, win, sv, si, sw);
+    // rows to publish go through LDS so that one lane stores each word
+    if (tid == win && idx < n) {
+#pragma unroll
+      for (int s = 0; s < IB; s++)
+        if (s < ib)
+#pragma unroll
+          for (int q = 0; q < WN; q++) pubw[WN * s + q] = Words<T>::get(v[s], q);
+    }
+    if (own && r == j) {  // position j lies in workgroup 0
+#pragma unroll
+      for (int s = 0; s < IB; s++)
+        if (s < ib)
+#pragma unroll
+          for (int q = 0; q < WN; q++) pdw[WN * s + q] = Words<T>::get(v[s], q);
+    }
+    __syncthreads();
+    const int nw = WN * ib;
+    {
+      // value and row index every round (a workgroup without rows >= j
+      // offers -1, which never wins); row values only for a real row
+      u64 *slot = cand + (size_t)g * kCandWords;
+      if (tid < VW) put_word(slot + tid, tag, Words<T>::mag(val, tid));
+      if (tid == VW) put_word(slot + VW, tag, (unsigned)idx);
+      if (idx < n && tid < nw) put_word(slot + VW + 1 + tid, tag, pubw[tid]);
+      if (g == 0 && tid < nw) put_word(diag + tid, tag, pdw[tid]);
+    }
+    // 2. every workgroup reduces all candidates to the same pivot
+    double bv = -1.0;
+    int bi = n, bw = 0;
+    if (tid < G) {
+      u64 *slot = cand + (size_t)tid * kCandWords;
+      u64 *const ws[3] = {slot, VW > 1 ? slot + 1 : nullptr, slot + VW};
+      unsigned got[3];
+      if (!poll_words<3>(ws, tag, got, sync.err)) {
+        fail = 1;
+      } else {
+        const unsigned mw[2] = {got[0], got[1]};
+        bv = Words<T>::mag_of(mw);
+        bi = (int)got[2];
+        bw = tid;
+      }
+    }
+    block_argmax(bv, bi, bw, sv, si, sw);  // contains barriers: `fail` is visible after it
+    if (fail) return;
+    const int p = (bi < n) ? bi : j;
+    const bool zero = !(bv > 0.0);
+    // 3. pivot row (and old row j for p's owner) into LDS: whole rows, the
+    //    multipliers of columns i0..j-1 move with the interchange as well
+    if (tid < nw) {
+      const bool need_dj = p != j && (p - i0) / 256 == g;
+      u64 *const ws[2] = {cand + (size_t)bw * kCandWords + VW + 1 + tid, need_dj ? diag + tid : nullptr};
+      unsigned got[2];
+      if (!poll_words<2>(ws, tag, got, sync.err)) {
+        fail = 1;
+      } else {
+        uw[tid] = got[0];
+        if (need_dj) djw[tid] = got[1];
+      }
+    }
+    __syncthreads();
+    if (fail) return;
+    if (g == 0 && tid == 0) {
+      ipiv[j] = p + 1;
+      if (zero) atomicCAS(info, 0, j + 1);
+    }
+    if (own && r == j) {
+#pragma unroll
+      for (int s = 0; s < IB; s++)
+        if (s < ib) v[s] = Words<T>::make(uw + WN * s);
+    } else if (own && p != j && r == p) {
+#pragma unroll
+      for (int s = 0; s < IB; s++)
+        if (s < ib) v[s] = Words<T>::make(djw + WN * s);
+    }
+    if (own && r > j) {
+      C2 l;
+      l.x = 0;
+      l.y = 0;
+#pragma unroll
+      for (int s = 0; s < IB; s++)
+        if (s == k) l = v[s];
+      const C2 piv = Words<T>::make(uw + WN * k);
+      if (piv.x != 0 || piv.y != 0) {
+        const T a = piv.x, bb = piv.y;
+        C2 q;
+        if (fabs(a) >= fabs(bb)) {
+          const T rr = bb / a, den = a + bb * rr;
+          q.x = (l.x + l.y * rr) / den;
+          q.y = (l.y - l.x * rr) / den;
+        } else {
+          const T rr = a / bb, den = a * rr + bb;
+          q.x = (l.x * rr + l.y) / den;
+          q.y = (l.y * rr - l.x) / den;
+        }
+        l = q;
+      }
+#pragma unroll
+      for (int s = 0; s < IB; s++) {
+        if (s == k) v[s] = l;
+        if (s > k && s < ib) {
+          const C2 uu = Words<T>::make(uw + WN * s);
+          v[s].x -= l.x * uu.x - l.y * uu.y;
+          v[s].y -= l.x * uu.y + l.y * uu.x;
+        }
+      }
+    }
+    __syncthreads();  // uw / djw are rewritten in the next round
+  }
+#pragma unroll
+  for (int s = 0; s < IB; s++)
+    if (own && s < ib) A[(int64_t)(i0 + s) * ld + r] = v[s];
+}
+
 // Row interchanges k1..k2-1 (ipiv 1-based, k2 - k1 <= NB) as one gather.
 // The plan kernel replays the interchanges on slot indices: slot s < jb is
 // row k1 + s, slot jb + k is row p_k for an interchange k whose pivot row
@@ -963,17 +1208,51 @@ static int trsm_block(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t
   return 0;
 }
 
+// Cooperative inner-block panel factorisation: synchronisation words, the
+// largest co-resident grid of panel_block_kernel<T> (0: not available) and
+// the next free round tag.
+struct PanelCoop {
+  PanelSync sync;
+  int gmax = 0;
+  unsigned *tag = nullptr;
+};
+
 // LU with partial pivoting of the n x n matrix A (ld), in place.
 template <typename T>
 static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, int *ipiv, PivotWork w,
-                 PhaseClock *st) {
+                 PhaseClock *st, PanelCoop coop = PanelCoop()) {
   hipStream_t s = ctx->stream;
   for (int64_t j0 = 0; j0 < n; j0 += NB) {
     const int64_t jb = std::min<int64_t>(NB, n - j0);
     for (int64_t i0 = j0; i0 < j0 + jb; i0 += IB) {
       const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
-      // candidates of the block's first column, then one launch per column
       Timed tp(st, s, kPhPanel);
+      const int64_t G = (n - i0 + 255) / 256;
+      if (G <= coop.gmax) {
+        // the whole inner block in one cooperative launch, then its row
+        // interchanges on the other columns of the panel
+        unsigned tag0 = *coop.tag;
+        *coop.tag += 64;
+        int i0i = (int)i0, ibi = (int)ib, ni = (int)n;
+        int *infop = w.info;
+        typename Cx<T>::type *Ap = A;
+        int64_t ldv = ld;
+        int *ipivp = ipiv;
+        PanelSync sy = coop.sync;
+        void *args[] = {&Ap, &ldv, &i0i, &ibi, &ni, &ipivp, &infop, &sy, &tag0};
+        MOM_HIP(hipLaunchCooperativeKernel((const void *)panel_block_kernel<T>, dim3((unsigned)G), dim3(256), args, 0,
+                                           s));
+        const int64_t left = i0 - j0, right = j0 + jb - (i0 + ib);
+        if (left + right > 0) {
+          hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), w.plan);
+          MOM_HIP(hipGetLastError());
+          const int64_t groups = (left + right + kLaswpCols - 1) / kLaswpCols;
+          hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, s,
+                             A, ld, w.plan, (int)j0, (int)i0, (int)(i0 + ib), (int)(j0 + jb));
+          MOM_HIP(hipGetLastError());
+        }
+      } else {
+      // candidates of the block's first column, then one launch per column
       hipLaunchKernelGGL(pivot_candidates_kernel<T>, dim3((unsigned)((n - i0 + 255) / 256)), dim3(256), 0, s, A, ld,
                          (int)i0, (int)n, (int)i0, (int)(i0 + ib), w, (int)(i0 & 1));
       MOM_HIP(hipGetLastError());
@@ -986,6 +1265,7 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
                            (int)i0, (int)(i0 + ib), (int)j0, (int)(j0 + jb), ipiv, w, nb_in, fuse);
         MOM_HIP(hipGetLastError());
         nb_in = nblk;
+      }
       }
       MOM_TRY(tp.done());
       const int64_t rest = j0 + jb - (i0 + ib);
@@ -1350,6 +1630,11 @@ struct mom_system {
   double t_fill = 0, t_factor = 0, t_solve = 0;
   mom::PhaseClock clock;
   int info = 0;
+  // cooperative panel factorisation: synchronisation words (zeroed when
+  // allocated), next round tag, co-resident grid limit per precision
+  mom::DevBuf panel_sync;
+  unsigned panel_tag = 1;
+  int coop_gmax[2] = {-1, -1};
 };
 
 namespace mom {
@@ -1373,6 +1658,44 @@ static int ensure_pivwork(mom_system *S, PivotWork *w) {
   return 0;
 }
 
+static int g_panel_mode = 0;  // 0: cooperative inner blocks when they fit, 1: one launch per column
+
+// Sets up the cooperative panel path for precision T: the synchronisation
+// buffer (allocated once per system and zeroed) and the largest grid whose
+// workgroups are all resident at once.
+template <typename T>
+static int panel_coop(mom_system *S, PanelCoop *c) {
+  const int pi = std::is_same<T, double>::value ? 1 : 0;
+  if (S->coop_gmax[pi] < 0) {
+    int dev = 0, coop = 0, nb = 0;
+    MOM_HIP(hipGetDevice(&dev));
+    MOM_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    hipDeviceProp_t prop;
+    MOM_HIP(hipGetDeviceProperties(&prop, dev));
+    MOM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)panel_block_kernel<T>, 256, 0));
+    S->coop_gmax[pi] = coop ? std::min(kPanelMaxGroups, nb * prop.multiProcessorCount) : 0;
+  }
+  c->gmax = g_panel_mode == 1 ? 0 : S->coop_gmax[pi];
+  if (c->gmax == 0) return 0;
+  const size_t cand = sizeof(u64) * 2 * kPanelMaxGroups * kCandWords, diag = sizeof(u64) * 2 * kRowWords;
+  if (!S->panel_sync.ptr) {
+    MOM_TRY(S->panel_sync.alloc(cand + diag + 64));
+    MOM_HIP(hipMemsetAsync(S->panel_sync.ptr, 0, cand + diag + 64, S->ctx->stream));
+    S->panel_tag = 1;
+  }
+  char *base = S->panel_sync.as<char>();
+  c->sync.cand = (u64 *)base;
+  c->sync.diag = (u64 *)(base + cand);
+  c->sync.err = (int *)(base + cand + diag);
+  c->sync.gmax = kPanelMaxGroups;
+  if (S->panel_tag > 0xF0000000u) {  // tags are about to wrap: start over from clean words
+    MOM_HIP(hipMemsetAsync(S->panel_sync.ptr, 0, cand + diag, S->ctx->stream));
+    S->panel_tag = 1;
+  }
+  c->tag = &S->panel_tag;
+  return 0;
+}
+
 // Factor the n x n matrix held by the system (complex128 in place, or the
 // scaled complex64 copy), build the permutation and diagonal-block inverses.
 template <typename T>
@@ -1383,7 +1706,9 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
   int rc;
   if ((rc = ensure_pivwork(S, &w))) return rc;
   if (S->clock.enabled) S->clock.reset();
-  if ((rc = getrf<T>(S->ctx, A, n, n, S->ipiv.as<int>(), w, &S->clock))) return rc;
+  PanelCoop coop;
+  if ((rc = panel_coop<T>(S, &coop))) return rc;
+  if ((rc = getrf<T>(S->ctx, A, n, n, S->ipiv.as<int>(), w, &S->clock, coop))) return rc;
   const int64_t nblk = (n + NB - 1) / NB;
   const size_t inv_bytes = sizeof(typename Cx<T>::type) * nblk * NB * NB;
   if (S->Linv.bytes < inv_bytes && ((rc = S->Linv.alloc(inv_bytes)) || (rc = S->Uinv.alloc(inv_bytes)))) return rc;
@@ -1393,6 +1718,11 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
   int info = 0;
   MOM_TRY(copy_d2h(S->ctx, ip.data(), S->ipiv.ptr, sizeof(int) * n));
   MOM_TRY(copy_d2h(S->ctx, &info, w.info, sizeof(int)));
+  if (coop.gmax > 0) {
+    int timed_out = 0;
+    MOM_TRY(copy_d2h(S->ctx, &timed_out, coop.sync.err, sizeof(int)));
+    if (timed_out) return fail(MOM_ERR_HIP, "panel factorisation: workgroup exchange timed out");
+  }
   if (S->clock.enabled && (rc = S->clock.collect())) return rc;
   S->perm.resize(n);
   for (int64_t i = 0; i < n; i++) S->perm[i] = (int)i;
@@ -1800,6 +2130,12 @@ extern "C" int mom_test_mfma_layout(mom_ctx *ctx, int32_t precision, double *out
     hipLaunchKernelGGL(mfma_probe_kernel<float>, dim3(4096), dim3(64), 0, ctx->stream, d.as<double>());
   MOM_HIP(hipGetLastError());
   MOM_TRY(copy_d2h(ctx, out, d.ptr, cnt * sizeof(double)));
+  return 0;
+}
+
+extern "C" int mom_test_panel_mode(int32_t mode) {
+  if (mode != 0 && mode != 1) return fail(MOM_ERR_ARG, "panel mode must be 0 or 1");
+  g_panel_mode = mode;
   return 0;
 }
 

@@ -34,6 +34,14 @@ int mom_test_mfma_layout(mom_ctx *ctx, int32_t precision, double *out);
 int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int64_t N, int64_t K, const double *A,
                       int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc);
 
+/*
+ * Panel factorisation path of mom_system_factor and mom_solve_multi_linsys:
+ * 0 (default) factors each inner block of 32 columns in one cooperative
+ * launch when its workgroups fit on the device at once, 1 always uses one
+ * launch per column.  Process-wide; lets the tests check both paths.
+ */
+int mom_test_panel_mode(int32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

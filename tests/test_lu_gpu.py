@@ -8,10 +8,19 @@ import numpy as np
 import pytest
 import scipy.linalg as sl
 
-from mom2amd import MU0, System, solve_multi_linsys
+from mom2amd import MU0, System, capi, solve_multi_linsys
 from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED, MomError
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=[0, 1], ids=["coop_panel", "column_panel"])
+def panel_mode(request):
+    """Both panel factorisation paths: one cooperative launch per inner
+    block of 32 columns, and one launch per column."""
+    capi.check(capi.lib().mom_test_panel_mode(request.param))
+    yield request.param
+    capi.check(capi.lib().mom_test_panel_mode(0))
 
 
 def rand_complex(n, m=None, seed=0, diag=0.0):
@@ -28,8 +37,8 @@ def berr(A, x, b):
     return np.abs(r).max() / (np.abs(A).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())
 
 
-@pytest.mark.parametrize("n", [7, 64, 300, 517, 1100])
-def test_lu_matches_lapack(gpu_ctx, n):
+@pytest.mark.parametrize("n", [7, 64, 300, 517, 1100, 3000])
+def test_lu_matches_lapack(gpu_ctx, panel_mode, n):
     A = rand_complex(n, seed=n)
     B = rand_complex(n, 3, seed=n + 1)
     LU, X, ipiv = solve_multi_linsys(gpu_ctx, A, B)
@@ -44,7 +53,7 @@ def test_lu_matches_lapack(gpu_ctx, n):
     assert berr(A, X[:, 0], B[:, 0]) < 1e-14
 
 
-def test_lu_zero_pivot_reports_info(gpu_ctx):
+def test_lu_zero_pivot_reports_info(gpu_ctx, panel_mode):
     A = rand_complex(40, seed=3)
     A[:, 5] = 0.0
     with pytest.raises(MomError) as e:
@@ -65,7 +74,7 @@ def test_fp64_system_solve(gpu_ctx, n2):
 
 
 @pytest.mark.parametrize("n2", [512, 1500])
-def test_mixed_system_reaches_fp64_backward_error(gpu_ctx, n2):
+def test_mixed_system_reaches_fp64_backward_error(gpu_ctx, panel_mode, n2):
     A = rand_complex(n2, seed=20 + n2)
     # rows and columns of very different scales, as the PMCHWT M-block has
     A[n2 // 2:, :] *= 1e-5
