@@ -40,13 +40,17 @@ def test_mfma_lane_layout(gpu_ctx, prec):
 
 
 @pytest.mark.parametrize("prec", [64, 32])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (16, 64, 4), (100, 70, 45), (300, 257, 256), (513, 130, 32),
-                                   (1000, 900, 256), (64, 700, 32), (2, 5, 256), (258, 1, 32)])
+# (M, N, K, pad): leading dimensions M + pad and M + pad + 2.  An even pad
+# with K = 32 or 256 takes the complex64 direct-transfer kernel, an odd pad
+# (or another K) the register-staged one.
+@pytest.mark.parametrize("shape", [(32, 32, 32, 4), (16, 64, 4, 3), (100, 70, 45, 3), (300, 257, 256, 3),
+                                   (300, 257, 256, 4), (513, 130, 32, 3), (1000, 900, 256, 4), (64, 700, 32, 4),
+                                   (2, 5, 256, 4), (258, 1, 32, 4), (4096, 4096, 256, 4)])
 def test_gemm_sub(gpu_ctx, prec, shape):
-    M, N, K = shape
+    M, N, K, pad = shape
     rng = np.random.default_rng(M * 1000 + N + K)
     cplx = lambda *s: rng.standard_normal(s) + 1j * rng.standard_normal(s)
-    lda, ldb, ldc = M + 3, K + 1, M + 5
+    lda, ldb, ldc = M + pad, K + 1, M + pad + 2
     A = cplx(lda, K)
     B = cplx(ldb, N)
     C = cplx(ldc, N)

@@ -90,3 +90,23 @@ def test_mixed_system_reaches_fp64_backward_error(gpu_ctx, panel_mode, n2):
     x_ref = np.linalg.solve(A, b)
     # forward error is bounded by cond(A) * backward error
     assert np.abs(x - x_ref).max() / np.abs(x_ref).max() < 1e-7
+
+
+def test_mixed_factor_accuracy_at_scale(gpu_ctx):
+    """The FP32 factors of an n = 8192 random system must give a first
+    solution whose backward error is that of a complex64 LU (a few 1e-6):
+    this catches stage races in the trailing-update kernels and inverse
+    rounding in the triangular solves, which leave the refinement converging
+    but from a much worse start."""
+    n2 = 8192
+    A = rand_complex(n2, seed=8192)
+    b = rand_complex(n2, 1, seed=8193)
+    S = System(gpu_ctx, n2 // 2, MOM_PREC_MIXED)
+    S.set_matrix(A)
+    S.factor()
+    S.solve(b, tol=1e-13, maxit=30)
+    hist = S.stats()["berr_history"]
+    print("n=%d mixed backward errors %s" % (n2, ["%.1e" % v for v in hist]), flush=True)
+    assert hist[0] < 1e-5
+    assert hist[1] < 1e-7
+    S.close()
