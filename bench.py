@@ -135,10 +135,10 @@ def main():
         gemm_launches += int(st["gemm_launches"])
         last = st
         log("step %d: %.2f s (fill %.2f factor %.2f solve %.2f, IR iters %d (%d factor solves, %d FP64 products), "
-            "berr %s); factor: gemm %.2f panel %.2f trsm %.2f laswp %.2f"
+            "berr %s); factor: gemm %.2f (narrow %.2f) panel %.2f trsm %.2f laswp %.2f"
             % (i, st["wall_s"], st["fill_s"], st["factor_s"], st["solve_s"], st["iters"], st["n_minv"],
-               st["n_matvec"], " ".join("%.1e" % v for v in st["berr_history"]), st["gemm_s"], st["panel_s"],
-               st["trsm_s"], st["laswp_s"]))
+               st["n_matvec"], " ".join("%.1e" % v for v in st["berr_history"]), st["gemm_s"], st["gemm_narrow_s"],
+               st["panel_s"], st["trsm_s"], st["laswp_s"]))
     mixed_s = float(np.mean(times))
 
     fp64 = None
@@ -169,7 +169,7 @@ def main():
         "phases_s": {"fill": round(last["fill_s"], 4), "factor": round(last["factor_s"], 4),
                      "solve_ir": round(last["solve_s"], 4), "ir_iters": last["iters"], "ir_factor_solves": last["n_minv"],
                      "ir_fp64_products": last["n_matvec"], "berr": last["berr"]},
-        "factor_breakdown_s": {k: round(last[k + "_s"], 4) for k in ("gemm", "panel", "trsm", "laswp")},
+        "factor_breakdown_s": {k: round(last[k + "_s"], 4) for k in ("gemm", "gemm_narrow", "panel", "trsm", "laswp")},
         "roofline": None if achieved is None else {
             "kernel": "gemm_sub_kernel<float,2,2,16,3M> (LU trailing update A22 -= L21 U12, complex64 as three real "
                       "products on v_mfma_f32_16x16x4f32); achieved = 8MNK algorithmic flops / kernel time",

@@ -28,6 +28,8 @@
 // runs one thread per near pair over the pair list built on the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 
 #include "mom_internal.hpp"
@@ -223,8 +225,9 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
 // faces; assemble_kernel then gathers them into the matrix.
 struct PlaneArgs {
   FarArgs f;
-  double2 *B;     // [2][9][S][F]
+  double2 *B;     // [2][9][S][F - mlo]
   int n0, S;      // slab of source faces [n0, n0 + S)
+  int mlo;        // first test face: the pairs with both faces below n0 were done by earlier slabs
   int mode;       // 0: D+H and K; 1: D; 2: H; 3: K
 };
 
@@ -238,7 +241,8 @@ __global__ __launch_bounds__(256) void pair_plane_kernel(PlaneArgs pa) {
   __shared__ double spts[kSrcPerBlock][3 * kMaxQ];  // source points, broadcast
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int Q = QT > 0 ? QT : a.Q, F = a.F;
-  const int m0 = blockIdx.x * 64;
+  const int m0 = pa.mlo + blockIdx.x * 64;
+  const int W = F - pa.mlo;
   const int nb = pa.n0 + blockIdx.y * kSrcPerBlock;
   for (int idx = threadIdx.x; idx < 64 * Q * 3; idx += blockDim.x) {
     const int l = idx % 64, jc = idx / 64;
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(256) void pair_plane_kernel(PlaneArgs pa) {
   const bool valid = m < F;
   const int mm = valid ? m : 0;
   const double cmx = a.cp[3 * mm], cmy = a.cp[3 * mm + 1], cmz = a.cp[3 * mm + 2];
-  const int64_t plane = (int64_t)pa.S * F;
+  const int64_t plane = (int64_t)pa.S * W;
 
   for (int t = 0; t < kSrcPerWave; t++) {
     const int tl = wave * kSrcPerWave + t;
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(256) void pair_plane_kernel(PlaneArgs pa) {
           vr = Dr + Hr;
           vi = Di + Hi;
         }
-        const int64_t off = (int64_t)(3 * p + q) * plane + (int64_t)nl * F + mm;
+        const int64_t off = (int64_t)(3 * p + q) * plane + (int64_t)nl * W + (mm - pa.mlo);
         pa.B[off] = make_double2(vr, vi);
         pa.B[9 * plane + off] = make_double2(Kr, Ki);
       }
@@ -354,52 +358,25 @@ __global__ __launch_bounds__(256) void pair_plane_kernel(PlaneArgs pa) {
   }
 }
 
-// Gathers the face-side planes of one slab into the matrix: thread
-// (row r of the output numbering, source edge e2 of the slab) sums the
-// contributions of the (<= 2) test face sides of r's edge and the (<= 2)
-// source face sides of e2 that lie in the slab, then updates the entry with
-// one read-modify-write (each entry is owned by one thread per launch).
-struct AssembleArgs {
-  const double2 *B;
-  int F, S, n0;
-  const int *row_local;   // output row -> local edge of this mesh, -1 if none
-  int64_t nrows;
-  const int *slab_edges;  // local source edges of this slab
-  const int *eside;       // [E][2] face-side codes 3f+s of each local edge, -1 if none
-  const int *col_index;   // local edge -> output column
-  FillTarget out;
-};
-
-__global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs g) {
-  const int64_t r = blockIdx.x * 256 + (int64_t)threadIdx.x;
-  if (r >= g.nrows) return;
-  const int e1 = g.row_local[r];
-  if (e1 < 0) return;
-  const int e2 = g.slab_edges[blockIdx.y];
-  const int64_t plane = (int64_t)g.S * g.F;
-  const int s1[2] = {g.eside[2 * e1], g.eside[2 * e1 + 1]};
-  const int s2[2] = {g.eside[2 * e2], g.eside[2 * e2 + 1]};
-  double ar = 0, ai = 0, kr = 0, ki = 0;
-#pragma unroll
-  for (int b = 0; b < 2; b++) {
-    if (s2[b] < 0) continue;
-    const int n = s2[b] / 3, q = s2[b] - 3 * n;
-    const int nl = n - g.n0;
-    if (nl < 0 || nl >= g.S) continue;
-#pragma unroll
-    for (int a = 0; a < 2; a++) {
-      if (s1[a] < 0) continue;
-      const int m = s1[a] / 3, p = s1[a] - 3 * m;
-      const int64_t off = (int64_t)(3 * p + q) * plane + (int64_t)nl * g.F + m;
-      const double2 v = g.B[off], k = g.B[9 * plane + off];
-      ar += v.x;
-      ai += v.y;
-      kr += k.x;
-      ki += k.y;
-    }
-  }
-  const FillTarget &o = g.out;
-  const int64_t gi = r, gj = g.col_index[e2];
+// Gathers the face-side planes of one slab into the matrix.  A slab holds
+// the blocks of the face pairs (test m >= n0, source n in [n0, n0 + S)).
+// The far and smoothed-kernel integrals are symmetric in the two faces
+// (G(r, r') = G(r', r); the D, H and K bilinear forms are symmetric under
+// exchanging the faces, the sign of grad' G cancelling against the order of
+// the triple product), so the block of (test n, source m) is the transpose
+// of that of (test m, source n), and every unordered pair is integrated
+// once.  Direct launch: thread (row r of the output numbering, source edge
+// e2 of the slab) sums the contributions of the (<= 2) test face sides of
+// r's edge with a face >= n0 and the (<= 2) source face sides of e2 that
+// lie in the slab.  Transposed launch: thread (test edge e1 of the slab,
+// source edge e2 with a face beyond the slab) adds the transposed blocks of
+// the test sides of e1 in the slab and the source sides of e2 beyond it.
+// Either updates its entry with one read-modify-write (each entry is owned
+// by one thread per launch).
+// Adds the D+H (ar, ai) and K (kr, ki) sums of edge pair (gi, gj) to the
+// output: the four PMCHWT blocks (mode 0) or the single operator.
+__device__ __forceinline__ void store_entry(const FillTarget &o, int64_t gi, int64_t gj, double ar, double ai,
+                                                 double kr, double ki) {
   if (o.mode == 0) {
     const int64_t N = o.Noff, ld = o.ld;
     double2 *z = o.Z + gi + gj * ld;
@@ -421,6 +398,119 @@ __global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs g) {
       *z = make_double2(v.x + kr, v.y + ki);
     else
       *z = make_double2(v.x + ar, v.y + ai);
+  }
+}
+
+struct AssembleArgs {
+  const double2 *B;
+  int F, S, n0, mlo;
+  const int *row_local;   // output row -> local edge of this mesh, -1 if none
+  int64_t nrows;
+  const int *slab_edges;  // local source edges of this slab
+  const int *eside;       // [E][2] face-side codes 3f+s of each local edge, -1 if none
+  const int *col_index;   // local edge -> output column
+  FillTarget out;
+};
+
+__global__ __launch_bounds__(256) void assemble_kernel(AssembleArgs g) {
+  const int64_t r = blockIdx.x * 256 + (int64_t)threadIdx.x;
+  if (r >= g.nrows) return;
+  const int e1 = g.row_local[r];
+  if (e1 < 0) return;
+  const int e2 = g.slab_edges[blockIdx.y];
+  const int W = g.F - g.mlo;
+  const int64_t plane = (int64_t)g.S * W;
+  const int s1[2] = {g.eside[2 * e1], g.eside[2 * e1 + 1]};
+  const int s2[2] = {g.eside[2 * e2], g.eside[2 * e2 + 1]};
+  double ar = 0, ai = 0, kr = 0, ki = 0;
+#pragma unroll
+  for (int b = 0; b < 2; b++) {
+    if (s2[b] < 0) continue;
+    const int n = s2[b] / 3, q = s2[b] - 3 * n;
+    const int nl = n - g.n0;
+    if (nl < 0 || nl >= g.S) continue;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      if (s1[a] < 0) continue;
+      const int m = s1[a] / 3, p = s1[a] - 3 * m;
+      if (m < g.mlo) continue;  // integrated by the slab of m, added there transposed
+      const int64_t off = (int64_t)(3 * p + q) * plane + (int64_t)nl * W + (m - g.mlo);
+      const double2 v = g.B[off], k = g.B[9 * plane + off];
+      ar += v.x;
+      ai += v.y;
+      kr += k.x;
+      ki += k.y;
+    }
+  }
+  store_entry(g.out, r, g.col_index[e2], ar, ai, kr, ki);
+}
+
+// Transposed launch, tiled: a workgroup takes kTR test edges of the slab x
+// kTC source edges beyond it.  The plane reads run along the source faces
+// (lanes over source edges), the matrix updates along the rows (lanes over
+// test edges); the sums pass through LDS between the two phases.
+constexpr int kTR = 32, kTC = 64;
+
+__global__ __launch_bounds__(256) void assemble_transposed_kernel(AssembleArgs g, const int *__restrict__ cols,
+                                                                  int ne, int nc) {
+  __shared__ double2 sa[kTR][kTC + 1], sk[kTR][kTC + 1];
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x * kTR, c0 = blockIdx.y * kTC;
+  const int W = g.F - g.mlo;
+  const int64_t plane = (int64_t)g.S * W;
+  {
+    const int c = tid % kTC, rsub = tid / kTC;  // 4 rows per pass
+    const int cc = c0 + c;
+    const int e2 = cc < nc ? cols[cc] : -1;
+    int s2[2] = {-1, -1};
+    if (e2 >= 0) {
+      s2[0] = g.eside[2 * e2];
+      s2[1] = g.eside[2 * e2 + 1];
+    }
+    for (int rr = rsub; rr < kTR; rr += 256 / kTC) {
+      const int t = r0 + rr;
+      double ar = 0, ai = 0, kr = 0, ki = 0;
+      if (t < ne && e2 >= 0) {
+        const int e1 = g.slab_edges[t];  // test edge with a side in the slab
+        const int s1[2] = {g.eside[2 * e1], g.eside[2 * e1 + 1]};
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          if (s1[a] < 0) continue;
+          const int fa = s1[a] / 3, pa = s1[a] - 3 * fa;
+          const int al = fa - g.n0;
+          if (al < 0 || al >= g.S) continue;
+#pragma unroll
+          for (int b = 0; b < 2; b++) {
+            if (s2[b] < 0) continue;
+            const int fb = s2[b] / 3, qb = s2[b] - 3 * fb;
+            if (fb < g.n0 + g.S) continue;
+            // block of (test fb, source fa), entry (side qb, side pa)
+            const int64_t off = (int64_t)(3 * qb + pa) * plane + (int64_t)al * W + (fb - g.mlo);
+            const double2 v = g.B[off], k = g.B[9 * plane + off];
+            ar += v.x;
+            ai += v.y;
+            kr += k.x;
+            ki += k.y;
+          }
+        }
+      }
+      sa[rr][c] = make_double2(ar, ai);
+      sk[rr][c] = make_double2(kr, ki);
+    }
+  }
+  __syncthreads();
+  {
+    const int rr = tid % kTR, csub = tid / kTR;  // 8 columns per pass
+    const int t = r0 + rr;
+    if (t < ne) {
+      const int64_t gi = g.col_index[g.slab_edges[t]];
+      for (int c = csub; c < kTC; c += 256 / kTR) {
+        const int cc = c0 + c;
+        if (cc >= nc) break;
+        const double2 v = sa[rr][c], k = sk[rr][c];
+        store_entry(g.out, gi, g.col_index[cols[cc]], v.x, v.y, k.x, k.y);
+      }
+    }
   }
 }
 
@@ -729,10 +819,23 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     rl[g] = (int)e;
     colidx[e] = g;
   }
-  DevBuf drl, dcol;
-  if ((rc = drl.alloc(sizeof(int) * nrows)) || (rc = dcol.alloc(sizeof(int) * m->E))) return rc;
+  // local edges by their highest face, descending: the source edges with a
+  // side beyond slab k form a prefix
+  std::vector<int> hside(2 * m->E);
+  MOM_TRY(copy_d2h(ctx, hside.data(), m->eside.ptr, sizeof(int) * 2 * m->E));
+  std::vector<int> maxface(m->E), bydesc(m->E);
+  for (int64_t e = 0; e < m->E; e++) {
+    maxface[e] = std::max(hside[2 * e] < 0 ? -1 : hside[2 * e] / 3, hside[2 * e + 1] < 0 ? -1 : hside[2 * e + 1] / 3);
+    bydesc[e] = (int)e;
+  }
+  std::stable_sort(bydesc.begin(), bydesc.end(), [&](int x, int y) { return maxface[x] > maxface[y]; });
+  DevBuf drl, dcol, ddesc;
+  if ((rc = drl.alloc(sizeof(int) * nrows)) || (rc = dcol.alloc(sizeof(int) * m->E)) ||
+      (rc = ddesc.alloc(sizeof(int) * m->E)))
+    return rc;
   MOM_TRY(copy_h2d(ctx, drl.ptr, rl.data(), sizeof(int) * nrows));
   MOM_TRY(copy_h2d(ctx, dcol.ptr, colidx.data(), sizeof(int) * m->E));
+  MOM_TRY(copy_h2d(ctx, ddesc.ptr, bydesc.data(), sizeof(int) * m->E));
   PlaneArgs pa;
   pa.f = a;
   pa.B = ctx->scratch.as<double2>();
@@ -750,8 +853,9 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   const int nslab = (int)((m->F + S - 1) / S);
   for (int k = 0; k < nslab; k++) {
     pa.n0 = k * S;
+    pa.mlo = pa.n0;  // pairs with both faces below n0 were integrated by earlier slabs
     const int scur = (int)std::min<int64_t>(S, m->F - pa.n0);
-    dim3 grid((unsigned)((m->F + 63) / 64), (unsigned)((scur + kSrcPerBlock - 1) / kSrcPerBlock));
+    dim3 grid((unsigned)((m->F - pa.mlo + 63) / 64), (unsigned)((scur + kSrcPerBlock - 1) / kSrcPerBlock));
     if (rule.Q == 13)
       hipLaunchKernelGGL(pair_plane_kernel<13>, grid, dim3(256), 0, ctx->stream, pa);
     else
@@ -760,10 +864,20 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     const int ne = m->h_slab_ptr[k + 1] - m->h_slab_ptr[k];
     if (ne > 0) {
       ga.n0 = pa.n0;
+      ga.mlo = pa.mlo;
       ga.slab_edges = m->slab_edges.as<int>() + m->h_slab_ptr[k];
       hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)((nrows + 255) / 256), (unsigned)ne), dim3(256), 0,
                          ctx->stream, ga);
       MOM_HIP(hipGetLastError());
+      // transposed blocks: test sides in the slab, source sides beyond it
+      const int beyond = pa.n0 + scur;
+      int nb = 0;
+      while (nb < (int)m->E && maxface[bydesc[nb]] >= beyond) nb++;
+      if (nb > 0) {
+        hipLaunchKernelGGL(assemble_transposed_kernel, dim3((unsigned)((ne + kTR - 1) / kTR), (unsigned)((nb + kTC - 1) / kTC)),
+                           dim3(256), 0, ctx->stream, ga, ddesc.as<int>(), ne, nb);
+        MOM_HIP(hipGetLastError());
+      }
     }
   }
   // the row/column maps must outlive the queued kernels

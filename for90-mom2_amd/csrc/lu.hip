@@ -490,7 +490,7 @@ template <> struct GemmCfg<float> {
 // interchanges.  Pairs live in a fixed ring; when it is full the host waits
 // for the oldest pairs and folds them into the totals, so the number of
 // live events stays bounded.
-enum Phase { kPhGemm = 0, kPhPanel, kPhTrsm, kPhLaswp, kPhases };
+enum Phase { kPhGemm = 0, kPhPanel, kPhTrsm, kPhLaswp, kPhGemmNarrow, kPhases };
 
 struct PhaseClock {
   static constexpr int kRing = 1024;  // event pairs
@@ -534,7 +534,7 @@ struct PhaseClock {
       MOM_HIP(hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]));
       seconds[phase_of[i]] += ms * 1e-3;
       launches[phase_of[i]] += 1;
-      if (phase_of[i] == kPhGemm) gemm_flops += flops_of[i];
+      if (phase_of[i] == kPhGemm || phase_of[i] == kPhGemmNarrow) gemm_flops += flops_of[i];
     }
     used = 0;
     return 0;
@@ -590,7 +590,9 @@ static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, co
                     PhaseClock *st = nullptr) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   hipStream_t s = ctx->stream;
-  Timed t(st, s, kPhGemm, 8.0 * (double)M * (double)N * (double)K);
+  // updates of depth NB are the trailing updates; shallower ones (inner
+  // panel, triangular-solve blocks) are timed separately
+  Timed t(st, s, K >= NB ? kPhGemm : kPhGemmNarrow, 8.0 * (double)M * (double)N * (double)K);
   if constexpr (std::is_same<T, double>::value) {
     rocblas_handle h;
     MOM_TRY(blas_handle(ctx, &h));
@@ -2114,8 +2116,8 @@ int mom_system_stats(const mom_system *S, double *out) {
   out[0] = S->t_fill;
   out[1] = S->t_factor;
   out[2] = S->t_solve;
-  out[3] = S->clock.seconds[mom::kPhGemm];
-  out[4] = S->clock.launches[mom::kPhGemm];
+  out[3] = S->clock.seconds[mom::kPhGemm] + S->clock.seconds[mom::kPhGemmNarrow];
+  out[4] = S->clock.launches[mom::kPhGemm] + S->clock.launches[mom::kPhGemmNarrow];
   out[5] = S->clock.gemm_flops;
   out[6] = S->clock.seconds[mom::kPhPanel];
   out[7] = S->clock.seconds[mom::kPhTrsm];
@@ -2123,6 +2125,7 @@ int mom_system_stats(const mom_system *S, double *out) {
   out[9] = S->n_minv;
   out[10] = S->n_matvec;
   for (int i = 0; i < mom::kBeHistory; i++) out[11 + i] = S->be_hist[i];
+  out[19] = S->clock.seconds[mom::kPhGemmNarrow];
   return 0;
 }
 
