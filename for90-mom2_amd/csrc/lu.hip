@@ -969,31 +969,39 @@ __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *
       idx = r;
     }
     block_argmax(val, idx, win, sv, si, sw);
-    // rows to publish go through LDS so that one lane stores each word
-    if (tid == win && idx < n) {
-#pragma unroll
-      for (int s = 0; s < IB; s++)
-        if (s < ib)
-#pragma unroll
-          for (int q = 0; q < WN; q++) pubw[WN * s + q] = Words<T>::get(v[s], q);
-    }
-    if (own && r == j) {  // position j lies in workgroup 0
-#pragma unroll
-      for (int s = 0; s < IB; s++)
-        if (s < ib)
-#pragma unroll
-          for (int q = 0; q < WN; q++) pdw[WN * s + q] = Words<T>::get(v[s], q);
-    }
-    __syncthreads();
     const int nw = WN * ib;
-    {
-      // value and row index every round (a workgroup without rows >= j
-      // offers -1, which never wins); row values only for a real row
-      u64 *slot = cand + (size_t)g * kCandWords;
-      if (tid < VW) put_word(slot + tid, tag, Words<T>::mag(val, tid));
-      if (tid == VW) put_word(slot + VW, tag, (unsigned)idx);
-      if (idx < n && tid < nw) put_word(slot + VW + 1 + tid, tag, pubw[tid]);
-      if (g == 0 && tid < nw) put_word(diag + tid, tag, pdw[tid]);
+    u64 *slot = cand + (size_t)g * kCandWords;
+    // value and row index every round (a workgroup without rows >= j
+    // offers -1, which never wins)
+    if (tid < VW) put_word(slot + tid, tag, Words<T>::mag(val, tid));
+    if (tid == VW) put_word(slot + VW, tag, (unsigned)idx);
+    // rows to publish go through LDS within the owning wave, so that its
+    // lanes store one or two words each without a workgroup barrier
+    if (idx < n && (tid >> 6) == (win >> 6)) {
+      if (tid == win) {
+#pragma unroll
+        for (int s = 0; s < IB; s++)
+          if (s < ib)
+#pragma unroll
+            for (int q = 0; q < WN; q++) pubw[WN * s + q] = Words<T>::get(v[s], q);
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int w = tid & 63; w < nw; w += 64) put_word(slot + VW + 1 + w, tag, pubw[w]);
+    }
+    if (g == 0 && (tid >> 6) == ((j - i0) >> 6)) {  // position j lies in workgroup 0
+      if (r == j) {
+#pragma unroll
+        for (int s = 0; s < IB; s++)
+          if (s < ib)
+#pragma unroll
+            for (int q = 0; q < WN; q++) pdw[WN * s + q] = Words<T>::get(v[s], q);
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int w = tid & 63; w < nw; w += 64) put_word(diag + w, tag, pdw[w]);
     }
     // 2. every workgroup reduces all candidates to the same pivot
     double bv = -1.0;
@@ -1075,7 +1083,8 @@ __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *
         }
       }
     }
-    __syncthreads();  // uw / djw are rewritten in the next round
+    // uw / djw are next rewritten after the two barriers of the next
+    // round's candidate reduction, which order these reads before that
   }
 #pragma unroll
   for (int s = 0; s < IB; s++)
