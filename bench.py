@@ -170,9 +170,12 @@ def main():
                      "solve_ir": round(last["solve_s"], 4), "ir_iters": last["iters"], "ir_factor_solves": last["n_minv"],
                      "ir_fp64_products": last["n_matvec"], "berr": last["berr"]},
         "factor_breakdown_s": {k: round(last[k + "_s"], 4) for k in ("gemm", "gemm_narrow", "panel", "trsm", "laswp")},
+        "gemm_narrow": {"launches": last["gemm_narrow_launches"],
+                        "tflops": round(last["gemm_narrow_flops"] / max(last["gemm_narrow_s"], 1e-12) / 1e12, 2)},
         "roofline": None if achieved is None else {
-            "kernel": "gemm_sub_kernel<float,2,2,16,3M> (LU trailing update A22 -= L21 U12, complex64 as three real "
-                      "products on v_mfma_f32_16x16x4f32); achieved = 8MNK algorithmic flops / kernel time",
+            "kernel": "gemm3m_dma_kernel<32> (LU trailing update A22 -= L21 U12 of depth 256, complex64 as three "
+                      "real products on v_mfma_f32_32x32x2f32); achieved = 8MNK algorithmic flops / kernel time "
+                      "(HIP events around each launch)",
             "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_MFMA, 4), "traffic": None,
             "mfma_issued_frac": round(achieved * 6 / 8 / PEAK_FP32_MFMA, 4),

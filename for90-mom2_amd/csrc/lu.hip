@@ -499,12 +499,10 @@ struct PhaseClock {
   int used = 0;
   int phase_of[kRing];
   double flops_of[kRing];
-  double seconds[kPhases] = {}, launches[kPhases] = {};
-  double gemm_flops = 0;
+  double seconds[kPhases] = {}, launches[kPhases] = {}, flops[kPhases] = {};
   void reset() {
     used = 0;
-    for (int p = 0; p < kPhases; p++) seconds[p] = launches[p] = 0;
-    gemm_flops = 0;
+    for (int p = 0; p < kPhases; p++) seconds[p] = launches[p] = flops[p] = 0;
   }
   int begin(hipStream_t s) {
     if (pool.empty()) {
@@ -534,7 +532,7 @@ struct PhaseClock {
       MOM_HIP(hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]));
       seconds[phase_of[i]] += ms * 1e-3;
       launches[phase_of[i]] += 1;
-      if (phase_of[i] == kPhGemm || phase_of[i] == kPhGemmNarrow) gemm_flops += flops_of[i];
+      flops[phase_of[i]] += flops_of[i];
     }
     used = 0;
     return 0;
@@ -591,9 +589,13 @@ static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, co
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   hipStream_t s = ctx->stream;
   // updates of depth NB are the trailing updates; shallower ones (inner
-  // panel, triangular-solve blocks) are timed separately
-  Timed t(st, s, K >= NB ? kPhGemm : kPhGemmNarrow, 8.0 * (double)M * (double)N * (double)K);
+  // panel, triangular-solve blocks) are timed separately.  The bracket holds
+  // the GEMM kernel alone (the panel transpose is timed with the narrow
+  // group of its own).
+  const Phase ph = K >= NB ? kPhGemm : kPhGemmNarrow;
+  const double fl = 8.0 * (double)M * (double)N * (double)K;
   if constexpr (std::is_same<T, double>::value) {
+    Timed t(st, s, ph, fl);
     rocblas_handle h;
     MOM_TRY(blas_handle(ctx, &h));
     const rocblas_double_complex alpha(-1.0, 0.0), beta(1.0, 0.0);
@@ -603,6 +605,7 @@ static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, co
         reinterpret_cast<const rocblas_double_complex *>(B), (rocblas_int)ldb, &beta,
         reinterpret_cast<rocblas_double_complex *>(C), (rocblas_int)ldc);
     if (rs != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_zgemm failed: " + std::to_string((int)rs));
+    return t.done();
   } else {
     const bool dma = (K == 256 || K == 32) && M >= 2 && M % 2 == 0 && lda % 2 == 0 && ldc % 2 == 0 &&
                      (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
@@ -612,8 +615,13 @@ static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, co
       const size_t bytes = sizeof(float2) * (size_t)K * ldbt;
       if (ctx->panel_t.bytes < bytes) MOM_TRY(ctx->panel_t.alloc(bytes));
       float2 *Bt = ctx->panel_t.as<float2>();
-      hipLaunchKernelGGL(transpose_panel_kernel, dim3((unsigned)(ldbt / 32), (unsigned)((K + 31) / 32)), dim3(256), 0,
-                         s, B, ldb, (int)K, (int)N, Bt, ldbt);
+      {
+        Timed tt(st, s, kPhGemmNarrow, 0.0);
+        hipLaunchKernelGGL(transpose_panel_kernel, dim3((unsigned)(ldbt / 32), (unsigned)((K + 31) / 32)), dim3(256),
+                           0, s, B, ldb, (int)K, (int)N, Bt, ldbt);
+        MOM_TRY(tt.done());
+      }
+      Timed t(st, s, ph, fl);
       if (K == 256)
         hipLaunchKernelGGL(gemm3m_dma_kernel<256 / kDmaBK>, dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt, ldbt,
                            C, ldc, (int)M, (int)N, gm, gn);
@@ -621,15 +629,17 @@ static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, co
         hipLaunchKernelGGL(gemm3m_dma_kernel<32 / kDmaBK>, dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt, ldbt,
                            C, ldc, (int)M, (int)N, gm, gn);
       MOM_HIP(hipGetLastError());
+      return t.done();
     } else {
+      Timed t(st, s, ph, fl);
       constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
       const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
       hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn),
                          dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn);
       MOM_HIP(hipGetLastError());
+      return t.done();
     }
   }
-  return t.done();
 }
 
 // ---------------------------------------------------------------------------
@@ -2125,9 +2135,9 @@ int mom_system_stats(const mom_system *S, double *out) {
   out[0] = S->t_fill;
   out[1] = S->t_factor;
   out[2] = S->t_solve;
-  out[3] = S->clock.seconds[mom::kPhGemm] + S->clock.seconds[mom::kPhGemmNarrow];
-  out[4] = S->clock.launches[mom::kPhGemm] + S->clock.launches[mom::kPhGemmNarrow];
-  out[5] = S->clock.gemm_flops;
+  out[3] = S->clock.seconds[mom::kPhGemm];
+  out[4] = S->clock.launches[mom::kPhGemm];
+  out[5] = S->clock.flops[mom::kPhGemm];
   out[6] = S->clock.seconds[mom::kPhPanel];
   out[7] = S->clock.seconds[mom::kPhTrsm];
   out[8] = S->clock.seconds[mom::kPhLaswp];
@@ -2135,6 +2145,8 @@ int mom_system_stats(const mom_system *S, double *out) {
   out[10] = S->n_matvec;
   for (int i = 0; i < mom::kBeHistory; i++) out[11 + i] = S->be_hist[i];
   out[19] = S->clock.seconds[mom::kPhGemmNarrow];
+  out[20] = S->clock.launches[mom::kPhGemmNarrow];
+  out[21] = S->clock.flops[mom::kPhGemmNarrow];
   return 0;
 }
 

@@ -152,17 +152,19 @@ int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, dou
 int mom_system_profile(mom_system *sys, int32_t enable);
 /*
  * Wall times of the last fill / factor / solve [s] and, with profiling
- * enabled, the factorisation's kernel time by group (20 doubles):
- * out[0] fill, out[1] factor, out[2] solve, out[3] trailing-update GEMM
- * kernel time [s], out[4] number of those GEMM launches, out[5] their
- * algorithmic flops (8 M N K each), out[6] panel factorisation (pivot
- * search, row swaps in the panel, column updates) [s], out[7] triangular
- * solves of the U blocks [s], out[8] row interchanges outside the panels [s],
+ * enabled, the factorisation's kernel time by group (22 doubles):
+ * out[0] fill, out[1] factor, out[2] solve;
+ * out[3] kernel time of the trailing updates (depth NB = 256) [s], out[4]
+ * their number of launches, out[5] their algorithmic flops (8 M N K each);
+ * out[6] panel factorisation (pivot search, row swaps in the panel, column
+ * updates) [s], out[7] triangular solves of the U blocks [s], out[8] row
+ * interchanges outside the panels [s];
  * out[9] applications of the FP32 factors and out[10] FP64 matrix-vector
  * products in the last mixed-precision refinement, out[11..18] its backward
- * error after outer steps 0..7 (-1 where no step was taken), out[19] the
- * part of out[3] spent in updates shallower than the block depth (inner
- * panel and triangular-solve blocks) [s].
+ * error after outer steps 0..7 (-1 where no step was taken);
+ * out[19] time of the shallower updates (inner panel and triangular-solve
+ * blocks, depth 32) with their panel transposes [s], out[20] their number
+ * of launches, out[21] their algorithmic flops.
  */
 int mom_system_stats(const mom_system *sys, double *out);
 
