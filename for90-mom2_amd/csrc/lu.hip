@@ -487,14 +487,17 @@ template <> struct GemmCfg<float> {
 // Event-pair timing of the factorisation's kernel groups (enabled per
 // system): trailing-update GEMM launches (with their algorithmic flops),
 // the panel column kernels of each inner block, triangular solves and row
-// interchanges.  Pairs live in a fixed ring; when it is full the host waits
-// for the oldest pairs and folds them into the totals, so the number of
-// live events stays bounded.
+// interchanges.  Level 1 times the depth-NB trailing updates only (one pair
+// per block column, no host wait inside a factorisation of up to kRing
+// block columns); level 2 times every group.  Pairs live in a fixed ring;
+// when it is full the host waits for the oldest pairs and folds them into
+// the totals, so the number of live events stays bounded.
 enum Phase { kPhGemm = 0, kPhPanel, kPhTrsm, kPhLaswp, kPhGemmNarrow, kPhases };
 
 struct PhaseClock {
   static constexpr int kRing = 1024;  // event pairs
   bool enabled = false;
+  bool all_phases = false;
   std::vector<hipEvent_t> pool;
   int used = 0;
   int phase_of[kRing];
@@ -550,7 +553,8 @@ struct Timed {
   Phase ph;
   double fl;
   int rc = 0;
-  Timed(PhaseClock *c_, hipStream_t s_, Phase ph_, double fl_ = 0) : c(c_ && c_->enabled ? c_ : nullptr), s(s_), ph(ph_), fl(fl_) {
+  Timed(PhaseClock *c_, hipStream_t s_, Phase ph_, double fl_ = 0)
+      : c(c_ && c_->enabled && (c_->all_phases || ph_ == kPhGemm) ? c_ : nullptr), s(s_), ph(ph_), fl(fl_) {
     if (c) rc = c->begin(s);
   }
   int done() { return (c && !rc) ? c->end(s, ph, fl) : rc; }
@@ -2002,7 +2006,9 @@ int mom_system_destroy(mom_system *sys) {
 
 int mom_system_profile(mom_system *sys, int32_t enable) {
   if (!sys) return fail(MOM_ERR_ARG, "NULL argument");
+  if (enable < 0 || enable > 2) return fail(MOM_ERR_ARG, "profile level must be 0, 1 or 2");
   sys->clock.enabled = enable != 0;
+  sys->clock.all_phases = enable == 2;
   return 0;
 }
 

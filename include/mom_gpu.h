@@ -147,8 +147,9 @@ int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, dou
                         double ri_im, int64_t nsrc, const double *angles, int32_t nq,
                         const double *qw, const double *qnodes, double tol, int32_t maxit,
                         double *X, int64_t ldx, int32_t *iters, double *berr);
-/* Enable (1) or disable (0) event timing of the kernel groups of
- * mom_system_factor (reported by mom_system_stats). */
+/* Event timing of the kernel groups of mom_system_factor (reported by
+ * mom_system_stats): 0 off, 1 the depth-256 trailing updates only (cheap
+ * enough for timed runs), 2 every group (adds host work per launch). */
 int mom_system_profile(mom_system *sys, int32_t enable);
 /*
  * Wall times of the last fill / factor / solve [s] and, with profiling
