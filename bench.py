@@ -183,11 +183,14 @@ def main():
         "gemm_narrow": {"launches": bd["gemm_narrow_launches"],
                         "tflops": round(bd["gemm_narrow_flops"] / max(bd["gemm_narrow_s"], 1e-12) / 1e12, 2)},
         "roofline": None if achieved is None else {
-            "kernel": "gemm3m_dma_kernel<32> (LU trailing update A22 -= L21 U12 of depth 256, complex64 as three "
-                      "real products on v_mfma_f32_32x32x2f32); achieved = 8MNK algorithmic flops / kernel time "
-                      "(HIP events around each launch)",
+            "kernel": "gemm3m_dma_kernel<32> (LU trailing update A22 -= L21 U12 of depth 256: the next block "
+                      "column on the look-ahead stream and the remaining columns on the main stream; complex64 as "
+                      "three real products on v_mfma_f32_32x32x2f32); achieved = 8MNK algorithmic flops / kernel "
+                      "time (HIP events around each launch)",
             "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_MFMA, 4),
+            "frac_note": "the 3M product issues 6MNK real flops for the 8MNK of a complex product, so the 8MNK rate "
+                         "may exceed the FP32 matrix peak; matrix-core utilisation is mfma_issued_frac",
             "traffic": TRAFFIC_S5_BYTES if args.subdiv == 5 else None, "traffic_unit": "bytes/launch",
             "mfma_issued_frac": round(achieved * 6 / 8 / PEAK_FP32_MFMA, 4),
             "launches": gemm_launches, "avg_launch_ms": round(gemm_s / max(gemm_launches, 1) * 1e3, 4)},

@@ -234,8 +234,13 @@ int mom_ctx_destroy(mom_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->qp.release();
   ctx->scratch.release();
+  if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
   ctx->panel_t.release();
+  ctx->panel_t2.release();
   release_blas(ctx);
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+  if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
   for (int k = 0; k < 2; k++) {
     if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
     if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);

@@ -529,9 +529,9 @@ struct PhaseClock {
   // waits for the recorded pairs and adds them to the totals
   int collect() {
     if (used == 0) return 0;
-    MOM_HIP(hipEventSynchronize(pool[2 * used - 1]));
     for (int i = 0; i < used; i++) {
       float ms = 0;
+      MOM_HIP(hipEventSynchronize(pool[2 * i + 1]));  // pairs may sit on either stream
       MOM_HIP(hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]));
       seconds[phase_of[i]] += ms * 1e-3;
       launches[phase_of[i]] += 1;
@@ -560,38 +560,60 @@ struct Timed {
   int done() { return (c && !rc) ? c->end(s, ph, fl) : rc; }
 };
 
-// rocBLAS handle of a context, created on first use and bound to the
-// context's stream.
-static int blas_handle(mom_ctx *ctx, rocblas_handle *h) {
-  if (!ctx->blas) {
+// A stream of the factorisation with the scratch its GEMMs use: the
+// transposed-panel buffer of the complex64 kernels and a rocBLAS handle
+// bound to the stream (created on first use).  Two lanes never share
+// scratch, so their launches may overlap.
+struct Lane {
+  hipStream_t s;
+  DevBuf *bt;
+  void **blas;
+};
+
+static Lane main_lane(mom_ctx *ctx) { return Lane{ctx->stream, &ctx->panel_t, &ctx->blas}; }
+
+// The look-ahead lane: second stream (default priority) and its events.
+static int aux_lane(mom_ctx *ctx, Lane *l) {
+  if (!ctx->aux) {
+    MOM_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    MOM_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    MOM_HIP(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+  }
+  *l = Lane{ctx->aux, &ctx->panel_t2, &ctx->blas2};
+  return 0;
+}
+
+static int blas_handle(const Lane &L, rocblas_handle *h) {
+  if (!*L.blas) {
     rocblas_handle b = nullptr;
     if (rocblas_create_handle(&b) != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_create_handle failed");
-    if (rocblas_set_stream(b, ctx->stream) != rocblas_status_success) {
+    if (rocblas_set_stream(b, L.s) != rocblas_status_success) {
       rocblas_destroy_handle(b);
       return fail(MOM_ERR_HIP, "rocblas_set_stream failed");
     }
-    ctx->blas = b;
+    *L.blas = b;
   }
-  *h = static_cast<rocblas_handle>(ctx->blas);
+  *h = static_cast<rocblas_handle>(*L.blas);
   return 0;
 }
 
 void release_blas(mom_ctx *ctx) {
   if (ctx->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
-  ctx->blas = nullptr;
+  if (ctx->blas2) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas2));
+  ctx->blas = ctx->blas2 = nullptr;
 }
 
-// C -= A B on the context's stream.  complex64: the hand-written 3M
+// C -= A B on lane L.  complex64: the hand-written 3M
 // matrix-core kernels above (the direct-transfer kernel for K = 32 and 256,
 // the register-staged one otherwise).  complex128: rocBLAS ZGEMM (the plain library
 // GEMM; its FP64 matrix-core kernels are the strongest available baseline
 // for the all-FP64 path).
 template <typename T>
-static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
+static int gemm_sub(const Lane &L, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
                     int64_t ldb, typename Cx<T>::type *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
                     PhaseClock *st = nullptr) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = L.s;
   // updates of depth NB are the trailing updates; shallower ones (inner
   // panel, triangular-solve blocks) are timed separately.  The bracket holds
   // the GEMM kernel alone (the panel transpose is timed with the narrow
@@ -601,7 +623,7 @@ static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, co
   if constexpr (std::is_same<T, double>::value) {
     Timed t(st, s, ph, fl);
     rocblas_handle h;
-    MOM_TRY(blas_handle(ctx, &h));
+    MOM_TRY(blas_handle(L, &h));
     const rocblas_double_complex alpha(-1.0, 0.0), beta(1.0, 0.0);
     const rocblas_status rs = rocblas_zgemm(
         h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &alpha,
@@ -617,8 +639,8 @@ static int gemm_sub(mom_ctx *ctx, const typename Cx<T>::type *A, int64_t lda, co
       const int gm = (int)((M + kDmaBM - 1) / kDmaBM), gn = (int)((N + kDmaBN - 1) / kDmaBN);
       const int64_t ldbt = (int64_t)gn * kDmaBN;
       const size_t bytes = sizeof(float2) * (size_t)K * ldbt;
-      if (ctx->panel_t.bytes < bytes) MOM_TRY(ctx->panel_t.alloc(bytes));
-      float2 *Bt = ctx->panel_t.as<float2>();
+      if (L.bt->bytes < bytes) MOM_TRY(L.bt->alloc(bytes));
+      float2 *Bt = L.bt->as<float2>();
       {
         Timed tt(st, s, kPhGemmNarrow, 0.0);
         hipLaunchKernelGGL(transpose_panel_kernel, dim3((unsigned)(ldbt / 32), (unsigned)((K + 31) / 32)), dim3(256),
@@ -1255,10 +1277,10 @@ static unsigned trsm_grid(int64_t ncols) {
 // processed in IB-row slices: small triangle then MFMA update of the rows
 // below within the block.
 template <typename T>
-static int trsm_block(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t r0, int64_t w, int64_t c0,
+static int trsm_block(const Lane &L, typename Cx<T>::type *A, int64_t ld, int64_t r0, int64_t w, int64_t c0,
                       int64_t ncols, PhaseClock *st) {
   if (ncols <= 0) return 0;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = L.s;
   for (int64_t b = 0; b < w; b += IB) {
     const int bw = (int)std::min<int64_t>(IB, w - b);
     {
@@ -1270,7 +1292,7 @@ static int trsm_block(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t
     }
     const int64_t below = w - b - bw;
     if (below > 0) {
-      int rc = gemm_sub<T>(ctx, A + (r0 + b) * ld + r0 + b + bw, ld, A + c0 * ld + r0 + b, ld,
+      int rc = gemm_sub<T>(L, A + (r0 + b) * ld + r0 + b + bw, ld, A + c0 * ld + r0 + b, ld,
                            A + c0 * ld + r0 + b + bw, ld, below, ncols, bw, st);
       if (rc) return rc;
     }
@@ -1287,41 +1309,43 @@ struct PanelCoop {
   unsigned *tag = nullptr;
 };
 
-// LU with partial pivoting of the n x n matrix A (ld), in place.
+// Factorisation of the panel of block columns [j0, j0+jb) over rows
+// [j0, n) on lane L: inner blocks of IB columns, each factored (one
+// cooperative launch, or one launch per column), its row interchanges
+// carried across the other panel columns, then the U block and update of
+// the panel's remaining columns.
 template <typename T>
-static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, int *ipiv, PivotWork w,
-                 PhaseClock *st, PanelCoop coop = PanelCoop()) {
-  hipStream_t s = ctx->stream;
-  for (int64_t j0 = 0; j0 < n; j0 += NB) {
-    const int64_t jb = std::min<int64_t>(NB, n - j0);
-    for (int64_t i0 = j0; i0 < j0 + jb; i0 += IB) {
-      const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
-      Timed tp(st, s, kPhPanel);
-      const int64_t G = (n - i0 + 255) / 256;
-      if (G <= coop.gmax) {
-        // the whole inner block in one cooperative launch, then its row
-        // interchanges on the other columns of the panel
-        unsigned tag0 = *coop.tag;
-        *coop.tag += 64;
-        int i0i = (int)i0, ibi = (int)ib, ni = (int)n;
-        int *infop = w.info;
-        typename Cx<T>::type *Ap = A;
-        int64_t ldv = ld;
-        int *ipivp = ipiv;
-        PanelSync sy = coop.sync;
-        void *args[] = {&Ap, &ldv, &i0i, &ibi, &ni, &ipivp, &infop, &sy, &tag0};
-        MOM_HIP(hipLaunchCooperativeKernel((const void *)panel_block_kernel<T>, dim3((unsigned)G), dim3(256), args, 0,
-                                           s));
-        const int64_t left = i0 - j0, right = j0 + jb - (i0 + ib);
-        if (left + right > 0) {
-          hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), w.plan);
-          MOM_HIP(hipGetLastError());
-          const int64_t groups = (left + right + kLaswpCols - 1) / kLaswpCols;
-          hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, s,
-                             A, ld, w.plan, (int)j0, (int)i0, (int)(i0 + ib), (int)(j0 + jb));
-          MOM_HIP(hipGetLastError());
-        }
-      } else {
+static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int64_t n, int64_t j0, int64_t jb,
+                        int *ipiv, PivotWork w, PhaseClock *st, const PanelCoop &coop) {
+  hipStream_t s = L.s;
+  for (int64_t i0 = j0; i0 < j0 + jb; i0 += IB) {
+    const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
+    Timed tp(st, s, kPhPanel);
+    const int64_t G = (n - i0 + 255) / 256;
+    if (G <= coop.gmax) {
+      // the whole inner block in one cooperative launch, then its row
+      // interchanges on the other columns of the panel
+      unsigned tag0 = *coop.tag;
+      *coop.tag += 64;
+      int i0i = (int)i0, ibi = (int)ib, ni = (int)n;
+      int *infop = w.info;
+      typename Cx<T>::type *Ap = A;
+      int64_t ldv = ld;
+      int *ipivp = ipiv;
+      PanelSync sy = coop.sync;
+      void *args[] = {&Ap, &ldv, &i0i, &ibi, &ni, &ipivp, &infop, &sy, &tag0};
+      MOM_HIP(hipLaunchCooperativeKernel((const void *)panel_block_kernel<T>, dim3((unsigned)G), dim3(256), args, 0,
+                                         s));
+      const int64_t left = i0 - j0, right = j0 + jb - (i0 + ib);
+      if (left + right > 0) {
+        hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), w.plan);
+        MOM_HIP(hipGetLastError());
+        const int64_t groups = (left + right + kLaswpCols - 1) / kLaswpCols;
+        hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, s,
+                           A, ld, w.plan, (int)j0, (int)i0, (int)(i0 + ib), (int)(j0 + jb));
+        MOM_HIP(hipGetLastError());
+      }
+    } else {
       // candidates of the block's first column, then one launch per column
       hipLaunchKernelGGL(pivot_candidates_kernel<T>, dim3((unsigned)((n - i0 + 255) / 256)), dim3(256), 0, s, A, ld,
                          (int)i0, (int)n, (int)i0, (int)(i0 + ib), w, (int)(i0 & 1));
@@ -1336,21 +1360,47 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
         MOM_HIP(hipGetLastError());
         nb_in = nblk;
       }
-      }
-      MOM_TRY(tp.done());
-      const int64_t rest = j0 + jb - (i0 + ib);
-      if (rest > 0) {
-        // U block of the remaining panel columns, then their update
-        Timed tt(st, s, kPhTrsm);
-        hipLaunchKernelGGL(trsm_unit_lower_kernel<T>, dim3(trsm_grid(rest)), dim3(256), 0, s, A, ld, i0, (int)ib,
-                           i0 + ib, rest);
-        MOM_HIP(hipGetLastError());
-        MOM_TRY(tt.done());
-        int rc = gemm_sub<T>(ctx, A + i0 * ld + i0 + ib, ld, A + (i0 + ib) * ld + i0, ld,
-                             A + (i0 + ib) * ld + i0 + ib, ld, n - i0 - ib, rest, ib, st);
-        if (rc) return rc;
-      }
     }
+    MOM_TRY(tp.done());
+    const int64_t rest = j0 + jb - (i0 + ib);
+    if (rest > 0) {
+      // U block of the remaining panel columns, then their update
+      Timed tt(st, s, kPhTrsm);
+      hipLaunchKernelGGL(trsm_unit_lower_kernel<T>, dim3(trsm_grid(rest)), dim3(256), 0, s, A, ld, i0, (int)ib,
+                         i0 + ib, rest);
+      MOM_HIP(hipGetLastError());
+      MOM_TRY(tt.done());
+      int rc = gemm_sub<T>(L, A + i0 * ld + i0 + ib, ld, A + (i0 + ib) * ld + i0, ld, A + (i0 + ib) * ld + i0 + ib,
+                           ld, n - i0 - ib, rest, ib, st);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+// Look-ahead of the blocked LU (on unless switched off by the test hook).
+static int g_lookahead = 1;
+
+// LU with partial pivoting of the n x n matrix A (ld), in place, blocked
+// right-looking with look-ahead of depth one.  Step k, once panel k is
+// factored: its row interchanges on the columns left and right of it and the
+// U12 row block on the main stream; then the trailing update splits in two.
+// The next block column is updated on the look-ahead stream, which goes on
+// to factor panel k+1 there, while the main stream updates the remaining
+// columns; the main stream waits for panel k+1 before step k+1.  The two
+// streams touch disjoint columns between fork and join.
+template <typename T>
+static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, int *ipiv, PivotWork w,
+                 PhaseClock *st, PanelCoop coop = PanelCoop()) {
+  const Lane M = main_lane(ctx);
+  hipStream_t s = M.s;
+  Lane X = M;
+  const bool la = g_lookahead != 0 && n > NB;
+  if (la) MOM_TRY(aux_lane(ctx, &X));
+  if (la) MOM_TRY(factor_panel<T>(M, A, ld, n, 0, std::min<int64_t>(NB, n), ipiv, w, st, coop));
+  for (int64_t j0 = 0; j0 < n; j0 += NB) {
+    const int64_t jb = std::min<int64_t>(NB, n - j0);
+    if (!la) MOM_TRY(factor_panel<T>(M, A, ld, n, j0, jb, ipiv, w, st, coop));
     // row interchanges of this panel on the columns left and right of it
     const int64_t left = j0, right = n - (j0 + jb);
     if (left + right > 0) {
@@ -1363,13 +1413,23 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
       MOM_HIP(hipGetLastError());
       MOM_TRY(tl.done());
     }
-    if (right > 0) {
-      int rc = trsm_block<T>(ctx, A, ld, j0, jb, j0 + jb, right, st);
-      if (rc) return rc;
-      rc = gemm_sub<T>(ctx, A + j0 * ld + j0 + jb, ld, A + (j0 + jb) * ld + j0, ld, A + (j0 + jb) * ld + j0 + jb, ld,
-                       n - j0 - jb, right, jb, st);
-      if (rc) return rc;
+    if (right <= 0) continue;
+    MOM_TRY(trsm_block<T>(M, A, ld, j0, jb, j0 + jb, right, st));
+    const int64_t r0 = j0 + jb, rows = n - r0;
+    const typename Cx<T>::type *L21 = A + j0 * ld + r0, *U12 = A + r0 * ld + j0;
+    typename Cx<T>::type *A22 = A + r0 * ld + r0;
+    if (!la) {
+      MOM_TRY(gemm_sub<T>(M, L21, ld, U12, ld, A22, ld, rows, right, jb, st));
+      continue;
     }
+    const int64_t nxt = std::min<int64_t>(NB, right);
+    MOM_HIP(hipEventRecord(ctx->fork_ev, s));
+    MOM_HIP(hipStreamWaitEvent(X.s, ctx->fork_ev, 0));
+    MOM_TRY(gemm_sub<T>(X, L21, ld, U12, ld, A22, ld, rows, nxt, jb, st));
+    MOM_TRY(factor_panel<T>(X, A, ld, n, r0, nxt, ipiv, w, st, coop));
+    MOM_HIP(hipEventRecord(ctx->join_ev, X.s));
+    MOM_TRY(gemm_sub<T>(M, L21, ld, U12 + nxt * ld, ld, A22 + nxt * ld, ld, rows, right - nxt, jb, st));
+    MOM_HIP(hipStreamWaitEvent(s, ctx->join_ev, 0));
   }
   return 0;
 }
@@ -2236,6 +2296,12 @@ extern "C" int mom_test_panel_mode(int32_t mode) {
   return 0;
 }
 
+extern "C" int mom_test_lookahead(int32_t enable) {
+  if (enable != 0 && enable != 1) return fail(MOM_ERR_ARG, "look-ahead switch must be 0 or 1");
+  g_lookahead = enable;
+  return 0;
+}
+
 extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int64_t N, int64_t K, const double *A,
                                  int64_t lda, const double *B, int64_t ldb, double *C, int64_t ldc) {
   if (!ctx || !A || !B || !C) return fail(MOM_ERR_ARG, "NULL argument");
@@ -2250,7 +2316,7 @@ extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int
   MOM_TRY(copy_h2d(ctx, dB.ptr, B, sizeof(double2) * ldb * N));
   MOM_TRY(copy_h2d(ctx, dC.ptr, C, sizeof(double2) * ldc * N));
   if (precision == 64) {
-    if ((rc = gemm_sub<double>(ctx, dA.as<double2>(), lda, dB.as<double2>(), ldb, dC.as<double2>(), ldc, M, N, K)))
+    if ((rc = gemm_sub<double>(main_lane(ctx), dA.as<double2>(), lda, dB.as<double2>(), ldb, dC.as<double2>(), ldc, M, N, K)))
       return rc;
   } else {
     DevBuf fA, fB, fC;
@@ -2264,7 +2330,7 @@ extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int
     nar(dA, fA, lda * K);
     nar(dB, fB, ldb * N);
     nar(dC, fC, ldc * N);
-    if ((rc = gemm_sub<float>(ctx, fA.as<float2>(), lda, fB.as<float2>(), ldb, fC.as<float2>(), ldc, M, N, K)))
+    if ((rc = gemm_sub<float>(main_lane(ctx), fA.as<float2>(), lda, fB.as<float2>(), ldb, fC.as<float2>(), ldc, M, N, K)))
       return rc;
     hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((ldc * N + 255) / 256)), dim3(256), 0, s, fC.as<float2>(),
                        dC.as<double2>(), ldc * N);

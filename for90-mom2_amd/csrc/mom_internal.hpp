@@ -79,6 +79,13 @@ struct mom_ctx {
   void *stage[2] = {nullptr, nullptr};   // pinned staging buffers
   void *blas = nullptr;                  // rocBLAS handle on `stream` (lu.hip)
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  // second stream of the LU look-ahead (panel k+1 factored while the
+  // trailing update of step k runs on `stream`), created on first use, with
+  // its own GEMM scratch and rocBLAS handle; fork/join events between the two
+  hipStream_t aux = nullptr;
+  mom::DevBuf panel_t2;
+  void *blas2 = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 // Device copy of one domain mesh, with per-face-side RWG data precomputed.

@@ -42,6 +42,13 @@ int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int64_t N, int
  */
 int mom_test_panel_mode(int32_t mode);
 
+/*
+ * Look-ahead of the blocked LU: 1 (default) factors panel k+1 on a second
+ * stream while the trailing update of step k runs, 0 runs every step on
+ * one stream.  Process-wide; lets the tests check both schedules.
+ */
+int mom_test_lookahead(int32_t enable);
+
 #ifdef __cplusplus
 }
 #endif
