@@ -30,10 +30,11 @@ import numpy as np  # noqa: E402
 METRIC = "fill+LU+solve s @N=61440: FP32-MFMA+IR vs FP64 GPU path, σ rel err≤1e-8"
 PEAK_FP32_MFMA = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 PEAK_FP64_MFMA = 78.6
-# HBM bytes per launch of the depth-256 trailing update at N = 61440 (mean over its 239 launches):
-# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a 1 GiB elementwise copy.
-# Recorded in profiles/gemm3m_dma_kernel32_N61440_hbm_traffic.md; counters cannot be read in-process.
-TRAFFIC_S5_BYTES = 26.57e9
+# HBM bytes per launch of the depth-256 trailing update at N = 61440 (mean over its 477 launches per
+# solve, next-column and remaining-columns launches together): rocprofv3 FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, the factors read off a 1 GiB elementwise copy.  Recorded in
+# profiles/gemm3m_dma_kernel32_N61440_hbm_traffic.md; counters cannot be read in-process.
+TRAFFIC_S5_BYTES = 13.32e9
 
 
 def log(msg):
