@@ -237,6 +237,7 @@ int mom_ctx_destroy(mom_ctx *ctx) {
   if (ctx->aux) (void)hipStreamSynchronize(ctx->aux);
   ctx->panel_t.release();
   ctx->panel_t2.release();
+  ctx->ublk.release();
   release_blas(ctx);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);

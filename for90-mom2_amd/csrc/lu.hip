@@ -310,7 +310,9 @@ __device__ __forceinline__ void wait_stage() {
 #define MOM_LDS(p) ((__attribute__((address_space(3))) void *)(p))
 #define MOM_COMPILER_FENCE() asm volatile("" ::: "memory")
 
-template <int NK>
+// Role only names the launch site in profiles (0: updates, 1: the product
+// that forms a U block from the inverse of the diagonal L block).
+template <int NK, int Role = 0>
 __global__ __launch_bounds__(256) void gemm3m_dma_kernel(const float2 *__restrict__ A, int64_t lda,
                                                          const float2 *__restrict__ Bt, int64_t ldbt,
                                                          float2 *__restrict__ Cm, int64_t ldc, int M, int N, int gm,
@@ -608,18 +610,29 @@ void release_blas(mom_ctx *ctx) {
 // the register-staged one otherwise).  complex128: rocBLAS ZGEMM (the plain library
 // GEMM; its FP64 matrix-core kernels are the strongest available baseline
 // for the all-FP64 path).
+// Whether the complex64 direct-transfer kernel takes C -= A B of these
+// shapes (it reads B through a transposed copy, so B may alias C).
+static bool dma_gemm_ok(int64_t M, int64_t K, int64_t lda, int64_t ldc, const void *A, const void *C) {
+  return (K == 256 || K == 32) && M >= 2 && M % 2 == 0 && lda % 2 == 0 && ldc % 2 == 0 &&
+         (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
+}
+
+// u_solve: the product forms a U block (U12 = A12 - E A12, E from
+// unit_lower_inverse_kernel); it is timed with the triangular solves and
+// launched as its own kernel instance so profiles keep it apart from the
+// updates.
 template <typename T>
 static int gemm_sub(const Lane &L, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
                     int64_t ldb, typename Cx<T>::type *C, int64_t ldc, int64_t M, int64_t N, int64_t K,
-                    PhaseClock *st = nullptr) {
+                    PhaseClock *st = nullptr, bool u_solve = false) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   hipStream_t s = L.s;
   // updates of depth NB are the trailing updates; shallower ones (inner
   // panel, triangular-solve blocks) are timed separately.  The bracket holds
   // the GEMM kernel alone (the panel transpose is timed with the narrow
   // group of its own).
-  const Phase ph = K >= NB ? kPhGemm : kPhGemmNarrow;
-  const double fl = 8.0 * (double)M * (double)N * (double)K;
+  const Phase ph = u_solve ? kPhTrsm : K >= NB ? kPhGemm : kPhGemmNarrow;
+  const double fl = u_solve ? 0.0 : 8.0 * (double)M * (double)N * (double)K;
   if constexpr (std::is_same<T, double>::value) {
     Timed t(st, s, ph, fl);
     rocblas_handle h;
@@ -633,8 +646,8 @@ static int gemm_sub(const Lane &L, const typename Cx<T>::type *A, int64_t lda, c
     if (rs != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_zgemm failed: " + std::to_string((int)rs));
     return t.done();
   } else {
-    const bool dma = (K == 256 || K == 32) && M >= 2 && M % 2 == 0 && lda % 2 == 0 && ldc % 2 == 0 &&
-                     (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
+    const bool dma = dma_gemm_ok(M, K, lda, ldc, A, C);
+    if (u_solve && (!dma || K != NB)) return fail(MOM_ERR_ARG, "U-block product needs the depth-NB direct-transfer kernel");
     if (dma) {
       const int gm = (int)((M + kDmaBM - 1) / kDmaBM), gn = (int)((N + kDmaBN - 1) / kDmaBN);
       const int64_t ldbt = (int64_t)gn * kDmaBN;
@@ -648,7 +661,10 @@ static int gemm_sub(const Lane &L, const typename Cx<T>::type *A, int64_t lda, c
         MOM_TRY(tt.done());
       }
       Timed t(st, s, ph, fl);
-      if (K == 256)
+      if (K == 256 && u_solve)
+        hipLaunchKernelGGL((gemm3m_dma_kernel<256 / kDmaBK, 1>), dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt,
+                           ldbt, C, ldc, (int)M, (int)N, gm, gn);
+      else if (K == 256)
         hipLaunchKernelGGL(gemm3m_dma_kernel<256 / kDmaBK>, dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt, ldbt,
                            C, ldc, (int)M, (int)N, gm, gn);
       else
@@ -965,6 +981,15 @@ template <> struct Words<double> {
   }
 };
 
+// A workgroup that leaves on a timeout: workgroup 0 records identity
+// interchanges for the columns [j, jend) it did not finish, so that the
+// kernels after the panel only ever see pivot rows inside the matrix (the
+// host reports the timeout after the factorisation).
+__device__ __forceinline__ void leave_identity(int *ipiv, int g, int tid, int j, int jend) {
+  if (g == 0 && tid == 0)
+    for (int q = j; q < jend; q++) ipiv[q] = q + 1;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *A, int64_t ld, int i0, int ib,
                                                           int n, int *ipiv, int *info, PanelSync sync,
@@ -1056,14 +1081,21 @@ __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *
       }
     }
     block_argmax(bv, bi, bw, sv, si, sw);  // contains barriers: `fail` is visible after it
-    if (fail) return;
-    const int p = (bi < n) ? bi : j;
+    if (fail) {
+      leave_identity(ipiv, g, tid, j, i0 + ib);
+      return;
+    }
+    // no candidate with a magnitude that compares (all rows NaN): keep row j,
+    // whose values workgroup 0 publishes every round
+    const bool none = bi >= n;
+    const int p = none ? j : bi;
     const bool zero = !(bv > 0.0);
     // 3. pivot row (and old row j for p's owner) into LDS: whole rows, the
     //    multipliers of columns i0..j-1 move with the interchange as well
     if (tid < nw) {
       const bool need_dj = p != j && (p - i0) / 256 == g;
-      u64 *const ws[2] = {cand + (size_t)bw * kCandWords + VW + 1 + tid, need_dj ? diag + tid : nullptr};
+      u64 *const ws[2] = {none ? diag + tid : cand + (size_t)bw * kCandWords + VW + 1 + tid,
+                          need_dj ? diag + tid : nullptr};
       unsigned got[2];
       if (!poll_words<2>(ws, tag, got, sync.err)) {
         fail = 1;
@@ -1073,7 +1105,10 @@ __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *
       }
     }
     __syncthreads();
-    if (fail) return;
+    if (fail) {
+      leave_identity(ipiv, g, tid, j, i0 + ib);
+      return;
+    }
     if (g == 0 && tid == 0) {
       ipiv[j] = p + 1;
       if (zero) atomicCAS(info, 0, j + 1);
@@ -1134,10 +1169,15 @@ __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *
 // After the replay slot s holds the old content of slot idx[s]; the plan is
 // the list of (destination row, source row) pairs, -1 for unchanged slots:
 // plan[0] = 2 jb, plan[1 + e] = destination, plan[1 + 2 NB + e] = source.
-__global__ __launch_bounds__(256) void laswp_plan_kernel(const int *ipiv, int k1, int k2, int *plan) {
+__global__ __launch_bounds__(256) void laswp_plan_kernel(const int *ipiv, int k1, int k2, int n, int *plan) {
   __shared__ int pv[NB], first[NB], idx[2 * NB];
   const int tid = threadIdx.x, jb = k2 - k1;
-  if (tid < jb) pv[tid] = ipiv[k1 + tid] - 1;
+  if (tid < jb) {
+    // a pivot row of column k lies in [k, n); anything else (a factorisation
+    // left early) is replaced by no interchange
+    const int p = ipiv[k1 + tid] - 1;
+    pv[tid] = (p >= k1 + tid && p < n) ? p : k1 + tid;
+  }
   for (int q = tid; q < 2 * NB; q += 256) idx[q] = q;
   __syncthreads();
   if (tid < jb) {
@@ -1300,6 +1340,76 @@ static int trsm_block(const Lane &L, typename Cx<T>::type *A, int64_t ld, int64_
   return 0;
 }
 
+// E = I - inv(L11) for the unit lower triangular NB x NB diagonal block
+// L11 of a factored panel (complex64, column-major, ld), written
+// column-major with leading dimension NB (zero on and above the diagonal).
+// With it the U block of the panel's row block is one depth-NB product:
+// U12 = inv(L11) A12 = A12 - E A12.  One workgroup per kInvCols columns:
+// its block column X of inv(L11) is formed in LDS in FP64 by block forward
+// substitution, rows in blocks of 32: first the product with the rows above
+// (L staged through LDS 32 columns at a time), then the 32 x 32 unit
+// triangle; E is rounded to complex64 once.  The LDS and register footprint
+// fits in the slot one trailing-update workgroup leaves, so the kernel runs
+// beside the update on the look-ahead stream.
+constexpr int kInvCols = 8;
+
+__global__ __launch_bounds__(256) void unit_lower_inverse_kernel(const float2 *__restrict__ L, int64_t ld,
+                                                                 float2 *__restrict__ E) {
+  __shared__ double2 X[NB][kInvCols + 1];
+  __shared__ double2 Lc[32][33];
+  const int tid = threadIdx.x, c0 = kInvCols * blockIdx.x;
+  const int c = tid % kInvCols, g = tid / kInvCols;  // column of the workgroup, row of a 32-row block
+  const int b0 = c0 / 32;                             // row block holding the diagonal of these columns
+  for (int b = b0; b < NB / 32; b++) {
+    const int r0 = 32 * b;
+    // acc = sum_{k in [32 b0, r0)} L[r0 + g][k] X[k][c] (X is zero above the diagonal)
+    double2 acc = make_double2(0.0, 0.0);
+    for (int kb = 32 * b0; kb < r0; kb += 32) {
+      for (int e = tid; e < 1024; e += 256) {
+        const int rr = e & 31, kk = e >> 5;
+        const float2 l = L[(int64_t)(kb + kk) * ld + r0 + rr];
+        Lc[rr][kk] = make_double2(l.x, l.y);
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (int kk = 0; kk < 32; kk++) {
+        const double2 x = X[kb + kk][c], l = Lc[g][kk];
+        acc.x += l.x * x.x - l.y * x.y;
+        acc.y += l.x * x.y + l.y * x.x;
+      }
+      __syncthreads();
+    }
+    // right-hand side of the block: identity rows minus the product
+    for (int e = tid; e < 1024; e += 256) {
+      const int rr = e & 31, kk = e >> 5;
+      const float2 l = L[(int64_t)(r0 + kk) * ld + r0 + rr];
+      Lc[rr][kk] = make_double2(l.x, l.y);
+    }
+    X[r0 + g][c] = make_double2((r0 + g == c0 + c ? 1.0 : 0.0) - acc.x, -acc.y);
+    __syncthreads();
+    // unit lower triangle of the block, column by column
+    for (int kk = 0; kk < 31; kk++) {
+      const double2 xk = X[r0 + kk][c];
+      if (g > kk) {
+        const double2 l = Lc[g][kk];
+        double2 &t = X[r0 + g][c];
+        t.x -= l.x * xk.x - l.y * xk.y;
+        t.y -= l.x * xk.y + l.y * xk.x;
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < NB * kInvCols; e += 256) {
+    const int r = e % NB, cc = e / NB;
+    float2 v = make_float2(0.f, 0.f);
+    if (r > c0 + cc) {
+      v.x = (float)-X[r][cc].x;
+      v.y = (float)-X[r][cc].y;
+    }
+    E[(int64_t)(c0 + cc) * NB + r] = v;
+  }
+}
+
 // Cooperative inner-block panel factorisation: synchronisation words, the
 // largest co-resident grid of panel_block_kernel<T> (0: not available) and
 // the next free round tag.
@@ -1338,7 +1448,7 @@ static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int6
                                          s));
       const int64_t left = i0 - j0, right = j0 + jb - (i0 + ib);
       if (left + right > 0) {
-        hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), w.plan);
+        hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), (int)n, w.plan);
         MOM_HIP(hipGetLastError());
         const int64_t groups = (left + right + kLaswpCols - 1) / kLaswpCols;
         hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, s,
@@ -1397,15 +1507,37 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
   Lane X = M;
   const bool la = g_lookahead != 0 && n > NB;
   if (la) MOM_TRY(aux_lane(ctx, &X));
-  if (la) MOM_TRY(factor_panel<T>(M, A, ld, n, 0, std::min<int64_t>(NB, n), ipiv, w, st, coop));
+  // complex64: U blocks by one product with E = I - inv(L11) (E is formed
+  // after each panel on the panel's stream, read by the next U block)
+  bool inv_u = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (n > NB) {
+      if (ctx->ublk.bytes < sizeof(float2) * NB * NB) MOM_TRY(ctx->ublk.alloc(sizeof(float2) * NB * NB));
+      inv_u = dma_gemm_ok(NB, NB, NB, ld, ctx->ublk.ptr, A + NB * ld) && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+    }
+  }
+  auto panel = [&](const Lane &P, int64_t j0, int64_t jb) -> int {
+    MOM_TRY(factor_panel<T>(P, A, ld, n, j0, jb, ipiv, w, st, coop));
+    if constexpr (std::is_same<T, float>::value) {
+      if (inv_u && jb == NB && j0 + jb < n) {
+        Timed tt(st, P.s, kPhTrsm);
+        hipLaunchKernelGGL(unit_lower_inverse_kernel, dim3(NB / kInvCols), dim3(256), 0, P.s, A + j0 * ld + j0, ld,
+                           ctx->ublk.as<float2>());
+        MOM_HIP(hipGetLastError());
+        MOM_TRY(tt.done());
+      }
+    }
+    return 0;
+  };
+  if (la) MOM_TRY(panel(M, 0, std::min<int64_t>(NB, n)));
   for (int64_t j0 = 0; j0 < n; j0 += NB) {
     const int64_t jb = std::min<int64_t>(NB, n - j0);
-    if (!la) MOM_TRY(factor_panel<T>(M, A, ld, n, j0, jb, ipiv, w, st, coop));
+    if (!la) MOM_TRY(panel(M, j0, jb));
     // row interchanges of this panel on the columns left and right of it
     const int64_t left = j0, right = n - (j0 + jb);
     if (left + right > 0) {
       Timed tl(st, s, kPhLaswp);
-      hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)j0, (int)(j0 + jb), w.plan);
+      hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)j0, (int)(j0 + jb), (int)n, w.plan);
       MOM_HIP(hipGetLastError());
       const int64_t groups = (left + right + kLaswpCols - 1) / kLaswpCols;
       hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, s, A,
@@ -1414,7 +1546,13 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
       MOM_TRY(tl.done());
     }
     if (right <= 0) continue;
-    MOM_TRY(trsm_block<T>(M, A, ld, j0, jb, j0 + jb, right, st));
+    if (inv_u && jb == NB) {
+      typename Cx<T>::type *A12 = A + (j0 + jb) * ld + j0;
+      MOM_TRY(gemm_sub<T>(M, reinterpret_cast<const typename Cx<T>::type *>(ctx->ublk.ptr), NB, A12, ld, A12, ld, NB,
+                          right, NB, st, true));
+    } else {
+      MOM_TRY(trsm_block<T>(M, A, ld, j0, jb, j0 + jb, right, st));
+    }
     const int64_t r0 = j0 + jb, rows = n - r0;
     const typename Cx<T>::type *L21 = A + j0 * ld + r0, *U12 = A + r0 * ld + j0;
     typename Cx<T>::type *A22 = A + r0 * ld + r0;
@@ -1426,7 +1564,7 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
     MOM_HIP(hipEventRecord(ctx->fork_ev, s));
     MOM_HIP(hipStreamWaitEvent(X.s, ctx->fork_ev, 0));
     MOM_TRY(gemm_sub<T>(X, L21, ld, U12, ld, A22, ld, rows, nxt, jb, st));
-    MOM_TRY(factor_panel<T>(X, A, ld, n, r0, nxt, ipiv, w, st, coop));
+    MOM_TRY(panel(X, r0, nxt));
     MOM_HIP(hipEventRecord(ctx->join_ev, X.s));
     MOM_TRY(gemm_sub<T>(M, L21, ld, U12 + nxt * ld, ld, A22 + nxt * ld, ld, rows, right - nxt, jb, st));
     MOM_HIP(hipStreamWaitEvent(s, ctx->join_ev, 0));
@@ -1825,6 +1963,8 @@ static int panel_coop(mom_system *S, PanelCoop *c) {
     S->panel_tag = 1;
   }
   c->tag = &S->panel_tag;
+  // a timeout of an earlier factorisation must not end this one's polls
+  MOM_HIP(hipMemsetAsync(c->sync.err, 0, sizeof(int), S->ctx->stream));
   return 0;
 }
 
@@ -1858,7 +1998,10 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
   if (S->clock.enabled && (rc = S->clock.collect())) return rc;
   S->perm.resize(n);
   for (int64_t i = 0; i < n; i++) S->perm[i] = (int)i;
-  for (int64_t k = 0; k < n; k++) std::swap(S->perm[k], S->perm[ip[k] - 1]);
+  for (int64_t k = 0; k < n; k++) {
+    if (ip[k] - 1 < k || ip[k] - 1 >= n) return fail(MOM_ERR_HIP, "factorisation produced an invalid pivot row");
+    std::swap(S->perm[k], S->perm[ip[k] - 1]);
+  }
   S->info = info;
   return 0;
 }
@@ -2300,6 +2443,25 @@ extern "C" int mom_test_lookahead(int32_t enable) {
   if (enable != 0 && enable != 1) return fail(MOM_ERR_ARG, "look-ahead switch must be 0 or 1");
   g_lookahead = enable;
   return 0;
+}
+
+extern "C" int mom_test_unit_lower_inverse(mom_ctx *ctx, const double *L, int64_t ldl, double *E) {
+  if (!ctx || !L || !E) return fail(MOM_ERR_ARG, "NULL argument");
+  if (ldl < NB) return fail(MOM_ERR_ARG, "bad leading dimension");
+  hipStream_t s = ctx->stream;
+  DevBuf dL, fL, fE, dE;
+  int rc;
+  if ((rc = dL.alloc(sizeof(double2) * ldl * NB)) || (rc = fL.alloc(sizeof(float2) * ldl * NB)) ||
+      (rc = fE.alloc(sizeof(float2) * NB * NB)) || (rc = dE.alloc(sizeof(double2) * NB * NB)))
+    return rc;
+  MOM_TRY(copy_h2d(ctx, dL.ptr, L, sizeof(double2) * ldl * NB));
+  hipLaunchKernelGGL(narrow_kernel, dim3((unsigned)((ldl * NB + 255) / 256)), dim3(256), 0, s, dL.as<double2>(),
+                     fL.as<float2>(), ldl * NB);
+  hipLaunchKernelGGL(unit_lower_inverse_kernel, dim3(NB / kInvCols), dim3(256), 0, s, fL.as<float2>(), ldl, fE.as<float2>());
+  hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((NB * NB + 255) / 256)), dim3(256), 0, s, fE.as<float2>(),
+                     dE.as<double2>(), (int64_t)NB * NB);
+  MOM_HIP(hipGetLastError());
+  return copy_d2h(ctx, E, dE.ptr, sizeof(double2) * NB * NB);
 }
 
 extern "C" int mom_test_gemm_sub(mom_ctx *ctx, int32_t precision, int64_t M, int64_t N, int64_t K, const double *A,

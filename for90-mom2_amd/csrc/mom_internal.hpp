@@ -84,6 +84,7 @@ struct mom_ctx {
   // its own GEMM scratch and rocBLAS handle; fork/join events between the two
   hipStream_t aux = nullptr;
   mom::DevBuf panel_t2;
+  mom::DevBuf ublk;  // E = I - inv(L11) of the last factored panel (complex64, NB x NB; lu.hip)
   void *blas2 = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };

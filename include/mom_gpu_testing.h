@@ -49,6 +49,14 @@ int mom_test_panel_mode(int32_t mode);
  */
 int mom_test_lookahead(int32_t enable);
 
+/*
+ * E = I - inv(L11) for a unit lower triangular 256 x 256 block L11 (the
+ * strict lower part of L[256 x 256] (ldl), complex128, rounded to complex64
+ * on upload), as the complex64 LU forms it after each panel; E (256 x 256,
+ * leading dimension 256) is returned widened to complex128.
+ */
+int mom_test_unit_lower_inverse(mom_ctx *ctx, const double *L, int64_t ldl, double *E);
+
 #ifdef __cplusplus
 }
 #endif

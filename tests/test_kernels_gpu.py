@@ -69,3 +69,26 @@ def test_gemm_sub(gpu_ctx, prec, shape):
     assert err < tol
     # rows outside M untouched
     assert np.array_equal(got[M:, :], C[M:, :].astype(np.complex64).astype(np.complex128) if prec == 32 else C[M:, :])
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_unit_lower_inverse(gpu_ctx, seed):
+    """E = I - inv(L11) formed in complex64 for the U-block product of the
+    complex64 LU, against numpy in complex128 on the same rounded L."""
+    import scipy.linalg as sl
+
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((700, 256)) + 1j * rng.standard_normal((700, 256))
+    _, Lfull, _ = sl.lu(G)
+    ldl = 260
+    L = np.zeros((ldl, 256), np.complex128, order="F")
+    L[:256, :] = np.tril(Lfull[:256, :], -1) + 9.0 * np.triu(np.ones((256, 256)))  # only the strict lower part is read
+    E = np.zeros((256, 256), np.complex128, order="F")
+    capi.check(capi.lib().mom_test_unit_lower_inverse(gpu_ctx.h, L.ctypes.data, ldl, E.ctypes.data))
+    L32 = np.tril(L[:256, :].astype(np.complex64).astype(np.complex128), -1) + np.eye(256)
+    E_ref = np.eye(256) - np.linalg.inv(L32)
+    scale = np.abs(E_ref).max()
+    err = np.abs(E - E_ref).max() / scale
+    print("seed %d: max|E| %.3g, rel err %.3e" % (seed, scale, err), flush=True)
+    assert np.all(np.triu(E) == 0)
+    assert err < 3e-7

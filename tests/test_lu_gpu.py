@@ -88,7 +88,12 @@ def test_mixed_system_reaches_fp64_backward_error(gpu_ctx, panel_mode, n2):
     S = System(gpu_ctx, n2 // 2, MOM_PREC_MIXED)
     S.set_matrix(A)
     S.factor()
-    x = S.solve(b, tol=1e-13, maxit=20)
+    try:
+        x = S.solve(b, tol=1e-13, maxit=20)
+    finally:
+        st = S.stats()
+        print("n=%d mixed: backward errors %s, %d factor solves" % (
+            n2, ["%.1e" % v for v in st["berr_history"]], st["n_minv"]), flush=True)
     print("n=%d mixed: iters %d berr %.3e" % (n2, S.iters, S.berr), flush=True)
     assert S.berr <= 1e-13
     assert berr(A, x[:, 0], b[:, 0]) < 1e-13
