@@ -303,6 +303,16 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
   m->F = F;
   m->E = E;
   m->avelen = avelen;
+  {
+    double lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
+    for (int64_t v = 0; v < nnodes; v++)
+      for (int c = 0; c < 3; c++) {
+        lo[c] = std::min(lo[c], p[3 * v + c]);
+        hi[c] = std::max(hi[c], p[3 * v + c]);
+      }
+    m->extent = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                          (hi[2] - lo[2]) * (hi[2] - lo[2]));
+  }
   m->h_fcp.assign(fcp, fcp + 3 * F);
   // face-side codes of each edge and the slab edge lists of the fill
   std::vector<int32_t> eside(2 * E, -1);

@@ -61,6 +61,69 @@ __device__ __forceinline__ void cadd_atomic(double2 *p, double re, double im) {
   unsafeAtomicAdd(&p->y, im);
 }
 
+// exp(a) (cos b + i sin b) for the point-pair loop, where the library
+// exp and sincos are about half of the instructions.  Cody-Waite reduction
+// (Cody and Waite, "Software Manual for the Elementary Functions", 1980):
+// a = n ln2 + r with ln2 in two parts (the first with 32 significant bits,
+// so n * part exact for |n| < 2^21), |r| <= ln2/2; b = q pi/2 + t with pi/2
+// in three parts of 33, 33 and 53 bits (q * part exact for |q| < 2^20),
+// |t| <= pi/4.  Then Taylor polynomials in Horner form: exp(r) to degree 13
+// (truncation below 5e-18), sin t to t^15 and cos t to t^16 (below 6e-17),
+// coefficients 1/k! rounded to double.  Needs |b| < kCisMaxArg (the fill
+// checks Re(k) times the mesh extent on the host); a is clamped to
+// [-1000, 710]: exp underflows to 0 below and overflows to inf above.
+constexpr double kCisMaxArg = 524288.0;  // 2^19
+
+__device__ __forceinline__ void cis_exp(double a, double b, double &er, double &ei) {
+  a = fmin(fmax(a, -1000.0), 710.0);
+  // exp(a) = 2^n exp(r)
+  const double nf = rint(a * 0x1.71547652b82fep+0);  // a / ln2
+  double r = fma(-nf, 0x1.62e42fee00000p-1, a);
+  r = fma(-nf, 0x1.a39ef35793c76p-33, r);
+  double p = 0x1.6124613a86d09p-33;                   // 1/13!
+  p = fma(p, r, 0x1.1eed8eff8d898p-29);               // 1/12!
+  p = fma(p, r, 0x1.ae64567f544e4p-26);               // 1/11!
+  p = fma(p, r, 0x1.27e4fb7789f5cp-22);               // 1/10!
+  p = fma(p, r, 0x1.71de3a556c734p-19);               // 1/9!
+  p = fma(p, r, 0x1.a01a01a01a01ap-16);               // 1/8!
+  p = fma(p, r, 0x1.a01a01a01a01ap-13);               // 1/7!
+  p = fma(p, r, 0x1.6c16c16c16c17p-10);               // 1/6!
+  p = fma(p, r, 0x1.1111111111111p-7);                // 1/5!
+  p = fma(p, r, 0x1.5555555555555p-5);                // 1/4!
+  p = fma(p, r, 0x1.5555555555555p-3);                // 1/3!
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  const double ex = ldexp(p, (int)nf);
+  // sin, cos of b = q pi/2 + t
+  const double qf = rint(b * 0x1.45f306dc9c883p-1);  // b * 2/pi
+  double t = fma(-qf, 0x1.921fb54400000p+0, b);
+  t = fma(-qf, 0x1.0b4611a600000p-34, t);
+  t = fma(-qf, 0x1.3198a2e037073p-69, t);
+  const double t2 = t * t;
+  double sp = -0x1.ae7f3e733b81fp-41;                 // -1/15!
+  sp = fma(sp, t2, 0x1.6124613a86d09p-33);            // 1/13!
+  sp = fma(sp, t2, -0x1.ae64567f544e4p-26);           // -1/11!
+  sp = fma(sp, t2, 0x1.71de3a556c734p-19);            // 1/9!
+  sp = fma(sp, t2, -0x1.a01a01a01a01ap-13);           // -1/7!
+  sp = fma(sp, t2, 0x1.1111111111111p-7);             // 1/5!
+  sp = fma(sp, t2, -0x1.5555555555555p-3);            // -1/3!
+  const double sn = fma(sp * t2, t, t);
+  double cp = 0x1.ae7f3e733b81fp-45;                  // 1/16!
+  cp = fma(cp, t2, -0x1.93974a8c07c9dp-37);           // -1/14!
+  cp = fma(cp, t2, 0x1.1eed8eff8d898p-29);            // 1/12!
+  cp = fma(cp, t2, -0x1.27e4fb7789f5cp-22);           // -1/10!
+  cp = fma(cp, t2, 0x1.a01a01a01a01ap-16);            // 1/8!
+  cp = fma(cp, t2, -0x1.6c16c16c16c17p-10);           // -1/6!
+  cp = fma(cp, t2, 0x1.5555555555555p-5);             // 1/4!
+  cp = fma(cp, t2, -0.5);
+  const double cs = fma(cp, t2, 1.0);
+  const int q = (int)qf & 3;
+  const double s1 = (q & 1) ? cs : sn, c1 = (q & 1) ? sn : cs;
+  ei = ex * ((q & 2) ? -s1 : s1);
+  er = ex * (((q + 1) & 2) ? -c1 : c1);
+}
+
 // near_faces (sysmat.f90:233-260) without FMA contraction, so the
 // classification of borderline pairs follows the reference's arithmetic.
 __device__ bool near_pair(const double *cpm, const double *cpn, double thr2) {
@@ -138,12 +201,11 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
       if (!SMOOTH) {
         const double iR = rsqrt(R2);
         const double R = R2 * iR;
-        const double ex = (ki == 0.0) ? 1.0 : exp(-ki * R);
-        double sn, cs;
-        sincos(kr * R, &sn, &cs);
-        const double e = ex * iR * kOofpi;
-        Gr = e * cs;
-        Gi = e * sn;
+        double Er, Ei;
+        cis_exp(-ki * R, kr * R, Er, Ei);
+        const double e = iR * kOofpi;
+        Gr = e * Er;
+        Gi = e * Ei;
         // g = G * ((1/R^2 + ki/R) - i kr/R)
         const double ar = iR * iR + ki * iR, ai = -kr * iR;
         gr = Gr * ar - Gi * ai;
@@ -157,10 +219,8 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
       } else {
         const double iR = rsqrt(R2);
         const double R = R2 * iR;
-        const double ex = (ki == 0.0) ? 1.0 : exp(-ki * R);
-        double sn, cs;
-        sincos(kr * R, &sn, &cs);
-        const double Er = ex * cs, Ei = ex * sn;
+        double Er, Ei;
+        cis_exp(-ki * R, kr * R, Er, Ei);
         // Gs = [(e^{ikR} - 1)/R + k^2 R/2] / 4pi
         Gr = ((Er - 1.0) * iR + 0.5 * k2r * R) * kOofpi;
         Gi = (Ei * iR + 0.5 * k2i * R) * kOofpi;
@@ -235,7 +295,7 @@ constexpr int kSrcPerWave = 4;
 constexpr int kSrcPerBlock = kWaves * kSrcPerWave;
 
 template <int QT>
-__global__ __launch_bounds__(256) void pair_plane_kernel(PlaneArgs pa) {
+__global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
   const FarArgs &a = pa.f;
   __shared__ double tpts[3 * kMaxQ * 64];          // test points, [3j+c][lane]
   __shared__ double spts[kSrcPerBlock][3 * kMaxQ];  // source points, broadcast
@@ -800,6 +860,8 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   a.side = sides(m, t.mode == 0);
   a.kr = k.real();
   a.ki = k.imag();
+  if (!(std::fabs(a.kr) * m->extent < kCisMaxArg))
+    return fail(MOM_ERR_ARG, "Re(k) times the mesh extent is beyond the fill's phase range (2^19 rad)");
   const double thr = m->avelen * 3;
   a.thr2 = thr * thr;
   a.c1 = make_double2(c1.real(), c1.imag());

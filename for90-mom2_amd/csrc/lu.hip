@@ -1490,6 +1490,12 @@ static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int6
 
 // Look-ahead of the blocked LU (on unless switched off by the test hook).
 static int g_lookahead = 1;
+// U blocks of the complex64 LU: 1 (default) blocked substitution, 0 one
+// product with E = I - inv(L11) where the shapes allow it (test hook).  The
+// product saves about a third of the U-block time but loses about a decade
+// of the factors' backward error on the PMCHWT matrices (|E| |A12| is much
+// larger than |U12| there), which costs more refinement than it saves.
+static int g_u_subst = 1;
 
 // LU with partial pivoting of the n x n matrix A (ld), in place, blocked
 // right-looking with look-ahead of depth one.  Step k, once panel k is
@@ -1513,7 +1519,8 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
   if constexpr (std::is_same<T, float>::value) {
     if (n > NB) {
       if (ctx->ublk.bytes < sizeof(float2) * NB * NB) MOM_TRY(ctx->ublk.alloc(sizeof(float2) * NB * NB));
-      inv_u = dma_gemm_ok(NB, NB, NB, ld, ctx->ublk.ptr, A + NB * ld) && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+      inv_u = g_u_subst == 0 && dma_gemm_ok(NB, NB, NB, ld, ctx->ublk.ptr, A + NB * ld) &&
+              (reinterpret_cast<uintptr_t>(A) & 15) == 0;
     }
   }
   auto panel = [&](const Lane &P, int64_t j0, int64_t jb) -> int {
@@ -2436,6 +2443,12 @@ extern "C" int mom_test_mfma_layout(mom_ctx *ctx, int32_t precision, double *out
 extern "C" int mom_test_panel_mode(int32_t mode) {
   if (mode != 0 && mode != 1) return fail(MOM_ERR_ARG, "panel mode must be 0 or 1");
   g_panel_mode = mode;
+  return 0;
+}
+
+extern "C" int mom_test_u_solve(int32_t mode) {
+  if (mode != 0 && mode != 1) return fail(MOM_ERR_ARG, "U-block mode must be 0 or 1");
+  g_u_subst = mode;
   return 0;
 }
 

@@ -94,6 +94,7 @@ struct mom_mesh {
   mom_ctx *ctx = nullptr;
   int64_t V = 0, F = 0, E = 0;
   double avelen = 0;
+  double extent = 0;  // diagonal of the nodes' bounding box (bounds every point distance)
   int64_t nnear = 0;
   int64_t max_parent = -1;  // largest global index among the mesh's edges
   std::vector<double> h_fcp;  // host centroids (near list)

@@ -50,6 +50,13 @@ int mom_test_panel_mode(int32_t mode);
 int mom_test_lookahead(int32_t enable);
 
 /*
+ * U blocks of the complex64 LU: 1 (default) blocked substitution, 0 one
+ * depth-256 product with E = I - inv(L11) where the shapes allow it.
+ * Process-wide; lets the tests check both.
+ */
+int mom_test_u_solve(int32_t mode);
+
+/*
  * E = I - inv(L11) for a unit lower triangular 256 x 256 block L11 (the
  * strict lower part of L[256 x 256] (ldl), complex128, rounded to complex64
  * on upload), as the complex64 LU forms it after each panel; E (256 x 256,

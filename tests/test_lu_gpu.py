@@ -14,18 +14,22 @@ from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED, MomError
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[(0, 1), (0, 0), (1, 1)], ids=["coop_panel", "coop_panel_no_lookahead", "column_panel"])
+@pytest.fixture(params=[(0, 1, 1), (0, 0, 1), (1, 1, 1), (0, 1, 0)],
+                ids=["coop_panel", "coop_panel_no_lookahead", "column_panel", "coop_panel_inverse_u"])
 def panel_mode(request):
     """Both panel factorisation paths (one cooperative launch per inner
     block of 32 columns, one launch per column), with the look-ahead
     schedule (panel k+1 on a second stream beside the trailing update of
-    step k) and without it."""
-    mode, lookahead = request.param
+    step k) and without it, and the complex64 U blocks by substitution and
+    by the product with I - inv(L11)."""
+    mode, lookahead, u_subst = request.param
     capi.check(capi.lib().mom_test_panel_mode(mode))
     capi.check(capi.lib().mom_test_lookahead(lookahead))
+    capi.check(capi.lib().mom_test_u_solve(u_subst))
     yield request.param
     capi.check(capi.lib().mom_test_panel_mode(0))
     capi.check(capi.lib().mom_test_lookahead(1))
+    capi.check(capi.lib().mom_test_u_solve(1))
 
 
 def rand_complex(n, m=None, seed=0, diag=0.0):
