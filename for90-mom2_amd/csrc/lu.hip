@@ -1581,58 +1581,95 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
 
 // ---------------------------------------------------------------------------
 // Inverses of the NB x NB diagonal blocks of L (unit lower) and U (upper),
-// stored as nblk consecutive NB x NB column-major blocks.  One workgroup
-// per block and per factor; column c of the inverse is solved by thread c.
+// stored as nblk consecutive NB x NB column-major blocks in complex128.
+// Grid (nblk, NB / kInvCols, 2): one workgroup per block, kInvCols columns
+// of the inverse and factor (z = 0: L, 1: U).  The block column X is formed
+// in LDS in FP64 by block substitution over rows in blocks of 32 (forward
+// for L, backward for U): the product with the rows already solved (the
+// factor staged through LDS 32 columns at a time), then the 32 x 32
+// triangle.  Outside the matrix (a last block narrower than NB) the factor
+// reads as the identity, so those columns of the inverse are unit vectors.
 template <typename T>
 __global__ __launch_bounds__(256) void diag_inverse_kernel(const typename Cx<T>::type *__restrict__ A, int64_t ld,
                                                             int64_t n, double2 *Linv, double2 *Uinv) {
-  using C2 = double2;  // inverses are formed and stored in complex128
-  const int b = blockIdx.x;
-  const bool upper = blockIdx.y == 1;
-  const int64_t r0 = (int64_t)b * NB;
-  const int w = (int)std::min<int64_t>(NB, n - r0);
-  C2 *out = (upper ? Uinv : Linv) + (int64_t)b * NB * NB;
-  const int c = threadIdx.x;
-  if (c >= NB) return;
-  // solve T x = e_c by substitution, writing x into out(:, c)
-  C2 *x = out + (int64_t)c * NB;
-  for (int i = 0; i < NB; i++) {
-    x[i].x = 0;
-    x[i].y = 0;
-  }
-  if (c >= w) {
-    x[c].x = 1;  // identity padding outside the matrix
-    return;
-  }
-  if (!upper) {
-    x[c].x = 1;
-    for (int k = c; k < w; k++) {
-      const C2 xk = x[k];
-      const typename Cx<T>::type *lk = A + (r0 + k) * ld + r0;
-      for (int i = k + 1; i < w; i++) {
-        const C2 l = make_double2(lk[i].x, lk[i].y);
-        x[i].x -= l.x * xk.x - l.y * xk.y;
-        x[i].y -= l.x * xk.y + l.y * xk.x;
+  __shared__ double2 X[NB][kInvCols + 1];
+  __shared__ double2 Fc[32][33];
+  const int blk = blockIdx.x, c0 = kInvCols * blockIdx.y;
+  const bool upper = blockIdx.z == 1;
+  const int64_t d0 = (int64_t)blk * NB;
+  const int w = (int)std::min<int64_t>(NB, n - d0);
+  const int tid = threadIdx.x, c = tid % kInvCols, g = tid / kInvCols;
+  // factor entry (r, k) of the diagonal block, identity outside the matrix
+  auto F = [&](int r, int k) -> double2 {
+    if (r >= w || k >= w) return make_double2(r == k ? 1.0 : 0.0, 0.0);
+    if (!upper && r == k) return make_double2(1.0, 0.0);
+    const typename Cx<T>::type v = A[(d0 + k) * ld + d0 + r];
+    return make_double2(v.x, v.y);
+  };
+  const int bd = c0 / 32;                          // row block holding the diagonal of these columns
+  const int bfirst = upper ? (c0 + kInvCols - 1) / 32 : bd;
+  const int bstep = upper ? -1 : 1;
+  const int blast = upper ? -1 : NB / 32;
+  for (int b = bfirst; b != blast; b += bstep) {
+    const int r0 = 32 * b;
+    double2 acc = make_double2(0.0, 0.0);
+    // rows already solved: L, blocks [bd, b); U, blocks (b, bfirst]
+    const int klo = upper ? r0 + 32 : 32 * bd, khi = upper ? 32 * (bfirst + 1) : r0;
+    for (int kb = klo; kb < khi; kb += 32) {
+      for (int e = tid; e < 1024; e += 256) {
+        const int rr = e & 31, kk = e >> 5;
+        Fc[rr][kk] = F(r0 + rr, kb + kk);
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (int kk = 0; kk < 32; kk++) {
+        const double2 x = X[kb + kk][c], f = Fc[g][kk];
+        acc.x += f.x * x.x - f.y * x.y;
+        acc.y += f.x * x.y + f.y * x.x;
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < 1024; e += 256) {
+      const int rr = e & 31, kk = e >> 5;
+      Fc[rr][kk] = F(r0 + rr, r0 + kk);
+    }
+    X[r0 + g][c] = make_double2((r0 + g == c0 + c ? 1.0 : 0.0) - acc.x, -acc.y);
+    __syncthreads();
+    if (!upper) {
+      for (int kk = 0; kk < 31; kk++) {
+        const double2 xk = X[r0 + kk][c];
+        if (g > kk) {
+          const double2 f = Fc[g][kk];
+          double2 &t = X[r0 + g][c];
+          t.x -= f.x * xk.x - f.y * xk.y;
+          t.y -= f.x * xk.y + f.y * xk.x;
+        }
+        __syncthreads();
+      }
+    } else {
+      for (int kk = 31; kk >= 0; kk--) {
+        if (g == kk) {  // divide row kk by the diagonal
+          const double2 d = Fc[kk][kk], v = X[r0 + kk][c];
+          const double den = d.x * d.x + d.y * d.y;
+          X[r0 + kk][c] = make_double2((v.x * d.x + v.y * d.y) / den, (v.y * d.x - v.x * d.y) / den);
+        }
+        __syncthreads();
+        const double2 xk = X[r0 + kk][c];
+        if (g < kk) {
+          const double2 f = Fc[g][kk];
+          double2 &t = X[r0 + g][c];
+          t.x -= f.x * xk.x - f.y * xk.y;
+          t.y -= f.x * xk.y + f.y * xk.x;
+        }
+        __syncthreads();
       }
     }
-  } else {
-    for (int i = c; i >= 0; i--) {
-      // x_i = (delta_ic - sum_{k>i} U_ik x_k) / U_ii
-      C2 acc;
-      acc.x = (i == c) ? 1 : 0;
-      acc.y = 0;
-      for (int k = i + 1; k <= c; k++) {
-        const C2 u = make_double2(A[(r0 + k) * ld + r0 + i].x, A[(r0 + k) * ld + r0 + i].y);
-        acc.x -= u.x * x[k].x - u.y * x[k].y;
-        acc.y -= u.x * x[k].y + u.y * x[k].x;
-      }
-      const C2 d = make_double2(A[(r0 + i) * ld + r0 + i].x, A[(r0 + i) * ld + r0 + i].y);
-      const double den = d.x * d.x + d.y * d.y;
-      C2 q;
-      q.x = (acc.x * d.x + acc.y * d.y) / den;
-      q.y = (acc.y * d.x - acc.x * d.y) / den;
-      x[i] = q;
-    }
+  }
+  double2 *out = (upper ? Uinv : Linv) + (int64_t)blk * NB * NB;
+  for (int e = tid; e < NB * kInvCols; e += 256) {
+    const int r = e % NB, cc = e / NB;
+    const bool inside = upper ? r <= c0 + cc && r < 32 * (bfirst + 1) : r >= 32 * bd;
+    out[(int64_t)(c0 + cc) * NB + r] = inside ? X[r][cc] : make_double2(0.0, 0.0);
   }
 }
 
@@ -1880,7 +1917,8 @@ template <typename T>
 static int diag_inverses(hipStream_t s, const typename Cx<T>::type *LU, int64_t ld, int64_t n, double2 *Linv,
                          double2 *Uinv) {
   const int64_t nblk = (n + NB - 1) / NB;
-  hipLaunchKernelGGL(diag_inverse_kernel<T>, dim3((unsigned)nblk, 2), dim3(NB), 0, s, LU, ld, n, Linv, Uinv);
+  hipLaunchKernelGGL(diag_inverse_kernel<T>, dim3((unsigned)nblk, NB / kInvCols, 2), dim3(256), 0, s, LU, ld, n, Linv,
+                     Uinv);
   MOM_HIP(hipGetLastError());
   return 0;
 }
