@@ -759,10 +759,15 @@ __global__ __launch_bounds__(256) void pivot_candidates_kernel(const typename Cx
     idx = (int)r;
   }
   block_argmax(v, idx, win, sv, si, sw);
-  if (tid == win && idx < n) {
+  // value and row index always (n: no candidate, e.g. all NaN), the row's
+  // values only for a candidate
+  if (tid == win) {
     const size_t slot = (size_t)par * w.mb + blockIdx.x;
     w.val[slot] = v;
     w.idx[slot] = idx;
+  }
+  if (tid == win && idx < n) {
+    const size_t slot = (size_t)par * w.mb + blockIdx.x;
     C2 *row = (C2 *)w.row + slot * IB;
     C2 x[IB];
 #pragma unroll
@@ -809,10 +814,11 @@ __global__ __launch_bounds__(256) void panel_column_kernel(typename Cx<T>::type 
   for (int t = tid; t < nb_in; t += 256)
     argmax_merge(bv, bi, bw, w.val[(size_t)par * w.mb + t], w.idx[(size_t)par * w.mb + t], t);
   block_argmax(bv, bi, bw, sv, si, sw);
-  const int p = (bi < n) ? bi : j;
+  const bool none = !(bi >= j && bi < n);  // no candidate compares (NaN): keep row j
+  const int p = none ? j : bi;
   const bool zero = !(bv > 0.0);
   const C2 *cand = (const C2 *)w.row + ((size_t)par * w.mb + bw) * IB;
-  if (tid < kend - k) u[k + tid] = (bi < n) ? cand[k + tid] : A[(int64_t)(j + tid) * ld + j];
+  if (tid < kend - k) u[k + tid] = none ? A[(int64_t)(j + tid) * ld + j] : cand[k + tid];
   __syncthreads();
   if (b == 0 && tid == 0) {
     ipiv[j] = p + 1;
@@ -878,10 +884,13 @@ __global__ __launch_bounds__(256) void panel_column_kernel(typename Cx<T>::type 
     ci = (int)i;
   }
   block_argmax(cv, ci, cw, sv, si, sw);
-  if (tid == cw && ci < n) {
+  if (tid == cw) {
     const size_t slot = (size_t)(par ^ 1) * w.mb + b;
     w.val[slot] = cv;
     w.idx[slot] = ci;
+  }
+  if (tid == cw && ci < n) {
+    const size_t slot = (size_t)(par ^ 1) * w.mb + b;
     C2 *row = (C2 *)w.row + slot * IB;
 #pragma unroll
     for (int s = 0; s < IB; s++)
@@ -1087,7 +1096,7 @@ __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *
     }
     // no candidate with a magnitude that compares (all rows NaN): keep row j,
     // whose values workgroup 0 publishes every round
-    const bool none = bi >= n;
+    const bool none = !(bi >= j && bi < n);
     const int p = none ? j : bi;
     const bool zero = !(bv > 0.0);
     // 3. pivot row (and old row j for p's owner) into LDS: whole rows, the
@@ -1686,28 +1695,31 @@ __device__ __forceinline__ void cmac(typename Cx<TC>::type &acc, typename Cx<TS>
 
 // y[r] = sum_c M[r, c] x[c] for the NB x NB diagonal-block inverse M; the
 // result overwrites x (in tmp) and is kept in xb for the panel update.
-// Grid (NB/32, nrhs): 32 rows per workgroup, 8 column slices per row.
+// Grid (NB/8, nrhs): 8 rows per workgroup (128-byte row segments of M per
+// column), 32 column slices per row, so that 32 workgroups share the 1 MB
+// read of M.
 template <typename TS, typename TC>
 __global__ __launch_bounds__(256) void block_apply_kernel(const double2 *__restrict__ M,
                                                            typename Cx<TC>::type *x, int64_t ldx,
                                                            typename Cx<TC>::type *__restrict__ xb, int w) {
   using CV = typename Cx<TC>::type;
   __shared__ CV xs[NB];
-  __shared__ CV part[8][32];
+  __shared__ CV part[32][8];
   const int64_t rhs = blockIdx.y;
   for (int i = threadIdx.x; i < NB; i += 256) xs[i] = (i < w) ? x[rhs * ldx + i] : CV{0, 0};
   __syncthreads();
-  const int rl = threadIdx.x & 31, sl = threadIdx.x >> 5;
-  const int r = blockIdx.x * 32 + rl;
+  const int rl = threadIdx.x & 7, sl = threadIdx.x >> 3;
+  const int r = blockIdx.x * 8 + rl;
   CV acc{0, 0};
   if (r < w) {
-    for (int c = sl; c < w; c += 8) cmac<double, TC>(acc, M[(int64_t)c * NB + r], xs[c]);
+#pragma unroll 4
+    for (int c = sl; c < w; c += 32) cmac<double, TC>(acc, M[(int64_t)c * NB + r], xs[c]);
   }
   part[sl][rl] = acc;
   __syncthreads();
   if (sl == 0 && r < w) {
     CV s = part[0][rl];
-    for (int k = 1; k < 8; k++) {
+    for (int k = 1; k < 32; k++) {
       s.x += part[k][rl].x;
       s.y += part[k][rl].y;
     }
@@ -1776,7 +1788,7 @@ static int getrs(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, int
   for (int64_t b = 0; b < nblk; b++) {
     const int64_t r0 = b * NB;
     const int w = (int)std::min<int64_t>(NB, n - r0);
-    hipLaunchKernelGGL((block_apply_kernel<TS, TC>), dim3(NB / 32, (unsigned)nrhs), dim3(256), 0, s,
+    hipLaunchKernelGGL((block_apply_kernel<TS, TC>), dim3(NB / 8, (unsigned)nrhs), dim3(256), 0, s,
                        Linv + b * NB * NB, tmp + r0, n, xb, w);
     if (r0 + w < n)
       hipLaunchKernelGGL((panel_gemv_sub_kernel<TS, TC>), dim3((unsigned)((n - r0 - w + 63) / 64), (unsigned)nrhs),
@@ -1785,7 +1797,7 @@ static int getrs(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, int
   for (int64_t b = nblk - 1; b >= 0; b--) {
     const int64_t r0 = b * NB;
     const int w = (int)std::min<int64_t>(NB, n - r0);
-    hipLaunchKernelGGL((block_apply_kernel<TS, TC>), dim3(NB / 32, (unsigned)nrhs), dim3(256), 0, s,
+    hipLaunchKernelGGL((block_apply_kernel<TS, TC>), dim3(NB / 8, (unsigned)nrhs), dim3(256), 0, s,
                        Uinv + b * NB * NB, tmp + r0, n, xb, w);
     if (r0 > 0)
       hipLaunchKernelGGL((panel_gemv_sub_kernel<TS, TC>), dim3((unsigned)((r0 + 63) / 64), (unsigned)nrhs), dim3(256),

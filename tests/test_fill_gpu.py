@@ -83,3 +83,20 @@ def test_near_pair_count_matches_oracle_rule(gpu_ctx, msh_dir):
     ds = diff[..., 0] * diff[..., 0] + diff[..., 1] * diff[..., 1] + diff[..., 2] * diff[..., 2]
     thr = d.avelen * 3
     assert DeviceMesh(gpu_ctx, d).near_pairs == int((ds < thr * thr).sum())
+
+
+def test_phase_range_is_checked_before_launch(gpu_ctx, msh_dir):
+    """The point-pair loop reduces k R modulo pi/2 exactly only while
+    Re(k) times the mesh extent stays below 2^19 rad; beyond that the fill
+    refuses with an argument error (checked on the host, before any kernel
+    runs) instead of returning inaccurate entries."""
+    from mom2amd.capi import MomError
+
+    case = sphere(msh_dir, 1)
+    dm = DeviceMesh(gpu_ctx, case["problem"].domains[0])
+    with pytest.raises(MomError) as e:
+        computeD(gpu_ctx, dm, case["omega"] * 1e7, case["ri"][0])
+    assert e.value.status < 0 and "phase range" in str(e.value)
+    # the context stays usable
+    M = computeD(gpu_ctx, dm, case["omega"], case["ri"][0])
+    assert np.isfinite(M).all()

@@ -124,3 +124,25 @@ def test_mixed_factor_accuracy_at_scale(gpu_ctx):
     assert hist[0] < 1e-5
     assert hist[1] < 1e-7
     S.close()
+
+
+@pytest.mark.parametrize("where", ["entry", "column"])
+def test_lu_nan_reports_without_fault(gpu_ctx, panel_mode, where):
+    """NaN in the matrix (one entry, or a whole column so that no pivot
+    candidate compares): the factorisation ends normally, every pivot row
+    lies inside the matrix, and the NaN shows either as INFO > 0 (zero
+    pivot, the reference's STOP path) or in the solution."""
+    n = 300
+    A = rand_complex(n, seed=5)
+    if where == "entry":
+        A[17, 3] = np.nan
+    else:
+        A[:, 3] = np.nan
+    B = rand_complex(n, 1, seed=6)
+    try:
+        _, X, ipiv = solve_multi_linsys(gpu_ctx, A, B)
+    except MomError as e:
+        assert e.status > 0, e
+        return
+    assert ((ipiv >= np.arange(1, n + 1)) & (ipiv <= n)).all()
+    assert not np.isfinite(X).all()
