@@ -55,7 +55,7 @@ def build_case(subdiv):
 
 
 def run_path(ctx, dms, case, prec, profile=0, tol=1e-12):
-    from mom2amd import DEGTORAD, System, cs_prtsrf
+    from mom2amd import DEGTORAD, System, cs_prtsrf, rcs
     from mom2amd import capi
 
     N = case["problem"].nedges
@@ -71,6 +71,8 @@ def run_path(ctx, dms, case, prec, profile=0, tol=1e-12):
     st = S.stats()
     st.update(wall_s=wall, iters=S.iters, berr=S.berr)
     cs = cs_prtsrf(ctx, dms[0], N, case["omega"], case["ri"][0], x, *ang)
+    # bistatic radar cross-section on the reference's 37 x 72 grid, outside the timed region
+    st["rcs"] = rcs(ctx, dms[0], N, case["omega"], case["ri"][0], x, 37, 72)
     S.close()
     return st, cs
 
@@ -205,6 +207,7 @@ def main():
                             "gemm_frac_of_fp64_peak": None if ach64 is None else round(ach64 / PEAK_FP64_MFMA, 4)}
         out["speedup_vs_fp64_path"] = round(fp64["wall_s"] / mixed_s, 3)
         out["sigma_relerr_vs_fp64_path"] = {"sca": abs(cs_mixed[0] / cs64[0] - 1), "abs": abs(cs_mixed[1] / cs64[1] - 1)}
+        out["rcs_relerr_vs_fp64_path"] = float(np.abs(last["rcs"] - st64["rcs"]).max() / np.abs(st64["rcs"]).max())
         out["sigma"] = {"sca_m2": cs_mixed[0], "abs_m2": cs_mixed[1]}
     ctx.close()
     if not args.no_cpu:

@@ -451,6 +451,70 @@ int mom_cs_prtsrf(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, doub
   return 0;
 }
 
+int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                   double ri_im, const double *x, double r, int64_t ndir, const double *theta, const double *phi,
+                   int32_t nq, const double *qw, const double *qnodes, double *out) {
+  if (!ctx || !mesh || !x || !theta || !phi || !out) return fail(MOM_ERR_ARG, "NULL argument");
+  if (!(omega > 0) || !(r > 0)) return fail(MOM_ERR_ARG, "omega and r must be > 0");
+  if (nedges_total < 1 || mesh->max_parent >= nedges_total) return fail(MOM_ERR_ARG, "nedges_total too small");
+  QuadRule rule;
+  int rc = make_rule(nq, qw, qnodes, &rule);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const int64_t N2 = 2 * nedges_total;
+  DevBuf dx;
+  if ((rc = dx.alloc(sizeof(double2) * N2)) || (rc = copy_h2d(ctx, dx.ptr, x, sizeof(double2) * N2))) return rc;
+  const cd ri(ri_re, ri_im), I(0.0, 1.0);
+  const cd k = ri * omega / kC0;
+  std::vector<cd> v;
+  if ((rc = far_field_integrals(ctx, mesh, rule, k, dx.as<double2>(), nedges_total, ndir, theta, phi, v))) return rc;
+  // ffields.f90:107-113
+  const cd prefix = I * k * std::exp(I * k * r) / (4.0 * MOM_PI * r);
+  for (int64_t d = 0; d < ndir; d++) {
+    const cd Nt = v[4 * d], Np = v[4 * d + 1], Lt = v[4 * d + 2], Lp = v[4 * d + 3];
+    const cd f[4] = {prefix * (Lp + kEta0 / ri * Nt), -prefix * (Lt - kEta0 / ri * Np),
+                     -prefix * (Np - Lt * ri / kEta0), prefix * (Nt + Lp * ri / kEta0)};
+    for (int c = 0; c < 4; c++) {
+      out[8 * d + 2 * c] = f[c].real();
+      out[8 * d + 2 * c + 1] = f[c].imag();
+    }
+  }
+  return 0;
+}
+
+int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                double ri_im, const double *x, int32_t ntheta, int32_t nphi, int32_t filter, const double *fdir,
+                int32_t nq, const double *qw, const double *qnodes, double *rcs) {
+  if (!ctx || !mesh || !x || !rcs || (filter && !fdir)) return fail(MOM_ERR_ARG, "NULL argument");
+  if (ntheta < 2 || nphi < 2) return fail(MOM_ERR_ARG, "ntheta and nphi must be >= 2");
+  const int64_t nd = (int64_t)ntheta * nphi;
+  // ffields.f90:233-234: theta = pi (n-1)/(ntheta-1), phi = 2 pi (m-1)/(nphi-1), both ends included
+  std::vector<double> th(nd), ph(nd), f(8 * nd);
+  for (int m = 0; m < nphi; m++)
+    for (int n = 0; n < ntheta; n++) {
+      th[(int64_t)m * ntheta + n] = MOM_PI * (double)n / (double)(ntheta - 1);
+      ph[(int64_t)m * ntheta + n] = 2.0 * MOM_PI * (double)m / (double)(nphi - 1);
+    }
+  int rc = mom_far_fields(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, 1.0, nd, th.data(), ph.data(), nq, qw,
+                          qnodes, f.data());
+  if (rc) return rc;
+  for (int64_t d = 0; d < nd; d++) {
+    const cd et(f[8 * d], f[8 * d + 1]), ep(f[8 * d + 2], f[8 * d + 3]);
+    if (!filter) {
+      rcs[d] = std::norm(et) + std::norm(ep);  // ffields.f90:238, c = 1
+    } else {
+      // ffields.f90:240-243: |fdir . e|^2, e = et v_theta + ep v_phi (dotc does not conjugate)
+      const double t = th[d], p = ph[d];
+      const double vt[3] = {std::cos(p) * std::cos(t), std::sin(p) * std::cos(t), -std::sin(t)};
+      const double vp[3] = {-std::sin(p), std::cos(p), 0.0};
+      cd s = 0;
+      for (int c = 0; c < 3; c++) s += fdir[c] * (et * vt[c] + ep * vp[c]);
+      rcs[d] = std::norm(s);
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"
 
 extern "C" int mom_sysmat_pmchwt(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms,

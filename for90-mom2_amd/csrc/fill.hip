@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include <cmath>
 
@@ -1162,6 +1163,154 @@ int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
                      m->fn.as<double>(), sides(m, true), x, Nt, S, k.real(), k.imag(),
                      make_double2(hfac.real(), hfac.imag()), partial);
   MOM_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Far fields (ffields.f90:16-122, identity group) for many directions at
+// once.  The direction-independent part of far_fields, the surface currents
+// at the quadrature points weighted by w_t A_n, is formed once per point:
+// J(s) = w_t A_n sum_q x[g_q] f_q(r_t) and M(s) likewise with x[g_q + N].
+// Per direction the four radiation integrals are then one pass over the
+// points: Nt, Np = sum_s phasor(s) (u_theta, u_phi) . J(s), Lt, Lp from M,
+// phasor = exp(-i k dir . r_s).  The reference sums edge by edge within a
+// face; here the three edges of a face are summed at each point first, a
+// difference at the level of rounding.
+__global__ void farfield_sources_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
+                                        const double *__restrict__ area, FaceSide side,
+                                        const double2 *__restrict__ x, int64_t Nt, double *__restrict__ pts,
+                                        double2 *__restrict__ cur, int64_t S) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int f = (int)(s / Q), t = (int)(s - (int64_t)f * Q);
+  const double *r = qp + s * 3;
+  double J[3][2] = {{0, 0}, {0, 0}, {0, 0}}, M[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int p = 0; p < 3; p++) {
+    const int g = side.g[3 * f + p];
+    if (g < 0) continue;
+    const double cf = side.coef[3 * f + p];
+    const double *v = side.v + 3 * (3 * f + p);
+    const double fv[3] = {cf * (r[0] - v[0]), cf * (r[1] - v[1]), cf * (r[2] - v[2])};
+    const double2 xj = x[g], xm = x[g + Nt];
+    for (int c = 0; c < 3; c++) {
+      J[c][0] += xj.x * fv[c];
+      J[c][1] += xj.y * fv[c];
+      M[c][0] += xm.x * fv[c];
+      M[c][1] += xm.y * fv[c];
+    }
+  }
+  const double wa = rule.w[t] * area[f];
+  for (int c = 0; c < 3; c++) {
+    pts[3 * s + c] = r[c];
+    cur[c * S + s] = make_double2(wa * J[c][0], wa * J[c][1]);
+    cur[(3 + c) * S + s] = make_double2(wa * M[c][0], wa * M[c][1]);
+  }
+}
+
+constexpr int kFfTile = 256, kFfChunk = 16 * kFfTile;  // points per LDS tile and per partial sum
+
+// One thread per direction; blockIdx.y takes kFfChunk points, staged through
+// LDS a tile at a time (every thread reads the same point: broadcast).
+// dirs[8 d ..]: dir (st cp, st sp, ct), u_theta (ct cp, ct sp, -st), u_phi
+// (-sp, cp).  part[chunk][d][4] = (Nt, Np, Lt, Lp) over the chunk.
+__global__ __launch_bounds__(256) void farfield_dir_kernel(int64_t S, const double *__restrict__ pts,
+                                                            const double2 *__restrict__ cur, int ndir,
+                                                            const double *__restrict__ dirs, double kr,
+                                                            double ki, double2 *__restrict__ part) {
+  __shared__ double sp[3][kFfTile];
+  __shared__ double2 sc[6][kFfTile];
+  const int tid = threadIdx.x, d = blockIdx.x * 256 + tid;
+  const int64_t s0 = (int64_t)blockIdx.y * kFfChunk, s1 = std::min<int64_t>(S, s0 + kFfChunk);
+  double u[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (d < ndir)
+    for (int c = 0; c < 8; c++) u[c] = dirs[8 * d + c];
+  double a[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  for (int64_t tile = s0; tile < s1; tile += kFfTile) {
+    const int64_t i = tile + tid;
+    if (i < s1) {
+      for (int c = 0; c < 3; c++) sp[c][tid] = pts[3 * i + c];
+      for (int c = 0; c < 6; c++) sc[c][tid] = cur[c * S + i];
+    }
+    __syncthreads();
+    const int nt = (int)std::min<int64_t>(kFfTile, s1 - tile);
+    for (int q = 0; q < nt; q++) {
+      const double rd = u[0] * sp[0][q] + u[1] * sp[1][q] + u[2] * sp[2][q];
+      double er, ei;
+      cis_exp(ki * rd, -kr * rd, er, ei);  // exp(-i k rd)
+      double2 pr[4];
+      pr[0] = make_double2(u[3] * sc[0][q].x + u[4] * sc[1][q].x + u[5] * sc[2][q].x,
+                           u[3] * sc[0][q].y + u[4] * sc[1][q].y + u[5] * sc[2][q].y);
+      pr[1] = make_double2(u[6] * sc[0][q].x + u[7] * sc[1][q].x, u[6] * sc[0][q].y + u[7] * sc[1][q].y);
+      pr[2] = make_double2(u[3] * sc[3][q].x + u[4] * sc[4][q].x + u[5] * sc[5][q].x,
+                           u[3] * sc[3][q].y + u[4] * sc[4][q].y + u[5] * sc[5][q].y);
+      pr[3] = make_double2(u[6] * sc[3][q].x + u[7] * sc[4][q].x, u[6] * sc[3][q].y + u[7] * sc[4][q].y);
+      for (int k = 0; k < 4; k++) {
+        a[k][0] += pr[k].x * er - pr[k].y * ei;
+        a[k][1] += pr[k].x * ei + pr[k].y * er;
+      }
+    }
+    __syncthreads();
+  }
+  if (d < ndir)
+    for (int k = 0; k < 4; k++) part[((int64_t)blockIdx.y * ndir + d) * 4 + k] = make_double2(a[k][0], a[k][1]);
+}
+
+// out[d][4] = sum over the chunks, in chunk order
+__global__ void farfield_reduce_kernel(int ndir, int nchunk, const double2 *__restrict__ part, double2 *out) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndir) return;
+  for (int k = 0; k < 4; k++) {
+    double re = 0, im = 0;
+    for (int c = 0; c < nchunk; c++) {
+      const double2 v = part[((int64_t)c * ndir + d) * 4 + k];
+      re += v.x;
+      im += v.y;
+    }
+    out[4 * d + k] = make_double2(re, im);
+  }
+}
+
+int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, cd k, const double2 *x,
+                        int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out) {
+  if (!(std::fabs(k.real()) * m->extent < kCisMaxArg))
+    return fail(MOM_ERR_ARG, "Re(k) times the mesh extent is beyond the far field's phase range (2^19 rad)");
+  if (ndir < 1 || ndir > (1 << 24)) return fail(MOM_ERR_ARG, "number of directions out of range");
+  int rc = compute_quad_points(ctx, m, rule);
+  if (rc) return rc;
+  const int64_t S = m->F * rule.Q;
+  const int nchunk = (int)((S + kFfChunk - 1) / kFfChunk);
+  std::vector<double> dirs(8 * ndir);
+  for (int64_t d = 0; d < ndir; d++) {
+    const double st = std::sin(theta[d]), ct = std::cos(theta[d]), sp = std::sin(phi[d]), cp = std::cos(phi[d]);
+    double *u = dirs.data() + 8 * d;
+    u[0] = st * cp;
+    u[1] = st * sp;
+    u[2] = ct;
+    u[3] = ct * cp;
+    u[4] = ct * sp;
+    u[5] = -st;
+    u[6] = -sp;
+    u[7] = cp;
+  }
+  DevBuf pts, cur, ddir, part, red;
+  if ((rc = pts.alloc(sizeof(double) * 3 * S)) || (rc = cur.alloc(sizeof(double2) * 6 * S)) ||
+      (rc = ddir.alloc(sizeof(double) * 8 * ndir)) || (rc = part.alloc(sizeof(double2) * 4 * ndir * nchunk)) ||
+      (rc = red.alloc(sizeof(double2) * 4 * ndir)))
+    return rc;
+  MOM_TRY(copy_h2d(ctx, ddir.ptr, dirs.data(), sizeof(double) * 8 * ndir));
+  hipLaunchKernelGGL(farfield_sources_kernel, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, ctx->stream, (int)m->F,
+                     rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(), sides(m, true), x, Nt,
+                     pts.as<double>(), cur.as<double2>(), S);
+  hipLaunchKernelGGL(farfield_dir_kernel, dim3((unsigned)((ndir + 255) / 256), (unsigned)nchunk), dim3(256), 0,
+                     ctx->stream, S, pts.as<double>(), cur.as<double2>(), (int)ndir, ddir.as<double>(), k.real(),
+                     k.imag(), part.as<double2>());
+  hipLaunchKernelGGL(farfield_reduce_kernel, dim3((unsigned)((ndir + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (int)ndir, nchunk, part.as<double2>(), red.as<double2>());
+  MOM_HIP(hipGetLastError());
+  std::vector<double2> h(4 * ndir);
+  MOM_TRY(copy_d2h(ctx, h.data(), red.ptr, sizeof(double2) * 4 * ndir));
+  out.resize(4 * ndir);
+  for (int64_t i = 0; i < 4 * ndir; i++) out[i] = cd(h[i].x, h[i].y);
   return 0;
 }
 

@@ -142,5 +142,9 @@ int fill_system(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const d
                 const double *qnodes, double2 *Z, int64_t ldz);
 int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
                 const double2 *x, int64_t Nt, double th, double ph, double ps, double *partial);
+// (Nt, Np, Lt, Lp) of far_fields (ffields.f90:16-122) for ndir directions,
+// x the device solution [2 Nt], k the medium's wavenumber
+int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, cd k, const double2 *x,
+                        int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out);
 
 }  // namespace mom

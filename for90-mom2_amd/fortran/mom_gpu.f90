@@ -1,8 +1,9 @@
 ! MODULE: mom_gpu
 ! ISO_C_BINDING interface of libmomgpu.so (include/mom_gpu.h) for Fortran
-! hosts.  The drop-in bodies of sysmat_pmchwt, computeD/H/K, srcvec and
-! solve_multi_linsys (sysmat.f90:18, 272, 359, 446; source.f90:136;
-! linalg.f90:159) call these functions; see INTEGRATION.md for the bodies.
+! hosts.  The drop-in bodies of sysmat_pmchwt, computeD/H/K, srcvec,
+! solve_multi_linsys, cs_prtsrf, far_fields and rcs_ext (sysmat.f90:18,
+! 272, 359, 446; source.f90:136; linalg.f90:159; cs.f90:14;
+! ffields.f90:16, 203) call these functions.
 !
 ! Conventions: COMPLEX(C_DOUBLE_COMPLEX) arrays are passed by address,
 ! column-major with an explicit leading dimension; mesh index arrays are
@@ -147,6 +148,30 @@ MODULE mom_gpu
        REAL(C_DOUBLE) :: csca, cabs
        INTEGER(C_INT) :: st
      END FUNCTION mom_cs_prtsrf
+
+     FUNCTION mom_far_fields(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, r, ndir, theta, phi, nq, qw, &
+          qnodes, out) BIND(C, NAME='mom_far_fields') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, mesh, x
+       INTEGER(C_INT64_T), VALUE :: nedges_total, ndir
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im, r
+       REAL(C_DOUBLE), INTENT(IN) :: theta(*), phi(*), qw(*), qnodes(*)
+       INTEGER(C_INT32_T), VALUE :: nq
+       REAL(C_DOUBLE) :: out(*)
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_far_fields
+
+     FUNCTION mom_rcs_ext(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, ntheta, nphi, filter, fdir, nq, qw, &
+          qnodes, rcs) BIND(C, NAME='mom_rcs_ext') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, mesh, x
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       INTEGER(C_INT32_T), VALUE :: ntheta, nphi, filter, nq
+       REAL(C_DOUBLE), INTENT(IN) :: fdir(3), qw(*), qnodes(*)
+       REAL(C_DOUBLE) :: rcs(*)
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_rcs_ext
   END INTERFACE
 
 CONTAINS

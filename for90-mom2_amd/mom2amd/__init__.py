@@ -6,4 +6,4 @@ source.f90, linalg.f90, cs.f90); all arithmetic runs in the HIP library.
 from .constants import C0, DEGTORAD, EPS0, ETA0, MU0, PI, omega_of, tri_quad_data  # noqa: F401
 from .mesh import DomainMesh, Problem, build_problem, load_problem  # noqa: F401
 from .gpu import (Context, DeviceMesh, System, computeD, computeH, computeK,  # noqa: F401
-                  cs_prtsrf, solve_multi_linsys, srcvec, sysmat_pmchwt)
+                  cs_prtsrf, far_fields, rcs, rcs_ext, solve_multi_linsys, srcvec, sysmat_pmchwt)

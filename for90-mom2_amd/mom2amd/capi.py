@@ -23,7 +23,8 @@ EXPORTS = [
     "mom_mesh_near_pairs", "mom_compute_operator", "mom_sysmat_pmchwt", "mom_srcvec_pw",
     "mom_solve_multi_linsys", "mom_system_create", "mom_system_destroy", "mom_system_fill",
     "mom_system_get_matrix", "mom_system_set_matrix", "mom_system_factor", "mom_system_solve",
-    "mom_system_solve_pw", "mom_system_profile", "mom_system_stats", "mom_cs_prtsrf",
+    "mom_system_solve_pw", "mom_system_profile", "mom_system_stats", "mom_cs_prtsrf", "mom_far_fields",
+    "mom_rcs_ext",
 ]
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
@@ -84,6 +85,8 @@ def lib():
     L.mom_test_unit_lower_inverse.argtypes = [_vp, _vp, _i64, _vp]
     L.mom_cs_prtsrf.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _d, _d, _i32, _dp, _dp, C.POINTER(_d),
                                 C.POINTER(_d)]
+    L.mom_far_fields.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _i64, _dp, _dp, _i32, _dp, _dp, _dp]
+    L.mom_rcs_ext.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _i32, _i32, _i32, C.c_void_p, _i32, _dp, _dp, _dp]
     _lib = L
     return L
 

@@ -243,3 +243,40 @@ class System:
             self.close()
         except Exception:
             pass
+
+
+def far_fields(ctx, dmesh, nedges_total, omega, ri, x, r, theta, phi, quad=None):
+    """far_fields (ffields.f90:16-122) for arrays of directions theta, phi
+    [rad] at distance r: returns (et, ep, ht, hp), complex arrays of the
+    directions' shape."""
+    Q, w, n = _quad(quad)
+    th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+    ph = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+    if th.shape != ph.shape:
+        raise ValueError("theta and phi must have the same number of directions")
+    xv = np.ascontiguousarray(x, dtype=np.complex128).view(np.float64)
+    out = np.zeros(8 * th.size)
+    ri = complex(ri)
+    capi.check(capi.lib().mom_far_fields(ctx.h, dmesh.h, nedges_total, omega, ri.real, ri.imag, xv, r, th.size,
+                                         th, ph, Q, w, n, out))
+    f = out.view(np.complex128).reshape(-1, 4)
+    shape = np.shape(theta)
+    return tuple(f[:, c].reshape(shape) for c in range(4))
+
+
+def rcs_ext(ctx, dmesh, nedges_total, omega, ri, x, ntheta, nphi, filter=False, fdir=(0.0, 0.0, 0.0), quad=None):
+    """rcs_ext (ffields.f90:203-249): ntheta x nphi array of |E_far|^2 at
+    r = 1, optionally through the polarisation filter fdir."""
+    Q, w, n = _quad(quad)
+    xv = np.ascontiguousarray(x, dtype=np.complex128).view(np.float64)
+    out = np.zeros(ntheta * nphi)
+    fd = np.ascontiguousarray(fdir, dtype=np.float64)
+    ri = complex(ri)
+    capi.check(capi.lib().mom_rcs_ext(ctx.h, dmesh.h, nedges_total, omega, ri.real, ri.imag, xv, ntheta, nphi,
+                                      1 if filter else 0, fd.ctypes.data, Q, w, n, out))
+    return out.reshape(ntheta, nphi, order="F")
+
+
+def rcs(ctx, dmesh, nedges_total, omega, ri, x, ntheta, nphi, quad=None):
+    """rcs (ffields.f90:251-265): rcs_ext without the filter."""
+    return rcs_ext(ctx, dmesh, nedges_total, omega, ri, x, ntheta, nphi, quad=quad)

@@ -178,6 +178,27 @@ int mom_cs_prtsrf(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, doub
                   double ri_re, double ri_im, const double *x, double theta, double phi, double psi,
                   int32_t nq, const double *qw, const double *qnodes, double *csca, double *cabs);
 
+/*
+ * far_fields (ffields.f90:16-122), identity group, for ndir directions at
+ * once: the solution x[2N] (host, complex128) radiates through the domain
+ * mesh (normally domain 1) in the medium of index ri; for direction d,
+ * angles theta[d], phi[d] [rad], distance r, out[8d .. 8d+7] = et, ep, ht,
+ * hp (complex, interleaved).
+ */
+int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                   double ri_im, const double *x, double r, int64_t ndir, const double *theta, const double *phi,
+                   int32_t nq, const double *qw, const double *qnodes, double *out);
+/*
+ * rcs_ext (ffields.f90:203-249): bistatic radar cross-section on the grid
+ * theta = pi i/(ntheta-1), phi = 2 pi j/(nphi-1) (both ends included, pi as
+ * the reference's), r = 1: rcs[i + ntheta j] = |et|^2 + |ep|^2, or with
+ * filter != 0, |fdir . (et v_theta + ep v_phi)|^2 (no conjugation, as dotc).
+ * rcs(...) of the reference is filter = 0.
+ */
+int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                double ri_im, const double *x, int32_t ntheta, int32_t nphi, int32_t filter, const double *fdir,
+                int32_t nq, const double *qw, const double *qnodes, double *rcs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,3 +100,55 @@ def test_phase_range_is_checked_before_launch(gpu_ctx, msh_dir):
     # the context stays usable
     M = computeD(gpu_ctx, dm, case["omega"], case["ri"][0])
     assert np.isfinite(M).all()
+
+
+@pytest.mark.parametrize("name", ["sphere_s1", "sphere_s2", "dimer_s1"])
+def test_rcs_golden(gpu_ctx, msh_dir, name):
+    """rcs on the 37 x 72 grid against the reference's own values for the
+    same solution x (ffields.f90:203-265)."""
+    from mom2amd import rcs
+
+    g = golden(name)
+    case = {"sphere_s1": lambda: sphere(msh_dir, 1), "sphere_s2": lambda: sphere(msh_dir, 2),
+            "dimer_s1": lambda: dimer(msh_dir, 1)}[name]()
+    dm = DeviceMesh(gpu_ctx, case["problem"].domains[0])
+    r = rcs(gpu_ctx, dm, case["problem"].nedges, case["omega"], case["ri"][0], g["x"], 37, 72)
+    err = np.abs(r - g["rcs"]).max() / np.abs(g["rcs"]).max()
+    print("%s rcs err %.3e" % (name, err), flush=True)
+    assert err < 1e-12
+
+
+def test_far_fields_relations(gpu_ctx, msh_dir):
+    """Closed-form relations of the far field that a slip in far_fields'
+    assembly would break: H = (ri/eta0) r x E (ht = -(ri/eta0) ep, hp =
+    (ri/eta0) et), the exp(ikr)/r dependence on the distance, the filter
+    (the filtered values for x, y and z sum to the unfiltered ones, since
+    v_theta, v_phi are orthonormal) and the grid of rcs_ext."""
+    from mom2amd import ETA0, far_fields, rcs_ext
+
+    g = golden("sphere_s1")
+    case = sphere(msh_dir, 1)
+    dm = DeviceMesh(gpu_ctx, case["problem"].domains[0])
+    N, om, ri = case["problem"].nedges, case["omega"], case["ri"][0]
+    rng = np.random.default_rng(3)
+    th, ph = rng.uniform(0, np.pi, 50), rng.uniform(0, 2 * np.pi, 50)
+    et, ep, ht, hp = far_fields(gpu_ctx, dm, N, om, ri, g["x"], 1.0, th, ph)
+    s = np.abs(et).max()
+    assert np.abs(ht + ri / ETA0 * ep).max() < 1e-12 * s * abs(ri / ETA0)
+    assert np.abs(hp - ri / ETA0 * et).max() < 1e-12 * s * abs(ri / ETA0)
+    # distance: the phase k r is about 1e7 rad at r = 1 m, so the prefactor
+    # carries an absolute phase rounding of |k r| u (a few 1e-9 relative)
+    k = ri * om / 2.997924580003e8
+    et2, ep2, _, _ = far_fields(gpu_ctx, dm, N, om, ri, g["x"], 3.0, th, ph)
+    fac = np.exp(1j * k * 2.0) / 3.0
+    tol = 1e-15 * abs(k) * 3.0
+    assert np.abs(et2 - fac * et).max() < tol * s
+    assert np.abs(ep2 - fac * ep).max() < tol * s
+    full = rcs_ext(gpu_ctx, dm, N, om, ri, g["x"], 9, 12)
+    parts = [rcs_ext(gpu_ctx, dm, N, om, ri, g["x"], 9, 12, filter=True, fdir=e) for e in np.eye(3)]
+    assert np.abs(sum(parts) - full).max() < 1e-12 * full.max()
+    # the grid: theta = pi i/8, phi = 2 pi j/11 (pi as the reference's float32-rounded value)
+    from mom2amd import PI
+    T, P = np.meshgrid(PI * np.arange(9) / 8, 2 * PI * np.arange(12) / 11, indexing="ij")
+    et3, ep3, _, _ = far_fields(gpu_ctx, dm, N, om, ri, g["x"], 1.0, T, P)
+    assert np.abs(np.abs(et3) ** 2 + np.abs(ep3) ** 2 - full).max() < 1e-12 * full.max()
