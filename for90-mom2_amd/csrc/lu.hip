@@ -1925,6 +1925,135 @@ __global__ void dscale_kernel(const double2 *__restrict__ v, const double *__res
   y[r] = make_double2(v[r].x * d[r], v[r].y * d[r]);
 }
 
+// ---------------------------------------------------------------------------
+// Many right-hand sides: triangular solves and refinement as matrix
+// products (rocBLAS ZGEMM) on n x nrhs blocks, instead of one vector at a
+// time.
+
+// y[:, j] = d .* v[:, j]
+__global__ void dscale_cols_kernel(const double2 *__restrict__ v, int64_t ldv, const double *__restrict__ d,
+                                   int64_t n, double2 *y, int64_t ldy) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (r >= n) return;
+  const double2 a = v[j * ldv + r];
+  y[j * ldy + r] = make_double2(a.x * d[r], a.y * d[r]);
+}
+
+// x[:, j] += d .* z[:, j]
+__global__ void scaled_add_cols_kernel(double2 *x, int64_t ldx, const double *__restrict__ d,
+                                       const double2 *__restrict__ z, int64_t ldz, int64_t n) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (r >= n) return;
+  const double2 a = z[j * ldz + r];
+  double2 &o = x[j * ldx + r];
+  o.x += a.x * d[r];
+  o.y += a.y * d[r];
+}
+
+// out[j] = max_r |v[r, j]| (one workgroup per column)
+__global__ __launch_bounds__(256) void col_absmax_kernel(const double2 *__restrict__ v, int64_t ld, int64_t n,
+                                                          double *out) {
+  __shared__ double sm[4];
+  const int64_t j = blockIdx.x;
+  double m = 0;
+  for (int64_t r = threadIdx.x; r < n; r += 256) {
+    const double2 a = v[j * ld + r];
+    m = fmax(m, sqrt(a.x * a.x + a.y * a.y));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+}
+
+// columns j with keep[j] == 0 are set to zero
+__global__ void mask_cols_kernel(double2 *v, int64_t ld, int64_t n, const int *__restrict__ keep) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (r >= n || keep[j]) return;
+  v[j * ld + r] = make_double2(0.0, 0.0);
+}
+
+// P[r, c] = A[r, c] widened to complex128, rows x w
+__global__ void widen_panel_kernel(const float2 *__restrict__ A, int64_t ld, int64_t rows, double2 *P,
+                                   int64_t ldp) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (r >= rows) return;
+  const float2 a = A[c * ld + r];
+  P[c * ldp + r] = make_double2(a.x, a.y);
+}
+
+static int zgemm(rocblas_handle h, int64_t M, int64_t N, int64_t K, cd alpha, const double2 *A, int64_t lda,
+                 const double2 *B, int64_t ldb, cd beta, double2 *C, int64_t ldc) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  const rocblas_double_complex al(alpha.real(), alpha.imag()), be(beta.real(), beta.imag());
+  const rocblas_status rs =
+      rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, (rocblas_int)N, (rocblas_int)K,
+                    &al, reinterpret_cast<const rocblas_double_complex *>(A), (rocblas_int)lda,
+                    reinterpret_cast<const rocblas_double_complex *>(B), (rocblas_int)ldb, &be,
+                    reinterpret_cast<rocblas_double_complex *>(C), (rocblas_int)ldc);
+  if (rs != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_zgemm failed: " + std::to_string((int)rs));
+  return 0;
+}
+
+// Solve L U X = P B in place on B (n x nrhs, ldb), complex128 arithmetic
+// with the factors stored in TS: per 256-row block the diagonal-block
+// inverse (complex128) times the block, then the panel below (forward) or
+// above (backward) times the solved block subtracted from the rest; a
+// complex64 panel is first widened into P (n x NB).  T is n x nrhs and W
+// NB x nrhs scratch.
+template <typename TS>
+static int getrs_blocked(mom_ctx *ctx, const typename Cx<TS>::type *LU, int64_t ld, int64_t n, const int *perm,
+                         const double2 *Linv, const double2 *Uinv, double2 *B, int64_t ldb, int64_t nrhs,
+                         double2 *T, double2 *W, double2 *P) {
+  hipStream_t s = ctx->stream;
+  rocblas_handle h;
+  MOM_TRY(blas_handle(main_lane(ctx), &h));
+  hipLaunchKernelGGL(gather_rows_kernel<double2>, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, B, ldb,
+                     T, n, perm, n, nrhs);
+  MOM_HIP(hipGetLastError());
+  auto panel = [&](int64_t r0, int64_t rows, int64_t c0, int64_t w, const double2 **A, int64_t *lda) -> int {
+    if constexpr (std::is_same<TS, double>::value) {
+      *A = LU + c0 * ld + r0;
+      *lda = ld;
+    } else {
+      hipLaunchKernelGGL(widen_panel_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)w), dim3(256), 0, s,
+                         LU + c0 * ld + r0, ld, rows, P, rows);
+      MOM_HIP(hipGetLastError());
+      *A = P;
+      *lda = rows;
+    }
+    return 0;
+  };
+  const int64_t nblk = (n + NB - 1) / NB;
+  for (int64_t b = 0; b < nblk; b++) {
+    const int64_t r0 = b * NB, w = std::min<int64_t>(NB, n - r0);
+    MOM_TRY(zgemm(h, w, nrhs, w, 1.0, Linv + b * NB * NB, NB, T + r0, n, 0.0, W, NB));
+    MOM_HIP(hipMemcpy2DAsync(T + r0, n * sizeof(double2), W, NB * sizeof(double2), w * sizeof(double2), nrhs,
+                             hipMemcpyDeviceToDevice, s));
+    if (r0 + w < n) {
+      const double2 *A;
+      int64_t lda;
+      MOM_TRY(panel(r0 + w, n - r0 - w, r0, w, &A, &lda));
+      MOM_TRY(zgemm(h, n - r0 - w, nrhs, w, -1.0, A, lda, W, NB, 1.0, T + r0 + w, n));
+    }
+  }
+  for (int64_t b = nblk - 1; b >= 0; b--) {
+    const int64_t r0 = b * NB, w = std::min<int64_t>(NB, n - r0);
+    MOM_TRY(zgemm(h, w, nrhs, w, 1.0, Uinv + b * NB * NB, NB, T + r0, n, 0.0, W, NB));
+    MOM_HIP(hipMemcpy2DAsync(T + r0, n * sizeof(double2), W, NB * sizeof(double2), w * sizeof(double2), nrhs,
+                             hipMemcpyDeviceToDevice, s));
+    if (r0 > 0) {
+      const double2 *A;
+      int64_t lda;
+      MOM_TRY(panel(0, r0, r0, w, &A, &lda));
+      MOM_TRY(zgemm(h, r0, nrhs, w, -1.0, A, lda, W, NB, 1.0, T, n));
+    }
+  }
+  MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), T, n * sizeof(double2), n * sizeof(double2), nrhs,
+                           hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
 template <typename T>
 static int diag_inverses(hipStream_t s, const typename Cx<T>::type *LU, int64_t ld, int64_t n, double2 *Linv,
                          double2 *Uinv) {
@@ -2215,6 +2344,121 @@ static int gmres_ir(mom_system *S, IrWork &W, const std::vector<cd> &b, std::vec
   return be <= tol ? 0 : fail(MOM_ERR_NOCONV, "GMRES-IR did not reach the backward-error tolerance");
 }
 
+// From this many right-hand sides on, solves run on the whole block at once.
+constexpr int64_t kBatchRhs = 4;
+
+static int ir_alloc(mom_system *S, IrWork &W);
+
+// Mixed-precision refinement of a block of right-hand sides: X0 = M^-1 B,
+// then plain refinement steps R = B - Z X (one ZGEMM on the resident
+// complex128 Z), X += M^-1 R on the columns whose backward error is above
+// tol and still falling at least tenfold per step.  Columns that stall are
+// finished one by one by gmres_ir.  Backward errors per column as in
+// gmres_ir; be_hist records the largest over the columns.
+static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb, double tol, int maxit,
+                             int32_t *iters, double *berr) {
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  rocblas_handle h;
+  MOM_TRY(blas_handle(main_lane(S->ctx), &h));
+  DevBuf dB, dX, dR, dT, dT2, dW, dP, dperm, dn, dkeep;
+  int rc;
+  const size_t blk = sizeof(double2) * n * nrhs;
+  if ((rc = dB.alloc(blk)) || (rc = dX.alloc(blk)) || (rc = dR.alloc(blk)) || (rc = dT.alloc(blk)) ||
+      (rc = dT2.alloc(blk)) || (rc = dW.alloc(sizeof(double2) * NB * nrhs)) ||
+      (rc = dP.alloc(sizeof(double2) * n * NB)) || (rc = upload_perm(S, dperm)) ||
+      (rc = dn.alloc(sizeof(double) * 2 * nrhs)) || (rc = dkeep.alloc(sizeof(int) * nrhs)))
+    return rc;
+  MOM_TRY(copy_h2d_2d(S->ctx, dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs));
+  const dim3 g2((unsigned)((n + 255) / 256), (unsigned)nrhs);
+  S->n_minv = S->n_matvec = 0;
+  for (int i = 0; i < kBeHistory; i++) S->be_hist[i] = -1;
+  // dst (= or +=) C U^-1 L^-1 P R src
+  auto minv = [&](const double2 *src, bool add) -> int {
+    S->n_minv++;
+    hipLaunchKernelGGL(dscale_cols_kernel, g2, dim3(256), 0, s, src, n, S->rscale.as<double>(), n, dT.as<double2>(),
+                       n);
+    MOM_TRY(getrs_blocked<float>(S->ctx, S->Z32.as<float2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
+                                 S->Uinv.as<double2>(), dT.as<double2>(), n, nrhs, dT2.as<double2>(), dW.as<double2>(),
+                                 dP.as<double2>()));
+    if (add)
+      hipLaunchKernelGGL(scaled_add_cols_kernel, g2, dim3(256), 0, s, dX.as<double2>(), n, S->cscale.as<double>(),
+                         dT.as<double2>(), n, n);
+    else
+      hipLaunchKernelGGL(dscale_cols_kernel, g2, dim3(256), 0, s, dT.as<double2>(), n, S->cscale.as<double>(), n,
+                         dX.as<double2>(), n);
+    MOM_HIP(hipGetLastError());
+    return 0;
+  };
+  auto colmax = [&](const double2 *v, std::vector<double> &out) -> int {
+    hipLaunchKernelGGL(col_absmax_kernel, dim3((unsigned)nrhs), dim3(256), 0, s, v, n, n, dn.as<double>());
+    MOM_HIP(hipGetLastError());
+    out.resize(nrhs);
+    return copy_d2h(S->ctx, out.data(), dn.ptr, sizeof(double) * nrhs);
+  };
+  std::vector<double> bn, rn, xn, be(nrhs, 0.0), prev(nrhs, 0.0);
+  MOM_TRY(colmax(dB.as<double2>(), bn));
+  MOM_TRY(minv(dB.as<double2>(), false));
+  int it = 0;
+  std::vector<int> keep(nrhs);
+  for (;; it++) {
+    S->n_matvec++;
+    MOM_HIP(hipMemcpyAsync(dR.ptr, dB.ptr, blk, hipMemcpyDeviceToDevice, s));
+    MOM_TRY(zgemm(h, n, nrhs, n, -1.0, S->Z64.as<double2>(), n, dX.as<double2>(), n, 1.0, dR.as<double2>(), n));
+    MOM_TRY(colmax(dR.as<double2>(), rn));
+    MOM_TRY(colmax(dX.as<double2>(), xn));
+    double bmax = 0;
+    int nkeep = 0;
+    for (int64_t j = 0; j < nrhs; j++) {
+      be[j] = rn[j] / (S->znorm_inf * xn[j] + bn[j]);
+      bmax = std::max(bmax, be[j]);
+      keep[j] = be[j] > tol && (it == 0 || be[j] <= 0.1 * prev[j]);
+      nkeep += keep[j];
+    }
+    if (it < kBeHistory) S->be_hist[it] = bmax;
+    if (bmax <= tol || it >= maxit || nkeep == 0) break;
+    MOM_TRY(copy_h2d(S->ctx, dkeep.ptr, keep.data(), sizeof(int) * nrhs));
+    hipLaunchKernelGGL(mask_cols_kernel, g2, dim3(256), 0, s, dR.as<double2>(), n, n, dkeep.as<int>());
+    MOM_TRY(minv(dR.as<double2>(), true));
+    prev = be;
+  }
+  MOM_TRY(copy_d2h_2d(S->ctx, B, ldb * sizeof(double2), dX.ptr, n * sizeof(double2), n * sizeof(double2), nrhs));
+  // columns that stalled: restarted GMRES on M^-1 Z, one at a time, from the
+  // original right-hand side (still in dB); the block's statistics are kept
+  const int nm = S->n_minv, nv = S->n_matvec;
+  double hist[kBeHistory];
+  std::copy(S->be_hist, S->be_hist + kBeHistory, hist);
+  IrWork W;
+  bool have_w = false;
+  int itmax = it, extra_m = 0, extra_v = 0;
+  double bemax = 0;
+  for (int64_t j = 0; j < nrhs; j++) {
+    if (be[j] <= tol) {
+      bemax = std::max(bemax, be[j]);
+      continue;
+    }
+    if (!have_w && (rc = ir_alloc(S, W))) return rc;
+    have_w = true;
+    std::vector<cd> b(n), x;
+    MOM_TRY(copy_d2h(S->ctx, b.data(), dB.as<double2>() + j * n, sizeof(double2) * n));
+    int itj = 0;
+    double bej = 0;
+    rc = gmres_ir(S, W, b, x, tol, maxit, &itj, &bej);
+    if (rc && rc != MOM_ERR_NOCONV) return rc;
+    extra_m += S->n_minv;
+    extra_v += S->n_matvec;
+    itmax = std::max(itmax, it + itj);
+    bemax = std::max(bemax, bej);
+    std::memcpy(B + 2 * j * ldb, x.data(), sizeof(double2) * n);
+  }
+  S->n_minv = nm + extra_m;
+  S->n_matvec = nv + extra_v;
+  std::copy(hist, hist + kBeHistory, S->be_hist);
+  if (iters) *iters = itmax;
+  if (berr) *berr = bemax;
+  return bemax <= tol ? 0 : fail(MOM_ERR_NOCONV, "GMRES-IR did not reach the backward-error tolerance");
+}
+
 static int ir_alloc(mom_system *S, IrWork &W) {
   const int64_t n = S->n;
   int rc;
@@ -2356,12 +2600,19 @@ int mom_system_solve(mom_system *S, int64_t nrhs, double *B, int64_t ldb, double
         (rc = xb.alloc(sizeof(double2) * NB * nrhs)) || (rc = upload_perm(S, dperm)))
       return rc;
     MOM_TRY(copy_h2d_2d(S->ctx, dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs));
-    if ((rc = getrs<double, double>(s, S->Z64.as<double2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
-                            S->Uinv.as<double2>(), dB.as<double2>(), n, nrhs, tmp.as<double2>(), xb.as<double2>())))
-      return rc;
+    if (nrhs >= kBatchRhs)
+      rc = getrs_blocked<double>(S->ctx, S->Z64.as<double2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
+                                 S->Uinv.as<double2>(), dB.as<double2>(), n, nrhs, tmp.as<double2>(),
+                                 xb.as<double2>(), nullptr);
+    else
+      rc = getrs<double, double>(s, S->Z64.as<double2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
+                                 S->Uinv.as<double2>(), dB.as<double2>(), n, nrhs, tmp.as<double2>(), xb.as<double2>());
+    if (rc) return rc;
     MOM_TRY(copy_d2h_2d(S->ctx, B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs));
     if (iters) *iters = 0;
     if (berr) *berr = -1.0;
+  } else if (nrhs >= kBatchRhs) {
+    if ((rc = mixed_solve_batch(S, nrhs, B, ldb, tol, maxit, iters, berr))) return rc;
   } else {
     IrWork W;
     if ((rc = ir_alloc(S, W))) return rc;

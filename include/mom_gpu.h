@@ -137,7 +137,10 @@ int mom_system_factor(mom_system *sys);
  * refinement stops at backward error ||b - Zx|| / (||Z|| ||x|| + ||b||)
  * (infinity norms) <= tol or after maxit refinement steps; iters and berr
  * (may be NULL) return the steps taken and the final backward error of the
- * worst column.  MOM_ERR_NOCONV if tol is not met.
+ * worst column.  MOM_ERR_NOCONV if tol is not met.  From 4 right-hand sides
+ * on, the triangular solves and the refinement run on the whole block
+ * (matrix products per 256-row block, one residual product per step);
+ * columns whose refinement stalls are finished one by one by GMRES.
  */
 int mom_system_solve(mom_system *sys, int64_t nrhs, double *B, int64_t ldb, double tol,
                      int32_t maxit, int32_t *iters, double *berr);
