@@ -3,7 +3,9 @@
 ! hosts.  The drop-in bodies of sysmat_pmchwt, computeD/H/K, srcvec,
 ! solve_multi_linsys, cs_prtsrf, far_fields and rcs_ext (sysmat.f90:18,
 ! 272, 359, 446; source.f90:136; linalg.f90:159; cs.f90:14;
-! ffields.f90:16, 203) call these functions.
+! ffields.f90:16, 203) call these functions, and the mesh/sdom set-up
+! (build_mesh, extract_submesh, submesh_edge_connectivity, orient_basis of
+! mesh.f90) can be replaced by mom_problem_build.
 !
 ! Conventions: COMPLEX(C_DOUBLE_COMPLEX) arrays are passed by address,
 ! column-major with an explicit leading dimension; mesh index arrays are
@@ -172,6 +174,62 @@ MODULE mom_gpu
        REAL(C_DOUBLE) :: rcs(*)
        INTEGER(C_INT) :: st
      END FUNCTION mom_rcs_ext
+
+     ! Problem set-up (mesh + sdom commands): dom_ptr(ndom+1) offsets into
+     ! the signed surface ids dom_ids, 0-based, dom_ptr(1) = 0.
+     FUNCTION mom_problem_build(nnodes, p, nfaces, fnodes, fids, scale, ndom, dom_ptr, dom_ids, prob) &
+          BIND(C, NAME='mom_problem_build') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       INTEGER(C_INT64_T), VALUE :: nnodes, nfaces
+       REAL(C_DOUBLE), INTENT(IN) :: p(*)
+       INTEGER(C_INT32_T), INTENT(IN) :: fnodes(*), fids(*), dom_ptr(*), dom_ids(*)
+       REAL(C_DOUBLE), VALUE :: scale
+       INTEGER(C_INT32_T), VALUE :: ndom
+       TYPE(C_PTR) :: prob
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_build
+
+     FUNCTION mom_problem_destroy(prob) BIND(C, NAME='mom_problem_destroy') RESULT(st)
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: prob
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_destroy
+
+     FUNCTION mom_problem_sizes(prob, dom, nedges_total, nnodes, nfaces, nedges) &
+          BIND(C, NAME='mom_problem_sizes') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR
+       TYPE(C_PTR), VALUE :: prob
+       INTEGER(C_INT32_T), VALUE :: dom
+       INTEGER(C_INT64_T) :: nedges_total, nnodes, nfaces, nedges
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_sizes
+
+     FUNCTION mom_problem_domain(prob, dom, p, fnodes, fedges, fn, fcp, fs, fm, farea, efaces, ebn, elen, &
+          eparent, avelen) BIND(C, NAME='mom_problem_domain') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: prob
+       INTEGER(C_INT32_T), VALUE :: dom
+       REAL(C_DOUBLE) :: p(*), fn(*), fcp(*), fs(*), fm(*), farea(*), elen(*), avelen
+       INTEGER(C_INT32_T) :: fnodes(*), fedges(*), efaces(*), ebn(*), eparent(*)
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_domain
+
+     FUNCTION mom_problem_topology(prob, dom, enodes, fid, fparent, nparent) &
+          BIND(C, NAME='mom_problem_topology') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_PTR
+       TYPE(C_PTR), VALUE :: prob
+       INTEGER(C_INT32_T), VALUE :: dom
+       INTEGER(C_INT32_T) :: enodes(*), fid(*), fparent(*), nparent(*)
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_topology
+
+     FUNCTION mom_problem_mesh_create(ctx, prob, dom, mesh) BIND(C, NAME='mom_problem_mesh_create') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, prob
+       INTEGER(C_INT32_T), VALUE :: dom
+       TYPE(C_PTR) :: mesh
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_mesh_create
   END INTERFACE
 
 CONTAINS

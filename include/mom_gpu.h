@@ -202,6 +202,44 @@ int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double
                 double ri_im, const double *x, int32_t ntheta, int32_t nphi, int32_t filter, const double *fdir,
                 int32_t nq, const double *qw, const double *qnodes, double *rcs);
 
+/*
+ * Problem set-up on the host (no GPU work): the master mesh and the domain
+ * meshes that the `mesh` and `sdom` commands build (interface.f90:357-366,
+ * 44-105), identity group, no periodic boundaries.  Replaces build_mesh
+ * (mesh.f90:1044-1137), compute_basis_data (1265-1322), extract_submesh
+ * (290-376), submesh_edge_connectivity (378-452) and orient_basis (454-558):
+ * the same edge numbering, orientation and geometry bits, found with hash
+ * maps instead of nested scans.
+ *   p[3*nnodes]        node coordinates in file units (scaled by `scale`)
+ *   fnodes[3*nfaces]   face nodes, 0-based; fids[nfaces] physical ids
+ *   domain d takes the surface ids dom_ids[dom_ptr[d] .. dom_ptr[d+1]-1]
+ *   (signed: a negative id reverses that surface's faces), dom_ptr[0] = 0.
+ */
+typedef struct mom_problem mom_problem;
+int mom_problem_build(int64_t nnodes, const double *p, int64_t nfaces, const int32_t *fnodes,
+                      const int32_t *fids, double scale, int32_t ndom, const int32_t *dom_ptr,
+                      const int32_t *dom_ids, mom_problem **prob);
+int mom_problem_destroy(mom_problem *prob);
+/* Sizes of domain dom (dom = -1: the master mesh); nedges_total is the
+ * master mesh's edge count, the number of RWG functions per current.  Any
+ * output pointer may be NULL. */
+int mom_problem_sizes(const mom_problem *prob, int32_t dom, int64_t *nedges_total, int64_t *nnodes,
+                      int64_t *nfaces, int64_t *nedges);
+/* Copy out domain dom (dom = -1: the master mesh) in mom_mesh_create's
+ * layout; eparent is -1 throughout for the master.  Any output pointer may
+ * be NULL. */
+int mom_problem_domain(const mom_problem *prob, int32_t dom, double *p, int32_t *fnodes, int32_t *fedges,
+                       double *fn, double *fcp, double *fs, double *fm, double *farea, int32_t *efaces,
+                       int32_t *ebn, double *elen, int32_t *eparent, double *avelen);
+/* The rest of the mesh_container (mesh.f90:75-83) of domain dom (-1:
+ * master): enodes[2*nedges] edge nodes, fid[nfaces] physical ids,
+ * fparent[nfaces] / nparent[nnodes] master face / node of each face / node
+ * (the identity on the master).  Any output pointer may be NULL. */
+int mom_problem_topology(const mom_problem *prob, int32_t dom, int32_t *enodes, int32_t *fid,
+                         int32_t *fparent, int32_t *nparent);
+/* mom_mesh_create of domain dom (0-based) straight from the set-up. */
+int mom_problem_mesh_create(mom_ctx *ctx, const mom_problem *prob, int32_t dom, mom_mesh **mesh);
+
 #ifdef __cplusplus
 }
 #endif

@@ -152,3 +152,48 @@ def test_far_fields_relations(gpu_ctx, msh_dir):
     T, P = np.meshgrid(PI * np.arange(9) / 8, 2 * PI * np.arange(12) / 11, indexing="ij")
     et3, ep3, _, _ = far_fields(gpu_ctx, dm, N, om, ri, g["x"], 1.0, T, P)
     assert np.abs(np.abs(et3) ** 2 + np.abs(ep3) ** 2 - full).max() < 1e-12 * full.max()
+
+
+def test_mesh_from_native_setup(gpu_ctx, msh_dir):
+    """mom_problem_mesh_create uploads the set-up's domains directly; the
+    meshes it makes give the same near lists and matrix as meshes uploaded
+    from the copied-out arrays."""
+    import ctypes as C
+
+    from mom2amd import capi, meshgen
+
+    case = dimer(msh_dir, 1)
+    L = capi.lib()
+    nodes, faces, ids = meshgen.read_gmsh(case["path"])
+    doms = case["domains"]
+    ptr = np.cumsum([0] + [len(s) for s in doms]).astype(np.int32)
+    di = np.array([s for d in doms for s in d], np.int32)
+    h = C.c_void_p()
+    capi.check(L.mom_problem_build(len(nodes), np.ascontiguousarray(nodes, np.float64).ravel(), len(faces),
+                                   np.ascontiguousarray(faces, np.int32).ravel(),
+                                   np.ascontiguousarray(ids, np.int32), 1e-9, len(doms), ptr, di, C.byref(h)))
+
+    class Handle:
+        def __init__(self, d):
+            self.h = C.c_void_p()
+            capi.check(L.mom_problem_mesh_create(gpu_ctx.h, h, d, C.byref(self.h)))
+
+    try:
+        native = [Handle(d) for d in range(len(doms))]
+        with pytest.raises(capi.MomError):
+            Handle(len(doms))
+    finally:
+        L.mom_problem_destroy(h)
+    try:
+        copied = _dev(gpu_ctx, case)
+        for a, b in zip(native, copied):
+            assert L.mom_mesh_near_pairs(a.h) == b.near_pairs
+        N = case["problem"].nedges
+        A = sysmat_pmchwt(gpu_ctx, native, case["ri"], N, case["omega"])
+        B = sysmat_pmchwt(gpu_ctx, copied, case["ri"], N, case["omega"])
+        err = np.abs(A - B).max() / np.abs(B).max()
+        print("native set-up vs copied arrays: %.3e" % err, flush=True)
+        assert err < 1e-14
+    finally:
+        for a in native:
+            L.mom_mesh_destroy(a.h)
