@@ -42,14 +42,14 @@ def log(msg):
 
 
 def build_case(subdiv):
-    from mom2amd import build_problem, meshgen, omega_of
+    from mom2amd import load_problem, meshgen, omega_of
 
     nodes, faces = meshgen.icosphere(subdiv, 50.0)
     d = tempfile.mkdtemp(prefix="mom_bench_")
     path = os.path.join(d, "ico%d.msh" % subdiv)
     meshgen.write_gmsh(path, nodes, faces, 15)
     doms = [[15], [-15]]
-    prob = build_problem(*meshgen.read_gmsh(path), 1e-9, doms)
+    prob = load_problem(path, 1e-9, doms)
     return dict(path=path, domains=doms, problem=prob, omega=omega_of(600e-9),
                 ri=[1.0 + 0j, complex(np.sqrt(-9.4 + 1.5j))])
 

@@ -1,7 +1,8 @@
 // Problem set-up on the host: master mesh, domain meshes and the RWG edge
 // numbering, for the identity group without periodic boundaries.
 //
-// Reference algorithm (for90-mom2): build_mesh (mesh.f90:1044-1137),
+// Reference algorithm (for90-mom2): load_mesh_gmsh (mesh.f90:610-736),
+// build_mesh (mesh.f90:1044-1137),
 // compute_basis_data (1265-1322), average_edge_length (138-153),
 // extract_submesh (290-376), submesh_edge_connectivity (378-452),
 // orient_basis (454-558); the `mesh` and `sdom` commands that drive them
@@ -25,8 +26,12 @@
 //    child, and, as in the reference, a success found only in the last pass
 //    is reported as a failure and ends the orientation.
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -248,6 +253,97 @@ static int connect_and_orient(const HostMesh &master, std::vector<HostMesh> &dom
   return 0;
 }
 
+// load_mesh_gmsh (mesh.f90:610-736): Gmsh 2.1/2.2 ASCII; node coordinates
+// stored at their node number (numbers 1..nnodes), triangles (element type
+// 2) with their first tag as the surface id; other elements are skipped.
+static std::string trimmed(const std::string &s) {
+  size_t a = s.find_first_not_of(" \t\r"), b = s.find_last_not_of(" \t\r");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+static bool parse_longs(const std::string &line, std::vector<long> &out) {
+  out.clear();
+  const char *c = line.c_str();
+  while (true) {
+    while (*c == ' ' || *c == '\t' || *c == '\r') c++;
+    if (!*c) return true;
+    char *end;
+    errno = 0;
+    const long v = std::strtol(c, &end, 10);
+    if (end == c || errno) return false;
+    out.push_back(v);
+    c = end;
+  }
+}
+
+static int read_gmsh(const char *path, std::vector<double> &p, std::vector<int32_t> &fnodes,
+                     std::vector<int32_t> &fids) {
+  std::ifstream in(path);
+  if (!in) return fail(MOM_ERR_ARG, std::string("Could not open mesh file! ") + path);
+  std::string line;
+  bool have_nodes = false, have_elems = false;
+  std::vector<long> v;
+  auto expect_end = [&](const char *tag, const char *msg) -> int {
+    if (!std::getline(in, line) || trimmed(line) != tag) return fail(MOM_ERR_ARG, msg);
+    return 0;
+  };
+  while (std::getline(in, line)) {
+    const std::string tok = trimmed(line);
+    if (tok == "$MeshFormat") {
+      if (!std::getline(in, line)) return fail(MOM_ERR_ARG, "Mesh version is unsupported!");
+      const std::string ver = trimmed(line).substr(0, 3);
+      if (ver != "2.1" && ver != "2.2") return fail(MOM_ERR_ARG, "Mesh version is unsupported!");
+      MOM_TRY(expect_end("$EndMeshFormat", "Could not find $MeshFormatNodes specifier!"));
+    } else if (tok == "$Nodes") {
+      if (!std::getline(in, line) || !parse_longs(line, v) || v.size() != 1 || v[0] < 1 || v[0] > (1L << 28))
+        return fail(MOM_ERR_ARG, "bad node count in mesh file");
+      const long nn = v[0];
+      p.assign(3 * nn, 0.0);
+      std::vector<char> seen(nn, 0);
+      for (long n = 0; n < nn; n++) {
+        if (!std::getline(in, line)) return fail(MOM_ERR_ARG, "mesh file ends inside $Nodes");
+        const char *c = line.c_str();
+        char *end;
+        const long num = std::strtol(c, &end, 10);
+        if (end == c || num < 1 || num > nn) return fail(MOM_ERR_ARG, "node number out of range in mesh file");
+        double x[3];
+        for (int k = 0; k < 3; k++) {
+          c = end;
+          x[k] = std::strtod(c, &end);
+          if (end == c) return fail(MOM_ERR_ARG, "bad node line in mesh file");
+        }
+        for (int k = 0; k < 3; k++) p[3 * (num - 1) + k] = x[k];
+        seen[num - 1] = 1;
+      }
+      if (std::count(seen.begin(), seen.end(), 0)) return fail(MOM_ERR_ARG, "node numbers are not 1..nnodes");
+      MOM_TRY(expect_end("$EndNodes", "Could not find $EndNodes specifier!"));
+      have_nodes = true;
+    } else if (tok == "$Elements") {
+      if (!std::getline(in, line) || !parse_longs(line, v) || v.size() != 1 || v[0] < 0)
+        return fail(MOM_ERR_ARG, "bad element count in mesh file");
+      const long ne = v[0];
+      fnodes.clear();
+      fids.clear();
+      for (long n = 0; n < ne; n++) {
+        if (!std::getline(in, line)) return fail(MOM_ERR_ARG, "mesh file ends inside $Elements");
+        if (!parse_longs(line, v) || v.size() < 3) return fail(MOM_ERR_ARG, "bad element line in mesh file");
+        if (v[1] != 2) continue;
+        const long nt = v[2];
+        if (nt < 1 || (long)v.size() < 6 + nt) return fail(MOM_ERR_ARG, "bad triangle line in mesh file");
+        fids.push_back((int32_t)v[3]);
+        for (int k = 0; k < 3; k++) fnodes.push_back((int32_t)(v[3 + nt + k] - 1));
+      }
+      MOM_TRY(expect_end("$EndElements", "Could not find $EndElements specifier!"));
+      have_elems = true;
+    }
+  }
+  if (!have_nodes || !have_elems || fids.empty()) return fail(MOM_ERR_ARG, "mesh file has no nodes or no triangles");
+  const long nn = (long)(p.size() / 3);
+  for (int32_t i : fnodes)
+    if (i < 0 || i >= nn) return fail(MOM_ERR_ARG, "face node index out of range");
+  return 0;
+}
+
 }  // namespace mom
 
 using namespace mom;
@@ -302,6 +398,16 @@ int mom_problem_build(int64_t nnodes, const double *p, int64_t nfaces, const int
   }
   *prob = P;
   return 0;
+}
+
+int mom_problem_load_gmsh(const char *path, double scale, int32_t ndom, const int32_t *dom_ptr,
+                          const int32_t *dom_ids, mom_problem **prob) {
+  if (!path || !prob) return fail(MOM_ERR_ARG, "NULL argument");
+  std::vector<double> p;
+  std::vector<int32_t> fnodes, fids;
+  MOM_TRY(read_gmsh(path, p, fnodes, fids));
+  return mom_problem_build((int64_t)(p.size() / 3), p.data(), (int64_t)fids.size(), fnodes.data(), fids.data(),
+                           scale, ndom, dom_ptr, dom_ids, prob);
 }
 
 int mom_problem_destroy(mom_problem *prob) {

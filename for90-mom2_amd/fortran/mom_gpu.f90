@@ -189,6 +189,18 @@ MODULE mom_gpu
        INTEGER(C_INT) :: st
      END FUNCTION mom_problem_build
 
+     ! path: TRIM(mesh_file)//C_NULL_CHAR
+     FUNCTION mom_problem_load_gmsh(path, scale, ndom, dom_ptr, dom_ids, prob) &
+          BIND(C, NAME='mom_problem_load_gmsh') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR, C_CHAR
+       CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+       REAL(C_DOUBLE), VALUE :: scale
+       INTEGER(C_INT32_T), VALUE :: ndom
+       INTEGER(C_INT32_T), INTENT(IN) :: dom_ptr(*), dom_ids(*)
+       TYPE(C_PTR) :: prob
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_load_gmsh
+
      FUNCTION mom_problem_destroy(prob) BIND(C, NAME='mom_problem_destroy') RESULT(st)
        IMPORT :: C_INT, C_PTR
        TYPE(C_PTR), VALUE :: prob

@@ -30,7 +30,6 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import capi
-from .meshgen import read_gmsh
 
 __all__ = ["DomainMesh", "Problem", "build_problem", "load_problem"]
 
@@ -100,6 +99,21 @@ def _fetch(L, h, d) -> DomainMesh:
     return m
 
 
+def _domain_lists(domains):
+    dom_ptr = np.cumsum([0] + [len(s) for s in domains]).astype(np.int32)
+    dom_ids = np.array([int(s) for sids in domains for s in sids], dtype=np.int32)
+    return dom_ptr, dom_ids
+
+
+def _collect(L, h, ndom) -> Problem:
+    try:
+        master = _fetch(L, h, -1)
+        doms = [_fetch(L, h, d) for d in range(ndom)]
+    finally:
+        L.mom_problem_destroy(h)
+    return Problem(master=master, domains=doms)
+
+
 def build_problem(nodes, faces, ids, scale: float, domains) -> Problem:
     """Build the master mesh and domain meshes with the library's set-up
     (mom_problem_build: build_mesh, extract_submesh,
@@ -115,19 +129,19 @@ def build_problem(nodes, faces, ids, scale: float, domains) -> Problem:
     ids = np.ascontiguousarray(ids, dtype=np.int32).reshape(-1)
     if ids.shape[0] != faces.shape[0]:
         raise ValueError("one physical id per face is needed")
-    dom_ptr = np.cumsum([0] + [len(s) for s in domains]).astype(np.int32)
-    dom_ids = np.array([int(s) for sids in domains for s in sids], dtype=np.int32)
+    dom_ptr, dom_ids = _domain_lists(domains)
     h = C.c_void_p()
     capi.check(L.mom_problem_build(nodes.shape[0], nodes.reshape(-1), faces.shape[0], faces.reshape(-1), ids,
                                    float(scale), len(domains), dom_ptr, dom_ids, C.byref(h)))
-    try:
-        master = _fetch(L, h, -1)
-        doms = [_fetch(L, h, d) for d in range(len(domains))]
-    finally:
-        L.mom_problem_destroy(h)
-    return Problem(master=master, domains=doms)
+    return _collect(L, h, len(domains))
 
 
 def load_problem(path: str, scale: float, domains) -> Problem:
-    nodes, faces, ids = read_gmsh(path)
-    return build_problem(nodes, faces, ids, scale, domains)
+    """`mesh` + `sdom` + `fdom` from a Gmsh 2.x file, read and built by the
+    library (mom_problem_load_gmsh; load_mesh_gmsh, mesh.f90:610-736)."""
+    L = capi.lib()
+    dom_ptr, dom_ids = _domain_lists(domains)
+    h = C.c_void_p()
+    capi.check(L.mom_problem_load_gmsh(str(path).encode(), float(scale), len(domains), dom_ptr, dom_ids,
+                                       C.byref(h)))
+    return _collect(L, h, len(domains))

@@ -219,6 +219,11 @@ typedef struct mom_problem mom_problem;
 int mom_problem_build(int64_t nnodes, const double *p, int64_t nfaces, const int32_t *fnodes,
                       const int32_t *fids, double scale, int32_t ndom, const int32_t *dom_ptr,
                       const int32_t *dom_ids, mom_problem **prob);
+/* The same from a Gmsh 2.1/2.2 ASCII file, read as load_mesh_gmsh does
+ * (mesh.f90:610-736): nodes at their node number, triangles with their first
+ * tag as the surface id, other elements skipped. */
+int mom_problem_load_gmsh(const char *path, double scale, int32_t ndom, const int32_t *dom_ptr,
+                          const int32_t *dom_ids, mom_problem **prob);
 int mom_problem_destroy(mom_problem *prob);
 /* Sizes of domain dom (dom = -1: the master mesh); nedges_total is the
  * master mesh's edge count, the number of RWG functions per current.  Any

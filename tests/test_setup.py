@@ -1,6 +1,7 @@
-"""CPU checks of the library's host set-up (mom_problem_build: the `mesh` and
-`sdom` commands, mesh.f90:1044-1137, 1265-1322, 290-558).  No GPU work: the
-set-up runs on the host inside libmomgpu.so.
+"""CPU checks of the library's host set-up (mom_problem_build and
+mom_problem_load_gmsh: the `mesh`, `sdom` and `fdom` commands, mesh.f90:610-736,
+1044-1137, 1265-1322, 290-558).  No GPU work: the set-up runs on the host
+inside libmomgpu.so.
 
 Pins: the reference's own mesh dumps (tests/golden, bit for bit, in
 test_oracle.py::test_host_mesh_builder_matches_reference), the oracle's
@@ -132,3 +133,57 @@ def test_setup_errors():
     assert st == -1 and "scale" in msg
     with pytest.raises(capi.MomError):
         build_problem(nodes, faces, ids, 1e-9, [[15]])
+
+
+@pytest.mark.parametrize("kind,s", [("sphere", 2), ("dimer", 1)])
+def test_gmsh_loader_matches_file_arrays(msh_dir, kind, s):
+    """mom_problem_load_gmsh reads the file as load_mesh_gmsh does and gives
+    the same problem as mom_problem_build on the file's arrays."""
+    from mom2amd import load_problem
+
+    case = sphere(msh_dir, s) if kind == "sphere" else dimer(msh_dir, s)
+    P = load_problem(case["path"], 1e-9, case["domains"])
+    Q = case["problem"]
+    for a, b in zip([P.master] + P.domains, [Q.master] + Q.domains):
+        for k in KEYS + ("eparent", "fid", "fparent", "node_parent"):
+            assert np.array_equal(getattr(a, k), getattr(b, k)), k
+        assert a.avelen == b.avelen
+
+
+def test_gmsh_loader_node_order_and_other_elements(msh_dir):
+    """Nodes are stored at their node number, whatever the order of the
+    lines; points, lines and tetrahedra are skipped; the first tag is the
+    surface id."""
+    from mom2amd import load_problem
+
+    nodes, faces = meshgen.icosphere(1, 50.0)
+    ids = np.full(len(faces), 15)
+    perm = np.random.default_rng(3).permutation(len(nodes))
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(len(nodes))]
+    lines += ["%d %.17g %.17g %.17g" % (k + 1, *nodes[k]) for k in perm]
+    lines += ["$EndNodes", "$Elements", str(len(faces) + 3)]
+    lines += ["1 15 2 99 1 1", "2 1 2 98 3 1 2"]
+    lines += ["%d 2 2 15 7 %d %d %d" % (k + 3, *(faces[k] + 1)) for k in range(len(faces))]
+    lines += ["%d 4 2 97 1 1 2 3 4" % (len(faces) + 3), "$EndElements", ""]
+    path = msh_dir / "perm.msh"
+    path.write_text("\n".join(lines))
+    P = load_problem(str(path), 1e-9, [[15], [-15]])
+    Q = build_problem(nodes, faces, ids, 1e-9, [[15], [-15]])
+    for a, b in zip([P.master] + P.domains, [Q.master] + Q.domains):
+        for k in KEYS:
+            assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+def test_gmsh_loader_errors(msh_dir):
+    from mom2amd import load_problem
+
+    with pytest.raises(capi.MomError, match="Could not open mesh file"):
+        load_problem(str(msh_dir / "missing.msh"), 1e-9, [[1], [-1]])
+    bad = msh_dir / "v4.msh"
+    bad.write_text("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n")
+    with pytest.raises(capi.MomError, match="Mesh version is unsupported"):
+        load_problem(str(bad), 1e-9, [[1], [-1]])
+    bad = msh_dir / "noend.msh"
+    bad.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n1\n1 0 0 0\n$Elements\n")
+    with pytest.raises(capi.MomError, match="EndNodes"):
+        load_problem(str(bad), 1e-9, [[1], [-1]])
