@@ -858,7 +858,8 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   a.qp = ctx->qp.as<double>();
   a.cp = m->fcp.as<double>();
   a.area = m->farea.as<double>();
-  a.side = sides(m, t.mode == 0);
+  const bool glob = t.mode == 0 && !t.local;  // global unknown numbering
+  a.side = sides(m, glob);
   a.kr = k.real();
   a.ki = k.imag();
   if (!(std::fabs(a.kr) * m->extent < kCisMaxArg))
@@ -874,11 +875,11 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   const size_t bbytes = sizeof(double2) * 18 * (size_t)S * m->F;
   if (ctx->scratch.bytes < bbytes && (rc = ctx->scratch.alloc(bbytes))) return rc;
   // output row -> local edge
-  const int64_t nrows = t.mode == 0 ? t.Noff : m->E;
+  const int64_t nrows = glob ? t.Noff : m->E;
   std::vector<int> rl(nrows, -1);
   std::vector<int> colidx(m->E);
   for (int64_t e = 0; e < m->E; e++) {
-    const int g = t.mode == 0 ? m->h_eparent[e] : (int)e;
+    const int g = glob ? m->h_eparent[e] : (int)e;
     rl[g] = (int)e;
     colidx[e] = g;
   }

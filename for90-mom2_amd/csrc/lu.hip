@@ -2530,6 +2530,22 @@ int mom_system_fill(mom_system *S, int32_t ndom, const mom_mesh *const *doms, co
   return 0;
 }
 
+int mom_system_fill_nls(mom_system *S, int32_t ndom, const mom_mesh *const *doms, const double *shri, double omega,
+                        int32_t nq, const double *qw, const double *qnodes, int64_t nsrc, const double *src_coef,
+                        int64_t ldsc, double *src, int64_t ldsrc) {
+  if (!S || !doms || !shri) return fail(MOM_ERR_ARG, "NULL argument");
+  (void)hipSetDevice(S->ctx->device);
+  const double t0 = now_s();
+  int rc = fill_system_nls(S->ctx, ndom, doms, shri, S->N, omega, nq, qw, qnodes, S->Z64.as<double2>(), S->n, nsrc,
+                           src_coef, ldsc, src, ldsrc);
+  if (rc) return rc;
+  MOM_HIP(hipStreamSynchronize(S->ctx->stream));
+  S->t_fill = now_s() - t0;
+  S->filled = true;
+  S->factored = false;
+  return 0;
+}
+
 int mom_system_get_matrix(mom_system *S, double *A, int64_t lda) {
   if (!S || !A) return fail(MOM_ERR_ARG, "NULL argument");
   if (lda < S->n) return fail(MOM_ERR_ARG, "lda < 2N");

@@ -111,6 +111,8 @@ struct mom_mesh {
   mom::DevBuf eside, slab_edges;
   std::vector<int> h_slab_ptr;
   std::vector<int> h_eparent;
+  // host connectivity (face edges, edge faces, edge face-side codes)
+  std::vector<int32_t> h_fedges, h_efaces, h_eside;
 };
 
 namespace mom {
@@ -128,6 +130,7 @@ struct FillTarget {
   int64_t ld = 0;
   int64_t Noff = 0;   // offset of the M rows/columns (nedges_total)
   int mode = 0;       // 0: PMCHWT blocks with global indices; 1/2/3: D/H/K, local indices
+  int local = 0;      // mode 0 with local indices (the domain's own 2E x 2E block, Noff = E)
   double2 zsi;        // Zsi = ri^2 / eta0^2 (mode 0)
 };
 
@@ -146,5 +149,12 @@ int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
 // x the device solution [2 Nt], k the medium's wavenumber
 int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, cd k, const double2 *x,
                         int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out);
+// sysmat_pmchwt_nls (sysmat.f90:112-229) into the device matrix Z, with the
+// products of the nonlinear domains' blocks and src_coef subtracted from the
+// host sources src (nls.hip)
+int fill_system_nls(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *shri,
+                    int64_t nedges_total, double omega, int32_t nq, const double *qw, const double *qnodes,
+                    double2 *Z, int64_t ldz, int64_t nsrc, const double *src_coef, int64_t ldsc, double *src,
+                    int64_t ldsrc);
 
 }  // namespace mom

@@ -7,3 +7,4 @@ from .constants import C0, DEGTORAD, EPS0, ETA0, MU0, PI, omega_of, tri_quad_dat
 from .mesh import DomainMesh, Problem, build_problem, load_problem  # noqa: F401
 from .gpu import (Context, DeviceMesh, System, computeD, computeH, computeK,  # noqa: F401
                   cs_prtsrf, far_fields, rcs, rcs_ext, solve_multi_linsys, srcvec, sysmat_pmchwt)
+from .nlsurf import determine_epsp, nlsurf_coef, solve_sh, sysmat_pmchwt_nls  # noqa: F401

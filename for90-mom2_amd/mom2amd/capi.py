@@ -24,7 +24,7 @@ EXPORTS = [
     "mom_solve_multi_linsys", "mom_system_create", "mom_system_destroy", "mom_system_fill",
     "mom_system_get_matrix", "mom_system_set_matrix", "mom_system_factor", "mom_system_solve",
     "mom_system_solve_pw", "mom_system_profile", "mom_system_stats", "mom_cs_prtsrf", "mom_far_fields",
-    "mom_rcs_ext", "mom_problem_build", "mom_problem_load_gmsh", "mom_problem_destroy", "mom_problem_sizes", "mom_problem_domain",
+    "mom_rcs_ext", "mom_nlsurf_coef", "mom_sysmat_pmchwt_nls", "mom_system_fill_nls", "mom_problem_build", "mom_problem_load_gmsh", "mom_problem_destroy", "mom_problem_sizes", "mom_problem_domain",
     "mom_problem_topology", "mom_problem_mesh_create",
 ]
 # test hooks declared in include/mom_gpu_testing.h
@@ -88,6 +88,11 @@ def lib():
                                 C.POINTER(_d)]
     L.mom_far_fields.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _i64, _dp, _dp, _i32, _dp, _dp, _dp]
     L.mom_rcs_ext.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _i32, _i32, _i32, C.c_void_p, _i32, _dp, _dp, _dp]
+    L.mom_nlsurf_coef.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _dp, _dp, _i32, _dp, _dp, _dp, _dp]
+    L.mom_sysmat_pmchwt_nls.argtypes = [_vp, _i32, C.POINTER(_vp), _dp, _i64, _d, _i32, _dp, _dp, _i64, _dp, _i64,
+                                        _dp, _i64, _dp, _i64]
+    L.mom_system_fill_nls.argtypes = [_vp, _i32, C.POINTER(_vp), _dp, _d, _i32, _dp, _dp, _i64, _dp, _i64, _dp,
+                                      _i64]
     L.mom_problem_build.argtypes = [_i64, _dp, _i64, _ip, _ip, _d, _i32, _ip, _ip, C.POINTER(_vp)]
     L.mom_problem_load_gmsh.argtypes = [C.c_char_p, _d, _i32, _ip, _ip, C.POINTER(_vp)]
     L.mom_problem_destroy.argtypes = [_vp]

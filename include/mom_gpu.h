@@ -203,6 +203,49 @@ int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double
                 int32_t nq, const double *qw, const double *qnodes, double *rcs);
 
 /*
+ * Second harmonic from a surface chi2 (centrosymmetric media), identity
+ * group, free space: the centrosymmetric branch of solve_batch
+ * (solver.f90:143-231) is
+ *   per nonlinear domain d: epsp (determine_epsp, host), then
+ *     mom_nlsurf_coef -> src_coef_d, src_vec_d; src_vec_d is added to the SH
+ *     source at the domain's global unknowns (rows parent, parent + N);
+ *   mom_sysmat_pmchwt_nls (or mom_system_fill_nls) -> SH matrix, source
+ *     updated; then a solve, and mom_rcs_ext at 2 omega in domain 1.
+ *
+ * nlsurf_coef (nlsurf.f90:137-338) of one domain mesh: omega and ri are the
+ * fundamental angular frequency and the domain's fundamental index,
+ * epsp[2*nfaces] the permittivity outside each face (complex), x[4N] the
+ * fundamental solution (complex, J then M), chi2[6] = chi2_nnn, chi2_ntt,
+ * chi2_ttn (complex).  Outputs (complex, local edge numbering):
+ * src_coef[4*nedges] = (M, J) expansion coefficients of the surface
+ * polarisation source, src_vec[4*nedges] = its (E, H) source terms.  The RWG
+ * Gram systems are solved by conjugate gradients on the sparse matrix
+ * (MOM_ERR_NOCONV if the relative residual stays above 1e-12).
+ */
+int mom_nlsurf_coef(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                    double ri_im, const double *epsp, const double *x, const double *chi2, int32_t nq,
+                    const double *qw, const double *qnodes, double *src_coef, double *src_vec);
+/*
+ * sysmat_pmchwt_nls (sysmat.f90:112-229): A[2N x 2N] (lda >= 2N) becomes the
+ * PMCHWT matrix at the SH angular frequency omega (twice the fundamental)
+ * with the SH indices shri[2*ndom], and for each source s the products of
+ * every domain's block with its src_coef are subtracted from src[:, s]
+ * (complex, ldsrc >= 2N).  src_coef is complex [ldsc x ndom x nsrc] as the
+ * reference's src_coef(2N, ndom, 1, nsrc): column (d, s) starts at element
+ * ldsc (d + ndom s) and holds domain d's (M, J) coefficients in its first
+ * 2 nedges_d entries (zero for linear domains).
+ */
+int mom_sysmat_pmchwt_nls(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *shri,
+                          int64_t nedges_total, double omega, int32_t nq, const double *qw, const double *qnodes,
+                          int64_t nsrc, const double *src_coef, int64_t ldsc, double *src, int64_t ldsrc,
+                          double *A, int64_t lda);
+/* The same into the device-resident system (then mom_system_factor and
+ * mom_system_solve on src). */
+int mom_system_fill_nls(mom_system *sys, int32_t ndom, const mom_mesh *const *doms, const double *shri,
+                        double omega, int32_t nq, const double *qw, const double *qnodes, int64_t nsrc,
+                        const double *src_coef, int64_t ldsc, double *src, int64_t ldsrc);
+
+/*
  * Problem set-up on the host (no GPU work): the master mesh and the domain
  * meshes that the `mesh` and `sdom` commands build (interface.f90:357-366,
  * 44-105), identity group, no periodic boundaries.  Replaces build_mesh

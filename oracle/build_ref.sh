@@ -15,10 +15,13 @@
 #   1. LOGICAL compared with ==, /= against .TRUE./.FALSE. -> .EQV./.NEQV.
 #   2. rank-1 array constructors initialising rank-2 PARAMETERs get RESHAPE
 #      (symmetry.f90:16-18 id33r, mesh.f90:976 cmp_triplets indices).
-# No LAPACK is linked: the driver never calls the reference's solve routines
-# (the LU is LAPACK, a third-party library absent from the image), so their
-# six LAPACK references (zgetrf zgetrs zgetri zgeev zggev zgemm) are left
-# unresolved at link time; nothing on the driver's path reaches them.
+# LAPACK (a third-party dependency of the reference, MKL in its own build)
+# is taken from the one LAPACK the image has: the OpenBLAS that scipy ships
+# (scipy.libs/libscipy_openblas-*.so), the library the oracle's scipy solves
+# also use.  Its entry points carry a "scipy_" prefix, so the reference's
+# six LAPACK/BLAS names (zgetrf zgetrs zgetri zgeev zggev zgemm) are bound to
+# them by renaming the references in the objects compiled here
+# (objcopy --redefine-sym); no code stands in for them.
 set -euo pipefail
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -29,7 +32,7 @@ mkdir -p "$OUT"
 WORK=$(mktemp -d /tmp/momref.XXXXXX)
 trap 'rm -rf "$WORK"' EXIT
 
-MODS="constants linalg aux mesh quad rwgf green greenprd symmetry int srcint nfields dipole bessel source bc nlsurf nlbulk common sysmat ffields cs"
+MODS="constants linalg aux mesh quad rwgf green greenprd symmetry int srcint nfields dipole bessel source bc nlsurf nlbulk common sysmat ffields cs time solver"
 # Fix 1 wraps the primary left of the operator in parentheses, so that the
 # operator keeps the precedence of == (a relational operator binds tighter
 # than .AND., .EQV. does not): `A .AND. P(x)==.FALSE.` becomes
@@ -77,5 +80,15 @@ for m in $MODS; do
   OBJS="$OBJS $m.o"
 done
 $FC $FFLAGS -c "$HERE/ref_driver.f90" -o ref_driver.o
-$FC -fopenmp -o "$OUT/mom_ref" ref_driver.o $OBJS -Wl,--unresolved-symbols=ignore-in-object-files -Wl,-z,execstack
+BLAS_SO=$(python3 -c "import glob, os, scipy; print(sorted(glob.glob(os.path.join(os.path.dirname(scipy.__file__) + '.libs', 'libscipy_openblas-*.so')))[0])")
+BLAS_DIR=$(dirname "$BLAS_SO")
+RENAME=""
+for f in zgetrf zgetrs zgetri zgeev zggev zgemm; do
+  RENAME="$RENAME --redefine-sym ${f}_=scipy_${f}_"
+done
+for o in $OBJS; do
+  objcopy $RENAME "$o"
+done
+$FC -fopenmp -o "$OUT/mom_ref" ref_driver.o $OBJS -L"$BLAS_DIR" -l:"$(basename "$BLAS_SO")" \
+  -Wl,-rpath,"$BLAS_DIR" -Wl,-z,execstack
 echo "built $OUT/mom_ref"

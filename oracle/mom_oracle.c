@@ -14,8 +14,10 @@
  *   - constants, quadrature tables: bit values of the float32-rounded
  *     literals (tests/test_oracle_tables.py);
  *   - mesh numbering/orientation, geometry, computeD/H/K, sysmat_pmchwt,
- *     srcvec, cs_prtsrf, far fields: the reference Fortran built by
- *     oracle/build_ref.sh (golden vectors under tests/golden/);
+ *     srcvec, cs_prtsrf, far fields, and the second-harmonic chain
+ *     (nlsurf_coef, sysmat_pmchwt_nls, the SH solution and its rcs): the
+ *     reference Fortran built by oracle/build_ref.sh (golden vectors under
+ *     tests/golden/);
  *   - the int.f90 closed forms: brute-force adaptive quadrature of the same
  *     integrals (tests/test_oracle_singular.py);
  *   - the whole chain: Mie series for spheres (tests/test_oracle_mie.py) and
@@ -1278,6 +1280,210 @@ double orc_fill_rows_seconds(void *h, double omega, const double *ri, int Q, con
     free(M);
   }
   return omp_get_wtime() - t0;
+}
+
+/* ---- second harmonic, surface chi2 (nlsurf.f90), identity group --------- */
+
+/* nlsurf.f90:21-49 efield: surface E of the fundamental solution x at r on
+ * face f, from the normal component div J/(i omega eps) and n x M */
+static void nls_efield(const omesh *m, int Nt, double omega, cplx ri, const cplx *x, int f, const double *r,
+                       cplx *e) {
+  cplx eps = (ri * ri) * orc_eps0;
+  cplx divj = 0, ms[3] = {0, 0, 0};
+  for (int n = 0; n < 3; n++) {
+    int ed = m->eparent[m->fedges[3 * f + n]];
+    double fv[3];
+    divj = divj + x[ed] * rwgDiv(m, f, n);
+    vrwg(r, 1, m, f, n, fv);
+    for (int k = 0; k < 3; k++) ms[k] = ms[k] + x[Nt + ed] * fv[k];
+  }
+  cplx en = divj / ((I * omega) * eps);
+  const double *nor = m->fn + 3 * f;
+  cplx cn[3] = {nor[0], nor[1], nor[2]}, t[3];
+  crossc(cn, ms, t);
+  for (int k = 0; k < 3; k++) e[k] = t[k] + nor[k] * en;
+}
+
+/* nlsurf.f90:74-103 Pnls_invmap and 106-126 Pnls_frag with the identity
+ * action (J = I, field factor 1): P = 2 eps0 chi_ttn e_t e_n
+ * + eps0 (chi_nnn e_n^2 + chi_ntt e_t.e_t) n, dotc without conjugation */
+static void nls_pnls(const omesh *m, int Nt, double omega, cplx ri, const cplx *x, int f, const cplx *chi,
+                     const double *r, cplx *P) {
+  cplx e[3], et[3];
+  nls_efield(m, Nt, omega, ri, x, f, r, e);
+  const double *nor = m->fn + 3 * f;
+  cplx en = nor[0] * e[0] + nor[1] * e[1] + nor[2] * e[2];
+  for (int k = 0; k < 3; k++) et[k] = e[k] - en * nor[k];
+  cplx Pn = orc_eps0 * (chi[0] * (en * en) + chi[1] * (et[0] * et[0] + et[1] * et[1] + et[2] * et[2]));
+  for (int k = 0; k < 3; k++) P[k] = 2.0 * orc_eps0 * chi[2] * et[k] * en + Pn * nor[k];
+}
+
+/* nlsurf_coef (nlsurf.f90:137-217): the projections before the first Gram
+ * solve.  Outputs (local edge numbering, complex interleaved):
+ *   coef[E]     sum_faces A <div f_m, -P_n>/epsp        (rhs 1 of the solve)
+ *   srcJ[E]     -i 2omega sum_faces A <f_m, P_t>         (rhs 2)
+ *   srcvec[2E]  (0.5 coef, 0.5 i 2omega sum_faces A <f_m, P_t x n>) */
+void orc_nls_stage1(void *h, int dom, double omega, double ri_re, double ri_im, const double *epsp_in,
+                    const double *xin, const double *chi_in, int Q, const double *w, const double *nodes,
+                    double *coef_out, double *srcJ_out, double *srcvec_out) {
+  oproblem *P = h;
+  omesh *m = P->dom + dom;
+  oquad qd;
+  mkquad(&qd, Q, w, nodes);
+  int Nt = P->master.nedges, E = m->nedges;
+  const cplx *x = (const cplx *)xin, *epsp = (const cplx *)epsp_in, *chi = (const cplx *)chi_in;
+  cplx ri = ri_re + I * ri_im;
+  double omegash = 2.0 * omega;
+  cplx *coef = (cplx *)coef_out, *srcJ = (cplx *)srcJ_out, *sv = (cplx *)srcvec_out;
+  for (int i = 0; i < E; i++) coef[i] = srcJ[i] = sv[i] = sv[i + E] = 0;
+  for (int f = 0; f < m->nfaces; f++) {
+    double qp[96];
+    cplx Pn[32], Pt[32][3];
+    quad_tri_points(&qd, m, f, qp);
+    double A = m->farea[f];
+    const double *nor = m->fn + 3 * f;
+    cplx cn[3] = {nor[0], nor[1], nor[2]};
+    for (int r = 0; r < Q; r++) {
+      cplx Pv[3];
+      nls_pnls(m, Nt, omega, ri, x, f, chi, qp + 3 * r, Pv);
+      Pn[r] = nor[0] * Pv[0] + nor[1] * Pv[1] + nor[2] * Pv[2];
+      for (int k = 0; k < 3; k++) Pt[r][k] = Pv[k] - Pn[r] * nor[k];
+    }
+    for (int q = 0; q < 3; q++) {
+      double fmDiv = rwgDiv(m, f, q);
+      cplx int1 = 0, int2 = 0, int3 = 0;
+      for (int r = 0; r < Q; r++) {
+        double fm[3];
+        cplx c[3];
+        vrwg(qp + 3 * r, 1, m, f, q, fm);
+        crossc(Pt[r], cn, c);
+        int1 = int1 - qd.w[r] * fmDiv * Pn[r];
+        int2 = int2 + qd.w[r] * (fm[0] * c[0] + fm[1] * c[1] + fm[2] * c[2]);
+        int3 = int3 + qd.w[r] * (fm[0] * Pt[r][0] + fm[1] * Pt[r][1] + fm[2] * Pt[r][2]);
+      }
+      int idx = m->fedges[3 * f + q];
+      coef[idx] = coef[idx] + int1 * A / epsp[f];
+      srcJ[idx] = srcJ[idx] + int3 * A;
+      sv[idx + E] = sv[idx + E] + int2 * A;
+    }
+  }
+  for (int i = 0; i < E; i++) {
+    srcJ[i] = -srcJ[i] * I * omegash;
+    sv[i] = 0.5 * coef[i];
+    sv[i + E] = 0.5 * I * omegash * sv[i + E];
+  }
+}
+
+/* rwgf.f90:148-221 rwg_moments: Gram matrix F[m,n] = <f_m, f_n> (E x E,
+ * column-major, real), the off-diagonal entry on the first common face */
+void orc_rwg_moments(void *h, int dom, int Q, const double *w, const double *nodes, double *F) {
+  oproblem *P = h;
+  omesh *m = P->dom + dom;
+  oquad qd;
+  mkquad(&qd, Q, w, nodes);
+  int E = m->nedges;
+  memset(F, 0, sizeof(double) * (size_t)E * E);
+  for (int a = 0; a < E; a++) {
+    for (int s = 0; s < 2; s++) {
+      int f = m->efaces[2 * a + s];
+      if (f == -1) continue;
+      double qp[96], fm[3], in = 0;
+      quad_tri_points(&qd, m, f, qp);
+      int q = m->fedges[3 * f] == a ? 0 : (m->fedges[3 * f + 1] == a ? 1 : 2);
+      for (int r = 0; r < Q; r++) {
+        vrwg(qp + 3 * r, 1, m, f, q, fm);
+        in = in + qd.w[r] * dotr(fm, fm);
+      }
+      in = in * m->farea[f];
+      F[a + (size_t)a * E] = F[a + (size_t)a * E] + in;
+    }
+    for (int b = a + 1; b < E; b++) {
+      int fa0 = m->efaces[2 * a], fa1 = m->efaces[2 * a + 1], fb0 = m->efaces[2 * b], fb1 = m->efaces[2 * b + 1];
+      int f;
+      if (fa0 != -1 && (fa0 == fb0 || fa0 == fb1))
+        f = fa0;
+      else if (fa1 != -1 && (fa1 == fb0 || fa1 == fb1))
+        f = fa1;
+      else
+        continue;
+      int q = m->fedges[3 * f] == a ? 0 : (m->fedges[3 * f + 1] == a ? 1 : 2);
+      int p = m->fedges[3 * f] == b ? 0 : (m->fedges[3 * f + 1] == b ? 1 : 2);
+      double qp[96], fm[3], fn[3], in = 0;
+      quad_tri_points(&qd, m, f, qp);
+      for (int r = 0; r < Q; r++) {
+        vrwg(qp + 3 * r, 1, m, f, q, fm);
+        vrwg(qp + 3 * r, 1, m, f, p, fn);
+        in = in + qd.w[r] * dotr(fm, fn);
+      }
+      in = in * m->farea[f];
+      F[a + (size_t)b * E] = in;
+      F[b + (size_t)a * E] = in;
+    }
+  }
+}
+
+/* nlsurf_coef (nlsurf.f90:278-299): srcM[m] = sum_faces A <f_m, n x g>,
+ * g = sum_q coef[q] f_q (rwg_exp, rwgf.f90:278-298), coef from the first
+ * Gram solve; srcM is the rhs of the second solve */
+void orc_nls_stage2(void *h, int dom, int Q, const double *w, const double *nodes, const double *coef_in,
+                    double *srcM_out) {
+  oproblem *P = h;
+  omesh *m = P->dom + dom;
+  oquad qd;
+  mkquad(&qd, Q, w, nodes);
+  int E = m->nedges;
+  const cplx *coef = (const cplx *)coef_in;
+  cplx *srcM = (cplx *)srcM_out;
+  for (int i = 0; i < E; i++) srcM[i] = 0;
+  for (int f = 0; f < m->nfaces; f++) {
+    double qp[96];
+    quad_tri_points(&qd, m, f, qp);
+    double A = m->farea[f];
+    const double *nor = m->fn + 3 * f;
+    cplx cn[3] = {nor[0], nor[1], nor[2]};
+    for (int q = 0; q < 3; q++) {
+      cplx int1 = 0;
+      for (int r = 0; r < Q; r++) {
+        double fm[3];
+        cplx g[3] = {0, 0, 0}, c[3];
+        for (int t = 0; t < 3; t++) {
+          double ft[3];
+          vrwg(qp + 3 * r, 1, m, f, t, ft);
+          for (int k = 0; k < 3; k++) g[k] = g[k] + ft[k] * coef[m->fedges[3 * f + t]];
+        }
+        vrwg(qp + 3 * r, 1, m, f, q, fm);
+        crossc(cn, g, c);
+        int1 = int1 + qd.w[r] * (fm[0] * c[0] + fm[1] * c[1] + fm[2] * c[2]);
+      }
+      int idx = m->fedges[3 * f + q];
+      srcM[idx] = srcM[idx] + int1 * A;
+    }
+  }
+}
+
+/* solver.f90:393-433 determine_epsp: per face of domain dom, eps0 ri^2 of
+ * the first other domain holding the same master face (ri: 2*ndom doubles,
+ * the fundamental indices).  Returns -1 if a face has no such domain. */
+int orc_epsp(void *h, int dom, const double *ri, double *out) {
+  oproblem *P = h;
+  omesh *m = P->dom + dom;
+  cplx *e = (cplx *)out;
+  for (int f = 0; f < m->nfaces; f++) {
+    int found = 0;
+    for (int d = 0; d < P->ndom && !found; d++) {
+      if (d == dom) continue;
+      omesh *o = P->dom + d;
+      for (int l = 0; l < o->nfaces; l++)
+        if (o->fparent[l] == m->fparent[f]) {
+          cplx r = ri[2 * d] + I * ri[2 * d + 1];
+          e[f] = (r * r) * orc_eps0;
+          found = 1;
+          break;
+        }
+    }
+    if (!found) return -1;
+  }
+  return 0;
 }
 
 void orc_problem_free(void *h) {

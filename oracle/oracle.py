@@ -47,8 +47,16 @@ def lib():
         L.orc_int_scalar.restype = C.c_double
         L.orc_int_scalar.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp]
         L.orc_int_vector.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.orc_nls_stage1.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp,
+                                     C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_rwg_moments.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_epsp.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.orc_nls_stage2.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         _LIB = L
     return _LIB
+
+
+ETA0 = 3.767303134622e2
 
 
 def quad_gl13():
@@ -140,6 +148,89 @@ class OracleProblem:
         Q, w, n = self._q(quad)
         lib().orc_rcs(self.h, dom, omega, ri.real, ri.imag, x, ntheta, nphi, Q, w, n, out)
         return out.reshape(ntheta, nphi, order="F")
+
+    # ---- second harmonic (surface chi2), identity group --------------------
+    def epsp(self, dom, ri_list):
+        """determine_epsp (solver.f90:393-433) for domain dom."""
+        F = self.sizes(dom)[1]
+        out = np.zeros(2 * F)
+        ri = np.array([[complex(r).real, complex(r).imag] for r in ri_list]).ravel()
+        if lib().orc_epsp(self.h, dom, ri, out) != 0:
+            raise RuntimeError("Could not find epsp!")
+        return out.view(np.complex128)
+
+    def rwg_moments(self, dom, quad=None):
+        """rwg_moments (rwgf.f90:148-221): E x E Gram matrix <f_m, f_n>."""
+        E = self.sizes(dom)[2]
+        F = np.zeros(E * E)
+        Q, w, n = self._q(quad)
+        lib().orc_rwg_moments(self.h, dom, Q, w, n, F)
+        return F.reshape(E, E, order="F")
+
+    def nlsurf_coef(self, dom, omega, riff, epsp, x, chi2, quad=None):
+        """nlsurf_coef (nlsurf.f90:137-338), identity group: returns
+        (src_coef[2E], src_vec[2E]).  The two Gram solves use LAPACK
+        (scipy), as the reference's solve_multi_linsys/solve_linsys do."""
+        import scipy.linalg as sl
+
+        E = self.sizes(dom)[2]
+        Q, w, n = self._q(quad)
+        coef, srcJ, sv = np.zeros(2 * E), np.zeros(2 * E), np.zeros(4 * E)
+        riff = complex(riff)
+        lib().orc_nls_stage1(self.h, dom, omega, riff.real, riff.imag,
+                             np.ascontiguousarray(epsp, np.complex128).view(np.float64),
+                             np.ascontiguousarray(x, np.complex128).view(np.float64),
+                             np.ascontiguousarray(chi2, np.complex128).view(np.float64), Q, w, n, coef, srcJ, sv)
+        lu = sl.lu_factor(self.rwg_moments(dom, quad).astype(np.complex128))
+        sol = sl.lu_solve(lu, np.stack([coef.view(np.complex128), srcJ.view(np.complex128)], axis=1))
+        srcM = np.zeros(2 * E)
+        lib().orc_nls_stage2(self.h, dom, Q, w, n, np.ascontiguousarray(sol[:, 0]).view(np.float64), srcM)
+        srcM = sl.lu_solve(lu, srcM.view(np.complex128))
+        return np.concatenate([srcM, sol[:, 1]]), sv.view(np.complex128).copy()
+
+    def sh_chain(self, omega, ri_list, shri_list, nls, chi2, x, quad=None):
+        """solve_batch's centrosymmetric SH branch (solver.f90:143-231):
+        nlsurf_coef per nonlinear domain, the SH matrix of sysmat_pmchwt_nls
+        (sysmat.f90:112-229) at 2 omega with the SH indices and its products
+        with src_coef subtracted from the source, then the SH solution."""
+        import scipy.linalg as sl
+
+        Nt = self.nedges
+        rhs = np.zeros(2 * Nt, np.complex128)
+        out = {}
+        for d in range(self.ndom):
+            if not nls[d]:
+                continue
+            ep = self.epsp(d, ri_list)
+            sc, sv = self.nlsurf_coef(d, omega, ri_list[d], ep, x, chi2, quad)
+            ind = self.mesh(d)["eparent"]
+            E = ind.size
+            rhs[ind] += sv[:E]
+            rhs[ind + Nt] += sv[E:]
+            out["epsp_%d" % d], out["src_coef_%d" % d], out["src_vec_%d" % d] = ep, sc, sv
+        out["rhs0"] = rhs.copy()
+        om2 = 2.0 * omega
+        A = self.sysmat(om2, shri_list, quad)
+        for d in range(self.ndom):
+            if not nls[d]:
+                continue
+            sc = out["src_coef_%d" % d]
+            ind = self.mesh(d)["eparent"]
+            E = ind.size
+            scM, scJ = sc[:E], sc[E:]
+            zsi = complex(shri_list[d]) ** 2 / ETA0 ** 2
+            D = self.operator(d, "D", om2, complex(shri_list[d]), quad)
+            H = self.operator(d, "H", om2, complex(shri_list[d]), quad)
+            K = self.operator(d, "K", om2, complex(shri_list[d]), quad)
+            for M in (D, H):
+                rhs[ind] -= M @ scJ
+                rhs[ind + Nt] -= (M @ scM) * zsi
+            rhs[ind] += K @ scM
+            rhs[ind + Nt] -= K @ scJ
+        out["rhs"] = rhs.copy()
+        out["A"] = A
+        out["x"] = sl.solve(A, rhs)
+        return out
 
     def fill_rows_seconds(self, omega, ri_list, m0, m1, quad=None):
         """Wall time of the oracle fill (3 operators, every domain) for test

@@ -12,16 +12,25 @@
 !   post.bin       csca, cabs (cs_prtsrf) and rcs(ntheta,nphi) for the
 !                  solution read from x.bin (if present)
 !   times.txt      omp_get_wtime seconds of sysmat_pmchwt and srcvec
+!   sh.bin         (optional SH block, needs x.bin) the second-harmonic chain
+!                  of solve_batch's centrosymmetric branch (solver.f90:
+!                  143-231): per nonlinear domain src_coef and src_vec of
+!                  nlsurf_coef, the accumulated source, the source after
+!                  sysmat_pmchwt_nls, the SH solution of solve_systems and
+!                  its rcs at 2 omega (interface.f90:1316-1325)
 !
 ! Input on stdin, one item per line:
 !   mesh file / scale / ndom / per domain: "nsurf id1 id2 ..." /
 !   wavelength [m] / per domain: "re im" refractive index /
 !   "theta phi psi" [deg] / "dump_ops dump_sysmat" (0/1) /
-!   "ntheta nphi" / output directory
+!   "ntheta nphi" / output directory /
+!   optional: "1" / per domain "re im" SH refractive index / per domain
+!   "0|1" surface chi2 / "nnn_re nnn_im ntt_re ntt_im ttn_re ttn_im"
 PROGRAM ref_driver
   USE sysmat
   USE cs
   USE ffields
+  USE solver
   USE omp_lib
   IMPLICIT NONE
 
@@ -37,6 +46,12 @@ PROGRAM ref_driver
   INTEGER, DIMENSION(:), POINTER :: vol_ids
   TYPE(prdnfo), POINTER :: prd
   LOGICAL :: have_x
+  INTEGER :: do_sh, nlsflag, nind
+  REAL(KIND=dp) :: chi(6)
+  INTEGER, DIMENSION(:), ALLOCATABLE :: ind
+  COMPLEX(KIND=dp), DIMENSION(:,:,:,:), ALLOCATABLE :: src_coef
+  COMPLEX(KIND=dp), DIMENSION(:,:,:), ALLOCATABLE :: nlx, Ash
+  COMPLEX(KIND=dp), DIMENSION(:), ALLOCATABLE :: src_vec, epsp, rhs0
   CHARACTER(LEN=1), DIMENSION(3) :: opn = (/'D','H','K'/)
 
   READ(*,'(A)') mshfile
@@ -153,6 +168,53 @@ PROGRAM ref_driver
           RESHAPE(x, (/2*N,1/)), nth, nph, b%qd_tri, rcsdata)
      OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/post.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
      WRITE(u) csca, cabs, rcsdata
+     CLOSE(u)
+  END IF
+
+  do_sh = 0
+  READ(*,*,IOSTAT=ios) do_sh
+  IF(ios==0 .AND. do_sh==1 .AND. have_x) THEN
+     DO d=1,ndom
+        READ(*,*) re, im
+        b%media(d)%prop(1)%shri = CMPLX(re, im, KIND=dp)
+     END DO
+     DO d=1,ndom
+        READ(*,*) nlsflag
+        IF(nlsflag==1) b%media(d)%type = mtype_nls
+     END DO
+     READ(*,*) chi
+     DO d=1,ndom
+        b%media(d)%prop(1)%nls%chi2_nnn = CMPLX(chi(1), chi(2), KIND=dp)
+        b%media(d)%prop(1)%nls%chi2_ntt = CMPLX(chi(3), chi(4), KIND=dp)
+        b%media(d)%prop(1)%nls%chi2_ttn = CMPLX(chi(5), chi(6), KIND=dp)
+     END DO
+     ! solve_batch, centrosymmetric branch (solver.f90:143-231), identity group
+     ALLOCATE(nlx(2*N,1,1), src_coef(2*N,ndom,1,1), src_vec(2*N), epsp(b%mesh%nfaces), ind(N), rhs0(2*N))
+     nlx(:,:,:) = 0.0_dp
+     src_coef(:,:,:,:) = 0.0_dp
+     src_vec(:) = 0.0_dp
+     OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/sh.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+     DO d=1,ndom
+        IF(b%media(d)%type/=mtype_nls) CYCLE
+        CALL determine_epsp(b, d, 1, epsp(1:b%domains(d)%mesh%nfaces))
+        nind = b%domains(d)%mesh%nedges
+        ind(1:nind) = b%domains(d)%mesh%edges(:)%parent_index
+        CALL nlsurf_coef(b%domains(d)%mesh, N, omega, b%media(d)%prop(1)%ri, b%media(d)%prop(1)%shri, &
+             epsp(1:b%domains(d)%mesh%nfaces), RESHAPE(x, (/2*N,1/)), b%ga, 1, b%media(d)%prop(1)%nls, &
+             b%qd_tri, (1.0_dp,0.0_dp), (1.0_dp,0.0_dp), src_coef(1:(2*nind),d,1,1), src_vec(1:(2*nind)))
+        WRITE(u) epsp(1:b%domains(d)%mesh%nfaces), src_coef(1:(2*nind),d,1,1), src_vec(1:(2*nind))
+        nlx(ind(1:nind),1,1) = nlx(ind(1:nind),1,1) + src_vec(1:nind)
+        nlx(ind(1:nind)+N,1,1) = nlx(ind(1:nind)+N,1,1) + src_vec((nind+1):(2*nind))
+     END DO
+     rhs0(:) = nlx(:,1,1)
+     ALLOCATE(Ash(2*N,2*N,1))
+     CALL sysmat_pmchwt_nls(b, 1, Ash, src_coef, nlx)
+     WRITE(u) rhs0, nlx(:,1,1)
+     CALL solve_systems(b%mesh, b%ga, (1.0_dp,0.0_dp), (1.0_dp,0.0_dp), Ash, nlx)
+     IF(.NOT. ALLOCATED(rcsdata)) ALLOCATE(rcsdata(nth,nph))
+     CALL rcs(b%domains(1)%mesh, N, 2.0_dp*omega, b%media(1)%prop(1)%shri, b%ga, nlx(:,:,1), &
+          nth, nph, b%qd_tri, rcsdata)
+     WRITE(u) nlx(:,1,1), rcsdata
      CLOSE(u)
   END IF
 

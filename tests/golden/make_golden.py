@@ -12,7 +12,11 @@ Fixtures (all complex128 / float64 at full precision):
   sphere_s2.npz  icosphere s=2: sampled A entries, block norms, q, x, csca,
                  cabs, rcs 37x72, mesh index arrays
   dimer_s1.npz   two s=1 spheres (ids 15, 16), 3 domains, eps_in=-40+2.5i at
-                 1060 nm: sampled A entries, q, x, csca, cabs, mesh arrays
+                 1060 nm: sampled A entries, q, x, csca, cabs, mesh arrays;
+                 second harmonic (surface chi2 nnn = 1 on both spheres, SH
+                 index sqrt(-9.4+1.5i)): nlsurf_coef's src_coef/src_vec and
+                 epsp per sphere, the SH source before and after
+                 sysmat_pmchwt_nls, the SH solution and its rcs 37x72
 Usage: python tests/golden/make_golden.py
 """
 import os
@@ -42,12 +46,12 @@ def mesh_fields(m, full):
     return out
 
 
-def case(name, mshfile, domains, wl, ri, angles, full_A, nsample=3000, seed=1):
+def case(name, mshfile, domains, wl, ri, angles, full_A, nsample=3000, seed=1, sh=None):
     R = refrun.run(mshfile, 1e-9, domains, wl, ri, angles, dump_ops=False, dump_sys=True)
     A, q = R["A"], R["q"]
     x = sl.solve(A, q)
     P = refrun.run(mshfile, 1e-9, domains, wl, ri, angles, dump_ops=False, dump_sys=False, x=x,
-                   ntheta=37, nphi=72)
+                   ntheta=37, nphi=72, sh=sh)
     N = A.shape[0]
     data = dict(wl=np.array(wl), ri=np.array(ri, dtype=np.complex128), angles_deg=np.array(angles, float),
                 domains=np.array([d + [0] * (4 - len(d)) for d in domains]), q=q, x=x,
@@ -55,6 +59,13 @@ def case(name, mshfile, domains, wl, ri, angles, full_A, nsample=3000, seed=1):
     for d, m in enumerate(R["mesh"]):
         for k, v in mesh_fields(m, full_A).items():
             data["mesh%d_%s" % (d, k)] = v
+    if sh is not None:
+        data["sh_shri"] = np.array(sh["shri"], dtype=np.complex128)
+        data["sh_nls"] = np.array(sh["nls"], dtype=np.int32)
+        data["sh_chi2"] = np.array(sh["chi2"], dtype=np.complex128)
+        for k, v in P.items():
+            if k.startswith("sh_"):
+                data[k] = v
     E = N // 2
     data["block_fro"] = np.array([[np.linalg.norm(A[i * E:(i + 1) * E, j * E:(j + 1) * E]) for j in range(2)]
                                   for i in range(2)])
@@ -85,8 +96,10 @@ def main():
     nodes, faces, ids = meshgen.dimer(1, 40.0, 10.0)
     fn = os.path.join(tmp, "dimer1.msh")
     meshgen.write_gmsh(fn, nodes, faces, ids)
+    n_sh = np.sqrt(-9.4 + 1.5j)
     case("dimer_s1", fn, [[15, 16], [-15], [-16]], 1060e-9, [1.0 + 0j, np.sqrt(-40 + 2.5j), np.sqrt(-40 + 2.5j)],
-         (0.0, 90.0, 0.0), False)
+         (0.0, 90.0, 0.0), False,
+         sh=dict(shri=[1.0 + 0j, n_sh, n_sh], nls=[0, 1, 1], chi2=(1.0 + 0j, 0j, 0j)))
 
 
 if __name__ == "__main__":

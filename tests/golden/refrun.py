@@ -38,8 +38,11 @@ def read_mesh(path):
 
 
 def run(mshfile, scale, domains, wl, ri, angles_deg, dump_ops=True, dump_sys=True, x=None,
-        ntheta=5, nphi=7, outdir=None, threads=None):
-    """Run the reference set-up + hot path; returns a dict of arrays."""
+        ntheta=5, nphi=7, outdir=None, threads=None, sh=None):
+    """Run the reference set-up + hot path; returns a dict of arrays.
+
+    sh (needs x): dict(shri=[per domain], nls=[per domain 0/1], chi2=(nnn,
+    ntt, ttn)) runs the second-harmonic chain of solve_batch."""
     outdir = outdir or tempfile.mkdtemp(prefix="momref_")
     if x is not None:
         np.ascontiguousarray(x, dtype=np.complex128).tofile(os.path.join(outdir, "x.bin"))
@@ -49,6 +52,10 @@ def run(mshfile, scale, domains, wl, ri, angles_deg, dump_ops=True, dump_sys=Tru
     lines += ["%r %r" % (complex(r).real, complex(r).imag) for r in ri]
     lines += ["%r %r %r" % tuple(float(a) for a in angles_deg)]
     lines += ["%d %d" % (int(dump_ops), int(dump_sys)), "%d %d" % (ntheta, nphi), outdir]
+    if sh is not None:
+        lines += ["1"] + ["%r %r" % (complex(r).real, complex(r).imag) for r in sh["shri"]]
+        lines += [str(int(f)) for f in sh["nls"]]
+        lines += [" ".join("%r %r" % (complex(c).real, complex(c).imag) for c in sh["chi2"])]
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
@@ -69,6 +76,21 @@ def run(mshfile, scale, domains, wl, ri, angles_deg, dump_ops=True, dump_sys=Tru
         post = np.fromfile(os.path.join(outdir, "post.bin"), np.float64)
         res["csca"], res["cabs"] = post[0], post[1]
         res["rcs"] = post[2:].reshape(ntheta, nphi, order="F")
+    if x is not None and sh is not None:
+        N2 = 2 * res["mesh"][0]["enodes"].shape[0]
+        with open(os.path.join(outdir, "sh.bin"), "rb") as fh:
+            for d, f in enumerate(sh["nls"]):
+                if not f:
+                    continue
+                F = res["mesh"][d + 1]["fnodes"].shape[0]
+                E = res["mesh"][d + 1]["enodes"].shape[0]
+                res["sh_epsp_%d" % d] = np.fromfile(fh, np.complex128, F)
+                res["sh_src_coef_%d" % d] = np.fromfile(fh, np.complex128, 2 * E)
+                res["sh_src_vec_%d" % d] = np.fromfile(fh, np.complex128, 2 * E)
+            res["sh_rhs0"] = np.fromfile(fh, np.complex128, N2)
+            res["sh_rhs"] = np.fromfile(fh, np.complex128, N2)
+            res["sh_x"] = np.fromfile(fh, np.complex128, N2)
+            res["sh_rcs"] = np.fromfile(fh, np.float64, ntheta * nphi).reshape(ntheta, nphi, order="F")
     with open(os.path.join(outdir, "times.txt")) as fh:
         for ln in fh:
             k, v = ln.split()

@@ -164,3 +164,57 @@ def test_far_pairs_symmetric(msh_dir):
         M = op.operator(0, opn, case["omega"], case["ri"][0])
         asym = np.abs(M - M.T)[far].max() / np.abs(M[far]).max()
         assert asym < 1e-12, (opn, asym)
+
+
+def _dimer_sh(msh_dir):
+    g = golden("dimer_s1")
+    case = dimer(msh_dir, 1)
+    op = oracle.OracleProblem(case["path"], 1e-9, case["domains"])
+    return g, case, op
+
+
+def test_oracle_sh_chain_matches_reference(msh_dir):
+    """The second-harmonic chain of solve_batch (solver.f90:143-231) --
+    determine_epsp, nlsurf_coef with its two Gram solves, sysmat_pmchwt_nls
+    and the SH solve -- against the reference's own run on the dimer."""
+    g, case, op = _dimer_sh(msh_dir)
+    out = op.sh_chain(case["omega"], list(g["ri"]), list(g["sh_shri"]), list(g["sh_nls"]), g["sh_chi2"], g["x"])
+    rel = lambda a, b: np.abs(a - b).max() / np.abs(b).max()
+    for d in (1, 2):
+        assert np.array_equal(out["epsp_%d" % d], g["sh_epsp_%d" % d])
+        assert rel(out["src_coef_%d" % d], g["sh_src_coef_%d" % d]) < 1e-13
+        assert rel(out["src_vec_%d" % d], g["sh_src_vec_%d" % d]) < 1e-13
+    assert rel(out["rhs0"], g["sh_rhs0"]) < 1e-13
+    assert rel(out["rhs"], g["sh_rhs"]) < 1e-13
+    assert rel(out["x"], g["sh_x"]) < 1e-12
+    r = op.rcs(2 * case["omega"], complex(g["sh_shri"][0]), out["x"], 37, 72)
+    assert rel(r, g["sh_rcs"]) < 1e-12
+
+
+def test_oracle_gram_closed_form(msh_dir):
+    """rwg_moments against exact integration: f_m . f_n is quadratic on a
+    face, so the edge-midpoint rule (weights A/3) integrates it exactly; the
+    13-point table differs only by its float32 rounding."""
+    case = sphere(msh_dir, 1)
+    op = oracle.OracleProblem(case["path"], 1e-9, case["domains"])
+    F = op.rwg_moments(0)
+    m = op.mesh(0)
+    E = F.shape[0]
+    ref = np.zeros((E, E))
+    for f in range(m["fnodes"].shape[0]):
+        P = m["p"][m["fnodes"][f]]
+        mids = [(P[0] + P[1]) / 2, (P[1] + P[2]) / 2, (P[2] + P[0]) / 2]
+        A = m["farea"][f]
+        fun = []
+        for q in range(3):
+            e = m["fedges"][f, q]
+            plus = m["efaces"][e, 0] == f
+            v = m["p"][m["ebn"][e, 0 if plus else 1]]
+            c = m["elen"][e] / (2 * A) * (1 if plus else -1)
+            fun.append((e, [c * (x - v) for x in mids]))
+        for ea, fa in fun:
+            for eb, fb in fun:
+                ref[ea, eb] += A / 3 * sum(np.dot(x, y) for x, y in zip(fa, fb))
+    assert np.allclose(F, F.T)
+    assert np.abs(F - ref).max() / np.abs(ref).max() < 1e-7
+    assert np.linalg.eigvalsh(F).min() > 0
