@@ -106,6 +106,83 @@ def cpu_baseline(case, subdiv, threads):
                 fill_s=fill_est, lu_s=lu_est)
 
 
+def bench_sh(args):
+    """Second-harmonic workload: dimer of two R = 40 nm spheres (icosphere
+    s each, 10 nm gap along x), 3 domains, eps_in = -40+2.5i at 1060 nm and
+    -9.4+1.5i at the SH, surface chi2 nnn = 1 on both spheres, x-polarised
+    plane wave along +z.  One step = fundamental fill + LU + solve,
+    nlsurf_coef of both spheres, SH fill (sysmat_pmchwt_nls) + LU + solve and
+    the SH radar cross-section on the 37 x 72 grid, all in the mixed path."""
+    from mom2amd import (DEGTORAD, Context, DeviceMesh, System, load_problem, meshgen, omega_of, rcs,
+                         solve_sh)
+    from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
+
+    s = args.subdiv if args.subdiv != 5 else 4
+    nodes, faces, ids = meshgen.dimer(s, 40.0, 10.0)
+    d = tempfile.mkdtemp(prefix="mom_bench_")
+    path = os.path.join(d, "dimer%d.msh" % s)
+    meshgen.write_gmsh(path, nodes, faces, ids)
+    doms = [[15, 16], [-15], [-16]]
+    t0 = time.perf_counter()
+    prob = load_problem(path, 1e-9, doms)
+    N = prob.nedges
+    log("dimer s=%d: %d faces, N=%d unknowns (set-up %.2f s)" % (s, prob.master.nfaces, 2 * N,
+                                                                  time.perf_counter() - t0))
+    omega = omega_of(1060e-9)
+    ri = [1.0 + 0j, complex(np.sqrt(-40 + 2.5j)), complex(np.sqrt(-40 + 2.5j))]
+    shri = [1.0 + 0j, complex(np.sqrt(-9.4 + 1.5j)), complex(np.sqrt(-9.4 + 1.5j))]
+    nls, chi2 = [0, 1, 1], (1.0 + 0j, 0j, 0j)
+    ang = np.array([0.0, 0.0, 0.0]) * DEGTORAD
+    ctx = Context(0)
+    dms = [DeviceMesh(ctx, dm) for dm in prob.domains]
+
+    def step(prec):
+        S = System(ctx, N, prec)
+        try:
+            t = [time.perf_counter()]
+            S.fill(dms, ri, omega)
+            S.factor()
+            x = S.solve_pw(dms[0], omega, ri[0], ang)[:, 0]
+            t.append(time.perf_counter())
+            sh = solve_sh(ctx, prob, dms, omega, ri, shri, nls, chi2, x, precision=prec, system=S)
+            t.append(time.perf_counter())
+            r = rcs(ctx, dms[0], N, 2 * omega, shri[0], sh["x"], 37, 72)
+            t.append(time.perf_counter())
+        finally:
+            S.close()
+        return dict(wall=t[-1] - t[0], fund=t[1] - t[0], sh=t[2] - t[1], rcs=t[3] - t[2], rcs_grid=r,
+                    iters=sh["iters"], berr=sh["berr"])
+
+    for i in range(args.warmup):
+        w = step(MOM_PREC_MIXED)
+        log("warmup %d: %.2f s" % (i, w["wall"]))
+    runs = []
+    for i in range(args.steps):
+        r = step(MOM_PREC_MIXED)
+        runs.append(r)
+        log("step %d: %.2f s (fundamental fill+LU+solve %.2f, SH source+fill+LU+solve %.2f, SH rcs %.3f; "
+            "SH IR iters %d berr %.1e)" % (i, r["wall"], r["fund"], r["sh"], r["rcs"], r["iters"], r["berr"]))
+    mean = float(np.mean([r["wall"] for r in runs]))
+    out = {"metric": "SH chain s @N=%d: fundamental + SH fill/LU/solve, nlsurf_coef, SH rcs 37x72 (mixed path)"
+                     % (2 * N),
+           "value": round(mean, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(mean * 1e3, 2), "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": None, "dtype": "fp32-mfma LU + fp64 fill/residual", "data": "synthetic icosphere dimer",
+           "config": {"workload": "SH dimer: two R=40nm icospheres s=%d, 10 nm gap, %d faces, N=%d unknowns, "
+                                  "1060 nm / 530 nm, chi2_nnn=1" % (s, prob.master.nfaces, 2 * N),
+                      "n_unknowns": 2 * N},
+           "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fund", "sh", "rcs")}}
+    if not args.no_fp64:
+        r64 = step(MOM_PREC_FP64)
+        log("fp64 path: %.2f s (fundamental %.2f, SH %.2f)" % (r64["wall"], r64["fund"], r64["sh"]))
+        out["fp64_path_s"] = round(r64["wall"], 4)
+        out["speedup_vs_fp64_path"] = round(r64["wall"] / mean, 3)
+        out["sh_rcs_relerr_vs_fp64_path"] = float(np.abs(runs[-1]["rcs_grid"] - r64["rcs_grid"]).max()
+                                                  / np.abs(r64["rcs_grid"]).max())
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,10 +191,16 @@ def main():
     ap.add_argument("--subdiv", type=int, default=5, help="icosphere subdivision (5: N = 61440)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--workload", choices=("pmchwt", "sh"), default="pmchwt",
+                    help="pmchwt: the sphere solve (default); sh: the second-harmonic dimer chain (s=4 unless "
+                         "--subdiv is given)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "sh":
+        bench_sh(args)
+        return
 
     from mom2amd import Context, DeviceMesh
     from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
