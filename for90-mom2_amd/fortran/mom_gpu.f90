@@ -175,6 +175,44 @@ MODULE mom_gpu
        INTEGER(C_INT) :: st
      END FUNCTION mom_rcs_ext
 
+     ! Second harmonic, surface chi2 (nlsurf.f90:137, sysmat.f90:112)
+     FUNCTION mom_nlsurf_coef(ctx, mesh, nedges_total, omega, ri_re, ri_im, epsp, x, chi2, nq, qw, qnodes, &
+          src_coef, src_vec) BIND(C, NAME='mom_nlsurf_coef') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR, C_DOUBLE_COMPLEX
+       TYPE(C_PTR), VALUE :: ctx, mesh, x, src_coef
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       COMPLEX(C_DOUBLE_COMPLEX), INTENT(IN) :: epsp(*), chi2(3)
+       INTEGER(C_INT32_T), VALUE :: nq
+       REAL(C_DOUBLE), INTENT(IN) :: qw(*), qnodes(*)
+       COMPLEX(C_DOUBLE_COMPLEX) :: src_vec(*)
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_nlsurf_coef
+
+     FUNCTION mom_sysmat_pmchwt_nls(ctx, ndom, doms, shri, nedges_total, omega, nq, qw, qnodes, nsrc, &
+          src_coef, ldsc, src, ldsrc, A, lda) BIND(C, NAME='mom_sysmat_pmchwt_nls') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: ctx, src_coef, src, A
+       INTEGER(C_INT32_T), VALUE :: ndom, nq
+       TYPE(C_PTR), INTENT(IN) :: doms(*)
+       REAL(C_DOUBLE), INTENT(IN) :: shri(*), qw(*), qnodes(*)
+       INTEGER(C_INT64_T), VALUE :: nedges_total, nsrc, ldsc, ldsrc, lda
+       REAL(C_DOUBLE), VALUE :: omega
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_sysmat_pmchwt_nls
+
+     FUNCTION mom_system_fill_nls(sys, ndom, doms, shri, omega, nq, qw, qnodes, nsrc, src_coef, ldsc, &
+          src, ldsrc) BIND(C, NAME='mom_system_fill_nls') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: sys, src_coef, src
+       INTEGER(C_INT32_T), VALUE :: ndom, nq
+       TYPE(C_PTR), INTENT(IN) :: doms(*)
+       REAL(C_DOUBLE), INTENT(IN) :: shri(*), qw(*), qnodes(*)
+       REAL(C_DOUBLE), VALUE :: omega
+       INTEGER(C_INT64_T), VALUE :: nsrc, ldsc, ldsrc
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_fill_nls
+
      ! Problem set-up (mesh + sdom commands): dom_ptr(ndom+1) offsets into
      ! the signed surface ids dom_ids, 0-based, dom_ptr(1) = 0.
      FUNCTION mom_problem_build(nnodes, p, nfaces, fnodes, fids, scale, ndom, dom_ptr, dom_ids, prob) &
