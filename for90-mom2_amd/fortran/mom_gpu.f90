@@ -1,9 +1,10 @@
 ! MODULE: mom_gpu
 ! ISO_C_BINDING interface of libmomgpu.so (include/mom_gpu.h) for Fortran
 ! hosts.  The drop-in bodies of sysmat_pmchwt, computeD/H/K, srcvec,
-! solve_multi_linsys, cs_prtsrf, far_fields and rcs_ext (sysmat.f90:18,
-! 272, 359, 446; source.f90:136; linalg.f90:159; cs.f90:14;
-! ffields.f90:16, 203) call these functions, and the mesh/sdom set-up
+! solve_multi_linsys, cs_prtsrf, far_fields, rcs_ext, nlsurf_coef and
+! sysmat_pmchwt_nls (sysmat.f90:18, 272, 359, 446, 112; source.f90:136;
+! linalg.f90:159; cs.f90:14; ffields.f90:16, 203; nlsurf.f90:137) call
+! these functions, and the mesh/sdom set-up
 ! (build_mesh, extract_submesh, submesh_edge_connectivity, orient_basis of
 ! mesh.f90) can be replaced by mom_problem_build.
 !
