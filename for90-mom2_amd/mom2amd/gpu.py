@@ -10,6 +10,7 @@ Errors raise capi.MomError, the counterpart of the reference's WRITE + STOP.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import weakref
 
@@ -27,6 +28,20 @@ def _quad(quad):
     return len(w), w, n
 
 
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    # release device memory while the HIP runtime is still up, rather than
+    # from finalizers that may run after it has been torn down
+    for ctx in list(_live_contexts):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One HIP stream on one device (mom_ctx)."""
 
@@ -34,6 +49,7 @@ class Context:
         self.h = C.c_void_p()
         self._children = weakref.WeakSet()
         capi.check(capi.lib().mom_ctx_create(device, C.byref(self.h)))
+        _live_contexts.add(self)
 
     def _adopt(self, child):
         self._children.add(child)
