@@ -1698,7 +1698,7 @@ __device__ __forceinline__ void cmac(typename Cx<TC>::type &acc, typename Cx<TS>
 // Grid (NB/8, nrhs): 8 rows per workgroup (128-byte row segments of M per
 // column), 32 column slices per row, so that 32 workgroups share the 1 MB
 // read of M.
-template <typename TS, typename TC>
+template <typename TS, typename TC, bool H = false>
 __global__ __launch_bounds__(256) void block_apply_kernel(const double2 *__restrict__ M,
                                                            typename Cx<TC>::type *x, int64_t ldx,
                                                            typename Cx<TC>::type *__restrict__ xb, int w) {
@@ -1713,7 +1713,12 @@ __global__ __launch_bounds__(256) void block_apply_kernel(const double2 *__restr
   CV acc{0, 0};
   if (r < w) {
 #pragma unroll 4
-    for (int c = sl; c < w; c += 32) cmac<double, TC>(acc, M[(int64_t)c * NB + r], xs[c]);
+    for (int c = sl; c < w; c += 32) {
+      // H: the conjugate transpose of M
+      double2 m = H ? M[(int64_t)r * NB + c] : M[(int64_t)c * NB + r];
+      if (H) m.y = -m.y;
+      cmac<double, TC>(acc, m, xs[c]);
+    }
   }
   part[sl][rl] = acc;
   __syncthreads();
@@ -1809,6 +1814,85 @@ static int getrs(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, int
   return 0;
 }
 
+// b[r] -= sum_{i < w} conj(A[c0 + i, r]) xb[i] for rows [r0, r1): the block
+// row c0.. of a triangular factor, conjugate-transposed, read as column
+// segments (one wave per output row, four per workgroup)
+template <typename TS, typename TC>
+__global__ __launch_bounds__(256) void panel_gemv_sub_h_kernel(const typename Cx<TS>::type *__restrict__ A,
+                                                                int64_t ld, int64_t r0, int64_t r1, int64_t c0, int w,
+                                                                const typename Cx<TC>::type *__restrict__ xb,
+                                                                typename Cx<TC>::type *b, int64_t ldb) {
+  using CV = typename Cx<TC>::type;
+  const int64_t rhs = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int64_t r = r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= r1) return;
+  const typename Cx<TS>::type *a = A + r * ld + c0;
+  const CV *x = xb + rhs * NB;
+  CV acc{0, 0};
+  for (int i = lane; i < w; i += 64) {
+    typename Cx<TS>::type m = a[i];
+    m.y = -m.y;
+    cmac<TS, TC>(acc, m, x[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc.x += __shfl_xor(acc.x, o);
+    acc.y += __shfl_xor(acc.y, o);
+  }
+  if (lane == 0) {
+    CV *p = b + rhs * ldb + r;
+    CV v = *p;
+    v.x -= acc.x;
+    v.y -= acc.y;
+    *p = v;
+  }
+}
+
+template <typename TV>
+__global__ void scatter_rows_kernel(const TV *__restrict__ src, int64_t lds, TV *__restrict__ dst, int64_t ldd,
+                                    const int *__restrict__ perm, int64_t n, int64_t nrhs) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * nrhs) return;
+  const int64_t i = t % n, r = t / n;
+  dst[r * ldd + perm[i]] = src[r * lds + i];
+}
+
+// Solve Z^H x = b in place on B with P Z = L U (perm as in getrs):
+// Z^H = U^H L^H P, so forward substitution through U^H (diagonal blocks
+// Uinv^H), backward through L^H (Linv^H), and the rows scattered back.
+template <typename TS, typename TC>
+static int getrs_h(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, int64_t n, const int *perm,
+                   const double2 *Linv, const double2 *Uinv, typename Cx<TC>::type *B, int64_t ldb, int64_t nrhs,
+                   typename Cx<TC>::type *tmp, typename Cx<TC>::type *xb) {
+  using CV = typename Cx<TC>::type;
+  if (nrhs > 65535) return fail(MOM_ERR_ARG, "too many right-hand sides for one adjoint solve");
+  MOM_HIP(hipMemcpy2DAsync(tmp, n * sizeof(CV), B, ldb * sizeof(CV), n * sizeof(CV), nrhs, hipMemcpyDeviceToDevice,
+                           s));
+  const int64_t nblk = (n + NB - 1) / NB;
+  for (int64_t b = 0; b < nblk; b++) {
+    const int64_t r0 = b * NB;
+    const int w = (int)std::min<int64_t>(NB, n - r0);
+    hipLaunchKernelGGL((block_apply_kernel<TS, TC, true>), dim3(NB / 8, (unsigned)nrhs), dim3(256), 0, s,
+                       Uinv + b * NB * NB, tmp + r0, n, xb, w);
+    if (r0 + w < n)
+      hipLaunchKernelGGL((panel_gemv_sub_h_kernel<TS, TC>), dim3((unsigned)((n - r0 - w + 3) / 4), (unsigned)nrhs),
+                         dim3(256), 0, s, LU, ld, r0 + w, n, r0, w, xb, tmp, n);
+  }
+  for (int64_t b = nblk - 1; b >= 0; b--) {
+    const int64_t r0 = b * NB;
+    const int w = (int)std::min<int64_t>(NB, n - r0);
+    hipLaunchKernelGGL((block_apply_kernel<TS, TC, true>), dim3(NB / 8, (unsigned)nrhs), dim3(256), 0, s,
+                       Linv + b * NB * NB, tmp + r0, n, xb, w);
+    if (r0 > 0)
+      hipLaunchKernelGGL((panel_gemv_sub_h_kernel<TS, TC>), dim3((unsigned)((r0 + 3) / 4), (unsigned)nrhs), dim3(256),
+                         0, s, LU, ld, (int64_t)0, r0, r0, w, xb, tmp, n);
+  }
+  hipLaunchKernelGGL(scatter_rows_kernel<CV>, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, tmp, n, B,
+                     ldb, perm, n, nrhs);
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace mom
 
 // ---------------------------------------------------------------------------
@@ -1849,6 +1933,44 @@ __global__ void zgemv_reduce_kernel(const double2 *__restrict__ part, int64_t n,
   }
   double2 o = b ? b[r] : make_double2(0, 0);
   y[r] = make_double2(o.x - ar, o.y - ai);
+}
+
+// y[r] = b[r] - sum_c conj(Z[c, r]) x[c] (b may be NULL): Z^H x, one wave
+// per column of Z (contiguous reads), lane partial sums in a fixed order
+__global__ __launch_bounds__(256) void zgemv_h_kernel(const double2 *__restrict__ Z, int64_t ld, int64_t n,
+                                                       const double2 *__restrict__ x, const double2 *__restrict__ b,
+                                                       double2 *y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const double2 *z = Z + r * ld;
+  double ar = 0, ai = 0;
+  for (int64_t c = lane; c < n; c += 64) {
+    const double2 m = z[c], v = x[c];
+    ar += m.x * v.x + m.y * v.y;
+    ai += m.x * v.y - m.y * v.x;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    ar += __shfl_xor(ar, o);
+    ai += __shfl_xor(ai, o);
+  }
+  if (lane == 0) {
+    const double2 o = b ? b[r] : make_double2(0, 0);
+    y[r] = make_double2(o.x - ar, o.y - ai);
+  }
+}
+
+// out[r] = sum_c |Z[c, r]| (column sums: ||Z||_1 = ||Z^H||_inf)
+__global__ __launch_bounds__(256) void col_abssum_kernel(const double2 *__restrict__ Z, int64_t ld, int64_t n,
+                                                          double *out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= n) return;
+  const double2 *z = Z + r * ld;
+  double a = 0;
+  for (int64_t c = lane; c < n; c += 64) a += std::sqrt(z[c].x * z[c].x + z[c].y * z[c].y);
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+  if (lane == 0) out[r] = a;
 }
 
 // part[chunk][r] = (max_c |Z[r,c]|, sum_c |Z[r,c]|) over the chunk
@@ -2080,6 +2202,7 @@ struct mom_system {
   mom::DevBuf Z64, Z32, ipiv, Linv, Uinv, rscale, cscale, rowsum, part, pivwork;
   std::vector<int> perm;
   double znorm_inf = 0;
+  double znorm_one = 0;  // ||Z||_1 of the unfactored complex128 matrix (adjoint refinement; 0: not yet)
   double t_fill = 0, t_factor = 0, t_solve = 0;
   mom::PhaseClock clock;
   int info = 0;
@@ -2202,6 +2325,7 @@ static int upload_perm(mom_system *S, DevBuf &dperm) {
 // Mixed precision solve of one right-hand side b (host, complex128) into x.
 struct IrWork {
   DevBuf dperm, v64, w64, t64, tmp64, xb64, b64, x64;
+  bool adj = false;  // refine Z^H x = b (preconditioner R M^-H C, residual b - Z^H x)
 };
 
 static int apply_minv(mom_system *S, IrWork &W, const std::vector<cd> &v, std::vector<cd> &y) {
@@ -2212,14 +2336,18 @@ static int apply_minv(mom_system *S, IrWork &W, const std::vector<cd> &v, std::v
   const int64_t n = S->n;
   MOM_TRY(copy_h2d(S->ctx, W.v64.ptr, v.data(), sizeof(double2) * n));
   const unsigned g = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(dscale_kernel, dim3(g), dim3(256), 0, s, W.v64.as<double2>(), S->rscale.as<double>(), n,
-                     W.t64.as<double2>());
-  int rc = getrs<float, double>(s, S->Z32.as<float2>(), n, n, W.dperm.as<int>(), S->Linv.as<double2>(),
-                                S->Uinv.as<double2>(), W.t64.as<double2>(), n, 1, W.tmp64.as<double2>(),
-                                W.xb64.as<double2>());
+  // adjoint: Z^-H = R M^-H C with M = R Z C
+  hipLaunchKernelGGL(dscale_kernel, dim3(g), dim3(256), 0, s, W.v64.as<double2>(),
+                     (W.adj ? S->cscale : S->rscale).as<double>(), n, W.t64.as<double2>());
+  int rc = W.adj ? getrs_h<float, double>(s, S->Z32.as<float2>(), n, n, W.dperm.as<int>(), S->Linv.as<double2>(),
+                                         S->Uinv.as<double2>(), W.t64.as<double2>(), n, 1, W.tmp64.as<double2>(),
+                                         W.xb64.as<double2>())
+                 : getrs<float, double>(s, S->Z32.as<float2>(), n, n, W.dperm.as<int>(), S->Linv.as<double2>(),
+                                        S->Uinv.as<double2>(), W.t64.as<double2>(), n, 1, W.tmp64.as<double2>(),
+                                        W.xb64.as<double2>());
   if (rc) return rc;
-  hipLaunchKernelGGL(dscale_kernel, dim3(g), dim3(256), 0, s, W.t64.as<double2>(), S->cscale.as<double>(), n,
-                     W.w64.as<double2>());
+  hipLaunchKernelGGL(dscale_kernel, dim3(g), dim3(256), 0, s, W.t64.as<double2>(),
+                     (W.adj ? S->rscale : S->cscale).as<double>(), n, W.w64.as<double2>());
   MOM_HIP(hipGetLastError());
   y.resize(n);
   return copy_d2h(S->ctx, y.data(), W.w64.ptr, sizeof(double2) * n);
@@ -2233,10 +2361,15 @@ static int residual(mom_system *S, IrWork &W, const std::vector<cd> *b, const st
   const int64_t n = S->n;
   MOM_TRY(copy_h2d(S->ctx, W.x64.ptr, x.data(), sizeof(double2) * n));
   if (b) MOM_TRY(copy_h2d(S->ctx, W.b64.ptr, b->data(), sizeof(double2) * n));
-  hipLaunchKernelGGL(zgemv_part_kernel, dim3((unsigned)((n + 255) / 256), kChunks), dim3(256), 0, s,
-                     S->Z64.as<double2>(), n, n, W.x64.as<double2>(), S->part.as<double2>());
-  hipLaunchKernelGGL(zgemv_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     S->part.as<double2>(), n, b ? W.b64.as<double2>() : nullptr, W.w64.as<double2>());
+  if (W.adj) {
+    hipLaunchKernelGGL(zgemv_h_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, S->Z64.as<double2>(), n, n,
+                       W.x64.as<double2>(), b ? W.b64.as<double2>() : nullptr, W.w64.as<double2>());
+  } else {
+    hipLaunchKernelGGL(zgemv_part_kernel, dim3((unsigned)((n + 255) / 256), kChunks), dim3(256), 0, s,
+                       S->Z64.as<double2>(), n, n, W.x64.as<double2>(), S->part.as<double2>());
+    hipLaunchKernelGGL(zgemv_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       S->part.as<double2>(), n, b ? W.b64.as<double2>() : nullptr, W.w64.as<double2>());
+  }
   MOM_HIP(hipGetLastError());
   y.resize(n);
   return copy_d2h(S->ctx, y.data(), W.w64.ptr, sizeof(double2) * n);
@@ -2274,7 +2407,7 @@ static int gmres_ir(mom_system *S, IrWork &W, const std::vector<cd> &b, std::vec
   int it = 0;
   for (;; it++) {
     if ((rc = residual(S, W, &b, x, r))) return rc;
-    be = inf_norm(r) / (S->znorm_inf * inf_norm(x) + bn);
+    be = inf_norm(r) / ((W.adj ? S->znorm_one : S->znorm_inf) * inf_norm(x) + bn);
     if (it < kBeHistory) S->be_hist[it] = be;
     if (be <= tol || it >= maxit) break;
     if (plain && prev > 0 && be > 0.1 * prev) plain = false;
@@ -2526,6 +2659,7 @@ int mom_system_fill(mom_system *S, int32_t ndom, const mom_mesh *const *doms, co
   MOM_HIP(hipStreamSynchronize(S->ctx->stream));
   S->t_fill = now_s() - t0;
   S->filled = true;
+  S->znorm_one = 0;
   S->factored = false;
   return 0;
 }
@@ -2542,6 +2676,7 @@ int mom_system_fill_nls(mom_system *S, int32_t ndom, const mom_mesh *const *doms
   MOM_HIP(hipStreamSynchronize(S->ctx->stream));
   S->t_fill = now_s() - t0;
   S->filled = true;
+  S->znorm_one = 0;
   S->factored = false;
   return 0;
 }
@@ -2563,6 +2698,7 @@ int mom_system_set_matrix(mom_system *S, const double *A, int64_t lda) {
   MOM_TRY(copy_h2d_2d(S->ctx, S->Z64.ptr, n * sizeof(double2), A, lda * sizeof(double2), n * sizeof(double2), n));
   MOM_HIP(hipStreamSynchronize(S->ctx->stream));
   S->filled = true;
+  S->znorm_one = 0;
   S->factored = false;
   return 0;
 }
@@ -2632,6 +2768,69 @@ int mom_system_solve(mom_system *S, int64_t nrhs, double *B, int64_t ldb, double
   } else {
     IrWork W;
     if ((rc = ir_alloc(S, W))) return rc;
+    int itmax = 0;
+    double bemax = 0;
+    for (int64_t r = 0; r < nrhs; r++) {
+      std::vector<cd> b(n), x;
+      std::memcpy(b.data(), B + 2 * r * ldb, sizeof(double2) * n);
+      int it = 0;
+      double be = 0;
+      rc = gmres_ir(S, W, b, x, tol, maxit, &it, &be);
+      itmax = std::max(itmax, it);
+      bemax = std::max(bemax, be);
+      std::memcpy(B + 2 * r * ldb, x.data(), sizeof(double2) * n);
+      if (rc && rc != MOM_ERR_NOCONV) return rc;
+    }
+    if (iters) *iters = itmax;
+    if (berr) *berr = bemax;
+    if (bemax > tol) return fail(MOM_ERR_NOCONV, "GMRES-IR did not reach the backward-error tolerance");
+  }
+  S->t_solve = now_s() - t0;
+  return 0;
+}
+
+int mom_system_solve_adjoint(mom_system *S, int64_t nrhs, double *B, int64_t ldb, double tol, int32_t maxit,
+                             int32_t *iters, double *berr) {
+  if (!S || !B) return fail(MOM_ERR_ARG, "NULL argument");
+  if (!S->factored) return fail(MOM_ERR_STATE, "solve before factor");
+  if (S->info > 0) return fail(MOM_ERR_STATE, "the factorisation has a zero pivot");
+  if (nrhs < 1 || ldb < S->n) return fail(MOM_ERR_ARG, "bad sizes");
+  (void)hipSetDevice(S->ctx->device);
+  hipStream_t s = S->ctx->stream;
+  const int64_t n = S->n;
+  const double t0 = now_s();
+  int rc;
+  if (S->precision == MOM_PREC_FP64) {
+    const int64_t chunk = std::min<int64_t>(nrhs, 4096);
+    DevBuf dB, tmp, xb, dperm;
+    if ((rc = dB.alloc(sizeof(double2) * n * chunk)) || (rc = tmp.alloc(sizeof(double2) * n * chunk)) ||
+        (rc = xb.alloc(sizeof(double2) * NB * chunk)) || (rc = upload_perm(S, dperm)))
+      return rc;
+    for (int64_t c0 = 0; c0 < nrhs; c0 += chunk) {
+      const int64_t nc = std::min(chunk, nrhs - c0);
+      double *Bc = B + 2 * c0 * ldb;
+      MOM_TRY(copy_h2d_2d(S->ctx, dB.ptr, n * sizeof(double2), Bc, ldb * sizeof(double2), n * sizeof(double2), nc));
+      MOM_TRY((getrs_h<double, double>(s, S->Z64.as<double2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
+                                       S->Uinv.as<double2>(), dB.as<double2>(), n, nc, tmp.as<double2>(),
+                                       xb.as<double2>())));
+      MOM_TRY(copy_d2h_2d(S->ctx, Bc, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nc));
+    }
+    if (iters) *iters = 0;
+    if (berr) *berr = -1.0;
+  } else {
+    if (S->znorm_one == 0) {
+      DevBuf cs;
+      MOM_TRY(cs.alloc(sizeof(double) * n));
+      hipLaunchKernelGGL(col_abssum_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, S->Z64.as<double2>(), n,
+                         n, cs.as<double>());
+      MOM_HIP(hipGetLastError());
+      std::vector<double> h(n);
+      MOM_TRY(copy_d2h(S->ctx, h.data(), cs.ptr, sizeof(double) * n));
+      S->znorm_one = *std::max_element(h.begin(), h.end());
+    }
+    IrWork W;
+    if ((rc = ir_alloc(S, W))) return rc;
+    W.adj = true;
     int itmax = 0;
     double bemax = 0;
     for (int64_t r = 0; r < nrhs; r++) {

@@ -228,6 +228,18 @@ class System:
         capi.check(st)
         return Bv.view(np.complex128).reshape(N2, nrhs, order="F")
 
+    def solve_adjoint(self, B, tol=1e-12, maxit=30):
+        """Z^H X = B on the same factorisation (mom_system_solve_adjoint)."""
+        N2 = 2 * self.N
+        B2 = np.asfortranarray(np.atleast_2d(np.asarray(B).T).T, dtype=np.complex128).copy(order="F")
+        nrhs = B2.shape[1]
+        Bv = np.ascontiguousarray(B2.reshape(-1, order="F").view(np.float64))
+        it, be = C.c_int32(), C.c_double()
+        st = capi.lib().mom_system_solve_adjoint(self.h, nrhs, Bv, N2, tol, maxit, C.byref(it), C.byref(be))
+        self.iters, self.berr = it.value, be.value
+        capi.check(st)
+        return Bv.view(np.complex128).reshape(N2, nrhs, order="F")
+
     def solve_pw(self, dmesh, omega, ri, angles, tol=1e-12, maxit=30, quad=None):
         Q, w, n = _quad(quad)
         ang = np.ascontiguousarray(np.atleast_2d(np.asarray(angles, dtype=np.float64))).ravel()

@@ -144,6 +144,17 @@ int mom_system_factor(mom_system *sys);
  */
 int mom_system_solve(mom_system *sys, int64_t nrhs, double *B, int64_t ldb, double tol,
                      int32_t maxit, int32_t *iters, double *berr);
+/*
+ * Adjoint solve on the same factorisation: Z^H X = B (conjugate transpose;
+ * B[2N x nrhs] host, overwritten by X), the Z^H lambda = dsigma/dx solve of
+ * an adjoint sensitivity.  MOM_PREC_FP64: direct solve with the factors.
+ * MOM_PREC_MIXED: the FP32 factors as preconditioner (R M^-H C) and GMRES-IR
+ * against the complex128 Z^H to backward error ||b - Z^H x|| / (||Z||_1 ||x||
+ * + ||b||) (infinity norms) <= tol, one right-hand side at a time.  There is
+ * no counterpart in the reference; linalg.f90:137-180 only solves with Z.
+ */
+int mom_system_solve_adjoint(mom_system *sys, int64_t nrhs, double *B, int64_t ldb, double tol,
+                             int32_t maxit, int32_t *iters, double *berr);
 /* Build the plane-wave right-hand sides on the device and solve in place of
  * mom_srcvec_pw + mom_system_solve; X[2N x nsrc] receives the solutions. */
 int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, double ri_re,
