@@ -177,7 +177,10 @@ constexpr int kTestPerBlock = kWaves * kTestPerWave;
 // QT > 0 fixes the rule size at compile time (full unrolling of the source
 // point loop, so the exp/sincos chains of different points interleave);
 // QT == 0 takes a.Q at run time.
-template <bool SMOOTH, int QT>
+// DK: the k-derivatives of the kernels instead (derivative of the fill with
+// respect to the wavenumber): dG/dk = i R G, dg/dk = k G; smoothed parts
+// dGs/dk = (i e^{ikR} + k R)/4pi, dgs/dk = k (e^{ikR} - 1)/(4pi R).
+template <bool SMOOTH, int QT, bool DK = false>
 __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *__restrict__ tpts,
                                                 const double *__restrict__ spts, int lane,
                                                 double dx, double dy, double dz, double acc[36]) {
@@ -204,19 +207,39 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
         const double R = R2 * iR;
         double Er, Ei;
         cis_exp(-ki * R, kr * R, Er, Ei);
-        const double e = iR * kOofpi;
-        Gr = e * Er;
-        Gi = e * Ei;
-        // g = G * ((1/R^2 + ki/R) - i kr/R)
-        const double ar = iR * iR + ki * iR, ai = -kr * iR;
-        gr = Gr * ar - Gi * ai;
-        gi = Gr * ai + Gi * ar;
+        if constexpr (DK) {
+          // dG/dk = i e^{ikR}/4pi, dg/dk = k G
+          const double e = iR * kOofpi;
+          const double G0r = e * Er, G0i = e * Ei;
+          Gr = -Ei * kOofpi;
+          Gi = Er * kOofpi;
+          gr = kr * G0r - ki * G0i;
+          gi = kr * G0i + ki * G0r;
+        } else {
+          const double e = iR * kOofpi;
+          Gr = e * Er;
+          Gi = e * Ei;
+          // g = G * ((1/R^2 + ki/R) - i kr/R)
+          const double ar = iR * iR + ki * iR, ai = -kr * iR;
+          gr = Gr * ar - Gi * ai;
+          gi = Gr * ai + Gi * ar;
+        }
       } else if (R2 == 0.0) {
         // green.f90:33-34 (Gs = ik/4pi) and 88-91 (grad Gs = 0)
-        Gr = -ki * kOofpi;
-        Gi = kr * kOofpi;
+        Gr = DK ? 0.0 : -ki * kOofpi;
+        Gi = DK ? kOofpi : kr * kOofpi;
         gr = 0.0;
         gi = 0.0;
+      } else if (DK) {
+        const double iR = rsqrt(R2);
+        const double R = R2 * iR;
+        double Er, Ei;
+        cis_exp(-ki * R, kr * R, Er, Ei);
+        Gr = (-Ei + kr * R) * kOofpi;
+        Gi = (Er + ki * R) * kOofpi;
+        const double f = iR * kOofpi;
+        gr = (kr * (Er - 1.0) - ki * Ei) * f;
+        gi = (kr * Ei + ki * (Er - 1.0)) * f;
       } else {
         const double iR = rsqrt(R2);
         const double R = R2 * iR;
@@ -295,7 +318,7 @@ struct PlaneArgs {
 constexpr int kSrcPerWave = 4;
 constexpr int kSrcPerBlock = kWaves * kSrcPerWave;
 
-template <int QT>
+template <int QT, bool DK = false>
 __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
   const FarArgs &a = pa.f;
   __shared__ double tpts[3 * kMaxQ * 64];          // test points, [3j+c][lane]
@@ -342,9 +365,9 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
     double acc[36];
     for (int k = 0; k < 36; k++) acc[k] = 0.0;
     if (near)
-      pair_point_loop<true, QT>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
+      pair_point_loop<true, QT, DK>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
     else
-      pair_point_loop<false, QT>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
+      pair_point_loop<false, QT, DK>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
     if (!valid) continue;
 
     const double AA = a.area[mm] * a.area[n];
@@ -920,7 +943,9 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     pa.mlo = pa.n0;  // pairs with both faces below n0 were integrated by earlier slabs
     const int scur = (int)std::min<int64_t>(S, m->F - pa.n0);
     dim3 grid((unsigned)((m->F - pa.mlo + 63) / 64), (unsigned)((scur + kSrcPerBlock - 1) / kSrcPerBlock));
-    if (rule.Q == 13)
+    if (t.dk)
+      hipLaunchKernelGGL((pair_plane_kernel<0, true>), grid, dim3(256), 0, ctx->stream, pa);
+    else if (rule.Q == 13)
       hipLaunchKernelGGL(pair_plane_kernel<13>, grid, dim3(256), 0, ctx->stream, pa);
     else
       hipLaunchKernelGGL(pair_plane_kernel<0>, grid, dim3(256), 0, ctx->stream, pa);
@@ -961,8 +986,9 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     b.area = a.area;
     b.fnodes = m->fnodes.as<int>();
     b.side = a.side;
-    b.kk4 = make_double2(1.0 / (4.0 * MOM_PI), 0.0);
-    const cd kk8 = (k * k) / (8.0 * MOM_PI);
+    // analytic part [1/R - k^2 R/2]/4pi; with dk its k-derivative -k R/4pi
+    b.kk4 = make_double2(t.dk ? 0.0 : 1.0 / (4.0 * MOM_PI), 0.0);
+    const cd kk8 = t.dk ? k / (4.0 * MOM_PI) : (k * k) / (8.0 * MOM_PI);
     b.kk8 = make_double2(kk8.real(), kk8.imag());
     b.c1 = a.c1;
     b.c2 = a.c2;

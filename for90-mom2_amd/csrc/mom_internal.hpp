@@ -131,6 +131,7 @@ struct FillTarget {
   int64_t Noff = 0;   // offset of the M rows/columns (nedges_total)
   int mode = 0;       // 0: PMCHWT blocks with global indices; 1/2/3: D/H/K, local indices
   int local = 0;      // mode 0 with local indices (the domain's own 2E x 2E block, Noff = E)
+  int dk = 0;         // the k-derivative of the operators (same prefactors c1, c2, zsi)
   double2 zsi;        // Zsi = ri^2 / eta0^2 (mode 0)
 };
 

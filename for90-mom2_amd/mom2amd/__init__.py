@@ -8,3 +8,4 @@ from .mesh import DomainMesh, Problem, build_problem, load_problem  # noqa: F401
 from .gpu import (Context, DeviceMesh, System, computeD, computeH, computeK,  # noqa: F401
                   cs_prtsrf, far_fields, rcs, rcs_ext, solve_multi_linsys, srcvec, sysmat_pmchwt)
 from .nlsurf import determine_epsp, nlsurf_coef, solve_sh, sysmat_pmchwt_nls  # noqa: F401
+from .adjoint import cs_abs_gradient, pmchwt_deps_apply, sigma_abs_eps_gradient  # noqa: F401

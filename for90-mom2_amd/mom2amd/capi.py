@@ -23,7 +23,7 @@ EXPORTS = [
     "mom_mesh_near_pairs", "mom_compute_operator", "mom_sysmat_pmchwt", "mom_srcvec_pw",
     "mom_solve_multi_linsys", "mom_system_create", "mom_system_destroy", "mom_system_fill",
     "mom_system_get_matrix", "mom_system_set_matrix", "mom_system_factor", "mom_system_solve",
-    "mom_system_solve_adjoint",
+    "mom_system_solve_adjoint", "mom_cs_abs_gradient", "mom_pmchwt_deps_apply",
     "mom_system_solve_pw", "mom_system_profile", "mom_system_stats", "mom_cs_prtsrf", "mom_far_fields",
     "mom_rcs_ext", "mom_nlsurf_coef", "mom_sysmat_pmchwt_nls", "mom_system_fill_nls", "mom_problem_build", "mom_problem_load_gmsh", "mom_problem_destroy", "mom_problem_sizes", "mom_problem_domain",
     "mom_problem_topology", "mom_problem_mesh_create",
@@ -75,6 +75,8 @@ def lib():
     L.mom_system_set_matrix.argtypes = [_vp, _dp, _i64]
     L.mom_system_factor.argtypes = [_vp]
     L.mom_system_solve.argtypes = [_vp, _i64, _dp, _i64, _d, _i32, C.POINTER(_i32), C.POINTER(_d)]
+    L.mom_cs_abs_gradient.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _i32, _dp, _dp, _dp, C.POINTER(_d)]
+    L.mom_pmchwt_deps_apply.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _dp, _dp, _dp, _dp]
     L.mom_system_solve_adjoint.argtypes = [_vp, _i64, _dp, _i64, _d, _i32, C.POINTER(_i32), C.POINTER(_d)]
     L.mom_system_solve_pw.argtypes = [_vp, _vp, _d, _d, _d, _i64, _dp, _i32, _dp, _dp, _d, _i32, _dp, _i64,
                                       C.POINTER(_i32), C.POINTER(_d)]

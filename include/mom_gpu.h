@@ -155,6 +155,26 @@ int mom_system_solve(mom_system *sys, int64_t nrhs, double *B, int64_t ldb, doub
  */
 int mom_system_solve_adjoint(mom_system *sys, int64_t nrhs, double *B, int64_t ldb, double tol,
                              int32_t maxit, int32_t *iters, double *berr);
+/*
+ * Adjoint sensitivities (no counterpart in the reference).  For the
+ * solution x of Z x = b and an observable sigma(x) with dsigma = Re(g^T dx):
+ *   dsigma/dp = -Re(lambda^T (dZ/dp) x),  lambda = conj(mu),
+ *   Z^H mu = conj(g)  (mom_system_solve_adjoint).
+ * mom_cs_abs_gradient: sigma_abs of cs_prtsrf (cs.f90:14-104, medium index
+ * ri, domain mesh normally domain 1) for the solution x[4N] and its gradient
+ * g[4N] (complex).  mom_pmchwt_deps_apply: y[4N] = (dZ/deps) x for the
+ * relative permittivity eps = ri^2 of the medium of one domain mesh (the
+ * derivative of that domain's PMCHWT blocks, sysmat.f90:18-105, from a fill
+ * of the k-derivatives of its operators, dk/deps = k/(2 eps), and the eps
+ * dependence of the H prefactor and of Zsi).  With p = Re eps, dZ/dp = dZ/deps;
+ * with p = Im eps, dZ/dp = i dZ/deps.
+ */
+int mom_cs_abs_gradient(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                        double ri_im, const double *x, int32_t nq, const double *qw, const double *qnodes,
+                        double *g, double *cabs);
+int mom_pmchwt_deps_apply(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                          double ri_im, int32_t nq, const double *qw, const double *qnodes, const double *x,
+                          double *y);
 /* Build the plane-wave right-hand sides on the device and solve in place of
  * mom_srcvec_pw + mom_system_solve; X[2N x nsrc] receives the solutions. */
 int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, double ri_re,
