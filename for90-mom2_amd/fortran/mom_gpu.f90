@@ -176,6 +176,44 @@ MODULE mom_gpu
        INTEGER(C_INT) :: st
      END FUNCTION mom_rcs_ext
 
+     ! Adjoint sensitivities (no counterpart in the reference)
+     FUNCTION mom_system_solve_adjoint(sys, nrhs, B, ldb, tol, maxit, iters, berr) &
+          BIND(C, NAME='mom_system_solve_adjoint') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+       TYPE(C_PTR), VALUE :: sys, B
+       INTEGER(C_INT64_T), VALUE :: nrhs, ldb
+       REAL(C_DOUBLE), VALUE :: tol
+       INTEGER(C_INT32_T), VALUE :: maxit
+       INTEGER(C_INT32_T) :: iters
+       REAL(C_DOUBLE) :: berr
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_solve_adjoint
+
+     FUNCTION mom_cs_abs_gradient(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, nq, qw, qnodes, g, cabs) &
+          BIND(C, NAME='mom_cs_abs_gradient') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR, C_DOUBLE_COMPLEX
+       TYPE(C_PTR), VALUE :: ctx, mesh, x
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       INTEGER(C_INT32_T), VALUE :: nq
+       REAL(C_DOUBLE), INTENT(IN) :: qw(*), qnodes(*)
+       COMPLEX(C_DOUBLE_COMPLEX) :: g(*)
+       REAL(C_DOUBLE) :: cabs
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_cs_abs_gradient
+
+     FUNCTION mom_pmchwt_deps_apply(ctx, mesh, nedges_total, omega, ri_re, ri_im, nq, qw, qnodes, x, y) &
+          BIND(C, NAME='mom_pmchwt_deps_apply') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR, C_DOUBLE_COMPLEX
+       TYPE(C_PTR), VALUE :: ctx, mesh, x
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       INTEGER(C_INT32_T), VALUE :: nq
+       REAL(C_DOUBLE), INTENT(IN) :: qw(*), qnodes(*)
+       COMPLEX(C_DOUBLE_COMPLEX) :: y(*)
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_pmchwt_deps_apply
+
      ! Second harmonic, surface chi2 (nlsurf.f90:137, sysmat.f90:112)
      FUNCTION mom_nlsurf_coef(ctx, mesh, nedges_total, omega, ri_re, ri_im, epsp, x, chi2, nq, qw, qnodes, &
           src_coef, src_vec) BIND(C, NAME='mom_nlsurf_coef') RESULT(st)
