@@ -2197,7 +2197,9 @@ struct mom_system {
   mom_ctx *ctx = nullptr;
   int64_t N = 0;  // nedges_total
   int64_t n = 0;  // 2N
-  int precision = MOM_PREC_FP64;
+  int precision = MOM_PREC_FP64;       // precision of the current factors
+  int precision_req = MOM_PREC_FP64;   // precision asked for at creation (restored by every fill)
+  bool fell_back = false;              // the last block solve moved a mixed system to FP64 factors
   bool filled = false, factored = false;
   mom::DevBuf Z64, Z32, ipiv, Linv, Uinv, rscale, cscale, rowsum, part, pivwork;
   std::vector<int> perm;
@@ -2556,8 +2558,33 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
     prev = be;
   }
   MOM_TRY(copy_d2h_2d(S->ctx, B, ldb * sizeof(double2), dX.ptr, n * sizeof(double2), n * sizeof(double2), nrhs));
-  // columns that stalled: restarted GMRES on M^-1 Z, one at a time, from the
-  // original right-hand side (still in dB); the block's statistics are kept
+  // Columns that stalled (typically near a resonance, where the FP32 factors'
+  // backward error is large) are finished by GMRES one at a time.  When that
+  // would cost more than factoring the resident complex128 matrix -- about
+  // 15 GMRES steps per column, each one application of the factors and one
+  // product, against 8n^3/3 flops at about 60 TFLOP/s -- the system moves to
+  // FP64 factors instead and solves every column directly (until the next
+  // fill).
+  int64_t nstall = 0;
+  for (int64_t j = 0; j < nrhs; j++) nstall += be[j] > tol;
+  const double nd = (double)n;
+  const double t_lu64 = 8.0 / 3.0 * nd * nd * nd / 6.0e13;
+  const double t_col = 15.0 * (24.0 * nd * nd / 4.0e12 + 1.0e-3);
+  if (nstall > 0 && (double)nstall * t_col > t_lu64) {
+    MOM_TRY(factor_impl<double>(S, S->Z64.as<double2>()));
+    S->precision = MOM_PREC_FP64;
+    S->fell_back = true;
+    if (S->info > 0) return fail(MOM_ERR_STATE, "the FP64 factorisation has a zero pivot");
+    DevBuf dperm64;
+    MOM_TRY(upload_perm(S, dperm64));
+    MOM_TRY(getrs_blocked<double>(S->ctx, S->Z64.as<double2>(), n, n, dperm64.as<int>(), S->Linv.as<double2>(),
+                                  S->Uinv.as<double2>(), dB.as<double2>(), n, nrhs, dT2.as<double2>(),
+                                  dW.as<double2>(), nullptr));
+    MOM_TRY(copy_d2h_2d(S->ctx, B, ldb * sizeof(double2), dB.ptr, n * sizeof(double2), n * sizeof(double2), nrhs));
+    if (iters) *iters = it;
+    if (berr) *berr = -1.0;
+    return 0;
+  }
   const int nm = S->n_minv, nv = S->n_matvec;
   double hist[kBeHistory];
   std::copy(S->be_hist, S->be_hist + kBeHistory, hist);
@@ -2620,6 +2647,7 @@ int mom_system_create(mom_ctx *ctx, int64_t nedges_total, int32_t precision, mom
   S->N = nedges_total;
   S->n = 2 * nedges_total;
   S->precision = precision;
+  S->precision_req = precision;
   const int64_t n = S->n;
   int rc;
   if ((rc = S->Z64.alloc(sizeof(double2) * n * n)) || (rc = S->ipiv.alloc(sizeof(int) * n)) ||
@@ -2660,6 +2688,8 @@ int mom_system_fill(mom_system *S, int32_t ndom, const mom_mesh *const *doms, co
   S->t_fill = now_s() - t0;
   S->filled = true;
   S->znorm_one = 0;
+  S->precision = S->precision_req;
+  S->fell_back = false;
   S->factored = false;
   return 0;
 }
@@ -2677,6 +2707,8 @@ int mom_system_fill_nls(mom_system *S, int32_t ndom, const mom_mesh *const *doms
   S->t_fill = now_s() - t0;
   S->filled = true;
   S->znorm_one = 0;
+  S->precision = S->precision_req;
+  S->fell_back = false;
   S->factored = false;
   return 0;
 }
@@ -2699,6 +2731,8 @@ int mom_system_set_matrix(mom_system *S, const double *A, int64_t lda) {
   MOM_HIP(hipStreamSynchronize(S->ctx->stream));
   S->filled = true;
   S->znorm_one = 0;
+  S->precision = S->precision_req;
+  S->fell_back = false;
   S->factored = false;
   return 0;
 }
@@ -2879,6 +2913,7 @@ int mom_system_stats(const mom_system *S, double *out) {
   out[19] = S->clock.seconds[mom::kPhGemmNarrow];
   out[20] = S->clock.launches[mom::kPhGemmNarrow];
   out[21] = S->clock.flops[mom::kPhGemmNarrow];
+  out[22] = S->fell_back ? 1.0 : 0.0;
   return 0;
 }
 

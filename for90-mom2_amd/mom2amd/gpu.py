@@ -254,12 +254,13 @@ class System:
         return X.view(np.complex128).reshape(N2, nsrc, order="F")
 
     def stats(self):
-        out = np.zeros(22)
+        out = np.zeros(23)
         capi.check(capi.lib().mom_system_stats(self.h, out))
         return dict(fill_s=out[0], factor_s=out[1], solve_s=out[2], gemm_s=out[3], gemm_launches=out[4],
                     gemm_flops=out[5], panel_s=out[6], trsm_s=out[7], laswp_s=out[8], n_minv=int(out[9]),
                     n_matvec=int(out[10]), berr_history=[float(v) for v in out[11:19] if v >= 0],
-                    gemm_narrow_s=out[19], gemm_narrow_launches=int(out[20]), gemm_narrow_flops=out[21])
+                    gemm_narrow_s=out[19], gemm_narrow_launches=int(out[20]), gemm_narrow_flops=out[21],
+                    fell_back_to_fp64=bool(out[22]))
 
     def close(self):
         if self.h:

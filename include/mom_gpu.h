@@ -140,7 +140,10 @@ int mom_system_factor(mom_system *sys);
  * worst column.  MOM_ERR_NOCONV if tol is not met.  From 4 right-hand sides
  * on, the triangular solves and the refinement run on the whole block
  * (matrix products per 256-row block, one residual product per step);
- * columns whose refinement stalls are finished one by one by GMRES.
+ * columns whose refinement stalls are finished one by one by GMRES, or,
+ * when that is estimated to cost more than an FP64 factorisation, the
+ * resident complex128 matrix is factored and every column solved directly:
+ * the system then keeps FP64 factors until the next fill (berr = -1).
  */
 int mom_system_solve(mom_system *sys, int64_t nrhs, double *B, int64_t ldb, double tol,
                      int32_t maxit, int32_t *iters, double *berr);
@@ -187,7 +190,7 @@ int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, dou
 int mom_system_profile(mom_system *sys, int32_t enable);
 /*
  * Wall times of the last fill / factor / solve [s] and, with profiling
- * enabled, the factorisation's kernel time by group (22 doubles):
+ * enabled, the factorisation's kernel time by group (23 doubles):
  * out[0] fill, out[1] factor, out[2] solve;
  * out[3] kernel time of the trailing updates (depth NB = 256) [s], out[4]
  * their number of launches, out[5] their algorithmic flops (8 M N K each);
@@ -199,7 +202,8 @@ int mom_system_profile(mom_system *sys, int32_t enable);
  * error after outer steps 0..7 (-1 where no step was taken);
  * out[19] time of the shallower updates (inner panel and triangular-solve
  * blocks, depth 32) with their panel transposes [s], out[20] their number
- * of launches, out[21] their algorithmic flops.
+ * of launches, out[21] their algorithmic flops; out[22] 1 if the last block
+ * solve moved a mixed system to FP64 factors (see mom_system_solve).
  */
 int mom_system_stats(const mom_system *sys, double *out);
 

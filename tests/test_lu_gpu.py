@@ -173,3 +173,39 @@ def test_lu_nan_reports_without_fault(gpu_ctx, panel_mode, where):
         return
     assert ((ipiv >= np.arange(1, n + 1)) & (ipiv <= n)).all()
     assert not np.isfinite(X).all()
+
+
+def test_block_refinement_falls_back_to_fp64(gpu_ctx):
+    """A matrix with condition number 1e9 defeats the FP32 factors (kappa
+    eps32 > 1): every column of a block solve stalls, the system factors its
+    complex128 matrix instead and solves directly; the next matrix set
+    brings the mixed path back."""
+    n = 1024
+    rng = np.random.default_rng(21)
+    U, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    A = np.asfortranarray((U * np.logspace(0, -9, n)) @ V.conj().T)
+    B = rand_complex(n, 8, seed=22)
+    S = System(gpu_ctx, n // 2, MOM_PREC_MIXED)
+    try:
+        S.set_matrix(A)
+        S.factor()
+        X = S.solve(B)
+        st = S.stats()
+        print("ill-conditioned block: fell back %s, refinement history %s" % (st["fell_back_to_fp64"],
+                                                                             st["berr_history"]), flush=True)
+        assert st["fell_back_to_fp64"]
+        for j in range(B.shape[1]):
+            assert berr(A, X[:, j], B[:, j]) < 1e-14, j
+        x1 = S.solve(B[:, :1])  # later solves use the FP64 factors
+        assert np.abs(x1[:, 0] - X[:, 0]).max() <= 1e-12 * np.abs(X[:, 0]).max()
+        A2 = rand_complex(n, seed=23, diag=30.0)
+        S.set_matrix(A2)
+        S.factor()
+        X2 = S.solve(B)
+        st2 = S.stats()
+        assert not st2["fell_back_to_fp64"] and S.berr > 0
+        for j in range(B.shape[1]):
+            assert berr(A2, X2[:, j], B[:, j]) < 1e-13, j
+    finally:
+        S.close()
