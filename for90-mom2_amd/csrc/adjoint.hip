@@ -12,6 +12,7 @@
 //    eps_d = ri_d^2 of one domain, from a fill of the k-derivatives of that
 //    domain's operators (dk/deps = k / (2 eps)) plus the explicit eps
 //    dependence of the H prefactor (1/eps) and of Zsi = eps/eta0^2.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -85,25 +86,47 @@ __global__ __launch_bounds__(128) void cs_abs_grad_kernel(CsGradArgs a) {
   a.part[f] = a.half_c * sre;
 }
 
-// y = M x for an nr x nc complex128 matrix (ld), rows by thread, columns in
-// order; several right-hand sides over blockIdx.y
-__global__ __launch_bounds__(256) void matvec_kernel(const double2 *__restrict__ M, int64_t ld, int64_t nr, int64_t nc,
-                                                     const double2 *__restrict__ x, double2 *y) {
+// y = M x for an nr x nc complex128 matrix (ld): 32 column chunks per row
+// block (partial sums in a fixed order, then one reduction), so that a
+// block of a few thousand rows still spreads over the whole device
+constexpr int kMvChunks = 32;
+
+__global__ __launch_bounds__(256) void matvec_part_kernel(const double2 *__restrict__ M, int64_t ld, int64_t nr,
+                                                          int64_t nc, const double2 *__restrict__ x, double2 *part) {
+  const int64_t cw = (nc + kMvChunks - 1) / kMvChunks;
+  const int64_t c0 = blockIdx.y * cw, c1 = std::min<int64_t>(nc, c0 + cw);
   const int64_t r = blockIdx.x * 256 + threadIdx.x;
   if (r >= nr) return;
   double ar = 0, ai = 0;
-  for (int64_t c = 0; c < nc; c++) {
+  for (int64_t c = c0; c < c1; c++) {
     const double2 m = M[r + c * ld], v = x[c];
     ar += m.x * v.x - m.y * v.y;
     ai += m.x * v.y + m.y * v.x;
   }
+  part[blockIdx.y * nr + r] = make_double2(ar, ai);
+}
+
+__global__ void matvec_reduce_kernel(const double2 *__restrict__ part, int64_t nr, double2 *y) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  double ar = 0, ai = 0;
+  for (int k = 0; k < kMvChunks; k++) {
+    ar += part[k * nr + r].x;
+    ai += part[k * nr + r].y;
+  }
   y[r] = make_double2(ar, ai);
 }
 
-static int matvec(mom_ctx *ctx, const DevBuf &M, int64_t ld, int64_t nr, int64_t nc, const double2 *x, double2 *y) {
-  hipLaunchKernelGGL(matvec_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, ctx->stream, M.as<double2>(), ld,
-                     nr, nc, x, y);
+int matvec(mom_ctx *ctx, const double2 *M, int64_t ld, int64_t nr, int64_t nc, const double2 *x, double2 *y) {
+  DevBuf part;
+  MOM_TRY(part.alloc(sizeof(double2) * nr * kMvChunks));
+  hipLaunchKernelGGL(matvec_part_kernel, dim3((unsigned)((nr + 255) / 256), kMvChunks), dim3(256), 0, ctx->stream, M,
+                     ld, nr, nc, x, part.as<double2>());
+  hipLaunchKernelGGL(matvec_reduce_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, ctx->stream,
+                     part.as<double2>(), nr, y);
   MOM_HIP(hipGetLastError());
+  // the partial sums must outlive the kernels
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
@@ -225,10 +248,10 @@ int mom_pmchwt_deps_apply(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_tot
   }
   MOM_TRY(copy_h2d(ctx, dxl.ptr, xl.data(), sizeof(double2) * 2 * E));
   const double2 *xJ = dxl.as<double2>(), *xM = dxl.as<double2>() + E;
-  MOM_TRY(matvec(ctx, P, 2 * E, 2 * E, 2 * E, xJ, dyp.as<double2>()));
-  MOM_TRY(matvec(ctx, D, E, E, E, xM, dyd.as<double2>()));
-  MOM_TRY(matvec(ctx, H, E, E, E, xJ, dyh.as<double2>()));
-  MOM_TRY(matvec(ctx, H, E, E, E, xM, dyhm.as<double2>()));
+  MOM_TRY(matvec(ctx, P.as<double2>(), 2 * E, 2 * E, 2 * E, xJ, dyp.as<double2>()));
+  MOM_TRY(matvec(ctx, D.as<double2>(), E, E, E, xM, dyd.as<double2>()));
+  MOM_TRY(matvec(ctx, H.as<double2>(), E, E, E, xJ, dyh.as<double2>()));
+  MOM_TRY(matvec(ctx, H.as<double2>(), E, E, E, xM, dyhm.as<double2>()));
   std::vector<cd> yp(2 * E), ydm(E), yhj(E), yhm(E);
   MOM_TRY(copy_d2h(ctx, yp.data(), dyp.ptr, sizeof(double2) * 2 * E));
   MOM_TRY(copy_d2h(ctx, ydm.data(), dyd.ptr, sizeof(double2) * E));

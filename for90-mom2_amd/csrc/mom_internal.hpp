@@ -150,6 +150,9 @@ int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
 // x the device solution [2 Nt], k the medium's wavenumber
 int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, cd k, const double2 *x,
                         int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out);
+// y = M x (device, complex128, M nr x nc with leading dimension ld), in
+// column chunks spread over the device (adjoint.hip)
+int matvec(mom_ctx *ctx, const double2 *M, int64_t ld, int64_t nr, int64_t nc, const double2 *x, double2 *y);
 // sysmat_pmchwt_nls (sysmat.f90:112-229) into the device matrix Z, with the
 // products of the nonlinear domains' blocks and src_coef subtracted from the
 // host sources src (nls.hip)

@@ -573,22 +573,6 @@ __global__ void scatter_block_kernel(const double2 *__restrict__ Zd, int64_t E, 
   }
 }
 
-// y = Zd v for nsrc columns (rows-by-thread, columns in fixed order)
-__global__ __launch_bounds__(256) void block_matvec_kernel(const double2 *__restrict__ Zd, int64_t n,
-                                                           const double2 *__restrict__ V, double2 *Y) {
-  const int64_t r = blockIdx.x * 256 + threadIdx.x;
-  const int s = blockIdx.y;
-  if (r >= n) return;
-  const double2 *v = V + (int64_t)s * n;
-  double ar = 0, ai = 0;
-  for (int64_t c = 0; c < n; c++) {
-    const double2 m = Zd[r + c * n], x = v[c];
-    ar += m.x * x.x - m.y * x.y;
-    ai += m.x * x.y + m.y * x.x;
-  }
-  Y[(int64_t)s * n + r] = make_double2(ar, ai);
-}
-
 int fill_system_nls(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *shri,
                     int64_t nedges_total, double omega, int32_t nq, const double *qw, const double *qnodes,
                     double2 *Z, int64_t ldz, int64_t nsrc, const double *src_coef, int64_t ldsc, double *src,
@@ -652,9 +636,9 @@ int fill_system_nls(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, con
     }
     MOM_TRY(copy_h2d(ctx, dV.ptr, V.data(), sizeof(double) * V.size()));
     MOM_TRY(copy_h2d(ctx, dg.ptr, m->h_eparent.data(), sizeof(int) * E));
-    hipLaunchKernelGGL(block_matvec_kernel, dim3((unsigned)((2 * E + 255) / 256), (unsigned)nsrc), dim3(256), 0,
-                       ctx->stream, Zd.as<double2>(), 2 * E, dV.as<double2>(), dY.as<double2>());
-    MOM_HIP(hipGetLastError());
+    for (int64_t sidx = 0; sidx < nsrc; sidx++)
+      MOM_TRY(matvec(ctx, Zd.as<double2>(), 2 * E, 2 * E, 2 * E, dV.as<double2>() + 2 * E * sidx,
+                     dY.as<double2>() + 2 * E * sidx));
     hipLaunchKernelGGL(scatter_block_kernel, dim3((unsigned)((2 * E + 255) / 256), (unsigned)std::min<int64_t>(2 * E, 32768)), dim3(256), 0,
                        ctx->stream, Zd.as<double2>(), E, dg.as<int>(), Nt, Z, ldz);
     MOM_HIP(hipGetLastError());
