@@ -155,9 +155,12 @@ int make_rule(int32_t nq, const double *qw, const double *qnodes, QuadRule *r) {
 }
 
 // Near face pairs: centroid distance^2 < (3*avelen)^2 (sysmat.f90:233-248),
-// evaluated exactly as the reference does, found with a uniform grid.
-static void near_pairs(const std::vector<double> &cp, int64_t F, double avelen,
-                       std::vector<int32_t> &out) {
+// evaluated exactly as the reference does, found with a uniform grid.  With
+// js, the pairs of the test faces reflected by J = diag(js) and the source
+// faces: |J c_m - c_n|^2, which has the bits of the reference's
+// |c_m - J c_n|^2 since J only flips signs.
+void near_pairs(const std::vector<double> &cp, int64_t F, double avelen, std::vector<int32_t> &out,
+                const double *js) {
   const double thr = avelen * 3;
   const double thr2 = thr * thr;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -181,16 +184,23 @@ static void near_pairs(const std::vector<double> &cp, int64_t F, double avelen,
   std::vector<int32_t> nb;
   for (int64_t m = 0; m < F; m++) {
     nb.clear();
+    double tc[3];
+    int64_t tcell[3];
+    for (int c = 0; c < 3; c++) {
+      tc[c] = js ? js[c] * cp[3 * m + c] : cp[3 * m + c];
+      // a reflected centroid may lie outside the grid: cells beyond it are skipped below
+      const double u = (tc[c] - lo[c]) / cell;
+      tcell[c] = js ? (u < -2.0 ? -3 : std::min(dim[c] + 1, (int64_t)std::floor(u))) : cell_of[3 * m + c];
+    }
     for (int64_t a = -1; a <= 1; a++)
       for (int64_t b = -1; b <= 1; b++)
         for (int64_t c = -1; c <= 1; c++) {
-          int64_t i = cell_of[3 * m] + a, j = cell_of[3 * m + 1] + b, k = cell_of[3 * m + 2] + c;
+          int64_t i = tcell[0] + a, j = tcell[1] + b, k = tcell[2] + c;
           if (i < 0 || j < 0 || k < 0 || i >= dim[0] || j >= dim[1] || k >= dim[2]) continue;
           auto it = grid.find(key(i, j, k));
           if (it == grid.end()) continue;
           for (int32_t n : it->second) {
-            double d0 = cp[3 * m] - cp[3 * n], d1 = cp[3 * m + 1] - cp[3 * n + 1],
-                   d2 = cp[3 * m + 2] - cp[3 * n + 2];
+            double d0 = tc[0] - cp[3 * n], d1 = tc[1] - cp[3 * n + 1], d2 = tc[2] - cp[3 * n + 2];
             double ds = d0 * d0 + d1 * d1 + d2 * d2;
             if (ds < thr2) nb.push_back(n);
           }
@@ -424,21 +434,22 @@ int mom_srcvec_pw(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, doub
   return 0;
 }
 
-int mom_cs_prtsrf(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega,
-                  double ri_re, double ri_im, const double *x, double theta, double phi, double psi,
-                  int32_t nq, const double *qw, const double *qnodes, double *csca, double *cabs) {
+static int cs_impl(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                   double ri_im, const double *x, double theta, double phi, double psi, int32_t nq, const double *qw,
+                   const double *qnodes, double *csca, double *cabs, const Group &G) {
   if (!ctx || !mesh || !x || !csca || !cabs) return fail(MOM_ERR_ARG, "NULL argument");
   if (!(ri_re > 0)) return fail(MOM_ERR_ARG, "Re(ri) must be > 0");
+  if (nedges_total < 1 || mesh->max_parent >= nedges_total) return fail(MOM_ERR_ARG, "nedges_total too small");
   QuadRule rule;
   int rc = make_rule(nq, qw, qnodes, &rule);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
   const int64_t N2 = 2 * nedges_total;
   DevBuf dx, dpart;
-  if ((rc = dx.alloc(sizeof(double2) * N2)) || (rc = dpart.alloc(sizeof(double) * 2 * mesh->F))) return rc;
-  if ((rc = copy_h2d(ctx, dx.ptr, x, sizeof(double2) * N2))) return rc;
+  if ((rc = dx.alloc(sizeof(double2) * N2 * G.n)) || (rc = dpart.alloc(sizeof(double) * 2 * mesh->F))) return rc;
+  if ((rc = copy_h2d(ctx, dx.ptr, x, sizeof(double2) * N2 * G.n))) return rc;
   if ((rc = cs_partials(ctx, mesh, rule, omega, cd(ri_re, ri_im), dx.as<double2>(), nedges_total, theta,
-                        phi, psi, dpart.as<double>())))
+                        phi, psi, dpart.as<double>(), G, N2)))
     return rc;
   std::vector<double> part(2 * mesh->F);
   if ((rc = copy_d2h(ctx, part.data(), dpart.ptr, sizeof(double) * 2 * mesh->F))) return rc;
@@ -454,9 +465,47 @@ int mom_cs_prtsrf(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, doub
   return 0;
 }
 
-int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
-                   double ri_im, const double *x, double r, int64_t ndir, const double *theta, const double *phi,
-                   int32_t nq, const double *qw, const double *qnodes, double *out) {
+int mom_cs_prtsrf(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                  double ri_im, const double *x, double theta, double phi, double psi, int32_t nq, const double *qw,
+                  const double *qnodes, double *csca, double *cabs) {
+  return cs_impl(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, theta, phi, psi, nq, qw, qnodes, csca, cabs,
+                 identity_group());
+}
+
+int mom_cs_prtsrf_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                       double ri_im, int32_t ngen, const int32_t *planes, const double *x, double theta, double phi,
+                       double psi, int32_t nq, const double *qw, const double *qnodes, double *csca, double *cabs) {
+  Group G;
+  MOM_TRY(mirror_group(ngen, planes, &G));
+  return cs_impl(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, theta, phi, psi, nq, qw, qnodes, csca, cabs, G);
+}
+
+int mom_srcvec_pw_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                       double ri_im, int32_t ngen, const int32_t *planes, int64_t nsrc, const double *angles,
+                       int32_t nq, const double *qw, const double *qnodes, double *q, int64_t ldq) {
+  if (!ctx || !mesh || !angles || !q) return fail(MOM_ERR_ARG, "NULL argument");
+  if (nsrc < 1 || nedges_total < 1 || ldq < 2 * nedges_total || mesh->max_parent >= nedges_total)
+    return fail(MOM_ERR_ARG, "bad sizes");
+  Group G;
+  MOM_TRY(mirror_group(ngen, planes, &G));
+  QuadRule rule;
+  int rc = make_rule(nq, qw, qnodes, &rule);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const int64_t ncol = nsrc * G.n;
+  DevBuf dq;
+  if ((rc = dq.alloc(sizeof(double2) * ldq * ncol))) return rc;
+  MOM_HIP(hipMemsetAsync(dq.ptr, 0, sizeof(double2) * ldq * ncol, ctx->stream));
+  if ((rc = srcvec_pw(ctx, mesh, rule, omega, cd(ri_re, ri_im), nsrc, angles, nedges_total, dq.as<double2>(), ldq,
+                      G)))
+    return rc;
+  return copy_d2h(ctx, q, dq.ptr, sizeof(double2) * ldq * ncol);
+}
+
+static int far_fields_impl(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                           double ri_im, const double *x, double r, int64_t ndir, const double *theta,
+                           const double *phi, int32_t nq, const double *qw, const double *qnodes, double *out,
+                           const Group &G) {
   if (!ctx || !mesh || !x || !theta || !phi || !out) return fail(MOM_ERR_ARG, "NULL argument");
   if (!(omega > 0) || !(r > 0)) return fail(MOM_ERR_ARG, "omega and r must be > 0");
   if (nedges_total < 1 || mesh->max_parent >= nedges_total) return fail(MOM_ERR_ARG, "nedges_total too small");
@@ -466,11 +515,13 @@ int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, dou
   (void)hipSetDevice(ctx->device);
   const int64_t N2 = 2 * nedges_total;
   DevBuf dx;
-  if ((rc = dx.alloc(sizeof(double2) * N2)) || (rc = copy_h2d(ctx, dx.ptr, x, sizeof(double2) * N2))) return rc;
+  if ((rc = dx.alloc(sizeof(double2) * N2 * G.n)) || (rc = copy_h2d(ctx, dx.ptr, x, sizeof(double2) * N2 * G.n)))
+    return rc;
   const cd ri(ri_re, ri_im), I(0.0, 1.0);
   const cd k = ri * omega / kC0;
   std::vector<cd> v;
-  if ((rc = far_field_integrals(ctx, mesh, rule, k, dx.as<double2>(), nedges_total, ndir, theta, phi, v))) return rc;
+  if ((rc = far_field_integrals(ctx, mesh, rule, k, dx.as<double2>(), nedges_total, ndir, theta, phi, v, G, N2)))
+    return rc;
   // ffields.f90:107-113
   const cd prefix = I * k * std::exp(I * k * r) / (4.0 * MOM_PI * r);
   for (int64_t d = 0; d < ndir; d++) {
@@ -485,9 +536,16 @@ int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, dou
   return 0;
 }
 
-int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
-                double ri_im, const double *x, int32_t ntheta, int32_t nphi, int32_t filter, const double *fdir,
-                int32_t nq, const double *qw, const double *qnodes, double *rcs) {
+int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                   double ri_im, const double *x, double r, int64_t ndir, const double *theta, const double *phi,
+                   int32_t nq, const double *qw, const double *qnodes, double *out) {
+  return far_fields_impl(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, r, ndir, theta, phi, nq, qw, qnodes, out,
+                         identity_group());
+}
+
+static int rcs_impl(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                    double ri_im, const double *x, int32_t ntheta, int32_t nphi, int32_t filter, const double *fdir,
+                    int32_t nq, const double *qw, const double *qnodes, double *rcs, const Group &G) {
   if (!ctx || !mesh || !x || !rcs || (filter && !fdir)) return fail(MOM_ERR_ARG, "NULL argument");
   if (ntheta < 2 || nphi < 2) return fail(MOM_ERR_ARG, "ntheta and nphi must be >= 2");
   const int64_t nd = (int64_t)ntheta * nphi;
@@ -498,8 +556,8 @@ int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double
       th[(int64_t)m * ntheta + n] = MOM_PI * (double)n / (double)(ntheta - 1);
       ph[(int64_t)m * ntheta + n] = 2.0 * MOM_PI * (double)m / (double)(nphi - 1);
     }
-  int rc = mom_far_fields(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, 1.0, nd, th.data(), ph.data(), nq, qw,
-                          qnodes, f.data());
+  int rc = far_fields_impl(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, 1.0, nd, th.data(), ph.data(), nq, qw,
+                           qnodes, f.data(), G);
   if (rc) return rc;
   for (int64_t d = 0; d < nd; d++) {
     const cd et(f[8 * d], f[8 * d + 1]), ep(f[8 * d + 2], f[8 * d + 3]);
@@ -516,6 +574,23 @@ int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double
     }
   }
   return 0;
+}
+
+int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                double ri_im, const double *x, int32_t ntheta, int32_t nphi, int32_t filter, const double *fdir,
+                int32_t nq, const double *qw, const double *qnodes, double *rcs) {
+  return rcs_impl(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, ntheta, nphi, filter, fdir, nq, qw, qnodes, rcs,
+                  identity_group());
+}
+
+int mom_rcs_ext_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                     double ri_im, int32_t ngen, const int32_t *planes, const double *x, int32_t ntheta, int32_t nphi,
+                     int32_t filter, const double *fdir, int32_t nq, const double *qw, const double *qnodes,
+                     double *rcs) {
+  Group G;
+  MOM_TRY(mirror_group(ngen, planes, &G));
+  return rcs_impl(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, ntheta, nphi, filter, fdir, nq, qw, qnodes, rcs,
+                  G);
 }
 
 }  // extern "C"

@@ -54,6 +54,8 @@ struct FarArgs {
   double kr, ki;      // wavenumber
   double thr2;        // near threshold^2
   double2 c1, c2;     // D and H prefactors
+  double js[3];       // test faces reflected by diag(js) (all 1: no image)
+  int image;          // js is a mirror image (the self-pair rule does not apply)
   FillTarget out;
 };
 
@@ -136,9 +138,43 @@ __device__ bool near_pair(const double *cpm, const double *cpn, double thr2) {
 
 // Writes the 3x3 edge block of one face pair.  D, H, K are complex values
 // of the pair's contribution to computeD/H/K entries.
+// Adds one edge pair's D+H (ar, ai) and K (kr, ki) to every representation
+// matrix: e (J,J), e detj Zsi (M,M), e detj K (M,J), -e K (J,M), the
+// blocks of sysmat.f90:66-98 with the element's field action e, at the
+// rows and columns the representation keeps.
+template <bool ATOMIC>
+__device__ __forceinline__ void store_reps(const FillTarget &o, int64_t gi, int64_t gj, double ar, double ai,
+                                           double kr, double ki) {
+  const int64_t N = o.Noff;
+  const double mr = o.zsi.x * ar - o.zsi.y * ai, mi = o.zsi.x * ai + o.zsi.y * ar;
+  for (int r = 0; r < o.nrep; r++) {
+    const int *mp = o.rep.map[r];
+    const int a = mp[gi], b = mp[gj], am = mp[gi + N], bm = mp[gj + N];
+    const double e = o.rep.e[r], ed = e * o.detj;
+    const int64_t ld = o.rep.ld[r];
+    double2 *Z = o.rep.Z[r];
+    auto add = [&](int i, int j, double vr, double vi) {
+      if (i < 0 || j < 0) return;
+      double2 *z = Z + i + (int64_t)j * ld;
+      if (ATOMIC) {
+        cadd_atomic(z, vr, vi);
+      } else {
+        const double2 v = *z;
+        *z = make_double2(v.x + vr, v.y + vi);
+      }
+    };
+    add(a, b, e * ar, e * ai);
+    add(am, bm, ed * mr, ed * mi);
+    add(am, b, ed * kr, ed * ki);
+    add(a, bm, -e * kr, -e * ki);
+  }
+}
+
 __device__ __forceinline__ void emit(const FillTarget &o, int gi, int gj, double Dr, double Di,
                                      double Hr, double Hi, double Kr, double Ki) {
-  if (o.mode == 0) {
+  if (o.nrep > 0) {
+    store_reps<true>(o, gi, gj, Dr + Hr, Di + Hi, Kr, Ki);
+  } else if (o.mode == 0) {
     const int64_t N = o.Noff, ld = o.ld;
     double Sr = Dr + Hr, Si = Di + Hi;
     cadd_atomic(o.Z + gi + (int64_t)gj * ld, Sr, Si);
@@ -332,7 +368,7 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
     const int l = idx % 64, jc = idx / 64;
     const int m = m0 + l;
     double v = 0.0;
-    if (m < F) v = a.qp[(int64_t)m * 3 * Q + jc] - a.cp[3 * m + jc % 3];
+    if (m < F) v = a.js[jc % 3] * (a.qp[(int64_t)m * 3 * Q + jc] - a.cp[3 * m + jc % 3]);
     tpts[jc * 64 + l] = v;
   }
   for (int idx = threadIdx.x; idx < kSrcPerBlock * Q * 3; idx += blockDim.x) {
@@ -347,7 +383,8 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
   const int m = m0 + lane;
   const bool valid = m < F;
   const int mm = valid ? m : 0;
-  const double cmx = a.cp[3 * mm], cmy = a.cp[3 * mm + 1], cmz = a.cp[3 * mm + 2];
+  const double cmx = a.js[0] * a.cp[3 * mm], cmy = a.js[1] * a.cp[3 * mm + 1], cmz = a.js[2] * a.cp[3 * mm + 2];
+  const double cm3[3] = {cmx, cmy, cmz};
   const int64_t plane = (int64_t)pa.S * W;
 
   for (int t = 0; t < kSrcPerWave; t++) {
@@ -361,7 +398,7 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
     // acc[2..7] = S_rho' (source), acc[8..13] = S_rho (test),
     // acc[18..23] = M_rho', acc[24..29] = M_rho, acc[30..35] = -M_x.
     const double dx = cmx - cpn[0], dy = cmy - cpn[1], dz = cmz - cpn[2];
-    const bool near = near_pair(a.cp + 3 * mm, cpn, a.thr2);
+    const bool near = near_pair(cm3, cpn, a.thr2);
     double acc[36];
     for (int k = 0; k < 36; k++) acc[k] = 0.0;
     if (near)
@@ -371,7 +408,7 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
     if (!valid) continue;
 
     const double AA = a.area[mm] * a.area[n];
-    const bool self = (mm == n);
+    const bool self = (mm == n) && !a.image;
     const double D3[3] = {dx, dy, dz};
     const int nl = n - pa.n0;
 #pragma unroll 1
@@ -383,7 +420,8 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
       for (int p = 0; p < 3; p++) {
         const int sp = 3 * mm + p;
         const double coefm = a.side.coef[sp], divm = a.side.div[sp];
-        const double vm[3] = {a.side.v[3 * sp] - cmx, a.side.v[3 * sp + 1] - cmy, a.side.v[3 * sp + 2] - cmz};
+        const double vm[3] = {a.js[0] * a.side.v[3 * sp] - cmx, a.js[1] * a.side.v[3 * sp + 1] - cmy,
+                              a.js[2] * a.side.v[3 * sp + 2] - cmz};
         // D: Srr' - nu_n.Srho - nu_m.Srho' + (nu_m.nu_n) S0
         const double vmn = vm[0] * vn[0] + vm[1] * vn[1] + vm[2] * vn[2];
         double sr = acc[14] + vmn * acc[0], si = acc[15] + vmn * acc[1];
@@ -461,7 +499,9 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
 // output: the four PMCHWT blocks (mode 0) or the single operator.
 __device__ __forceinline__ void store_entry(const FillTarget &o, int64_t gi, int64_t gj, double ar, double ai,
                                                  double kr, double ki) {
-  if (o.mode == 0) {
+  if (o.nrep > 0) {
+    store_reps<false>(o, gi, gj, ar, ai, kr, ki);
+  } else if (o.mode == 0) {
     const int64_t N = o.Noff, ld = o.ld;
     double2 *z = o.Z + gi + gj * ld;
     double2 v = *z;
@@ -493,6 +533,7 @@ struct AssembleArgs {
   const int *slab_edges;  // local source edges of this slab
   const int *eside;       // [E][2] face-side codes 3f+s of each local edge, -1 if none
   const int *col_index;   // local edge -> output column
+  double ktr;             // factor of K in the transposed blocks: det J of the image (1 without)
   FillTarget out;
 };
 
@@ -592,7 +633,7 @@ __global__ __launch_bounds__(256) void assemble_transposed_kernel(AssembleArgs g
         const int cc = c0 + c;
         if (cc >= nc) break;
         const double2 v = sa[rr][c], k = sk[rr][c];
-        store_entry(g.out, gi, g.col_index[cols[cc]], v.x, v.y, k.x, k.y);
+        store_entry(g.out, gi, g.col_index[cols[cc]], v.x, v.y, g.ktr * k.x, g.ktr * k.y);
       }
     }
   }
@@ -727,6 +768,8 @@ struct NearArgs {
   double2 kk4;  // 1/(4pi) (real, kept complex for symmetry)
   double2 kk8;  // k^2/(8pi)
   double2 c1, c2;
+  double js[3];  // test faces reflected by diag(js)
+  int image;
   FillTarget out;
 };
 
@@ -734,7 +777,7 @@ __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= a.npairs) return;
   const int m = a.pairs[2 * id], n = a.pairs[2 * id + 1];
-  const bool self = (m == n);
+  const bool self = (m == n) && !a.image;
   NearFace f;
   for (int v = 0; v < 3; v++) {
     int node = a.fnodes[3 * n + v];
@@ -755,7 +798,7 @@ __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
   double coefm[3], vm[3][3];
   for (int p = 0; p < 3; p++) {
     coefm[p] = a.side.coef[3 * m + p];
-    for (int k = 0; k < 3; k++) vm[p][k] = a.side.v[3 * (3 * m + p) + k];
+    for (int k = 0; k < 3; k++) vm[p][k] = a.js[k] * a.side.v[3 * (3 * m + p) + k];
   }
   double accD[3][3][2], accK[3][3][2], accH[3][2];
   for (int p = 0; p < 3; p++)
@@ -763,7 +806,8 @@ __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
   for (int q = 0; q < 3; q++) accH[q][0] = accH[q][1] = 0.0;
 
   for (int i = 0; i < a.Q; i++) {
-    const double *r = a.qp + ((int64_t)m * a.Q + i) * 3;
+    const double *rq = a.qp + ((int64_t)m * a.Q + i) * 3;
+    const double r[3] = {a.js[0] * rq[0], a.js[1] * rq[1], a.js[2] * rq[2]};
     PointInts P;
     d_point_ints(r, f, P);
     // intK31 / intK3m1 (int.f90:330-374), shared by the three edges
@@ -866,7 +910,7 @@ static FaceSide sides(const mom_mesh *m, bool global) {
 }
 
 int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
-                const FillTarget &t) {
+                const FillTarget &t, const Image *img) {
   int rc = compute_quad_points(ctx, m, rule);
   if (rc) return rc;
   const cd I(0.0, 1.0);
@@ -891,6 +935,8 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   a.thr2 = thr * thr;
   a.c1 = make_double2(c1.real(), c1.imag());
   a.c2 = make_double2(c2.real(), c2.imag());
+  for (int c = 0; c < 3; c++) a.js[c] = img ? img->js[c] : 1.0;
+  a.image = img ? 1 : 0;
   a.out = t;
   // far/smooth pairs, slab by slab of source faces, through the face-side
   // planes (no atomics on the matrix)
@@ -936,6 +982,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   ga.nrows = nrows;
   ga.eside = m->eside.as<int>();
   ga.col_index = dcol.as<int>();
+  ga.ktr = img ? img->detj : 1.0;
   ga.out = t;
   const int nslab = (int)((m->F + S - 1) / S);
   for (int k = 0; k < nslab; k++) {
@@ -972,12 +1019,26 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   // the row/column maps must outlive the queued kernels
   MOM_HIP(hipStreamSynchronize(ctx->stream));
 
-  if (m->nnear > 0) {
+  // near pairs of the image: the reflected test faces against the sources
+  DevBuf inear;
+  int64_t nnear = m->nnear;
+  const int *pairs = m->near.as<int>();
+  if (img) {
+    std::vector<int32_t> np;
+    near_pairs(m->h_fcp, m->F, m->avelen, np, img->js);
+    nnear = (int64_t)np.size() / 2;
+    if (nnear > 0) {
+      if ((rc = inear.alloc(sizeof(int32_t) * np.size()))) return rc;
+      MOM_TRY(copy_h2d(ctx, inear.ptr, np.data(), sizeof(int32_t) * np.size()));
+    }
+    pairs = inear.as<int>();
+  }
+  if (nnear > 0) {
     NearArgs b;
     b.Q = rule.Q;
     for (int i = 0; i < kMaxQ; i++) b.w[i] = a.w[i];
-    b.pairs = m->near.as<int>();
-    b.npairs = m->nnear;
+    b.pairs = pairs;
+    b.npairs = nnear;
     b.qp = a.qp;
     b.p = m->p.as<double>();
     b.fn = m->fn.as<double>();
@@ -992,50 +1053,86 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     b.kk8 = make_double2(kk8.real(), kk8.imag());
     b.c1 = a.c1;
     b.c2 = a.c2;
+    for (int c = 0; c < 3; c++) b.js[c] = a.js[c];
+    b.image = a.image;
     b.out = t;
-    hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((m->nnear + 127) / 128)), dim3(128), 0,
+    hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((nnear + 127) / 128)), dim3(128), 0,
                        ctx->stream, b);
     MOM_HIP(hipGetLastError());
+    // the image's pair list must outlive the kernel
+    if (img) MOM_HIP(hipStreamSynchronize(ctx->stream));
   }
   return 0;
 }
 
 // ---------------------------------------------------------------------------
 // Plane-wave right-hand sides (source.f90:136-247 with pw_fields 84-101):
-// one thread per (face, source); each face adds <f_p, E_inc> and
-// <f_p, H_inc> of its three sides.
+// one thread per (face, source, representation); each face adds
+// <f_p, E_inc> and <f_p, H_inc> of its three sides.  For a symmetry group
+// the incident fields are projected on the representation first
+// (src_field_frags, source.f90:249-315): E_r(r) = (1/n) sum_g ef[g][r]
+// J_g E(J_g r) and H_r likewise with ef det J_g.
 struct PwSrc {
   double dir[3], pol[3];
 };
 
+// the plane wave's E and H = k/(omega mu0) dir x E at r (pw_fields)
+__device__ __forceinline__ void pw_eh(const PwSrc &S, const double *r, double kr, double ki, double2 hfac,
+                                      double E[3][2], double H[3][2]) {
+  double kd = S.dir[0] * r[0] + S.dir[1] * r[1] + S.dir[2] * r[2];
+  // exp(i k d.r)
+  double ex = exp(-ki * kd), sn, cs;
+  sincos(kr * kd, &sn, &cs);
+  double er = ex * cs, ei = ex * sn;
+  for (int c = 0; c < 3; c++) {
+    E[c][0] = S.pol[c] * er;
+    E[c][1] = S.pol[c] * ei;
+  }
+  for (int c = 0; c < 3; c++) {
+    int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+    double xr = S.dir[c1] * E[c2][0] - S.dir[c2] * E[c1][0];
+    double xi = S.dir[c1] * E[c2][1] - S.dir[c2] * E[c1][1];
+    H[c][0] = hfac.x * xr - hfac.y * xi;
+    H[c][1] = hfac.x * xi + hfac.y * xr;
+  }
+}
+
 __global__ void srcvec_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
                               const double *__restrict__ area, FaceSide side, int nsrc,
                               const PwSrc *__restrict__ src, double kr, double ki, double2 hfac,
-                              int64_t Nt, double2 *q, int64_t ldq) {
+                              int64_t Nt, double2 *q, int64_t ldq, Group G) {
   int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (id >= (int64_t)F * nsrc) return;
-  int f = (int)(id % F), s = (int)(id / F);
+  if (id >= (int64_t)F * nsrc * G.n) return;
+  const int f = (int)(id % F);
+  const int64_t sr = id / F;
+  const int rep = (int)(sr % G.n), s = (int)(sr / G.n);
   const PwSrc S = src[s];
   double Ie[3][2] = {{0, 0}, {0, 0}, {0, 0}}, Ih[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   for (int t = 0; t < Q; t++) {
     const double *r = qp + ((int64_t)f * Q + t) * 3;
-    double kd = S.dir[0] * r[0] + S.dir[1] * r[1] + S.dir[2] * r[2];
-    // exp(i k d.r)
-    double ex = exp(-ki * kd), sn, cs;
-    sincos(kr * kd, &sn, &cs);
-    double er = ex * cs, ei = ex * sn;
     double E[3][2], H[3][2];
-    for (int c = 0; c < 3; c++) {
-      E[c][0] = S.pol[c] * er;
-      E[c][1] = S.pol[c] * ei;
-    }
-    // H = k/(omega mu0) dir x E
-    for (int c = 0; c < 3; c++) {
-      int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-      double xr = S.dir[c1] * E[c2][0] - S.dir[c2] * E[c1][0];
-      double xi = S.dir[c1] * E[c2][1] - S.dir[c2] * E[c1][1];
-      H[c][0] = hfac.x * xr - hfac.y * xi;
-      H[c][1] = hfac.x * xi + hfac.y * xr;
+    if (G.n == 1) {
+      pw_eh(S, r, kr, ki, hfac, E, H);
+    } else {
+      for (int c = 0; c < 3; c++) E[c][0] = E[c][1] = H[c][0] = H[c][1] = 0.0;
+      for (int g = 0; g < G.n; g++) {
+        const double pt[3] = {G.js[g][0] * r[0], G.js[g][1] * r[1], G.js[g][2] * r[2]};
+        double Eg[3][2], Hg[3][2];
+        pw_eh(S, pt, kr, ki, hfac, Eg, Hg);
+        const double e = G.ef[g][rep], eh = e * G.detj[g];
+        for (int c = 0; c < 3; c++) {
+          E[c][0] += G.js[g][c] * Eg[c][0] * e;
+          E[c][1] += G.js[g][c] * Eg[c][1] * e;
+          H[c][0] += G.js[g][c] * Hg[c][0] * eh;
+          H[c][1] += G.js[g][c] * Hg[c][1] * eh;
+        }
+      }
+      for (int c = 0; c < 3; c++) {
+        E[c][0] /= G.n;
+        E[c][1] /= G.n;
+        H[c][0] /= G.n;
+        H[c][1] /= G.n;
+      }
     }
     for (int p = 0; p < 3; p++) {
       const double cf = side.coef[3 * f + p];
@@ -1055,7 +1152,7 @@ __global__ void srcvec_kernel(int F, int Q, QuadRule rule, const double *__restr
     }
   }
   const double Am = area[f];
-  double2 *qs = q + (int64_t)s * ldq;
+  double2 *qs = q + sr * ldq;
   for (int p = 0; p < 3; p++) {
     const int g = side.g[3 * f + p];
     cadd_atomic(qs + g, Ie[p][0] * Am, Ie[p][1] * Am);
@@ -1064,7 +1161,7 @@ __global__ void srcvec_kernel(int F, int Q, QuadRule rule, const double *__restr
 }
 
 int srcvec_pw(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
-              int64_t nsrc, const double *angles, int64_t Nt, double2 *q, int64_t ldq) {
+              int64_t nsrc, const double *angles, int64_t Nt, double2 *q, int64_t ldq, const Group &G) {
   int rc = compute_quad_points(ctx, m, rule);
   if (rc) return rc;
   std::vector<PwSrc> hs(nsrc);
@@ -1083,62 +1180,60 @@ int srcvec_pw(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omeg
   MOM_TRY(copy_h2d(ctx, dsrc.ptr, hs.data(), sizeof(PwSrc) * nsrc));
   const cd k = ri * omega / kC0;
   const cd hfac = k / (omega * mu0());
-  int64_t total = m->F * nsrc;
+  int64_t total = m->F * nsrc * G.n;
   hipLaunchKernelGGL(srcvec_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, ctx->stream,
                      (int)m->F, rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(),
                      sides(m, true), (int)nsrc, dsrc.as<PwSrc>(), k.real(), k.imag(),
-                     make_double2(hfac.real(), hfac.imag()), Nt, q, ldq);
+                     make_double2(hfac.real(), hfac.imag()), Nt, q, ldq, G);
   MOM_HIP(hipGetLastError());
   MOM_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
 // ---------------------------------------------------------------------------
-// cs_prtsrf (cs.f90:14-104), identity group: per face, the quadrature sums
-// of the total (absorption) and scattered Poynting flux; the host adds the
-// per-face values in face order as the reference does.
+// cs_prtsrf (cs.f90:14-104): per face, the quadrature sums of the total
+// (absorption) and scattered Poynting flux; the host adds the per-face
+// values in face order as the reference does.  With a symmetry group every
+// image g of the face contributes: normal J n, tangential fields
+// sum_r ef[g][r] J (n x f) x_r (with det J for H) from the representations'
+// solutions, incident field at J r.
 __global__ void cs_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
                           const double *__restrict__ area, const double *__restrict__ fnrm,
                           FaceSide side, const double2 *__restrict__ x, int64_t Nt, PwSrc S,
-                          double kr, double ki, double2 hfac, double *out) {
+                          double kr, double ki, double2 hfac, double *out, Group G, int64_t ldx) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
-  const double *nor = fnrm + 3 * f;
+  const double *nor0 = fnrm + 3 * f;
   double int1 = 0, int2 = 0;
   for (int t = 0; t < Q; t++) {
-    const double *r = qp + ((int64_t)f * Q + t) * 3;
+    const double *r0 = qp + ((int64_t)f * Q + t) * 3;
+   for (int gi = 0; gi < G.n; gi++) {
+    const double *js = G.js[gi];
+    const double nor[3] = {js[0] * nor0[0], js[1] * nor0[1], js[2] * nor0[2]};
+    const double r[3] = {js[0] * r0[0], js[1] * r0[1], js[2] * r0[2]};
     double Et[3][2] = {{0, 0}, {0, 0}, {0, 0}}, Ht[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-    for (int p = 0; p < 3; p++) {
-      const double cf = side.coef[3 * f + p];
-      const double *v = side.v + 3 * (3 * f + p);
-      double fv[3] = {cf * (r[0] - v[0]), cf * (r[1] - v[1]), cf * (r[2] - v[2])};
-      double fn[3] = {nor[1] * fv[2] - nor[2] * fv[1], nor[2] * fv[0] - nor[0] * fv[2],
-                      nor[0] * fv[1] - nor[1] * fv[0]};
-      const int g = side.g[3 * f + p];
-      const double2 xj = x[g], xm = x[g + Nt];
-      for (int c = 0; c < 3; c++) {
-        Ht[c][0] -= xj.x * fn[c];
-        Ht[c][1] -= xj.y * fn[c];
-        Et[c][0] += xm.x * fn[c];
-        Et[c][1] += xm.y * fn[c];
+    for (int rep = 0; rep < G.n; rep++) {
+      const double e = G.ef[gi][rep], eh = e * G.detj[gi];
+      const double2 *xr = x + rep * ldx;
+      for (int p = 0; p < 3; p++) {
+        const double cf = side.coef[3 * f + p];
+        const double *v = side.v + 3 * (3 * f + p);
+        double fv[3] = {cf * (r0[0] - v[0]), cf * (r0[1] - v[1]), cf * (r0[2] - v[2])};
+        double fn[3] = {nor0[1] * fv[2] - nor0[2] * fv[1], nor0[2] * fv[0] - nor0[0] * fv[2],
+                        nor0[0] * fv[1] - nor0[1] * fv[0]};
+        for (int c = 0; c < 3; c++) fn[c] *= js[c];
+        const int g = side.g[3 * f + p];
+        const double2 xj = xr[g], xm = xr[g + Nt];
+        for (int c = 0; c < 3; c++) {
+          Ht[c][0] -= xj.x * fn[c] * eh;
+          Ht[c][1] -= xj.y * fn[c] * eh;
+          Et[c][0] += xm.x * fn[c] * e;
+          Et[c][1] += xm.y * fn[c] * e;
+        }
       }
     }
-    double kd = S.dir[0] * r[0] + S.dir[1] * r[1] + S.dir[2] * r[2];
-    double ex = exp(-ki * kd), sn, cs;
-    sincos(kr * kd, &sn, &cs);
-    double er = ex * cs, ei = ex * sn;
     double Ei[3][2], Hi[3][2];
-    for (int c = 0; c < 3; c++) {
-      Ei[c][0] = S.pol[c] * er;
-      Ei[c][1] = S.pol[c] * ei;
-    }
-    for (int c = 0; c < 3; c++) {
-      int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-      double xr = S.dir[c1] * Ei[c2][0] - S.dir[c2] * Ei[c1][0];
-      double xi = S.dir[c1] * Ei[c2][1] - S.dir[c2] * Ei[c1][1];
-      Hi[c][0] = hfac.x * xr - hfac.y * xi;
-      Hi[c][1] = hfac.x * xi + hfac.y * xr;
-    }
+    pw_eh(S, r, kr, ki, hfac, Ei, Hi);
     // tangential incident fields (n x F) x n = F - (n.F) n
     double nE[2] = {0, 0}, nH[2] = {0, 0};
     for (int c = 0; c < 3; c++) {
@@ -1167,13 +1262,15 @@ __global__ void cs_kernel(int F, int Q, QuadRule rule, const double *__restrict_
     }
     int1 += rule.w[t] * 0.5 * s1;
     int2 += rule.w[t] * 0.5 * s2;
+   }
   }
   out[2 * f] = int1 * area[f];
   out[2 * f + 1] = int2 * area[f];
 }
 
 int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
-                const double2 *x, int64_t Nt, double th, double ph, double ps, double *partial) {
+                const double2 *x, int64_t Nt, double th, double ph, double ps, double *partial, const Group &G,
+                int64_t ldx) {
   int rc = compute_quad_points(ctx, m, rule);
   if (rc) return rc;
   PwSrc S;
@@ -1188,7 +1285,7 @@ int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   hipLaunchKernelGGL(cs_kernel, dim3((unsigned)((m->F + 127) / 128)), dim3(128), 0, ctx->stream,
                      (int)m->F, rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(),
                      m->fn.as<double>(), sides(m, true), x, Nt, S, k.real(), k.imag(),
-                     make_double2(hfac.real(), hfac.imag()), partial);
+                     make_double2(hfac.real(), hfac.imag()), partial, G, ldx > 0 ? ldx : 2 * Nt);
   MOM_HIP(hipGetLastError());
   return 0;
 }
@@ -1203,32 +1300,44 @@ int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
 // phasor = exp(-i k dir . r_s).  The reference sums edge by edge within a
 // face; here the three edges of a face are summed at each point first, a
 // difference at the level of rounding.
+// With a symmetry group the points are those of every image g of the
+// surface (J r) with currents J f sum_r ef[g][r] x_r (det J for M), as
+// ffields.f90:67-94 sums them.
 __global__ void farfield_sources_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
                                         const double *__restrict__ area, FaceSide side,
                                         const double2 *__restrict__ x, int64_t Nt, double *__restrict__ pts,
-                                        double2 *__restrict__ cur, int64_t S) {
+                                        double2 *__restrict__ cur, int64_t S, Group G, int64_t ldx) {
   const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (s >= S) return;
-  const int f = (int)(s / Q), t = (int)(s - (int64_t)f * Q);
-  const double *r = qp + s * 3;
+  const int64_t FQ = (int64_t)F * Q;
+  const int gi = (int)(s / FQ);
+  const int64_t s0 = s - gi * FQ;
+  const int f = (int)(s0 / Q), t = (int)(s0 - (int64_t)f * Q);
+  const double *js = G.js[gi];
+  const double *r = qp + s0 * 3;
   double J[3][2] = {{0, 0}, {0, 0}, {0, 0}}, M[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-  for (int p = 0; p < 3; p++) {
-    const int g = side.g[3 * f + p];
-    if (g < 0) continue;
-    const double cf = side.coef[3 * f + p];
-    const double *v = side.v + 3 * (3 * f + p);
-    const double fv[3] = {cf * (r[0] - v[0]), cf * (r[1] - v[1]), cf * (r[2] - v[2])};
-    const double2 xj = x[g], xm = x[g + Nt];
-    for (int c = 0; c < 3; c++) {
-      J[c][0] += xj.x * fv[c];
-      J[c][1] += xj.y * fv[c];
-      M[c][0] += xm.x * fv[c];
-      M[c][1] += xm.y * fv[c];
+  for (int rep = 0; rep < G.n; rep++) {
+    const double e = G.ef[gi][rep], em = e * G.detj[gi];
+    const double2 *xr = x + rep * ldx;
+    for (int p = 0; p < 3; p++) {
+      const int g = side.g[3 * f + p];
+      if (g < 0) continue;
+      const double cf = side.coef[3 * f + p];
+      const double *v = side.v + 3 * (3 * f + p);
+      const double fv[3] = {js[0] * (cf * (r[0] - v[0])), js[1] * (cf * (r[1] - v[1])),
+                            js[2] * (cf * (r[2] - v[2]))};
+      const double2 xj = xr[g], xm = xr[g + Nt];
+      for (int c = 0; c < 3; c++) {
+        J[c][0] += xj.x * e * fv[c];
+        J[c][1] += xj.y * e * fv[c];
+        M[c][0] += xm.x * em * fv[c];
+        M[c][1] += xm.y * em * fv[c];
+      }
     }
   }
   const double wa = rule.w[t] * area[f];
   for (int c = 0; c < 3; c++) {
-    pts[3 * s + c] = r[c];
+    pts[3 * s + c] = js[c] * r[c];
     cur[c * S + s] = make_double2(wa * J[c][0], wa * J[c][1]);
     cur[(3 + c) * S + s] = make_double2(wa * M[c][0], wa * M[c][1]);
   }
@@ -1298,13 +1407,14 @@ __global__ void farfield_reduce_kernel(int ndir, int nchunk, const double2 *__re
 }
 
 int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, cd k, const double2 *x,
-                        int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out) {
+                        int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out,
+                        const Group &G, int64_t ldx) {
   if (!(std::fabs(k.real()) * m->extent < kCisMaxArg))
     return fail(MOM_ERR_ARG, "Re(k) times the mesh extent is beyond the far field's phase range (2^19 rad)");
   if (ndir < 1 || ndir > (1 << 24)) return fail(MOM_ERR_ARG, "number of directions out of range");
   int rc = compute_quad_points(ctx, m, rule);
   if (rc) return rc;
-  const int64_t S = m->F * rule.Q;
+  const int64_t S = m->F * rule.Q * G.n;
   const int nchunk = (int)((S + kFfChunk - 1) / kFfChunk);
   std::vector<double> dirs(8 * ndir);
   for (int64_t d = 0; d < ndir; d++) {
@@ -1327,7 +1437,7 @@ int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, c
   MOM_TRY(copy_h2d(ctx, ddir.ptr, dirs.data(), sizeof(double) * 8 * ndir));
   hipLaunchKernelGGL(farfield_sources_kernel, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, ctx->stream, (int)m->F,
                      rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(), sides(m, true), x, Nt,
-                     pts.as<double>(), cur.as<double2>(), S);
+                     pts.as<double>(), cur.as<double2>(), S, G, ldx > 0 ? ldx : 2 * Nt);
   hipLaunchKernelGGL(farfield_dir_kernel, dim3((unsigned)((ndir + 255) / 256), (unsigned)nchunk), dim3(256), 0,
                      ctx->stream, S, pts.as<double>(), cur.as<double2>(), (int)ndir, ddir.as<double>(), k.real(),
                      k.imag(), part.as<double2>());

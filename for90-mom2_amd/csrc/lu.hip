@@ -2636,16 +2636,15 @@ using namespace mom;
 
 extern "C" {
 
-int mom_system_create(mom_ctx *ctx, int64_t nedges_total, int32_t precision, mom_system **sys) {
+static int system_create(mom_ctx *ctx, int64_t nedges_total, int64_t order, int32_t precision, mom_system **sys) {
   if (!ctx || !sys) return fail(MOM_ERR_ARG, "NULL argument");
-  if (nedges_total < 1 || nedges_total > (1 << 29)) return fail(MOM_ERR_ARG, "nedges_total out of range");
   if (precision != MOM_PREC_FP64 && precision != MOM_PREC_MIXED)
     return fail(MOM_ERR_ARG, "precision must be MOM_PREC_FP64 or MOM_PREC_MIXED");
   (void)hipSetDevice(ctx->device);
   mom_system *S = new mom_system;
   S->ctx = ctx;
   S->N = nedges_total;
-  S->n = 2 * nedges_total;
+  S->n = order;
   S->precision = precision;
   S->precision_req = precision;
   const int64_t n = S->n;
@@ -2659,6 +2658,16 @@ int mom_system_create(mom_ctx *ctx, int64_t nedges_total, int32_t precision, mom
   }
   *sys = S;
   return 0;
+}
+
+int mom_system_create(mom_ctx *ctx, int64_t nedges_total, int32_t precision, mom_system **sys) {
+  if (nedges_total < 1 || nedges_total > (1 << 29)) return fail(MOM_ERR_ARG, "nedges_total out of range");
+  return system_create(ctx, nedges_total, 2 * nedges_total, precision, sys);
+}
+
+int mom_system_create_order(mom_ctx *ctx, int64_t order, int32_t precision, mom_system **sys) {
+  if (order < 2 || order > (1 << 30) || (order & 1)) return fail(MOM_ERR_ARG, "order must be even and in 2..2^30");
+  return system_create(ctx, 0, order, precision, sys);
 }
 
 int mom_system_destroy(mom_system *sys) {
@@ -2680,6 +2689,7 @@ int mom_system_profile(mom_system *sys, int32_t enable) {
 int mom_system_fill(mom_system *S, int32_t ndom, const mom_mesh *const *doms, const double *ri, double omega,
                     int32_t nq, const double *qw, const double *qnodes) {
   if (!S) return fail(MOM_ERR_ARG, "NULL system");
+  if (S->N == 0) return fail(MOM_ERR_STATE, "a system of a symmetry representation is filled by mom_system_fill_symm");
   (void)hipSetDevice(S->ctx->device);
   const double t0 = now_s();
   int rc = fill_system(S->ctx, ndom, doms, ri, S->N, omega, nq, qw, qnodes, S->Z64.as<double2>(), S->n);
@@ -2698,6 +2708,7 @@ int mom_system_fill_nls(mom_system *S, int32_t ndom, const mom_mesh *const *doms
                         int32_t nq, const double *qw, const double *qnodes, int64_t nsrc, const double *src_coef,
                         int64_t ldsc, double *src, int64_t ldsrc) {
   if (!S || !doms || !shri) return fail(MOM_ERR_ARG, "NULL argument");
+  if (S->N == 0) return fail(MOM_ERR_STATE, "the SH fill needs a system of the full problem");
   (void)hipSetDevice(S->ctx->device);
   const double t0 = now_s();
   int rc = fill_system_nls(S->ctx, ndom, doms, shri, S->N, omega, nq, qw, qnodes, S->Z64.as<double2>(), S->n, nsrc,
@@ -2710,6 +2721,42 @@ int mom_system_fill_nls(mom_system *S, int32_t ndom, const mom_mesh *const *doms
   S->precision = S->precision_req;
   S->fell_back = false;
   S->factored = false;
+  return 0;
+}
+
+int mom_system_fill_symm(int32_t nrep, mom_system *const *systems, int32_t ndom, const mom_mesh *const *doms,
+                         const double *ri, int64_t nedges_total, const int32_t *bnd, int32_t ngen,
+                         const int32_t *planes, double omega, int32_t nq, const double *qw, const double *qnodes) {
+  if (!systems || !doms || !ri || !bnd) return fail(MOM_ERR_ARG, "NULL argument");
+  if (nedges_total < 1 || ndom < 1) return fail(MOM_ERR_ARG, "bad sizes");
+  Group G;
+  MOM_TRY(mirror_group(ngen, planes, &G));
+  if (nrep != G.n) return fail(MOM_ERR_ARG, "one system per representation of the group is needed");
+  double2 *Z[kMaxReps];
+  int64_t ld[kMaxReps], order[kMaxReps];
+  for (int r = 0; r < nrep; r++) {
+    mom_system *S = systems[r];
+    if (!S) return fail(MOM_ERR_ARG, "NULL system");
+    if (S->ctx != systems[0]->ctx) return fail(MOM_ERR_ARG, "systems belong to different contexts");
+    if (S->N != 0) return fail(MOM_ERR_ARG, "representation systems are made by mom_system_create_order");
+    Z[r] = S->Z64.as<double2>();
+    ld[r] = S->n;
+    order[r] = S->n;
+  }
+  mom_ctx *ctx = systems[0]->ctx;
+  (void)hipSetDevice(ctx->device);
+  const double t0 = now_s();
+  MOM_TRY(fill_system_symm(ctx, ndom, doms, ri, nedges_total, bnd, G, omega, nq, qw, qnodes, Z, ld, order));
+  const double t = now_s() - t0;
+  for (int r = 0; r < nrep; r++) {
+    mom_system *S = systems[r];
+    S->t_fill = t;
+    S->filled = true;
+    S->znorm_one = 0;
+    S->precision = S->precision_req;
+    S->fell_back = false;
+    S->factored = false;
+  }
   return 0;
 }
 
@@ -2890,6 +2937,7 @@ int mom_system_solve_pw(mom_system *S, const mom_mesh *mesh, double omega, doubl
                         const double *angles, int32_t nq, const double *qw, const double *qnodes, double tol,
                         int32_t maxit, double *X, int64_t ldx, int32_t *iters, double *berr) {
   if (!S || !mesh || !X) return fail(MOM_ERR_ARG, "NULL argument");
+  if (S->N == 0) return fail(MOM_ERR_STATE, "a system of a symmetry representation takes reduced sources");
   if (ldx < S->n || nsrc < 1) return fail(MOM_ERR_ARG, "bad sizes");
   int rc = mom_srcvec_pw(S->ctx, mesh, S->N, omega, ri_re, ri_im, nsrc, angles, nq, qw, qnodes, X, ldx);
   if (rc) return rc;

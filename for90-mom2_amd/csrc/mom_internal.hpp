@@ -124,6 +124,19 @@ struct QuadRule {
 };
 int make_rule(int32_t nq, const double *qw, const double *qnodes, QuadRule *r);
 
+// Representation matrices of a mirror-symmetry group (sysmat.f90:56-101
+// with bc.f90:170-193): the image of a source face under the group element
+// adds e * (its block) to every representation's matrix, whose rows and
+// columns are the unknowns that the representation's boundary conditions
+// keep, numbered in order (map: unknown u in [0, 2 Noff) -> row, -1 removed).
+constexpr int kMaxReps = 8;
+struct RepOut {
+  double2 *Z[kMaxReps];
+  int64_t ld[kMaxReps];
+  const int *map[kMaxReps];
+  double e[kMaxReps];  // field action of the element in each representation
+};
+
 // Output of the fill kernels.
 struct FillTarget {
   double2 *Z = nullptr;  // column-major complex matrix
@@ -133,23 +146,69 @@ struct FillTarget {
   int local = 0;      // mode 0 with local indices (the domain's own 2E x 2E block, Noff = E)
   int dk = 0;         // the k-derivative of the operators (same prefactors c1, c2, zsi)
   double2 zsi;        // Zsi = ri^2 / eta0^2 (mode 0)
+  int nrep = 0;       // > 0: mode 0 into the representation matrices in `rep` instead of Z
+  double detj = 1.0;  // determinant of the element's Jacobian (nrep > 0)
+  RepOut rep;
 };
+
+// A group of mirror symmetries (symmetry.f90:26-241 restricted to the
+// planes x = 0, y = 0, z = 0 and their products): per element its diagonal
+// Jacobian, determinant, identifier and generator bits, and the field
+// action ef[element][representation] (the conjugated representation).
+struct Group {
+  int n = 1;
+  double js[kMaxReps][3];
+  double detj[kMaxReps];
+  double ef[kMaxReps][kMaxReps];
+  int id[kMaxReps];
+  int genbits[kMaxReps];
+};
+constexpr int kGidIdentity = 1, kGidMxp = 2, kGidMyp = 3, kGidMzp = 4;  // symmetry.f90:20-24
+constexpr int kBndNone = 1, kBndXplane = 2, kBndYplane = 3, kBndZplane = 4;  // mesh.f90:16-25
+Group identity_group();
+// read_symm's product of group_mp(plane) for planes[i] in 1..3 (x, y, z)
+int mirror_group(int ngen, const int32_t *planes, Group *g);
+
+// A mirror image J = diag(js) of the source faces (symmetry.f90:61-104):
+// the fill integrates the test faces reflected by J against the source
+// faces, which equals the reference's integral over the reflected source
+// (srcint.f90:155-224, 368-448) since J is its own inverse.
+struct Image {
+  double js[3];
+  double detj;
+};
+
+// api.hip: near face pairs (test, source) sorted, of the mesh with itself or,
+// with js, of the test faces reflected by diag(js) against the source faces
+void near_pairs(const std::vector<double> &cp, int64_t F, double avelen, std::vector<int32_t> &out,
+                const double *js = nullptr);
 
 // fill.hip
 int compute_quad_points(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule);
 int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
-                const FillTarget &t);
+                const FillTarget &t, const Image *img = nullptr);
+// q[:, s * G.n + r]: the plane wave s projected on representation r (identity group: one column per source)
 int srcvec_pw(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
-              int64_t nsrc, const double *angles, int64_t Nt, double2 *q, int64_t ldq);
+              int64_t nsrc, const double *angles, int64_t Nt, double2 *q, int64_t ldq,
+              const Group &G = identity_group());
 int fill_system(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *ri,
                 int64_t nedges_total, double omega, int32_t nq, const double *qw,
                 const double *qnodes, double2 *Z, int64_t ldz);
+// x: one column of 2 Nt per representation of G (leading dimension ldx)
 int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
-                const double2 *x, int64_t Nt, double th, double ph, double ps, double *partial);
+                const double2 *x, int64_t Nt, double th, double ph, double ps, double *partial,
+                const Group &G = identity_group(), int64_t ldx = 0);
 // (Nt, Np, Lt, Lp) of far_fields (ffields.f90:16-122) for ndir directions,
 // x the device solution [2 Nt], k the medium's wavenumber
 int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, cd k, const double2 *x,
-                        int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out);
+                        int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out,
+                        const Group &G = identity_group(), int64_t ldx = 0);
+// sysmat_pmchwt for a mirror group (symm.hip): every representation r's
+// reduced matrix Z[r] (order[r] = kept unknowns, padded to even by a unit
+// diagonal), bnd[N] the boundary class of each master edge
+int fill_system_symm(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *ri, int64_t N,
+                     const int32_t *bnd, const Group &G, double omega, int32_t nq, const double *qw,
+                     const double *qnodes, double2 *const *Z, const int64_t *ld, const int64_t *order);
 // y = M x (device, complex128, M nr x nc with leading dimension ld), in
 // column chunks spread over the device (adjoint.hip)
 int matvec(mom_ctx *ctx, const double2 *M, int64_t ld, int64_t nr, int64_t nc, const double2 *x, double2 *y);

@@ -277,7 +277,7 @@ static bool parse_longs(const std::string &line, std::vector<long> &out) {
 }
 
 static int read_gmsh(const char *path, std::vector<double> &p, std::vector<int32_t> &fnodes,
-                     std::vector<int32_t> &fids) {
+                     std::vector<int32_t> &fids, std::vector<int32_t> &lnodes, std::vector<int32_t> &lids) {
   std::ifstream in(path);
   if (!in) return fail(MOM_ERR_ARG, std::string("Could not open mesh file! ") + path);
   std::string line;
@@ -324,9 +324,19 @@ static int read_gmsh(const char *path, std::vector<double> &p, std::vector<int32
       const long ne = v[0];
       fnodes.clear();
       fids.clear();
+      lnodes.clear();
+      lids.clear();
       for (long n = 0; n < ne; n++) {
         if (!std::getline(in, line)) return fail(MOM_ERR_ARG, "mesh file ends inside $Elements");
         if (!parse_longs(line, v) || v.size() < 3) return fail(MOM_ERR_ARG, "bad element line in mesh file");
+        if (v[1] == 1) {
+          // line element: its first tag is the id that `sbnd` refers to (mesh.f90:692-697)
+          const long nt = v[2];
+          if (nt < 1 || (long)v.size() < 5 + nt) return fail(MOM_ERR_ARG, "bad line element in mesh file");
+          lids.push_back((int32_t)v[3]);
+          for (int k = 0; k < 2; k++) lnodes.push_back((int32_t)(v[3 + nt + k] - 1));
+          continue;
+        }
         if (v[1] != 2) continue;
         const long nt = v[2];
         if (nt < 1 || (long)v.size() < 6 + nt) return fail(MOM_ERR_ARG, "bad triangle line in mesh file");
@@ -341,6 +351,8 @@ static int read_gmsh(const char *path, std::vector<double> &p, std::vector<int32
   const long nn = (long)(p.size() / 3);
   for (int32_t i : fnodes)
     if (i < 0 || i >= nn) return fail(MOM_ERR_ARG, "face node index out of range");
+  for (int32_t i : lnodes)
+    if (i < 0 || i >= nn) return fail(MOM_ERR_ARG, "line node index out of range");
   return 0;
 }
 
@@ -351,6 +363,8 @@ using namespace mom;
 struct mom_problem {
   HostMesh master;
   std::vector<HostMesh> doms;
+  std::vector<int32_t> lnodes, lids;  // line elements (node pairs, ids) for edge classification
+  std::vector<int32_t> bnd;           // boundary class of each master edge (mesh_bnd_*, 1 = none)
 };
 
 extern "C" {
@@ -404,10 +418,57 @@ int mom_problem_load_gmsh(const char *path, double scale, int32_t ndom, const in
                           const int32_t *dom_ids, mom_problem **prob) {
   if (!path || !prob) return fail(MOM_ERR_ARG, "NULL argument");
   std::vector<double> p;
-  std::vector<int32_t> fnodes, fids;
-  MOM_TRY(read_gmsh(path, p, fnodes, fids));
-  return mom_problem_build((int64_t)(p.size() / 3), p.data(), (int64_t)fids.size(), fnodes.data(), fids.data(),
-                           scale, ndom, dom_ptr, dom_ids, prob);
+  std::vector<int32_t> fnodes, fids, lnodes, lids;
+  MOM_TRY(read_gmsh(path, p, fnodes, fids, lnodes, lids));
+  MOM_TRY(mom_problem_build((int64_t)(p.size() / 3), p.data(), (int64_t)fids.size(), fnodes.data(), fids.data(),
+                            scale, ndom, dom_ptr, dom_ids, prob));
+  (*prob)->lnodes = std::move(lnodes);
+  (*prob)->lids = std::move(lids);
+  return 0;
+}
+
+int mom_problem_set_lines(mom_problem *prob, int64_t nlines, const int32_t *nodes, const int32_t *ids) {
+  if (!prob || (nlines > 0 && (!nodes || !ids)) || nlines < 0) return fail(MOM_ERR_ARG, "NULL argument");
+  const int64_t V = prob->master.V();
+  for (int64_t i = 0; i < 2 * nlines; i++)
+    if (nodes[i] < 0 || nodes[i] >= V) return fail(MOM_ERR_ARG, "line node index out of range");
+  prob->lnodes.assign(nodes, nodes + 2 * nlines);
+  prob->lids.assign(ids, ids + nlines);
+  return 0;
+}
+
+// classify_edges (mesh.f90:155-174) as the `sbnd` command runs it on the
+// master mesh (interface.f90:120-157): every edge whose node pair is a line
+// of the given id gets the boundary class; the domain edges inherit their
+// parent's class (extract_submesh's inheritance, mesh.f90:427-445).
+int mom_problem_classify_edges(mom_problem *prob, int32_t id, int32_t bnd, int64_t *nclassified) {
+  if (!prob) return fail(MOM_ERR_ARG, "NULL argument");
+  if (bnd < 1 || bnd > 10) return fail(MOM_ERR_ARG, "boundary class must be one of mesh_bnd_* (1..10)");
+  HostMesh &M = prob->master;
+  if (prob->bnd.empty()) prob->bnd.assign(M.E(), kBndNone);
+  std::unordered_map<uint64_t, int64_t> count;
+  for (size_t l = 0; l < prob->lids.size(); l++)
+    if (prob->lids[l] == id) count[pair_key(prob->lnodes[2 * l], prob->lnodes[2 * l + 1])]++;
+  int64_t n = 0;
+  for (int64_t e = 0; e < M.E(); e++) {
+    auto it = count.find(pair_key(M.enodes[2 * e], M.enodes[2 * e + 1]));
+    if (it == count.end()) continue;
+    prob->bnd[e] = bnd;
+    n += it->second;  // the reference counts every matching line
+  }
+  if (nclassified) *nclassified = n;
+  return 0;
+}
+
+int mom_problem_edge_bnd(const mom_problem *prob, int32_t dom, int32_t *bnd) {
+  if (!prob || !bnd) return fail(MOM_ERR_ARG, "NULL argument");
+  if (dom < -1 || dom >= (int32_t)prob->doms.size()) return fail(MOM_ERR_ARG, "domain index out of range");
+  const HostMesh &m = dom < 0 ? prob->master : prob->doms[dom];
+  for (int64_t e = 0; e < m.E(); e++) {
+    const int64_t g = dom < 0 ? e : m.eparent[e];
+    bnd[e] = prob->bnd.empty() ? kBndNone : prob->bnd[g];
+  }
+  return 0;
 }
 
 int mom_problem_destroy(mom_problem *prob) {

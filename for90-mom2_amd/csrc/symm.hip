@@ -1,0 +1,235 @@
+// Mirror-symmetry decomposition of the PMCHWT system.
+//
+// Reference algorithm (for90-mom2): the groups of `symm` (read_symm,
+// interface.f90:160-210) built from group_id and group_mp (symmetry.f90:
+// 45-104) by product_group (symmetry.f90:173-241); the representation
+// matrices of sysmat_pmchwt (sysmat.f90:18-105) with the group-transformed
+// source integrals (srcint.f90:155-224, 368-448) and near rule
+// (sysmat.f90:233-260); the boundary conditions of edges on the mirror
+// planes and the reduced systems (edge_bc, reduce_system, expand_solution,
+// admissible_ga: bc.f90:28-257) solved per representation (solve_systems,
+// solver.f90:327-357).
+//
+// Scope: the planes x = 0, y = 0, z = 0 (generators mxp, myp, mzp) and their
+// products, groups of order 1, 2, 4 or 8 with real field actions +-1.
+// Rotation groups (group_rz) and periodic boundaries are not supported and
+// are rejected.
+//
+// Each element g of the group has a diagonal Jacobian J_g.  The image fill
+// of g integrates the test faces reflected by J_g against the source faces
+// (fill_domain with an Image): the reference integrates the test faces
+// against the sources reflected by J_g, and since J_g is its own inverse the
+// two integrals are equal, the K operator taking the factor det J_g.  The
+// image blocks are added to every representation r's matrix with the field
+// action ef[g][r] (and det J_g where the reference puts it), each written
+// straight into the reduced numbering of that representation: the rows and
+// columns that edge_bc removes are never stored.
+#include <algorithm>
+#include <vector>
+
+#include "mom_internal.hpp"
+
+namespace mom {
+
+Group identity_group() {
+  Group g;
+  g.n = 1;
+  for (int i = 0; i < kMaxReps; i++) {
+    for (int c = 0; c < 3; c++) g.js[i][c] = 1.0;
+    g.detj[i] = 1.0;
+    for (int r = 0; r < kMaxReps; r++) g.ef[i][r] = 1.0;
+    g.id[i] = kGidIdentity;
+    g.genbits[i] = 1 << kGidIdentity;
+  }
+  return g;
+}
+
+// product_group (symmetry.f90:173-241): element k = n * |b| + m has the
+// Jacobian J_n J_m, the union of the generator bits, the identifier of the
+// non-identity factor when the other one is the identity (else 0), and the
+// field action ef[k][k2] = ef_a[n][n2] ef_b[m][m2] with k2 = n2 |b| + m2.
+static Group product(const Group &a, const Group &b) {
+  Group g = identity_group();
+  g.n = a.n * b.n;
+  for (int n = 0; n < a.n; n++)
+    for (int m = 0; m < b.n; m++) {
+      const int k = n * b.n + m;
+      for (int c = 0; c < 3; c++) g.js[k][c] = a.js[n][c] * b.js[m][c];
+      g.detj[k] = a.detj[n] * b.detj[m];
+      g.genbits[k] = a.genbits[n] | b.genbits[m];
+      int id = 0;
+      if (a.id[n] == kGidIdentity && b.id[m] == kGidIdentity)
+        id = kGidIdentity;
+      else if (a.id[n] != 0 && a.id[n] != kGidIdentity && b.id[m] == kGidIdentity)
+        id = a.id[n];
+      else if (b.id[m] != 0 && b.id[m] != kGidIdentity && a.id[n] == kGidIdentity)
+        id = b.id[m];
+      g.id[k] = id;
+      for (int n2 = 0; n2 < a.n; n2++)
+        for (int m2 = 0; m2 < b.n; m2++) g.ef[k][n2 * b.n + m2] = a.ef[n][n2] * b.ef[m][m2];
+    }
+  return g;
+}
+
+int mirror_group(int ngen, const int32_t *planes, Group *out) {
+  if (ngen < 0 || ngen > 3 || (ngen > 0 && !planes)) return fail(MOM_ERR_ARG, "0 to 3 mirror generators");
+  Group g = identity_group();
+  int seen = 0;
+  for (int i = 0; i < ngen; i++) {
+    const int np = planes[i];
+    if (np < 1 || np > 3) return fail(MOM_ERR_ARG, "mirror plane must be 1 (x), 2 (y) or 3 (z)");
+    if (seen & (1 << np)) return fail(MOM_ERR_ARG, "mirror plane listed twice");
+    seen |= 1 << np;
+    // group_mp (symmetry.f90:61-104): {identity, mirror}, ef = [[1, 1], [1, -1]]
+    Group mp = identity_group();
+    mp.n = 2;
+    mp.ef[0][0] = 1.0;
+    mp.ef[0][1] = 1.0;
+    mp.ef[1][0] = 1.0;
+    mp.ef[1][1] = -1.0;
+    mp.js[1][np - 1] = -1.0;
+    mp.detj[1] = -1.0;
+    mp.id[1] = kGidIdentity + np;
+    mp.genbits[1] = 1 << (kGidIdentity + np);
+    g = product(g, mp);
+  }
+  *out = g;
+  return 0;
+}
+
+// edge_bc (bc.f90:28-118) for mirror groups: an edge on the plane of a pure
+// mirror element loses its J coefficient when Re(-ef) < 0.99 and its M
+// coefficient when Re(-ef det J) < 0.99 (-1 marks a removed unknown).
+static void edge_bc(int64_t N, const int32_t *bnd, const Group &G, int rep, std::vector<int> &id) {
+  id.assign(2 * N, 0);
+  for (int64_t k = 0; k < N; k++)
+    for (int n = 0; n < G.n; n++) {
+      const int b = bnd[k];
+      if ((G.id[n] == kGidMxp && b == kBndXplane) || (G.id[n] == kGidMyp && b == kBndYplane) ||
+          (G.id[n] == kGidMzp && b == kBndZplane)) {
+        const double gae = G.ef[n][rep], gah = gae * G.detj[n];
+        if (-gae < 0.99) id[k] = -1;
+        if (-gah < 0.99) id[k + N] = -1;
+      }
+    }
+}
+
+// reduce_system's numbering (bc.f90:170-193): the kept unknowns in order
+static int64_t reduce_map(const std::vector<int> &id, std::vector<int> &map) {
+  map.assign(id.size(), -1);
+  int64_t dim = 0;
+  for (size_t u = 0; u < id.size(); u++)
+    if (id[u] != -1) map[u] = (int)dim++;
+  return dim;
+}
+
+__global__ void set_diag_kernel(double2 *Z, int64_t ld, int64_t i0, int64_t n) {
+  const int64_t i = i0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) Z[i + i * ld] = make_double2(1.0, 0.0);
+}
+
+int fill_system_symm(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *ri, int64_t N,
+                     const int32_t *bnd, const Group &G, double omega, int32_t nq, const double *qw,
+                     const double *qnodes, double2 *const *Z, const int64_t *ld, const int64_t *order) {
+  if (!(omega > 0)) return fail(MOM_ERR_ARG, "omega must be > 0");
+  QuadRule rule;
+  int rc = make_rule(nq, qw, qnodes, &rule);
+  if (rc) return rc;
+  for (int d = 0; d < ndom; d++) {
+    if (!doms[d]) return fail(MOM_ERR_ARG, "NULL domain mesh");
+    if (doms[d]->ctx != ctx) return fail(MOM_ERR_ARG, "domain mesh belongs to another context");
+    if (doms[d]->max_parent >= N) return fail(MOM_ERR_ARG, "domain edge parent index >= nedges_total");
+  }
+  std::vector<DevBuf> dmap(G.n);
+  RepOut out;
+  for (int r = 0; r < G.n; r++) {
+    std::vector<int> id, map;
+    edge_bc(N, bnd, G, r, id);
+    const int64_t dim = reduce_map(id, map);
+    // an odd order is padded by one decoupled unknown (unit diagonal)
+    if (order[r] != dim + (dim & 1))
+      return fail(MOM_ERR_ARG, "system order of representation " + std::to_string(r) + " must be " +
+                                   std::to_string(dim + (dim & 1)));
+    if ((rc = dmap[r].alloc(sizeof(int) * 2 * N))) return rc;
+    MOM_TRY(copy_h2d(ctx, dmap[r].ptr, map.data(), sizeof(int) * 2 * N));
+    MOM_HIP(hipMemset2DAsync(Z[r], ld[r] * sizeof(double2), 0, order[r] * sizeof(double2), order[r], ctx->stream));
+    if (order[r] > dim)
+      hipLaunchKernelGGL(set_diag_kernel, dim3(1), dim3(64), 0, ctx->stream, Z[r], ld[r], dim, order[r]);
+    out.Z[r] = Z[r];
+    out.ld[r] = ld[r];
+    out.map[r] = dmap[r].as<int>();
+  }
+  for (int d = 0; d < ndom; d++) {
+    // has_mesh_bnd of the domain (mesh.f90:86-100) through the edges' parents
+    bool has[5] = {false, false, false, false, false};
+    for (int64_t e = 0; e < doms[d]->E; e++) {
+      const int b = bnd[doms[d]->h_eparent[e]];
+      if (b >= 1 && b <= 4) has[b] = true;
+    }
+    const cd r(ri[2 * d], ri[2 * d + 1]);
+    const cd zsi = (r * r) / (kEta0 * kEta0);
+    for (int g = 0; g < G.n; g++) {
+      // admissible_ga (bc.f90:237-257): the first (exterior) domain takes every
+      // image, another one only the mirrors whose planes cut its surface
+      if (d > 0 && (((G.genbits[g] >> kGidMxp) & 1 && !has[kBndXplane]) ||
+                    ((G.genbits[g] >> kGidMyp) & 1 && !has[kBndYplane]) ||
+                    ((G.genbits[g] >> kGidMzp) & 1 && !has[kBndZplane])))
+        continue;
+      FillTarget t;
+      t.Noff = N;
+      t.mode = 0;
+      t.zsi = make_double2(zsi.real(), zsi.imag());
+      t.nrep = G.n;
+      t.detj = G.detj[g];
+      t.rep = out;
+      for (int k = 0; k < G.n; k++) t.rep.e[k] = G.ef[g][k];
+      Image img;
+      for (int c = 0; c < 3; c++) img.js[c] = G.js[g][c];
+      img.detj = G.detj[g];
+      const bool ident = G.js[g][0] == 1.0 && G.js[g][1] == 1.0 && G.js[g][2] == 1.0;
+      if ((rc = fill_domain(ctx, doms[d], rule, omega, r, t, ident ? nullptr : &img))) return rc;
+    }
+  }
+  // the maps must outlive the queued kernels
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // namespace mom
+
+using namespace mom;
+
+extern "C" {
+
+int mom_symm_group(int32_t ngen, const int32_t *planes, int32_t *nga, double *ef, double *detj, double *jdiag,
+                   int32_t *id) {
+  if (!nga) return fail(MOM_ERR_ARG, "NULL argument");
+  Group G;
+  MOM_TRY(mirror_group(ngen, planes, &G));
+  *nga = G.n;
+  for (int g = 0; g < G.n; g++) {
+    if (ef)
+      for (int r = 0; r < G.n; r++) ef[g * G.n + r] = G.ef[g][r];
+    if (detj) detj[g] = G.detj[g];
+    if (jdiag)
+      for (int c = 0; c < 3; c++) jdiag[3 * g + c] = G.js[g][c];
+    if (id) id[g] = G.id[g];
+  }
+  return 0;
+}
+
+int mom_symm_edge_bc(int64_t nedges_total, const int32_t *bnd, int32_t ngen, const int32_t *planes, int32_t rep,
+                     int32_t *id, int64_t *dim) {
+  if (!bnd || !id || !dim) return fail(MOM_ERR_ARG, "NULL argument");
+  if (nedges_total < 1) return fail(MOM_ERR_ARG, "nedges_total must be >= 1");
+  Group G;
+  MOM_TRY(mirror_group(ngen, planes, &G));
+  if (rep < 0 || rep >= G.n) return fail(MOM_ERR_ARG, "representation index out of range");
+  std::vector<int> v, map;
+  edge_bc(nedges_total, bnd, G, rep, v);
+  *dim = reduce_map(v, map);
+  std::copy(v.begin(), v.end(), id);
+  return 0;
+}
+
+}  // extern "C"

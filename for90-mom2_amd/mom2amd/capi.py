@@ -26,7 +26,9 @@ EXPORTS = [
     "mom_system_solve_adjoint", "mom_cs_abs_gradient", "mom_pmchwt_deps_apply",
     "mom_system_solve_pw", "mom_system_profile", "mom_system_stats", "mom_cs_prtsrf", "mom_far_fields",
     "mom_rcs_ext", "mom_nlsurf_coef", "mom_sysmat_pmchwt_nls", "mom_system_fill_nls", "mom_problem_build", "mom_problem_load_gmsh", "mom_problem_destroy", "mom_problem_sizes", "mom_problem_domain",
-    "mom_problem_topology", "mom_problem_mesh_create",
+    "mom_problem_topology", "mom_problem_mesh_create", "mom_problem_set_lines", "mom_problem_classify_edges",
+    "mom_problem_edge_bnd", "mom_symm_group", "mom_symm_edge_bc", "mom_system_create_order",
+    "mom_system_fill_symm", "mom_srcvec_pw_symm", "mom_cs_prtsrf_symm", "mom_rcs_ext_symm",
 ]
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
@@ -105,6 +107,19 @@ def lib():
                                      C.POINTER(_d)]
     L.mom_problem_topology.argtypes = [_vp, _i32, _ip, _ip, _ip, _ip]
     L.mom_problem_mesh_create.argtypes = [_vp, _vp, _i32, C.POINTER(_vp)]
+    L.mom_problem_set_lines.argtypes = [_vp, _i64, _ip, _ip]
+    L.mom_problem_classify_edges.argtypes = [_vp, _i32, _i32, C.POINTER(_i64)]
+    L.mom_problem_edge_bnd.argtypes = [_vp, _i32, _ip]
+    L.mom_symm_group.argtypes = [_i32, _ip, C.POINTER(_i32), _dp, _dp, _dp, _ip]
+    L.mom_symm_edge_bc.argtypes = [_i64, _ip, _i32, _ip, _i32, _ip, C.POINTER(_i64)]
+    L.mom_system_create_order.argtypes = [_vp, _i64, _i32, C.POINTER(_vp)]
+    L.mom_system_fill_symm.argtypes = [_i32, C.POINTER(_vp), _i32, C.POINTER(_vp), _dp, _i64, _ip, _i32, _ip, _d,
+                                       _i32, _dp, _dp]
+    L.mom_srcvec_pw_symm.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _ip, _i64, _dp, _i32, _dp, _dp, _dp, _i64]
+    L.mom_cs_prtsrf_symm.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _ip, _dp, _d, _d, _d, _i32, _dp, _dp,
+                                     C.POINTER(_d), C.POINTER(_d)]
+    L.mom_rcs_ext_symm.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _ip, _dp, _i32, _i32, _i32, C.c_void_p, _i32,
+                                   _dp, _dp, _dp]
     _lib = L
     return L
 

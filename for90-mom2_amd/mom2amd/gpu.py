@@ -188,12 +188,23 @@ def cs_prtsrf(ctx, dmesh, nedges_total, omega, ri, x, theta, phi, psi, quad=None
 class System:
     """Device-resident PMCHWT system (mom_system): fill, factor, solve."""
 
-    def __init__(self, ctx, nedges_total, precision=capi.MOM_PREC_MIXED):
+    def __init__(self, ctx, nedges_total, precision=capi.MOM_PREC_MIXED, order=None):
         self.ctx = ctx
-        self.N = nedges_total
         self.h = C.c_void_p()
-        capi.check(capi.lib().mom_system_create(ctx.h, nedges_total, precision, C.byref(self.h)))
+        if order is None:
+            self.N = nedges_total
+            self.n = 2 * nedges_total
+            capi.check(capi.lib().mom_system_create(ctx.h, nedges_total, precision, C.byref(self.h)))
+        else:
+            # one representation of a symmetry group (mom_system_create_order)
+            self.N = 0
+            self.n = int(order)
+            capi.check(capi.lib().mom_system_create_order(ctx.h, self.n, precision, C.byref(self.h)))
         ctx._adopt(self)
+
+    @classmethod
+    def of_order(cls, ctx, order, precision=capi.MOM_PREC_MIXED):
+        return cls(ctx, 0, precision, order=order)
 
     def fill(self, dmeshes, ri_list, omega, quad=None):
         Q, w, n = _quad(quad)
@@ -201,13 +212,13 @@ class System:
         capi.check(capi.lib().mom_system_fill(self.h, len(dmeshes), ptr, _ri_array(ri_list), omega, Q, w, n))
 
     def get_matrix(self):
-        N2 = 2 * self.N
+        N2 = self.n
         A = np.zeros(2 * N2 * N2)
         capi.check(capi.lib().mom_system_get_matrix(self.h, A, N2))
         return A.view(np.complex128).reshape(N2, N2, order="F")
 
     def set_matrix(self, A):
-        N2 = 2 * self.N
+        N2 = self.n
         Av = np.asfortranarray(A, dtype=np.complex128).reshape(-1, order="F").view(np.float64)
         capi.check(capi.lib().mom_system_set_matrix(self.h, np.ascontiguousarray(Av), N2))
 
@@ -218,7 +229,7 @@ class System:
         capi.check(st)
 
     def solve(self, B, tol=1e-12, maxit=30):
-        N2 = 2 * self.N
+        N2 = self.n
         B2 = np.asfortranarray(np.atleast_2d(np.asarray(B).T).T, dtype=np.complex128).copy(order="F")
         nrhs = B2.shape[1]
         Bv = np.ascontiguousarray(B2.reshape(-1, order="F").view(np.float64))
@@ -230,7 +241,7 @@ class System:
 
     def solve_adjoint(self, B, tol=1e-12, maxit=30):
         """Z^H X = B on the same factorisation (mom_system_solve_adjoint)."""
-        N2 = 2 * self.N
+        N2 = self.n
         B2 = np.asfortranarray(np.atleast_2d(np.asarray(B).T).T, dtype=np.complex128).copy(order="F")
         nrhs = B2.shape[1]
         Bv = np.ascontiguousarray(B2.reshape(-1, order="F").view(np.float64))

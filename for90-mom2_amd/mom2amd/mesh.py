@@ -55,6 +55,7 @@ class DomainMesh:
     elen: np.ndarray = None  # (E,)
     eparent: np.ndarray = None  # (E,) master edge index
     avelen: float = 0.0
+    bnd: np.ndarray = None   # (E,) boundary class (mesh_bnd_*: 1 none, 2/3/4 the x/y/z mirror planes)
 
     @property
     def nnodes(self):
@@ -95,6 +96,8 @@ def _fetch(L, h, d) -> DomainMesh:
                                     flat(m.fs), flat(m.fm), m.farea, flat(m.efaces), flat(m.ebn), m.elen,
                                     m.eparent, C.byref(av)))
     capi.check(L.mom_problem_topology(h, d, flat(m.enodes), m.fid, m.fparent, m.node_parent))
+    m.bnd = np.zeros(E, np.int32)
+    capi.check(L.mom_problem_edge_bnd(h, d, m.bnd))
     m.avelen = av.value
     return m
 
@@ -105,8 +108,24 @@ def _domain_lists(domains):
     return dom_ptr, dom_ids
 
 
-def _collect(L, h, ndom) -> Problem:
+# boundary classes of the `sbnd` command (interface.f90:120-157, mesh.f90:16-25)
+BND_CLASSES = {"none": 1, "xplane": 2, "yplane": 3, "zplane": 4}
+
+
+def _classify(L, h, lines, sbnd):
+    if lines is not None:
+        ln, lid = lines
+        ln = np.ascontiguousarray(ln, dtype=np.int32).reshape(-1)
+        lid = np.ascontiguousarray(lid, dtype=np.int32).reshape(-1)
+        capi.check(L.mom_problem_set_lines(h, lid.size, ln, lid))
+    for lid, cls in sbnd:
+        code = BND_CLASSES[cls] if isinstance(cls, str) else int(cls)
+        capi.check(L.mom_problem_classify_edges(h, int(lid), code, None))
+
+
+def _collect(L, h, ndom, lines=None, sbnd=()) -> Problem:
     try:
+        _classify(L, h, lines, sbnd)
         master = _fetch(L, h, -1)
         doms = [_fetch(L, h, d) for d in range(ndom)]
     finally:
@@ -114,14 +133,17 @@ def _collect(L, h, ndom) -> Problem:
     return Problem(master=master, domains=doms)
 
 
-def build_problem(nodes, faces, ids, scale: float, domains) -> Problem:
+def build_problem(nodes, faces, ids, scale: float, domains, lines=None, sbnd=()) -> Problem:
     """Build the master mesh and domain meshes with the library's set-up
     (mom_problem_build: build_mesh, extract_submesh,
     submesh_edge_connectivity and orient_basis of mesh.f90).
 
     nodes: (V,3) coordinates in file units; faces: (F,3) 0-based; ids: (F,)
     physical ids; scale: the `mesh` command's scale factor; domains: list of
-    lists of signed surface ids, one list per `sdom` (negative = reversed).
+    lists of signed surface ids, one list per `sdom` (negative = reversed);
+    lines: (node pairs (L,2) 0-based, ids (L,)) of the mesh's line elements;
+    sbnd: (line id, class) pairs of the `sbnd` command, class one of
+    BND_CLASSES (mom_problem_classify_edges).
     """
     L = capi.lib()
     nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 3)
@@ -133,15 +155,16 @@ def build_problem(nodes, faces, ids, scale: float, domains) -> Problem:
     h = C.c_void_p()
     capi.check(L.mom_problem_build(nodes.shape[0], nodes.reshape(-1), faces.shape[0], faces.reshape(-1), ids,
                                    float(scale), len(domains), dom_ptr, dom_ids, C.byref(h)))
-    return _collect(L, h, len(domains))
+    return _collect(L, h, len(domains), lines, sbnd)
 
 
-def load_problem(path: str, scale: float, domains) -> Problem:
-    """`mesh` + `sdom` + `fdom` from a Gmsh 2.x file, read and built by the
-    library (mom_problem_load_gmsh; load_mesh_gmsh, mesh.f90:610-736)."""
+def load_problem(path: str, scale: float, domains, sbnd=()) -> Problem:
+    """`mesh` + `sbnd` + `sdom` + `fdom` from a Gmsh 2.x file, read and built
+    by the library (mom_problem_load_gmsh; load_mesh_gmsh, mesh.f90:610-736);
+    sbnd classifies edges by the file's line elements as build_problem's."""
     L = capi.lib()
     dom_ptr, dom_ids = _domain_lists(domains)
     h = C.c_void_p()
     capi.check(L.mom_problem_load_gmsh(str(path).encode(), float(scale), len(domains), dom_ptr, dom_ids,
                                        C.byref(h)))
-    return _collect(L, h, len(domains))
+    return _collect(L, h, len(domains), None, sbnd)

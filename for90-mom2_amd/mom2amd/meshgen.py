@@ -14,7 +14,8 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["icosphere", "ellipsoid", "perturbed_sphere", "dimer", "write_gmsh", "read_gmsh"]
+__all__ = ["icosphere", "ellipsoid", "perturbed_sphere", "dimer", "octant_sphere", "sphere_cell", "octasphere",
+           "write_gmsh", "read_gmsh"]
 
 
 def _icosahedron():
@@ -124,15 +125,134 @@ def dimer(subdiv: int, radius: float, gap: float):
     return nodes, faces, ids
 
 
-def write_gmsh(path: str, nodes: np.ndarray, faces: np.ndarray, ids=15) -> None:
-    """Write a Gmsh 2.2 ASCII triangle mesh; node coordinates use %.17g so
-    that every reader recovers the exact doubles."""
+# line-element ids of the mirror-plane boundaries written by octant_sphere
+OCTANT_LINE_IDS = {"xplane": 101, "yplane": 102, "zplane": 103}
+
+
+def _octant_grid(n):
+    """Nodes P(i, j) = (i e_x + j e_y + (n-i-j) e_z)/n of the octahedron face,
+    i + j <= n, projected to the unit sphere (coordinates on the planes are
+    exactly 0), and their index."""
+    index, pts = {}, []
+    for i in range(n + 1):
+        for j in range(n + 1 - i):
+            v = np.array([i, j, n - i - j], dtype=np.float64) / n
+            index[(i, j)] = len(pts)
+            pts.append(v / np.linalg.norm(v))
+    return np.array(pts), index
+
+
+def octant_sphere(n: int, radius: float = 1.0):
+    """The octant x, y, z >= 0 of a sphere: the octahedron face (e_x, e_y,
+    e_z) split into n^2 triangles, projected to `radius`.  Its edges on the
+    planes x = 0, y = 0, z = 0 are open boundaries, returned as line
+    elements with the ids of OCTANT_LINE_IDS.  Mirrored in the three planes
+    (octasphere) it closes into a sphere of 8 n^2 faces.
+
+    Returns nodes[V,3], faces[F,3] (0-based, outward normals), lines[L,2],
+    line_ids[L]."""
+    pts, index = _octant_grid(n)
+    faces = []
+    for i in range(n):
+        for j in range(n - i):
+            faces.append((index[(i, j)], index[(i + 1, j)], index[(i, j + 1)]))
+            if i + j < n - 1:
+                faces.append((index[(i + 1, j)], index[(i + 1, j + 1)], index[(i, j + 1)]))
+    faces = np.array(faces, dtype=np.int64)
+    nodes = pts * radius
+    p1, p2, p3 = nodes[faces[:, 0]], nodes[faces[:, 1]], nodes[faces[:, 2]]
+    flip = np.einsum("ij,ij->i", np.cross(p2 - p1, p3 - p1), (p1 + p2 + p3) / 3.0) < 0
+    faces[flip] = faces[flip][:, ::-1]
+    lines, lids = [], []
+    for j in range(n):  # x = 0: i = 0
+        lines.append((index[(0, j)], index[(0, j + 1)]))
+        lids.append(OCTANT_LINE_IDS["xplane"])
+    for i in range(n):  # y = 0: j = 0
+        lines.append((index[(i, 0)], index[(i + 1, 0)]))
+        lids.append(OCTANT_LINE_IDS["yplane"])
+    for i in range(n):  # z = 0: i + j = n
+        lines.append((index[(i, n - i)], index[(i + 1, n - i - 1)]))
+        lids.append(OCTANT_LINE_IDS["zplane"])
+    return nodes, faces, np.array(lines, dtype=np.int64), np.array(lids, dtype=np.int64)
+
+
+def sphere_cell(n: int, radius: float = 1.0, reflect=()):
+    """The octant of octant_sphere(n) together with its images in the planes
+    listed in `reflect` (0 = x, 1 = y, 2 = z): a symmetry cell of the sphere
+    whose open boundaries lie on the remaining planes, with their line
+    elements (ids of OCTANT_LINE_IDS).  reflect=() is the octant itself,
+    (0, 1, 2) the closed sphere (octasphere).
+
+    Returns nodes, faces (outward), lines[L,2], line_ids[L]."""
+    on, of, ol, oid = octant_sphere(n, radius)
+    key_of, nodes, faces, lines, lids = {}, [], [], [], []
+    signs = [[1.0, -1.0] if a in reflect else [1.0] for a in range(3)]
+    for sx in signs[0]:
+        for sy in signs[1]:
+            for sz in signs[2]:
+                js = np.array([sx, sy, sz])
+                idx = []
+                for p in on * js:
+                    k = tuple(float(c) + 0.0 for c in p)  # -0.0 and 0.0 are one node
+                    if k not in key_of:
+                        key_of[k] = len(nodes)
+                        nodes.append(k)
+                    idx.append(key_of[k])
+                for a, b, c in of:
+                    f = (idx[a], idx[b], idx[c])
+                    faces.append(f if sx * sy * sz > 0 else f[::-1])
+                for (a, b), lid in zip(ol, oid):
+                    axis = {OCTANT_LINE_IDS["xplane"]: 0, OCTANT_LINE_IDS["yplane"]: 1,
+                            OCTANT_LINE_IDS["zplane"]: 2}[int(lid)]
+                    if axis not in reflect:
+                        lines.append((idx[a], idx[b]))
+                        lids.append(int(lid))
+    return (np.array(nodes, dtype=np.float64), np.array(faces, dtype=np.int64),
+            np.array(lines, dtype=np.int64).reshape(-1, 2), np.array(lids, dtype=np.int64))
+
+
+def octasphere(n: int, radius: float = 1.0):
+    """The closed sphere made of the 8 mirror images of octant_sphere(n)
+    (shared nodes on the planes merged), outward normals.  The same
+    triangles as the octant problem with the group mxp myp mzp."""
+    on, of, _, _ = octant_sphere(n, radius)
+    key_of, nodes, faces = {}, [], []
+    for sx in (1.0, -1.0):
+        for sy in (1.0, -1.0):
+            for sz in (1.0, -1.0):
+                js = np.array([sx, sy, sz])
+                idx = []
+                for p in on * js:
+                    k = tuple(float(c) + 0.0 for c in p)  # -0.0 and 0.0 are one node
+                    if k not in key_of:
+                        key_of[k] = len(nodes)
+                        nodes.append(k)
+                    idx.append(key_of[k])
+                for a, b, c in of:
+                    f = (idx[a], idx[b], idx[c])
+                    faces.append(f if sx * sy * sz > 0 else f[::-1])
+    nodes = np.array(nodes, dtype=np.float64)
+    faces = np.array(faces, dtype=np.int64)
+    return nodes, faces
+
+
+def write_gmsh(path: str, nodes: np.ndarray, faces: np.ndarray, ids=15, lines=None, line_ids=None) -> None:
+    """Write a Gmsh 2.2 ASCII triangle mesh (and optional line elements,
+    written after the triangles: the reference's reader fails on a file
+    whose line elements come first); node coordinates use %.17g so that
+    every reader recovers the exact doubles."""
     ids = np.broadcast_to(np.asarray(ids), (faces.shape[0],))
-    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(nodes.shape[0])]
-    lines += ["%d %.17g %.17g %.17g" % (i + 1, x, y, z) for i, (x, y, z) in enumerate(nodes)]
-    lines += ["$EndNodes", "$Elements", str(faces.shape[0])]
-    lines += ["%d 2 2 %d %d %d %d %d" % (i + 1, ids[i], ids[i], a + 1, b + 1, c + 1)
-              for i, (a, b, c) in enumerate(faces)]
+    nl = 0 if lines is None else len(lines)
+    nf = faces.shape[0]
+    out = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(nodes.shape[0])]
+    out += ["%d %.17g %.17g %.17g" % (i + 1, x, y, z) for i, (x, y, z) in enumerate(nodes)]
+    out += ["$EndNodes", "$Elements", str(nf + nl)]
+    out += ["%d 2 2 %d %d %d %d %d" % (i + 1, ids[i], ids[i], a + 1, b + 1, c + 1)
+            for i, (a, b, c) in enumerate(faces)]
+    if nl:
+        out += ["%d 1 2 %d %d %d %d" % (nf + i + 1, line_ids[i], line_ids[i], a + 1, b + 1)
+                for i, (a, b) in enumerate(lines)]
+    lines = out
     lines += ["$EndElements", ""]
     with open(path, "w") as fh:
         fh.write("\n".join(lines))

@@ -323,6 +323,70 @@ int mom_problem_topology(const mom_problem *prob, int32_t dom, int32_t *enodes, 
 /* mom_mesh_create of domain dom (0-based) straight from the set-up. */
 int mom_problem_mesh_create(mom_ctx *ctx, const mom_problem *prob, int32_t dom, mom_mesh **mesh);
 
+/* ---------------------------------------------------------------------------
+ * Mirror symmetry (the `symm` and `sbnd` commands with the planes x = 0,
+ * y = 0, z = 0).  The mesh holds one symmetry cell; its edges on a mirror
+ * plane are boundary edges (one face), classified by the line elements that
+ * the mesh file puts on them.  planes[ngen] lists the generators in the
+ * order of the `symm` line: 1 = mxp, 2 = myp, 3 = mzp.  Rotation groups and
+ * periodic boundaries are not supported (MOM_ERR_ARG).
+ */
+/* Line elements (0-based master node pairs, ids) for classification, for a
+ * problem built from arrays; mom_problem_load_gmsh keeps the file's lines. */
+int mom_problem_set_lines(mom_problem *prob, int64_t nlines, const int32_t *nodes, const int32_t *ids);
+/* classify_edges (mesh.f90:155-174), the `sbnd id name` command
+ * (interface.f90:120-157): master edges whose node pair is a line of that id
+ * get boundary class bnd (mesh_bnd_*: 2 xplane, 3 yplane, 4 zplane, mesh.f90:
+ * 16-25); domain edges inherit their parent's class (mesh.f90:427-445).
+ * nclassified (may be NULL) counts the matches as the reference prints them. */
+int mom_problem_classify_edges(mom_problem *prob, int32_t id, int32_t bnd, int64_t *nclassified);
+/* Boundary class of every edge of domain dom (-1: master), 1 = none. */
+int mom_problem_edge_bnd(const mom_problem *prob, int32_t dom, int32_t *bnd);
+/* read_symm (interface.f90:160-210): the group of the mirror generators,
+ * built by group_mp and product_group (symmetry.f90:61-104, 173-241).
+ * nga = 2^ngen elements; ef[g*nga + r] field action of element g in
+ * representation r; detj[g]; jdiag[3g..3g+2] the diagonal Jacobian; id[g]
+ * (1 identity, 2/3/4 the pure mirrors, 0 products).  Outputs may be NULL. */
+int mom_symm_group(int32_t ngen, const int32_t *planes, int32_t *nga, double *ef, double *detj, double *jdiag,
+                   int32_t *id);
+/* edge_bc (bc.f90:28-118) of representation rep (0-based) for the master
+ * edges' boundary classes bnd[N]: id[2N] = -1 for the unknowns the
+ * representation removes, 0 otherwise; dim = the kept count, the order of
+ * reduce_system's system (bc.f90:170-193).  The representation's system is
+ * made with mom_system_create_order(dim + (dim & 1)): an odd order gets one
+ * decoupled unknown with a unit diagonal, whose right-hand side is 0. */
+int mom_symm_edge_bc(int64_t nedges_total, const int32_t *bnd, int32_t ngen, const int32_t *planes, int32_t rep,
+                     int32_t *id, int64_t *dim);
+/* A device system of even order n for one representation (filled by
+ * mom_system_fill_symm; factor, solve and the stats work as for
+ * mom_system_create's systems). */
+int mom_system_create_order(mom_ctx *ctx, int64_t order, int32_t precision, mom_system **sys);
+/* sysmat_pmchwt (sysmat.f90:18-105) for the group: systems[r], r < nga,
+ * receive representation r's matrix in reduced numbering (the unknowns of
+ * mom_symm_edge_bc with id != -1, in order), the image of every source face
+ * under every admissible element (admissible_ga, bc.f90:237-257) being
+ * integrated once and added to all representations.  bnd[N]: master edge
+ * classes.  Other arguments as mom_system_fill. */
+int mom_system_fill_symm(int32_t nrep, mom_system *const *systems, int32_t ndom, const mom_mesh *const *doms,
+                         const double *ri, int64_t nedges_total, const int32_t *bnd, int32_t ngen,
+                         const int32_t *planes, double omega, int32_t nq, const double *qw, const double *qnodes);
+/* srcvec (source.f90:136-247) with src_field_frags' projection on the
+ * representations (source.f90:249-315): q[:, r + nga*s] (2N rows, ldq) is
+ * plane wave s (angles as mom_srcvec_pw) in representation r, unreduced. */
+int mom_srcvec_pw_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                       double ri_im, int32_t ngen, const int32_t *planes, int64_t nsrc, const double *angles,
+                       int32_t nq, const double *qw, const double *qnodes, double *q, int64_t ldq);
+/* cs_prtsrf (cs.f90:14-104) and rcs_ext (ffields.f90:203-249) over every
+ * image of the mesh, x[2N * nga] the expanded solutions of the nga
+ * representations (expand_solution, bc.f90:199-232: removed unknowns 0). */
+int mom_cs_prtsrf_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                       double ri_im, int32_t ngen, const int32_t *planes, const double *x, double theta, double phi,
+                       double psi, int32_t nq, const double *qw, const double *qnodes, double *csca, double *cabs);
+int mom_rcs_ext_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                     double ri_im, int32_t ngen, const int32_t *planes, const double *x, int32_t ntheta, int32_t nphi,
+                     int32_t filter, const double *fdir, int32_t nq, const double *qw, const double *qnodes,
+                     double *rcs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,8 +1,10 @@
 #!/usr/bin/env bash
-# Builds oracle/_ref/mom_ref: the reference Fortran (jmakitalo/for90-mom2,
-# read from /root/reference) linked with our own driver ref_driver.f90,
-# which dumps golden vectors (mesh numbering, computeD/H/K, sysmat_pmchwt,
-# srcvec, cs_prtsrf, rcs) at full precision.
+# Builds oracle/_ref/mom_ref and oracle/_ref/mom_ref_symm: the reference
+# Fortran (jmakitalo/for90-mom2, read from /root/reference) linked with our
+# own drivers ref_driver.f90 and ref_symm.f90, which dump golden vectors
+# (mesh numbering, computeD/H/K, sysmat_pmchwt, srcvec, cs_prtsrf, rcs; the
+# latter for a mirror-symmetric problem with every representation) at full
+# precision.
 #
 # TEST INFRASTRUCTURE ONLY.  The reference source never enters the
 # repository: this script compiles it from a scratch copy outside the repo
@@ -89,6 +91,9 @@ done
 for o in $OBJS; do
   objcopy $RENAME "$o"
 done
-$FC -fopenmp -o "$OUT/mom_ref" ref_driver.o $OBJS -L"$BLAS_DIR" -l:"$(basename "$BLAS_SO")" \
-  -Wl,-rpath,"$BLAS_DIR" -Wl,-z,execstack
-echo "built $OUT/mom_ref"
+$FC $FFLAGS -c "$HERE/ref_symm.f90" -o ref_symm.o
+for prog in ref_driver:mom_ref ref_symm:mom_ref_symm; do
+  $FC -fopenmp -o "$OUT/${prog#*:}" "${prog%%:*}.o" $OBJS -L"$BLAS_DIR" -l:"$(basename "$BLAS_SO")" \
+    -Wl,-rpath,"$BLAS_DIR" -Wl,-z,execstack
+  echo "built $OUT/${prog#*:}"
+done

@@ -36,6 +36,9 @@ PROGRAM ref_driver
 
   TYPE(batch) :: b
   CHARACTER(LEN=512) :: mshfile, outdir, ln
+  ! get_mesh_element_lines (mesh.f90:578-608) declares its file name LEN=256:
+  ! the name is passed blank-padded to exactly that length
+  CHARACTER(LEN=256) :: mshname
   REAL(KIND=dp) :: scale, wl, omega, th, ph, ps, t0, tfill, tsrc, csca, cabs, re, im
   INTEGER :: ndom, d, nsurf, i, ids(16), dump_ops, dump_sys, nth, nph, N, u, ios
   TYPE(srcdata) :: src
@@ -61,7 +64,8 @@ PROGRAM ref_driver
   CALL batch_defaults(b)
   CALL group_id(b%ga)
 
-  b%mesh = load_mesh(TRIM(mshfile))
+  mshname = mshfile
+  b%mesh = load_mesh(mshname)
   CALL build_mesh(b%mesh, scale)
 
   ALLOCATE(b%domains(ndom))

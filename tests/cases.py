@@ -3,7 +3,7 @@ import os
 
 import numpy as np
 
-from mom2amd import meshgen, build_problem, omega_of
+from mom2amd import meshgen, build_problem, load_problem, omega_of
 
 N_IN_SPHERE = np.sqrt(-9.4 + 1.5j)
 N_IN_DIMER = np.sqrt(-40 + 2.5j)
@@ -40,3 +40,18 @@ def blockwise_relerr(A, B, nblk=2):
             b = B[i * n:(i + 1) * n, j * n:(j + 1) * n]
             err = max(err, np.abs(a - b).max() / np.abs(b).max())
     return err
+
+
+def sphere_cell(msh_dir, n, reflect=(), radius=50.0):
+    """Mirror-symmetry cell of the octasphere (meshgen.sphere_cell) with its
+    boundary edges classified (`sbnd`), at the sphere case's wavelength and
+    permittivity."""
+    nodes, faces, lines, lids = meshgen.sphere_cell(n, radius, reflect)
+    tag = "".join(str(a) for a in reflect)
+    path = os.path.join(str(msh_dir), "cell%d_%s_r%g.msh" % (n, tag, radius))
+    if not os.path.exists(path):
+        meshgen.write_gmsh(path, nodes, faces, 15, lines, lids)
+    sbnd = [(i, c) for c, i in meshgen.OCTANT_LINE_IDS.items() if i in set(int(v) for v in lids)]
+    doms = [[15], [-15]]
+    return dict(path=path, domains=doms, sbnd=sbnd, problem=load_problem(path, 1e-9, doms, sbnd=sbnd),
+                wl=600e-9, omega=omega_of(600e-9), ri=[1.0 + 0j, complex(N_IN_SPHERE)])

@@ -17,7 +17,7 @@ Fixtures (all complex128 / float64 at full precision):
                  index sqrt(-9.4+1.5i)): nlsurf_coef's src_coef/src_vec and
                  epsp per sphere, the SH source before and after
                  sysmat_pmchwt_nls, the SH solution and its rcs 37x72
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [symm]   (symm: the symmetric fixtures only)
 """
 import os
 import sys
@@ -82,6 +82,46 @@ def case(name, mshfile, domains, wl, ri, angles, full_A, nsample=3000, seed=1, s
     print(name, "N=%d" % N, "csca=%.6e cabs=%.6e" % (P["csca"], P["cabs"]))
 
 
+def symm_case(name, n, reflect, gens, angles, with_A):
+    """Mirror-symmetric sphere cell (meshgen.sphere_cell) solved by the
+    reference per representation (oracle/_ref/mom_ref_symm)."""
+    tmp = os.path.join(HERE, "_msh")
+    nodes, faces, lines, lids = meshgen.sphere_cell(n, 50.0, reflect)
+    fn = os.path.join(tmp, name + ".msh")
+    meshgen.write_gmsh(fn, nodes, faces, 15, lines, lids)
+    sbnd = [(i, c) for c, i in meshgen.OCTANT_LINE_IDS.items() if i in set(int(v) for v in lids)]
+    ri = [1.0 + 0j, np.sqrt(-9.4 + 1.5j)]
+    R = refrun.run_symm(fn, 1e-9, gens, sbnd, [[15], [-15]], 600e-9, ri, angles, ntheta=37, nphi=72)
+    data = dict(n=np.array(n), reflect=np.array(list(reflect) + [-1] * (3 - len(reflect))),
+                gens=np.array(gens), wl=np.array(600e-9), ri=np.array(ri, dtype=np.complex128),
+                angles_deg=np.array(angles, float), q=R["q"], x=R["x"], csca=np.array(R["csca"]),
+                cabs=np.array(R["cabs"]), rcs=R["rcs"])
+    for d, m in enumerate(R["mesh"]):
+        for k, v in mesh_fields(m, False).items():
+            data["mesh%d_%s" % (d, k)] = v
+        data["mesh%d_bnd" % d] = m["bnd"]
+    if with_A:
+        data["A"] = R["A"]
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **data)
+    print(name, "N=%d nga=%d" % (R["q"].shape[0] // 2, R["q"].shape[1]),
+          "csca=%.6e cabs=%.6e" % (R["csca"], R["cabs"]))
+
+
+def main_symm():
+    """Fixtures of the mirror-symmetric solve:
+      octant_n4.npz  octant (x, y, z >= 0) of the n=4 octasphere, R=50 nm,
+                     group mxp myp mzp (8 representations), incidence
+                     (40, 25, 30) deg: A[60,60,8] before reduction, q, the
+                     expanded solutions x, csca, cabs, rcs 37x72, mesh
+                     arrays and edge classes
+      quarter_n4.npz the n=4 cell x, y >= 0 (both z halves), group myp mxp
+                     (4 representations, generators in that order): q, x,
+                     csca, cabs, rcs, mesh arrays and edge classes"""
+    os.makedirs(os.path.join(HERE, "_msh"), exist_ok=True)
+    symm_case("octant_n4", 4, (), ["mxp", "myp", "mzp"], (40.0, 25.0, 30.0), True)
+    symm_case("quarter_n4", 4, (2,), ["myp", "mxp"], (40.0, 25.0, 30.0), False)
+
+
 def main():
     if not refrun.available():
         raise SystemExit("oracle/_ref/mom_ref is not built (run oracle/build_ref.sh)")
@@ -103,4 +143,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["symm"]:
+        main_symm()
+    else:
+        main()
+        main_symm()

@@ -96,3 +96,39 @@ def run(mshfile, scale, domains, wl, ri, angles_deg, dump_ops=True, dump_sys=Tru
             k, v = ln.split()
             res["time_" + k] = float(v)
     return res
+
+
+REF_SYMM_BIN = os.path.join(ROOT, "oracle", "_ref", "mom_ref_symm")
+
+
+def run_symm(mshfile, scale, gens, sbnd, domains, wl, ri, angles_deg, ntheta=5, nphi=7, outdir=None, threads=None):
+    """Run the reference's mirror-symmetric set-up and linear solve
+    (oracle/_ref/mom_ref_symm): gens e.g. ["mxp", "myp", "mzp"], sbnd a list
+    of (line id, "xplane"|"yplane"|"zplane").  Returns the meshes with their
+    edge classes, A[2N,2N,nga] before reduction, q[2N,nga], the expanded
+    solutions x[2N,nga], csca, cabs and rcs."""
+    outdir = outdir or tempfile.mkdtemp(prefix="momrefs_")
+    lines = [mshfile, repr(float(scale)), " ".join([str(len(gens))] + list(gens)), str(len(sbnd))]
+    lines += ["%d %s" % (int(i), name) for i, name in sbnd]
+    lines += [str(len(domains))]
+    lines += [" ".join(str(v) for v in [len(d)] + list(d)) for d in domains]
+    lines += [repr(float(wl))]
+    lines += ["%r %r" % (complex(r).real, complex(r).imag) for r in ri]
+    lines += ["%r %r %r" % tuple(float(a) for a in angles_deg), "%d %d" % (ntheta, nphi), outdir]
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    subprocess.run([REF_SYMM_BIN], input="\n".join(lines) + "\n", text=True, check=True,
+                   stdout=subprocess.DEVNULL, env=env)
+    res = {"mesh": [read_mesh(os.path.join(outdir, "mesh_%d.bin" % d)) for d in range(len(domains) + 1)]}
+    for d in range(len(domains) + 1):
+        res["mesh"][d]["bnd"] = np.fromfile(os.path.join(outdir, "bnd_%d.bin" % d), np.int32)
+    N2 = 2 * res["mesh"][0]["enodes"].shape[0]
+    nga = 1 << len(gens)
+    res["A"] = np.fromfile(os.path.join(outdir, "A.bin"), np.complex128).reshape(N2, N2, nga, order="F")
+    res["q"] = np.fromfile(os.path.join(outdir, "q.bin"), np.complex128).reshape(N2, nga, order="F")
+    res["x"] = np.fromfile(os.path.join(outdir, "x.bin"), np.complex128).reshape(N2, nga, order="F")
+    post = np.fromfile(os.path.join(outdir, "post.bin"), np.float64)
+    res["csca"], res["cabs"] = post[0], post[1]
+    res["rcs"] = post[2:].reshape(ntheta, nphi, order="F")
+    return res

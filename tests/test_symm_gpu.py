@@ -1,0 +1,115 @@
+"""GPU mirror-symmetry decomposition against the reference's own symmetric
+run (tests/golden/octant_n4.npz: group mxp myp mzp, 8 representations;
+quarter_n4.npz: group myp mxp, 4 representations) and against the full
+sphere solved without symmetry.
+
+Tolerances: every representation's reduced matrix <= 1e-10 blockwise
+(the fill's bar); projected sources <= 1e-12; FP64 solutions <= 1e-10 and
+sigma <= 1e-10 relative; mixed precision: backward error <= 1e-12,
+solutions <= 1e-7, sigma <= 1e-8.  The octant of the n = 8 octasphere
+solved per representation against the closed octasphere solved whole:
+sigma and rcs <= 1e-9 relative (two exact decompositions of one
+discretisation; the difference is rounding through the two solves)."""
+import numpy as np
+import pytest
+
+from cases import blockwise_relerr, sphere_cell
+from conftest import golden
+from mom2amd import DEGTORAD, DeviceMesh, SymmetricSystems, System, cs_prtsrf, cs_prtsrf_symm, rcs, rcs_symm, srcvec_symm
+from mom2amd import build_problem, meshgen
+from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
+
+pytestmark = pytest.mark.gpu
+
+CELLS = {"octant_n4": (), "quarter_n4": (2,)}
+
+
+def _solve(ctx, case, gens, angles_deg, prec):
+    P = case["problem"]
+    N = P.nedges
+    dms = [DeviceMesh(ctx, d) for d in P.domains]
+    SS = SymmetricSystems(ctx, N, P.master.bnd, gens, prec)
+    SS.fill(dms, case["ri"], case["omega"])
+    mats = [SS.reduced_matrix(r) for r in range(SS.nga)] if prec == MOM_PREC_FP64 else None
+    SS.factor()
+    # degrees to radians with the reference's float32 pi (constants.f90)
+    ang = np.asarray(angles_deg, float) * DEGTORAD
+    q = srcvec_symm(ctx, dms[0], N, case["omega"], case["ri"][0], gens, ang)[:, :, 0]
+    x = SS.solve(q)[:, :, 0]
+    return SS, dms, q, x, mats, ang
+
+
+@pytest.mark.parametrize("name", list(CELLS))
+@pytest.mark.parametrize("prec", [MOM_PREC_FP64, MOM_PREC_MIXED], ids=["fp64", "mixed"])
+def test_symmetric_solve_matches_reference(gpu_ctx, msh_dir, name, prec):
+    g = golden(name)
+    gens = [str(s) for s in g["gens"]]
+    case = sphere_cell(msh_dir, int(g["n"]), CELLS[name])
+    N = case["problem"].nedges
+    print("case %s, N=%d, group %s, precision %d" % (name, N, gens, prec), flush=True)
+    SS, dms, q, x, mats, ang = _solve(gpu_ctx, case, gens, g["angles_deg"], prec)
+    try:
+        qerr = np.abs(q - g["q"]).max() / np.abs(g["q"]).max()
+        print("q rel err %.2e" % qerr, flush=True)
+        assert qerr < 1e-12
+        if mats is not None and "A" in g:
+            for r in range(SS.nga):
+                k = SS.keep[r]
+                ref = g["A"][:, :, r][np.ix_(k, k)]
+                err = np.abs(mats[r] - ref).max() / np.abs(ref).max()
+                print("rep %d: dim %d, matrix rel err %.2e" % (r, SS.dims[r], err), flush=True)
+                assert err < 1e-10
+        xerr = np.abs(x - g["x"]).max() / np.abs(g["x"]).max()
+        print("x rel err %.2e, berr %s" % (xerr, ["%.1e" % b for b in SS.berr]), flush=True)
+        assert np.all(x[g["x"] == 0] == 0)
+        assert xerr < (1e-10 if prec == MOM_PREC_FP64 else 1e-7)
+        if prec == MOM_PREC_MIXED:
+            assert max(SS.berr) <= 1e-12
+        cs, ca = cs_prtsrf_symm(gpu_ctx, dms[0], N, case["omega"], case["ri"][0], gens, x, *ang)
+        tol = 1e-10 if prec == MOM_PREC_FP64 else 1e-8
+        print("csca %.12e (ref %.12e), cabs %.12e (ref %.12e)" % (cs, g["csca"], ca, g["cabs"]), flush=True)
+        assert abs(cs / g["csca"] - 1) < tol and abs(ca / g["cabs"] - 1) < tol
+        R = rcs_symm(gpu_ctx, dms[0], N, case["omega"], case["ri"][0], gens, x, 37, 72)
+        rerr = np.abs(R - g["rcs"]).max() / np.abs(g["rcs"]).max()
+        print("rcs rel err %.2e" % rerr, flush=True)
+        assert rerr < tol
+    finally:
+        SS.close()
+        for d in dms:
+            d.close()
+
+
+def test_octant_equals_full_sphere(gpu_ctx, msh_dir):
+    n = 8
+    case = sphere_cell(msh_dir, n, ())
+    gens = ["mxp", "myp", "mzp"]
+    angles = (40.0, 25.0, 30.0)
+    SS, dms, q, x, _, ang = _solve(gpu_ctx, case, gens, angles, MOM_PREC_FP64)
+    N = case["problem"].nedges
+    try:
+        cs, ca = cs_prtsrf_symm(gpu_ctx, dms[0], N, case["omega"], case["ri"][0], gens, x, *ang)
+        R = rcs_symm(gpu_ctx, dms[0], N, case["omega"], case["ri"][0], gens, x, 19, 36)
+    finally:
+        SS.close()
+        for d in dms:
+            d.close()
+    fn, ff = meshgen.octasphere(n, 50.0)
+    P = build_problem(fn, ff, np.full(ff.shape[0], 15), 1e-9, [[15], [-15]])
+    Nf = P.nedges
+    fdms = [DeviceMesh(gpu_ctx, d) for d in P.domains]
+    S = System(gpu_ctx, Nf, MOM_PREC_FP64)
+    try:
+        S.fill(fdms, case["ri"], case["omega"])
+        S.factor()
+        xf = S.solve_pw(fdms[0], case["omega"], case["ri"][0], ang)[:, 0]
+        csf, caf = cs_prtsrf(gpu_ctx, fdms[0], Nf, case["omega"], case["ri"][0], xf, *ang)
+        Rf = rcs(gpu_ctx, fdms[0], Nf, case["omega"], case["ri"][0], xf, 19, 36)
+    finally:
+        S.close()
+        for d in fdms:
+            d.close()
+    rerr = np.abs(R - Rf).max() / np.abs(Rf).max()
+    print("octant N=%d x 8 reps vs sphere N=%d: csca %.12e / %.12e, cabs %.12e / %.12e, rcs rel diff %.2e"
+          % (N, Nf, cs, csf, ca, caf, rerr), flush=True)
+    assert abs(cs / csf - 1) < 1e-9 and abs(ca / caf - 1) < 1e-9
+    assert rerr < 1e-9
