@@ -395,6 +395,99 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_symmetric(args):
+    """Mirror symmetry: the sphere (R = 50 nm, eps_in = -9.4+1.5i, 600 nm)
+    meshed as an octasphere of 8 n^2 faces, n = 50 (2E = 60000 unknowns for
+    the closed surface), solved on its octant cell with the group mxp myp
+    mzp: 8 representations of about 7500 unknowns each.  One step = the
+    fill of all representations (each image integrated once), their LU
+    factorisations, the plane-wave sources projected on the representations,
+    the solves and cs_prtsrf over the images, in the mixed path.  The closed
+    octasphere solved whole (mixed path, once, not in value) gives the
+    comparison time and the sigma agreement."""
+    from mom2amd import (Context, DeviceMesh, DEGTORAD, SymmetricSystems, System, build_problem, cs_prtsrf,
+                         cs_prtsrf_symm, load_problem, meshgen, omega_of, srcvec_symm)
+    from mom2amd.capi import MOM_PREC_MIXED
+
+    n = args.cell_n
+    nodes, faces, lines, lids = meshgen.sphere_cell(n, 50.0, ())
+    d = tempfile.mkdtemp(prefix="mom_bench_")
+    path = os.path.join(d, "octant%d.msh" % n)
+    meshgen.write_gmsh(path, nodes, faces, 15, lines, lids)
+    sbnd = [(meshgen.OCTANT_LINE_IDS[c], c) for c in ("xplane", "yplane", "zplane")]
+    prob = load_problem(path, 1e-9, [[15], [-15]], sbnd=sbnd)
+    N = prob.nedges
+    omega = omega_of(600e-9)
+    ri = [1.0 + 0j, np.sqrt(-9.4 + 1.5j)]
+    ang = np.array([40.0, 25.0, 30.0]) * DEGTORAD
+    gens = ["mxp", "myp", "mzp"]
+    ctx = Context(0)
+    dms = [DeviceMesh(ctx, dm) for dm in prob.domains]
+    log("octant cell n=%d: %d faces, %d edges (%d on the planes); closed sphere %d faces, %d unknowns"
+        % (n, prob.master.nfaces, N, int((prob.master.bnd > 1).sum()), 8 * n * n, 24 * n * n))
+
+    def step():
+        t0 = time.perf_counter()
+        SS = SymmetricSystems(ctx, N, prob.master.bnd, gens, MOM_PREC_MIXED)
+        try:
+            SS.fill(dms, ri, omega)
+            t1 = time.perf_counter()
+            SS.factor()
+            t2 = time.perf_counter()
+            q = srcvec_symm(ctx, dms[0], N, omega, ri[0], gens, ang)[:, :, 0]
+            x = SS.solve(q)[:, :, 0]
+            cs = cs_prtsrf_symm(ctx, dms[0], N, omega, ri[0], gens, x, *ang)
+            t3 = time.perf_counter()
+            return dict(wall=t3 - t0, fill=t1 - t0, factor=t2 - t1, solve=t3 - t2, cs=cs, dims=list(SS.dims),
+                        berr=max(SS.berr))
+        finally:
+            SS.close()
+
+    for i in range(args.warmup):
+        w = step()
+        log("warmup %d: %.3f s" % (i, w["wall"]))
+    runs = []
+    for i in range(args.steps):
+        r = step()
+        runs.append(r)
+        log("step %d: %.3f s (fill %.3f, factor %.3f, sources+solve+cs %.3f); csca %.9e cabs %.9e, berr %.1e"
+            % (i, r["wall"], r["fill"], r["factor"], r["solve"], r["cs"][0], r["cs"][1], r["berr"]))
+    mean = float(np.mean([r["wall"] for r in runs]))
+    out = {"metric": "fill+LU+solve s, octant of a %d-unknown sphere by mirror symmetry (mixed path)" % (24 * n * n),
+           "value": round(mean, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(mean * 1e3, 2), "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": None, "dtype": "fp32-mfma LU + fp64 fill/residual", "data": "synthetic octasphere",
+           "config": {"workload": "sphere R=50nm as octasphere n=%d (%d faces), octant cell %d faces, group mxp myp "
+                                  "mzp (8 representations of %s unknowns), 600 nm, eps_in=-9.4+1.5i"
+                                  % (n, 8 * n * n, prob.master.nfaces, runs[-1]["dims"]),
+                      "n_unknowns_closed": 24 * n * n},
+           "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fill", "factor", "solve")},
+           "csca_m2": runs[-1]["cs"][0], "cabs_m2": runs[-1]["cs"][1]}
+    if not args.no_fp64:
+        fn, ff = meshgen.octasphere(n, 50.0)
+        P = build_problem(fn, ff, np.full(ff.shape[0], 15), 1e-9, [[15], [-15]])
+        fdms = [DeviceMesh(ctx, dm) for dm in P.domains]
+        t0 = time.perf_counter()
+        S = System(ctx, P.nedges, MOM_PREC_MIXED)
+        try:
+            S.fill(fdms, ri, omega)
+            S.factor()
+            xf = S.solve_pw(fdms[0], omega, ri[0], ang)[:, 0]
+            csf = cs_prtsrf(ctx, fdms[0], P.nedges, omega, ri[0], xf, *ang)
+        finally:
+            S.close()
+        tf = time.perf_counter() - t0
+        for dm in fdms:
+            dm.close()
+        cs = runs[-1]["cs"]
+        out["closed_sphere"] = {"seconds": round(tf, 4), "n_unknowns": 2 * P.nedges, "speedup": round(tf / mean, 2),
+                                "csca_relerr": abs(cs[0] / csf[0] - 1), "cabs_relerr": abs(cs[1] / csf[1] - 1)}
+        log("closed octasphere (N=%d, mixed path): %.3f s, %.1fx the symmetric solve; sigma rel diff %.1e / %.1e"
+            % (2 * P.nedges, tf, tf / mean, out["closed_sphere"]["csca_relerr"], out["closed_sphere"]["cabs_relerr"]))
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -403,11 +496,12 @@ def main():
     ap.add_argument("--subdiv", type=int, default=5, help="icosphere subdivision (5: N = 61440)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", choices=("pmchwt", "sh", "spectrum", "adjoint"), default="pmchwt",
+    ap.add_argument("--workload", choices=("pmchwt", "sh", "spectrum", "adjoint", "symmetric"), default="pmchwt",
                     help="pmchwt: the sphere solve (default); sh: the second-harmonic dimer chain; spectrum: the "
                          "ellipsoid over many wavelengths with 256 plane waves each; adjoint: sigma_abs and its "
                          "permittivity gradient on a perturbed sphere (sh, spectrum and adjoint use s=4 unless "
-                         "--subdiv is given)")
+                         "--subdiv is given); symmetric: the octant of an octasphere by mirror symmetry")
+    ap.add_argument("--cell-n", type=int, default=50, help="octasphere divisions of the symmetric workload")
     ap.add_argument("--nwl", type=int, default=64, help="wavelengths of the spectrum workload")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -421,6 +515,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "symmetric":
+        bench_symmetric(args)
         return
 
     from mom2amd import Context, DeviceMesh

@@ -1,5 +1,6 @@
 // Problem set-up on the host: master mesh, domain meshes and the RWG edge
-// numbering, for the identity group without periodic boundaries.
+// numbering without periodic boundaries, and the classification of edges
+// on mirror planes by the mesh's line elements (the `sbnd` command).
 //
 // Reference algorithm (for90-mom2): load_mesh_gmsh (mesh.f90:610-736),
 // build_mesh (mesh.f90:1044-1137),

@@ -319,6 +319,116 @@ MODULE mom_gpu
        TYPE(C_PTR) :: mesh
        INTEGER(C_INT) :: st
      END FUNCTION mom_problem_mesh_create
+
+     ! Mirror symmetry (the `sbnd` and `symm` commands; planes 1 = mxp,
+     ! 2 = myp, 3 = mzp; boundary classes 2/3/4 = xplane/yplane/zplane)
+     FUNCTION mom_problem_set_lines(prob, nlines, nodes, ids) BIND(C, NAME='mom_problem_set_lines') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR
+       TYPE(C_PTR), VALUE :: prob
+       INTEGER(C_INT64_T), VALUE :: nlines
+       INTEGER(C_INT32_T), DIMENSION(*), INTENT(IN) :: nodes, ids
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_set_lines
+
+     FUNCTION mom_problem_classify_edges(prob, id, bnd, nclassified) &
+          BIND(C, NAME='mom_problem_classify_edges') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR
+       TYPE(C_PTR), VALUE :: prob
+       INTEGER(C_INT32_T), VALUE :: id, bnd
+       INTEGER(C_INT64_T) :: nclassified
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_classify_edges
+
+     FUNCTION mom_problem_edge_bnd(prob, dom, bnd) BIND(C, NAME='mom_problem_edge_bnd') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_PTR
+       TYPE(C_PTR), VALUE :: prob
+       INTEGER(C_INT32_T), VALUE :: dom
+       INTEGER(C_INT32_T), DIMENSION(*) :: bnd
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_problem_edge_bnd
+
+     FUNCTION mom_symm_group(ngen, planes, nga, ef, detj, jdiag, id) BIND(C, NAME='mom_symm_group') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+       INTEGER(C_INT32_T), VALUE :: ngen
+       INTEGER(C_INT32_T), DIMENSION(*), INTENT(IN) :: planes
+       INTEGER(C_INT32_T) :: nga
+       REAL(C_DOUBLE), DIMENSION(*) :: ef, detj, jdiag
+       INTEGER(C_INT32_T), DIMENSION(*) :: id
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_symm_group
+
+     FUNCTION mom_symm_edge_bc(nedges_total, bnd, ngen, planes, rep, id, dim) &
+          BIND(C, NAME='mom_symm_edge_bc') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       INTEGER(C_INT32_T), DIMENSION(*), INTENT(IN) :: bnd, planes
+       INTEGER(C_INT32_T), VALUE :: ngen, rep
+       INTEGER(C_INT32_T), DIMENSION(*) :: id
+       INTEGER(C_INT64_T) :: dim
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_symm_edge_bc
+
+     FUNCTION mom_system_create_order(ctx, order, precision, sys) BIND(C, NAME='mom_system_create_order') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR
+       TYPE(C_PTR), VALUE :: ctx
+       INTEGER(C_INT64_T), VALUE :: order
+       INTEGER(C_INT32_T), VALUE :: precision
+       TYPE(C_PTR) :: sys
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_create_order
+
+     FUNCTION mom_system_fill_symm(nrep, systems, ndom, doms, ri, nedges_total, bnd, ngen, planes, omega, &
+          nq, qw, qnodes) BIND(C, NAME='mom_system_fill_symm') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE
+       INTEGER(C_INT32_T), VALUE :: nrep, ndom, ngen, nq
+       TYPE(C_PTR), DIMENSION(*), INTENT(IN) :: systems, doms
+       REAL(C_DOUBLE), DIMENSION(*), INTENT(IN) :: ri, qw, qnodes
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       INTEGER(C_INT32_T), DIMENSION(*), INTENT(IN) :: bnd, planes
+       REAL(C_DOUBLE), VALUE :: omega
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_system_fill_symm
+
+     FUNCTION mom_srcvec_pw_symm(ctx, mesh, nedges_total, omega, ri_re, ri_im, ngen, planes, nsrc, angles, &
+          nq, qw, qnodes, q, ldq) BIND(C, NAME='mom_srcvec_pw_symm') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE, C_DOUBLE_COMPLEX
+       TYPE(C_PTR), VALUE :: ctx, mesh
+       INTEGER(C_INT64_T), VALUE :: nedges_total, nsrc, ldq
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       INTEGER(C_INT32_T), VALUE :: ngen, nq
+       INTEGER(C_INT32_T), DIMENSION(*), INTENT(IN) :: planes
+       REAL(C_DOUBLE), DIMENSION(*), INTENT(IN) :: angles, qw, qnodes
+       COMPLEX(C_DOUBLE_COMPLEX), DIMENSION(*) :: q
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_srcvec_pw_symm
+
+     FUNCTION mom_cs_prtsrf_symm(ctx, mesh, nedges_total, omega, ri_re, ri_im, ngen, planes, x, theta, phi, psi, &
+          nq, qw, qnodes, csca, cabs) BIND(C, NAME='mom_cs_prtsrf_symm') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE, C_DOUBLE_COMPLEX
+       TYPE(C_PTR), VALUE :: ctx, mesh
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im, theta, phi, psi
+       INTEGER(C_INT32_T), VALUE :: ngen, nq
+       INTEGER(C_INT32_T), DIMENSION(*), INTENT(IN) :: planes
+       COMPLEX(C_DOUBLE_COMPLEX), DIMENSION(*), INTENT(IN) :: x
+       REAL(C_DOUBLE), DIMENSION(*), INTENT(IN) :: qw, qnodes
+       REAL(C_DOUBLE) :: csca, cabs
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_cs_prtsrf_symm
+
+     FUNCTION mom_rcs_ext_symm(ctx, mesh, nedges_total, omega, ri_re, ri_im, ngen, planes, x, ntheta, nphi, &
+          filter, fdir, nq, qw, qnodes, rcs) BIND(C, NAME='mom_rcs_ext_symm') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE, C_DOUBLE_COMPLEX
+       TYPE(C_PTR), VALUE :: ctx, mesh
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       INTEGER(C_INT32_T), VALUE :: ngen, ntheta, nphi, filter, nq
+       INTEGER(C_INT32_T), DIMENSION(*), INTENT(IN) :: planes
+       COMPLEX(C_DOUBLE_COMPLEX), DIMENSION(*), INTENT(IN) :: x
+       REAL(C_DOUBLE), DIMENSION(*), INTENT(IN) :: fdir, qw, qnodes
+       REAL(C_DOUBLE), DIMENSION(*) :: rcs
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_rcs_ext_symm
   END INTERFACE
 
 CONTAINS
