@@ -321,7 +321,7 @@ def bench_adjoint(args):
     domain), in the mixed path; central differences of the FP64 solve check
     the gradient once."""
     from mom2amd import (Context, DeviceMesh, System, cs_prtsrf, load_problem, meshgen, omega_of,
-                         sigma_abs_eps_gradient)
+                         sigma_abs_eps_gradient, sigma_abs_vertex_gradient)
     from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
 
     s = args.subdiv if args.subdiv != 5 else 4
@@ -376,7 +376,46 @@ def bench_adjoint(args):
                                   "adjoint" % (s, prob.master.nfaces, 2 * N), "n_unknowns": 2 * N},
            "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("forward", "adjoint")},
            "sigma_abs_m2": runs[-1]["sab"], "dsigma_deps": list(runs[-1]["grad"])}
+    # the node-coordinate gradient of sigma_abs (all 3V coordinates), reported beside the value
+    S, x, ri = forward(eps, MOM_PREC_MIXED)
+    try:
+        t0 = time.perf_counter()
+        sab_v, gv = sigma_abs_vertex_gradient(ctx, S, x, dms, prob, omega, ri, ang)
+        tv = time.perf_counter() - t0
+    finally:
+        S.close()
+    out["vertex_gradient"] = {"seconds": round(tv, 3), "n_coordinates": int(gv.size),
+                              "max_abs_m": float(np.abs(gv).max())}
+    log("vertex gradient (%d node coordinates, one adjoint solve + pair kernel): %.3f s" % (gv.size, tv))
     if not args.no_fp64:
+        # central differences of the whole FP64 solve in the two largest components
+        flat = np.argsort(np.abs(gv).ravel())[::-1][:2]
+        fdv = []
+        for idx in flat:
+            v, c = divmod(int(idx), 3)
+            vals = []
+            for sgn in (1.0, -1.0):
+                pn = nodes.copy()
+                pn[v, c] += sgn * 1e-3
+                path2 = os.path.join(d, "pert_fd.msh")
+                meshgen.write_gmsh(path2, pn, faces, 15)
+                p2 = load_problem(path2, 1e-9, [[15], [-15]])
+                dms2 = [DeviceMesh(ctx, dm) for dm in p2.domains]
+                S2 = System(ctx, N, MOM_PREC_FP64)
+                try:
+                    S2.fill(dms2, [1.0 + 0j, np.sqrt(eps)], omega)
+                    S2.factor()
+                    x2 = S2.solve_pw(dms2[0], omega, 1.0, ang)[:, 0]
+                    vals.append(cs_prtsrf(ctx, dms2[0], N, omega, 1.0, x2, *ang)[1])
+                finally:
+                    S2.close()
+                    for dm in dms2:
+                        dm.close()
+            fd_v = (vals[0] - vals[1]) / (2e-3 * 1e-9)
+            fdv.append({"node": v, "coord": c, "adjoint": float(gv[v, c]), "fd": fd_v,
+                        "relerr": abs(gv[v, c] - fd_v) / abs(fd_v)})
+            log("vertex %d coord %d: adjoint %.8e fd %.8e" % (v, c, gv[v, c], fd_v))
+        out["vertex_gradient"]["fd_check"] = fdv
         fd = []
         for hdir in (1.0, 1j):
             h = 1e-4 * abs(eps) * hdir

@@ -272,4 +272,61 @@ int mom_pmchwt_deps_apply(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_tot
   return 0;
 }
 
+
+// The vertex-coordinate terms of an adjoint sensitivity: dvert[3V] (host,
+// this mesh's nodes) += Re d/dP (lambda^T Z_mesh x) for the mesh's blocks of
+// the PMCHWT matrix (mom_pmchwt_vertex_adjoint), or += d/dP (sigma_abs at
+// fixed x + Re lambda^T q) for the exterior mesh and one plane wave
+// (mom_cs_abs_vertex_explicit).
+static int vertex_common(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, const double *x, const double *lam,
+                         double *dvert, int32_t nq, const double *qw, const double *qnodes, QuadRule &rule, DevBuf &dx,
+                         DevBuf &dl, DevBuf &dv) {
+  if (!ctx || !mesh || !x || !lam || !dvert) return fail(MOM_ERR_ARG, "NULL argument");
+  if (mesh->ctx != ctx) return fail(MOM_ERR_ARG, "mesh belongs to another context");
+  if (nedges_total < 1 || mesh->max_parent >= nedges_total)
+    return fail(MOM_ERR_ARG, "domain edge parent index >= nedges_total");
+  (void)hipSetDevice(ctx->device);
+  MOM_TRY(make_rule(nq, qw, qnodes, &rule));
+  int rc;
+  if ((rc = dx.alloc(sizeof(double2) * 2 * nedges_total)) || (rc = dl.alloc(sizeof(double2) * 2 * nedges_total)) ||
+      (rc = dv.alloc(sizeof(double) * 3 * mesh->V)))
+    return rc;
+  MOM_TRY(copy_h2d(ctx, dx.ptr, x, sizeof(double2) * 2 * nedges_total));
+  MOM_TRY(copy_h2d(ctx, dl.ptr, lam, sizeof(double2) * 2 * nedges_total));
+  MOM_HIP(hipMemsetAsync(dv.ptr, 0, dv.bytes, ctx->stream));
+  return 0;
+}
+
+static int vertex_finish(mom_ctx *ctx, const mom_mesh *mesh, DevBuf &dv, double *dvert) {
+  std::vector<double> h(3 * mesh->V);
+  MOM_TRY(copy_d2h(ctx, h.data(), dv.ptr, sizeof(double) * h.size()));
+  for (size_t i = 0; i < h.size(); i++) dvert[i] += h[i];
+  return 0;
+}
+
+int mom_pmchwt_vertex_adjoint(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                              double ri_im, int32_t nq, const double *qw, const double *qnodes, const double *x,
+                              const double *lam, double *dvert) {
+  if (!(omega > 0)) return fail(MOM_ERR_ARG, "omega must be > 0");
+  QuadRule rule;
+  DevBuf dx, dl, dv;
+  MOM_TRY(vertex_common(ctx, mesh, nedges_total, x, lam, dvert, nq, qw, qnodes, rule, dx, dl, dv));
+  MOM_TRY(vertex_adjoint_fill(ctx, mesh, rule, omega, cd(ri_re, ri_im), nedges_total, dx.as<double2>(),
+                              dl.as<double2>(), dv.as<double>()));
+  return vertex_finish(ctx, mesh, dv, dvert);
+}
+
+int mom_cs_abs_vertex_explicit(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                               double ri_im, const double *x, const double *lam, double theta, double phi, double psi,
+                               int32_t nq, const double *qw, const double *qnodes, double *dvert) {
+  if (!(omega > 0)) return fail(MOM_ERR_ARG, "omega must be > 0");
+  if (!(ri_re > 0)) return fail(MOM_ERR_ARG, "Re(ri) must be > 0");
+  QuadRule rule;
+  DevBuf dx, dl, dv;
+  MOM_TRY(vertex_common(ctx, mesh, nedges_total, x, lam, dvert, nq, qw, qnodes, rule, dx, dl, dv));
+  MOM_TRY(vertex_adjoint_faces(ctx, mesh, rule, omega, cd(ri_re, ri_im), nedges_total, dx.as<double2>(),
+                               dl.as<double2>(), theta, phi, psi, dv.as<double>()));
+  return vertex_finish(ctx, mesh, dv, dvert);
+}
+
 }  // extern "C"

@@ -1451,4 +1451,534 @@ int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, c
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Vertex-coordinate adjoint of the fill and of sigma_abs (no counterpart in
+// the reference; the sensitivities of sysmat.f90:18-105, source.f90:136-247
+// and cs.f90:14-104 with respect to the mesh nodes).
+//
+// With x the solution and lambda = Z^-T g (g = dsigma/dx), the derivative
+// of sigma with respect to a node coordinate P is
+//   dsigma/dP = dsigma/dP|_x + Re(lambda^T dq/dP) - Re(lambda^T (dZ/dP) x).
+// Every term is a sum over faces or face pairs whose value depends only on
+// the nodes of those faces.  The kernels below evaluate, per unordered face
+// pair of one domain, Phi = sum_pq a_pq (D+H)_pq + b_pq K_pq over both
+// orders of the pair (a = lambda_J x_J + Zsi lambda_M x_M, b = lambda_M x_J
+// - lambda_J x_M at the pair's edges), rebuilt from the node positions with
+// the fill's formulas (point pairs with G or the smoothed kernels, the
+// analytic near parts, the self-pair rule), and differentiate it in each
+// coordinate of each distinct node of the pair by the fourth-order central
+// difference [8(f(h) - f(-h)) - (f(2h) - f(-2h))] / 12h with h = 1e-3 of the
+// pair's mean edge length (truncation near 1e-12, rounding near 1e-10
+// relative).  The near/far classification is frozen at the unperturbed
+// mesh, as the reference's is piecewise constant in the nodes.
+struct VFace {
+  double P[3][3];   // nodes
+  double sg[3];     // sign of each side's RWG function
+  int lv[3];        // local index of each side's free vertex
+};
+
+// One face's RWG data from its nodes (compute_basis_data, mesh.f90:1265-1322;
+// rwgf.f90:34-78): area, coef = s L/(2A), div = s L/A, free vertex.
+__device__ __forceinline__ void vface_rwg(const VFace &f, double &A, double coef[3], double divv[3],
+                                          double v[3][3]) {
+  double a[3], b[3], c[3];
+  for (int k = 0; k < 3; k++) {
+    a[k] = f.P[2][k] - f.P[0][k];
+    b[k] = f.P[2][k] - f.P[1][k];
+  }
+  cross3(a, b, c);
+  A = 0.5 * norm3(c);
+  for (int p = 0; p < 3; p++) {
+    double e[3];
+    for (int k = 0; k < 3; k++) e[k] = f.P[(p + 1) % 3][k] - f.P[p][k];
+    const double L = norm3(e);
+    coef[p] = f.sg[p] * (L / (2.0 * A));
+    divv[p] = f.sg[p] * L / A;
+    for (int k = 0; k < 3; k++) v[p][k] = f.P[f.lv[p]][k];
+  }
+}
+
+__device__ __forceinline__ void vface_point(const VFace &f, const QuadRule &rule, int t, double r[3]) {
+  const int Q = rule.Q;
+  for (int k = 0; k < 3; k++)
+    r[k] = rule.nodes[2 * Q + t] * f.P[0][k] + rule.nodes[t] * f.P[1][k] + rule.nodes[Q + t] * f.P[2][k];
+}
+
+__device__ void vface_near(const VFace &f, NearFace &o) {
+  double a[3], b[3], c[3];
+  for (int v = 0; v < 3; v++)
+    for (int k = 0; k < 3; k++) o.p[v][k] = f.P[v][k];
+  for (int k = 0; k < 3; k++) {
+    a[k] = f.P[1][k] - f.P[0][k];
+    b[k] = f.P[2][k] - f.P[0][k];
+  }
+  cross3(a, b, c);
+  const double nc = norm3(c);
+  for (int k = 0; k < 3; k++) o.n[k] = c[k] / nc;
+  for (int e = 0; e < 3; e++) {
+    double d[3];
+    for (int k = 0; k < 3; k++) d[k] = f.P[(e + 1) % 3][k] - f.P[e][k];
+    const double l = norm3(d);
+    for (int k = 0; k < 3; k++) o.s[e][k] = d[k] / l;
+    cross3(o.s[e], o.n, c);
+    const double lm = norm3(c);
+    for (int k = 0; k < 3; k++) o.m[e][k] = c[k] / lm;
+  }
+}
+
+struct VPairArgs {
+  int F;
+  QuadRule rule;
+  const double *p;     // nodes [V][3]
+  const int *fnodes;   // [F][3]
+  FaceSide side;       // global indices, coefficient signs, free vertices
+  const double *cp;    // centroids (classification)
+  double thr2;
+  double kr, ki;
+  double2 c1, c2, zsi, kk4, kk8;
+  const double2 *x, *lam;  // [2N]
+  int64_t N;
+  double *dvert;       // [V][3] accumulated
+};
+
+__device__ __forceinline__ double2 cm(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Sum over the analytic near parts of test face ft against source face fs
+// (near_fill_kernel's terms) contracted with a[3][3], b[3][3]: per test
+// side p and source side q, a (c1 D + c2 H) + b K.
+__device__ double2 vnear_part(const VPairArgs &g, const VFace &ft, const VFace &fs, bool self, const double2 a[3][3],
+                              const double2 b[3][3]) {
+  NearFace f;
+  vface_near(fs, f);
+  double At, ct[3], dt[3], vt[3][3], As, cs[3], ds[3], vs[3][3];
+  vface_rwg(ft, At, ct, dt, vt);
+  vface_rwg(fs, As, cs, ds, vs);
+  double2 acc = make_double2(0.0, 0.0);
+  for (int i = 0; i < g.rule.Q; i++) {
+    double r[3];
+    vface_point(ft, g.rule, i, r);
+    PointInts P;
+    d_point_ints(r, f, P);
+    double k31[3], k3m1[3], rho[3];
+    for (int c = 0; c < 3; c++) {
+      double sm = f.m[0][c] * P.l1[0] + f.m[1][c] * P.l1[1] + f.m[2][c] * P.l1[2];
+      k31[c] = (P.h == 0) ? sm : sm - P.h * f.n[c] * P.sm1;
+      k3m1[c] = f.m[0][c] * P.lm1[0] + f.m[1][c] * P.lm1[1] + f.m[2][c] * P.lm1[2] + f.n[c] * P.sm3h;
+      rho[c] = r[c] - P.h * f.n[c];
+    }
+    double2 s = make_double2(0.0, 0.0);
+    for (int q = 0; q < 3; q++) {
+      const double h1 = ds[q] * P.sm1, h0 = ds[q] * P.s1;
+      const double2 aH = make_double2(g.kk4.x * h1 - g.kk8.x * h0, -(g.kk8.y * h0));
+      double2 aD[3], aK[3];
+      for (int c = 0; c < 3; c++) {
+        const double k2m1 = cs[q] * (f.m[0][c] * P.l1[0] + f.m[1][c] * P.l1[1] + f.m[2][c] * P.l1[2] +
+                                     (rho[c] - vs[q][c]) * P.sm1);
+        const double k21 = cs[q] * (1.0 / 3.0 * (f.m[0][c] * P.l3[0] + f.m[1][c] * P.l3[1] + f.m[2][c] * P.l3[2]) +
+                                    (rho[c] - vs[q][c]) * P.s1);
+        aD[c] = make_double2(g.kk4.x * k2m1 - g.kk8.x * k21, -(g.kk8.y * k21));
+        aK[c] = make_double2(0.0, 0.0);
+      }
+      if (!self) {
+        double d[3] = {r[0] - vs[q][0], r[1] - vs[q][1], r[2] - vs[q][2]}, c1v[3], c0v[3];
+        cross3(d, k3m1, c1v);
+        cross3(d, k31, c0v);
+        for (int c = 0; c < 3; c++) {
+          const double k4m1 = -cs[q] * c1v[c], k41 = -cs[q] * c0v[c];
+          aK[c] = make_double2(g.kk4.x * k4m1 - g.kk8.x * k41, -(g.kk8.y * k41));
+        }
+      }
+      for (int p = 0; p < 3; p++) {
+        double fm[3];
+        for (int c = 0; c < 3; c++) fm[c] = ct[p] * (r[c] - vt[p][c]);
+        double2 dd = make_double2(0.0, 0.0), kk = make_double2(0.0, 0.0);
+        for (int c = 0; c < 3; c++) {
+          dd.x += fm[c] * aD[c].x;
+          dd.y += fm[c] * aD[c].y;
+          kk.x += fm[c] * aK[c].x;
+          kk.y += fm[c] * aK[c].y;
+        }
+        const double2 Dv = cm(g.c1, dd);
+        const double2 Hv = cm(g.c2, make_double2(dt[p] * aH.x, dt[p] * aH.y));
+        const double2 t1 = cm(a[p][q], make_double2(Dv.x + Hv.x, Dv.y + Hv.y)), t2 = cm(b[p][q], kk);
+        s.x += t1.x + t2.x;
+        s.y += t1.y + t2.y;
+      }
+    }
+    acc.x += g.rule.w[i] * s.x * At;
+    acc.y += g.rule.w[i] * s.y * At;
+  }
+  return acc;
+}
+
+// The point-pair sums of the unordered pair (ft, fs) with the coefficients of
+// both orders combined (a[p][q] for test side p of ft, source side q of fs).
+__device__ double2 vfar_part(const VPairArgs &g, const VFace &ft, const VFace &fs, bool near, bool self,
+                             const double2 a[3][3], const double2 b[3][3]) {
+  double At, ct[3], dt[3], vt[3][3], As, cs[3], ds[3], vs[3][3];
+  vface_rwg(ft, At, ct, dt, vt);
+  vface_rwg(fs, As, cs, ds, vs);
+  double2 T2 = make_double2(0.0, 0.0);
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) {
+      T2.x += a[p][q].x * dt[p] * ds[q];
+      T2.y += a[p][q].y * dt[p] * ds[q];
+    }
+  const double2 c2T2 = cm(g.c2, T2);
+  const double kr = g.kr, ki = g.ki, k2r = kr * kr - ki * ki, k2i = 2.0 * kr * ki;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int i = 0; i < g.rule.Q; i++) {
+    double r[3];
+    vface_point(ft, g.rule, i, r);
+    double u[3][3];
+    for (int p = 0; p < 3; p++)
+      for (int c = 0; c < 3; c++) u[p][c] = ct[p] * (r[c] - vt[p][c]);
+    double2 X[3][3], Z[3][3];  // X_q = sum_p a_pq u_p, Z_q = sum_p b_pq u_p
+    for (int q = 0; q < 3; q++)
+      for (int c = 0; c < 3; c++) {
+        X[q][c] = make_double2(0.0, 0.0);
+        Z[q][c] = make_double2(0.0, 0.0);
+        for (int p = 0; p < 3; p++) {
+          X[q][c].x += a[p][q].x * u[p][c];
+          X[q][c].y += a[p][q].y * u[p][c];
+          Z[q][c].x += b[p][q].x * u[p][c];
+          Z[q][c].y += b[p][q].y * u[p][c];
+        }
+      }
+    double2 si = make_double2(0.0, 0.0);
+    for (int j = 0; j < g.rule.Q; j++) {
+      double rs[3];
+      vface_point(fs, g.rule, j, rs);
+      const double d[3] = {r[0] - rs[0], r[1] - rs[1], r[2] - rs[2]};
+      const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double Gr, Gi, gr, gi;
+      if (!near) {
+        const double iR = rsqrt(R2), R = R2 * iR;
+        double Er, Ei;
+        cis_exp(-ki * R, kr * R, Er, Ei);
+        const double e = iR * kOofpi;
+        Gr = e * Er;
+        Gi = e * Ei;
+        const double ar = iR * iR + ki * iR, ai = -kr * iR;
+        gr = Gr * ar - Gi * ai;
+        gi = Gr * ai + Gi * ar;
+      } else if (R2 == 0.0) {
+        Gr = -ki * kOofpi;
+        Gi = kr * kOofpi;
+        gr = 0.0;
+        gi = 0.0;
+      } else {
+        const double iR = rsqrt(R2), R = R2 * iR;
+        double Er, Ei;
+        cis_exp(-ki * R, kr * R, Er, Ei);
+        Gr = ((Er - 1.0) * iR + 0.5 * k2r * R) * kOofpi;
+        Gi = (Ei * iR + 0.5 * k2i * R) * kOofpi;
+        const double iR2 = iR * iR, iR3 = iR2 * iR;
+        const double ar = iR3 + ki * iR2, ai = -kr * iR2;
+        gr = ((Er * ar - Ei * ai) - iR3 - 0.5 * k2r * iR) * kOofpi;
+        gi = ((Er * ai + Ei * ar) - 0.5 * k2i * iR) * kOofpi;
+      }
+      double w[3][3];
+      for (int q = 0; q < 3; q++)
+        for (int c = 0; c < 3; c++) w[q][c] = cs[q] * (rs[c] - vs[q][c]);
+      double2 T1 = make_double2(0.0, 0.0), T3 = make_double2(0.0, 0.0);
+      for (int q = 0; q < 3; q++) {
+        for (int c = 0; c < 3; c++) {
+          T1.x += X[q][c].x * w[q][c];
+          T1.y += X[q][c].y * w[q][c];
+        }
+        if (!self) {
+          // d . (w_q x Z_q)
+          for (int c = 0; c < 3; c++) {
+            const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            T3.x += d[c] * (w[q][c1] * Z[q][c2].x - w[q][c2] * Z[q][c1].x);
+            T3.y += d[c] * (w[q][c1] * Z[q][c2].y - w[q][c2] * Z[q][c1].y);
+          }
+        }
+      }
+      const double2 G = make_double2(Gr, Gi), gg = make_double2(gr, gi);
+      const double2 c1T1 = cm(g.c1, T1);
+      const double2 v1 = cm(G, make_double2(c1T1.x + c2T2.x, c1T1.y + c2T2.y)), v2 = cm(gg, T3);
+      si.x += g.rule.w[j] * (v1.x + v2.x);
+      si.y += g.rule.w[j] * (v1.y + v2.y);
+    }
+    acc.x += g.rule.w[i] * si.x;
+    acc.y += g.rule.w[i] * si.y;
+  }
+  const double AA = At * As;
+  return make_double2(acc.x * AA, acc.y * AA);
+}
+
+__device__ void vload_face(const VPairArgs &g, int f, VFace &o) {
+  for (int v = 0; v < 3; v++) {
+    const int node = g.fnodes[3 * f + v];
+    for (int k = 0; k < 3; k++) o.P[v][k] = g.p[3 * node + k];
+  }
+  for (int s = 0; s < 3; s++) {
+    o.sg[s] = g.side.coef[3 * f + s] < 0 ? -1.0 : 1.0;
+    const double *fv = g.side.v + 3 * (3 * f + s);
+    int lv = 0;
+    for (int v = 0; v < 3; v++)
+      if (o.P[v][0] == fv[0] && o.P[v][1] == fv[1] && o.P[v][2] == fv[2]) lv = v;
+    o.lv[s] = lv;
+  }
+}
+
+// a, b of test face ft (sides p) and source face fs (sides q)
+__device__ void vcoef(const VPairArgs &g, int ft, int fs, double2 a[3][3], double2 b[3][3]) {
+  for (int p = 0; p < 3; p++) {
+    const int64_t gi = g.side.g[3 * ft + p];
+    const double2 lJ = g.lam[gi], lM = g.lam[gi + g.N];
+    for (int q = 0; q < 3; q++) {
+      const int64_t gj = g.side.g[3 * fs + q];
+      const double2 xJ = g.x[gj], xM = g.x[gj + g.N];
+      const double2 jj = cm(lJ, xJ), mm = cm(cm(g.zsi, lM), xM), mj = cm(lM, xJ), jm = cm(lJ, xM);
+      a[p][q] = make_double2(jj.x + mm.x, jj.y + mm.y);
+      b[p][q] = make_double2(mj.x - jm.x, mj.y - jm.y);
+    }
+  }
+}
+
+__device__ double2 vpair_value(const VPairArgs &g, const VFace &fm, const VFace &fn, bool near, bool self,
+                               const double2 amn[3][3], const double2 bmn[3][3], const double2 anm[3][3],
+                               const double2 bnm[3][3]) {
+  double2 ac[3][3], bc[3][3];
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) {
+      // (n, m) enters with its sides exchanged: (D+H) and K are symmetric in the faces
+      ac[p][q] = self ? amn[p][q] : make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
+      bc[p][q] = self ? bmn[p][q] : make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
+    }
+  double2 v = vfar_part(g, fm, fn, near, self, ac, bc);
+  if (near) {
+    const double2 n1 = vnear_part(g, fm, fn, self, amn, bmn);
+    v.x += n1.x;
+    v.y += n1.y;
+    if (!self) {
+      const double2 n2 = vnear_part(g, fn, fm, false, anm, bnm);
+      v.x += n2.x;
+      v.y += n2.y;
+    }
+  }
+  return v;
+}
+
+// One thread per unordered face pair (m <= n): Re d Phi / d(node coordinate)
+// for the distinct nodes of the pair, added to dvert.
+__global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
+  if (n >= g.F || n < m) return;
+  const bool self = (m == n);
+  const bool near = near_pair(g.cp + 3 * m, g.cp + 3 * n, g.thr2);
+  VFace fm, fn;
+  vload_face(g, m, fm);
+  vload_face(g, n, fn);
+  double2 amn[3][3], bmn[3][3], anm[3][3], bnm[3][3];
+  vcoef(g, m, n, amn, bmn);
+  vcoef(g, n, m, anm, bnm);
+  // distinct nodes and their places in the two faces
+  int nodes[6], nn = 0;
+  for (int v = 0; v < 6; v++) {
+    const int node = v < 3 ? g.fnodes[3 * m + v] : g.fnodes[3 * n + v - 3];
+    bool seen = false;
+    for (int k = 0; k < nn; k++) seen |= nodes[k] == node;
+    if (!seen) nodes[nn++] = node;
+  }
+  double L = 0;
+  for (int v = 0; v < 3; v++) {
+    double e1[3], e2[3];
+    for (int k = 0; k < 3; k++) {
+      e1[k] = fm.P[(v + 1) % 3][k] - fm.P[v][k];
+      e2[k] = fn.P[(v + 1) % 3][k] - fn.P[v][k];
+    }
+    L += norm3(e1) + norm3(e2);
+  }
+  const double h = 1e-3 * L / 6.0;
+  const double steps[4] = {h, -h, 2.0 * h, -2.0 * h};
+  for (int k = 0; k < nn; k++) {
+    const int node = nodes[k];
+    for (int c = 0; c < 3; c++) {
+      double val[4];
+      for (int st = 0; st < 4; st++) {
+        VFace pm = fm, pn = fn;
+        for (int v = 0; v < 3; v++) {
+          if (g.fnodes[3 * m + v] == node) pm.P[v][c] += steps[st];
+          if (g.fnodes[3 * n + v] == node) pn.P[v][c] += steps[st];
+        }
+        val[st] = vpair_value(g, pm, pn, near, self, amn, bmn, anm, bnm).x;
+      }
+      unsafeAtomicAdd(g.dvert + 3 * node + c, (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h));
+    }
+  }
+}
+
+// Per face of the exterior domain: Psi = the face's share of sigma_abs
+// (cs_prtsrf's int1 term, cs.f90:38-102) + Re sum_p lambda_J q^E_p +
+// lambda_M q^H_p (srcvec's terms, source.f90:224-244), differentiated like
+// the pairs.
+struct VFaceArgs {
+  int F;
+  QuadRule rule;
+  const double *p;
+  const int *fnodes;
+  FaceSide side;
+  PwSrc S;
+  double kr, ki;
+  double2 hfac;
+  double cabs_c;  // -2/(eps0 c0 Re ri)
+  const double2 *x, *lam;
+  int64_t N;
+  double *dvert;
+};
+
+__device__ double vface_value(const VFaceArgs &g, int f, const VFace &F) {
+  double A, coef[3], divv[3], v[3][3];
+  vface_rwg(F, A, coef, divv, v);
+  double a[3], b[3], nr[3];
+  for (int k = 0; k < 3; k++) {
+    a[k] = F.P[1][k] - F.P[0][k];
+    b[k] = F.P[2][k] - F.P[0][k];
+  }
+  cross3(a, b, nr);
+  const double nl = norm3(nr);
+  for (int k = 0; k < 3; k++) nr[k] /= nl;
+  double int1 = 0.0;
+  double2 src = make_double2(0.0, 0.0);
+  for (int t = 0; t < g.rule.Q; t++) {
+    double r[3];
+    vface_point(F, g.rule, t, r);
+    double Et[3][2] = {{0, 0}, {0, 0}, {0, 0}}, Ht[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    double Ei[3][2], Hi[3][2];
+    pw_eh(g.S, r, g.kr, g.ki, g.hfac, Ei, Hi);
+    for (int p = 0; p < 3; p++) {
+      double fv[3] = {coef[p] * (r[0] - v[p][0]), coef[p] * (r[1] - v[p][1]), coef[p] * (r[2] - v[p][2])};
+      double fnv[3];
+      cross3(nr, fv, fnv);
+      const int gi = g.side.g[3 * f + p];
+      const double2 xj = g.x[gi], xm = g.x[gi + g.N];
+      for (int c = 0; c < 3; c++) {
+        Ht[c][0] -= xj.x * fnv[c];
+        Ht[c][1] -= xj.y * fnv[c];
+        Et[c][0] += xm.x * fnv[c];
+        Et[c][1] += xm.y * fnv[c];
+      }
+      double2 qe = make_double2(0.0, 0.0), qh = make_double2(0.0, 0.0);
+      for (int c = 0; c < 3; c++) {
+        qe.x += fv[c] * Ei[c][0];
+        qe.y += fv[c] * Ei[c][1];
+        qh.x += fv[c] * Hi[c][0];
+        qh.y += fv[c] * Hi[c][1];
+      }
+      const double2 t1 = cm(g.lam[gi], qe), t2 = cm(g.lam[gi + g.N], qh);
+      src.x += g.rule.w[t] * (t1.x + t2.x) * A;
+      src.y += g.rule.w[t] * (t1.y + t2.y) * A;
+    }
+    double s1 = 0;
+    for (int c = 0; c < 3; c++) {
+      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      const double re = (Et[c1][0] * Ht[c2][0] + Et[c1][1] * Ht[c2][1]) - (Et[c2][0] * Ht[c1][0] + Et[c2][1] * Ht[c1][1]);
+      s1 += re * nr[c];
+    }
+    int1 += g.rule.w[t] * 0.5 * s1;
+  }
+  return g.cabs_c * int1 * A + src.x;
+}
+
+__global__ __launch_bounds__(128) void vertex_face_kernel(VFaceArgs g) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.F) return;
+  VFace F0;
+  VPairArgs dummy;  // vload_face needs only the node and side tables
+  dummy.p = g.p;
+  dummy.fnodes = g.fnodes;
+  dummy.side = g.side;
+  vload_face(dummy, f, F0);
+  double L = 0;
+  for (int v = 0; v < 3; v++) {
+    double e[3];
+    for (int k = 0; k < 3; k++) e[k] = F0.P[(v + 1) % 3][k] - F0.P[v][k];
+    L += norm3(e);
+  }
+  const double h = 1e-3 * L / 3.0;
+  const double steps[4] = {h, -h, 2.0 * h, -2.0 * h};
+  for (int v = 0; v < 3; v++)
+    for (int c = 0; c < 3; c++) {
+      double val[4];
+      for (int st = 0; st < 4; st++) {
+        VFace Fs = F0;
+        Fs.P[v][c] += steps[st];
+        val[st] = vface_value(g, f, Fs);
+      }
+      unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * f + v] + c,
+                      (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h));
+    }
+}
+
+int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
+                        const double2 *x, const double2 *lam, double *dvert) {
+  const cd I(0.0, 1.0);
+  const cd k = ri * omega / kC0;
+  const cd c1 = -(I * omega) * mu0();
+  const cd c2 = -1.0 / ((I * omega) * kEps0 * (ri * ri));
+  const cd zsi = (ri * ri) / (kEta0 * kEta0);
+  const cd kk8 = (k * k) / (8.0 * MOM_PI);
+  VPairArgs g;
+  g.F = (int)m->F;
+  g.rule = rule;
+  g.p = m->p.as<double>();
+  g.fnodes = m->fnodes.as<int>();
+  g.side = sides(m, true);
+  g.cp = m->fcp.as<double>();
+  const double thr = m->avelen * 3;
+  g.thr2 = thr * thr;
+  g.kr = k.real();
+  g.ki = k.imag();
+  if (!(std::fabs(g.kr) * m->extent < kCisMaxArg))
+    return fail(MOM_ERR_ARG, "Re(k) times the mesh extent is beyond the fill's phase range (2^19 rad)");
+  g.c1 = make_double2(c1.real(), c1.imag());
+  g.c2 = make_double2(c2.real(), c2.imag());
+  g.zsi = make_double2(zsi.real(), zsi.imag());
+  g.kk4 = make_double2(1.0 / (4.0 * MOM_PI), 0.0);
+  g.kk8 = make_double2(kk8.real(), kk8.imag());
+  g.x = x;
+  g.lam = lam;
+  g.N = N;
+  g.dvert = dvert;
+  hipLaunchKernelGGL(vertex_pair_kernel, dim3((unsigned)((m->F + 127) / 128), (unsigned)m->F), dim3(128), 0,
+                     ctx->stream, g);
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
+int vertex_adjoint_faces(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
+                         const double2 *x, const double2 *lam, double th, double ph, double ps, double *dvert) {
+  VFaceArgs g;
+  g.F = (int)m->F;
+  g.rule = rule;
+  g.p = m->p.as<double>();
+  g.fnodes = m->fnodes.as<int>();
+  g.side = sides(m, true);
+  g.S.dir[0] = sin(th) * cos(ph);
+  g.S.dir[1] = sin(th) * sin(ph);
+  g.S.dir[2] = cos(th);
+  g.S.pol[0] = cos(ps) * sin(ph) - sin(ps) * cos(th) * cos(ph);
+  g.S.pol[1] = -cos(ps) * cos(ph) - sin(ps) * cos(th) * sin(ph);
+  g.S.pol[2] = sin(ps) * sin(th);
+  const cd k = ri * omega / kC0;
+  const cd hfac = k / (omega * mu0());
+  g.kr = k.real();
+  g.ki = k.imag();
+  g.hfac = make_double2(hfac.real(), hfac.imag());
+  g.cabs_c = -2.0 / (kEps0 * kC0 * ri.real());
+  g.x = x;
+  g.lam = lam;
+  g.N = N;
+  g.dvert = dvert;
+  hipLaunchKernelGGL(vertex_face_kernel, dim3((unsigned)((m->F + 127) / 128)), dim3(128), 0, ctx->stream, g);
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace mom

@@ -209,6 +209,14 @@ int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, c
 int fill_system_symm(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *ri, int64_t N,
                      const int32_t *bnd, const Group &G, double omega, int32_t nq, const double *qw,
                      const double *qnodes, double2 *const *Z, const int64_t *ld, const int64_t *order);
+// Vertex-coordinate adjoint (fill.hip), dvert[3V] (device) accumulated:
+// Re d/dP of lambda^T Z_d x for the domain's blocks (central differences of
+// each face pair's contraction, frozen near/far classification) ...
+int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
+                        const double2 *x, const double2 *lam, double *dvert);
+// ... and of sigma_abs|_x + Re(lambda^T q) for the exterior domain and one plane wave
+int vertex_adjoint_faces(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
+                         const double2 *x, const double2 *lam, double th, double ph, double ps, double *dvert);
 // y = M x (device, complex128, M nr x nc with leading dimension ld), in
 // column chunks spread over the device (adjoint.hip)
 int matvec(mom_ctx *ctx, const double2 *M, int64_t ld, int64_t nr, int64_t nc, const double2 *x, double2 *y);

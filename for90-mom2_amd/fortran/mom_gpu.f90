@@ -320,6 +320,34 @@ MODULE mom_gpu
        INTEGER(C_INT) :: st
      END FUNCTION mom_problem_mesh_create
 
+     ! Vertex-coordinate adjoint: dvert(3*V) += Re d/dP (lambda^T Z_mesh x), or
+     ! += d/dP (sigma_abs at fixed x + Re lambda^T q) for the exterior mesh
+     FUNCTION mom_pmchwt_vertex_adjoint(ctx, mesh, nedges_total, omega, ri_re, ri_im, nq, qw, qnodes, x, lam, &
+          dvert) BIND(C, NAME='mom_pmchwt_vertex_adjoint') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE, C_DOUBLE_COMPLEX
+       TYPE(C_PTR), VALUE :: ctx, mesh
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im
+       INTEGER(C_INT32_T), VALUE :: nq
+       REAL(C_DOUBLE), DIMENSION(*), INTENT(IN) :: qw, qnodes
+       COMPLEX(C_DOUBLE_COMPLEX), DIMENSION(*), INTENT(IN) :: x, lam
+       REAL(C_DOUBLE), DIMENSION(*) :: dvert
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_pmchwt_vertex_adjoint
+
+     FUNCTION mom_cs_abs_vertex_explicit(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, lam, theta, phi, psi, &
+          nq, qw, qnodes, dvert) BIND(C, NAME='mom_cs_abs_vertex_explicit') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_PTR, C_DOUBLE, C_DOUBLE_COMPLEX
+       TYPE(C_PTR), VALUE :: ctx, mesh
+       INTEGER(C_INT64_T), VALUE :: nedges_total
+       REAL(C_DOUBLE), VALUE :: omega, ri_re, ri_im, theta, phi, psi
+       INTEGER(C_INT32_T), VALUE :: nq
+       COMPLEX(C_DOUBLE_COMPLEX), DIMENSION(*), INTENT(IN) :: x, lam
+       REAL(C_DOUBLE), DIMENSION(*), INTENT(IN) :: qw, qnodes
+       REAL(C_DOUBLE), DIMENSION(*) :: dvert
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_cs_abs_vertex_explicit
+
      ! Mirror symmetry (the `sbnd` and `symm` commands; planes 1 = mxp,
      ! 2 = myp, 3 = mzp; boundary classes 2/3/4 = xplane/yplane/zplane)
      FUNCTION mom_problem_set_lines(prob, nlines, nodes, ids) BIND(C, NAME='mom_problem_set_lines') RESULT(st)

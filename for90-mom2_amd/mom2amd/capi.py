@@ -29,6 +29,7 @@ EXPORTS = [
     "mom_problem_topology", "mom_problem_mesh_create", "mom_problem_set_lines", "mom_problem_classify_edges",
     "mom_problem_edge_bnd", "mom_symm_group", "mom_symm_edge_bc", "mom_system_create_order",
     "mom_system_fill_symm", "mom_srcvec_pw_symm", "mom_cs_prtsrf_symm", "mom_rcs_ext_symm",
+    "mom_pmchwt_vertex_adjoint", "mom_cs_abs_vertex_explicit",
 ]
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
@@ -120,6 +121,8 @@ def lib():
                                      C.POINTER(_d), C.POINTER(_d)]
     L.mom_rcs_ext_symm.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _ip, _dp, _i32, _i32, _i32, C.c_void_p, _i32,
                                    _dp, _dp, _dp]
+    L.mom_pmchwt_vertex_adjoint.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _dp, _dp, _dp, _dp, _dp]
+    L.mom_cs_abs_vertex_explicit.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _dp, _d, _d, _d, _i32, _dp, _dp, _dp]
     _lib = L
     return L
 

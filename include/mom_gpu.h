@@ -178,6 +178,30 @@ int mom_cs_abs_gradient(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total
 int mom_pmchwt_deps_apply(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
                           double ri_im, int32_t nq, const double *qw, const double *qnodes, const double *x,
                           double *y);
+/*
+ * Vertex-coordinate adjoint (the vertex part of mom_fill_adjoint; no
+ * counterpart in the reference): with x the solution and lambda = Z^-T g
+ * (g = dsigma/dx, i.e. conj of mom_system_solve_adjoint's solution for
+ * conj(g)),
+ *   dsigma/dP = dsigma/dP|_x + Re(lambda^T dq/dP) - Re(lambda^T dZ/dP x).
+ * mom_pmchwt_vertex_adjoint adds Re d/dP (lambda^T Z_mesh x) of one domain's
+ * blocks of the PMCHWT matrix (sysmat.f90:56-98) to dvert[3 V] (this mesh's
+ * nodes, host); mom_cs_abs_vertex_explicit adds d/dP (sigma_abs at fixed x +
+ * Re lambda^T q) for the exterior mesh and one plane wave (cs.f90:14-104,
+ * source.f90:136-247).  Each face pair's (or face's) contribution is rebuilt
+ * from its nodes with the fill's formulas and differentiated by the
+ * fourth-order central difference (h = 1e-3 of its mean edge length) in
+ * every coordinate of its nodes; the near/far classification stays that of
+ * the unperturbed mesh.
+ * x, lambda: [2N] complex (host).  The caller sums the domains' dvert through
+ * their nodes' master indices.
+ */
+int mom_pmchwt_vertex_adjoint(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                              double ri_im, int32_t nq, const double *qw, const double *qnodes, const double *x,
+                              const double *lam, double *dvert);
+int mom_cs_abs_vertex_explicit(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                               double ri_im, const double *x, const double *lam, double theta, double phi, double psi,
+                               int32_t nq, const double *qw, const double *qnodes, double *dvert);
 /* Build the plane-wave right-hand sides on the device and solve in place of
  * mom_srcvec_pw + mom_system_solve; X[2N x nsrc] receives the solutions. */
 int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, double ri_re,
