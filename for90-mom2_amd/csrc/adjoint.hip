@@ -304,6 +304,14 @@ static int vertex_finish(mom_ctx *ctx, const mom_mesh *mesh, DevBuf &dv, double 
   return 0;
 }
 
+// test hook: far pairs by differences as well (mom_test_vertex_fd)
+static int g_vertex_fd = 0;
+
+int mom_test_vertex_fd(int32_t enable) {
+  g_vertex_fd = enable ? 1 : 0;
+  return 0;
+}
+
 int mom_pmchwt_vertex_adjoint(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
                               double ri_im, int32_t nq, const double *qw, const double *qnodes, const double *x,
                               const double *lam, double *dvert) {
@@ -312,7 +320,7 @@ int mom_pmchwt_vertex_adjoint(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges
   DevBuf dx, dl, dv;
   MOM_TRY(vertex_common(ctx, mesh, nedges_total, x, lam, dvert, nq, qw, qnodes, rule, dx, dl, dv));
   MOM_TRY(vertex_adjoint_fill(ctx, mesh, rule, omega, cd(ri_re, ri_im), nedges_total, dx.as<double2>(),
-                              dl.as<double2>(), dv.as<double>()));
+                              dl.as<double2>(), dv.as<double>(), g_vertex_fd));
   return vertex_finish(ctx, mesh, dv, dvert);
 }
 

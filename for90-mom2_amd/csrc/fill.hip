@@ -1539,6 +1539,7 @@ struct VPairArgs {
   const double2 *x, *lam;  // [2N]
   int64_t N;
   double *dvert;       // [V][3] accumulated
+  int fd_only;         // differences for far pairs too (check of the closed form)
 };
 
 __device__ __forceinline__ double2 cm(double2 a, double2 b) {
@@ -1765,8 +1766,227 @@ __device__ double2 vpair_value(const VPairArgs &g, const VFace &fm, const VFace 
   return v;
 }
 
+// Far pairs: the derivative of the point-pair sum in closed form (reverse
+// mode by hand).  With e_p = r_i - v_p, f_q = r'_j - v_q, d = r_i - r'_j,
+// alpha = s L/2 and beta = s L (the areas cancel against the RWG
+// coefficients), each point pair adds
+//   G (c1 sum_pq a_pq alpha_p alpha_q e_p.f_q + c2 sum_pq a_pq beta_p beta_q)
+//   + g sum_pq b_pq alpha_p alpha_q d.(f_q x e_p),
+// with G' = G (ik - 1/R) and g' = G (k^2/R - 3/R^3 + 3ik/R^2); the gradients
+// with respect to e, f, d, R and alpha are carried to the barycentric
+// points, the free vertices and the edge lengths.  gt/gs: Re d/d(node) of
+// the test and source faces' nodes (local order).
+__device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, const double2 a[3][3],
+                          const double2 b[3][3], double gt[3][3], double gs[3][3]) {
+  double al_t[3], al_s[3], ut[3][3], us[3][3], vt[3][3], vs[3][3];
+  for (int p = 0; p < 3; p++) {
+    double et[3], es[3];
+    for (int k = 0; k < 3; k++) {
+      et[k] = ft.P[(p + 1) % 3][k] - ft.P[p][k];
+      es[k] = fs.P[(p + 1) % 3][k] - fs.P[p][k];
+      vt[p][k] = ft.P[ft.lv[p]][k];
+      vs[p][k] = fs.P[fs.lv[p]][k];
+    }
+    const double Lt = norm3(et), Ls = norm3(es);
+    al_t[p] = ft.sg[p] * Lt / 2.0;
+    al_s[p] = fs.sg[p] * Ls / 2.0;
+    for (int k = 0; k < 3; k++) {
+      ut[p][k] = et[k] / Lt;
+      us[p][k] = es[k] / Ls;
+    }
+  }
+  double2 T2 = make_double2(0.0, 0.0);
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) {
+      const double bb = 4.0 * al_t[p] * al_s[q];
+      T2.x += a[p][q].x * bb;
+      T2.y += a[p][q].y * bb;
+    }
+  const double2 c2T2 = cm(g.c2, T2);
+  const double kr = g.kr, ki = g.ki, k2r = kr * kr - ki * ki, k2i = 2.0 * kr * ki;
+  double2 GT[3][3], GS[3][3], dAt[3], dAs[3], SG = make_double2(0.0, 0.0);
+  for (int k = 0; k < 3; k++) {
+    dAt[k] = dAs[k] = make_double2(0.0, 0.0);
+    for (int c = 0; c < 3; c++) GT[k][c] = GS[k][c] = make_double2(0.0, 0.0);
+  }
+  const int Q = g.rule.Q;
+  for (int i = 0; i < Q; i++) {
+    double r[3];
+    vface_point(ft, g.rule, i, r);
+    double e[3][3];
+    for (int p = 0; p < 3; p++)
+      for (int c = 0; c < 3; c++) e[p][c] = r[c] - vt[p][c];
+    double2 Xa[3][3], Xb[3][3];  // sum_p {a,b}_pq alpha_p e_p
+    for (int q = 0; q < 3; q++)
+      for (int c = 0; c < 3; c++) {
+        Xa[q][c] = Xb[q][c] = make_double2(0.0, 0.0);
+        for (int p = 0; p < 3; p++) {
+          const double ae = al_t[p] * e[p][c];
+          Xa[q][c].x += a[p][q].x * ae;
+          Xa[q][c].y += a[p][q].y * ae;
+          Xb[q][c].x += b[p][q].x * ae;
+          Xb[q][c].y += b[p][q].y * ae;
+        }
+      }
+    double2 dRi[3], dE[3][3], dAti[3];
+    for (int k = 0; k < 3; k++) {
+      dRi[k] = dAti[k] = make_double2(0.0, 0.0);
+      for (int c = 0; c < 3; c++) dE[k][c] = make_double2(0.0, 0.0);
+    }
+    for (int j = 0; j < Q; j++) {
+      double rs[3];
+      vface_point(fs, g.rule, j, rs);
+      const double d[3] = {r[0] - rs[0], r[1] - rs[1], r[2] - rs[2]};
+      const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      const double iR = rsqrt(R2), R = R2 * iR;
+      double Er, Ei;
+      cis_exp(-ki * R, kr * R, Er, Ei);
+      const double2 G = make_double2(iR * kOofpi * Er, iR * kOofpi * Ei);
+      const double2 gg = cm(G, make_double2(iR * iR + ki * iR, -kr * iR));
+      // G' = G (ik - 1/R) = G (-ki - 1/R + i kr)
+      const double2 Gp = cm(G, make_double2(-ki - iR, kr));
+      // g' = G (k^2/R - 3/R^3 + 3ik/R^2)
+      const double2 gp = cm(G, make_double2(k2r * iR - 3.0 * iR * iR * iR - 3.0 * ki * iR * iR,
+                                            k2i * iR + 3.0 * kr * iR * iR));
+      const double ww = g.rule.w[i] * g.rule.w[j];
+      double f[3][3];
+      for (int q = 0; q < 3; q++)
+        for (int c = 0; c < 3; c++) f[q][c] = rs[c] - vs[q][c];
+      // T1 = sum_q alpha_q f_q . Xa_q ; T3 = sum_q alpha_q d . (f_q x Xb_q)
+      double2 T1 = make_double2(0.0, 0.0), T3 = make_double2(0.0, 0.0), Dx[3];
+      for (int c = 0; c < 3; c++) Dx[c] = make_double2(0.0, 0.0);
+      double2 dF[3][3];  // d Psi / d f_q
+      const double2 Gc1 = cm(G, g.c1);
+      for (int q = 0; q < 3; q++) {
+        double2 fxX[3], Xxd[3];
+        for (int c = 0; c < 3; c++) {
+          const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+          fxX[c] = make_double2(f[q][c1] * Xb[q][c2].x - f[q][c2] * Xb[q][c1].x,
+                                f[q][c1] * Xb[q][c2].y - f[q][c2] * Xb[q][c1].y);
+          Xxd[c] = make_double2(Xb[q][c1].x * d[c2] - Xb[q][c2].x * d[c1], Xb[q][c1].y * d[c2] - Xb[q][c2].y * d[c1]);
+        }
+        for (int c = 0; c < 3; c++) {
+          T1.x += al_s[q] * f[q][c] * Xa[q][c].x;
+          T1.y += al_s[q] * f[q][c] * Xa[q][c].y;
+          T3.x += al_s[q] * d[c] * fxX[c].x;
+          T3.y += al_s[q] * d[c] * fxX[c].y;
+          Dx[c].x += al_s[q] * fxX[c].x;
+          Dx[c].y += al_s[q] * fxX[c].y;
+          const double2 t1 = cm(Gc1, Xa[q][c]), t2 = cm(gg, Xxd[c]);
+          dF[q][c] = make_double2(al_s[q] * (t1.x + t2.x), al_s[q] * (t1.y + t2.y));
+        }
+        // d Psi / d alpha_q = f_q . (G c1 Xa_q + g Xb_q x d)
+        double2 sa = make_double2(0.0, 0.0);
+        for (int c = 0; c < 3; c++) {
+          const double2 t1 = cm(Gc1, Xa[q][c]), t2 = cm(gg, Xxd[c]);
+          sa.x += f[q][c] * (t1.x + t2.x);
+          sa.y += f[q][c] * (t1.y + t2.y);
+        }
+        dAs[q].x += ww * sa.x;
+        dAs[q].y += ww * sa.y;
+      }
+      // d Psi / d R and the total d Psi / d d
+      const double2 c1T1 = cm(g.c1, T1);
+      const double2 dPR1 = cm(Gp, make_double2(c1T1.x + c2T2.x, c1T1.y + c2T2.y)), dPR2 = cm(gp, T3);
+      const double2 dPR = make_double2(dPR1.x + dPR2.x, dPR1.y + dPR2.y);
+      double2 DD[3];
+      for (int c = 0; c < 3; c++) {
+        const double2 ex = cm(gg, Dx[c]);
+        DD[c] = make_double2(ex.x + dPR.x * d[c] * iR, ex.y + dPR.y * d[c] * iR);
+      }
+      // d Psi / d e_p = alpha_p (G c1 Fa_p + g Fb_p), Fa_p = sum_q a_pq alpha_q f_q, Fb_p = sum_q b_pq alpha_q d x f_q
+      for (int p = 0; p < 3; p++) {
+        double2 tot[3];
+        for (int c = 0; c < 3; c++) tot[c] = make_double2(0.0, 0.0);
+        for (int q = 0; q < 3; q++) {
+          double dxf[3];
+          cross3(d, f[q], dxf);
+          const double2 ga = cm(Gc1, a[p][q]), gb = cm(gg, b[p][q]);
+          for (int c = 0; c < 3; c++) {
+            tot[c].x += al_s[q] * (ga.x * f[q][c] + gb.x * dxf[c]);
+            tot[c].y += al_s[q] * (ga.y * f[q][c] + gb.y * dxf[c]);
+          }
+        }
+        double2 sa = make_double2(0.0, 0.0);
+        for (int c = 0; c < 3; c++) {
+          sa.x += e[p][c] * tot[c].x;
+          sa.y += e[p][c] * tot[c].y;
+          const double2 de = make_double2(al_t[p] * tot[c].x, al_t[p] * tot[c].y);
+          dE[p][c].x += ww * de.x;
+          dE[p][c].y += ww * de.y;
+          dRi[c].x += ww * de.x;
+          dRi[c].y += ww * de.y;
+        }
+        dAti[p].x += ww * sa.x;
+        dAti[p].y += ww * sa.y;
+      }
+      for (int c = 0; c < 3; c++) {
+        dRi[c].x += ww * DD[c].x;
+        dRi[c].y += ww * DD[c].y;
+      }
+      // source side: r'_j = sum_k b_jk S_k, d depends on -r'_j; free vertices
+      const double bj[3] = {g.rule.nodes[2 * Q + j], g.rule.nodes[j], g.rule.nodes[Q + j]};
+      for (int c = 0; c < 3; c++) {
+        double2 sf = make_double2(-DD[c].x, -DD[c].y);
+        for (int q = 0; q < 3; q++) {
+          sf.x += dF[q][c].x;
+          sf.y += dF[q][c].y;
+          GS[fs.lv[q]][c].x -= ww * dF[q][c].x;
+          GS[fs.lv[q]][c].y -= ww * dF[q][c].y;
+        }
+        for (int k = 0; k < 3; k++) {
+          GS[k][c].x += ww * bj[k] * sf.x;
+          GS[k][c].y += ww * bj[k] * sf.y;
+        }
+      }
+      SG.x += ww * G.x;
+      SG.y += ww * G.y;
+    }
+    const double bi[3] = {g.rule.nodes[2 * Q + i], g.rule.nodes[i], g.rule.nodes[Q + i]};
+    for (int c = 0; c < 3; c++) {
+      for (int k = 0; k < 3; k++) {
+        GT[k][c].x += bi[k] * dRi[c].x;
+        GT[k][c].y += bi[k] * dRi[c].y;
+      }
+      for (int p = 0; p < 3; p++) {
+        GT[ft.lv[p]][c].x -= dE[p][c].x;
+        GT[ft.lv[p]][c].y -= dE[p][c].y;
+      }
+    }
+    for (int p = 0; p < 3; p++) {
+      dAt[p].x += dAti[p].x;
+      dAt[p].y += dAti[p].y;
+    }
+  }
+  // the c2 T2 term's edge-length dependence: d/d alpha_p of c2 SG sum a_pq 4 alpha_p alpha_q
+  const double2 c2SG = cm(g.c2, SG);
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) {
+      const double2 t = cm(c2SG, a[p][q]);
+      dAt[p].x += 4.0 * al_s[q] * t.x;
+      dAt[p].y += 4.0 * al_s[q] * t.y;
+      dAs[q].x += 4.0 * al_t[p] * t.x;
+      dAs[q].y += 4.0 * al_t[p] * t.y;
+    }
+  // alpha_p = s_p L_p / 2 with L_p = |P_{p+1} - P_p|
+  for (int p = 0; p < 3; p++)
+    for (int c = 0; c < 3; c++) {
+      const double ht = ft.sg[p] * 0.5 * ut[p][c], hs = fs.sg[p] * 0.5 * us[p][c];
+      GT[(p + 1) % 3][c].x += dAt[p].x * ht;
+      GT[p][c].x -= dAt[p].x * ht;
+      GS[(p + 1) % 3][c].x += dAs[p].x * hs;
+      GS[p][c].x -= dAs[p].x * hs;
+    }
+  for (int k = 0; k < 3; k++)
+    for (int c = 0; c < 3; c++) {
+      gt[k][c] = GT[k][c].x;
+      gs[k][c] = GS[k][c].x;
+    }
+}
+
 // One thread per unordered face pair (m <= n): Re d Phi / d(node coordinate)
-// for the distinct nodes of the pair, added to dvert.
+// for the distinct nodes of the pair, added to dvert (far pairs in closed
+// form, near pairs by differences).
 __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
   if (n >= g.F || n < m) return;
@@ -1785,6 +2005,22 @@ __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
     bool seen = false;
     for (int k = 0; k < nn; k++) seen |= nodes[k] == node;
     if (!seen) nodes[nn++] = node;
+  }
+  if (!near && nn == 6 && !g.fd_only) {
+    double2 ac[3][3], bc[3][3];
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) {
+        ac[p][q] = make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
+        bc[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
+      }
+    double gt[3][3], gs[3][3];
+    vfar_grad(g, fm, fn, ac, bc, gt, gs);
+    for (int v = 0; v < 3; v++)
+      for (int c = 0; c < 3; c++) {
+        unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * m + v] + c, gt[v][c]);
+        unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * n + v] + c, gs[v][c]);
+      }
+    return;
   }
   double L = 0;
   for (int v = 0; v < 3; v++) {
@@ -1917,7 +2153,7 @@ __global__ __launch_bounds__(128) void vertex_face_kernel(VFaceArgs g) {
 }
 
 int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
-                        const double2 *x, const double2 *lam, double *dvert) {
+                        const double2 *x, const double2 *lam, double *dvert, int fd_only) {
   const cd I(0.0, 1.0);
   const cd k = ri * omega / kC0;
   const cd c1 = -(I * omega) * mu0();
@@ -1946,6 +2182,7 @@ int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
   g.lam = lam;
   g.N = N;
   g.dvert = dvert;
+  g.fd_only = fd_only;
   hipLaunchKernelGGL(vertex_pair_kernel, dim3((unsigned)((m->F + 127) / 128), (unsigned)m->F), dim3(128), 0,
                      ctx->stream, g);
   MOM_HIP(hipGetLastError());

@@ -212,8 +212,9 @@ int fill_system_symm(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, co
 // Vertex-coordinate adjoint (fill.hip), dvert[3V] (device) accumulated:
 // Re d/dP of lambda^T Z_d x for the domain's blocks (central differences of
 // each face pair's contraction, frozen near/far classification) ...
+// (fd_only: the far pairs by differences too, as a check of their closed form)
 int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
-                        const double2 *x, const double2 *lam, double *dvert);
+                        const double2 *x, const double2 *lam, double *dvert, int fd_only = 0);
 // ... and of sigma_abs|_x + Re(lambda^T q) for the exterior domain and one plane wave
 int vertex_adjoint_faces(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
                          const double2 *x, const double2 *lam, double th, double ph, double ps, double *dvert);

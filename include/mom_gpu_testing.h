@@ -62,6 +62,9 @@ int mom_test_u_solve(int32_t mode);
  * on upload), as the complex64 LU forms it after each panel; E (256 x 256,
  * leading dimension 256) is returned widened to complex128.
  */
+/* Node-coordinate adjoint: 1 = every face pair by fourth-order differences
+ * (the far pairs' closed form is checked against it), 0 = the default. */
+int mom_test_vertex_fd(int32_t enable);
 int mom_test_unit_lower_inverse(mom_ctx *ctx, const double *L, int64_t ldl, double *E);
 
 #ifdef __cplusplus

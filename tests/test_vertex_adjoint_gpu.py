@@ -103,3 +103,38 @@ def test_vertex_gradient_matches_finite_differences(gpu_ctx):
         print("node %d coord %d: adjoint %.10e fd %.10e (rel to max %.1e)"
               % (v, c, grad[v, c], fd, abs(grad[v, c] - fd) / scale), flush=True)
         assert abs(grad[v, c] - fd) <= 2e-6 * scale
+
+
+def test_far_pair_closed_form_matches_differences(gpu_ctx):
+    """The far pairs' closed-form derivative against fourth-order differences
+    of the same pair sums (mom_test_vertex_fd), per domain and node
+    coordinate: <= 1e-8 of the largest component."""
+    import ctypes as C  # noqa: F401
+    from mom2amd import capi
+    from mom2amd.gpu import _quad
+
+    nodes, faces = meshgen.icosphere(2, 50.0)
+    P, dms, S, x = _solve(gpu_ctx, nodes, faces, MOM_PREC_FP64)
+    L = capi.lib()
+    try:
+        rng = np.random.default_rng(2)
+        lam = rng.standard_normal(x.size) + 1j * rng.standard_normal(x.size)
+        Q, w, n = _quad(None)
+        xv = np.ascontiguousarray(x).view(np.float64)
+        lv = np.ascontiguousarray(lam).view(np.float64)
+        for d, (dm, mesh) in enumerate(zip(dms, P.domains)):
+            out = []
+            for fd in (0, 1):
+                L.mom_test_vertex_fd(fd)
+                dv = np.zeros(3 * mesh.nnodes)
+                capi.check(L.mom_pmchwt_vertex_adjoint(gpu_ctx.h, dm.h, P.nedges, OMEGA, RI[d].real, RI[d].imag, Q,
+                                                       w, n, xv, lv, dv))
+                out.append(dv)
+            err = np.abs(out[0] - out[1]).max() / np.abs(out[1]).max()
+            print("domain %d: closed form vs differences %.2e" % (d, err), flush=True)
+            assert err < 1e-8
+    finally:
+        L.mom_test_vertex_fd(0)
+        S.close()
+        for dm in dms:
+            dm.close()
