@@ -1477,6 +1477,11 @@ struct VFace {
   int lv[3];        // local index of each side's free vertex
 };
 
+// node l's coordinate k by selects (a dynamic index would move the face to scratch)
+__device__ __forceinline__ double vnode(const VFace &f, int l, int k) {
+  return l == 0 ? f.P[0][k] : (l == 1 ? f.P[1][k] : f.P[2][k]);
+}
+
 // One face's RWG data from its nodes (compute_basis_data, mesh.f90:1265-1322;
 // rwgf.f90:34-78): area, coef = s L/(2A), div = s L/A, free vertex.
 __device__ __forceinline__ void vface_rwg(const VFace &f, double &A, double coef[3], double divv[3],
@@ -1494,7 +1499,7 @@ __device__ __forceinline__ void vface_rwg(const VFace &f, double &A, double coef
     const double L = norm3(e);
     coef[p] = f.sg[p] * (L / (2.0 * A));
     divv[p] = f.sg[p] * L / A;
-    for (int k = 0; k < 3; k++) v[p][k] = f.P[f.lv[p]][k];
+    for (int k = 0; k < 3; k++) v[p][k] = vnode(f, f.lv[p], k);
   }
 }
 
@@ -1540,6 +1545,8 @@ struct VPairArgs {
   int64_t N;
   double *dvert;       // [V][3] accumulated
   int fd_only;         // differences for far pairs too (check of the closed form)
+  const int *pairs;    // near pairs (test, source), sorted; the near launch takes those with test <= source
+  int64_t npairs;
 };
 
 __device__ __forceinline__ double2 cm(double2 a, double2 b) {
@@ -1784,8 +1791,8 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
     for (int k = 0; k < 3; k++) {
       et[k] = ft.P[(p + 1) % 3][k] - ft.P[p][k];
       es[k] = fs.P[(p + 1) % 3][k] - fs.P[p][k];
-      vt[p][k] = ft.P[ft.lv[p]][k];
-      vs[p][k] = fs.P[fs.lv[p]][k];
+      vt[p][k] = vnode(ft, ft.lv[p], k);
+      vs[p][k] = vnode(fs, fs.lv[p], k);
     }
     const double Lt = norm3(et), Ls = norm3(es);
     al_t[p] = ft.sg[p] * Lt / 2.0;
@@ -1804,10 +1811,12 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
     }
   const double2 c2T2 = cm(g.c2, T2);
   const double kr = g.kr, ki = g.ki, k2r = kr * kr - ki * ki, k2i = 2.0 * kr * ki;
-  double2 GT[3][3], GS[3][3], dAt[3], dAs[3], SG = make_double2(0.0, 0.0);
+  // GT/GS: by node (barycentric points); VT/VS: by side (free vertices), mapped
+  // to nodes at the end so that every array index stays static
+  double2 GT[3][3], GS[3][3], VT[3][3], VS[3][3], dAt[3], dAs[3], SG = make_double2(0.0, 0.0);
   for (int k = 0; k < 3; k++) {
     dAt[k] = dAs[k] = make_double2(0.0, 0.0);
-    for (int c = 0; c < 3; c++) GT[k][c] = GS[k][c] = make_double2(0.0, 0.0);
+    for (int c = 0; c < 3; c++) GT[k][c] = GS[k][c] = VT[k][c] = VS[k][c] = make_double2(0.0, 0.0);
   }
   const int Q = g.rule.Q;
   for (int i = 0; i < Q; i++) {
@@ -1931,8 +1940,8 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
         for (int q = 0; q < 3; q++) {
           sf.x += dF[q][c].x;
           sf.y += dF[q][c].y;
-          GS[fs.lv[q]][c].x -= ww * dF[q][c].x;
-          GS[fs.lv[q]][c].y -= ww * dF[q][c].y;
+          VS[q][c].x -= ww * dF[q][c].x;
+          VS[q][c].y -= ww * dF[q][c].y;
         }
         for (int k = 0; k < 3; k++) {
           GS[k][c].x += ww * bj[k] * sf.x;
@@ -1949,8 +1958,8 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
         GT[k][c].y += bi[k] * dRi[c].y;
       }
       for (int p = 0; p < 3; p++) {
-        GT[ft.lv[p]][c].x -= dE[p][c].x;
-        GT[ft.lv[p]][c].y -= dE[p][c].y;
+        VT[p][c].x -= dE[p][c].x;
+        VT[p][c].y -= dE[p][c].y;
       }
     }
     for (int p = 0; p < 3; p++) {
@@ -1979,19 +1988,21 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
     }
   for (int k = 0; k < 3; k++)
     for (int c = 0; c < 3; c++) {
-      gt[k][c] = GT[k][c].x;
-      gs[k][c] = GS[k][c].x;
+      double t = GT[k][c].x, u = GS[k][c].x;
+      for (int p = 0; p < 3; p++) {
+        t += ft.lv[p] == k ? VT[p][c].x : 0.0;
+        u += fs.lv[p] == k ? VS[p][c].x : 0.0;
+      }
+      gt[k][c] = t;
+      gs[k][c] = u;
     }
 }
 
-// One thread per unordered face pair (m <= n): Re d Phi / d(node coordinate)
-// for the distinct nodes of the pair, added to dvert (far pairs in closed
-// form, near pairs by differences).
-__global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
-  if (n >= g.F || n < m) return;
+// Re d Phi / d(node coordinate) of the unordered face pair (m <= n) for its
+// distinct nodes, added to dvert (far pairs in closed form, near pairs by
+// differences).
+__device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near) {
   const bool self = (m == n);
-  const bool near = near_pair(g.cp + 3 * m, g.cp + 3 * n, g.thr2);
   VFace fm, fn;
   vload_face(g, m, fm);
   vload_face(g, n, fn);
@@ -2005,22 +2016,6 @@ __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
     bool seen = false;
     for (int k = 0; k < nn; k++) seen |= nodes[k] == node;
     if (!seen) nodes[nn++] = node;
-  }
-  if (!near && nn == 6 && !g.fd_only) {
-    double2 ac[3][3], bc[3][3];
-    for (int p = 0; p < 3; p++)
-      for (int q = 0; q < 3; q++) {
-        ac[p][q] = make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
-        bc[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
-      }
-    double gt[3][3], gs[3][3];
-    vfar_grad(g, fm, fn, ac, bc, gt, gs);
-    for (int v = 0; v < 3; v++)
-      for (int c = 0; c < 3; c++) {
-        unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * m + v] + c, gt[v][c]);
-        unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * n + v] + c, gs[v][c]);
-      }
-    return;
   }
   double L = 0;
   for (int v = 0; v < 3; v++) {
@@ -2048,6 +2043,50 @@ __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
       unsafeAtomicAdd(g.dvert + 3 * node + c, (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h));
     }
   }
+}
+
+// Grid launch over (m, n >= m).  FD = false: the far pairs in closed form (a
+// node shared by the two faces receives both faces' partial derivatives);
+// the near pairs have their own launch, so that their differences do not
+// hold whole waves of far pairs.  FD = true: every pair by differences (the
+// check of the closed form).
+template <bool FD>
+__global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
+  if (n >= g.F || n < m) return;
+  const bool near = near_pair(g.cp + 3 * m, g.cp + 3 * n, g.thr2);
+  if (FD) {
+    vertex_pair(g, m, n, near);
+    return;
+  }
+  if (near) return;
+  VFace fm, fn;
+  vload_face(g, m, fm);
+  vload_face(g, n, fn);
+  double2 amn[3][3], bmn[3][3], anm[3][3], bnm[3][3];
+  vcoef(g, m, n, amn, bmn);
+  vcoef(g, n, m, anm, bnm);
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) {
+      amn[p][q] = make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
+      bmn[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
+    }
+  double gt[3][3], gs[3][3];
+  vfar_grad(g, fm, fn, amn, bmn, gt, gs);
+  for (int v = 0; v < 3; v++)
+    for (int c = 0; c < 3; c++) {
+      unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * m + v] + c, gt[v][c]);
+      unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * n + v] + c, gs[v][c]);
+    }
+}
+
+// One thread per entry of the near list with test <= source.
+__global__ __launch_bounds__(128) void vertex_near_kernel(VPairArgs g) {
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id >= g.npairs) return;
+  const int m = g.pairs[2 * id], n = g.pairs[2 * id + 1];
+  if (n < m) return;
+  vertex_pair(g, m, n, true);
 }
 
 // Per face of the exterior domain: Psi = the face's share of sigma_abs
@@ -2183,9 +2222,19 @@ int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
   g.N = N;
   g.dvert = dvert;
   g.fd_only = fd_only;
-  hipLaunchKernelGGL(vertex_pair_kernel, dim3((unsigned)((m->F + 127) / 128), (unsigned)m->F), dim3(128), 0,
-                     ctx->stream, g);
+  g.pairs = m->near.as<int>();
+  g.npairs = m->nnear;
+  if (fd_only)
+    hipLaunchKernelGGL(vertex_pair_kernel<true>, dim3((unsigned)((m->F + 127) / 128), (unsigned)m->F), dim3(128), 0,
+                       ctx->stream, g);
+  else
+    hipLaunchKernelGGL(vertex_pair_kernel<false>, dim3((unsigned)((m->F + 127) / 128), (unsigned)m->F), dim3(128), 0,
+                       ctx->stream, g);
   MOM_HIP(hipGetLastError());
+  if (!fd_only && m->nnear > 0) {
+    hipLaunchKernelGGL(vertex_near_kernel, dim3((unsigned)((m->nnear + 127) / 128)), dim3(128), 0, ctx->stream, g);
+    MOM_HIP(hipGetLastError());
+  }
   return 0;
 }
 
