@@ -623,6 +623,8 @@ def main():
         "factor_breakdown_s": dict({k: round(bd[k + "_s"], 4) for k in ("factor", "gemm", "gemm_narrow", "panel",
                                                                          "trsm", "laswp")},
                                    note="an extra untimed step with every kernel group event-timed"),
+        "fill_tuning": {"pair_kernel_sources_per_wave": ctx.fill_tuning()[0],
+                        "first_slab_ms_by_candidate": {str(k): round(v, 3) for k, v in ctx.fill_tuning()[1].items()}},
         "gemm_narrow": {"launches": bd["gemm_narrow_launches"],
                         "tflops": round(bd["gemm_narrow_flops"] / max(bd["gemm_narrow_s"], 1e-12) / 1e12, 2)},
         "roofline": None if achieved is None else {

@@ -238,6 +238,14 @@ int mom_ctx_create(int32_t device, mom_ctx **ctx) {
   return 0;
 }
 
+int mom_ctx_fill_tuning(const mom_ctx *ctx, int32_t *spw, double *ms) {
+  if (!ctx || !spw) return fail(MOM_ERR_ARG, "NULL argument");
+  *spw = ctx->fill_spw;
+  if (ms)
+    for (int c = 0; c < 3; c++) ms[c] = ctx->fill_spw_ms[c];
+  return 0;
+}
+
 int mom_ctx_destroy(mom_ctx *ctx) {
   if (!ctx) return 0;
   (void)hipSetDevice(ctx->device);

@@ -351,11 +351,12 @@ struct PlaneArgs {
   int mode;       // 0: D+H and K; 1: D; 2: H; 3: K
 };
 
-constexpr int kSrcPerWave = 4;
-constexpr int kSrcPerBlock = kWaves * kSrcPerWave;
-
-template <int QT, bool DK = false>
+// SPW source faces per wave (wave-uniform); the fill's autotuner picks SPW
+// among 2, 4 and 8 per context.  The per-pair arithmetic does not depend on
+// SPW, so every choice gives the same bits.
+template <int QT, bool DK = false, int SPW = 4>
 __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
+  constexpr int kSrcPerWave = SPW, kSrcPerBlock = kWaves * SPW;
   const FarArgs &a = pa.f;
   __shared__ double tpts[3 * kMaxQ * 64];          // test points, [3j+c][lane]
   __shared__ double spts[kSrcPerBlock][3 * kMaxQ];  // source points, broadcast
@@ -989,13 +990,48 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     pa.n0 = k * S;
     pa.mlo = pa.n0;  // pairs with both faces below n0 were integrated by earlier slabs
     const int scur = (int)std::min<int64_t>(S, m->F - pa.n0);
-    dim3 grid((unsigned)((m->F - pa.mlo + 63) / 64), (unsigned)((scur + kSrcPerBlock - 1) / kSrcPerBlock));
-    if (t.dk)
-      hipLaunchKernelGGL((pair_plane_kernel<0, true>), grid, dim3(256), 0, ctx->stream, pa);
-    else if (rule.Q == 13)
-      hipLaunchKernelGGL(pair_plane_kernel<13>, grid, dim3(256), 0, ctx->stream, pa);
-    else
-      hipLaunchKernelGGL(pair_plane_kernel<0>, grid, dim3(256), 0, ctx->stream, pa);
+    auto grid_of = [&](int spw) {
+      return dim3((unsigned)((m->F - pa.mlo + 63) / 64), (unsigned)((scur + kWaves * spw - 1) / (kWaves * spw)));
+    };
+    auto launch13 = [&](int spw) {
+      if (spw == 2)
+        hipLaunchKernelGGL((pair_plane_kernel<13, false, 2>), grid_of(2), dim3(256), 0, ctx->stream, pa);
+      else if (spw == 8)
+        hipLaunchKernelGGL((pair_plane_kernel<13, false, 8>), grid_of(8), dim3(256), 0, ctx->stream, pa);
+      else
+        hipLaunchKernelGGL((pair_plane_kernel<13, false, 4>), grid_of(4), dim3(256), 0, ctx->stream, pa);
+    };
+    if (t.dk) {
+      hipLaunchKernelGGL((pair_plane_kernel<0, true>), grid_of(4), dim3(256), 0, ctx->stream, pa);
+    } else if (rule.Q == 13) {
+      if (ctx->fill_spw == 0 && k == 0) {
+        // autotune on this slab: each candidate rewrites the same planes
+        hipEvent_t e0, e1;
+        MOM_HIP(hipEventCreate(&e0));
+        MOM_HIP(hipEventCreate(&e1));
+        const int cand[3] = {2, 4, 8};
+        float best = 1e30f;
+        for (int c = 0; c < 3; c++) {
+          MOM_HIP(hipEventRecord(e0, ctx->stream));
+          launch13(cand[c]);
+          MOM_HIP(hipGetLastError());
+          MOM_HIP(hipEventRecord(e1, ctx->stream));
+          MOM_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          MOM_HIP(hipEventElapsedTime(&ms, e0, e1));
+          ctx->fill_spw_ms[c] = ms;
+          if (ms < best) {
+            best = ms;
+            ctx->fill_spw = cand[c];
+          }
+        }
+        MOM_HIP(hipEventDestroy(e0));
+        MOM_HIP(hipEventDestroy(e1));
+      }
+      launch13(ctx->fill_spw ? ctx->fill_spw : 4);
+    } else {
+      hipLaunchKernelGGL(pair_plane_kernel<0>, grid_of(4), dim3(256), 0, ctx->stream, pa);
+    }
     MOM_HIP(hipGetLastError());
     const int ne = m->h_slab_ptr[k + 1] - m->h_slab_ptr[k];
     if (ne > 0) {

@@ -87,6 +87,10 @@ struct mom_ctx {
   mom::DevBuf ublk;  // E = I - inv(L11) of the last factored panel (complex64, NB x NB; lu.hip)
   void *blas2 = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // fill autotuner: source faces per wave of the pair kernel chosen by timing
+  // the candidates on the first slab of the first fill (0: not yet tuned)
+  int fill_spw = 0;
+  double fill_spw_ms[3] = {0, 0, 0};  // measured slab times of the candidates 2, 4, 8
 };
 
 // Device copy of one domain mesh, with per-face-side RWG data precomputed.

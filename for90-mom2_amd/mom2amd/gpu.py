@@ -54,6 +54,12 @@ class Context:
     def _adopt(self, child):
         self._children.add(child)
 
+    def fill_tuning(self):
+        """(source faces per wave chosen by the fill's autotuner, {candidate: ms})."""
+        spw, ms = C.c_int32(), np.zeros(3)
+        capi.check(capi.lib().mom_ctx_fill_tuning(self.h, C.byref(spw), ms))
+        return spw.value, {2: ms[0], 4: ms[1], 8: ms[2]}
+
     def close(self):
         """Destroy the context; meshes and systems created on it are
         destroyed first (they hold device memory and the context's stream)."""

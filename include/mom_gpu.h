@@ -44,6 +44,12 @@ const char *mom_last_error(void);
 /* One context per host thread and device; owns the HIP stream. */
 int mom_ctx_create(int32_t device, mom_ctx **ctx);
 int mom_ctx_destroy(mom_ctx *ctx);
+/* The fill's tile choice in this context: the pair kernel's source faces per
+ * wave (2, 4 or 8), picked on the first fill with the 13-point rule by timing
+ * each candidate on the first slab of source faces (0 before that fill), and
+ * the measured times [ms] of the candidates 2, 4, 8 (ms may be NULL).  The
+ * choice changes no result: each face pair's arithmetic is the same. */
+int mom_ctx_fill_tuning(const mom_ctx *ctx, int32_t *spw, double *ms);
 
 /*
  * Upload one domain mesh (a flattened mesh_container, mesh.f90:75-83, as
