@@ -277,7 +277,10 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
 //    in 32 different columns.
 // Rows of A beyond M are clamped to M - 2 (M even): they only feed output
 // rows that are not stored.
-constexpr int kDmaBK = 8;
+#ifndef MOM_DMA_BK
+#define MOM_DMA_BK 8
+#endif
+constexpr int kDmaBK = MOM_DMA_BK;  // K depth of one transfer stage of the 3M trailing update
 constexpr int kDmaBM = 64, kDmaBN = 128;
 
 __global__ __launch_bounds__(256) void transpose_panel_kernel(const float2 *__restrict__ B, int64_t ldb, int K,
