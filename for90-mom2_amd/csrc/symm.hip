@@ -114,6 +114,17 @@ static void edge_bc(int64_t N, const int32_t *bnd, const Group &G, int rep, std:
     }
 }
 
+// Boundary classes other than none and the mirror planes (the periodic and
+// rotation classes, mesh.f90:20-25) carry conditions this path does not
+// build (bc.f90:55-110); a problem that uses them is rejected.
+static int check_bnd(int64_t N, const int32_t *bnd) {
+  for (int64_t k = 0; k < N; k++)
+    if (bnd[k] < kBndNone || bnd[k] > kBndZplane)
+      return fail(MOM_ERR_ARG, "edge " + std::to_string(k) + " has boundary class " + std::to_string(bnd[k]) +
+                                   ": only none (1) and the mirror planes (2, 3, 4) are supported");
+  return 0;
+}
+
 // reduce_system's numbering (bc.f90:170-193): the kept unknowns in order
 static int64_t reduce_map(const std::vector<int> &id, std::vector<int> &map) {
   map.assign(id.size(), -1);
@@ -140,6 +151,7 @@ int fill_system_symm(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, co
     if (doms[d]->ctx != ctx) return fail(MOM_ERR_ARG, "domain mesh belongs to another context");
     if (doms[d]->max_parent >= N) return fail(MOM_ERR_ARG, "domain edge parent index >= nedges_total");
   }
+  MOM_TRY(check_bnd(N, bnd));
   std::vector<DevBuf> dmap(G.n);
   RepOut out;
   for (int r = 0; r < G.n; r++) {
@@ -225,6 +237,7 @@ int mom_symm_edge_bc(int64_t nedges_total, const int32_t *bnd, int32_t ngen, con
   Group G;
   MOM_TRY(mirror_group(ngen, planes, &G));
   if (rep < 0 || rep >= G.n) return fail(MOM_ERR_ARG, "representation index out of range");
+  MOM_TRY(check_bnd(nedges_total, bnd));
   std::vector<int> v, map;
   edge_bc(nedges_total, bnd, G, rep, v);
   *dim = reduce_map(v, map);

@@ -109,3 +109,16 @@ def test_sphere_cells_close_into_the_octasphere():
     for sub in itertools.chain.from_iterable(itertools.combinations(range(3), k) for k in range(3)):
         _, sf, sl, _ = meshgen.sphere_cell(n, 1.0, sub)
         assert sf.shape[0] == n * n * 2 ** len(sub) and sl.shape[0] == n * (3 - len(sub)) * 2 ** len(sub)
+
+
+def test_edge_bc_rejects_periodic_and_rotation_classes():
+    """Only the mirror classes are built: a periodic (5..8) or rotation (9,
+    10) boundary class is an error, not a silently unbounded edge."""
+    from mom2amd.capi import MomError
+    bnd = np.ones(10, np.int32)
+    bnd[3] = 5
+    with pytest.raises(MomError):
+        symm_edge_bc(bnd, ["mxp"], 0)
+    bnd[3] = 2
+    ids, dim = symm_edge_bc(bnd, ["mxp"], 0)
+    assert dim == 19 and ids[3] == -1
