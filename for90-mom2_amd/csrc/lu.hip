@@ -281,7 +281,11 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
 #define MOM_DMA_BK 8
 #endif
 constexpr int kDmaBK = MOM_DMA_BK;  // K depth of one transfer stage of the 3M trailing update
-constexpr int kDmaBM = 64, kDmaBN = 128;
+#ifndef MOM_DMA_TM
+#define MOM_DMA_TM 1
+#endif
+constexpr int kDmaTM = MOM_DMA_TM;  // 32-row MFMA tiles per wave along M (1: 32x64 wave tile, 2: 64x64)
+constexpr int kDmaBM = 64 * kDmaTM, kDmaBN = 128;
 
 __global__ __launch_bounds__(256) void transpose_panel_kernel(const float2 *__restrict__ B, int64_t ldb, int K,
                                                               int N, float2 *__restrict__ Bt, int64_t ldbt) {
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(256) void gemm3m_dma_kernel(const float2 *__restric
                                                          const float2 *__restrict__ Bt, int64_t ldbt,
                                                          float2 *__restrict__ Cm, int64_t ldc, int M, int N, int gm,
                                                          int gn) {
-  constexpr int BK = kDmaBK, BM = kDmaBM, BN = kDmaBN, TM = 1, TN = 2;
+  constexpr int BK = kDmaBK, BM = kDmaBM, BN = kDmaBN, TM = kDmaTM, TN = 2;
   constexpr int NA = BK * BM / 128, NB = BK * BN / 128;  // 1 KB transfers per stage
   static_assert(NA % 4 == 0 && NB % 4 == 0, "transfers per stage must split over 4 waves");
   constexpr int PER_WAVE = NA / 4 + NB / 4;
