@@ -1787,7 +1787,7 @@ __device__ void vcoef(const VPairArgs &g, int ft, int fs, double2 a[3][3], doubl
 
 __device__ double2 vpair_value(const VPairArgs &g, const VFace &fm, const VFace &fn, bool near, bool self,
                                const double2 amn[3][3], const double2 bmn[3][3], const double2 anm[3][3],
-                               const double2 bnm[3][3]) {
+                               const double2 bnm[3][3], bool analytic_only = false) {
   double2 ac[3][3], bc[3][3];
   for (int p = 0; p < 3; p++)
     for (int q = 0; q < 3; q++) {
@@ -1795,7 +1795,7 @@ __device__ double2 vpair_value(const VPairArgs &g, const VFace &fm, const VFace 
       ac[p][q] = self ? amn[p][q] : make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
       bc[p][q] = self ? bmn[p][q] : make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
     }
-  double2 v = vfar_part(g, fm, fn, near, self, ac, bc);
+  double2 v = analytic_only ? make_double2(0.0, 0.0) : vfar_part(g, fm, fn, near, self, ac, bc);
   if (near) {
     const double2 n1 = vnear_part(g, fm, fn, self, amn, bmn);
     v.x += n1.x;
@@ -1819,6 +1819,13 @@ __device__ double2 vpair_value(const VPairArgs &g, const VFace &fm, const VFace 
 // with respect to e, f, d, R and alpha are carried to the barycentric
 // points, the free vertices and the edge lengths.  gt/gs: Re d/d(node) of
 // the test and source faces' nodes (local order).
+// NEAR: the smoothed kernels of near pairs, Gs = [(e^{ikR} - 1)/R + k^2 R/2]/4pi
+// and gs = [e^{ikR}(1/R^3 - ik/R^2) - 1/R^3 - k^2/(2R)]/4pi, with
+// Gs' = [ik e^{ikR}/R - (e^{ikR} - 1)/R^2 + k^2/2]/4pi and
+// gs' = [ik e^{ikR}(1/R^3 - ik/R^2) + e^{ikR}(2ik/R^3 - 3/R^4) + 3/R^4 + k^2/(2R^2)]/4pi;
+// coincident points (R = 0, self pairs) contribute Gs(0) = ik/4pi and no
+// derivative.  A self pair passes b = 0 (its K block is zero).
+template <bool NEAR>
 __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, const double2 a[3][3],
                           const double2 b[3][3], double gt[3][3], double gs[3][3]) {
   double al_t[3], al_s[3], ut[3][3], us[3][3], vt[3][3], vs[3][3];
@@ -1883,16 +1890,41 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
       vface_point(fs, g.rule, j, rs);
       const double d[3] = {r[0] - rs[0], r[1] - rs[1], r[2] - rs[2]};
       const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      const double iR = rsqrt(R2), R = R2 * iR;
-      double Er, Ei;
-      cis_exp(-ki * R, kr * R, Er, Ei);
-      const double2 G = make_double2(iR * kOofpi * Er, iR * kOofpi * Ei);
-      const double2 gg = cm(G, make_double2(iR * iR + ki * iR, -kr * iR));
-      // G' = G (ik - 1/R) = G (-ki - 1/R + i kr)
-      const double2 Gp = cm(G, make_double2(-ki - iR, kr));
-      // g' = G (k^2/R - 3/R^3 + 3ik/R^2)
-      const double2 gp = cm(G, make_double2(k2r * iR - 3.0 * iR * iR * iR - 3.0 * ki * iR * iR,
-                                            k2i * iR + 3.0 * kr * iR * iR));
+      double iR;
+      double2 G, gg, Gp, gp;
+      if (!NEAR) {
+        iR = rsqrt(R2);
+        const double R = R2 * iR;
+        double Er, Ei;
+        cis_exp(-ki * R, kr * R, Er, Ei);
+        G = make_double2(iR * kOofpi * Er, iR * kOofpi * Ei);
+        gg = cm(G, make_double2(iR * iR + ki * iR, -kr * iR));
+        // G' = G (ik - 1/R) = G (-ki - 1/R + i kr)
+        Gp = cm(G, make_double2(-ki - iR, kr));
+        // g' = G (k^2/R - 3/R^3 + 3ik/R^2)
+        gp = cm(G, make_double2(k2r * iR - 3.0 * iR * iR * iR - 3.0 * ki * iR * iR, k2i * iR + 3.0 * kr * iR * iR));
+      } else if (R2 == 0.0) {
+        iR = 0.0;
+        G = make_double2(-ki * kOofpi, kr * kOofpi);
+        gg = Gp = gp = make_double2(0.0, 0.0);
+      } else {
+        iR = rsqrt(R2);
+        const double R = R2 * iR, iR2 = iR * iR, iR3 = iR2 * iR, iR4 = iR2 * iR2;
+        double Er, Ei;
+        cis_exp(-ki * R, kr * R, Er, Ei);
+        const double2 E = make_double2(Er, Ei), ik = make_double2(-ki, kr);
+        const double2 Em1 = make_double2(Er - 1.0, Ei);
+        G = make_double2((Em1.x * iR + 0.5 * k2r * R) * kOofpi, (Em1.y * iR + 0.5 * k2i * R) * kOofpi);
+        const double2 h = make_double2(iR3 + ki * iR2, -kr * iR2);  // 1/R^3 - ik/R^2
+        const double2 Eh = cm(E, h);
+        gg = make_double2((Eh.x - iR3 - 0.5 * k2r * iR) * kOofpi, (Eh.y - 0.5 * k2i * iR) * kOofpi);
+        const double2 ikE = cm(ik, E);
+        Gp = make_double2((ikE.x * iR - Em1.x * iR2 + 0.5 * k2r) * kOofpi, (ikE.y * iR - Em1.y * iR2 + 0.5 * k2i) * kOofpi);
+        const double2 ikEh = cm(ikE, h);
+        const double2 Ehp = cm(E, make_double2(-3.0 * iR4 - 2.0 * ki * iR3, 2.0 * kr * iR3));  // e^{ikR}(2ik/R^3 - 3/R^4)
+        gp = make_double2((ikEh.x + Ehp.x + 3.0 * iR4 + 0.5 * k2r * iR2) * kOofpi,
+                          (ikEh.y + Ehp.y + 0.5 * k2i * iR2) * kOofpi);
+      }
       const double ww = g.rule.w[i] * g.rule.w[j];
       double f[3][3];
       for (int q = 0; q < 3; q++)
@@ -2037,7 +2069,7 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
 // Re d Phi / d(node coordinate) of the unordered face pair (m <= n) for its
 // distinct nodes, added to dvert (far pairs in closed form, near pairs by
 // differences).
-__device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near) {
+__device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near, bool analytic_only = false) {
   const bool self = (m == n);
   VFace fm, fn;
   vload_face(g, m, fm);
@@ -2074,7 +2106,7 @@ __device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near) {
           if (g.fnodes[3 * m + v] == node) pm.P[v][c] += steps[st];
           if (g.fnodes[3 * n + v] == node) pn.P[v][c] += steps[st];
         }
-        val[st] = vpair_value(g, pm, pn, near, self, amn, bmn, anm, bnm).x;
+        val[st] = vpair_value(g, pm, pn, near, self, amn, bmn, anm, bnm, analytic_only).x;
       }
       unsafeAtomicAdd(g.dvert + 3 * node + c, (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h));
     }
@@ -2108,7 +2140,7 @@ __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
       bmn[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
     }
   double gt[3][3], gs[3][3];
-  vfar_grad(g, fm, fn, amn, bmn, gt, gs);
+  vfar_grad<false>(g, fm, fn, amn, bmn, gt, gs);
   for (int v = 0; v < 3; v++)
     for (int c = 0; c < 3; c++) {
       unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * m + v] + c, gt[v][c]);
@@ -2116,13 +2148,39 @@ __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
     }
 }
 
-// One thread per entry of the near list with test <= source.
+// One thread per entry of the near list with test <= source: the smoothed
+// point loop in closed form, the int.f90 analytic parts by differences.
 __global__ __launch_bounds__(128) void vertex_near_kernel(VPairArgs g) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= g.npairs) return;
   const int m = g.pairs[2 * id], n = g.pairs[2 * id + 1];
   if (n < m) return;
-  vertex_pair(g, m, n, true);
+  const bool self = (m == n);
+  {
+    VFace fm, fn;
+    vload_face(g, m, fm);
+    vload_face(g, n, fn);
+    double2 amn[3][3], bmn[3][3], anm[3][3], bnm[3][3];
+    vcoef(g, m, n, amn, bmn);
+    vcoef(g, n, m, anm, bnm);
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) {
+        if (self) {
+          bmn[p][q] = make_double2(0.0, 0.0);
+        } else {
+          amn[p][q] = make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
+          bmn[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
+        }
+      }
+    double gt[3][3], gs[3][3];
+    vfar_grad<true>(g, fm, fn, amn, bmn, gt, gs);
+    for (int v = 0; v < 3; v++)
+      for (int c = 0; c < 3; c++) {
+        unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * m + v] + c, gt[v][c]);
+        unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * n + v] + c, gs[v][c]);
+      }
+  }
+  vertex_pair(g, m, n, true, true);
 }
 
 // Per face of the exterior domain: Psi = the face's share of sigma_abs

@@ -63,7 +63,8 @@ int mom_test_u_solve(int32_t mode);
  * leading dimension 256) is returned widened to complex128.
  */
 /* Node-coordinate adjoint: 1 = every face pair by fourth-order differences
- * (the far pairs' closed form is checked against it), 0 = the default. */
+ * (the closed forms of the far pairs and of the near pairs' smoothed point
+ * loop are checked against it), 0 = the default. */
 int mom_test_vertex_fd(int32_t enable);
 int mom_test_unit_lower_inverse(mom_ctx *ctx, const double *L, int64_t ldl, double *E);
 

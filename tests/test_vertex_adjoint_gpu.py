@@ -106,9 +106,10 @@ def test_vertex_gradient_matches_finite_differences(gpu_ctx):
 
 
 def test_far_pair_closed_form_matches_differences(gpu_ctx):
-    """The far pairs' closed-form derivative against fourth-order differences
-    of the same pair sums (mom_test_vertex_fd), per domain and node
-    coordinate: <= 1e-8 of the largest component."""
+    """The closed-form derivatives (far pairs, and the near pairs' smoothed
+    point loop) against fourth-order differences of the same pair sums
+    (mom_test_vertex_fd), per domain and node coordinate: <= 1e-8 of the
+    largest component."""
     import ctypes as C  # noqa: F401
     from mom2amd import capi
     from mom2amd.gpu import _quad
