@@ -256,7 +256,6 @@ int mom_ctx_destroy(mom_ctx *ctx) {
   ctx->panel_t.release();
   ctx->panel_t2.release();
   ctx->ublk.release();
-  release_blas(ctx);
   if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
   if (ctx->aux) (void)hipStreamDestroy(ctx->aux);

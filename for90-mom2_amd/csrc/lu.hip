@@ -7,7 +7,8 @@
 // Factorisation: blocked right-looking LU (outer block NB = 256) whose
 // trailing update A22 -= L21 U12 runs on the matrix cores
 // (v_mfma_f64_16x16x4f64 for complex128 with four real products per complex
-// product; v_mfma_f32_16x16x4f32 for complex64 with three, Gauss's form).
+// product; v_mfma_f32_32x32x2f32 for complex64 with three, Gauss's form),
+// both fed by direct global->LDS transfers.
 // The panel is factored in inner blocks of IB = 32 columns with one launch
 // per column: every block reduces the previous launch's per-block pivot
 // candidates (|re|+|im|, lowest index on ties, as LAPACK's IZAMAX), the
@@ -34,7 +35,6 @@
 // FP32 factors, refines the solution until the complex128 backward error
 // ||b - Zx||_inf / (||Z||_inf ||x||_inf + ||b||_inf) reaches the tolerance.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdint>
@@ -112,11 +112,13 @@ __device__ __forceinline__ int xcd_contiguous(int pid, int G) {
   return x < tall ? x * per + l : tall * per + (x - tall) * (per - 1) + l;
 }
 
+// C = beta C + sgn A B with sgn = +-1 and beta = 0 or 1 (beta0 != 0: C is
+// only written); the factorisation's updates are sgn = -1, beta = 1.
 template <typename T, int WM, int WN, int BK, bool M3>
 __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::type *__restrict__ A, int64_t lda,
                                                        const typename Cx<T>::type *__restrict__ B, int64_t ldb,
                                                        typename Cx<T>::type *__restrict__ Cm, int64_t ldc, int M,
-                                                       int N, int K, int gm, int gn) {
+                                                       int N, int K, int gm, int gn, T sgn, int beta0) {
   using C2 = typename Cx<T>::type;
   using acc4 = typename Cx<T>::acc4;
   constexpr int BM = 32 * WM, BN = 32 * WN;
@@ -243,13 +245,19 @@ __global__ __launch_bounds__(256) void gemm_sub_kernel(const typename Cx<T>::typ
         const int64_t row = row0 + wr + 16 * i + Cx<T>::acc_row(kq, v);
         if (row < M) {
           C2 *p = Cm + row + col * ldc;
-          C2 c = *p;
-          if (M3) {
-            c.x -= acc0[i][j][v] - acc1[i][j][v];
-            c.y -= acc2[i][j][v] - acc0[i][j][v] - acc1[i][j][v];
+          C2 c;
+          if (beta0) {
+            c.x = 0;
+            c.y = 0;
           } else {
-            c.x -= acc0[i][j][v];
-            c.y -= acc1[i][j][v];
+            c = *p;
+          }
+          if (M3) {
+            c.x += sgn * (acc0[i][j][v] - acc1[i][j][v]);
+            c.y += sgn * (acc2[i][j][v] - acc0[i][j][v] - acc1[i][j][v]);
+          } else {
+            c.x += sgn * acc0[i][j][v];
+            c.y += sgn * acc1[i][j][v];
           }
           *p = c;
         }
@@ -287,14 +295,18 @@ constexpr int kDmaBK = MOM_DMA_BK;  // K depth of one transfer stage of the 3M t
 constexpr int kDmaTM = MOM_DMA_TM;  // 32-row MFMA tiles per wave along M (1: 32x64 wave tile, 2: 64x64)
 constexpr int kDmaBM = 64 * kDmaTM, kDmaBN = 128;
 
-__global__ __launch_bounds__(256) void transpose_panel_kernel(const float2 *__restrict__ B, int64_t ldb, int K,
-                                                              int N, float2 *__restrict__ Bt, int64_t ldbt) {
-  __shared__ float2 t[32][33];
+template <typename C2>
+__global__ __launch_bounds__(256) void transpose_panel_kernel(const C2 *__restrict__ B, int64_t ldb, int K, int N,
+                                                              C2 *__restrict__ Bt, int64_t ldbt) {
+  __shared__ C2 t[32][33];
   const int c0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   for (int r = ty; r < 32; r += 8) {
     const int c = c0 + r, k = k0 + tx;
-    t[r][tx] = (c < N && k < K) ? B[k + (int64_t)c * ldb] : make_float2(0.f, 0.f);
+    C2 z;
+    z.x = 0;
+    z.y = 0;
+    t[r][tx] = (c < N && k < K) ? B[k + (int64_t)c * ldb] : z;
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
@@ -395,23 +407,23 @@ __global__ __launch_bounds__(256) void gemm3m_dma_kernel(const float2 *__restric
     MOM_COMPILER_FENCE();
     __builtin_amdgcn_s_barrier();
     MOM_COMPILER_FENCE();
-    if (it + 2 < NK) issue(it + 2, stageA((it + 2) % 3), stageB((it + 2) % 3));
+    // every fragment of this stage is read before the next transfers are
+    // issued: the compiler makes an LDS read that follows a direct transfer
+    // wait for that transfer
     float2(*as)[BM] = stageA(it % 3);
     float2(*bs)[BN] = stageB(it % 3);
-    float2 fa[2][TM], fb[2][TN];
+    float2 fa[BK / 2][TM], fb[BK / 2][TN];
 #pragma unroll
-    for (int i = 0; i < TM; i++) fa[0][i] = as[kh][wr + 32 * i + l32];
+    for (int q = 0; q < BK / 2; q++) {
 #pragma unroll
-    for (int j = 0; j < TN; j++) fb[0][j] = bs[kh][wc + 32 * j + l32];
+      for (int i = 0; i < TM; i++) fa[q][i] = as[2 * q + kh][wr + 32 * i + l32];
+#pragma unroll
+      for (int j = 0; j < TN; j++) fb[q][j] = bs[2 * q + kh][wc + 32 * j + l32];
+    }
+    if (it + 2 < NK) issue(it + 2, stageA((it + 2) % 3), stageB((it + 2) % 3));
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 2) {
-      const int cur = (kk / 2) & 1;
-      if (kk + 2 < BK) {
-#pragma unroll
-        for (int i = 0; i < TM; i++) fa[cur ^ 1][i] = as[kk + 2 + kh][wr + 32 * i + l32];
-#pragma unroll
-        for (int j = 0; j < TN; j++) fb[cur ^ 1][j] = bs[kk + 2 + kh][wc + 32 * j + l32];
-      }
+      const int cur = kk / 2;
       float aS[TM], bS[TN];
 #pragma unroll
       for (int i = 0; i < TM; i++) aS[i] = fa[cur][i].x + fa[cur][i].y;
@@ -479,6 +491,210 @@ __global__ __launch_bounds__(256) void gemm3m_dma_kernel(const float2 *__restric
       }
 }
 
+// ---------------------------------------------------------------------------
+// Trailing update C -= A B for complex128 with K = 32 or 256 on the FP64
+// matrix cores, fed by direct global->LDS transfers like the complex64
+// kernel above:
+//  - B (K x N, column-major) is first transposed into Bt[k][c]
+//    (transpose_panel_kernel), so that k-rows of both operands are
+//    contiguous;
+//  - each stage holds BK = 8 k-rows of the A tile (128 rows) and of the B
+//    tile (64 columns) as interleaved complex; a transfer moves 1 KB with one
+//    complex (16 bytes) per lane; three stages in flight, one barrier per
+//    stage; the K loop is fully unrolled (explicit vmcnt waits only);
+//  - 4 waves in 2 x 2, wave tile 64 x 32 complex = 4 x 2 blocks of
+//    v_mfma_f64_16x16x4f64; fragments are whole complex (one 16-byte LDS
+//    read gives the real and imaginary operand of a lane), read for the
+//    next k-step while the current MFMAs run;
+//  - M3 = false: four real products per complex product (the operation
+//    count of ZGEMM); M3 = true: three (Gauss), with the sums formed in
+//    registers;
+//  - the MFMAs of one k-step are issued product by product over all eight
+//    blocks, so that consecutive instructions never feed the same
+//    accumulator;
+//  - C is updated in place from the accumulators: lane l holds rows
+//    l/16 + 4v (v = 0..3) of column l%16 of each 16x16 block.
+// Rows of A beyond M are clamped to M - 1: they only feed output rows that
+// are not stored.
+#ifndef MOM_Z_TM
+#define MOM_Z_TM 4
+#define MOM_Z_TN 2
+#endif
+constexpr int kZTM = MOM_Z_TM, kZTN = MOM_Z_TN;  // 16x16 blocks per wave along M and N
+constexpr int kZBK = 8, kZBM = 32 * kZTM, kZBN = 32 * kZTN;
+
+template <int NK, bool M3>
+__global__ __launch_bounds__(256) void zgemm_dma_kernel(const double2 *__restrict__ A, int64_t lda,
+                                                        const double2 *__restrict__ Bt, int64_t ldbt,
+                                                        double2 *__restrict__ Cm, int64_t ldc, int M, int N, int gm,
+                                                        int gn) {
+  constexpr int BK = kZBK, BM = kZBM, BN = kZBN, TM = kZTM, TN = kZTN;
+  constexpr int NA = BK * BM / 64, NBT = BK * BN / 64;  // 1 KB transfers per stage
+  static_assert(NA % 4 == 0 && NBT % 4 == 0, "transfers per stage must split over 4 waves");
+  constexpr int PER_WAVE = NA / 4 + NBT / 4;
+  typedef double acc4 __attribute__((ext_vector_type(4)));
+  __shared__ double2 As0[BK][BM], As1[BK][BM], As2[BK][BM];
+  __shared__ double2 Bs0[BK][BN], Bs1[BK][BN], Bs2[BK][BN];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, kq = lane >> 4;
+  const int wr = (wave & 1) * 16 * TM, wc = (wave >> 1) * 16 * TN;
+  const int pid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int per_group = kGroupM * gn;
+  const int first_m = (pid / per_group) * kGroupM;
+  const int gsize = min(gm - first_m, kGroupM);
+  const int pm = first_m + (pid % per_group) % gsize;
+  const int pn = (pid % per_group) / gsize;
+  const int64_t row0 = (int64_t)pm * BM, col0 = (int64_t)pn * BN;
+  // transfer t of this wave covers stage element e = 64 (wave + 4 t) + lane
+  const double2 *pa[NA / 4];
+  const double2 *pb[NBT / 4];
+#pragma unroll
+  for (int t = 0; t < NA / 4; t++) {
+    const int e = 64 * (wave + 4 * t) + lane, k = e / BM, r = e % BM;
+    int64_t gr = row0 + r;
+    if (gr > M - 1) gr = M - 1;
+    pa[t] = A + gr + (int64_t)k * lda;
+  }
+#pragma unroll
+  for (int t = 0; t < NBT / 4; t++) {
+    const int e = 64 * (wave + 4 * t) + lane, k = e / BN, c = e % BN;
+    pb[t] = Bt + col0 + c + (int64_t)k * ldbt;
+  }
+  const int64_t astep = (int64_t)BK * lda, bstep = (int64_t)BK * ldbt;
+  auto issue = [&](int c, double2(*as)[BM], double2(*bs)[BN]) {
+#pragma unroll
+    for (int t = 0; t < NA / 4; t++)
+      __builtin_amdgcn_global_load_lds((const void *)(pa[t] + c * astep), MOM_LDS(&as[0][0] + 64 * (wave + 4 * t)),
+                                       16, 0, 0);
+#pragma unroll
+    for (int t = 0; t < NBT / 4; t++)
+      __builtin_amdgcn_global_load_lds((const void *)(pb[t] + c * bstep), MOM_LDS(&bs[0][0] + 64 * (wave + 4 * t)),
+                                       16, 0, 0);
+  };
+  auto stageA = [&](int st) -> double2(*)[BM] { return st == 0 ? As0 : (st == 1 ? As1 : As2); };
+  auto stageB = [&](int st) -> double2(*)[BN] { return st == 0 ? Bs0 : (st == 1 ? Bs1 : Bs2); };
+  constexpr int NACC = M3 ? 3 : 2;
+  acc4 acc[NACC][TM][TN];
+#pragma unroll
+  for (int a = 0; a < NACC; a++)
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+      for (int j = 0; j < TN; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[a][i][j][v] = 0.0;
+  issue(0, As0, Bs0);
+  if (NK > 1) issue(1, As1, Bs1);
+#pragma unroll
+  for (int it = 0; it < NK; it++) {
+    // this wave's transfers of stage `it` have landed and its reads of the
+    // previous stage are complete; after the barrier the same holds for
+    // every wave, so stage `it` is readable and stage it + 2 (= it - 1)
+    // can be refilled
+    if (it + 1 < NK)
+      wait_stage<PER_WAVE>();
+    else
+      wait_stage<0>();
+    MOM_COMPILER_FENCE();
+    __builtin_amdgcn_s_barrier();
+    MOM_COMPILER_FENCE();
+    // every fragment of this stage is read before the next transfers are
+    // issued: the compiler makes an LDS read that follows a direct transfer
+    // wait for that transfer
+    double2(*as)[BM] = stageA(it % 3);
+    double2(*bs)[BN] = stageB(it % 3);
+    double2 fa[BK / 4][TM], fb[BK / 4][TN];
+#pragma unroll
+    for (int q = 0; q < BK / 4; q++) {
+#pragma unroll
+      for (int i = 0; i < TM; i++) fa[q][i] = as[4 * q + kq][wr + 16 * i + l16];
+#pragma unroll
+      for (int j = 0; j < TN; j++) fb[q][j] = bs[4 * q + kq][wc + 16 * j + l16];
+    }
+    if (it + 2 < NK) issue(it + 2, stageA((it + 2) % 3), stageB((it + 2) % 3));
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int cur = kk / 4;
+      if (M3) {
+        double aS[TM], bS[TN];
+#pragma unroll
+        for (int i = 0; i < TM; i++) aS[i] = fa[cur][i].x + fa[cur][i].y;
+#pragma unroll
+        for (int j = 0; j < TN; j++) bS[j] = fb[cur][j].x + fb[cur][j].y;
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[cur][i].x, fb[cur][j].x, acc[0][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[cur][i].y, fb[cur][j].y, acc[1][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[NACC - 1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aS[i], bS[j], acc[NACC - 1][i][j], 0, 0, 0);
+      } else {
+        double nai[TM];
+#pragma unroll
+        for (int i = 0; i < TM; i++) nai[i] = -fa[cur][i].y;
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[cur][i].x, fb[cur][j].x, acc[0][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[cur][i].x, fb[cur][j].y, acc[1][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[0][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[i], fb[cur][j].y, acc[0][i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++)
+            acc[1][i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[cur][i].y, fb[cur][j].x, acc[1][i][j], 0, 0, 0);
+      }
+    }
+  }
+  auto result = [&](int i, int j, int v) -> double2 {
+    if (M3)
+      return make_double2(acc[0][i][j][v] - acc[1][i][j][v],
+                          acc[NACC - 1][i][j][v] - acc[0][i][j][v] - acc[1][i][j][v]);
+    return make_double2(acc[0][i][j][v], acc[1][i][j][v]);
+  };
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      const int64_t col = col0 + wc + 16 * j + l16;
+      if (col >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int64_t row = row0 + wr + 16 * i + kq + 4 * v;
+        if (row < M) {
+          double2 *p = Cm + row + col * ldc;
+          double2 c = *p;
+          const double2 r = result(i, j, v);
+          c.x -= r.x;
+          c.y -= r.y;
+          *p = c;
+        }
+      }
+    }
+}
+
+#ifndef MOM_Z_M3
+#define MOM_Z_M3 false
+#endif
+constexpr bool kZM3 = MOM_Z_M3;  // complex128 trailing update by three real products instead of four
+
 // Tile configuration (wave tile 16 WM x 16 WN complex, K chunk BK); the
 // MOM_GEMM_* macros let a build choose another configuration.
 #ifndef MOM_GEMM_F32_WM
@@ -491,6 +707,10 @@ template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> {
   static constexpr int WM = MOM_GEMM_F32_WM, WN = MOM_GEMM_F32_WN, BK = MOM_GEMM_F32_BK;
   static constexpr bool M3 = MOM_GEMM_F32_M3;
+};
+template <> struct GemmCfg<double> {
+  static constexpr int WM = 2, WN = 2, BK = 8;
+  static constexpr bool M3 = false;
 };
 
 // Event-pair timing of the factorisation's kernel groups (enabled per
@@ -569,17 +789,15 @@ struct Timed {
   int done() { return (c && !rc) ? c->end(s, ph, fl) : rc; }
 };
 
-// A stream of the factorisation with the scratch its GEMMs use: the
-// transposed-panel buffer of the complex64 kernels and a rocBLAS handle
-// bound to the stream (created on first use).  Two lanes never share
-// scratch, so their launches may overlap.
+// A stream of the factorisation with the scratch its GEMMs use (the
+// transposed B panel of the direct-transfer kernels).  Two lanes never
+// share scratch, so their launches may overlap.
 struct Lane {
   hipStream_t s;
   DevBuf *bt;
-  void **blas;
 };
 
-static Lane main_lane(mom_ctx *ctx) { return Lane{ctx->stream, &ctx->panel_t, &ctx->blas}; }
+static Lane main_lane(mom_ctx *ctx) { return Lane{ctx->stream, &ctx->panel_t}; }
 
 // The look-ahead lane: second stream (default priority) and its events.
 static int aux_lane(mom_ctx *ctx, Lane *l) {
@@ -588,42 +806,56 @@ static int aux_lane(mom_ctx *ctx, Lane *l) {
     MOM_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     MOM_HIP(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
   }
-  *l = Lane{ctx->aux, &ctx->panel_t2, &ctx->blas2};
+  *l = Lane{ctx->aux, &ctx->panel_t2};
   return 0;
 }
 
-static int blas_handle(const Lane &L, rocblas_handle *h) {
-  if (!*L.blas) {
-    rocblas_handle b = nullptr;
-    if (rocblas_create_handle(&b) != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_create_handle failed");
-    if (rocblas_set_stream(b, L.s) != rocblas_status_success) {
-      rocblas_destroy_handle(b);
-      return fail(MOM_ERR_HIP, "rocblas_set_stream failed");
-    }
-    *L.blas = b;
-  }
-  *h = static_cast<rocblas_handle>(*L.blas);
-  return 0;
-}
-
-void release_blas(mom_ctx *ctx) {
-  if (ctx->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
-  if (ctx->blas2) rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas2));
-  ctx->blas = ctx->blas2 = nullptr;
-}
-
-// C -= A B on lane L.  complex64: the hand-written 3M
-// matrix-core kernels above (the direct-transfer kernel for K = 32 and 256,
-// the register-staged one otherwise).  complex128: rocBLAS ZGEMM (the plain library
-// GEMM; its FP64 matrix-core kernels are the strongest available baseline
-// for the all-FP64 path).
-// Whether the complex64 direct-transfer kernel takes C -= A B of these
-// shapes (it reads B through a transposed copy, so B may alias C).
+// Whether the direct-transfer kernels take C -= A B of these shapes (they
+// read B through a transposed copy, so B may alias C).  complex64 moves rows
+// in pairs (8-byte elements, 16-byte transfers).
 static bool dma_gemm_ok(int64_t M, int64_t K, int64_t lda, int64_t ldc, const void *A, const void *C) {
   return (K == 256 || K == 32) && M >= 2 && M % 2 == 0 && lda % 2 == 0 && ldc % 2 == 0 &&
          (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
 }
+static bool zdma_gemm_ok(int64_t M, int64_t K, const void *A, const void *B, const void *C) {
+  return (K == 256 || K == 32) && M >= 1 && (reinterpret_cast<uintptr_t>(A) & 15) == 0 &&
+         (reinterpret_cast<uintptr_t>(B) & 15) == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0;
+}
 
+// B (K x N, ldb) transposed into the lane's buffer Bt[k][c], padded with
+// zero columns to a multiple of bn; returns Bt and its row stride.
+template <typename C2>
+static int transpose_b(const Lane &L, const C2 *B, int64_t ldb, int64_t K, int64_t N, int64_t bn, C2 **Bt,
+                       int64_t *ldbt, PhaseClock *st) {
+  *ldbt = (N + bn - 1) / bn * bn;
+  const size_t bytes = sizeof(C2) * (size_t)K * *ldbt;
+  if (L.bt->bytes < bytes) MOM_TRY(L.bt->alloc(bytes));
+  *Bt = L.bt->as<C2>();
+  Timed tt(st, L.s, kPhGemmNarrow, 0.0);
+  hipLaunchKernelGGL(transpose_panel_kernel<C2>, dim3((unsigned)(*ldbt / 32), (unsigned)((K + 31) / 32)), dim3(256), 0,
+                     L.s, B, ldb, (int)K, (int)N, *Bt, *ldbt);
+  MOM_HIP(hipGetLastError());
+  return tt.done();
+}
+
+// C = beta C + sgn A B (sgn = +-1, beta = 0 or 1) by the register-staged
+// kernel, any shape.
+template <typename T>
+static int gemm_generic(hipStream_t s, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
+                        int64_t ldb, typename Cx<T>::type *C, int64_t ldc, int64_t M, int64_t N, int64_t K, T sgn,
+                        bool beta0) {
+  constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
+  const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
+  hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn), dim3(256),
+                     0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn, sgn, beta0 ? 1 : 0);
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
+// C -= A B on lane L, all by the hand-written matrix-core kernels:
+// complex64 by the 3M direct-transfer kernel for K = 32 and 256, complex128
+// by the FP64 direct-transfer kernel for K = 32 and 256, other depths by
+// the register-staged kernel.
 // u_solve: the product forms a U block (U12 = A12 - E A12, E from
 // unit_lower_inverse_kernel); it is timed with the triangular solves and
 // launched as its own kernel instance so profiles keep it apart from the
@@ -641,32 +873,33 @@ static int gemm_sub(const Lane &L, const typename Cx<T>::type *A, int64_t lda, c
   const Phase ph = u_solve ? kPhTrsm : K >= NB ? kPhGemm : kPhGemmNarrow;
   const double fl = u_solve ? 0.0 : 8.0 * (double)M * (double)N * (double)K;
   if constexpr (std::is_same<T, double>::value) {
+    if (u_solve) return fail(MOM_ERR_ARG, "U-block product is a complex64 path");
+    if (zdma_gemm_ok(M, K, A, B, C)) {
+      const int gm = (int)((M + kZBM - 1) / kZBM), gn = (int)((N + kZBN - 1) / kZBN);
+      double2 *Bt;
+      int64_t ldbt;
+      MOM_TRY(transpose_b<double2>(L, B, ldb, K, N, kZBN, &Bt, &ldbt, st));
+      Timed t(st, s, ph, fl);
+      if (K == 256)
+        hipLaunchKernelGGL((zgemm_dma_kernel<256 / kZBK, kZM3>), dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt,
+                           ldbt, C, ldc, (int)M, (int)N, gm, gn);
+      else
+        hipLaunchKernelGGL((zgemm_dma_kernel<32 / kZBK, kZM3>), dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt,
+                           ldbt, C, ldc, (int)M, (int)N, gm, gn);
+      MOM_HIP(hipGetLastError());
+      return t.done();
+    }
     Timed t(st, s, ph, fl);
-    rocblas_handle h;
-    MOM_TRY(blas_handle(L, &h));
-    const rocblas_double_complex alpha(-1.0, 0.0), beta(1.0, 0.0);
-    const rocblas_status rs = rocblas_zgemm(
-        h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &alpha,
-        reinterpret_cast<const rocblas_double_complex *>(A), (rocblas_int)lda,
-        reinterpret_cast<const rocblas_double_complex *>(B), (rocblas_int)ldb, &beta,
-        reinterpret_cast<rocblas_double_complex *>(C), (rocblas_int)ldc);
-    if (rs != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_zgemm failed: " + std::to_string((int)rs));
+    MOM_TRY(gemm_generic<double>(s, A, lda, B, ldb, C, ldc, M, N, K, -1.0, false));
     return t.done();
   } else {
     const bool dma = dma_gemm_ok(M, K, lda, ldc, A, C);
     if (u_solve && (!dma || K != NB)) return fail(MOM_ERR_ARG, "U-block product needs the depth-NB direct-transfer kernel");
     if (dma) {
       const int gm = (int)((M + kDmaBM - 1) / kDmaBM), gn = (int)((N + kDmaBN - 1) / kDmaBN);
-      const int64_t ldbt = (int64_t)gn * kDmaBN;
-      const size_t bytes = sizeof(float2) * (size_t)K * ldbt;
-      if (L.bt->bytes < bytes) MOM_TRY(L.bt->alloc(bytes));
-      float2 *Bt = L.bt->as<float2>();
-      {
-        Timed tt(st, s, kPhGemmNarrow, 0.0);
-        hipLaunchKernelGGL(transpose_panel_kernel, dim3((unsigned)(ldbt / 32), (unsigned)((K + 31) / 32)), dim3(256),
-                           0, s, B, ldb, (int)K, (int)N, Bt, ldbt);
-        MOM_TRY(tt.done());
-      }
+      float2 *Bt;
+      int64_t ldbt;
+      MOM_TRY(transpose_b<float2>(L, B, ldb, K, N, kDmaBN, &Bt, &ldbt, st));
       Timed t(st, s, ph, fl);
       if (K == 256 && u_solve)
         hipLaunchKernelGGL((gemm3m_dma_kernel<256 / kDmaBK, 1>), dim3((unsigned)gm * gn), dim3(256), 0, s, A, lda, Bt,
@@ -679,16 +912,32 @@ static int gemm_sub(const Lane &L, const typename Cx<T>::type *A, int64_t lda, c
                            C, ldc, (int)M, (int)N, gm, gn);
       MOM_HIP(hipGetLastError());
       return t.done();
-    } else {
-      Timed t(st, s, ph, fl);
-      constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
-      const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
-      hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn),
-                         dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn);
-      MOM_HIP(hipGetLastError());
-      return t.done();
     }
+    Timed t(st, s, ph, fl);
+    MOM_TRY(gemm_generic<float>(s, A, lda, B, ldb, C, ldc, M, N, K, -1.0f, false));
+    return t.done();
   }
+}
+
+// C = beta C + sgn A B in complex128 for the block solves and residuals
+// (sgn = +-1, beta = 0 or 1).  C -= A B with K = NB goes to the
+// direct-transfer kernel; a deep product C -= A B (K > NB, the residual
+// B - Z X) runs as depth-NB slices through it, and the rest through the
+// register-staged kernel.
+static int zgemm(const Lane &L, int64_t M, int64_t N, int64_t K, double sgn, bool beta0, const double2 *A,
+                 int64_t lda, const double2 *B, int64_t ldb, double2 *C, int64_t ldc) {
+  if (M <= 0 || N <= 0) return 0;
+  if (K <= 0) {
+    if (!beta0) return 0;
+    return gemm_generic<double>(L.s, A, lda, B, ldb, C, ldc, M, N, 0, sgn, true);
+  }
+  if (sgn < 0 && !beta0 && K >= NB && zdma_gemm_ok(M, NB, A, B, C)) {
+    int64_t k0 = 0;
+    for (; k0 + NB <= K; k0 += NB) MOM_TRY(gemm_sub<double>(L, A + k0 * lda, lda, B + k0, ldb, C, ldc, M, N, NB));
+    if (k0 < K) MOM_TRY(gemm_generic<double>(L.s, A + k0 * lda, lda, B + k0, ldb, C, ldc, M, N, K - k0, -1.0, false));
+    return 0;
+  }
+  return gemm_generic<double>(L.s, A, lda, B, ldb, C, ldc, M, N, K, sgn, beta0);
 }
 
 // ---------------------------------------------------------------------------
@@ -2111,19 +2360,6 @@ __global__ void widen_panel_kernel(const float2 *__restrict__ A, int64_t ld, int
   P[c * ldp + r] = make_double2(a.x, a.y);
 }
 
-static int zgemm(rocblas_handle h, int64_t M, int64_t N, int64_t K, cd alpha, const double2 *A, int64_t lda,
-                 const double2 *B, int64_t ldb, cd beta, double2 *C, int64_t ldc) {
-  if (M <= 0 || N <= 0 || K <= 0) return 0;
-  const rocblas_double_complex al(alpha.real(), alpha.imag()), be(beta.real(), beta.imag());
-  const rocblas_status rs =
-      rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, (rocblas_int)N, (rocblas_int)K,
-                    &al, reinterpret_cast<const rocblas_double_complex *>(A), (rocblas_int)lda,
-                    reinterpret_cast<const rocblas_double_complex *>(B), (rocblas_int)ldb, &be,
-                    reinterpret_cast<rocblas_double_complex *>(C), (rocblas_int)ldc);
-  if (rs != rocblas_status_success) return fail(MOM_ERR_HIP, "rocblas_zgemm failed: " + std::to_string((int)rs));
-  return 0;
-}
-
 // Solve L U X = P B in place on B (n x nrhs, ldb), complex128 arithmetic
 // with the factors stored in TS: per 256-row block the diagonal-block
 // inverse (complex128) times the block, then the panel below (forward) or
@@ -2135,8 +2371,7 @@ static int getrs_blocked(mom_ctx *ctx, const typename Cx<TS>::type *LU, int64_t 
                          const double2 *Linv, const double2 *Uinv, double2 *B, int64_t ldb, int64_t nrhs,
                          double2 *T, double2 *W, double2 *P) {
   hipStream_t s = ctx->stream;
-  rocblas_handle h;
-  MOM_TRY(blas_handle(main_lane(ctx), &h));
+  const Lane L = main_lane(ctx);
   hipLaunchKernelGGL(gather_rows_kernel<double2>, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, B, ldb,
                      T, n, perm, n, nrhs);
   MOM_HIP(hipGetLastError());
@@ -2156,26 +2391,26 @@ static int getrs_blocked(mom_ctx *ctx, const typename Cx<TS>::type *LU, int64_t 
   const int64_t nblk = (n + NB - 1) / NB;
   for (int64_t b = 0; b < nblk; b++) {
     const int64_t r0 = b * NB, w = std::min<int64_t>(NB, n - r0);
-    MOM_TRY(zgemm(h, w, nrhs, w, 1.0, Linv + b * NB * NB, NB, T + r0, n, 0.0, W, NB));
+    MOM_TRY(zgemm(L, w, nrhs, w, 1.0, true, Linv + b * NB * NB, NB, T + r0, n, W, NB));
     MOM_HIP(hipMemcpy2DAsync(T + r0, n * sizeof(double2), W, NB * sizeof(double2), w * sizeof(double2), nrhs,
                              hipMemcpyDeviceToDevice, s));
     if (r0 + w < n) {
       const double2 *A;
       int64_t lda;
       MOM_TRY(panel(r0 + w, n - r0 - w, r0, w, &A, &lda));
-      MOM_TRY(zgemm(h, n - r0 - w, nrhs, w, -1.0, A, lda, W, NB, 1.0, T + r0 + w, n));
+      MOM_TRY(zgemm(L, n - r0 - w, nrhs, w, -1.0, false, A, lda, W, NB, T + r0 + w, n));
     }
   }
   for (int64_t b = nblk - 1; b >= 0; b--) {
     const int64_t r0 = b * NB, w = std::min<int64_t>(NB, n - r0);
-    MOM_TRY(zgemm(h, w, nrhs, w, 1.0, Uinv + b * NB * NB, NB, T + r0, n, 0.0, W, NB));
+    MOM_TRY(zgemm(L, w, nrhs, w, 1.0, true, Uinv + b * NB * NB, NB, T + r0, n, W, NB));
     MOM_HIP(hipMemcpy2DAsync(T + r0, n * sizeof(double2), W, NB * sizeof(double2), w * sizeof(double2), nrhs,
                              hipMemcpyDeviceToDevice, s));
     if (r0 > 0) {
       const double2 *A;
       int64_t lda;
       MOM_TRY(panel(0, r0, r0, w, &A, &lda));
-      MOM_TRY(zgemm(h, r0, nrhs, w, -1.0, A, lda, W, NB, 1.0, T, n));
+      MOM_TRY(zgemm(L, r0, nrhs, w, -1.0, false, A, lda, W, NB, T, n));
     }
   }
   MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), T, n * sizeof(double2), n * sizeof(double2), nrhs,
@@ -2501,8 +2736,7 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
                              int32_t *iters, double *berr) {
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
-  rocblas_handle h;
-  MOM_TRY(blas_handle(main_lane(S->ctx), &h));
+  const Lane L = main_lane(S->ctx);
   DevBuf dB, dX, dR, dT, dT2, dW, dP, dperm, dn, dkeep;
   int rc;
   const size_t blk = sizeof(double2) * n * nrhs;
@@ -2546,7 +2780,7 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
   for (;; it++) {
     S->n_matvec++;
     MOM_HIP(hipMemcpyAsync(dR.ptr, dB.ptr, blk, hipMemcpyDeviceToDevice, s));
-    MOM_TRY(zgemm(h, n, nrhs, n, -1.0, S->Z64.as<double2>(), n, dX.as<double2>(), n, 1.0, dR.as<double2>(), n));
+    MOM_TRY(zgemm(L, n, nrhs, n, -1.0, false, S->Z64.as<double2>(), n, dX.as<double2>(), n, dR.as<double2>(), n));
     MOM_TRY(colmax(dR.as<double2>(), rn));
     MOM_TRY(colmax(dX.as<double2>(), xn));
     double bmax = 0;

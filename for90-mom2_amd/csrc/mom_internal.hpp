@@ -66,7 +66,6 @@ int copy_h2d_2d(mom_ctx *ctx, void *dev, size_t dpitch, const void *host, size_t
                 size_t rows);
 int copy_d2h_2d(mom_ctx *ctx, void *host, size_t hpitch, const void *dev, size_t dpitch, size_t width,
                 size_t rows);
-void release_blas(mom_ctx *ctx);
 
 }  // namespace mom
 
@@ -75,17 +74,15 @@ struct mom_ctx {
   hipStream_t stream = nullptr;
   mom::DevBuf qp;       // quadrature points of the last mesh/rule (scratch)
   mom::DevBuf scratch;  // general scratch
-  mom::DevBuf panel_t;  // transposed U12 panel of the complex64 trailing update (lu.hip)
+  mom::DevBuf panel_t;  // transposed U12 panel of the trailing updates (lu.hip)
   void *stage[2] = {nullptr, nullptr};   // pinned staging buffers
-  void *blas = nullptr;                  // rocBLAS handle on `stream` (lu.hip)
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   // second stream of the LU look-ahead (panel k+1 factored while the
   // trailing update of step k runs on `stream`), created on first use, with
-  // its own GEMM scratch and rocBLAS handle; fork/join events between the two
+  // its own GEMM scratch; fork/join events between the two
   hipStream_t aux = nullptr;
   mom::DevBuf panel_t2;
   mom::DevBuf ublk;  // E = I - inv(L11) of the last factored panel (complex64, NB x NB; lu.hip)
-  void *blas2 = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // fill autotuner: source faces per wave of the pair kernel chosen by timing
   // the candidates on the first slab of the first fill (0: not yet tuned)

@@ -1,5 +1,6 @@
-! ref_driver.f90 -- golden-vector driver linked against the reference's own
-! modules (built by oracle/build_ref.sh).  TEST INFRASTRUCTURE ONLY.
+! ref_driver.f90 -- golden-vector and timing driver linked against the
+! reference's own modules (built by oracle/build_ref.sh).  TEST
+! INFRASTRUCTURE ONLY.
 !
 ! It performs the set-up of the `mesh`, `symm 1 id`, `sdom`, `fdom` commands
 ! (interface.f90:44-123, 160-210, 357-366) and then calls the reference
@@ -8,21 +9,29 @@
 !   mesh_<d>.bin   set-up arrays of the master mesh (d=0) and each domain
 !   op_<d>_<X>.bin computeD/H/K (X = D, H, K) of domain d, E x E complex
 !   A.bin          sysmat_pmchwt, 2N x 2N complex (column-major)
-!   q.bin          srcvec of the plane wave, 2N complex
-!   post.bin       csca, cabs (cs_prtsrf) and rcs(ntheta,nphi) for the
-!                  solution read from x.bin (if present)
-!   times.txt      omp_get_wtime seconds of sysmat_pmchwt and srcvec
-!   sh.bin         (optional SH block, needs x.bin) the second-harmonic chain
-!                  of solve_batch's centrosymmetric branch (solver.f90:
-!                  143-231): per nonlinear domain src_coef and src_vec of
-!                  nlsurf_coef, the accumulated source, the source after
-!                  sysmat_pmchwt_nls, the SH solution of solve_systems and
-!                  its rcs at 2 omega (interface.f90:1316-1325)
+!   q.bin          srcvec of each plane wave, 2N x nsrc complex
+!   xref.bin       (do_solve) the reference's own solve_systems solution of
+!                  A X = Q (ZGETRF + ZGETRS, solver.f90:325-357), 2N x nsrc
+!   post.bin       per source csca, cabs (cs_prtsrf), then per source
+!                  rcs(ntheta,nphi), for the solution read from x.bin or,
+!                  without x.bin, for xref
+!   times.txt      omp_get_wtime seconds of sysmat_pmchwt, the srcvec loop
+!                  and solve_systems (solver.f90:107-140 times the same
+!                  three phases)
+!   sh.bin         (optional SH block, single source, needs a solution) the
+!                  second-harmonic chain of solve_batch's centrosymmetric
+!                  branch (solver.f90:143-231): per nonlinear domain
+!                  src_coef and src_vec of nlsurf_coef, the accumulated
+!                  source, the source after sysmat_pmchwt_nls, the SH
+!                  solution of solve_systems and its rcs at 2 omega
+!                  (interface.f90:1316-1325)
 !
 ! Input on stdin, one item per line:
 !   mesh file / scale / ndom / per domain: "nsurf id1 id2 ..." /
-!   wavelength [m] / per domain: "re im" refractive index /
-!   "theta phi psi" [deg] / "dump_ops dump_sysmat" (0/1) /
+!   wavelength [m] / per domain: "re im" refractive index / nsrc /
+!   per source: "theta phi psi" [deg] / "dump_ops fill do_solve" with
+!   fill 0 (none), 1 (sysmat_pmchwt, written to A.bin) or 2 (sysmat_pmchwt
+!   timed, not written) and do_solve 0/1 (needs fill > 0) /
 !   "ntheta nphi" / output directory /
 !   optional: "1" / per domain "re im" SH refractive index / per domain
 !   "0|1" surface chi2 / "nnn_re nnn_im ntt_re ntt_im ttn_re ttn_im"
@@ -39,13 +48,13 @@ PROGRAM ref_driver
   ! get_mesh_element_lines (mesh.f90:578-608) declares its file name LEN=256:
   ! the name is passed blank-padded to exactly that length
   CHARACTER(LEN=256) :: mshname
-  REAL(KIND=dp) :: scale, wl, omega, th, ph, ps, t0, tfill, tsrc, csca, cabs, re, im
-  INTEGER :: ndom, d, nsurf, i, ids(16), dump_ops, dump_sys, nth, nph, N, u, ios
-  TYPE(srcdata) :: src
-  COMPLEX(KIND=dp), DIMENSION(:,:,:), ALLOCATABLE :: A
-  COMPLEX(KIND=dp), DIMENSION(:,:), ALLOCATABLE :: q, M
-  COMPLEX(KIND=dp), DIMENSION(:), ALLOCATABLE :: x
+  REAL(KIND=dp) :: scale, wl, omega, th, ph, ps, t0, tfill, tsrc, tsolve, re, im
+  INTEGER :: ndom, d, nsurf, i, ids(16), dump_ops, dump_sys, do_solve, nth, nph, N, u, ios, nsrc, l
+  COMPLEX(KIND=dp), DIMENSION(:,:,:), ALLOCATABLE :: A, xs
+  COMPLEX(KIND=dp), DIMENSION(:,:), ALLOCATABLE :: q, M, x
+  REAL(KIND=dp), DIMENSION(:), ALLOCATABLE :: csca, cabs
   REAL(KIND=dp), DIMENSION(:,:), ALLOCATABLE :: rcsdata
+  REAL(KIND=dp), DIMENSION(:,:,:), ALLOCATABLE :: rcsall
   INTEGER, DIMENSION(:), POINTER :: vol_ids
   TYPE(prdnfo), POINTER :: prd
   LOGICAL :: have_x
@@ -97,13 +106,17 @@ PROGRAM ref_driver
      ALLOCATE(b%media(d)%prop(1))
      b%media(d)%prop(1)%ri = CMPLX(re, im, KIND=dp)
   END DO
-  READ(*,*) th, ph, ps
-  src%type = src_pw
-  src%pos(:) = 0.0_dp
-  src%theta = th*degtorad
-  src%phi = ph*degtorad
-  src%psi = ps*degtorad
-  READ(*,*) dump_ops, dump_sys
+  READ(*,*) nsrc
+  ALLOCATE(b%src(nsrc))
+  DO l=1,nsrc
+     READ(*,*) th, ph, ps
+     b%src(l)%type = src_pw
+     b%src(l)%pos(:) = 0.0_dp
+     b%src(l)%theta = th*degtorad
+     b%src(l)%phi = ph*degtorad
+     b%src(l)%psi = ps*degtorad
+  END DO
+  READ(*,*) dump_ops, dump_sys, do_solve
   READ(*,*) nth, nph
   READ(*,'(A)') outdir
 
@@ -133,45 +146,74 @@ PROGRAM ref_driver
      END DO
   END IF
 
-  ALLOCATE(q(2*N,1))
+  ! the srcvec loop of solve_batch (solver.f90:123-127), one source per column
+  ALLOCATE(q(2*N,nsrc))
   t0 = omp_get_wtime()
-  CALL srcvec(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, src, b%qd_tri, q)
+  DO l=1,nsrc
+     CALL srcvec(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, b%src(l), b%qd_tri, q(:,l:l))
+  END DO
   tsrc = omp_get_wtime() - t0
   OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/q.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
-  WRITE(u) q(:,1)
+  WRITE(u) q
   CLOSE(u)
 
   tfill = -1.0_dp
-  IF(dump_sys==1) THEN
+  tsolve = -1.0_dp
+  IF(dump_sys>=1) THEN
      ALLOCATE(A(2*N,2*N,1))
      t0 = omp_get_wtime()
      CALL sysmat_pmchwt(b, 1, A)
      tfill = omp_get_wtime() - t0
-     OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/A.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
-     WRITE(u) A(:,:,1)
-     CLOSE(u)
+     IF(dump_sys==1) THEN
+        OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/A.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+        WRITE(u) A(:,:,1)
+        CLOSE(u)
+     END IF
+     IF(do_solve==1) THEN
+        ! solve_systems (solver.f90:136, 325-357) on all sources at once
+        ALLOCATE(xs(2*N,1,nsrc))
+        xs(:,1,:) = q(:,:)
+        t0 = omp_get_wtime()
+        CALL solve_systems(b%mesh, b%ga, (1.0_dp,0.0_dp), (1.0_dp,0.0_dp), A, xs)
+        tsolve = omp_get_wtime() - t0
+        OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/xref.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+        WRITE(u) xs(:,1,:)
+        CLOSE(u)
+     END IF
      DEALLOCATE(A)
   END IF
 
   OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/times.txt', STATUS='REPLACE')
   WRITE(u,'(A,ES25.17)') 'sysmat_pmchwt ', tfill
   WRITE(u,'(A,ES25.17)') 'srcvec ', tsrc
+  WRITE(u,'(A,ES25.17)') 'solve_systems ', tsolve
   WRITE(u,'(A,I0)') 'threads ', omp_get_max_threads()
   CLOSE(u)
 
   INQUIRE(FILE=TRIM(outdir)//'/x.bin', EXIST=have_x)
   IF(have_x) THEN
-     ALLOCATE(x(2*N))
+     ALLOCATE(x(2*N,nsrc))
      OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/x.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD', IOSTAT=ios)
      READ(u) x
      CLOSE(u)
-     CALL cs_prtsrf(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, &
-          RESHAPE(x, (/2*N,1/)), src, b%qd_tri, csca, cabs)
-     ALLOCATE(rcsdata(nth,nph))
-     CALL rcs(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, &
-          RESHAPE(x, (/2*N,1/)), nth, nph, b%qd_tri, rcsdata)
+  ELSE IF(ALLOCATED(xs)) THEN
+     ALLOCATE(x(2*N,nsrc))
+     x(:,:) = xs(:,1,:)
+     have_x = .TRUE.
+  END IF
+  IF(have_x .AND. nth>0) THEN
+     ALLOCATE(csca(nsrc), cabs(nsrc), rcsall(nth,nph,nsrc))
+     DO l=1,nsrc
+        CALL cs_prtsrf(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, &
+             x(:,l:l), b%src(l), b%qd_tri, csca(l), cabs(l))
+        CALL rcs(b%domains(1)%mesh, N, omega, b%media(1)%prop(1)%ri, b%ga, &
+             x(:,l:l), nth, nph, b%qd_tri, rcsall(:,:,l))
+     END DO
      OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/post.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
-     WRITE(u) csca, cabs, rcsdata
+     DO l=1,nsrc
+        WRITE(u) csca(l), cabs(l)
+     END DO
+     WRITE(u) rcsall
      CLOSE(u)
   END IF
 
@@ -204,7 +246,7 @@ PROGRAM ref_driver
         nind = b%domains(d)%mesh%nedges
         ind(1:nind) = b%domains(d)%mesh%edges(:)%parent_index
         CALL nlsurf_coef(b%domains(d)%mesh, N, omega, b%media(d)%prop(1)%ri, b%media(d)%prop(1)%shri, &
-             epsp(1:b%domains(d)%mesh%nfaces), RESHAPE(x, (/2*N,1/)), b%ga, 1, b%media(d)%prop(1)%nls, &
+             epsp(1:b%domains(d)%mesh%nfaces), x(:,1:1), b%ga, 1, b%media(d)%prop(1)%nls, &
              b%qd_tri, (1.0_dp,0.0_dp), (1.0_dp,0.0_dp), src_coef(1:(2*nind),d,1,1), src_vec(1:(2*nind)))
         WRITE(u) epsp(1:b%domains(d)%mesh%nfaces), src_coef(1:(2*nind),d,1,1), src_vec(1:(2*nind))
         nlx(ind(1:nind),1,1) = nlx(ind(1:nind),1,1) + src_vec(1:nind)

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY (used by tests/golden/make_golden.py and the
 reference-vs-oracle tests, which are skipped where oracle/_ref is not built).
 """
 import os
+import resource
 import subprocess
 import tempfile
 
@@ -15,6 +16,20 @@ REF_BIN = os.path.join(ROOT, "oracle", "_ref", "mom_ref")
 
 def available():
     return os.path.exists(REF_BIN)
+
+
+def _unlimited_stack():
+    # the reference keeps large automatic arrays (srcvec's field arrays,
+    # sysmat_pmchwt's vector-subscript temporaries) on the stack
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def _env(threads):
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    env.setdefault("OMP_STACKSIZE", "256M")
+    return env
 
 
 def read_mesh(path):
@@ -38,45 +53,60 @@ def read_mesh(path):
 
 
 def run(mshfile, scale, domains, wl, ri, angles_deg, dump_ops=True, dump_sys=True, x=None,
-        ntheta=5, nphi=7, outdir=None, threads=None, sh=None):
+        ntheta=5, nphi=7, outdir=None, threads=None, sh=None, solve=False, write_A=True):
     """Run the reference set-up + hot path; returns a dict of arrays.
 
-    sh (needs x): dict(shri=[per domain], nls=[per domain 0/1], chi2=(nnn,
-    ntt, ttn)) runs the second-harmonic chain of solve_batch."""
+    angles_deg: one (theta, phi, psi) triple or a list of them (one plane wave
+    per triple, all solved together as solve_batch does).  dump_sys fills A
+    (written unless write_A is False, for timing); solve runs the
+    reference's own solve_systems (ZGETRF + ZGETRS) on A and returns its
+    solution as "xref", which also feeds cs_prtsrf and rcs when x is None.
+    sh (needs a solution, single source): dict(shri=[per domain], nls=[per
+    domain 0/1], chi2=(nnn, ntt, ttn)) runs the second-harmonic chain of
+    solve_batch."""
     outdir = outdir or tempfile.mkdtemp(prefix="momref_")
+    ang = np.atleast_2d(np.asarray(angles_deg, dtype=np.float64))
+    nsrc = ang.shape[0]
     if x is not None:
-        np.ascontiguousarray(x, dtype=np.complex128).tofile(os.path.join(outdir, "x.bin"))
+        np.asfortranarray(np.asarray(x, dtype=np.complex128).reshape(-1, nsrc, order="F")).T.ravel().tofile(
+            os.path.join(outdir, "x.bin"))
     lines = [mshfile, repr(float(scale)), str(len(domains))]
     lines += [" ".join(str(v) for v in [len(d)] + list(d)) for d in domains]
     lines += [repr(float(wl))]
     lines += ["%r %r" % (complex(r).real, complex(r).imag) for r in ri]
-    lines += ["%r %r %r" % tuple(float(a) for a in angles_deg)]
-    lines += ["%d %d" % (int(dump_ops), int(dump_sys)), "%d %d" % (ntheta, nphi), outdir]
+    lines += [str(nsrc)] + ["%r %r %r" % tuple(float(a) for a in row) for row in ang]
+    fill = 0 if not dump_sys else (1 if write_A else 2)
+    lines += ["%d %d %d" % (int(dump_ops), fill, int(bool(solve) and fill > 0)), "%d %d" % (ntheta, nphi), outdir]
     if sh is not None:
         lines += ["1"] + ["%r %r" % (complex(r).real, complex(r).imag) for r in sh["shri"]]
         lines += [str(int(f)) for f in sh["nls"]]
         lines += [" ".join("%r %r" % (complex(c).real, complex(c).imag) for c in sh["chi2"])]
-    env = dict(os.environ)
-    if threads:
-        env["OMP_NUM_THREADS"] = str(threads)
     subprocess.run([REF_BIN], input="\n".join(lines) + "\n", text=True, check=True,
-                   stdout=subprocess.DEVNULL, env=env)
+                   stdout=subprocess.DEVNULL, env=_env(threads), preexec_fn=_unlimited_stack)
     res = {"mesh": [read_mesh(os.path.join(outdir, "mesh_%d.bin" % d)) for d in range(len(domains) + 1)]}
     N2 = 2 * res["mesh"][0]["enodes"].shape[0]
-    res["q"] = np.fromfile(os.path.join(outdir, "q.bin"), np.complex128)
-    if dump_sys:
+    q = np.fromfile(os.path.join(outdir, "q.bin"), np.complex128).reshape(N2, nsrc, order="F")
+    res["q"] = q[:, 0] if nsrc == 1 else q
+    if dump_sys and write_A:
         res["A"] = np.fromfile(os.path.join(outdir, "A.bin"), np.complex128).reshape(N2, N2, order="F")
+    if dump_sys and solve:
+        xr = np.fromfile(os.path.join(outdir, "xref.bin"), np.complex128).reshape(N2, nsrc, order="F")
+        res["xref"] = xr[:, 0] if nsrc == 1 else xr
     if dump_ops:
         for d in range(len(domains)):
             E = res["mesh"][d + 1]["enodes"].shape[0]
             for op in "DHK":
                 res["op_%d_%s" % (d, op)] = np.fromfile(os.path.join(outdir, "op_%d_%s.bin" % (d + 1, op)),
                                                         np.complex128).reshape(E, E, order="F")
-    if x is not None:
+    if (x is not None or (dump_sys and solve)) and ntheta > 0:
         post = np.fromfile(os.path.join(outdir, "post.bin"), np.float64)
-        res["csca"], res["cabs"] = post[0], post[1]
-        res["rcs"] = post[2:].reshape(ntheta, nphi, order="F")
-    if x is not None and sh is not None:
+        cs = post[:2 * nsrc].reshape(nsrc, 2)
+        rcs = post[2 * nsrc:].reshape(ntheta, nphi, nsrc, order="F")
+        if nsrc == 1:
+            res["csca"], res["cabs"], res["rcs"] = cs[0, 0], cs[0, 1], rcs[:, :, 0]
+        else:
+            res["csca"], res["cabs"], res["rcs"] = cs[:, 0], cs[:, 1], rcs
+    if sh is not None and (x is not None or solve):
         N2 = 2 * res["mesh"][0]["enodes"].shape[0]
         with open(os.path.join(outdir, "sh.bin"), "rb") as fh:
             for d, f in enumerate(sh["nls"]):
@@ -115,11 +145,8 @@ def run_symm(mshfile, scale, gens, sbnd, domains, wl, ri, angles_deg, ntheta=5, 
     lines += [repr(float(wl))]
     lines += ["%r %r" % (complex(r).real, complex(r).imag) for r in ri]
     lines += ["%r %r %r" % tuple(float(a) for a in angles_deg), "%d %d" % (ntheta, nphi), outdir]
-    env = dict(os.environ)
-    if threads:
-        env["OMP_NUM_THREADS"] = str(threads)
     subprocess.run([REF_SYMM_BIN], input="\n".join(lines) + "\n", text=True, check=True,
-                   stdout=subprocess.DEVNULL, env=env)
+                   stdout=subprocess.DEVNULL, env=_env(threads), preexec_fn=_unlimited_stack)
     res = {"mesh": [read_mesh(os.path.join(outdir, "mesh_%d.bin" % d)) for d in range(len(domains) + 1)]}
     for d in range(len(domains) + 1):
         res["mesh"][d]["bnd"] = np.fromfile(os.path.join(outdir, "bnd_%d.bin" % d), np.int32)
