@@ -227,6 +227,8 @@ int mom_ctx_create(int32_t device, mom_ctx **ctx) {
   MOM_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(MOM_ERR_ARG, "no such device");
   MOM_HIP(hipSetDevice(device));
+  int rc = no_alloc_init(device);
+  if (rc) return rc;
   mom_ctx *c = new mom_ctx;
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -329,6 +331,10 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
       }
     m->extent = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
                           (hi[2] - lo[2]) * (hi[2] - lo[2]));
+    double r2 = 0;
+    for (int64_t v = 0; v < nnodes; v++)
+      r2 = std::max(r2, p[3 * v] * p[3 * v] + p[3 * v + 1] * p[3 * v + 1] + p[3 * v + 2] * p[3 * v + 2]);
+    m->rmax = std::sqrt(r2);
   }
   m->h_fcp.assign(fcp, fcp + 3 * F);
   // face-side codes of each edge and the slab edge lists of the fill
@@ -516,31 +522,23 @@ static int far_fields_impl(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_to
   if (!ctx || !mesh || !x || !theta || !phi || !out) return fail(MOM_ERR_ARG, "NULL argument");
   if (!(omega > 0) || !(r > 0)) return fail(MOM_ERR_ARG, "omega and r must be > 0");
   if (nedges_total < 1 || mesh->max_parent >= nedges_total) return fail(MOM_ERR_ARG, "nedges_total too small");
+  if (ndir < 1 || ndir > (1 << 24)) return fail(MOM_ERR_ARG, "number of directions out of range");
   QuadRule rule;
   int rc = make_rule(nq, qw, qnodes, &rule);
   if (rc) return rc;
   (void)hipSetDevice(ctx->device);
   const int64_t N2 = 2 * nedges_total;
-  DevBuf dx;
-  if ((rc = dx.alloc(sizeof(double2) * N2 * G.n)) || (rc = copy_h2d(ctx, dx.ptr, x, sizeof(double2) * N2 * G.n)))
+  DevBuf dx, dth, dph, dout, work;
+  if ((rc = dx.alloc(sizeof(double2) * N2 * G.n)) || (rc = dth.alloc(sizeof(double) * ndir)) ||
+      (rc = dph.alloc(sizeof(double) * ndir)) || (rc = dout.alloc(sizeof(double) * 8 * ndir)) ||
+      (rc = work.alloc(far_work_bytes(mesh, rule.Q, G, ndir))))
     return rc;
-  const cd ri(ri_re, ri_im), I(0.0, 1.0);
-  const cd k = ri * omega / kC0;
-  std::vector<cd> v;
-  if ((rc = far_field_integrals(ctx, mesh, rule, k, dx.as<double2>(), nedges_total, ndir, theta, phi, v, G, N2)))
-    return rc;
-  // ffields.f90:107-113
-  const cd prefix = I * k * std::exp(I * k * r) / (4.0 * MOM_PI * r);
-  for (int64_t d = 0; d < ndir; d++) {
-    const cd Nt = v[4 * d], Np = v[4 * d + 1], Lt = v[4 * d + 2], Lp = v[4 * d + 3];
-    const cd f[4] = {prefix * (Lp + kEta0 / ri * Nt), -prefix * (Lt - kEta0 / ri * Np),
-                     -prefix * (Np - Lt * ri / kEta0), prefix * (Nt + Lp * ri / kEta0)};
-    for (int c = 0; c < 4; c++) {
-      out[8 * d + 2 * c] = f[c].real();
-      out[8 * d + 2 * c + 1] = f[c].imag();
-    }
-  }
-  return 0;
+  MOM_TRY(copy_h2d(ctx, dx.ptr, x, sizeof(double2) * N2 * G.n));
+  MOM_TRY(copy_h2d(ctx, dth.ptr, theta, sizeof(double) * ndir));
+  MOM_TRY(copy_h2d(ctx, dph.ptr, phi, sizeof(double) * ndir));
+  MOM_TRY(far_fields_device(ctx, mesh, rule, omega, cd(ri_re, ri_im), dx.as<double2>(), nedges_total, r, ndir,
+                            dth.as<double>(), dph.as<double>(), dout.as<double>(), work.ptr, G, N2));
+  return copy_d2h(ctx, out, dout.ptr, sizeof(double) * 8 * ndir);
 }
 
 int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
@@ -581,6 +579,20 @@ static int rcs_impl(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, do
     }
   }
   return 0;
+}
+
+int mom_far_fields_noalloc(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                           double ri_im, const double *x, double r, int64_t ndir, const double *theta,
+                           const double *phi, int32_t nq, const double *qw, const double *qnodes, double *out) {
+  if (!ctx || !mesh || !x || !theta || !phi || !out) return fail(MOM_ERR_ARG, "NULL argument");
+  if (!(omega > 0) || !(r > 0)) return fail(MOM_ERR_ARG, "omega and r must be > 0");
+  if (nedges_total < 1 || mesh->max_parent >= nedges_total) return fail(MOM_ERR_ARG, "nedges_total too small");
+  QuadRule rule;
+  int rc = make_rule(nq, qw, qnodes, &rule);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  return far_fields_no_alloc(ctx, mesh, rule, omega, cd(ri_re, ri_im), reinterpret_cast<const double2 *>(x),
+                             nedges_total, r, ndir, theta, phi, out);
 }
 
 int mom_rcs_ext(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,

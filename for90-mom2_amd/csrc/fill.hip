@@ -29,9 +29,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <vector>
-
 #include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "mom_internal.hpp"
 
@@ -887,18 +888,21 @@ __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
   }
 }
 
+static int quad_points_into(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double *qp) {
+  int total = (int)(m->F * rule.Q);
+  hipLaunchKernelGGL(quad_points_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
+                     (int)m->F, rule.Q, m->p.as<double>(), m->fnodes.as<int>(), rule, qp);
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
 int compute_quad_points(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule) {
   size_t need = (size_t)m->F * rule.Q * 3 * sizeof(double);
   if (ctx->qp.bytes < need) {
     int rc = ctx->qp.alloc(need);
     if (rc) return rc;
   }
-  int total = (int)(m->F * rule.Q);
-  hipLaunchKernelGGL(quad_points_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
-                     (int)m->F, rule.Q, m->p.as<double>(), m->fnodes.as<int>(), rule,
-                     ctx->qp.as<double>());
-  MOM_HIP(hipGetLastError());
-  return 0;
+  return quad_points_into(ctx, m, rule, ctx->qp.as<double>());
 }
 
 static FaceSide sides(const mom_mesh *m, bool global) {
@@ -1442,48 +1446,168 @@ __global__ void farfield_reduce_kernel(int ndir, int nchunk, const double2 *__re
   }
 }
 
-int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, cd k, const double2 *x,
-                        int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out,
-                        const Group &G, int64_t ldx) {
-  if (!(std::fabs(k.real()) * m->extent < kCisMaxArg))
-    return fail(MOM_ERR_ARG, "Re(k) times the mesh extent is beyond the far field's phase range (2^19 rad)");
+// Direction frames of far_fields (ffields.f90:30-45): dirs[8 d ..] = dir
+// (st cp, st sp, ct), u_theta (ct cp, ct sp, -st), u_phi (-sp, cp).
+__global__ void farfield_dirs_kernel(int64_t ndir, const double *__restrict__ theta, const double *__restrict__ phi,
+                                     double *__restrict__ dirs) {
+  const int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (d >= ndir) return;
+  double st, ct, sp, cp;
+  sincos(theta[d], &st, &ct);
+  sincos(phi[d], &sp, &cp);
+  double *u = dirs + 8 * d;
+  u[0] = st * cp;
+  u[1] = st * sp;
+  u[2] = ct;
+  u[3] = ct * cp;
+  u[4] = ct * sp;
+  u[5] = -st;
+  u[6] = -sp;
+  u[7] = cp;
+}
+
+// ffields.f90:107-113: with v = (Nt, Np, Lt, Lp) of direction d,
+// et = P (Lp + a Nt), ep = -P (Lt - a Np), ht = -P (Np - Lt b),
+// hp = P (Nt + Lp b), P = i k exp(ikr) / (4 pi r), a = eta0 / ri,
+// b = ri / eta0; out[8 d ..] = et, ep, ht, hp (interleaved complex).
+struct FarCoef {
+  double2 P, a, b;
+};
+
+#pragma clang fp contract(off)
+__device__ __forceinline__ double2 zmul(double2 u, double2 v) {
+  return make_double2(u.x * v.x - u.y * v.y, u.x * v.y + u.y * v.x);
+}
+__device__ __forceinline__ double2 zadd(double2 u, double2 v) { return make_double2(u.x + v.x, u.y + v.y); }
+__device__ __forceinline__ double2 zsub(double2 u, double2 v) { return make_double2(u.x - v.x, u.y - v.y); }
+__device__ __forceinline__ double2 zneg(double2 u) { return make_double2(-u.x, -u.y); }
+
+__global__ void farfield_combine_kernel(int64_t ndir, const double2 *__restrict__ v, FarCoef c,
+                                        double *__restrict__ out) {
+  const int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (d >= ndir) return;
+  const double2 Nt = v[4 * d], Np = v[4 * d + 1], Lt = v[4 * d + 2], Lp = v[4 * d + 3];
+  const double2 f[4] = {zmul(c.P, zadd(Lp, zmul(c.a, Nt))), zneg(zmul(c.P, zsub(Lt, zmul(c.a, Np)))),
+                        zneg(zmul(c.P, zsub(Np, zmul(Lt, c.b)))), zmul(c.P, zadd(Nt, zmul(Lp, c.b)))};
+  for (int k = 0; k < 4; k++) {
+    out[8 * d + 2 * k] = f[k].x;
+    out[8 * d + 2 * k + 1] = f[k].y;
+  }
+}
+#pragma clang fp contract(on)
+
+// Device work areas of one far-field evaluation, laid out in one block of
+// `far_work_bytes` bytes (each part 256-byte aligned).
+struct FarWork {
+  double *qp, *pts, *dirs;
+  double2 *cur, *part, *red;
+};
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+static size_t far_work_layout(const mom_mesh *m, int Q, const Group &G, int64_t ndir, char *base, FarWork *w) {
+  const int64_t S = m->F * Q * G.n;
+  const int64_t nchunk = (S + kFfChunk - 1) / kFfChunk;
+  const size_t sizes[6] = {sizeof(double) * 3 * (size_t)m->F * Q, sizeof(double) * 3 * (size_t)S,
+                           sizeof(double) * 8 * (size_t)ndir, sizeof(double2) * 6 * (size_t)S,
+                           sizeof(double2) * 4 * (size_t)ndir * nchunk, sizeof(double2) * 4 * (size_t)ndir};
+  size_t off[6], total = 0;
+  for (int i = 0; i < 6; i++) {
+    off[i] = total;
+    total += align256(sizes[i]);
+  }
+  if (w) {
+    w->qp = reinterpret_cast<double *>(base + off[0]);
+    w->pts = reinterpret_cast<double *>(base + off[1]);
+    w->dirs = reinterpret_cast<double *>(base + off[2]);
+    w->cur = reinterpret_cast<double2 *>(base + off[3]);
+    w->part = reinterpret_cast<double2 *>(base + off[4]);
+    w->red = reinterpret_cast<double2 *>(base + off[5]);
+  }
+  return total;
+}
+
+size_t far_work_bytes(const mom_mesh *m, int Q, const Group &G, int64_t ndir) {
+  return far_work_layout(m, Q, G, ndir, nullptr, nullptr);
+}
+
+// far_fields on the device: x the solution [2 Nt] (ldx per representation),
+// theta, phi [ndir] and out [8 ndir] device arrays, work a device block of
+// far_work_bytes bytes.  Every part of `work` is written before it is read,
+// so it needs no clearing.  Launches only; the caller synchronises.
+int far_fields_device(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, const double2 *x,
+                      int64_t Nt, double r, int64_t ndir, const double *theta, const double *phi, double *out,
+                      void *work, const Group &G, int64_t ldx) {
+  const cd I(0.0, 1.0);
+  const cd k = ri * omega / kC0;
+  // the phasors exp(-i k dir . r) of every image point, |r| <= the mesh's
+  // largest distance from the origin, reflected or not
+  if (!(std::fabs(k.real()) * m->rmax < kCisMaxArg))
+    return fail(MOM_ERR_ARG, "Re(k) times the mesh's distance from the origin is beyond the far field's phase range "
+                             "(2^19 rad)");
   if (ndir < 1 || ndir > (1 << 24)) return fail(MOM_ERR_ARG, "number of directions out of range");
-  int rc = compute_quad_points(ctx, m, rule);
-  if (rc) return rc;
+  FarWork w;
+  far_work_layout(m, rule.Q, G, ndir, static_cast<char *>(work), &w);
+  MOM_TRY(quad_points_into(ctx, m, rule, w.qp));
   const int64_t S = m->F * rule.Q * G.n;
   const int nchunk = (int)((S + kFfChunk - 1) / kFfChunk);
-  std::vector<double> dirs(8 * ndir);
-  for (int64_t d = 0; d < ndir; d++) {
-    const double st = std::sin(theta[d]), ct = std::cos(theta[d]), sp = std::sin(phi[d]), cp = std::cos(phi[d]);
-    double *u = dirs.data() + 8 * d;
-    u[0] = st * cp;
-    u[1] = st * sp;
-    u[2] = ct;
-    u[3] = ct * cp;
-    u[4] = ct * sp;
-    u[5] = -st;
-    u[6] = -sp;
-    u[7] = cp;
-  }
-  DevBuf pts, cur, ddir, part, red;
-  if ((rc = pts.alloc(sizeof(double) * 3 * S)) || (rc = cur.alloc(sizeof(double2) * 6 * S)) ||
-      (rc = ddir.alloc(sizeof(double) * 8 * ndir)) || (rc = part.alloc(sizeof(double2) * 4 * ndir * nchunk)) ||
-      (rc = red.alloc(sizeof(double2) * 4 * ndir)))
-    return rc;
-  MOM_TRY(copy_h2d(ctx, ddir.ptr, dirs.data(), sizeof(double) * 8 * ndir));
+  const unsigned gd = (unsigned)((ndir + 255) / 256);
+  hipLaunchKernelGGL(farfield_dirs_kernel, dim3(gd), dim3(256), 0, ctx->stream, ndir, theta, phi, w.dirs);
   hipLaunchKernelGGL(farfield_sources_kernel, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, ctx->stream, (int)m->F,
-                     rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(), sides(m, true), x, Nt,
-                     pts.as<double>(), cur.as<double2>(), S, G, ldx > 0 ? ldx : 2 * Nt);
-  hipLaunchKernelGGL(farfield_dir_kernel, dim3((unsigned)((ndir + 255) / 256), (unsigned)nchunk), dim3(256), 0,
-                     ctx->stream, S, pts.as<double>(), cur.as<double2>(), (int)ndir, ddir.as<double>(), k.real(),
-                     k.imag(), part.as<double2>());
-  hipLaunchKernelGGL(farfield_reduce_kernel, dim3((unsigned)((ndir + 255) / 256)), dim3(256), 0, ctx->stream,
-                     (int)ndir, nchunk, part.as<double2>(), red.as<double2>());
+                     rule.Q, rule, w.qp, m->farea.as<double>(), sides(m, true), x, Nt, w.pts, w.cur, S, G,
+                     ldx > 0 ? ldx : 2 * Nt);
+  hipLaunchKernelGGL(farfield_dir_kernel, dim3(gd, (unsigned)nchunk), dim3(256), 0, ctx->stream, S, w.pts, w.cur,
+                     (int)ndir, w.dirs, k.real(), k.imag(), w.part);
+  hipLaunchKernelGGL(farfield_reduce_kernel, dim3(gd), dim3(256), 0, ctx->stream, (int)ndir, nchunk, w.part, w.red);
+  FarCoef c;
+  const cd P = I * k * std::exp(I * k * r) / (4.0 * MOM_PI * r), a = kEta0 / ri, b = ri / kEta0;
+  c.P = make_double2(P.real(), P.imag());
+  c.a = make_double2(a.real(), a.imag());
+  c.b = make_double2(b.real(), b.imag());
+  hipLaunchKernelGGL(farfield_combine_kernel, dim3(gd), dim3(256), 0, ctx->stream, ndir, w.red, c, out);
   MOM_HIP(hipGetLastError());
-  std::vector<double2> h(4 * ndir);
-  MOM_TRY(copy_d2h(ctx, h.data(), red.ptr, sizeof(double2) * 4 * ndir));
-  out.resize(4 * ndir);
-  for (int64_t i = 0; i < 4 * ndir; i++) out[i] = cd(h[i].x, h[i].y);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free far-field entry point: a fixed device array
+// (each GPU the library runs on holds its own copy until the process ends).
+// Host code never uses the array's name as an address: its device address
+// is looked up once per device at context creation (no_alloc_init) and
+// kept in g_scratch_ptr.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+int no_alloc_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(MOM_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(MOM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+  return 0;
+}
+
+int far_fields_no_alloc(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
+                        const double2 *x, int64_t Nt, double r, int64_t ndir, const double *theta, const double *phi,
+                        double *out) {
+  void *base = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+  if (!base) return fail(MOM_ERR_STATE, "no-allocation scratch is not set up on this device");
+  const Group G = identity_group();
+  if (ndir < 1 || ndir > (1 << 24)) return fail(MOM_ERR_ARG, "number of directions out of range");
+  const size_t need = far_work_bytes(m, rule.Q, G, ndir);
+  if (need > kNoAllocScratchBytes)
+    return fail(MOM_ERR_ARG, "far field of " + std::to_string(m->F) + " faces x " + std::to_string(rule.Q) +
+                                 " points in " + std::to_string(ndir) + " directions needs " + std::to_string(need) +
+                                 " bytes of scratch, more than the " + std::to_string(kNoAllocScratchBytes) +
+                                 " bytes of the no-allocation entry point");
+  MOM_TRY(far_fields_device(ctx, m, rule, omega, ri, x, Nt, r, ndir, theta, phi, out, base, G, 0));
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 

@@ -96,6 +96,7 @@ struct mom_mesh {
   int64_t V = 0, F = 0, E = 0;
   double avelen = 0;
   double extent = 0;  // diagonal of the nodes' bounding box (bounds every point distance)
+  double rmax = 0;    // largest distance of a node from the origin (bounds |J r - r'| <= 2 rmax for images)
   int64_t nnear = 0;
   int64_t max_parent = -1;  // largest global index among the mesh's edges
   std::vector<double> h_fcp;  // host centroids (near list)
@@ -201,9 +202,16 @@ int cs_partials(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
                 const Group &G = identity_group(), int64_t ldx = 0);
 // (Nt, Np, Lt, Lp) of far_fields (ffields.f90:16-122) for ndir directions,
 // x the device solution [2 Nt], k the medium's wavenumber
-int far_field_integrals(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, cd k, const double2 *x,
-                        int64_t Nt, int64_t ndir, const double *theta, const double *phi, std::vector<cd> &out,
-                        const Group &G = identity_group(), int64_t ldx = 0);
+size_t far_work_bytes(const mom_mesh *m, int Q, const Group &G, int64_t ndir);
+int far_fields_device(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, const double2 *x,
+                      int64_t Nt, double r, int64_t ndir, const double *theta, const double *phi, double *out,
+                      void *work, const Group &G = identity_group(), int64_t ldx = 0);
+// the allocation-free far field: scratch in a fixed device array set up per
+// device by no_alloc_init (called when a context is created)
+int no_alloc_init(int device);
+int far_fields_no_alloc(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
+                        const double2 *x, int64_t Nt, double r, int64_t ndir, const double *theta, const double *phi,
+                        double *out);
 // sysmat_pmchwt for a mirror group (symm.hip): every representation r's
 // reduced matrix Z[r] (order[r] = kept unknowns, padded to even by a unit
 // diagonal), bnd[N] the boundary class of each master edge

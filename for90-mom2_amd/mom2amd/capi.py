@@ -29,7 +29,7 @@ EXPORTS = [
     "mom_problem_topology", "mom_problem_mesh_create", "mom_problem_set_lines", "mom_problem_classify_edges",
     "mom_problem_edge_bnd", "mom_symm_group", "mom_symm_edge_bc", "mom_system_create_order",
     "mom_system_fill_symm", "mom_srcvec_pw_symm", "mom_cs_prtsrf_symm", "mom_rcs_ext_symm",
-    "mom_pmchwt_vertex_adjoint", "mom_cs_abs_vertex_explicit", "mom_ctx_fill_tuning",
+    "mom_pmchwt_vertex_adjoint", "mom_cs_abs_vertex_explicit", "mom_ctx_fill_tuning", "mom_far_fields_noalloc",
 ]
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
@@ -96,6 +96,7 @@ def lib():
     L.mom_cs_prtsrf.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _d, _d, _i32, _dp, _dp, C.POINTER(_d),
                                 C.POINTER(_d)]
     L.mom_far_fields.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _i64, _dp, _dp, _i32, _dp, _dp, _dp]
+    L.mom_far_fields_noalloc.argtypes = [_vp, _vp, _i64, _d, _d, _d, _vp, _d, _i64, _vp, _vp, _i32, _dp, _dp, _vp]
     L.mom_rcs_ext.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _i32, _i32, _i32, C.c_void_p, _i32, _dp, _dp, _dp]
     L.mom_nlsurf_coef.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _dp, _dp, _i32, _dp, _dp, _dp, _dp]
     L.mom_sysmat_pmchwt_nls.argtypes = [_vp, _i32, C.POINTER(_vp), _dp, _i64, _d, _i32, _dp, _dp, _i64, _dp, _i64,

@@ -257,6 +257,22 @@ int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, dou
                    double ri_im, const double *x, double r, int64_t ndir, const double *theta, const double *phi,
                    int32_t nq, const double *qw, const double *qnodes, double *out);
 /*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mom_far_fields with every array on the device and no
+ * allocation -- x[4N] (complex128 solution), theta[ndir], phi[ndir] and
+ * out[8 ndir] are device pointers; the work areas live in a fixed 64 MiB
+ * device array per GPU (set up by mom_ctx_create).  It runs the same
+ * kernels as mom_far_fields and returns the same values.  A problem needing
+ * more scratch than 64 MiB (about 144 F Q + 64 ndir (F Q / 4096 + 3) bytes
+ * for F faces of Q points; F = 20480, Q = 13 and the 2664 directions of a
+ * 37 x 72 grid fit) is rejected with MOM_ERR_ARG.  Calls that overlap on one GPU (two contexts on the same
+ * device at once) race on the shared array; the call is synchronous.
+ */
+int mom_far_fields_noalloc(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                           double ri_im, const double *x, double r, int64_t ndir, const double *theta,
+                           const double *phi, int32_t nq, const double *qw, const double *qnodes, double *out);
+/*
  * rcs_ext (ffields.f90:203-249): bistatic radar cross-section on the grid
  * theta = pi i/(ntheta-1), phi = 2 pi j/(nphi-1) (both ends included, pi as
  * the reference's), r = 1: rcs[i + ntheta j] = |et|^2 + |ep|^2, or with
