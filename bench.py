@@ -186,15 +186,17 @@ def bench_sh(args):
 def bench_spectrum(args):
     """Batched spectrum with many right-hand sides: a prolate ellipsoid (the
     s-subdivided unit icosphere scaled to semi-axes 80/40/40 nm), 64 vacuum
-    wavelengths uniform in 400-900 nm, interior eps from the Drude model
+    wavelengths of `wlrg 64 400d-9 900d-9`, interior eps from the Drude model
     eps_inf - wp^2/(w(w + i gamma)) (eps_inf 9.5, wp 9.0 eV, gamma 0.07 eV; a
     single pole), 256 plane waves per wavelength (psi 0 and 90 deg times 128
     directions uniform on the sphere, numpy seed 20251203).  One step = for
     every wavelength the fill, the mixed-precision LU and the 256 solves on
     the reused factors (block refinement to backward error 1e-12), with one
-    device-resident system reused across the wavelengths."""
-    from mom2amd import Context, DeviceMesh, System, cs_prtsrf, load_problem, meshgen, omega_of
+    device-resident system reused across the wavelengths
+    (mom2amd.spectrum.solve_spectrum, the path the spectrum parity test runs)."""
+    from mom2amd import DEGTORAD, Context, DeviceMesh, cs_prtsrf, load_problem, meshgen, omega_of
     from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
+    from mom2amd.spectrum import drude_index, plane_waves_deg, solve_spectrum, wavelengths
 
     s = args.subdiv if args.subdiv != 5 else 4
     nodes, faces = meshgen.icosphere(s, 1.0)
@@ -205,61 +207,33 @@ def bench_spectrum(args):
     prob = load_problem(path, 1e-9, [[15], [-15]])
     N = prob.nedges
     nwl = args.nwl
-    wls = 400e-9 + np.arange(nwl) / max(nwl - 1, 1) * 500e-9
-    hc_evnm = 1239.841984  # h c [eV nm]
-
-    def ri_in(wl):
-        w = hc_evnm / (wl * 1e9)
-        eps = 9.5 - 9.0 ** 2 / (w * (w + 0.07j))
-        n = np.sqrt(complex(eps))
-        return n if n.imag >= 0 else -n
-
-    rng = np.random.default_rng(20251203)
-    ct = rng.uniform(-1.0, 1.0, 128)
-    ph = rng.uniform(0.0, 2 * np.pi, 128)
-    th = np.arccos(ct)
-    ang = np.array([[t, p, ps] for ps in (0.0, np.pi / 2) for t, p in zip(th, ph)])
+    wls = wavelengths(nwl)
+    ang = plane_waves_deg(128)
+    ri_of = lambda wl: [1.0 + 0j, drude_index(wl)]  # noqa: E731
     log("ellipsoid s=%d: %d faces, N=%d unknowns, %d wavelengths x %d sources" % (s, prob.master.nfaces, 2 * N,
                                                                                   nwl, ang.shape[0]))
     ctx = Context(0)
     dms = [DeviceMesh(ctx, dm) for dm in prob.domains]
 
-    def sweep(prec, idx):
-        S = System(ctx, N, prec)
-        t = {"fill": 0.0, "factor": 0.0, "solve": 0.0}
-        X = {}
-        maxit, maxberr = 0, 0.0
-        try:
-            for i in idx:
-                om = omega_of(wls[i])
-                ri = [1.0 + 0j, ri_in(wls[i])]
-                t0 = time.perf_counter()
-                S.fill(dms, ri, om)
-                t1 = time.perf_counter()
-                S.factor()
-                t2 = time.perf_counter()
-                X[i] = S.solve_pw(dms[0], om, 1.0, ang)
-                t3 = time.perf_counter()
-                t["fill"] += t1 - t0
-                t["factor"] += t2 - t1
-                t["solve"] += t3 - t2
-                maxit, maxberr = max(maxit, S.iters), max(maxberr, S.berr)
-        finally:
-            S.close()
+    def sweep(prec, idx, keep=None):
+        X, info = solve_spectrum(ctx, dms, N, wls[list(idx)], ri_of, ang, precision=prec, keep=keep)
+        t = {k: float(sum(r[k] for r in info)) for k in ("fill", "factor", "solve")}
         t["wall"] = t["fill"] + t["factor"] + t["solve"]
-        return t, X, maxit, maxberr
+        return t, X, max(r["iters"] for r in info), max(r["berr"] for r in info), sum(r["fell_back"] for r in info)
 
     for i in range(args.warmup):
-        w, _, _, _ = sweep(MOM_PREC_MIXED, range(min(nwl, 4)))
+        w = sweep(MOM_PREC_MIXED, range(min(nwl, 4)))[0]
         log("warmup %d (4 wavelengths): %.2f s" % (i, w["wall"]))
     runs = []
+    check = [0, nwl - 1]
     for i in range(args.steps):
         t0 = time.perf_counter()
-        r, X, mi, mb = sweep(MOM_PREC_MIXED, range(nwl))
+        r, X, mi, mb, nfb = sweep(MOM_PREC_MIXED, range(nwl), keep=check)
         r["clock"] = time.perf_counter() - t0
         runs.append(r)
         log("step %d: %.2f s for %d wavelengths (fill %.2f factor %.2f solve %.2f; refinement steps <= %d, "
-            "berr <= %.1e)" % (i, r["clock"], nwl, r["fill"], r["factor"], r["solve"], mi, mb))
+            "berr <= %.1e, %d wavelengths refactored in FP64)" % (i, r["clock"], nwl, r["fill"], r["factor"],
+                                                                  r["solve"], mi, mb, nfb))
     mean = float(np.mean([r["clock"] for r in runs]))
     nrhs = ang.shape[0]
     out = {"metric": "spectrum s @N=%d: %d wavelengths x %d plane waves, fill+LU+solves (mixed path)"
@@ -273,20 +247,19 @@ def bench_spectrum(args):
            "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fill", "factor", "solve")},
            "per_wavelength_ms": round(mean / nwl * 1e3, 2)}
     if not args.no_fp64:
-        # FP64 path on two wavelengths; cross-sections of 8 sources compared with the mixed solutions
-        idx = [0, nwl - 1]
-        t64, X64, _, _ = sweep(MOM_PREC_FP64, idx)
+        # FP64 path on the first and last wavelengths; cross-sections of 16 sources compared with the mixed solutions
+        t64, X64, _, _, _ = sweep(MOM_PREC_FP64, check)
         err = 0.0
-        for i in idx:
+        for j, i in enumerate(check):
             om = omega_of(wls[i])
             for k in range(0, nrhs, nrhs // 8):
-                a = cs_prtsrf(ctx, dms[0], N, om, 1.0, X[i][:, k], *ang[k])
-                b = cs_prtsrf(ctx, dms[0], N, om, 1.0, X64[i][:, k], *ang[k])
+                a = cs_prtsrf(ctx, dms[0], N, om, 1.0, X[i][:, k], *(ang[k] * DEGTORAD))
+                b = cs_prtsrf(ctx, dms[0], N, om, 1.0, X64[j][:, k], *(ang[k] * DEGTORAD))
                 err = max(err, abs(a[0] / b[0] - 1), abs(a[1] / b[1] - 1))
-        out["fp64_path_per_wavelength_s"] = round(t64["wall"] / len(idx), 4)
-        out["speedup_vs_fp64_path"] = round(t64["wall"] / len(idx) / (mean / nwl), 3)
+        out["fp64_path_per_wavelength_s"] = round(t64["wall"] / len(check), 4)
+        out["speedup_vs_fp64_path"] = round(t64["wall"] / len(check) / (mean / nwl), 3)
         out["sigma_relerr_vs_fp64_path"] = err
-        log("fp64 path: %.2f s per wavelength; sigma rel err (16 checks) %.1e" % (t64["wall"] / len(idx), err))
+        log("fp64 path: %.2f s per wavelength; sigma rel err (16 checks) %.1e" % (t64["wall"] / len(check), err))
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -934,8 +934,10 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   a.side = sides(m, glob);
   a.kr = k.real();
   a.ki = k.imag();
-  if (!(std::fabs(a.kr) * m->extent < kCisMaxArg))
-    return fail(MOM_ERR_ARG, "Re(k) times the mesh extent is beyond the fill's phase range (2^19 rad)");
+  // point distances: |r - r'| <= extent; with an image |J r - r'| <= 2 rmax
+  const double reach = img ? std::max(m->extent, 2.0 * m->rmax) : m->extent;
+  if (!(std::fabs(a.kr) * reach < kCisMaxArg))
+    return fail(MOM_ERR_ARG, "Re(k) times the largest point distance is beyond the fill's phase range (2^19 rad)");
   const double thr = m->avelen * 3;
   a.thr2 = thr * thr;
   a.c1 = make_double2(c1.real(), c1.imag());

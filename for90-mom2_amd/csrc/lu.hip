@@ -2723,6 +2723,10 @@ static int gmres_ir(mom_system *S, IrWork &W, const std::vector<cd> &b, std::vec
 
 // From this many right-hand sides on, solves run on the whole block at once.
 constexpr int64_t kBatchRhs = 4;
+// Block refinement: 1 takes the FP64-factor branch whenever a column is
+// above the tolerance after the first residual (test hook), 0 decides by
+// the cost estimate.
+static int g_force_fp64_fallback = 0;
 
 static int ir_alloc(mom_system *S, IrWork &W);
 
@@ -2788,7 +2792,7 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
     for (int64_t j = 0; j < nrhs; j++) {
       be[j] = rn[j] / (S->znorm_inf * xn[j] + bn[j]);
       bmax = std::max(bmax, be[j]);
-      keep[j] = be[j] > tol && (it == 0 || be[j] <= 0.1 * prev[j]);
+      keep[j] = be[j] > tol && (it == 0 || be[j] <= 0.1 * prev[j]) && !g_force_fp64_fallback;
       nkeep += keep[j];
     }
     if (it < kBeHistory) S->be_hist[it] = bmax;
@@ -2811,11 +2815,17 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
   const double nd = (double)n;
   const double t_lu64 = 8.0 / 3.0 * nd * nd * nd / 6.0e13;
   const double t_col = 15.0 * (24.0 * nd * nd / 4.0e12 + 1.0e-3);
-  if (nstall > 0 && (double)nstall * t_col > t_lu64) {
-    MOM_TRY(factor_impl<double>(S, S->Z64.as<double2>()));
+  if (nstall > 0 && (g_force_fp64_fallback || (double)nstall * t_col > t_lu64)) {
+    // the resident complex128 matrix is factored in place: if that fails
+    // part-way it holds neither Z nor its factors, so the system needs a
+    // new fill before it can factor or solve again
+    int frc = factor_impl<double>(S, S->Z64.as<double2>());
+    if (frc || S->info > 0) {
+      S->filled = S->factored = false;
+      return frc ? frc : fail(MOM_ERR_STATE, "the FP64 factorisation has a zero pivot");
+    }
     S->precision = MOM_PREC_FP64;
     S->fell_back = true;
-    if (S->info > 0) return fail(MOM_ERR_STATE, "the FP64 factorisation has a zero pivot");
     DevBuf dperm64;
     MOM_TRY(upload_perm(S, dperm64));
     MOM_TRY(getrs_blocked<double>(S->ctx, S->Z64.as<double2>(), n, n, dperm64.as<int>(), S->Linv.as<double2>(),
@@ -3069,6 +3079,7 @@ int mom_system_solve(mom_system *S, int64_t nrhs, double *B, int64_t ldb, double
   const double t0 = now_s();
   int rc;
   if (S->precision == MOM_PREC_FP64) {
+    if (S->info > 0) return fail(MOM_ERR_STATE, "the factorisation has a zero pivot");
     DevBuf dB, tmp, xb, dperm;
     if ((rc = dB.alloc(sizeof(double2) * n * nrhs)) || (rc = tmp.alloc(sizeof(double2) * n * nrhs)) ||
         (rc = xb.alloc(sizeof(double2) * NB * nrhs)) || (rc = upload_perm(S, dperm)))
@@ -3289,6 +3300,11 @@ extern "C" int mom_test_panel_mode(int32_t mode) {
 extern "C" int mom_test_u_solve(int32_t mode) {
   if (mode != 0 && mode != 1) return fail(MOM_ERR_ARG, "U-block mode must be 0 or 1");
   g_u_subst = mode;
+  return 0;
+}
+
+extern "C" int mom_test_batch_fallback(int32_t force) {
+  g_force_fp64_fallback = force ? 1 : 0;
   return 0;
 }
 

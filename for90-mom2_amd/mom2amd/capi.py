@@ -33,7 +33,7 @@ EXPORTS = [
 ]
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
-                "mom_test_unit_lower_inverse", "mom_test_u_solve", "mom_test_vertex_fd"]
+                "mom_test_unit_lower_inverse", "mom_test_u_solve", "mom_test_vertex_fd", "mom_test_batch_fallback"]
 
 
 class MomError(RuntimeError):
@@ -91,6 +91,7 @@ def lib():
     L.mom_test_lookahead.argtypes = [_i32]
     L.mom_test_u_solve.argtypes = [_i32]
     L.mom_test_vertex_fd.argtypes = [_i32]
+    L.mom_test_batch_fallback.argtypes = [_i32]
     L.mom_ctx_fill_tuning.argtypes = [_vp, C.POINTER(_i32), _dp]
     L.mom_test_unit_lower_inverse.argtypes = [_vp, _vp, _i64, _vp]
     L.mom_cs_prtsrf.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _d, _d, _i32, _dp, _dp, C.POINTER(_d),

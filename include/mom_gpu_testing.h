@@ -62,11 +62,19 @@ int mom_test_u_solve(int32_t mode);
  * on upload), as the complex64 LU forms it after each panel; E (256 x 256,
  * leading dimension 256) is returned widened to complex128.
  */
+int mom_test_unit_lower_inverse(mom_ctx *ctx, const double *L, int64_t ldl, double *E);
+
 /* Node-coordinate adjoint: 1 = every face pair by fourth-order differences
  * (the closed forms of the far pairs and of the near pairs' smoothed point
  * loop are checked against it), 0 = the default. */
 int mom_test_vertex_fd(int32_t enable);
-int mom_test_unit_lower_inverse(mom_ctx *ctx, const double *L, int64_t ldl, double *E);
+
+/* Block refinement of many right-hand sides in the mixed path: 1 = take the
+ * branch that refactors the resident complex128 matrix in FP64 whenever a
+ * column is above the tolerance after the first residual, 0 = the default
+ * (taken when finishing the stalled columns by GMRES would cost more).
+ * Process-wide; lets the tests run that branch on any matrix. */
+int mom_test_batch_fallback(int32_t force);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,14 @@ Fixtures (all complex128 / float64 at full precision):
                  index sqrt(-9.4+1.5i)): nlsurf_coef's src_coef/src_vec and
                  epsp per sphere, the SH source before and after
                  sysmat_pmchwt_nls, the SH solution and its rcs 37x72
-Usage: python tests/golden/make_golden.py [symm]   (symm: the symmetric fixtures only)
+  spectrum_s2.npz  the batched-spectrum workload at small size: icosphere s=2
+                 scaled to the 80/40/40 nm ellipsoid, 3 of the 64 wavelengths
+                 of `wlrg 64 400d-9 900d-9` (n = 1, 32, 64) with the Drude
+                 index, 8 plane waves (4 seeded directions x psi 0, 90 deg),
+                 all solved together by the reference's own solve_systems:
+                 per wavelength q, x, csca, cabs, the rcs 37x72 of source 1,
+                 sampled A entries and block norms
+Usage: python tests/golden/make_golden.py [symm|spectrum]   (only those fixtures)
 """
 import os
 import sys
@@ -32,6 +39,7 @@ sys.path.insert(0, os.path.join(ROOT, "for90-mom2_amd"))
 
 import refrun  # noqa: E402
 from mom2amd import meshgen  # noqa: E402
+from mom2amd.spectrum import drude_index, plane_waves_deg, wavelengths  # noqa: E402
 
 PI_F32 = float(np.float32(3.141592653589))
 C0 = 2.997924580003e8
@@ -122,6 +130,47 @@ def main_symm():
     symm_case("quarter_n4", 4, (2,), ["myp", "mxp"], (40.0, 25.0, 30.0), False)
 
 
+SPECTRUM_WL_INDEX = (0, 31, 63)
+
+
+def main_spectrum(nsample=2000, seed=3):
+    """The spectrum fixture: one reference run per wavelength, every source
+    solved at once (solver.f90:123-136) and post-processed per source."""
+    tmp = os.path.join(HERE, "_msh")
+    os.makedirs(tmp, exist_ok=True)
+    nodes, faces = meshgen.icosphere(2, 1.0)
+    nodes = nodes * np.array([80.0, 40.0, 40.0])
+    fn = os.path.join(tmp, "ellipsoid2.msh")
+    meshgen.write_gmsh(fn, nodes, faces, 15)
+    wls = wavelengths(64)[list(SPECTRUM_WL_INDEX)]
+    ang = plane_waves_deg(128)
+    ang = np.concatenate([ang[:4], ang[128:132]])
+    data = dict(wl=wls, wl_index=np.array(SPECTRUM_WL_INDEX), angles_deg=ang)
+    rng = np.random.default_rng(seed)
+    for i, wl in enumerate(wls):
+        ri = [1.0 + 0j, drude_index(wl)]
+        R = refrun.run(fn, 1e-9, [[15], [-15]], wl, ri, ang, dump_ops=False, dump_sys=True, solve=True,
+                       ntheta=37, nphi=72)
+        A = R["A"]
+        N = A.shape[0]
+        E = N // 2
+        data["ri_%d" % i] = np.array(ri, dtype=np.complex128)
+        data["q_%d" % i] = R["q"]
+        data["x_%d" % i] = R["xref"]
+        data["csca_%d" % i], data["cabs_%d" % i] = R["csca"], R["cabs"]
+        data["rcs0_%d" % i] = R["rcs"][:, :, 0]
+        data["block_fro_%d" % i] = np.array([[np.linalg.norm(A[a * E:(a + 1) * E, b * E:(b + 1) * E])
+                                              for b in range(2)] for a in range(2)])
+        ii = np.concatenate([rng.integers(0, N, nsample), np.arange(N)])
+        jj = np.concatenate([rng.integers(0, N, nsample), np.arange(N)])
+        data["A_i_%d" % i], data["A_j_%d" % i], data["A_v_%d" % i] = ii.astype(np.int32), jj.astype(np.int32), A[ii, jj]
+        print("spectrum wl=%.1f nm n_in=%s N=%d csca[0]=%.6e" % (wl * 1e9, ri[1], N, R["csca"][0]))
+    for d, m in enumerate(R["mesh"]):
+        for k, v in mesh_fields(m, False).items():
+            data["mesh%d_%s" % (d, k)] = v
+    np.savez_compressed(os.path.join(HERE, "spectrum_s2.npz"), **data)
+
+
 def main():
     if not refrun.available():
         raise SystemExit("oracle/_ref/mom_ref is not built (run oracle/build_ref.sh)")
@@ -145,6 +194,9 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["symm"]:
         main_symm()
+    elif sys.argv[1:] == ["spectrum"]:
+        main_spectrum()
     else:
         main()
         main_symm()
+        main_spectrum()
