@@ -213,6 +213,13 @@ void near_pairs(const std::vector<double> &cp, int64_t F, double avelen, std::ve
   }
 }
 
+void near_offsets(const std::vector<int32_t> &pairs, int64_t F, std::vector<int64_t> &off) {
+  off.assign(F + 1, 0);
+  const int64_t np = (int64_t)pairs.size() / 2;
+  for (int64_t k = 0; k < np; k++) off[pairs[2 * k] + 1]++;
+  for (int64_t f = 0; f < F; f++) off[f + 1] += off[f];
+}
+
 }  // namespace mom
 
 using namespace mom;
@@ -369,6 +376,8 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
   std::vector<int32_t> np;
   near_pairs(m->h_fcp, F, avelen, np);
   m->nnear = (int64_t)np.size() / 2;
+  std::vector<int64_t> noff;
+  near_offsets(np, F, noff);
   hipStream_t s = ctx->stream;
   int rc = 0;
   (void)hipSetDevice(ctx->device);
@@ -378,7 +387,8 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
       (rc = upload(ctx, m->fcp, fcp, 3 * F)) || (rc = upload(ctx, m->s_g, sg.data(), 3 * F)) ||
       (rc = upload(ctx, m->s_l, sl.data(), 3 * F)) || (rc = upload(ctx, m->s_coef, scoef.data(), 3 * F)) ||
       (rc = upload(ctx, m->s_div, sdiv.data(), 3 * F)) || (rc = upload(ctx, m->s_v, sv.data(), 9 * F)) ||
-      (rc = upload(ctx, m->near, np.data(), np.size())) || (rc = upload(ctx, m->eside, eside.data(), eside.size())) ||
+      (rc = upload(ctx, m->near, np.data(), np.size())) || (rc = upload(ctx, m->near_off, noff.data(), noff.size())) ||
+      (rc = upload(ctx, m->eside, eside.data(), eside.size())) ||
       (rc = upload(ctx, m->slab_edges, slab_edges.data(), slab_edges.size()))) {
     delete m;
     return rc;

@@ -60,11 +60,6 @@ struct FarArgs {
   FillTarget out;
 };
 
-__device__ __forceinline__ void cadd_atomic(double2 *p, double re, double im) {
-  unsafeAtomicAdd(&p->x, re);
-  unsafeAtomicAdd(&p->y, im);
-}
-
 // exp(a) (cos b + i sin b) for the point-pair loop, where the library
 // exp and sincos are about half of the instructions.  Cody-Waite reduction
 // (Cody and Waite, "Software Manual for the Elementary Functions", 1980):
@@ -137,13 +132,12 @@ __device__ bool near_pair(const double *cpm, const double *cpn, double thr2) {
   return ds < thr2;
 }
 
-// Writes the 3x3 edge block of one face pair.  D, H, K are complex values
-// of the pair's contribution to computeD/H/K entries.
 // Adds one edge pair's D+H (ar, ai) and K (kr, ki) to every representation
 // matrix: e (J,J), e detj Zsi (M,M), e detj K (M,J), -e K (J,M), the
 // blocks of sysmat.f90:66-98 with the element's field action e, at the
 // rows and columns the representation keeps.
-template <bool ATOMIC>
+// Each caller owns the entries it adds to (no other thread writes them
+// during the launch), so the adds are plain read-modify-writes.
 __device__ __forceinline__ void store_reps(const FillTarget &o, int64_t gi, int64_t gj, double ar, double ai,
                                            double kr, double ki) {
   const int64_t N = o.Noff;
@@ -157,36 +151,13 @@ __device__ __forceinline__ void store_reps(const FillTarget &o, int64_t gi, int6
     auto add = [&](int i, int j, double vr, double vi) {
       if (i < 0 || j < 0) return;
       double2 *z = Z + i + (int64_t)j * ld;
-      if (ATOMIC) {
-        cadd_atomic(z, vr, vi);
-      } else {
-        const double2 v = *z;
-        *z = make_double2(v.x + vr, v.y + vi);
-      }
+      const double2 v = *z;
+      *z = make_double2(v.x + vr, v.y + vi);
     };
     add(a, b, e * ar, e * ai);
     add(am, bm, ed * mr, ed * mi);
     add(am, b, ed * kr, ed * ki);
     add(a, bm, -e * kr, -e * ki);
-  }
-}
-
-__device__ __forceinline__ void emit(const FillTarget &o, int gi, int gj, double Dr, double Di,
-                                     double Hr, double Hi, double Kr, double Ki) {
-  if (o.nrep > 0) {
-    store_reps<true>(o, gi, gj, Dr + Hr, Di + Hi, Kr, Ki);
-  } else if (o.mode == 0) {
-    const int64_t N = o.Noff, ld = o.ld;
-    double Sr = Dr + Hr, Si = Di + Hi;
-    cadd_atomic(o.Z + gi + (int64_t)gj * ld, Sr, Si);
-    double Mr = o.zsi.x * Sr - o.zsi.y * Si, Mi = o.zsi.x * Si + o.zsi.y * Sr;
-    cadd_atomic(o.Z + (gi + N) + (int64_t)(gj + N) * ld, Mr, Mi);
-    cadd_atomic(o.Z + (gi + N) + (int64_t)gj * ld, Kr, Ki);
-    cadd_atomic(o.Z + gi + (int64_t)(gj + N) * ld, -Kr, -Ki);
-  } else {
-    double vr = o.mode == 1 ? Dr : (o.mode == 2 ? Hr : Kr);
-    double vi = o.mode == 1 ? Di : (o.mode == 2 ? Hi : Ki);
-    cadd_atomic(o.Z + gi + (int64_t)gj * o.ld, vr, vi);
   }
 }
 
@@ -502,7 +473,7 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
 __device__ __forceinline__ void store_entry(const FillTarget &o, int64_t gi, int64_t gj, double ar, double ai,
                                                  double kr, double ki) {
   if (o.nrep > 0) {
-    store_reps<false>(o, gi, gj, ar, ai, kr, ki);
+    store_reps(o, gi, gj, ar, ai, kr, ki);
   } else if (o.mode == 0) {
     const int64_t N = o.Noff, ld = o.ld;
     double2 *z = o.Z + gi + gj * ld;
@@ -767,12 +738,13 @@ struct NearArgs {
   const double *qp, *p, *fn, *fm, *fs, *area;
   const int *fnodes;
   FaceSide side;
+  int mode;     // output of FillTarget::mode
   double2 kk4;  // 1/(4pi) (real, kept complex for symmetry)
   double2 kk8;  // k^2/(8pi)
   double2 c1, c2;
   double js[3];  // test faces reflected by diag(js)
   int image;
-  FillTarget out;
+  double2 *nb;   // per pair and (test side, source side): D+H (or the selected operator), K
 };
 
 __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
@@ -874,17 +846,111 @@ __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
     }
   }
   const double Am = a.area[m];
+  double2 *out = a.nb + 18 * id;
   for (int p = 0; p < 3; p++) {
     const double divm = a.side.div[3 * m + p];
-    const int gi = a.side.g[3 * m + p];
     for (int q = 0; q < 3; q++) {
-      const int gj = a.side.g[3 * n + q];
       const double dr = accD[p][q][0] * Am, di = accD[p][q][1] * Am;
       const double Dr = a.c1.x * dr - a.c1.y * di, Di = a.c1.x * di + a.c1.y * dr;
       const double hr = accH[q][0] * Am * divm, hi = accH[q][1] * Am * divm;
       const double Hr = a.c2.x * hr - a.c2.y * hi, Hi = a.c2.x * hi + a.c2.y * hr;
-      emit(a.out, gi, gj, Dr, Di, Hr, Hi, accK[p][q][0] * Am, accK[p][q][1] * Am);
+      const double Kr = accK[p][q][0] * Am, Ki = accK[p][q][1] * Am;
+      double2 v0;
+      if (a.mode == 1)
+        v0 = make_double2(Dr, Di);
+      else if (a.mode == 2)
+        v0 = make_double2(Hr, Hi);
+      else if (a.mode == 3)
+        v0 = make_double2(Kr, Ki);
+      else
+        v0 = make_double2(Dr + Hr, Di + Hi);
+      out[2 * (3 * p + q)] = v0;
+      out[2 * (3 * p + q) + 1] = make_double2(Kr, Ki);
     }
+  }
+}
+
+// Index of the near pair (m, n) in the list sorted by test then source face
+// (off[m] .. off[m+1] the pairs of test face m), -1 if (m, n) is not near.
+__device__ __forceinline__ int64_t find_near(const int *__restrict__ pairs, const int64_t *__restrict__ off, int m,
+                                             int n) {
+  int64_t lo = off[m], hi = off[m + 1];
+  const int64_t end = hi;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (pairs[2 * mid + 1] < n)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (lo < end && pairs[2 * lo + 1] == n) ? lo : -1;
+}
+
+struct NearEmitArgs {
+  const int *pairs;
+  const int64_t *off;
+  int64_t npairs;
+  const int *eside;  // face-side codes 3 f + s of each local edge's two faces (-1 none)
+  const int *s_l;    // local edge of each face side
+  FaceSide side;     // output index of each face side
+  const double2 *nb;
+  int mode;
+  FillTarget out;
+};
+
+// The near pairs' analytic parts added to the matrix in a fixed order,
+// without atomics.  An entry (edge i, edge j) collects the pairs (m, n) with
+// m a face of i and n a face of j; thread (pair, test side, source side)
+// owns the entry when its pair is the first near one in the order (face of
+// i, face of j) = (+,+), (+,-), (-,+), (-,-), and then sums all of them in
+// that order and adds the sum once.
+__global__ __launch_bounds__(256) void near_emit_kernel(NearEmitArgs a) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 9 * a.npairs) return;
+  const int64_t id = t / 9;
+  const int pq = (int)(t - 9 * id), p = pq / 3, q = pq % 3;
+  const int m = a.pairs[2 * id], n = a.pairs[2 * id + 1];
+  const int sm = 3 * m + p, sn = 3 * n + q;
+  const int i = a.s_l[sm], j = a.s_l[sn];
+  double2 v0 = make_double2(0.0, 0.0), v1 = make_double2(0.0, 0.0);
+  bool first = true;
+  for (int ci = 0; ci < 2; ci++) {
+    const int ca = a.eside[2 * i + ci];
+    if (ca < 0) continue;
+    for (int cj = 0; cj < 2; cj++) {
+      const int cb = a.eside[2 * j + cj];
+      if (cb < 0) continue;
+      const int64_t k = find_near(a.pairs, a.off, ca / 3, cb / 3);
+      if (k < 0) continue;
+      if (first && k != id) return;  // another thread owns this entry
+      first = false;
+      const int slot = 3 * (ca % 3) + cb % 3;
+      const double2 u0 = a.nb[18 * k + 2 * slot], u1 = a.nb[18 * k + 2 * slot + 1];
+      v0.x += u0.x;
+      v0.y += u0.y;
+      v1.x += u1.x;
+      v1.y += u1.y;
+    }
+  }
+  const int gi = a.side.g[sm], gj = a.side.g[sn];
+  const FillTarget &o = a.out;
+  if (o.nrep > 0) {
+    store_reps(o, gi, gj, v0.x, v0.y, v1.x, v1.y);
+    return;
+  }
+  auto add = [&](int64_t r, int64_t c, double vr, double vi) {
+    double2 *z = o.Z + r + c * o.ld;
+    const double2 w = *z;
+    *z = make_double2(w.x + vr, w.y + vi);
+  };
+  if (a.mode == 0) {
+    const int64_t N = o.Noff;
+    add(gi, gj, v0.x, v0.y);
+    add(gi + N, gj + N, o.zsi.x * v0.x - o.zsi.y * v0.y, o.zsi.x * v0.y + o.zsi.y * v0.x);
+    add(gi + N, gj, v1.x, v1.y);
+    add(gi, gj + N, -v1.x, -v1.y);
+  } else {
+    add(gi, gj, v0.x, v0.y);
   }
 }
 
@@ -1062,20 +1128,30 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   MOM_HIP(hipStreamSynchronize(ctx->stream));
 
   // near pairs of the image: the reflected test faces against the sources
-  DevBuf inear;
+  DevBuf inear, ioff;
   int64_t nnear = m->nnear;
   const int *pairs = m->near.as<int>();
+  const int64_t *off = m->near_off.as<int64_t>();
   if (img) {
     std::vector<int32_t> np;
     near_pairs(m->h_fcp, m->F, m->avelen, np, img->js);
     nnear = (int64_t)np.size() / 2;
     if (nnear > 0) {
-      if ((rc = inear.alloc(sizeof(int32_t) * np.size()))) return rc;
+      std::vector<int64_t> ho;
+      near_offsets(np, m->F, ho);
+      if ((rc = inear.alloc(sizeof(int32_t) * np.size())) || (rc = ioff.alloc(sizeof(int64_t) * ho.size())))
+        return rc;
       MOM_TRY(copy_h2d(ctx, inear.ptr, np.data(), sizeof(int32_t) * np.size()));
+      MOM_TRY(copy_h2d(ctx, ioff.ptr, ho.data(), sizeof(int64_t) * ho.size()));
     }
     pairs = inear.as<int>();
+    off = ioff.as<int64_t>();
   }
   if (nnear > 0) {
+    // per pair blocks into scratch (the slab planes are assembled by now),
+    // then one fixed-order sum per matrix entry
+    const size_t nbytes = sizeof(double2) * 18 * (size_t)nnear;
+    if (ctx->scratch.bytes < nbytes && (rc = ctx->scratch.alloc(nbytes))) return rc;
     NearArgs b;
     b.Q = rule.Q;
     for (int i = 0; i < kMaxQ; i++) b.w[i] = a.w[i];
@@ -1089,6 +1165,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     b.area = a.area;
     b.fnodes = m->fnodes.as<int>();
     b.side = a.side;
+    b.mode = t.mode;
     // analytic part [1/R - k^2 R/2]/4pi; with dk its k-derivative -k R/4pi
     b.kk4 = make_double2(t.dk ? 0.0 : 1.0 / (4.0 * MOM_PI), 0.0);
     const cd kk8 = t.dk ? k / (4.0 * MOM_PI) : (k * k) / (8.0 * MOM_PI);
@@ -1097,11 +1174,23 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     b.c2 = a.c2;
     for (int c = 0; c < 3; c++) b.js[c] = a.js[c];
     b.image = a.image;
-    b.out = t;
+    b.nb = ctx->scratch.as<double2>();
     hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((nnear + 127) / 128)), dim3(128), 0,
                        ctx->stream, b);
     MOM_HIP(hipGetLastError());
-    // the image's pair list must outlive the kernel
+    NearEmitArgs e;
+    e.pairs = pairs;
+    e.off = off;
+    e.npairs = nnear;
+    e.eside = m->eside.as<int>();
+    e.s_l = m->s_l.as<int>();
+    e.side = a.side;
+    e.nb = b.nb;
+    e.mode = t.mode;
+    e.out = t;
+    hipLaunchKernelGGL(near_emit_kernel, dim3((unsigned)((9 * nnear + 255) / 256)), dim3(256), 0, ctx->stream, e);
+    MOM_HIP(hipGetLastError());
+    // the image's pair list must outlive the kernels
     if (img) MOM_HIP(hipStreamSynchronize(ctx->stream));
   }
   return 0;
@@ -1139,10 +1228,12 @@ __device__ __forceinline__ void pw_eh(const PwSrc &S, const double *r, double kr
   }
 }
 
+// w[(col 3F + 3 f + p) 2 + {0, 1}]: <f_p, E_r>, <f_p, H_r> of face side p of
+// face f for column col = source * G.n + representation
 __global__ void srcvec_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
                               const double *__restrict__ area, FaceSide side, int nsrc,
                               const PwSrc *__restrict__ src, double kr, double ki, double2 hfac,
-                              int64_t Nt, double2 *q, int64_t ldq, Group G) {
+                              double2 *__restrict__ w, Group G) {
   int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= (int64_t)F * nsrc * G.n) return;
   const int f = (int)(id % F);
@@ -1194,12 +1285,41 @@ __global__ void srcvec_kernel(int F, int Q, QuadRule rule, const double *__restr
     }
   }
   const double Am = area[f];
-  double2 *qs = q + sr * ldq;
+  double2 *ws = w + 2 * (sr * 3 * (int64_t)F + 3 * f);
   for (int p = 0; p < 3; p++) {
-    const int g = side.g[3 * f + p];
-    cadd_atomic(qs + g, Ie[p][0] * Am, Ie[p][1] * Am);
-    cadd_atomic(qs + g + Nt, Ih[p][0] * Am, Ih[p][1] * Am);
+    ws[2 * p] = make_double2(Ie[p][0] * Am, Ie[p][1] * Am);
+    ws[2 * p + 1] = make_double2(Ih[p][0] * Am, Ih[p][1] * Am);
   }
+}
+
+// q[g, col] += the edge's face sides in face order (srcvec's loop over the
+// faces, source.f90:198-244), one thread per (local edge, column): no
+// atomics, so the sums are the same on every run.
+__global__ void srcvec_gather_kernel(int64_t E, int F, int64_t ncol, const int *__restrict__ eside,
+                                     const int *__restrict__ s_g, const double2 *__restrict__ w, int64_t Nt,
+                                     double2 *q, int64_t ldq) {
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id >= E * ncol) return;
+  const int64_t e = id % E, col = id / E;
+  int c0 = eside[2 * e], c1 = eside[2 * e + 1];
+  if (c0 < 0 || (c1 >= 0 && c1 / 3 < c0 / 3)) {
+    const int t = c0;
+    c0 = c1;
+    c1 = t;
+  }
+  if (c0 < 0) return;
+  const double2 *wc = w + 2 * col * 3 * (int64_t)F;
+  double2 se = wc[2 * c0], sh = wc[2 * c0 + 1];
+  if (c1 >= 0) {
+    se.x += wc[2 * c1].x;
+    se.y += wc[2 * c1].y;
+    sh.x += wc[2 * c1 + 1].x;
+    sh.y += wc[2 * c1 + 1].y;
+  }
+  const int64_t g = s_g[c0];
+  double2 *qs = q + col * ldq;
+  qs[g] = make_double2(qs[g].x + se.x, qs[g].y + se.y);
+  qs[g + Nt] = make_double2(qs[g + Nt].x + sh.x, qs[g + Nt].y + sh.y);
 }
 
 int srcvec_pw(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
@@ -1217,16 +1337,19 @@ int srcvec_pw(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omeg
     hs[s].pol[1] = -cos(ps) * cos(ph) - sin(ps) * cos(th) * sin(ph);
     hs[s].pol[2] = sin(ps) * sin(th);
   }
-  DevBuf dsrc;
-  if ((rc = dsrc.alloc(sizeof(PwSrc) * nsrc))) return rc;
+  const int64_t ncol = nsrc * G.n;
+  DevBuf dsrc, dw;
+  if ((rc = dsrc.alloc(sizeof(PwSrc) * nsrc)) || (rc = dw.alloc(sizeof(double2) * 6 * m->F * ncol))) return rc;
   MOM_TRY(copy_h2d(ctx, dsrc.ptr, hs.data(), sizeof(PwSrc) * nsrc));
   const cd k = ri * omega / kC0;
   const cd hfac = k / (omega * mu0());
-  int64_t total = m->F * nsrc * G.n;
+  int64_t total = m->F * ncol;
   hipLaunchKernelGGL(srcvec_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, ctx->stream,
                      (int)m->F, rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(),
                      sides(m, true), (int)nsrc, dsrc.as<PwSrc>(), k.real(), k.imag(),
-                     make_double2(hfac.real(), hfac.imag()), Nt, q, ldq, G);
+                     make_double2(hfac.real(), hfac.imag()), dw.as<double2>(), G);
+  hipLaunchKernelGGL(srcvec_gather_kernel, dim3((unsigned)((m->E * ncol + 255) / 256)), dim3(256), 0, ctx->stream,
+                     m->E, (int)m->F, ncol, m->eside.as<int>(), m->s_g.as<int>(), dw.as<double2>(), Nt, q, ldq);
   MOM_HIP(hipGetLastError());
   MOM_HIP(hipStreamSynchronize(ctx->stream));
   return 0;

@@ -108,6 +108,7 @@ struct mom_mesh {
   mom::DevBuf s_div;   // sign * L / A (surface divergence)
   mom::DevBuf s_v;     // free vertex (3 doubles)
   mom::DevBuf near;    // int32 pairs (test face, source face), sorted
+  mom::DevBuf near_off;  // int64 F + 1: the pairs of test face m are near_off[m] .. near_off[m+1]
   // fill assembly: face-side codes 3f+s of each edge's two faces (-1 none),
   // and per slab of kSlab source faces the edges with a side in the slab
   mom::DevBuf eside, slab_edges;
@@ -184,6 +185,8 @@ struct Image {
 // with js, of the test faces reflected by diag(js) against the source faces
 void near_pairs(const std::vector<double> &cp, int64_t F, double avelen, std::vector<int32_t> &out,
                 const double *js = nullptr);
+// off[m] .. off[m+1]: the pairs of test face m in a list sorted by test face (F + 1 entries)
+void near_offsets(const std::vector<int32_t> &pairs, int64_t F, std::vector<int64_t> &off);
 
 // fill.hip
 int compute_quad_points(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule);

@@ -26,7 +26,9 @@
 !                  solution of solve_systems and its rcs at 2 omega
 !                  (interface.f90:1316-1325)
 !
-! Input on stdin, one item per line:
+! Input on stdin, one item per line (or "lu" / "n nrhs" / output directory:
+! only solve_multi_linsys, linalg.f90:159-180, on a seeded random complex
+! n x n system, timed into times.txt):
 !   mesh file / scale / ndom / per domain: "nsurf id1 id2 ..." /
 !   wavelength [m] / per domain: "re im" refractive index / nsrc /
 !   per source: "theta phi psi" [deg] / "dump_ops fill do_solve" with
@@ -65,8 +67,37 @@ PROGRAM ref_driver
   COMPLEX(KIND=dp), DIMENSION(:,:,:), ALLOCATABLE :: nlx, Ash
   COMPLEX(KIND=dp), DIMENSION(:), ALLOCATABLE :: src_vec, epsp, rhs0
   CHARACTER(LEN=1), DIMENSION(3) :: opn = (/'D','H','K'/)
+  INTEGER :: nlu, nrhslu
+  INTEGER, DIMENSION(:), ALLOCATABLE :: seed
+  REAL(KIND=dp), DIMENSION(:,:), ALLOCATABLE :: rre, rim
+  COMPLEX(KIND=dp), DIMENSION(:,:), ALLOCATABLE :: Alu, Blu
 
   READ(*,'(A)') mshfile
+  IF(TRIM(mshfile)=='lu') THEN
+     READ(*,*) nlu, nrhslu
+     READ(*,'(A)') outdir
+     CALL RANDOM_SEED(SIZE=i)
+     ALLOCATE(seed(i))
+     seed(:) = 20251203
+     CALL RANDOM_SEED(PUT=seed)
+     ALLOCATE(rre(nlu,nlu), rim(nlu,nlu), Alu(nlu,nlu), Blu(nlu,nrhslu))
+     CALL RANDOM_NUMBER(rre)
+     CALL RANDOM_NUMBER(rim)
+     Alu(:,:) = CMPLX(rre - 0.5_dp, rim - 0.5_dp, KIND=dp)
+     DO i=1,nlu
+        Alu(i,i) = Alu(i,i) + REAL(nlu, KIND=dp)
+     END DO
+     Blu(:,:) = Alu(:,1:nrhslu)
+     DEALLOCATE(rre, rim)
+     t0 = omp_get_wtime()
+     CALL solve_multi_linsys(Alu, Blu)
+     tsolve = omp_get_wtime() - t0
+     OPEN(NEWUNIT=u, FILE=TRIM(outdir)//'/times.txt', STATUS='REPLACE')
+     WRITE(u,'(A,ES25.17)') 'solve_multi_linsys ', tsolve
+     WRITE(u,'(A,I0)') 'threads ', omp_get_max_threads()
+     CLOSE(u)
+     STOP
+  END IF
   READ(*,*) scale
   READ(*,*) ndom
 

@@ -1,7 +1,8 @@
 """Run the reference golden-vector driver (oracle/_ref/mom_ref) and read its dumps.
 
-TEST INFRASTRUCTURE ONLY (used by tests/golden/make_golden.py and the
-reference-vs-oracle tests, which are skipped where oracle/_ref is not built).
+TEST INFRASTRUCTURE ONLY: used by tests/golden/make_golden.py, the
+reference-vs-oracle tests (skipped where oracle/_ref is not built) and
+bench.py's cpu_baseline leg, which times the reference's own fill and LU.
 """
 import os
 import resource
@@ -10,7 +11,7 @@ import tempfile
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "mom_ref")
 
 
@@ -126,6 +127,19 @@ def run(mshfile, scale, domains, wl, ri, angles_deg, dump_ops=True, dump_sys=Tru
             k, v = ln.split()
             res["time_" + k] = float(v)
     return res
+
+
+def run_lu(n, nrhs=1, threads=None, outdir=None):
+    """Seconds of the reference's solve_multi_linsys (linalg.f90:159-180:
+    ZGETRF + ZGETRS) on a seeded random complex n x n system with nrhs
+    right-hand sides."""
+    outdir = outdir or tempfile.mkdtemp(prefix="momreflu_")
+    lines = ["lu", "%d %d" % (n, nrhs), outdir]
+    subprocess.run([REF_BIN], input="\n".join(lines) + "\n", text=True, check=True, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL, env=_env(threads), preexec_fn=_unlimited_stack)
+    with open(os.path.join(outdir, "times.txt")) as fh:
+        t = dict((k, float(v)) for k, v in (ln.split() for ln in fh))
+    return t["solve_multi_linsys"], int(t["threads"])
 
 
 REF_SYMM_BIN = os.path.join(ROOT, "oracle", "_ref", "mom_ref_symm")

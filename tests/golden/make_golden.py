@@ -34,7 +34,7 @@ import scipy.linalg as sl
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
-sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "for90-mom2_amd"))
 
 import refrun  # noqa: E402
