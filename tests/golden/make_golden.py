@@ -17,6 +17,9 @@ Fixtures (all complex128 / float64 at full precision):
                  index sqrt(-9.4+1.5i)): nlsurf_coef's src_coef/src_vec and
                  epsp per sphere, the SH source before and after
                  sysmat_pmchwt_nls, the SH solution and its rcs 37x72
+  ops_s1.npz     computeD, computeH and computeK of both domains of the
+                 icosphere s=1 (E = 120 each), as the reference's driver dumps
+                 them one by one (dump_ops)
   spectrum_s2.npz  the batched-spectrum workload at small size: icosphere s=2
                  scaled to the 80/40/40 nm ellipsoid, 3 of the 64 wavelengths
                  of `wlrg 64 400d-9 900d-9` (n = 1, 32, 64) with the Drude
@@ -24,7 +27,7 @@ Fixtures (all complex128 / float64 at full precision):
                  all solved together by the reference's own solve_systems:
                  per wavelength q, x, csca, cabs, the rcs 37x72 of source 1,
                  sampled A entries and block norms
-Usage: python tests/golden/make_golden.py [symm|spectrum]   (only those fixtures)
+Usage: python tests/golden/make_golden.py [symm|spectrum|ops]   (only those fixtures)
 """
 import os
 import sys
@@ -130,6 +133,23 @@ def main_symm():
     symm_case("quarter_n4", 4, (2,), ["myp", "mxp"], (40.0, 25.0, 30.0), False)
 
 
+def main_ops():
+    """computeD/H/K of each domain of the s = 1 sphere (sysmat.f90:272-519)."""
+    tmp = os.path.join(HERE, "_msh")
+    os.makedirs(tmp, exist_ok=True)
+    nodes, faces = meshgen.icosphere(1, 50.0)
+    fn = os.path.join(tmp, "ico1.msh")
+    meshgen.write_gmsh(fn, nodes, faces, 15)
+    ri = [1.0 + 0j, np.sqrt(-9.4 + 1.5j)]
+    R = refrun.run(fn, 1e-9, [[15], [-15]], 600e-9, ri, (0.0, 90.0, 0.0), dump_ops=True, dump_sys=False)
+    data = dict(wl=np.array(600e-9), ri=np.array(ri, dtype=np.complex128))
+    for d in range(2):
+        for op in "DHK":
+            data["op_%d_%s" % (d, op)] = R["op_%d_%s" % (d, op)]
+    np.savez_compressed(os.path.join(HERE, "ops_s1.npz"), **data)
+    print("ops_s1", {k: v.shape for k, v in data.items() if k.startswith("op_")})
+
+
 SPECTRUM_WL_INDEX = (0, 31, 63)
 
 
@@ -196,7 +216,10 @@ if __name__ == "__main__":
         main_symm()
     elif sys.argv[1:] == ["spectrum"]:
         main_spectrum()
+    elif sys.argv[1:] == ["ops"]:
+        main_ops()
     else:
         main()
         main_symm()
         main_spectrum()
+        main_ops()

@@ -218,3 +218,18 @@ def test_oracle_gram_closed_form(msh_dir):
     assert np.allclose(F, F.T)
     assert np.abs(F - ref).max() / np.abs(ref).max() < 1e-7
     assert np.linalg.eigvalsh(F).min() > 0
+
+
+def test_oracle_operators_match_reference_one_by_one(msh_dir):
+    """computeD, computeH and computeK (sysmat.f90:272-519) of both domains of
+    the s = 1 sphere, each against the reference's own dump of that operator
+    (not only through their sums in sysmat_pmchwt)."""
+    g = golden("ops_s1")
+    case = sphere(msh_dir, 1)
+    op = oracle.OracleProblem(case["path"], 1e-9, case["domains"])
+    for d in range(2):
+        for name in "DHK":
+            R = g["op_%d_%s" % (d, name)]
+            M = op.operator(d, name, case["omega"], case["ri"][d])
+            err = np.abs(M - R).max() / np.abs(R).max()
+            assert err < 1e-13, (d, name, err)
