@@ -1682,6 +1682,10 @@ struct PanelCoop {
   PanelSync sync;
   int gmax = 0;
   unsigned *tag = nullptr;
+  // set when the runtime refuses a cooperative launch (another process
+  // holds the device's cooperative resources): the panels then take the
+  // one-launch-per-column path
+  bool *refused = nullptr;
 };
 
 // Factorisation of the panel of block columns [j0, j0+jb) over rows
@@ -1697,7 +1701,8 @@ static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int6
     const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
     Timed tp(st, s, kPhPanel);
     const int64_t G = (n - i0 + 255) / 256;
-    if (G <= coop.gmax) {
+    bool coop_done = false;
+    if (G <= coop.gmax && !(coop.refused && *coop.refused)) {
       // the whole inner block in one cooperative launch, then its row
       // interchanges on the other columns of the panel
       unsigned tag0 = *coop.tag;
@@ -1709,8 +1714,18 @@ static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int6
       int *ipivp = ipiv;
       PanelSync sy = coop.sync;
       void *args[] = {&Ap, &ldv, &i0i, &ibi, &ni, &ipivp, &infop, &sy, &tag0};
-      MOM_HIP(hipLaunchCooperativeKernel((const void *)panel_block_kernel<T>, dim3((unsigned)G), dim3(256), args, 0,
-                                         s));
+      const hipError_t e = hipLaunchCooperativeKernel((const void *)panel_block_kernel<T>, dim3((unsigned)G),
+                                                      dim3(256), args, 0, s);
+      if (e == hipSuccess) {
+        coop_done = true;
+      } else if (coop.refused) {
+        (void)hipGetLastError();
+        *coop.refused = true;
+      } else {
+        return fail(MOM_ERR_HIP, std::string("hipLaunchCooperativeKernel(panel): ") + hipGetErrorString(e));
+      }
+    }
+    if (coop_done) {
       const int64_t left = i0 - j0, right = j0 + jb - (i0 + ib);
       if (left + right > 0) {
         hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), (int)n, w.plan);
@@ -2418,6 +2433,84 @@ static int getrs_blocked(mom_ctx *ctx, const typename Cx<TS>::type *LU, int64_t 
   return 0;
 }
 
+// T[r, j] = (P R v)[r, j] rounded to complex64: v scaled by the row
+// equilibration d and permuted (row r of the result is row perm[r] of d v)
+__global__ void scale_gather_round_kernel(const double2 *__restrict__ v, int64_t ldv, const double *__restrict__ d,
+                                          const int *__restrict__ perm, int64_t n, float2 *__restrict__ T) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (r >= n) return;
+  const int64_t pr = perm[r];
+  const double2 a = v[j * ldv + pr];
+  T[j * n + r] = make_float2((float)(a.x * d[pr]), (float)(a.y * d[pr]));
+}
+
+// x[:, j] (+)= d .* T[:, j] (T complex64 widened)
+__global__ void widen_scale_store_kernel(double2 *x, int64_t ldx, const double *__restrict__ d,
+                                         const float2 *__restrict__ T, int64_t n, int add) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (r >= n) return;
+  const float2 t = T[j * n + r];
+  double2 &o = x[j * ldx + r];
+  const double2 v = make_double2((double)t.x * d[r], (double)t.y * d[r]);
+  if (add) {
+    o.x += v.x;
+    o.y += v.y;
+  } else {
+    o = v;
+  }
+}
+
+// rows [r0, r0 + w) of T (complex64, n x nrhs) widened into W (NB x nrhs),
+// and back rounded
+__global__ void widen_rows_kernel(const float2 *__restrict__ T, int64_t n, int64_t r0, int w, double2 *W) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (r >= w) return;
+  const float2 t = T[j * n + r0 + r];
+  W[(int64_t)j * NB + r] = make_double2(t.x, t.y);
+}
+__global__ void round_rows_kernel(const double2 *__restrict__ W, int64_t n, int64_t r0, int w, float2 *T) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (r >= w) return;
+  const double2 v = W[(int64_t)j * NB + r];
+  T[j * n + r0 + r] = make_float2((float)v.x, (float)v.y);
+}
+
+// L U T = T for a block of right-hand sides in complex64 (T, n x nrhs,
+// leading dimension n, already permuted): per 256-row block the complex128
+// diagonal-block inverse times the block (in FP64, then rounded), then the
+// complex64 panel below (forward) or above (backward) times the solved
+// block subtracted from the rest on the complex64 matrix cores (the
+// trailing-update kernels).  Used by the block refinement, whose steps only
+// need a preconditioner as accurate as the complex64 factors themselves;
+// the single-column GMRES keeps the exact FP64 application.  Wa, Wb: NB x
+// nrhs complex128 scratch.
+static int getrs_blocked32(mom_ctx *ctx, const float2 *LU, int64_t ld, int64_t n, const double2 *Linv,
+                           const double2 *Uinv, float2 *T, int64_t nrhs, double2 *Wa, double2 *Wb) {
+  hipStream_t s = ctx->stream;
+  const Lane L = main_lane(ctx);
+  const int64_t nblk = (n + NB - 1) / NB;
+  auto diag = [&](const double2 *Inv, int64_t r0, int w) -> int {
+    const dim3 g((unsigned)((w + 255) / 256), (unsigned)nrhs);
+    hipLaunchKernelGGL(widen_rows_kernel, g, dim3(256), 0, s, T, n, r0, w, Wa);
+    MOM_TRY(gemm_generic<double>(s, Inv, NB, Wa, NB, Wb, NB, w, nrhs, w, 1.0, true));
+    hipLaunchKernelGGL(round_rows_kernel, g, dim3(256), 0, s, Wb, n, r0, w, T);
+    MOM_HIP(hipGetLastError());
+    return 0;
+  };
+  for (int64_t b = 0; b < nblk; b++) {
+    const int64_t r0 = b * NB, w = std::min<int64_t>(NB, n - r0);
+    MOM_TRY(diag(Linv + b * NB * NB, r0, (int)w));
+    if (r0 + w < n)
+      MOM_TRY(gemm_sub<float>(L, LU + r0 * ld + r0 + w, ld, T + r0, n, T + r0 + w, n, n - r0 - w, nrhs, w));
+  }
+  for (int64_t b = nblk - 1; b >= 0; b--) {
+    const int64_t r0 = b * NB, w = std::min<int64_t>(NB, n - r0);
+    MOM_TRY(diag(Uinv + b * NB * NB, r0, (int)w));
+    if (r0 > 0) MOM_TRY(gemm_sub<float>(L, LU + r0 * ld, ld, T + r0, n, T, n, r0, nrhs, w));
+  }
+  return 0;
+}
+
 template <typename T>
 static int diag_inverses(hipStream_t s, const typename Cx<T>::type *LU, int64_t ld, int64_t n, double2 *Linv,
                          double2 *Uinv) {
@@ -2457,6 +2550,7 @@ struct mom_system {
   int n_minv = 0, n_matvec = 0;  // preconditioner applications / FP64 products of the last refinement
   double be_hist[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // backward error after each outer step
   int coop_gmax[2] = {-1, -1};
+  bool coop_refused = false;  // a cooperative launch was refused: panels by one launch per column
 };
 
 namespace mom {
@@ -2499,6 +2593,7 @@ static int panel_coop(mom_system *S, PanelCoop *c) {
     S->coop_gmax[pi] = coop ? std::min(kPanelMaxGroups, nb * prop.multiProcessorCount) : 0;
   }
   c->gmax = g_panel_mode == 1 ? 0 : S->coop_gmax[pi];
+  c->refused = &S->coop_refused;
   if (c->gmax == 0) return 0;
   const size_t cand = sizeof(u64) * 2 * kPanelMaxGroups * kCandWords, diag = sizeof(u64) * 2 * kRowWords;
   if (!S->panel_sync.ptr) {
@@ -2741,32 +2836,28 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
   hipStream_t s = S->ctx->stream;
   const int64_t n = S->n;
   const Lane L = main_lane(S->ctx);
-  DevBuf dB, dX, dR, dT, dT2, dW, dP, dperm, dn, dkeep;
+  DevBuf dB, dX, dR, dT, dT2, dW, dW2, dT32, dperm, dn, dkeep;
   int rc;
   const size_t blk = sizeof(double2) * n * nrhs;
   if ((rc = dB.alloc(blk)) || (rc = dX.alloc(blk)) || (rc = dR.alloc(blk)) || (rc = dT.alloc(blk)) ||
       (rc = dT2.alloc(blk)) || (rc = dW.alloc(sizeof(double2) * NB * nrhs)) ||
-      (rc = dP.alloc(sizeof(double2) * n * NB)) || (rc = upload_perm(S, dperm)) ||
-      (rc = dn.alloc(sizeof(double) * 2 * nrhs)) || (rc = dkeep.alloc(sizeof(int) * nrhs)))
+      (rc = dW2.alloc(sizeof(double2) * NB * nrhs)) || (rc = dT32.alloc(sizeof(float2) * n * nrhs)) ||
+      (rc = upload_perm(S, dperm)) || (rc = dn.alloc(sizeof(double) * 2 * nrhs)) ||
+      (rc = dkeep.alloc(sizeof(int) * nrhs)))
     return rc;
   MOM_TRY(copy_h2d_2d(S->ctx, dB.ptr, n * sizeof(double2), B, ldb * sizeof(double2), n * sizeof(double2), nrhs));
   const dim3 g2((unsigned)((n + 255) / 256), (unsigned)nrhs);
   S->n_minv = S->n_matvec = 0;
   for (int i = 0; i < kBeHistory; i++) S->be_hist[i] = -1;
-  // dst (= or +=) C U^-1 L^-1 P R src
+  // X (= or +=) C U^-1 L^-1 P R src, the solves in complex64
   auto minv = [&](const double2 *src, bool add) -> int {
     S->n_minv++;
-    hipLaunchKernelGGL(dscale_cols_kernel, g2, dim3(256), 0, s, src, n, S->rscale.as<double>(), n, dT.as<double2>(),
-                       n);
-    MOM_TRY(getrs_blocked<float>(S->ctx, S->Z32.as<float2>(), n, n, dperm.as<int>(), S->Linv.as<double2>(),
-                                 S->Uinv.as<double2>(), dT.as<double2>(), n, nrhs, dT2.as<double2>(), dW.as<double2>(),
-                                 dP.as<double2>()));
-    if (add)
-      hipLaunchKernelGGL(scaled_add_cols_kernel, g2, dim3(256), 0, s, dX.as<double2>(), n, S->cscale.as<double>(),
-                         dT.as<double2>(), n, n);
-    else
-      hipLaunchKernelGGL(dscale_cols_kernel, g2, dim3(256), 0, s, dT.as<double2>(), n, S->cscale.as<double>(), n,
-                         dX.as<double2>(), n);
+    hipLaunchKernelGGL(scale_gather_round_kernel, g2, dim3(256), 0, s, src, n, S->rscale.as<double>(),
+                       dperm.as<int>(), n, dT32.as<float2>());
+    MOM_TRY(getrs_blocked32(S->ctx, S->Z32.as<float2>(), n, n, S->Linv.as<double2>(), S->Uinv.as<double2>(),
+                            dT32.as<float2>(), nrhs, dW.as<double2>(), dW2.as<double2>()));
+    hipLaunchKernelGGL(widen_scale_store_kernel, g2, dim3(256), 0, s, dX.as<double2>(), n, S->cscale.as<double>(),
+                       dT32.as<float2>(), n, add ? 1 : 0);
     MOM_HIP(hipGetLastError());
     return 0;
   };
