@@ -35,6 +35,16 @@ PEAK_FP64_MFMA = 78.6
 # WRITE_SIZE x 1.0, the factors read off a 1 GiB elementwise copy.  Recorded in
 # profiles/gemm3m_dma_kernel32_N61440_hbm_traffic.md; counters cannot be read in-process.
 TRAFFIC_S5_BYTES = 13.32e9
+PEAK_FP64_VALU = 78.6  # TFLOP/s, MI355X FP64 vector
+PEAK_HBM = 8000.0  # GB/s
+# Fill: SURVEY's algorithmic count of 60 flops per (test point, source point) pair.  The FP64 VALU
+# flops the pair kernel issues per point pair (SQ_INSTS_VALU_FLOPS_FP64 over its launches of one
+# N = 61440 fill divided by the point pairs they evaluate) and the HBM bytes of the assembly kernels
+# per fill (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0) are recorded in
+# profiles/pair_plane_kernel_N61440_fp64_counters.md.
+FILL_FLOPS_PER_POINT_PAIR = 60
+FILL_ISSUED_FLOPS_PER_POINT_PAIR = 159.8
+ASSEMBLY_TRAFFIC_S5_BYTES = 7.704e11
 
 
 def log(msg):
@@ -63,13 +73,15 @@ def run_path(ctx, dms, case, prec, profile=0, tol=1e-12):
     S = System(ctx, N, prec)
     if profile:
         capi.check(capi.lib().mom_system_profile(S.h, profile))
+        ctx.fill_profile(1)  # events around the fill's pair and assembly launches
     t0 = time.perf_counter()
     S.fill(dms, case["ri"], case["omega"])
+    fill_prof = ctx.fill_profile(0) if profile else None
     S.factor()
     x = S.solve_pw(dms[0], case["omega"], case["ri"][0], ang, tol=tol, maxit=30)[:, 0]
     wall = time.perf_counter() - t0
     st = S.stats()
-    st.update(wall_s=wall, iters=S.iters, berr=S.berr)
+    st.update(wall_s=wall, iters=S.iters, berr=S.berr, fill_prof=fill_prof)
     cs = cs_prtsrf(ctx, dms[0], N, case["omega"], case["ri"][0], x, *ang)
     # bistatic radar cross-section on the reference's 37 x 72 grid, outside the timed region
     st["rcs"] = rcs(ctx, dms[0], N, case["omega"], case["ri"][0], x, 37, 72)
@@ -329,27 +341,6 @@ def bench_spectrum(args):
     print(json.dumps(out), flush=True)
 
 
-def perturbed_sphere(s, radius, amp, seed=20251203):
-    """Icosphere s of `radius` [nm] with a smooth radial perturbation
-    dr(v) = sum_{l<=6, m} c_lm Y_lm(v) (real spherical harmonics, c_lm ~ N(0,1)
-    from `seed`), scaled so that max |dr| = amp [nm]."""
-    import scipy.special as sp
-    from mom2amd import meshgen
-
-    nodes, faces = meshgen.icosphere(s, 1.0)
-    th = np.arccos(np.clip(nodes[:, 2], -1, 1))
-    ph = np.arctan2(nodes[:, 1], nodes[:, 0])
-    rng = np.random.default_rng(seed)
-    dr = np.zeros(len(nodes))
-    for l in range(7):
-        for m in range(-l, l + 1):
-            y = sp.sph_harm_y(l, abs(m), th, ph)
-            yr = y.real if m == 0 else np.sqrt(2) * (y.real if m > 0 else y.imag)
-            dr += rng.standard_normal() * yr
-    dr *= amp / np.abs(dr).max()
-    return nodes * (radius + dr)[:, None], faces
-
-
 def bench_adjoint(args):
     """Adjoint sensitivity: sphere of radius 60 nm (icosphere s=4, 5120
     faces, N=15360) with a seeded radial perturbation (l <= 6, 3 nm), eps_in
@@ -358,12 +349,12 @@ def bench_adjoint(args):
     Z^H solve on the same factors and one derivative fill of the interior
     domain), in the mixed path; central differences of the FP64 solve check
     the gradient once."""
-    from mom2amd import (Context, DeviceMesh, System, cs_prtsrf, load_problem, meshgen, omega_of,
+    from mom2amd import (Context, DeviceMesh, System, build_problem, cs_prtsrf, load_problem, meshgen, omega_of,
                          sigma_abs_eps_gradient, sigma_abs_vertex_gradient)
     from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
 
     s = args.subdiv if args.subdiv != 5 else 4
-    nodes, faces = perturbed_sphere(s, 60.0, 3.0)
+    nodes, faces = meshgen.perturbed_sphere(s, 60.0, 3.0)
     d = tempfile.mkdtemp(prefix="mom_bench_")
     path = os.path.join(d, "pert%d.msh" % s)
     meshgen.write_gmsh(path, nodes, faces, 15)
@@ -426,34 +417,44 @@ def bench_adjoint(args):
                               "max_abs_m": float(np.abs(gv).max())}
     log("vertex gradient (%d node coordinates, one adjoint solve + pair kernel): %.3f s" % (gv.size, tv))
     if not args.no_fp64:
-        # central differences of the whole FP64 solve in the two largest components
-        flat = np.argsort(np.abs(gv).ravel())[::-1][:2]
-        fdv = []
-        for idx in flat:
-            v, c = divmod(int(idx), 3)
-            vals = []
-            for sgn in (1.0, -1.0):
-                pn = nodes.copy()
-                pn[v, c] += sgn * 1e-3
-                path2 = os.path.join(d, "pert_fd.msh")
-                meshgen.write_gmsh(path2, pn, faces, 15)
-                p2 = load_problem(path2, 1e-9, [[15], [-15]])
-                dms2 = [DeviceMesh(ctx, dm) for dm in p2.domains]
-                S2 = System(ctx, N, MOM_PREC_FP64)
-                try:
-                    S2.fill(dms2, [1.0 + 0j, np.sqrt(eps)], omega)
-                    S2.factor()
-                    x2 = S2.solve_pw(dms2[0], omega, 1.0, ang)[:, 0]
-                    vals.append(cs_prtsrf(ctx, dms2[0], N, omega, 1.0, x2, *ang)[1])
-                finally:
-                    S2.close()
-                    for dm in dms2:
-                        dm.close()
-            fd_v = (vals[0] - vals[1]) / (2e-3 * 1e-9)
-            fdv.append({"node": v, "coord": c, "adjoint": float(gv[v, c]), "fd": fd_v,
-                        "relerr": abs(gv[v, c] - fd_v) / abs(fd_v)})
-            log("vertex %d coord %d: adjoint %.8e fd %.8e" % (v, c, gv[v, c], fd_v))
-        out["vertex_gradient"]["fd_check"] = fdv
+        # central differences (h = 1e-3 nm) of the whole FP64 solve in all three coordinates of
+        # 32 seeded nodes, each perturbed mesh classified like the unperturbed one (the gradient
+        # holds the near/far split fixed)
+        t0 = time.perf_counter()
+        rng = np.random.default_rng(20251204)
+        verts = rng.choice(nodes.shape[0], 32, replace=False)
+        scale = float(np.abs(gv).max())
+        fdv, worst = [], 0.0
+        for v in verts:
+            for c in range(3):
+                vals = []
+                for sgn in (1.0, -1.0):
+                    pn = nodes.copy()
+                    pn[v, c] += sgn * 1e-3
+                    p2 = build_problem(pn, faces, np.full(faces.shape[0], 15), 1e-9, [[15], [-15]])
+                    dms2 = [DeviceMesh(ctx, dm) for dm in p2.domains]
+                    S2 = System(ctx, N, MOM_PREC_FP64)
+                    try:
+                        for dm2, dm in zip(dms2, dms):
+                            dm2.classify_like(dm)
+                        S2.fill(dms2, [1.0 + 0j, np.sqrt(eps)], omega)
+                        S2.factor()
+                        x2 = S2.solve_pw(dms2[0], omega, 1.0, ang)[:, 0]
+                        vals.append(cs_prtsrf(ctx, dms2[0], N, omega, 1.0, x2, *ang)[1])
+                    finally:
+                        S2.close()
+                        for dm in dms2:
+                            dm.close()
+                fd_v = (vals[0] - vals[1]) / (2e-3 * 1e-9)
+                err = abs(gv[v, c] - fd_v) / scale
+                worst = max(worst, err)
+                fdv.append([int(v), c, float(gv[v, c]), fd_v])
+            log("node %d: adjoint %s fd %s" % (v, gv[v], np.array([f[3] for f in fdv[-3:]])))
+        out["vertex_gradient"]["fd_check"] = {
+            "nodes": 32, "coordinates": 96, "h_nm": 1e-3, "classification": "frozen to the unperturbed mesh",
+            "max_abs_err_over_max_grad": worst, "seconds": round(time.perf_counter() - t0, 1),
+            "node_coord_adjoint_fd": fdv}
+        log("vertex gradient vs central differences (32 nodes x 3): max |adjoint - fd| / max |grad| = %.2e" % worst)
         fd = []
         for hdir in (1.0, 1j):
             h = 1e-4 * abs(eps) * hdir
@@ -565,6 +566,76 @@ def bench_symmetric(args):
     print(json.dumps(out), flush=True)
 
 
+def roofline_fill(fp, steps, n):
+    """Roofline block of the fill's pair kernel (FP64 VALU) and its assembly
+    passes (HBM), from the event times of the timed steps."""
+    if not fp or fp.get("pair_launches", 0) == 0:
+        return None
+    pair_s = fp["pair_ms"] / 1e3
+    points = fp["pair_points"]
+    ach = FILL_FLOPS_PER_POINT_PAIR * points / pair_s / 1e12
+    out = {"kernel": "pair_plane_kernel<13> (far and smoothed near face pairs of one slab of 1024 source faces "
+                     "against the test faces m >= the slab start; each unordered pair once); achieved = 60 "
+                     "algorithmic flops per evaluated (test point, source point) pair / kernel time (HIP events "
+                     "around each launch)",
+           "bound": "valu-fp64", "achieved": round(ach, 3), "peak": PEAK_FP64_VALU, "unit": "TFLOP/s",
+           "frac": round(ach / PEAK_FP64_VALU, 4), "launches": int(fp["pair_launches"]),
+           "avg_launch_ms": round(fp["pair_ms"] / fp["pair_launches"], 4),
+           "point_pairs_per_solve": points / steps,
+           "pair_kernel_s_per_solve": round(pair_s / steps, 4)}
+    if FILL_ISSUED_FLOPS_PER_POINT_PAIR:
+        iss = FILL_ISSUED_FLOPS_PER_POINT_PAIR * points / pair_s / 1e12
+        out["issued_fp64_flops_per_point_pair"] = FILL_ISSUED_FLOPS_PER_POINT_PAIR
+        out["issued_tflops"] = round(iss, 3)
+        out["issued_frac"] = round(iss / PEAK_FP64_VALU, 4)
+    asm_s = fp["assemble_ms"] / 1e3 / steps
+    floor = 16.0 * n * n  # complex128 Z written once
+    out["assembly"] = {"kernels": "assemble_kernel + assemble_transposed_kernel (face-side planes of a slab "
+                                  "gathered into Z, one read-modify-write per entry and slab)",
+                       "s_per_solve": round(asm_s, 4), "launch_groups_per_solve": int(fp["assemble_launches"] / steps),
+                       "write_once_floor_bytes": floor,
+                       "floor_s_at_peak": round(floor / (PEAK_HBM * 1e9), 5)}
+    if ASSEMBLY_TRAFFIC_S5_BYTES and n == 61440:
+        out["assembly"]["traffic_bytes_per_solve"] = ASSEMBLY_TRAFFIC_S5_BYTES
+        out["assembly"]["achieved_GBs"] = round(ASSEMBLY_TRAFFIC_S5_BYTES / asm_s / 1e9, 1)
+        out["assembly"]["traffic_over_floor"] = round(ASSEMBLY_TRAFFIC_S5_BYTES / floor, 2)
+    out["near_part_s_per_solve"] = round(fp["near_ms"] / 1e3 / steps, 4)
+    return out
+
+
+def bench_fill(args):
+    """The complex128 PMCHWT fill of the sphere workload alone (no LU, no
+    solve), timed per step, with the pair kernel's and assembly's event
+    times: the workload for counter runs of the fill kernels."""
+    from mom2amd import Context, DeviceMesh, System
+    from mom2amd.capi import MOM_PREC_FP64
+
+    case = build_case(args.subdiv)
+    N = case["problem"].nedges
+    log("fill workload s=%d: N=%d unknowns" % (args.subdiv, 2 * N))
+    ctx = Context(0)
+    dms = [DeviceMesh(ctx, d) for d in case["problem"].domains]
+    S = System(ctx, N, MOM_PREC_FP64)
+    for _ in range(args.warmup):
+        S.fill(dms, case["ri"], case["omega"])
+    ctx.fill_profile(1)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        S.fill(dms, case["ri"], case["omega"])
+        log("fill step %d done" % i)
+    wall = (time.perf_counter() - t0) / args.steps
+    fp = ctx.fill_profile(0)
+    S.close()
+    ctx.close()
+    print(json.dumps({"metric": "PMCHWT fill s @N=%d" % (2 * N), "value": round(wall, 4), "unit": "s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 2),
+                      "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+                      "data": "synthetic icosphere mesh",
+                      "config": {"workload": "complex128 PMCHWT fill, sphere R=50nm icosphere s=%d, N=%d, 600 nm"
+                                             % (args.subdiv, 2 * N), "n_unknowns": 2 * N},
+                      "roofline_fill": roofline_fill(fp, args.steps, 2 * N)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -573,11 +644,13 @@ def main():
     ap.add_argument("--subdiv", type=int, default=5, help="icosphere subdivision (5: N = 61440)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the all-FP64 comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--workload", choices=("pmchwt", "sh", "spectrum", "adjoint", "symmetric"), default="pmchwt",
+    ap.add_argument("--workload", choices=("pmchwt", "sh", "spectrum", "adjoint", "symmetric", "fill"),
+                    default="pmchwt",
                     help="pmchwt: the sphere solve (default); sh: the second-harmonic dimer chain; spectrum: the "
                          "ellipsoid over many wavelengths with 256 plane waves each; adjoint: sigma_abs and its "
                          "permittivity gradient on a perturbed sphere (sh, spectrum and adjoint use s=4 unless "
-                         "--subdiv is given); symmetric: the octant of an octasphere by mirror symmetry")
+                         "--subdiv is given); symmetric: the octant of an octasphere by mirror symmetry; fill: the "
+                         "complex128 PMCHWT fill of the default sphere alone (for counter runs)")
     ap.add_argument("--cell-n", type=int, default=50, help="octasphere divisions of the symmetric workload")
     ap.add_argument("--nwl", type=int, default=64, help="wavelengths of the spectrum workload")
     args = ap.parse_args()
@@ -595,6 +668,9 @@ def main():
         return
     if args.workload == "symmetric":
         bench_symmetric(args)
+        return
+    if args.workload == "fill":
+        bench_fill(args)
         return
 
     from mom2amd import Context, DeviceMesh
@@ -614,6 +690,7 @@ def main():
         log("warmup %d: %.2f s (fill %.2f factor %.2f solve %.2f)" % (i, st["wall_s"], st["fill_s"], st["factor_s"],
                                                                      st["solve_s"]))
     times, gemm_s, gemm_flops, gemm_launches, last = [], 0.0, 0.0, 0, None
+    fprof = {}
     for i in range(args.steps):
         # timed steps: HIP events around the depth-256 trailing updates only
         st, cs_mixed = run_path(ctx, dms, case, MOM_PREC_MIXED, profile=1)
@@ -621,6 +698,8 @@ def main():
         gemm_s += st["gemm_s"]
         gemm_flops += st["gemm_flops"]
         gemm_launches += int(st["gemm_launches"])
+        for k, v in st["fill_prof"].items():
+            fprof[k] = fprof.get(k, 0.0) + v
         last = st
         log("step %d: %.2f s (fill %.2f factor %.2f solve %.2f, IR iters %d (%d factor solves, %d FP64 products), "
             "berr %s); trailing-update gemm %.2f"
@@ -680,6 +759,7 @@ def main():
             "mfma_issued_frac": round(achieved * 6 / 8 / PEAK_FP32_MFMA, 4),
             "launches": gemm_launches, "avg_launch_ms": round(gemm_s / max(gemm_launches, 1) * 1e3, 4)},
     }
+    out["roofline_fill"] = roofline_fill(fprof, args.steps, 2 * N)
     if fp64 is not None:
         ach64 = fp64["gemm_flops"] / fp64["gemm_s"] / 1e12 if fp64["gemm_s"] > 0 else None
         out["fp64_path"] = {"value_s": round(fp64["wall_s"], 4), "fill": round(fp64["fill_s"], 4),

@@ -255,6 +255,17 @@ int mom_ctx_fill_tuning(const mom_ctx *ctx, int32_t *spw, double *ms) {
   return 0;
 }
 
+int mom_ctx_fill_profile(mom_ctx *ctx, int32_t enable, double *out) {
+  if (!ctx) return fail(MOM_ERR_ARG, "NULL argument");
+  if (out)
+    for (int c = 0; c < 6; c++) out[c] = ctx->fill_prof[c];
+  if (enable >= 0) {
+    ctx->fill_profile = enable ? 1 : 0;
+    for (int c = 0; c < 6; c++) ctx->fill_prof[c] = 0;
+  }
+  return 0;
+}
+
 int mom_ctx_destroy(mom_ctx *ctx) {
   if (!ctx) return 0;
   (void)hipSetDevice(ctx->device);
@@ -411,6 +422,35 @@ int mom_mesh_destroy(mom_mesh *mesh) {
 }
 
 int64_t mom_mesh_near_pairs(const mom_mesh *mesh) { return mesh ? mesh->nnear : -1; }
+
+int mom_mesh_classify_like(mom_mesh *mesh, const mom_mesh *from) {
+  if (!mesh || !from) return fail(MOM_ERR_ARG, "NULL argument");
+  if (mesh->ctx != from->ctx) return fail(MOM_ERR_ARG, "classify_like: the meshes belong to different contexts");
+  if (mesh->F != from->F) return fail(MOM_ERR_ARG, "classify_like: the meshes have different face counts");
+  mom_ctx *ctx = mesh->ctx;
+  const int64_t F = mesh->F;
+  (void)hipSetDevice(ctx->device);
+  const std::vector<double> cls = from->h_cls();  // a copy: mesh may be from
+  const double avelen = from->avelen;
+  std::vector<int32_t> np;
+  near_pairs(cls, F, avelen, np);
+  std::vector<int64_t> noff;
+  near_offsets(np, F, noff);
+  DevBuf dcls, dnear, doff;
+  int rc = 0;
+  if ((rc = upload(ctx, dcls, cls.data(), cls.size())) || (rc = upload(ctx, dnear, np.data(), np.size())) ||
+      (rc = upload(ctx, doff, noff.data(), noff.size())))
+    return rc;
+  // fills queued on the stream may still read the old lists
+  MOM_HIP(hipStreamSynchronize(ctx->stream));
+  mesh->ccls.swap(dcls);
+  mesh->near.swap(dnear);
+  mesh->near_off.swap(doff);
+  mesh->h_ccls = cls;
+  mesh->avelen = avelen;
+  mesh->nnear = (int64_t)np.size() / 2;
+  return 0;
+}
 
 int mom_compute_operator(mom_ctx *ctx, const mom_mesh *mesh, char op, double omega, double ri_re,
                          double ri_im, int32_t nq, const double *qw, const double *qnodes,

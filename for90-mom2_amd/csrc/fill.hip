@@ -50,6 +50,7 @@ struct FarArgs {
   double w[kMaxQ];
   const double *qp;   // [F][Q][3] absolute quadrature points
   const double *cp;   // [F][3]
+  const double *ccls; // [F][3] centroids that classify near pairs (cp unless frozen to another mesh)
   const double *area; // [F]
   FaceSide side;      // per face side (3F)
   double kr, ki;      // wavenumber
@@ -371,7 +372,8 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
     // acc[2..7] = S_rho' (source), acc[8..13] = S_rho (test),
     // acc[18..23] = M_rho', acc[24..29] = M_rho, acc[30..35] = -M_x.
     const double dx = cmx - cpn[0], dy = cmy - cpn[1], dz = cmz - cpn[2];
-    const bool near = near_pair(cm3, cpn, a.thr2);
+    const double clm[3] = {a.js[0] * a.ccls[3 * mm], a.js[1] * a.ccls[3 * mm + 1], a.js[2] * a.ccls[3 * mm + 2]};
+    const bool near = near_pair(clm, a.ccls + 3 * n, a.thr2);
     double acc[36];
     for (int k = 0; k < 36; k++) acc[k] = 0.0;
     if (near)
@@ -980,6 +982,58 @@ static FaceSide sides(const mom_mesh *m, bool global) {
   return s;
 }
 
+// Event pairs around the fill's launches while ctx->fill_profile is on;
+// harvest() (after the stream has drained) adds their times to
+// ctx->fill_prof.  Kind 0: pair kernel, 1: assembly, 2: near part.
+struct FillTimer {
+  mom_ctx *ctx;
+  struct Rec {
+    hipEvent_t e0, e1;
+    int kind;
+    double points;
+  };
+  std::vector<Rec> recs;
+  explicit FillTimer(mom_ctx *c) : ctx(c) {}
+  hipEvent_t begin() {
+    if (!ctx->fill_profile) return nullptr;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    (void)hipEventRecord(e, ctx->stream);
+    return e;
+  }
+  void end(hipEvent_t e0, int kind, double points) {
+    if (!e0) return;
+    hipEvent_t e1 = nullptr;
+    if (hipEventCreate(&e1) != hipSuccess) {
+      (void)hipEventDestroy(e0);
+      return;
+    }
+    (void)hipEventRecord(e1, ctx->stream);
+    recs.push_back({e0, e1, kind, points});
+  }
+  void harvest() {
+    for (auto &r : recs) {
+      float ms = 0;
+      if (hipEventSynchronize(r.e1) == hipSuccess && hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
+        if (r.kind == 0) {
+          ctx->fill_prof[0] += ms;
+          ctx->fill_prof[1] += 1;
+          ctx->fill_prof[2] += r.points;
+        } else if (r.kind == 1) {
+          ctx->fill_prof[3] += ms;
+          ctx->fill_prof[4] += 1;
+        } else {
+          ctx->fill_prof[5] += ms;
+        }
+      }
+      (void)hipEventDestroy(r.e0);
+      (void)hipEventDestroy(r.e1);
+    }
+    recs.clear();
+  }
+  ~FillTimer() { harvest(); }
+};
+
 int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
                 const FillTarget &t, const Image *img) {
   int rc = compute_quad_points(ctx, m, rule);
@@ -995,6 +1049,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   for (int i = 0; i < kMaxQ; i++) a.w[i] = i < rule.Q ? rule.w[i] : 0.0;
   a.qp = ctx->qp.as<double>();
   a.cp = m->fcp.as<double>();
+  a.ccls = m->cls();
   a.area = m->farea.as<double>();
   const bool glob = t.mode == 0 && !t.local;  // global unknown numbering
   a.side = sides(m, glob);
@@ -1058,6 +1113,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   ga.ktr = img ? img->detj : 1.0;
   ga.out = t;
   const int nslab = (int)((m->F + S - 1) / S);
+  FillTimer timer(ctx);
   for (int k = 0; k < nslab; k++) {
     pa.n0 = k * S;
     pa.mlo = pa.n0;  // pairs with both faces below n0 were integrated by earlier slabs
@@ -1100,7 +1156,9 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
         MOM_HIP(hipEventDestroy(e0));
         MOM_HIP(hipEventDestroy(e1));
       }
+      hipEvent_t ev = timer.begin();
       launch13(ctx->fill_spw ? ctx->fill_spw : 4);
+      timer.end(ev, 0, (double)(m->F - pa.mlo) * scur * rule.Q * rule.Q);
     } else {
       hipLaunchKernelGGL(pair_plane_kernel<0>, grid_of(4), dim3(256), 0, ctx->stream, pa);
     }
@@ -1110,6 +1168,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
       ga.n0 = pa.n0;
       ga.mlo = pa.mlo;
       ga.slab_edges = m->slab_edges.as<int>() + m->h_slab_ptr[k];
+      hipEvent_t ev = timer.begin();
       hipLaunchKernelGGL(assemble_kernel, dim3((unsigned)((nrows + 255) / 256), (unsigned)ne), dim3(256), 0,
                          ctx->stream, ga);
       MOM_HIP(hipGetLastError());
@@ -1122,10 +1181,12 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
                            dim3(256), 0, ctx->stream, ga, ddesc.as<int>(), ne, nb);
         MOM_HIP(hipGetLastError());
       }
+      timer.end(ev, 1, 0.0);
     }
   }
   // the row/column maps must outlive the queued kernels
   MOM_HIP(hipStreamSynchronize(ctx->stream));
+  timer.harvest();
 
   // near pairs of the image: the reflected test faces against the sources
   DevBuf inear, ioff;
@@ -1134,7 +1195,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   const int64_t *off = m->near_off.as<int64_t>();
   if (img) {
     std::vector<int32_t> np;
-    near_pairs(m->h_fcp, m->F, m->avelen, np, img->js);
+    near_pairs(m->h_cls(), m->F, m->avelen, np, img->js);
     nnear = (int64_t)np.size() / 2;
     if (nnear > 0) {
       std::vector<int64_t> ho;
@@ -1175,6 +1236,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     for (int c = 0; c < 3; c++) b.js[c] = a.js[c];
     b.image = a.image;
     b.nb = ctx->scratch.as<double2>();
+    hipEvent_t ev = timer.begin();
     hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((nnear + 127) / 128)), dim3(128), 0,
                        ctx->stream, b);
     MOM_HIP(hipGetLastError());
@@ -1190,9 +1252,11 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     e.out = t;
     hipLaunchKernelGGL(near_emit_kernel, dim3((unsigned)((9 * nnear + 255) / 256)), dim3(256), 0, ctx->stream, e);
     MOM_HIP(hipGetLastError());
+    timer.end(ev, 2, 0.0);
     // the image's pair list must outlive the kernels
     if (img) MOM_HIP(hipStreamSynchronize(ctx->stream));
   }
+  if (!timer.recs.empty()) timer.harvest();
   return 0;
 }
 
@@ -2548,7 +2612,7 @@ int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
   g.p = m->p.as<double>();
   g.fnodes = m->fnodes.as<int>();
   g.side = sides(m, true);
-  g.cp = m->fcp.as<double>();
+  g.cp = m->cls();
   const double thr = m->avelen * 3;
   g.thr2 = thr * thr;
   g.kr = k.real();

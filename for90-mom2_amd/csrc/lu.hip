@@ -955,6 +955,7 @@ struct PivotWork {
   int mb;        // slots per parity (>= blocks of the widest launch)
   int *info;     // first zero pivot (1-based), 0 if none
   int *plan;     // laswp plan of one panel: 1 + 4 NB ints
+  int *tsel;     // tournament candidate sets: [2][mb][IB] row indices
 };
 
 
@@ -1675,6 +1676,235 @@ __global__ __launch_bounds__(256) void unit_lower_inverse_kernel(const float2 *_
   }
 }
 
+// ---------------------------------------------------------------------------
+// Tournament pivoting for the complex64 panels (communication-avoiding LU,
+// Grigori, Demmel and Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)): the
+// pivot rows of an inner block of ib <= 32 columns are chosen by partial
+// pivoting inside workgroups and a reduction tree over their choices, so
+// that the block needs a few launches instead of one grid-wide exchange per
+// column.  The rows are then moved to the top and the block is factored
+// without further pivoting (L21 = A21 inv(U11)).  The pivots differ from
+// LAPACK's but are as good in practice; the complex64 factors only
+// precondition the complex128 refinement, whose result does not depend on
+// them beyond rounding.  The complex128 path keeps LAPACK's partial pivoting.
+// Off by default (mom_test_tournament switches it on): on the 15360-unknown
+// ellipsoid its panels took longer than the cooperative partial-pivoting
+// panel (one wave runs each 32-step selection serially, ~85 us per level).
+//
+// Selection, one wave per group of 128 rows (two per lane, registers
+// only, no barriers): level 0 takes 128 consecutive rows of the block, a
+// later level up to four candidate sets of ib rows (their original values
+// read again from A).  Partial pivoting over the group -- pivot search by a
+// wave argmax of |re| + |im| with ties to the lowest row as IZAMAX, pivot
+// row broadcast with readlane -- writes the ib rows chosen, in pivot order,
+// as the group's candidate set (-1 where the group had fewer rows).  A NaN
+// magnitude counts as 0, so such rows only win columns without any other
+// nonzero candidate.
+constexpr int kTsluRows = 128;                // rows per wave
+constexpr int kTsluSets = kTsluRows / IB;     // candidate sets merged per wave
+
+__device__ __forceinline__ float2 cdiv_f(float2 a, float2 b) {
+  if (b.x == 0.f && b.y == 0.f) return a;
+  const float den = b.x * b.x + b.y * b.y;
+  return make_float2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
+}
+__global__ __launch_bounds__(256) void tslu_select_kernel(const float2 *__restrict__ A, int64_t ld, int i0, int ib,
+                                                          int n, const int *__restrict__ in, int nsets, int ngroups,
+                                                          int *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int grp = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (grp >= ngroups) return;
+  int row[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    row[h] = -1;
+    const int slot = lane + 64 * h;
+    if (!in) {
+      const int r = i0 + kTsluRows * grp + slot;
+      if (r < n) row[h] = r;
+    } else {
+      const int set = grp * kTsluSets + slot / IB;
+      if (set < nsets && slot % IB < ib) row[h] = in[set * IB + slot % IB];
+    }
+  }
+  float2 v0[IB], v1[IB];  // the lane's two rows
+#pragma unroll
+  for (int s = 0; s < IB; s++) {
+    v0[s] = (row[0] >= 0 && s < ib) ? A[(int64_t)(i0 + s) * ld + row[0]] : make_float2(0.f, 0.f);
+    v1[s] = (row[1] >= 0 && s < ib) ? A[(int64_t)(i0 + s) * ld + row[1]] : make_float2(0.f, 0.f);
+  }
+  bool act0 = row[0] >= 0, act1 = row[1] >= 0, done = false;
+#pragma clang loop unroll(full)
+  for (int k = 0; k < IB; k++) {
+    if (k >= ib || done) continue;
+    // this lane's best row, then the wave's
+    float val = -1.f;
+    int idx = 0x7fffffff;
+    {
+      float c0 = fabsf(v0[k].x) + fabsf(v0[k].y), c1 = fabsf(v1[k].x) + fabsf(v1[k].y);
+      if (!(c0 == c0)) c0 = 0.f;
+      if (!(c1 == c1)) c1 = 0.f;
+      if (act0) {
+        val = c0;
+        idx = row[0];
+      }
+      if (act1 && (c1 > val || (c1 == val && row[1] < idx))) {
+        val = c1;
+        idx = row[1];
+      }
+    }
+    float bv = val;
+    int bi = idx;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float v2 = __shfl_xor(bv, o);
+      const int i2 = __shfl_xor(bi, o);
+      if (v2 > bv || (v2 == bv && i2 < bi)) {
+        bv = v2;
+        bi = i2;
+      }
+    }
+    if (!(bv >= 0.f)) {  // no rows left in this group
+      if (lane == 0)
+        for (int q = k; q < ib; q++) out[grp * IB + q] = -1;
+      done = true;
+      continue;
+    }
+    const bool m0 = act0 && row[0] == bi, m1 = act1 && row[1] == bi;
+    const int wl = __builtin_ffsll((long long)__ballot(m0 || m1)) - 1;  // lane holding the pivot row
+    const int wh = __builtin_amdgcn_readlane(m1 ? 1 : 0, wl);
+    if (lane == 0) out[grp * IB + k] = bi;
+    if (m0) act0 = false;
+    if (m1) act1 = false;
+    float2 prow[IB];
+#pragma unroll
+    for (int s = 0; s < IB; s++) {
+      if (s < k || s >= ib) continue;
+      // both rows read out, then the uniform choice
+      const float2 p0 = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v0[s].x), wl)),
+                                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v0[s].y), wl)));
+      const float2 p1 = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v1[s].x), wl)),
+                                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v1[s].y), wl)));
+      prow[s] = wh ? p1 : p0;
+    }
+    const float2 l0 = cdiv_f(v0[k], prow[k]), l1 = cdiv_f(v1[k], prow[k]);
+#pragma unroll
+    for (int s = 0; s < IB; s++) {
+      if (s <= k || s >= ib) continue;
+      if (act0) {
+        v0[s].x -= l0.x * prow[s].x - l0.y * prow[s].y;
+        v0[s].y -= l0.x * prow[s].y + l0.y * prow[s].x;
+      }
+      if (act1) {
+        v1[s].x -= l1.x * prow[s].x - l1.y * prow[s].y;
+        v1[s].y -= l1.x * prow[s].y + l1.y * prow[s].x;
+      }
+    }
+  }
+}
+
+// LAPACK-style interchanges of the chosen rows sel[0..ib) into positions
+// i0..i0+ib-1 (one wave; lane s keeps slot s of the positions moved so far
+// and their rows): ipiv[i0 + k] = 1 + the position row sel[k] occupies after
+// the interchanges of steps < k.  A row below the block is still at its own
+// position until it is chosen; rows of the block may have moved.
+__global__ __launch_bounds__(64) void tslu_pivots_kernel(const int *__restrict__ sel, int i0, int ib, int *ipiv) {
+  const int lane = threadIdx.x;
+  int mpos = -1, mrow = -1, np = 0;
+  for (int k = 0; k < ib; k++) {
+    const int r = sel[k];
+    const unsigned long long br = __ballot(lane < np && mrow == r);
+    const int q = br ? __shfl(mpos, __builtin_ffsll((long long)br) - 1) : r;
+    if (lane == 0) ipiv[i0 + k] = q + 1;
+    if (q == i0 + k) continue;  // already in place
+    const int p = i0 + k;
+    const unsigned long long bp = __ballot(lane < np && mpos == p);
+    const int a = bp ? __shfl(mrow, __builtin_ffsll((long long)bp) - 1) : p;  // row now at position p
+    const unsigned long long bq = __ballot(lane < np && mpos == q);
+    int sp = bp ? __builtin_ffsll((long long)bp) - 1 : np++;
+    if (lane == sp) {
+      mpos = p;
+      mrow = r;
+    }
+    int sq = bq ? __builtin_ffsll((long long)bq) - 1 : np++;
+    if (lane == sq) {
+      mpos = q;
+      mrow = a;
+    }
+  }
+}
+
+// The inner block after its interchanges, factored without pivoting, one
+// wave per 64 rows: lanes 0..ib-1 form L11 U11 of the top ib x ib block in
+// registers (pivot rows broadcast with readlane) and every lane's row below
+// the block solves l U11 = a_row in place (top = 0); a second launch of one
+// wave (top = 1) stores L11 U11 over the top block once every row has read
+// it.  A zero pivot is recorded in info (first one) and skipped, as the
+// partial pivoting panel does.
+__global__ __launch_bounds__(256) void tslu_factor_kernel(float2 *A, int64_t ld, int i0, int ib, int n, int *info,
+                                                          int top) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  float2 u[IB];
+#pragma unroll
+  for (int s = 0; s < IB; s++) u[s] = (lane < ib && s < ib) ? A[(int64_t)(i0 + s) * ld + i0 + lane] : make_float2(0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < IB; k++) {
+    if (k >= ib) continue;
+    float2 pr[IB];
+#pragma unroll
+    for (int s = 0; s < IB; s++)
+      pr[s] = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].x), k)),
+                          __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].y), k)));
+    if (top && lane == 0 && pr[k].x == 0.f && pr[k].y == 0.f) atomicCAS(info, 0, i0 + k + 1);
+    if (lane > k && lane < ib) {
+      const float2 l = cdiv_f(u[k], pr[k]);
+      u[k] = l;
+#pragma unroll
+      for (int s = 0; s < IB; s++) {
+        if (s <= k || s >= ib) continue;
+        u[s].x -= l.x * pr[s].x - l.y * pr[s].y;
+        u[s].y -= l.x * pr[s].y + l.y * pr[s].x;
+      }
+    }
+  }
+  if (top) {
+    if (blockIdx.x == 0 && threadIdx.x < 64 && lane < ib)
+#pragma unroll
+      for (int s = 0; s < IB; s++)
+        if (s < ib) A[(int64_t)(i0 + s) * ld + i0 + lane] = u[s];
+    return;
+  }
+  const int64_t row = (int64_t)i0 + ib + 64 * wave + lane;
+  if (row >= n) return;
+  float2 a[IB];
+#pragma unroll
+  for (int s = 0; s < IB; s++) a[s] = s < ib ? A[(int64_t)(i0 + s) * ld + row] : make_float2(0.f, 0.f);
+#pragma unroll
+  for (int s = 0; s < IB; s++) {
+    if (s >= ib) continue;
+    float2 x = a[s];
+#pragma unroll
+    for (int t = 0; t < IB; t++) {
+      if (t >= s) continue;
+      const float2 ut = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].x), t)),
+                                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].y), t)));
+      x.x -= a[t].x * ut.x - a[t].y * ut.y;
+      x.y -= a[t].x * ut.y + a[t].y * ut.x;
+    }
+    const float2 d = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].x), s)),
+                                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].y), s)));
+    a[s] = cdiv_f(x, d);
+  }
+#pragma unroll
+  for (int s = 0; s < IB; s++)
+    if (s < ib) A[(int64_t)(i0 + s) * ld + row] = a[s];
+}
+
+// complex64 panels by partial pivoting as the complex128 ones (default) or,
+// switched on by the test hook, by tournament pivoting
+static int g_tournament = 0;
+
 // Cooperative inner-block panel factorisation: synchronisation words, the
 // largest co-resident grid of panel_block_kernel<T> (0: not available) and
 // the next free round tag.
@@ -1701,8 +1931,37 @@ static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int6
     const int64_t ib = std::min<int64_t>(IB, j0 + jb - i0);
     Timed tp(st, s, kPhPanel);
     const int64_t G = (n - i0 + 255) / 256;
-    bool coop_done = false;
-    if (G <= coop.gmax && !(coop.refused && *coop.refused)) {
+    bool coop_done = false, tournament = false;
+    if (std::is_same<T, float>::value && g_tournament) {
+      // tournament over the rows, interchanges on every panel column, then
+      // the block without pivoting
+      int *cur = w.tsel, *nxt = w.tsel + (size_t)w.mb * IB;
+      int64_t groups = (n - i0 + kTsluRows - 1) / kTsluRows;
+      if constexpr (std::is_same<T, float>::value) {
+        hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, A, ld, (int)i0,
+                           (int)ib, (int)n, (const int *)nullptr, 0, (int)groups, cur);
+        while (groups > 1) {
+          const int64_t gn = (groups + kTsluSets - 1) / kTsluSets;
+          hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)((gn + 3) / 4)), dim3(256), 0, s, A, ld, (int)i0,
+                             (int)ib, (int)n, (const int *)cur, (int)groups, (int)gn, nxt);
+          std::swap(cur, nxt);
+          groups = gn;
+        }
+        hipLaunchKernelGGL(tslu_pivots_kernel, dim3(1), dim3(64), 0, s, cur, (int)i0, (int)ib, ipiv);
+        hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), (int)n,
+                           w.plan);
+        const int64_t lg = (jb + kLaswpCols - 1) / kLaswpCols;
+        hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)lg), dim3(256), 0, s, A, ld, w.plan, (int)j0,
+                           (int)(j0 + jb), 0, 0);
+        const int64_t below = n - i0 - ib;
+        if (below > 0)
+          hipLaunchKernelGGL(tslu_factor_kernel, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, s, A, ld,
+                             (int)i0, (int)ib, (int)n, w.info, 0);
+        hipLaunchKernelGGL(tslu_factor_kernel, dim3(1), dim3(64), 0, s, A, ld, (int)i0, (int)ib, (int)n, w.info, 1);
+      }
+      MOM_HIP(hipGetLastError());
+      tournament = true;
+    } else if (G <= coop.gmax && !(coop.refused && *coop.refused)) {
       // the whole inner block in one cooperative launch, then its row
       // interchanges on the other columns of the panel
       unsigned tag0 = *coop.tag;
@@ -1725,7 +1984,9 @@ static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int6
         return fail(MOM_ERR_HIP, std::string("hipLaunchCooperativeKernel(panel): ") + hipGetErrorString(e));
       }
     }
-    if (coop_done) {
+    if (tournament) {
+      // interchanges and factors done above
+    } else if (coop_done) {
       const int64_t left = i0 - j0, right = j0 + jb - (i0 + ib);
       if (left + right > 0) {
         hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), (int)n, w.plan);
@@ -2558,7 +2819,9 @@ namespace mom {
 static int ensure_pivwork(mom_system *S, PivotWork *w) {
   const int64_t mb = (S->n + 255) / 256 + 1;
   const size_t rowbytes = 2 * mb * IB * sizeof(double2);
-  const size_t bytes = rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64 + (1 + 4 * NB) * sizeof(int);
+  const size_t planb = ((1 + 4 * NB) * sizeof(int) + 255) / 256 * 256;
+  const size_t bytes =
+      rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64 + planb + 2 * mb * IB * sizeof(int);
   if (S->pivwork.bytes < bytes) {
     int rc = S->pivwork.alloc(bytes);
     if (rc) return rc;
@@ -2569,6 +2832,7 @@ static int ensure_pivwork(mom_system *S, PivotWork *w) {
   w->idx = (int *)(base + rowbytes + 2 * mb * sizeof(double));
   w->info = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)));
   w->plan = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64);
+  w->tsel = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64 + planb);
   w->mb = (int)mb;
   MOM_HIP(hipMemsetAsync(w->info, 0, sizeof(int), S->ctx->stream));
   return 0;
@@ -3391,6 +3655,11 @@ extern "C" int mom_test_panel_mode(int32_t mode) {
 extern "C" int mom_test_u_solve(int32_t mode) {
   if (mode != 0 && mode != 1) return fail(MOM_ERR_ARG, "U-block mode must be 0 or 1");
   g_u_subst = mode;
+  return 0;
+}
+
+extern "C" int mom_test_tournament(int32_t enable) {
+  g_tournament = enable ? 1 : 0;
   return 0;
 }
 

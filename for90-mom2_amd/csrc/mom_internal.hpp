@@ -52,6 +52,10 @@ struct DevBuf {
   int alloc(size_t n);
   void release();
   template <class T> T *as() const { return static_cast<T *>(ptr); }
+  void swap(DevBuf &o) {
+    std::swap(ptr, o.ptr);
+    std::swap(bytes, o.bytes);
+  }
 };
 
 // Host <-> device copies on the context's stream through two pinned staging
@@ -88,6 +92,13 @@ struct mom_ctx {
   // the candidates on the first slab of the first fill (0: not yet tuned)
   int fill_spw = 0;
   double fill_spw_ms[3] = {0, 0, 0};  // measured slab times of the candidates 2, 4, 8
+  // event timing of the fill kernels (mom_ctx_fill_profile): when on, every
+  // pair-kernel and assembly launch is bracketed by events and the totals
+  // below accumulate: [0] pair ms, [1] pair launches, [2] pair point pairs
+  // (test x source quadrature points), [3] assembly ms, [4] assembly
+  // launches, [5] near-part ms
+  int fill_profile = 0;
+  double fill_prof[6] = {0, 0, 0, 0, 0, 0};
 };
 
 // Device copy of one domain mesh, with per-face-side RWG data precomputed.
@@ -101,6 +112,12 @@ struct mom_mesh {
   int64_t max_parent = -1;  // largest global index among the mesh's edges
   std::vector<double> h_fcp;  // host centroids (near list)
   mom::DevBuf p, fnodes, fn, fm, fs, farea, fcp;
+  // near/far classification by another mesh's centroids (and avelen, which
+  // only classifies), set by mom_mesh_classify_like; empty: its own
+  std::vector<double> h_ccls;
+  mom::DevBuf ccls;
+  const std::vector<double> &h_cls() const { return h_ccls.empty() ? h_fcp : h_ccls; }
+  const double *cls() const { return ccls.ptr ? ccls.as<double>() : fcp.as<double>(); }
   // per face side s = 3*f + e:
   mom::DevBuf s_g;     // int32 parent (global) index of the edge
   mom::DevBuf s_l;     // int32 local edge index

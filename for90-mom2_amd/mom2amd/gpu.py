@@ -60,6 +60,16 @@ class Context:
         capi.check(capi.lib().mom_ctx_fill_tuning(self.h, C.byref(spw), ms))
         return spw.value, {2: ms[0], 4: ms[1], 8: ms[2]}
 
+    def fill_profile(self, enable=-1):
+        """Event timing of the fill kernels: returns the totals accumulated
+        so far (pair_ms, pair_launches, pair_points, assemble_ms,
+        assemble_launches, near_ms), then switches timing on (1) or off (0)
+        and resets them; -1 only reads."""
+        out = np.zeros(6)
+        capi.check(capi.lib().mom_ctx_fill_profile(self.h, int(enable), out))
+        keys = ("pair_ms", "pair_launches", "pair_points", "assemble_ms", "assemble_launches", "near_ms")
+        return dict(zip(keys, out.tolist()))
+
     def close(self):
         """Destroy the context; meshes and systems created on it are
         destroyed first (they hold device memory and the context's stream)."""
@@ -95,6 +105,12 @@ class DeviceMesh:
     @property
     def near_pairs(self):
         return int(capi.lib().mom_mesh_near_pairs(self.h))
+
+    def classify_like(self, other: "DeviceMesh"):
+        """Classify near face pairs by `other`'s centroids and average edge
+        length (same face count), freezing the near/far split of a perturbed
+        mesh to that of the unperturbed one (mom_mesh_classify_like)."""
+        capi.check(capi.lib().mom_mesh_classify_like(self.h, other.h))
 
     def close(self):
         if self.h:

@@ -50,6 +50,14 @@ int mom_ctx_destroy(mom_ctx *ctx);
  * the measured times [ms] of the candidates 2, 4, 8 (ms may be NULL).  The
  * choice changes no result: each face pair's arithmetic is the same. */
 int mom_ctx_fill_tuning(const mom_ctx *ctx, int32_t *spw, double *ms);
+/* Event timing of the fill kernels of this context (no counterpart in the
+ * reference; for measurement).  out[6] (may be NULL) receives the totals
+ * accumulated since timing was last switched: pair-kernel ms, pair-kernel
+ * launches, point pairs evaluated by them (test x source quadrature points),
+ * assembly-kernel ms, assembly launches, near-part ms.  enable = 1 switches
+ * timing on, 0 off (both reset the totals after reading them), -1 only
+ * reads.  Timing adds one event pair per launch. */
+int mom_ctx_fill_profile(mom_ctx *ctx, int32_t enable, double *out);
 
 /*
  * Upload one domain mesh (a flattened mesh_container, mesh.f90:75-83, as
@@ -74,6 +82,14 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
 int mom_mesh_destroy(mom_mesh *mesh);
 /* Number of near (singular-subtraction) face pairs of the mesh. */
 int64_t mom_mesh_near_pairs(const mom_mesh *mesh);
+/* Classify the near face pairs of `mesh` by the centroids and average edge
+ * length of `from` (same context, same face count) instead of its own: its
+ * near list and the fill's near/far test (singular subtraction or plain
+ * quadrature per pair) then follow `from`.  No counterpart in the reference,
+ * which classifies every mesh by its own geometry (sysmat.f90:233-260); for
+ * central differences of a perturbed mesh against the unperturbed one with
+ * the classification frozen.  `from` may be `mesh` itself (a no-op). */
+int mom_mesh_classify_like(mom_mesh *mesh, const mom_mesh *from);
 
 /*
  * computeD / computeH / computeK (sysmat.f90:272-347, 359-434, 446-519),

@@ -76,6 +76,11 @@ int mom_test_vertex_fd(int32_t enable);
  * Process-wide; lets the tests run that branch on any matrix. */
 int mom_test_batch_fallback(int32_t force);
 
+/* Panels of the complex64 factorisation (mixed path): 0 = partial pivoting
+ * as LAPACK (the default), 1 = tournament pivoting (communication-avoiding
+ * LU; the complex128 path always uses partial pivoting).  Process-wide. */
+int mom_test_tournament(int32_t enable);
+
 #ifdef __cplusplus
 }
 #endif

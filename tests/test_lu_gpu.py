@@ -209,3 +209,39 @@ def test_block_refinement_falls_back_to_fp64(gpu_ctx):
             assert berr(A2, X2[:, j], B[:, j]) < 1e-13, j
     finally:
         S.close()
+
+
+@pytest.mark.parametrize("n2", [66, 1100, 3000])
+def test_tournament_panels(gpu_ctx, n2):
+    """The complex64 factorisation with tournament pivoting (the mixed
+    path's opt-in panel mode) against the same system factored with partial
+    pivoting (the default):
+    both first solutions have the backward error of a complex64 LU, and the
+    refined solutions agree with LAPACK's to the same backward error.  Sizes
+    cover a single partial inner block, several panels with a partial last
+    one, and more than eight candidate sets (two tree levels)."""
+    A = rand_complex(n2, seed=40 + n2)
+    A[n2 // 2:, :] *= 1e-5
+    A[:, n2 // 2:] *= 3e3
+    b = rand_complex(n2, 1, seed=41)
+    x_ref = np.linalg.solve(A, b)
+    out = {}
+    for mode in (1, 0):
+        capi.check(capi.lib().mom_test_tournament(mode))
+        try:
+            S = System(gpu_ctx, n2 // 2, MOM_PREC_MIXED)
+            S.set_matrix(A)
+            S.factor()
+            x = S.solve(b, tol=1e-13, maxit=20)
+            st = S.stats()
+            S.close()
+        finally:
+            capi.check(capi.lib().mom_test_tournament(0))
+        hist = st["berr_history"]
+        print("n=%d tournament=%d: backward errors %s" % (n2, mode, ["%.1e" % v for v in hist]), flush=True)
+        assert hist[0] < 1e-4, hist
+        assert berr(A, x[:, 0], b[:, 0]) < 1e-13
+        # forward errors: cond(A) (about 1e6 with these row and column scales) times the backward error
+        assert np.abs(x - x_ref).max() / np.abs(x_ref).max() < 1e-5
+        out[mode] = x
+    assert np.abs(out[1] - out[0]).max() / np.abs(out[0]).max() < 1e-5

@@ -6,7 +6,8 @@ with the Drude index, 8 plane waves solved together by its solve_systems).
 
 Tolerances: matrix entries 1e-10 of their block's largest entry, sources
 1e-12, solutions x and sigma_sca, sigma_abs, rcs 1e-8 (relative), for the
-mixed path, the FP64 path and the mixed path's FP64-factor branch."""
+mixed path (partial-pivoting panels, and tournament-pivoting panels), the
+FP64 path and the mixed path's FP64-factor branch."""
 import os
 
 import numpy as np
@@ -59,7 +60,7 @@ def test_spectrum_setup_and_fill_match_reference(gpu_ctx, spectrum_case):
         assert qerr < 1e-12
 
 
-@pytest.mark.parametrize("mode", ["mixed", "fp64", "mixed_fp64_fallback"])
+@pytest.mark.parametrize("mode", ["mixed", "mixed_tournament", "fp64", "mixed_fp64_fallback"])
 def test_spectrum_solutions_match_reference(gpu_ctx, spectrum_case, mode):
     g, prob = spectrum_case
     dms = [DeviceMesh(gpu_ctx, d) for d in prob.domains]
@@ -67,11 +68,13 @@ def test_spectrum_solutions_match_reference(gpu_ctx, spectrum_case, mode):
     ang = g["angles_deg"]
     prec = MOM_PREC_FP64 if mode == "fp64" else MOM_PREC_MIXED
     capi.check(capi.lib().mom_test_batch_fallback(1 if mode == "mixed_fp64_fallback" else 0))
+    capi.check(capi.lib().mom_test_tournament(1 if mode == "mixed_tournament" else 0))
     try:
         X, info = solve_spectrum(gpu_ctx, dms, N, g["wl"], lambda wl: [1.0 + 0j, drude_index(wl)], ang,
                                  precision=prec)
     finally:
         capi.check(capi.lib().mom_test_batch_fallback(0))
+        capi.check(capi.lib().mom_test_tournament(0))
     for i, wl in enumerate(g["wl"]):
         om = omega_of(wl)
         x_ref = g["x_%d" % i]
@@ -88,7 +91,7 @@ def test_spectrum_solutions_match_reference(gpu_ctx, spectrum_case, mode):
         assert rerr < 1e-8
         if mode == "mixed_fp64_fallback":
             assert info[i]["fell_back"]
-        elif mode == "mixed":
+        elif mode in ("mixed", "mixed_tournament"):
             assert info[i]["fell_back"] or info[i]["berr"] <= 1e-12
     for d in dms:
         d.close()

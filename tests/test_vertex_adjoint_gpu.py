@@ -13,7 +13,11 @@ Pinned by
   of the adjoint balance to that level;
 - central differences of the whole FP64 solve (set-up, fill, LU, solve,
   cs_prtsrf) in single node coordinates, h = 1e-3 nm on a 50 nm sphere
-  (the near/far classification does not change at that size).
+  (the near/far classification does not change at that size), and on all
+  three coordinates of 32 seeded nodes of a radially perturbed sphere
+  (icosphere s = 2, R = 60 nm, l <= 6 harmonics of 3 nm) with the
+  classification of every perturbed mesh frozen to the unperturbed one
+  (mom_mesh_classify_like), as the gradient assumes.
 Tolerances: the node sum against the translation derivative <= 1e-9 of the
 summed magnitudes (1e-7 for the mixed solve, whose x and lambda carry
 forward errors near 1e-8 that the cancelling sum exposes); the derivatives
@@ -139,3 +143,77 @@ def test_far_pair_closed_form_matches_differences(gpu_ctx):
         S.close()
         for dm in dms:
             dm.close()
+
+
+def test_perturbed_sphere_gradient_matches_frozen_differences(gpu_ctx):
+    nodes, faces = meshgen.perturbed_sphere(2, 60.0, 3.0)
+    base = build_problem(nodes, faces, np.full(faces.shape[0], 15), 1e-9, [[15], [-15]])
+    bdms = [DeviceMesh(gpu_ctx, d) for d in base.domains]
+    nnear = [d.near_pairs for d in bdms]
+    sab, grad = _gradient(gpu_ctx, nodes, faces, MOM_PREC_FP64)
+    scale = np.abs(grad).max()
+    h = 1e-3  # nm
+    rng = np.random.default_rng(20251204)
+    verts = rng.choice(nodes.shape[0], 32, replace=False)
+    worst = 0.0
+    try:
+        for v in verts:
+            for c in range(3):
+                vals = []
+                for sgn in (1.0, -1.0):
+                    p = nodes.copy()
+                    p[v, c] += sgn * h
+                    P = build_problem(p, faces, np.full(faces.shape[0], 15), 1e-9, [[15], [-15]])
+                    dms = [DeviceMesh(gpu_ctx, d) for d in P.domains]
+                    for dm, bm in zip(dms, bdms):
+                        dm.classify_like(bm)
+                    assert [d.near_pairs for d in dms] == nnear
+                    S = System(gpu_ctx, P.nedges, MOM_PREC_FP64)
+                    try:
+                        S.fill(dms, RI, OMEGA)
+                        S.factor()
+                        x = S.solve_pw(dms[0], OMEGA, RI[0], ANG)[:, 0]
+                        vals.append(cs_prtsrf(gpu_ctx, dms[0], P.nedges, OMEGA, RI[0], x, *ANG)[1])
+                    finally:
+                        S.close()
+                        for d in dms:
+                            d.close()
+                fd = (vals[0] - vals[1]) / (2 * h * 1e-9)
+                err = abs(grad[v, c] - fd) / scale
+                worst = max(worst, err)
+                assert err <= 2e-6, (int(v), c, grad[v, c], fd)
+    finally:
+        for d in bdms:
+            d.close()
+    print("perturbed s=2 sphere, 32 nodes x 3 coordinates, frozen classification: sigma_abs %.6e, "
+          "max |adjoint - fd| / max |grad| = %.2e" % (sab, worst), flush=True)
+
+
+def test_classify_like_keeps_or_freezes_the_near_list(gpu_ctx):
+    """mom_mesh_classify_like: a mesh classified like itself fills the same
+    bits; a mesh scaled by 1.5 classified like the original keeps the
+    original's near-pair count; meshes of different face counts are refused."""
+    from mom2amd import capi
+
+    nodes, faces = meshgen.icosphere(1, 50.0)
+    P = build_problem(nodes, faces, np.full(faces.shape[0], 15), 1e-9, [[15], [-15]])
+    dms = [DeviceMesh(gpu_ctx, d) for d in P.domains]
+    S = System(gpu_ctx, P.nedges, MOM_PREC_FP64)
+    S.fill(dms, RI, OMEGA)
+    A0 = S.get_matrix()
+    for d in dms:
+        d.classify_like(d)
+    S.fill(dms, RI, OMEGA)
+    assert np.array_equal(S.get_matrix(), A0)
+    S.close()
+    P2 = build_problem(nodes * 1.5, faces, np.full(faces.shape[0], 15), 1e-9, [[15], [-15]])
+    d2 = DeviceMesh(gpu_ctx, P2.domains[0])
+    d2.classify_like(dms[0])
+    assert d2.near_pairs == dms[0].near_pairs
+    n3, f3 = meshgen.icosphere(2, 50.0)
+    P3 = build_problem(n3, f3, np.full(f3.shape[0], 15), 1e-9, [[15], [-15]])
+    d3 = DeviceMesh(gpu_ctx, P3.domains[0])
+    with pytest.raises(capi.MomError):
+        d3.classify_like(dms[0])
+    for d in dms + [d2, d3]:
+        d.close()
