@@ -189,9 +189,11 @@ def bench_sh(args):
     -9.4+1.5i at the SH, surface chi2 nnn = 1 on both spheres, x-polarised
     plane wave along +z.  One step = fundamental fill + LU + solve,
     nlsurf_coef of both spheres, SH fill (sysmat_pmchwt_nls) + LU + solve and
-    the SH radar cross-section on the 37 x 72 grid, all in the mixed path."""
-    from mom2amd import (DEGTORAD, Context, DeviceMesh, System, load_problem, meshgen, omega_of, rcs,
-                         solve_sh)
+    the SH radar cross-section on the 37 x 72 grid, all in the mixed path.
+    After the timed steps: the derivative of the SH power on that grid with
+    respect to the gap width by the adjoint, and its central differences."""
+    from mom2amd import (DEGTORAD, Context, DeviceMesh, System, build_problem, load_problem, meshgen, omega_of, rcs,
+                         sh_power, sh_power_translation_gradient, solve_sh)
     from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
 
     s = args.subdiv if args.subdiv != 5 else 4
@@ -256,6 +258,40 @@ def bench_sh(args):
         out["speedup_vs_fp64_path"] = round(r64["wall"] / mean, 3)
         out["sh_rcs_relerr_vs_fp64_path"] = float(np.abs(runs[-1]["rcs_grid"] - r64["rcs_grid"]).max()
                                                   / np.abs(r64["rcs_grid"]).max())
+    # derivative of the SH power radiated into the 37 x 72 grid with respect to the gap, by the adjoint
+    # (mixed path), checked against central differences of the FP64 chain (h = 1e-3 nm, near/far
+    # classification of the perturbed meshes frozen to the unperturbed ones)
+    px = prob.master.p[:, 0]
+    bodies = [(px < 0, (-0.5e-9, 0.0, 0.0)), (px > 0, (0.5e-9, 0.0, 0.0))]  # per nm of gap
+    t0 = time.perf_counter()
+    gr = sh_power_translation_gradient(ctx, prob, dms, omega, ri, shri, nls, chi2, ang, bodies,
+                                       precision=MOM_PREC_MIXED)
+    tg = time.perf_counter() - t0
+    out["gap_gradient"] = {"P_sh_grid": gr["P"], "dP_dgap_per_nm": gr["dP"], "seconds": round(tg, 3),
+                           "terms": gr["terms"],
+                           "note": "P = sum over the 37 x 72 rcs_ext grid of W (|et|^2 + |ep|^2) at 2 omega, W the "
+                                   "trapezoid solid-angle weights; forward chain + adjoint (far-field transpose, Z2^T "
+                                   "and Z1^T solves, nlsurf_coef transpose, vertex adjoint of every domain's blocks)"}
+    log("SH power gap gradient by the adjoint: P %.6e, dP/dgap %.6e per nm (%.2f s incl. the forward chain)"
+        % (gr["P"], gr["dP"], tg))
+    if not args.no_fp64:
+        vals = []
+        for gap in (10.0 + 1e-3, 10.0 - 1e-3):
+            n2, f2, i2 = meshgen.dimer(s, 40.0, gap)
+            p2 = build_problem(n2, f2, i2, 1e-9, doms)
+            dms2 = [DeviceMesh(ctx, dm) for dm in p2.domains]
+            try:
+                for dm2, dm in zip(dms2, dms):
+                    dm2.classify_like(dm)
+                vals.append(sh_power(ctx, p2, dms2, omega, ri, shri, nls, chi2, ang, precision=MOM_PREC_FP64))
+            finally:
+                for dm in dms2:
+                    dm.close()
+        fd = (vals[0] - vals[1]) / 2e-3
+        out["gap_gradient"]["fd_fp64_per_nm"] = fd
+        out["gap_gradient"]["relerr_vs_fd"] = abs(gr["dP"] - fd) / abs(fd)
+        log("central differences of the FP64 chain: %.6e per nm (rel err of the adjoint %.1e)"
+            % (fd, abs(gr["dP"] - fd) / abs(fd)))
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -130,9 +130,91 @@ int matvec(mom_ctx *ctx, const double2 *M, int64_t ld, int64_t nr, int64_t nc, c
   return 0;
 }
 
+// y = M^T x (no conjugation): one wave per column of the column-major M,
+// lanes striding down the column, the wave's partial sums added in one
+// fixed order (deterministic).
+__global__ __launch_bounds__(256) void matvec_t_kernel(const double2 *__restrict__ M, int64_t ld, int64_t nr,
+                                                       int64_t nc, const double2 *__restrict__ x, double2 *y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= nc) return;
+  const double2 *col = M + c * ld;
+  double ar = 0, ai = 0;
+  for (int64_t r = lane; r < nr; r += 64) {
+    const double2 m = col[r], v = x[r];
+    ar += m.x * v.x - m.y * v.y;
+    ai += m.x * v.y + m.y * v.x;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    ar += __shfl_xor(ar, o);
+    ai += __shfl_xor(ai, o);
+  }
+  if (lane == 0) y[c] = make_double2(ar, ai);
+}
+
 }  // namespace mom
 
 using namespace mom;
+
+// y = Z_d x or Z_d^T x for one domain's blocks of the PMCHWT matrix
+// (sysmat.f90:56-98 for that domain alone) in global numbering: the block
+// is filled in the domain's local numbering, applied to x at the domain's
+// unknowns, and the product returned at them (zero elsewhere).
+extern "C" int mom_domain_block_apply(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega,
+                                      double ri_re, double ri_im, int32_t trans, int32_t nq, const double *qw,
+                                      const double *qnodes, const double *x, double *y) {
+  if (!ctx || !mesh || !x || !y) return fail(MOM_ERR_ARG, "NULL argument");
+  if (mesh->ctx != ctx) return fail(MOM_ERR_ARG, "mesh belongs to another context");
+  if (nedges_total < 1 || mesh->max_parent >= nedges_total)
+    return fail(MOM_ERR_ARG, "domain edge parent index >= nedges_total");
+  if (!(omega > 0)) return fail(MOM_ERR_ARG, "omega must be > 0");
+  if (trans != 0 && trans != 1) return fail(MOM_ERR_ARG, "trans must be 0 or 1");
+  const cd ri(ri_re, ri_im);
+  if (ri == cd(0.0)) return fail(MOM_ERR_ARG, "ri must be non-zero");
+  (void)hipSetDevice(ctx->device);
+  QuadRule rule;
+  MOM_TRY(make_rule(nq, qw, qnodes, &rule));
+  const int64_t Nt = nedges_total, E = mesh->E;
+  const cd zsi = (ri * ri) / (kEta0 * kEta0);
+  DevBuf Z, dx, dy;
+  int rc;
+  if ((rc = Z.alloc(sizeof(double2) * 4 * E * E)) || (rc = dx.alloc(sizeof(double2) * 2 * E)) ||
+      (rc = dy.alloc(sizeof(double2) * 2 * E)))
+    return rc;
+  MOM_HIP(hipMemsetAsync(Z.ptr, 0, Z.bytes, ctx->stream));
+  FillTarget t;
+  t.Z = Z.as<double2>();
+  t.ld = 2 * E;
+  t.Noff = E;
+  t.mode = 0;
+  t.local = 1;
+  t.zsi = make_double2(zsi.real(), zsi.imag());
+  MOM_TRY(fill_domain(ctx, mesh, rule, omega, ri, t));
+  std::vector<cd> xl(2 * E), yl(2 * E);
+  for (int64_t i = 0; i < E; i++) {
+    const int64_t gi = mesh->h_eparent[i];
+    xl[i] = cd(x[2 * gi], x[2 * gi + 1]);
+    xl[E + i] = cd(x[2 * (Nt + gi)], x[2 * (Nt + gi) + 1]);
+  }
+  MOM_TRY(copy_h2d(ctx, dx.ptr, xl.data(), sizeof(double2) * 2 * E));
+  if (trans) {
+    hipLaunchKernelGGL(matvec_t_kernel, dim3((unsigned)((2 * E + 3) / 4)), dim3(256), 0, ctx->stream, Z.as<double2>(),
+                       2 * E, 2 * E, 2 * E, dx.as<double2>(), dy.as<double2>());
+    MOM_HIP(hipGetLastError());
+  } else {
+    MOM_TRY(matvec(ctx, Z.as<double2>(), 2 * E, 2 * E, 2 * E, dx.as<double2>(), dy.as<double2>()));
+  }
+  MOM_TRY(copy_d2h(ctx, yl.data(), dy.ptr, sizeof(double2) * 2 * E));
+  for (int64_t i = 0; i < 4 * Nt; i++) y[i] = 0.0;
+  for (int64_t i = 0; i < E; i++) {
+    const int64_t gi = mesh->h_eparent[i];
+    y[2 * gi] = yl[i].real();
+    y[2 * gi + 1] = yl[i].imag();
+    y[2 * (Nt + gi)] = yl[E + i].real();
+    y[2 * (Nt + gi) + 1] = yl[E + i].imag();
+  }
+  return 0;
+}
 
 extern "C" {
 

@@ -598,6 +598,19 @@ int mom_far_fields(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, dou
                          identity_group());
 }
 
+int mom_far_fields_adjoint(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                           double ri_im, double r, int64_t ndir, const double *theta, const double *phi,
+                           const double *v, int32_t nq, const double *qw, const double *qnodes, double *g) {
+  if (!ctx || !mesh || !theta || !phi || !v || !g) return fail(MOM_ERR_ARG, "NULL argument");
+  if (mesh->ctx != ctx) return fail(MOM_ERR_ARG, "mesh belongs to another context");
+  if (!(omega > 0) || !(r > 0)) return fail(MOM_ERR_ARG, "omega and r must be > 0");
+  if (nedges_total < 1 || mesh->max_parent >= nedges_total) return fail(MOM_ERR_ARG, "nedges_total too small");
+  QuadRule rule;
+  MOM_TRY(make_rule(nq, qw, qnodes, &rule));
+  (void)hipSetDevice(ctx->device);
+  return far_fields_adjoint(ctx, mesh, rule, omega, cd(ri_re, ri_im), nedges_total, r, ndir, theta, phi, v, g);
+}
+
 static int rcs_impl(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
                     double ri_im, const double *x, int32_t ntheta, int32_t nphi, int32_t filter, const double *fdir,
                     int32_t nq, const double *qw, const double *qnodes, double *rcs, const Group &G) {

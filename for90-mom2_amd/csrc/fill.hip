@@ -1759,6 +1759,171 @@ int far_fields_device(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, dou
 }
 
 // ---------------------------------------------------------------------------
+// Transpose of the far-field map x -> out = (et, ep, ht, hp) of
+// far_fields_device (identity group): for adjoint values v[8 d ..] of the
+// outputs of direction d (complex, interleaved), g = L^T v, so that
+// Re(g^T dx) = Re(sum_d v_d . dout_d).  The outputs fold back into the four
+// radiation integrals per direction (farfield_adjoint_coef_kernel), the
+// direction sum gives adjoint currents per point
+// (farfield_adjoint_points_kernel), the currents project on each face's
+// RWG sides (farfield_adjoint_faces_kernel), and every edge sums its two
+// sides in face order (srcvec_gather_kernel).
+//
+// et = P (Lp + a Nt), ep = -P (Lt - a Np), ht = -P (Np - Lt b),
+// hp = P (Nt + Lp b) give the adjoint integrals
+// Nt' = P (a et' + hp'), Np' = P (a ep' - ht'), Lt' = P (b ht' - ep'),
+// Lp' = P (et' + b hp').
+#pragma clang fp contract(off)
+__global__ void farfield_adjoint_coef_kernel(int64_t ndir, const double *__restrict__ v, FarCoef c,
+                                             double2 *__restrict__ vb) {
+  const int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (d >= ndir) return;
+  const double *o = v + 8 * d;
+  const double2 et = make_double2(o[0], o[1]), ep = make_double2(o[2], o[3]), ht = make_double2(o[4], o[5]),
+                hp = make_double2(o[6], o[7]);
+  vb[4 * d] = zmul(c.P, zadd(zmul(c.a, et), hp));
+  vb[4 * d + 1] = zmul(c.P, zsub(zmul(c.a, ep), ht));
+  vb[4 * d + 2] = zmul(c.P, zsub(zmul(c.b, ht), ep));
+  vb[4 * d + 3] = zmul(c.P, zadd(et, zmul(c.b, hp)));
+}
+#pragma clang fp contract(on)
+
+constexpr int kFaTile = 128;  // directions per LDS tile of the adjoint point kernel
+
+// One thread per point s: cbar[c][s] = sum_d exp(-i k dir_d . r_s)
+// (u_theta Nt' + u_phi Np')_c (J, c < 3) and (u_theta Lt' + u_phi Lp')_c (M).
+__global__ __launch_bounds__(256) void farfield_adjoint_points_kernel(int64_t S, const double *__restrict__ qp,
+                                                                       int ndir, const double *__restrict__ dirs,
+                                                                       const double2 *__restrict__ vb, double kr,
+                                                                       double ki, double2 *__restrict__ cbar) {
+  __shared__ double su[kFaTile][8];
+  __shared__ double2 sv[kFaTile][4];
+  const int tid = threadIdx.x;
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + tid;
+  double r[3] = {0, 0, 0};
+  if (s < S)
+    for (int c = 0; c < 3; c++) r[c] = qp[3 * s + c];
+  double J[3][2] = {{0, 0}, {0, 0}, {0, 0}}, M[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int d0 = 0; d0 < ndir; d0 += kFaTile) {
+    const int nt = min(kFaTile, ndir - d0);
+    if (tid < nt) {
+      for (int c = 0; c < 8; c++) su[tid][c] = dirs[8 * (int64_t)(d0 + tid) + c];
+      for (int k = 0; k < 4; k++) sv[tid][k] = vb[4 * (int64_t)(d0 + tid) + k];
+    }
+    __syncthreads();
+    for (int q = 0; q < nt; q++) {
+      const double *u = su[q];
+      const double rd = u[0] * r[0] + u[1] * r[1] + u[2] * r[2];
+      double er, ei;
+      cis_exp(ki * rd, -kr * rd, er, ei);  // exp(-i k rd)
+      double2 a[4];
+      for (int k = 0; k < 4; k++)
+        a[k] = make_double2(er * sv[q][k].x - ei * sv[q][k].y, er * sv[q][k].y + ei * sv[q][k].x);
+      const double ut[3] = {u[3], u[4], u[5]}, up[3] = {u[6], u[7], 0.0};
+      for (int c = 0; c < 3; c++) {
+        J[c][0] += ut[c] * a[0].x + up[c] * a[1].x;
+        J[c][1] += ut[c] * a[0].y + up[c] * a[1].y;
+        M[c][0] += ut[c] * a[2].x + up[c] * a[3].x;
+        M[c][1] += ut[c] * a[2].y + up[c] * a[3].y;
+      }
+    }
+    __syncthreads();
+  }
+  if (s < S)
+    for (int c = 0; c < 3; c++) {
+      cbar[c * S + s] = make_double2(J[c][0], J[c][1]);
+      cbar[(3 + c) * S + s] = make_double2(M[c][0], M[c][1]);
+    }
+}
+
+// One thread per face: side q's adjoint J and M coefficients,
+// w[2 (3f + q)] = sum_t w_t A f_q(r_t) . Jbar(r_t), w[2 (3f + q) + 1] likewise
+// with Mbar (the transpose of farfield_sources_kernel).
+__global__ void farfield_adjoint_faces_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
+                                              const double *__restrict__ area, FaceSide side,
+                                              const double2 *__restrict__ cbar, int64_t S, double2 *__restrict__ w) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  double aj[3][2] = {{0, 0}, {0, 0}, {0, 0}}, am[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int t = 0; t < Q; t++) {
+    const int64_t s = (int64_t)f * Q + t;
+    const double *r = qp + 3 * s;
+    const double wa = rule.w[t] * area[f];
+    double2 cj[3], cm[3];
+    for (int c = 0; c < 3; c++) {
+      cj[c] = cbar[c * S + s];
+      cm[c] = cbar[(3 + c) * S + s];
+    }
+    for (int q = 0; q < 3; q++) {
+      const double cf = side.coef[3 * f + q];
+      const double *v = side.v + 3 * (3 * f + q);
+      for (int c = 0; c < 3; c++) {
+        const double fv = wa * (cf * (r[c] - v[c]));
+        aj[q][0] += fv * cj[c].x;
+        aj[q][1] += fv * cj[c].y;
+        am[q][0] += fv * cm[c].x;
+        am[q][1] += fv * cm[c].y;
+      }
+    }
+  }
+  for (int q = 0; q < 3; q++) {
+    w[2 * (3 * f + q)] = make_double2(aj[q][0], aj[q][1]);
+    w[2 * (3 * f + q) + 1] = make_double2(am[q][0], am[q][1]);
+  }
+}
+
+// Per-side pairs w[3F][2] (J, M) summed per edge into the global vector
+// g[2 Nt] (g[parent] and g[parent + Nt] += the edge's two sides, face order).
+int gather_sides(mom_ctx *ctx, const mom_mesh *m, const double2 *w, int64_t Nt, double2 *g) {
+  hipLaunchKernelGGL(srcvec_gather_kernel, dim3((unsigned)((m->E + 255) / 256)), dim3(256), 0, ctx->stream, m->E,
+                     (int)m->F, (int64_t)1, m->eside.as<int>(), m->s_g.as<int>(), w, Nt, g, 2 * Nt);
+  MOM_HIP(hipGetLastError());
+  return 0;
+}
+
+int far_fields_adjoint(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t Nt,
+                       double r, int64_t ndir, const double *theta, const double *phi, const double *v, double *g) {
+  const cd I(0.0, 1.0);
+  const cd k = ri * omega / kC0;
+  if (!(std::fabs(k.real()) * m->rmax < kCisMaxArg))
+    return fail(MOM_ERR_ARG, "Re(k) times the mesh's distance from the origin is beyond the far field's phase range "
+                             "(2^19 rad)");
+  if (ndir < 1 || ndir > (1 << 24)) return fail(MOM_ERR_ARG, "number of directions out of range");
+  MOM_TRY(compute_quad_points(ctx, m, rule));
+  const int64_t S = m->F * rule.Q;
+  DevBuf dth, dph, ddirs, dv, dvb, dcb, dw, dg;
+  int rc;
+  if ((rc = dth.alloc(sizeof(double) * ndir)) || (rc = dph.alloc(sizeof(double) * ndir)) ||
+      (rc = ddirs.alloc(sizeof(double) * 8 * ndir)) || (rc = dv.alloc(sizeof(double) * 8 * ndir)) ||
+      (rc = dvb.alloc(sizeof(double2) * 4 * ndir)) || (rc = dcb.alloc(sizeof(double2) * 6 * S)) ||
+      (rc = dw.alloc(sizeof(double2) * 6 * m->F)) || (rc = dg.alloc(sizeof(double2) * 2 * Nt)))
+    return rc;
+  MOM_TRY(copy_h2d(ctx, dth.ptr, theta, sizeof(double) * ndir));
+  MOM_TRY(copy_h2d(ctx, dph.ptr, phi, sizeof(double) * ndir));
+  MOM_TRY(copy_h2d(ctx, dv.ptr, v, sizeof(double) * 8 * ndir));
+  MOM_HIP(hipMemsetAsync(dg.ptr, 0, dg.bytes, ctx->stream));
+  const unsigned gd = (unsigned)((ndir + 255) / 256);
+  hipLaunchKernelGGL(farfield_dirs_kernel, dim3(gd), dim3(256), 0, ctx->stream, ndir, dth.as<double>(),
+                     dph.as<double>(), ddirs.as<double>());
+  FarCoef c;
+  const cd P = I * k * std::exp(I * k * r) / (4.0 * MOM_PI * r), a = kEta0 / ri, b = ri / kEta0;
+  c.P = make_double2(P.real(), P.imag());
+  c.a = make_double2(a.real(), a.imag());
+  c.b = make_double2(b.real(), b.imag());
+  hipLaunchKernelGGL(farfield_adjoint_coef_kernel, dim3(gd), dim3(256), 0, ctx->stream, ndir, dv.as<double>(), c,
+                     dvb.as<double2>());
+  hipLaunchKernelGGL(farfield_adjoint_points_kernel, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, ctx->stream,
+                     S, ctx->qp.as<double>(), (int)ndir, ddirs.as<double>(), dvb.as<double2>(), k.real(), k.imag(),
+                     dcb.as<double2>());
+  hipLaunchKernelGGL(farfield_adjoint_faces_kernel, dim3((unsigned)((m->F + 127) / 128)), dim3(128), 0, ctx->stream,
+                     (int)m->F, rule.Q, rule, ctx->qp.as<double>(), m->farea.as<double>(), sides(m, true),
+                     dcb.as<double2>(), S, dw.as<double2>());
+  MOM_HIP(hipGetLastError());
+  MOM_TRY(gather_sides(ctx, m, dw.as<double2>(), Nt, dg.as<double2>()));
+  return copy_d2h(ctx, g, dg.ptr, sizeof(double2) * 2 * Nt);
+}
+
+// ---------------------------------------------------------------------------
 // Scratch of the allocation-free far-field entry point: a fixed device array
 // (each GPU the library runs on holds its own copy until the process ends).
 // Host code never uses the array's name as an address: its device address

@@ -247,6 +247,13 @@ int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
 // ... and of sigma_abs|_x + Re(lambda^T q) for the exterior domain and one plane wave
 int vertex_adjoint_faces(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
                          const double2 *x, const double2 *lam, double th, double ph, double ps, double *dvert);
+// Transpose of the far-field map (fill.hip): g[2 Nt] (host) = L^T v for
+// adjoint values v[8 ndir] of (et, ep, ht, hp) per direction (host arrays)
+int far_fields_adjoint(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t Nt,
+                       double r, int64_t ndir, const double *theta, const double *phi, const double *v, double *g);
+// Per-side pairs w[3F][2] (device) summed per edge into the global device
+// vector g[2 Nt] at the edges' parents (rows parent, parent + Nt), face order
+int gather_sides(mom_ctx *ctx, const mom_mesh *m, const double2 *w, int64_t Nt, double2 *g);
 // y = M x (device, complex128, M nr x nc with leading dimension ld), in
 // column chunks spread over the device (adjoint.hip)
 int matvec(mom_ctx *ctx, const double2 *M, int64_t ld, int64_t nr, int64_t nc, const double2 *x, double2 *y);

@@ -228,6 +228,11 @@ __global__ void nls_gather2_kernel(int E, const int *__restrict__ eside, const d
   B[E + e] = m.y;
 }
 
+__global__ void negate_kernel(int64_t n, double *v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] = -v[i];
+}
+
 // src_coef = (srcM, srcJ) from the two solves, src_vec = (0.5 coef, srcH)
 __global__ void nls_output_kernel(int E, const double *__restrict__ X1, const double *__restrict__ X2,
                                   const double2 *__restrict__ coef0, const double2 *__restrict__ srch,
@@ -557,6 +562,224 @@ static int nlsurf_coef_impl(mom_ctx *ctx, const mom_mesh *m, int64_t Nt, double 
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of nlsurf_coef: the outputs src_coef = (srcM, srcJ) and
+// src_vec = (coef0 / 2, srcH) are holomorphic and quadratic in the
+// fundamental solution x, so for adjoint values (scb, svb) of the outputs
+// the transpose of the Jacobian at x gives xb with
+// Re(xb^T dx) = Re(scb^T dsrc_coef + svb^T dsrc_vec).  Backwards through
+// the forward steps:
+//   srcM = G^-1 m, srcJ = G^-1 jj:  mb = G^-1 srcMb, jjb = G^-1 srcJb
+//     (G real symmetric: its own transpose);
+//   m = gather(stage2(coef)): the per-face map of stage2 is antisymmetric
+//     ((n x a) . b = -(n x b) . a), so coefb = -gather(stage2(mb));
+//   coef = G^-1 c:  cb = G^-1 coefb + svb_J / 2;
+//   jj = -i omegash j:  jb = -i omegash jjb;  srcH = i omegash h / 2:
+//     hb = i omegash svb_M / 2;
+//   stage 1 (per face and point) reversed by nls_stage1_adj_kernel with the
+//   field and polarisation of x recomputed.
+// Per edge (c, j, h) adjoints: eb[3][E] = (cb, jb, hb).
+__global__ void nls_adj_edges_kernel(int E, const double *__restrict__ Xa, const double *__restrict__ Xc,
+                                     const double2 *__restrict__ svb, double omegash, double2 *__restrict__ eb) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const double2 s1 = svb[e], s2 = svb[E + e];
+  eb[e] = make_double2(Xc[e] + 0.5 * s1.x, Xc[E + e] + 0.5 * s1.y);
+  // -i omegash (a + i b) = omegash (b - i a)
+  eb[E + e] = make_double2(omegash * Xa[3 * E + e], -omegash * Xa[2 * E + e]);
+  // i omegash / 2 (a + i b) = omegash / 2 (-b + i a)
+  eb[2 * E + e] = make_double2(-0.5 * omegash * s2.y, 0.5 * omegash * s2.x);
+}
+
+// Stage 1 reversed, one thread per face: side q's output adjoints
+// (i1, i3, i2 scaled as the forward writes them) come from its edge's eb;
+// w[2 (3f + q)] = d/dxJ_q, w[2 (3f + q) + 1] = d/dxM_q (transposed).
+__global__ __launch_bounds__(128) void nls_stage1_adj_kernel(NlsFaceArgs a, const int *__restrict__ s_l, int E,
+                                                             const double2 *__restrict__ eb, double2 *__restrict__ w) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.F) return;
+  const double n[3] = {a.fn[3 * f], a.fn[3 * f + 1], a.fn[3 * f + 2]};
+  const double A = a.area[f];
+  const double2 ep = a.epsp[f];
+  double2 xj[3], xm[3], b1[3], b2[3], b3[3];
+  double cf[3], dv[3], vv[3][3];
+  for (int q = 0; q < 3; q++) {
+    const int s = 3 * f + q;
+    xj[q] = a.x[a.g[s]];
+    xm[q] = a.x[a.Nt + a.g[s]];
+    cf[q] = a.coef[s];
+    dv[q] = a.div[s];
+    for (int c = 0; c < 3; c++) vv[q][c] = a.v[3 * s + c];
+    const int l = s_l[s];
+    const double2 z = make_double2(0, 0);
+    // out = (i1 A / ep, i3 A, i2 A)
+    b1[q] = l >= 0 ? cdiv(cscale(eb[l], A), ep) : z;
+    b3[q] = l >= 0 ? cscale(eb[E + l], A) : z;
+    b2[q] = l >= 0 ? cscale(eb[2 * E + l], A) : z;
+  }
+  double2 divj = make_double2(0, 0);
+  for (int q = 0; q < 3; q++) divj = cadd(divj, cscale(xj[q], dv[q]));
+  const double2 en = cmul(divj, a.en_fac);
+  double2 xmb[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+  double2 enb = make_double2(0, 0);
+  for (int t = 0; t < a.Q; t++) {
+    const double *r = a.qp + ((int64_t)f * a.Q + t) * 3;
+    double fv[3][3];
+    for (int q = 0; q < 3; q++)
+      for (int c = 0; c < 3; c++) fv[q][c] = cf[q] * (r[c] - vv[q][c]);
+    // forward values at this point
+    double2 ms[3];
+    for (int c = 0; c < 3; c++) {
+      ms[c] = make_double2(0, 0);
+      for (int q = 0; q < 3; q++) ms[c] = cadd(ms[c], cscale(xm[q], fv[q][c]));
+    }
+    double2 e[3];
+    e[0] = cadd(csub(cscale(ms[2], n[1]), cscale(ms[1], n[2])), cscale(en, n[0]));
+    e[1] = cadd(csub(cscale(ms[0], n[2]), cscale(ms[2], n[0])), cscale(en, n[1]));
+    e[2] = cadd(csub(cscale(ms[1], n[0]), cscale(ms[0], n[1])), cscale(en, n[2]));
+    const double2 enn = cadd(cadd(cscale(e[0], n[0]), cscale(e[1], n[1])), cscale(e[2], n[2]));
+    double2 et[3];
+    for (int c = 0; c < 3; c++) et[c] = csub(e[c], cscale(enn, n[c]));
+    const double2 tfac = cmul(cscale(a.chi[2], 2.0 * kEps0), enn);
+    // adjoints of pn, ptn and pt from i1, i2, i3
+    const double wt = a.w[t];
+    double2 pnb = make_double2(0, 0), ptnb[3], ptb[3];
+    for (int c = 0; c < 3; c++) ptnb[c] = ptb[c] = make_double2(0, 0);
+    for (int q = 0; q < 3; q++) {
+      pnb = csub(pnb, cscale(b1[q], wt * dv[q]));
+      for (int c = 0; c < 3; c++) {
+        ptnb[c] = cadd(ptnb[c], cscale(b2[q], wt * fv[q][c]));
+        ptb[c] = cadd(ptb[c], cscale(b3[q], wt * fv[q][c]));
+      }
+    }
+    // ptn = pt x n:  ptb += n x ptnb
+    ptb[0] = cadd(ptb[0], csub(cscale(ptnb[2], n[1]), cscale(ptnb[1], n[2])));
+    ptb[1] = cadd(ptb[1], csub(cscale(ptnb[0], n[2]), cscale(ptnb[2], n[0])));
+    ptb[2] = cadd(ptb[2], csub(cscale(ptnb[1], n[0]), cscale(ptnb[0], n[1])));
+    // pt = P - pn n, pn = n . P:  Pb = ptb + n (pnb - n . ptb)
+    const double2 nptb = cadd(cadd(cscale(ptb[0], n[0]), cscale(ptb[1], n[1])), cscale(ptb[2], n[2]));
+    const double2 pnt = csub(pnb, nptb);
+    double2 Pb[3];
+    for (int c = 0; c < 3; c++) Pb[c] = cadd(ptb[c], cscale(pnt, n[c]));
+    // P = et tfac + pn0 n
+    double2 etb[3];
+    for (int c = 0; c < 3; c++) etb[c] = cmul(tfac, Pb[c]);
+    const double2 tfacb = cadd(cadd(cmul(et[0], Pb[0]), cmul(et[1], Pb[1])), cmul(et[2], Pb[2]));
+    const double2 pn0b = cadd(cadd(cscale(Pb[0], n[0]), cscale(Pb[1], n[1])), cscale(Pb[2], n[2]));
+    // tfac = 2 eps0 chi_ttn enn; pn0 = eps0 (chi_nnn enn^2 + chi_ntt ett)
+    double2 ennb = cmul(cscale(a.chi[2], 2.0 * kEps0), tfacb);
+    ennb = cadd(ennb, cmul(cscale(cmul(a.chi[0], enn), 2.0 * kEps0), pn0b));
+    const double2 ettb = cmul(cscale(a.chi[1], kEps0), pn0b);
+    // ett = et . et
+    for (int c = 0; c < 3; c++) etb[c] = cadd(etb[c], cmul(cscale(et[c], 2.0), ettb));
+    // et = e - enn n, enn = n . e:  eb = etb + n (ennb - n . etb)
+    const double2 netb = cadd(cadd(cscale(etb[0], n[0]), cscale(etb[1], n[1])), cscale(etb[2], n[2]));
+    const double2 ent = csub(ennb, netb);
+    double2 ebv[3];
+    for (int c = 0; c < 3; c++) ebv[c] = cadd(etb[c], cscale(ent, n[c]));
+    // e = n x ms + en n:  msb = ebv x n, enb += n . ebv
+    double2 msb[3];
+    msb[0] = csub(cscale(ebv[1], n[2]), cscale(ebv[2], n[1]));
+    msb[1] = csub(cscale(ebv[2], n[0]), cscale(ebv[0], n[2]));
+    msb[2] = csub(cscale(ebv[0], n[1]), cscale(ebv[1], n[0]));
+    enb = cadd(enb, cadd(cadd(cscale(ebv[0], n[0]), cscale(ebv[1], n[1])), cscale(ebv[2], n[2])));
+    // ms = sum_q xm_q f_q
+    for (int q = 0; q < 3; q++)
+      for (int c = 0; c < 3; c++) xmb[q] = cadd(xmb[q], cscale(msb[c], fv[q][c]));
+  }
+  // en = divj en_fac, divj = sum_q xj_q div_q
+  const double2 divjb = cmul(enb, a.en_fac);
+  for (int q = 0; q < 3; q++) {
+    w[2 * (3 * f + q)] = cscale(divjb, dv[q]);
+    w[2 * (3 * f + q) + 1] = xmb[q];
+  }
+}
+
+static int nlsurf_coef_adjoint_impl(mom_ctx *ctx, const mom_mesh *m, int64_t Nt, double omega, cd riff,
+                                    const double *epsp, const double *x, const double *chi2, const QuadRule &rule,
+                                    const double *scb, const double *svb, double *xb) {
+  if (m->max_parent >= Nt) return fail(MOM_ERR_ARG, "domain edge parent index >= nedges_total");
+  for (int64_t f = 0; f < m->F; f++)
+    if (!(epsp[2 * f] != 0 || epsp[2 * f + 1] != 0)) return fail(MOM_ERR_ARG, "epsp must be non-zero");
+  MOM_TRY(compute_quad_points(ctx, m, rule));
+  const int F = (int)m->F, E = (int)m->E;
+  DevBuf dx, deps, dside, dBa, dXa, dB2, dXc, dW, dinfo, dsvb, deb, dw, dg, deside;
+  int rc;
+  if ((rc = dx.alloc(sizeof(double2) * 2 * Nt)) || (rc = deps.alloc(sizeof(double2) * F)) ||
+      (rc = dside.alloc(sizeof(double2) * 3 * F)) || (rc = dBa.alloc(sizeof(double) * 4 * E)) ||
+      (rc = dXa.alloc(sizeof(double) * 4 * E)) || (rc = dB2.alloc(sizeof(double) * 2 * E)) ||
+      (rc = dXc.alloc(sizeof(double) * 2 * E)) || (rc = dW.alloc(sizeof(double) * 12 * E)) ||
+      (rc = dinfo.alloc(sizeof(double) * 8)) || (rc = dsvb.alloc(sizeof(double2) * 2 * E)) ||
+      (rc = deb.alloc(sizeof(double2) * 3 * E)) || (rc = dw.alloc(sizeof(double2) * 6 * F)) ||
+      (rc = dg.alloc(sizeof(double2) * 2 * Nt)) || (rc = deside.alloc(sizeof(int) * 2 * E)))
+    return rc;
+  // right-hand sides of the first adjoint Gram solve: Re, Im of srcMb, then of srcJb
+  std::vector<double> Ba(4 * (size_t)E);
+  for (int e = 0; e < E; e++) {
+    Ba[e] = scb[2 * e];
+    Ba[E + e] = scb[2 * e + 1];
+    Ba[2 * E + e] = scb[2 * (E + e)];
+    Ba[3 * E + e] = scb[2 * (E + e) + 1];
+  }
+  MOM_TRY(copy_h2d(ctx, dx.ptr, x, sizeof(double2) * 2 * Nt));
+  MOM_TRY(copy_h2d(ctx, deps.ptr, epsp, sizeof(double2) * F));
+  MOM_TRY(copy_h2d(ctx, dBa.ptr, Ba.data(), sizeof(double) * 4 * E));
+  MOM_TRY(copy_h2d(ctx, dsvb.ptr, svb, sizeof(double2) * 2 * E));
+  MOM_HIP(hipMemsetAsync(dg.ptr, 0, dg.bytes, ctx->stream));
+  Gram G;
+  MOM_TRY(build_gram(ctx, m, rule, G));
+  MOM_TRY(gram_solve(ctx, G, 4, dBa.as<double>(), dXa.as<double>(), dW.as<double>(), dinfo.as<double>()));
+  // coefb = -gather(stage2(mb)), mb = rows 0, 1 of Xa
+  NlsStage2Args b;
+  b.F = F;
+  b.Q = rule.Q;
+  b.E = E;
+  for (int i = 0; i < kMaxQ; i++) b.w[i] = i < rule.Q ? rule.w[i] : 0.0;
+  b.qp = ctx->qp.as<double>();
+  b.fn = m->fn.as<double>();
+  b.area = m->farea.as<double>();
+  b.l = m->s_l.as<int>();
+  b.coef = m->s_coef.as<double>();
+  b.v = m->s_v.as<double>();
+  b.X = dXa.as<double>();
+  b.side_out = dside.as<double2>();
+  MOM_TRY(copy_h2d(ctx, deside.ptr, m->h_eside.data(), sizeof(int) * 2 * E));
+  hipLaunchKernelGGL(nls_stage2_kernel, dim3((unsigned)((F + 127) / 128)), dim3(128), 0, ctx->stream, b);
+  hipLaunchKernelGGL(nls_gather2_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, ctx->stream, E,
+                     deside.as<int>(), dside.as<double2>(), dB2.as<double>());
+  hipLaunchKernelGGL(negate_kernel, dim3((unsigned)((2 * E + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (int64_t)2 * E, dB2.as<double>());
+  MOM_HIP(hipGetLastError());
+  MOM_TRY(gram_solve(ctx, G, 2, dB2.as<double>(), dXc.as<double>(), dW.as<double>(), dinfo.as<double>()));
+  const double omegash = 2.0 * omega;
+  hipLaunchKernelGGL(nls_adj_edges_kernel, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, ctx->stream, E,
+                     dXa.as<double>(), dXc.as<double>(), dsvb.as<double2>(), omegash, deb.as<double2>());
+  const cd I(0.0, 1.0);
+  const cd enf = 1.0 / ((I * omega) * (kEps0 * (riff * riff)));
+  NlsFaceArgs a;
+  a.F = F;
+  a.Q = rule.Q;
+  for (int i = 0; i < kMaxQ; i++) a.w[i] = i < rule.Q ? rule.w[i] : 0.0;
+  a.qp = ctx->qp.as<double>();
+  a.fn = m->fn.as<double>();
+  a.area = m->farea.as<double>();
+  a.g = m->s_g.as<int>();
+  a.coef = m->s_coef.as<double>();
+  a.div = m->s_div.as<double>();
+  a.v = m->s_v.as<double>();
+  a.x = dx.as<double2>();
+  a.Nt = Nt;
+  a.en_fac = make_double2(enf.real(), enf.imag());
+  for (int k = 0; k < 3; k++) a.chi[k] = make_double2(chi2[2 * k], chi2[2 * k + 1]);
+  a.epsp = deps.as<double2>();
+  a.side_out = nullptr;
+  hipLaunchKernelGGL(nls_stage1_adj_kernel, dim3((unsigned)((F + 127) / 128)), dim3(128), 0, ctx->stream, a,
+                     m->s_l.as<int>(), E, deb.as<double2>(), dw.as<double2>());
+  MOM_HIP(hipGetLastError());
+  MOM_TRY(gather_sides(ctx, m, dw.as<double2>(), Nt, dg.as<double2>()));
+  return copy_d2h(ctx, xb, dg.ptr, sizeof(double2) * 2 * Nt);
+}
+
 // Z[g(i) + bi Nt, g(j) + bj Nt] += Zd[i + bi E, j + bj E]: one domain's local
 // PMCHWT block added into the global matrix (g injective: no conflicts)
 __global__ void scatter_block_kernel(const double2 *__restrict__ Zd, int64_t E, const int *__restrict__ g,
@@ -676,6 +899,21 @@ int mom_nlsurf_coef(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, do
   MOM_TRY(make_rule(nq, qw, qnodes, &rule));
   return nlsurf_coef_impl(ctx, mesh, nedges_total, omega, cd(ri_re, ri_im), epsp, x, chi2, rule, src_coef,
                           src_vec);
+}
+
+int mom_nlsurf_coef_adjoint(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                            double ri_im, const double *epsp, const double *x, const double *chi2, int32_t nq,
+                            const double *qw, const double *qnodes, const double *src_coef_bar,
+                            const double *src_vec_bar, double *xbar) {
+  if (!ctx || !mesh || !epsp || !x || !chi2 || !src_coef_bar || !src_vec_bar || !xbar)
+    return fail(MOM_ERR_ARG, "NULL argument");
+  if (mesh->ctx != ctx) return fail(MOM_ERR_ARG, "mesh belongs to another context");
+  if (!(omega > 0)) return fail(MOM_ERR_ARG, "omega must be > 0");
+  (void)hipSetDevice(ctx->device);
+  QuadRule rule;
+  MOM_TRY(make_rule(nq, qw, qnodes, &rule));
+  return nlsurf_coef_adjoint_impl(ctx, mesh, nedges_total, omega, cd(ri_re, ri_im), epsp, x, chi2, rule,
+                                  src_coef_bar, src_vec_bar, xbar);
 }
 
 int mom_sysmat_pmchwt_nls(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *shri,

@@ -30,7 +30,8 @@ EXPORTS = [
     "mom_problem_edge_bnd", "mom_symm_group", "mom_symm_edge_bc", "mom_system_create_order",
     "mom_system_fill_symm", "mom_srcvec_pw_symm", "mom_cs_prtsrf_symm", "mom_rcs_ext_symm",
     "mom_pmchwt_vertex_adjoint", "mom_cs_abs_vertex_explicit", "mom_ctx_fill_tuning", "mom_far_fields_noalloc",
-    "mom_ctx_fill_profile", "mom_mesh_classify_like",
+    "mom_ctx_fill_profile", "mom_mesh_classify_like", "mom_far_fields_adjoint", "mom_nlsurf_coef_adjoint",
+    "mom_domain_block_apply",
 ]
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
@@ -98,6 +99,10 @@ def lib():
     L.mom_ctx_fill_tuning.argtypes = [_vp, C.POINTER(_i32), _dp]
     L.mom_ctx_fill_profile.argtypes = [_vp, _i32, _dp]
     L.mom_mesh_classify_like.argtypes = [_vp, _vp]
+    L.mom_far_fields_adjoint.argtypes = [_vp, _vp, _i64, _d, _d, _d, _d, _i64, _dp, _dp, _dp, _i32, _dp, _dp, _dp]
+    L.mom_nlsurf_coef_adjoint.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _dp, _dp, _i32, _dp, _dp, _dp, _dp,
+                                          _dp]
+    L.mom_domain_block_apply.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _i32, _dp, _dp, _dp, _dp]
     L.mom_test_unit_lower_inverse.argtypes = [_vp, _vp, _i64, _vp]
     L.mom_cs_prtsrf.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _d, _d, _i32, _dp, _dp, C.POINTER(_d),
                                 C.POINTER(_d)]

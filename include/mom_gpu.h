@@ -343,6 +343,38 @@ int mom_system_fill_nls(mom_system *sys, int32_t ndom, const mom_mesh *const *do
                         const double *src_coef, int64_t ldsc, double *src, int64_t ldsrc);
 
 /*
+ * Transposes for adjoint sensitivities of the second-harmonic chain (no
+ * counterparts in the reference; convention as mom_pmchwt_vertex_adjoint:
+ * an observable s with ds = Re(g^T dx) has g the non-conjugated gradient).
+ *
+ * mom_far_fields_adjoint: for adjoint values v[8 ndir] (complex,
+ * interleaved) of the outputs (et, ep, ht, hp) of mom_far_fields with the
+ * same arguments, g[4N] = L^T v, where out = L x: Re(g^T dx) =
+ * Re(sum_d v_d . dout_d).  For s = sum_d W_d (|et|^2 + |ep|^2),
+ * v = (2 W conj(et), 2 W conj(ep), 0, 0).
+ *
+ * mom_nlsurf_coef_adjoint: the outputs of mom_nlsurf_coef are holomorphic
+ * and quadratic in x; for adjoint values src_coef_bar[4 nedges] and
+ * src_vec_bar[4 nedges] (complex, the outputs' layout) it returns
+ * xbar[4N] = J^T (src_coef_bar, src_vec_bar) with J the Jacobian at x
+ * (complex, non-conjugated).
+ *
+ * mom_domain_block_apply: y[4N] = Z_d x (trans = 0) or Z_d^T x (trans = 1)
+ * for the PMCHWT blocks of one domain mesh alone (sysmat.f90:56-98 for that
+ * domain, medium index ri), at the domain's unknowns (zero elsewhere).
+ */
+int mom_far_fields_adjoint(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                           double ri_im, double r, int64_t ndir, const double *theta, const double *phi,
+                           const double *v, int32_t nq, const double *qw, const double *qnodes, double *g);
+int mom_nlsurf_coef_adjoint(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                            double ri_im, const double *epsp, const double *x, const double *chi2, int32_t nq,
+                            const double *qw, const double *qnodes, const double *src_coef_bar,
+                            const double *src_vec_bar, double *xbar);
+int mom_domain_block_apply(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, double omega, double ri_re,
+                           double ri_im, int32_t trans, int32_t nq, const double *qw, const double *qnodes,
+                           const double *x, double *y);
+
+/*
  * Problem set-up on the host (no GPU work): the master mesh and the domain
  * meshes that the `mesh` and `sdom` commands build (interface.f90:357-366,
  * 44-105), identity group, no periodic boundaries.  Replaces build_mesh
