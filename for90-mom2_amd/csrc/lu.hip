@@ -839,15 +839,25 @@ static int transpose_b(const Lane &L, const C2 *B, int64_t ldb, int64_t K, int64
 }
 
 // C = beta C + sgn A B (sgn = +-1, beta = 0 or 1) by the register-staged
-// kernel, any shape.
+// kernel, any shape; grids of fewer than kSmallGrid tiles take 32 x 32
+// tiles instead.
+constexpr int64_t kSmallGrid = 128;
 template <typename T>
 static int gemm_generic(hipStream_t s, const typename Cx<T>::type *A, int64_t lda, const typename Cx<T>::type *B,
                         int64_t ldb, typename Cx<T>::type *C, int64_t ldc, int64_t M, int64_t N, int64_t K, T sgn,
                         bool beta0) {
   constexpr int WM = GemmCfg<T>::WM, WN = GemmCfg<T>::WN;
   const int gm = (int)((M + 32 * WM - 1) / (32 * WM)), gn = (int)((N + 32 * WN - 1) / (32 * WN));
-  hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn), dim3(256),
-                     0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn, sgn, beta0 ? 1 : 0);
+  if ((int64_t)gm * gn < kSmallGrid) {
+    // few tiles (the 256 x 256 diagonal-block products of the block
+    // solves): 32 x 32 tiles, four times the workgroups
+    const int gm1 = (int)((M + 31) / 32), gn1 = (int)((N + 31) / 32);
+    hipLaunchKernelGGL((gemm_sub_kernel<T, 1, 1, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm1 * gn1),
+                       dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm1, gn1, sgn, beta0 ? 1 : 0);
+  } else {
+    hipLaunchKernelGGL((gemm_sub_kernel<T, WM, WN, GemmCfg<T>::BK, GemmCfg<T>::M3>), dim3((unsigned)gm * gn),
+                       dim3(256), 0, s, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K, gm, gn, sgn, beta0 ? 1 : 0);
+  }
   MOM_HIP(hipGetLastError());
   return 0;
 }
