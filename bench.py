@@ -34,7 +34,7 @@ PEAK_FP64_MFMA = 78.6
 # solve, next-column and remaining-columns launches together): rocprofv3 FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, the factors read off a 1 GiB elementwise copy.  Recorded in
 # profiles/gemm3m_dma_kernel32_N61440_hbm_traffic.md; counters cannot be read in-process.
-TRAFFIC_S5_BYTES = 13.32e9
+TRAFFIC_S5_BYTES = 13.06e9
 PEAK_FP64_VALU = 78.6  # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0  # GB/s
 # Fill: SURVEY's algorithmic count of 60 flops per (test point, source point) pair.  The FP64 VALU
