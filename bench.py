@@ -217,6 +217,7 @@ def bench_sh(args):
 
     def step(prec):
         S = System(ctx, N, prec)
+        S.set_pivoting(1 if args.pivoting == "tournament" else 0)
         try:
             t = [time.perf_counter()]
             S.fill(dms, ri, omega)
@@ -249,7 +250,7 @@ def bench_sh(args):
            "vs_baseline": None, "dtype": "fp32-mfma LU + fp64 fill/residual", "data": "synthetic icosphere dimer",
            "config": {"workload": "SH dimer: two R=40nm icospheres s=%d, 10 nm gap, %d faces, N=%d unknowns, "
                                   "1060 nm / 530 nm, chi2_nnn=1" % (s, prob.master.nfaces, 2 * N),
-                      "n_unknowns": 2 * N},
+                      "n_unknowns": 2 * N, "complex64_pivoting": args.pivoting},
            "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fund", "sh", "rcs")}}
     if not args.no_fp64:
         r64 = step(MOM_PREC_FP64)
@@ -265,7 +266,7 @@ def bench_sh(args):
     bodies = [(px < 0, (-0.5e-9, 0.0, 0.0)), (px > 0, (0.5e-9, 0.0, 0.0))]  # per nm of gap
     t0 = time.perf_counter()
     gr = sh_power_translation_gradient(ctx, prob, dms, omega, ri, shri, nls, chi2, ang, bodies,
-                                       precision=MOM_PREC_MIXED)
+                                       precision=MOM_PREC_MIXED, pivoting=1 if args.pivoting == "tournament" else 0)
     tg = time.perf_counter() - t0
     out["gap_gradient"] = {"P_sh_grid": gr["P"], "dP_dgap_per_nm": gr["dP"], "seconds": round(tg, 3),
                            "terms": gr["terms"],
@@ -329,7 +330,8 @@ def bench_spectrum(args):
     dms = [DeviceMesh(ctx, dm) for dm in prob.domains]
 
     def sweep(prec, idx, keep=None):
-        X, info = solve_spectrum(ctx, dms, N, wls[list(idx)], ri_of, ang, precision=prec, keep=keep)
+        X, info = solve_spectrum(ctx, dms, N, wls[list(idx)], ri_of, ang, precision=prec, keep=keep,
+                                 pivoting=1 if args.pivoting == "tournament" else 0)
         t = {k: float(sum(r[k] for r in info)) for k in ("fill", "factor", "solve")}
         t["wall"] = t["fill"] + t["factor"] + t["solve"]
         return t, X, max(r["iters"] for r in info), max(r["berr"] for r in info), sum(r["fell_back"] for r in info)
@@ -356,7 +358,8 @@ def bench_spectrum(args):
            "vs_baseline": None, "dtype": "fp32-mfma LU + fp64 fill/residual", "data": "synthetic icosphere ellipsoid",
            "config": {"workload": "ellipsoid 80/40/40 nm from icosphere s=%d, %d faces, N=%d, %d wavelengths "
                                   "400-900 nm, Drude eps_in, %d plane waves per wavelength"
-                                  % (s, prob.master.nfaces, 2 * N, nwl, nrhs), "n_unknowns": 2 * N},
+                                  % (s, prob.master.nfaces, 2 * N, nwl, nrhs), "n_unknowns": 2 * N,
+                      "complex64_pivoting": args.pivoting},
            "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fill", "factor", "solve")},
            "per_wavelength_ms": round(mean / nwl * 1e3, 2)}
     if not args.no_fp64:
@@ -405,6 +408,7 @@ def bench_adjoint(args):
 
     def forward(e, prec):
         S = System(ctx, N, prec)
+        S.set_pivoting(1 if args.pivoting == "tournament" else 0)
         ri = [1.0 + 0j, np.sqrt(e)]
         S.fill(dms, ri, omega)
         S.factor()
@@ -438,7 +442,8 @@ def bench_adjoint(args):
            "vs_baseline": None, "dtype": "fp32-mfma LU + fp64 fill/residual", "data": "synthetic perturbed icosphere",
            "config": {"workload": "sphere R=60nm icosphere s=%d with radial l<=6 perturbation of 3 nm, %d faces, "
                                   "N=%d, 600 nm, eps_in=-9.4+1.5i; forward solve + d sigma_abs / d eps_in by the "
-                                  "adjoint" % (s, prob.master.nfaces, 2 * N), "n_unknowns": 2 * N},
+                                  "adjoint" % (s, prob.master.nfaces, 2 * N), "n_unknowns": 2 * N,
+                      "complex64_pivoting": args.pivoting},
            "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("forward", "adjoint")},
            "sigma_abs_m2": runs[-1]["sab"], "dsigma_deps": list(runs[-1]["grad"])}
     # the node-coordinate gradient of sigma_abs (all 3V coordinates), reported beside the value
@@ -689,6 +694,9 @@ def main():
                          "complex128 PMCHWT fill of the default sphere alone (for counter runs)")
     ap.add_argument("--cell-n", type=int, default=50, help="octasphere divisions of the symmetric workload")
     ap.add_argument("--nwl", type=int, default=64, help="wavelengths of the spectrum workload")
+    ap.add_argument("--pivoting", choices=("tournament", "partial"), default="tournament",
+                    help="complex64 panels of the mixed path in the spectrum, sh and adjoint workloads (the library "
+                         "default, and the pmchwt workload's, is partial pivoting)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)

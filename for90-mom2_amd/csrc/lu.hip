@@ -966,6 +966,8 @@ struct PivotWork {
   int *info;     // first zero pivot (1-based), 0 if none
   int *plan;     // laswp plan of one panel: 1 + 4 NB ints
   int *tsel;     // tournament candidate sets: [2][mb][IB] row indices
+  float2 *lu_top;  // tournament: L11 U11 of the inner block by rows, IB x IB
+  int tournament;  // complex64 panels by tournament pivoting (the system's choice or the test hook)
 };
 
 
@@ -1445,8 +1447,9 @@ __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *
 // After the replay slot s holds the old content of slot idx[s]; the plan is
 // the list of (destination row, source row) pairs, -1 for unchanged slots:
 // plan[0] = 2 jb, plan[1 + e] = destination, plan[1 + 2 NB + e] = source.
-__global__ __launch_bounds__(256) void laswp_plan_kernel(const int *ipiv, int k1, int k2, int n, int *plan) {
-  __shared__ int pv[NB], first[NB], idx[2 * NB];
+// The plan of interchanges k1..k2-1 by the 256 threads of one workgroup
+// (pv[NB], first[NB], idx[2 NB] in LDS).
+__device__ void laswp_plan_block(const int *ipiv, int k1, int k2, int n, int *plan, int *pv, int *first, int *idx) {
   const int tid = threadIdx.x, jb = k2 - k1;
   if (tid < jb) {
     // a pivot row of column k lies in [k, n); anything else (a factorisation
@@ -1491,6 +1494,11 @@ __global__ __launch_bounds__(256) void laswp_plan_kernel(const int *ipiv, int k1
     plan[1 + e] = dst;
     plan[1 + 2 * NB + e] = src;
   }
+}
+
+__global__ __launch_bounds__(256) void laswp_plan_kernel(const int *ipiv, int k1, int k2, int n, int *plan) {
+  __shared__ int pv[NB], first[NB], idx[2 * NB];
+  laswp_plan_block(ipiv, k1, k2, n, plan, pv, first, idx);
 }
 
 // Applies a laswp plan to columns [c0, c1) and [c2, c3): each block gathers
@@ -1701,125 +1709,38 @@ __global__ __launch_bounds__(256) void unit_lower_inverse_kernel(const float2 *_
 // ellipsoid its panels took longer than the cooperative partial-pivoting
 // panel (one wave runs each 32-step selection serially, ~85 us per level).
 //
-// Selection, one wave per group of 128 rows (two per lane, registers
-// only, no barriers): level 0 takes 128 consecutive rows of the block, a
-// later level up to four candidate sets of ib rows (their original values
-// read again from A).  Partial pivoting over the group -- pivot search by a
-// wave argmax of |re| + |im| with ties to the lowest row as IZAMAX, pivot
-// row broadcast with readlane -- writes the ib rows chosen, in pivot order,
-// as the group's candidate set (-1 where the group had fewer rows).  A NaN
-// magnitude counts as 0, so such rows only win columns without any other
-// nonzero candidate.
-constexpr int kTsluRows = 128;                // rows per wave
-constexpr int kTsluSets = kTsluRows / IB;     // candidate sets merged per wave
+// Selection, one workgroup of 256 threads per group of 256 rows, one row
+// per thread in registers: level 0 takes 256 consecutive rows of the
+// block, a later level up to eight candidate sets of ib rows (their
+// original values read again from A).  Partial pivoting over the group --
+// pivot search by a workgroup argmax of |re| + |im| with ties to the lowest
+// row as IZAMAX, pivot row broadcast through LDS -- writes the ib rows
+// chosen, in pivot order, as the group's candidate set (-1 where the group
+// had fewer rows).  A NaN magnitude counts as 0, so such rows only win
+// columns without any other nonzero candidate.
+constexpr int kTsluRows = 256;                // rows per workgroup
+constexpr int kTsluSets = kTsluRows / IB;     // candidate sets merged per workgroup
 
 __device__ __forceinline__ float2 cdiv_f(float2 a, float2 b) {
   if (b.x == 0.f && b.y == 0.f) return a;
   const float den = b.x * b.x + b.y * b.y;
   return make_float2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
 }
-__global__ __launch_bounds__(256) void tslu_select_kernel(const float2 *__restrict__ A, int64_t ld, int i0, int ib,
-                                                          int n, const int *__restrict__ in, int nsets, int ngroups,
-                                                          int *__restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const int grp = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (grp >= ngroups) return;
-  int row[2];
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    row[h] = -1;
-    const int slot = lane + 64 * h;
-    if (!in) {
-      const int r = i0 + kTsluRows * grp + slot;
-      if (r < n) row[h] = r;
-    } else {
-      const int set = grp * kTsluSets + slot / IB;
-      if (set < nsets && slot % IB < ib) row[h] = in[set * IB + slot % IB];
-    }
-  }
-  float2 v0[IB], v1[IB];  // the lane's two rows
-#pragma unroll
-  for (int s = 0; s < IB; s++) {
-    v0[s] = (row[0] >= 0 && s < ib) ? A[(int64_t)(i0 + s) * ld + row[0]] : make_float2(0.f, 0.f);
-    v1[s] = (row[1] >= 0 && s < ib) ? A[(int64_t)(i0 + s) * ld + row[1]] : make_float2(0.f, 0.f);
-  }
-  bool act0 = row[0] >= 0, act1 = row[1] >= 0, done = false;
-#pragma clang loop unroll(full)
-  for (int k = 0; k < IB; k++) {
-    if (k >= ib || done) continue;
-    // this lane's best row, then the wave's
-    float val = -1.f;
-    int idx = 0x7fffffff;
-    {
-      float c0 = fabsf(v0[k].x) + fabsf(v0[k].y), c1 = fabsf(v1[k].x) + fabsf(v1[k].y);
-      if (!(c0 == c0)) c0 = 0.f;
-      if (!(c1 == c1)) c1 = 0.f;
-      if (act0) {
-        val = c0;
-        idx = row[0];
-      }
-      if (act1 && (c1 > val || (c1 == val && row[1] < idx))) {
-        val = c1;
-        idx = row[1];
-      }
-    }
-    float bv = val;
-    int bi = idx;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const float v2 = __shfl_xor(bv, o);
-      const int i2 = __shfl_xor(bi, o);
-      if (v2 > bv || (v2 == bv && i2 < bi)) {
-        bv = v2;
-        bi = i2;
-      }
-    }
-    if (!(bv >= 0.f)) {  // no rows left in this group
-      if (lane == 0)
-        for (int q = k; q < ib; q++) out[grp * IB + q] = -1;
-      done = true;
-      continue;
-    }
-    const bool m0 = act0 && row[0] == bi, m1 = act1 && row[1] == bi;
-    const int wl = __builtin_ffsll((long long)__ballot(m0 || m1)) - 1;  // lane holding the pivot row
-    const int wh = __builtin_amdgcn_readlane(m1 ? 1 : 0, wl);
-    if (lane == 0) out[grp * IB + k] = bi;
-    if (m0) act0 = false;
-    if (m1) act1 = false;
-    float2 prow[IB];
-#pragma unroll
-    for (int s = 0; s < IB; s++) {
-      if (s < k || s >= ib) continue;
-      // both rows read out, then the uniform choice
-      const float2 p0 = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v0[s].x), wl)),
-                                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v0[s].y), wl)));
-      const float2 p1 = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v1[s].x), wl)),
-                                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v1[s].y), wl)));
-      prow[s] = wh ? p1 : p0;
-    }
-    const float2 l0 = cdiv_f(v0[k], prow[k]), l1 = cdiv_f(v1[k], prow[k]);
-#pragma unroll
-    for (int s = 0; s < IB; s++) {
-      if (s <= k || s >= ib) continue;
-      if (act0) {
-        v0[s].x -= l0.x * prow[s].x - l0.y * prow[s].y;
-        v0[s].y -= l0.x * prow[s].y + l0.y * prow[s].x;
-      }
-      if (act1) {
-        v1[s].x -= l1.x * prow[s].x - l1.y * prow[s].y;
-        v1[s].y -= l1.x * prow[s].y + l1.y * prow[s].x;
-      }
-    }
-  }
+
+// Key of a candidate: magnitude bits (non-negative floats order as their
+// bits) above the complemented row, so the largest key is the largest
+// magnitude with the lowest row; 0 for no candidate.
+__device__ __forceinline__ unsigned long long tslu_key(float mag, int row) {
+  return ((unsigned long long)__float_as_uint(mag) << 32) | (unsigned long long)(0xffffffffu - (unsigned)row);
 }
 
 // LAPACK-style interchanges of the chosen rows sel[0..ib) into positions
-// i0..i0+ib-1 (one wave; lane s keeps slot s of the positions moved so far
+// i0..i0+ib-1 (by one whole wave; lane s keeps slot s of the positions moved so far
 // and their rows): ipiv[i0 + k] = 1 + the position row sel[k] occupies after
 // the interchanges of steps < k.  A row below the block is still at its own
 // position until it is chosen; rows of the block may have moved.
-__global__ __launch_bounds__(64) void tslu_pivots_kernel(const int *__restrict__ sel, int i0, int ib, int *ipiv) {
-  const int lane = threadIdx.x;
+__device__ void tslu_pivots_wave(const int *__restrict__ sel, int i0, int ib, int *ipiv) {
+  const int lane = threadIdx.x & 63;
   int mpos = -1, mrow = -1, np = 0;
   for (int k = 0; k < ib; k++) {
     const int r = sel[k];
@@ -1844,75 +1765,169 @@ __global__ __launch_bounds__(64) void tslu_pivots_kernel(const int *__restrict__
   }
 }
 
-// The inner block after its interchanges, factored without pivoting, one
-// wave per 64 rows: lanes 0..ib-1 form L11 U11 of the top ib x ib block in
-// registers (pivot rows broadcast with readlane) and every lane's row below
-// the block solves l U11 = a_row in place (top = 0); a second launch of one
-// wave (top = 1) stores L11 U11 over the top block once every row has read
-// it.  A zero pivot is recorded in info (first one) and skipped, as the
-// partial pivoting panel does.
-__global__ __launch_bounds__(256) void tslu_factor_kernel(float2 *A, int64_t ld, int i0, int ib, int n, int *info,
-                                                          int top) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  float2 u[IB];
+// Wave maximum of 64-bit keys through DPP (quad permutations, row
+// rotations, row broadcasts; lane 63 ends with the maximum of all lanes),
+// read back to every lane.
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long k) {
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)k, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(k >> 32), CTRL, 0xf, 0xf, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+  unsigned long long t;
+  t = dpp_u64<0xb1>(k);  // quad_perm [1, 0, 3, 2]
+  k = t > k ? t : k;
+  t = dpp_u64<0x4e>(k);  // quad_perm [2, 3, 0, 1]
+  k = t > k ? t : k;
+  t = dpp_u64<0x124>(k);  // row_ror:4
+  k = t > k ? t : k;
+  t = dpp_u64<0x128>(k);  // row_ror:8
+  k = t > k ? t : k;
+  t = dpp_u64<0x142>(k);  // row_bcast:15
+  k = t > k ? t : k;
+  t = dpp_u64<0x143>(k);  // row_bcast:31
+  k = t > k ? t : k;
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)k, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// One barrier per column: each wave publishes its best key and that row
+// (from the lane holding it) before the barrier, every thread then takes
+// the best of the four; the buffers alternate by column parity, so a wave
+// that runs ahead writes the other parity while the rest still read.
+// With lu_top (the last level, one group) the winner of step k also stores
+// its row -- multipliers left of k, U entries from k on -- as row k of the
+// block's L11 U11, and a zero pivot is recorded in info (first one).
+__global__ __launch_bounds__(256) void tslu_select_kernel(const float2 *__restrict__ A, int64_t ld, int i0, int ib,
+                                                          int n, const int *__restrict__ in, int nsets, int ngroups,
+                                                          int *__restrict__ out, float2 *__restrict__ lu_top,
+                                                          int *__restrict__ info, int *__restrict__ ipiv,
+                                                          int *__restrict__ plan) {
+  __shared__ unsigned long long part[2][4];
+  __shared__ float2 prow[2][4][IB];
+  __shared__ int pv[NB], first[NB], idx[2 * NB];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int grp = blockIdx.x;
+  int row = -1;
+  if (!in) {
+    const int r = i0 + kTsluRows * grp + tid;
+    if (r < n) row = r;
+  } else {
+    const int set = grp * kTsluSets + tid / IB;
+    if (set < nsets && tid % IB < ib) row = in[set * IB + tid % IB];
+  }
+  float2 v[IB];
 #pragma unroll
-  for (int s = 0; s < IB; s++) u[s] = (lane < ib && s < ib) ? A[(int64_t)(i0 + s) * ld + i0 + lane] : make_float2(0.f, 0.f);
-#pragma unroll
+  for (int s = 0; s < IB; s++)
+    v[s] = (row >= 0 && s < ib) ? A[(int64_t)(i0 + s) * ld + row] : make_float2(0.f, 0.f);
+  bool act = row >= 0, done = false;
+  // fully unrolled (static register indices); k < ib and done are uniform
+#pragma clang loop unroll(full)
   for (int k = 0; k < IB; k++) {
-    if (k >= ib) continue;
-    float2 pr[IB];
+    if (k >= ib || done) continue;
+    const int par = k & 1;
+    float c = fabsf(v[k].x) + fabsf(v[k].y);
+    if (!(c == c)) c = 0.f;
+    const unsigned long long key = act ? tslu_key(c, row) : 0ull;
+    const unsigned long long wmax = wave_max_u64(key);
+    // columns s >= ib hold zeros in every row and stay zero, so the loops
+    // below need no bound on s other than the static k
+    if (wmax != 0ull && key == wmax) {
 #pragma unroll
-    for (int s = 0; s < IB; s++)
-      pr[s] = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].x), k)),
-                          __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].y), k)));
-    if (top && lane == 0 && pr[k].x == 0.f && pr[k].y == 0.f) atomicCAS(info, 0, i0 + k + 1);
-    if (lane > k && lane < ib) {
-      const float2 l = cdiv_f(u[k], pr[k]);
-      u[k] = l;
+      for (int s = 0; s < IB; s++)
+        if (s >= k) prow[par][wv][s] = v[s];
+    }
+    if (lane == 0) part[par][wv] = wmax;
+    __syncthreads();
+    unsigned long long best = part[par][0];
+    int bw = 0;
+#pragma unroll
+    for (int q = 1; q < 4; q++)
+      if (part[par][q] > best) {
+        best = part[par][q];
+        bw = q;
+      }
+    if (best == 0ull) {  // no rows left in this group (uniform)
+      if (tid == 0)
+        for (int q = k; q < ib; q++) out[grp * IB + q] = -1;
+      done = true;
+      continue;
+    }
+    const int bi = (int)(0xffffffffu - (unsigned)(best & 0xffffffffull));
+    if (act && row == bi) {
+      out[grp * IB + k] = bi;
+      act = false;
+      if (lu_top) {
+#pragma unroll
+        for (int s = 0; s < IB; s++) lu_top[k * IB + s] = v[s];
+        if (v[k].x == 0.f && v[k].y == 0.f) atomicCAS(info, 0, i0 + k + 1);
+      }
+    }
+    if (act) {
+      const float2 l = cdiv_f(v[k], prow[par][bw][k]);
+      v[k] = l;
 #pragma unroll
       for (int s = 0; s < IB; s++) {
-        if (s <= k || s >= ib) continue;
-        u[s].x -= l.x * pr[s].x - l.y * pr[s].y;
-        u[s].y -= l.x * pr[s].y + l.y * pr[s].x;
+        if (s <= k) continue;
+        const float2 u = prow[par][bw][s];
+        v[s].x -= l.x * u.x - l.y * u.y;
+        v[s].y -= l.x * u.y + l.y * u.x;
       }
     }
   }
-  if (top) {
-    if (blockIdx.x == 0 && threadIdx.x < 64 && lane < ib)
-#pragma unroll
-      for (int s = 0; s < IB; s++)
-        if (s < ib) A[(int64_t)(i0 + s) * ld + i0 + lane] = u[s];
-    return;
+  if (lu_top) {
+    // the last level also turns its choice into LAPACK-style interchanges
+    // and their laswp plan (out and then ipiv written by this workgroup)
+    __syncthreads();
+    if (wv == 0) tslu_pivots_wave(out, i0, ib, ipiv);
+    __syncthreads();
+    laswp_plan_block(ipiv, i0, i0 + ib, n, plan, pv, first, idx);
   }
-  const int64_t row = (int64_t)i0 + ib + 64 * wave + lane;
+}
+
+// The inner block after its interchanges: the last selection level has
+// factored the chosen rows in pivot order (lu_top, ib x ib, L11 U11 by
+// rows); one launch stores it over the top block (workgroup 0) and solves
+// l U11 = a_row in place for every row below the block, one per thread,
+// with U11 from lu_top in LDS.  A zero pivot is skipped, as the partial
+// pivoting panel does.
+__global__ __launch_bounds__(256) void tslu_below_kernel(float2 *A, int64_t ld, int i0, int ib, int n,
+                                                         const float2 *__restrict__ lu_top) {
+  __shared__ float2 U[IB][IB + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < IB * IB; e += 256) {
+    const int r = e / IB, c = e % IB;
+    const float2 x = (r < ib && c < ib) ? lu_top[r * IB + c] : make_float2(0.f, 0.f);
+    U[r][c] = r <= c ? x : make_float2(0.f, 0.f);
+    if (blockIdx.x == 0 && r < ib && c < ib) A[(int64_t)(i0 + c) * ld + i0 + r] = x;
+  }
+  __syncthreads();
+  const int64_t row = (int64_t)i0 + ib + (int64_t)blockIdx.x * 256 + tid;
   if (row >= n) return;
   float2 a[IB];
 #pragma unroll
   for (int s = 0; s < IB; s++) a[s] = s < ib ? A[(int64_t)(i0 + s) * ld + row] : make_float2(0.f, 0.f);
 #pragma unroll
   for (int s = 0; s < IB; s++) {
-    if (s >= ib) continue;
     float2 x = a[s];
 #pragma unroll
     for (int t = 0; t < IB; t++) {
       if (t >= s) continue;
-      const float2 ut = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].x), t)),
-                                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].y), t)));
-      x.x -= a[t].x * ut.x - a[t].y * ut.y;
-      x.y -= a[t].x * ut.y + a[t].y * ut.x;
+      const float2 u = U[t][s];
+      x.x -= a[t].x * u.x - a[t].y * u.y;
+      x.y -= a[t].x * u.y + a[t].y * u.x;
     }
-    const float2 d = make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].x), s)),
-                                 __int_as_float(__builtin_amdgcn_readlane(__float_as_int(u[s].y), s)));
-    a[s] = cdiv_f(x, d);
+    a[s] = cdiv_f(x, U[s][s]);  // columns s >= ib: zero over a zero pivot, kept zero
   }
 #pragma unroll
   for (int s = 0; s < IB; s++)
     if (s < ib) A[(int64_t)(i0 + s) * ld + row] = a[s];
 }
 
-// complex64 panels by partial pivoting as the complex128 ones (default) or,
-// switched on by the test hook, by tournament pivoting
+// the test hook's process-wide switch to tournament pivoting for complex64
+// panels (a system chooses its own with mom_system_set_pivoting)
 static int g_tournament = 0;
 
 // Cooperative inner-block panel factorisation: synchronisation words, the
@@ -1942,32 +1957,30 @@ static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int6
     Timed tp(st, s, kPhPanel);
     const int64_t G = (n - i0 + 255) / 256;
     bool coop_done = false, tournament = false;
-    if (std::is_same<T, float>::value && g_tournament) {
+    if (std::is_same<T, float>::value && w.tournament) {
       // tournament over the rows, interchanges on every panel column, then
       // the block without pivoting
       int *cur = w.tsel, *nxt = w.tsel + (size_t)w.mb * IB;
       int64_t groups = (n - i0 + kTsluRows - 1) / kTsluRows;
       if constexpr (std::is_same<T, float>::value) {
-        hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, s, A, ld, (int)i0,
-                           (int)ib, (int)n, (const int *)nullptr, 0, (int)groups, cur);
+        // the last level (one group) also factors the chosen rows into lu_top
+        hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)groups), dim3(256), 0, s, A, ld, (int)i0,
+                           (int)ib, (int)n, (const int *)nullptr, 0, (int)groups, cur,
+                           groups == 1 ? w.lu_top : (float2 *)nullptr, w.info, ipiv, w.plan);
         while (groups > 1) {
           const int64_t gn = (groups + kTsluSets - 1) / kTsluSets;
-          hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)((gn + 3) / 4)), dim3(256), 0, s, A, ld, (int)i0,
-                             (int)ib, (int)n, (const int *)cur, (int)groups, (int)gn, nxt);
+          hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)gn), dim3(256), 0, s, A, ld, (int)i0,
+                             (int)ib, (int)n, (const int *)cur, (int)groups, (int)gn, nxt,
+                             gn == 1 ? w.lu_top : (float2 *)nullptr, w.info, ipiv, w.plan);
           std::swap(cur, nxt);
           groups = gn;
         }
-        hipLaunchKernelGGL(tslu_pivots_kernel, dim3(1), dim3(64), 0, s, cur, (int)i0, (int)ib, ipiv);
-        hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)i0, (int)(i0 + ib), (int)n,
-                           w.plan);
         const int64_t lg = (jb + kLaswpCols - 1) / kLaswpCols;
         hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)lg), dim3(256), 0, s, A, ld, w.plan, (int)j0,
                            (int)(j0 + jb), 0, 0);
         const int64_t below = n - i0 - ib;
-        if (below > 0)
-          hipLaunchKernelGGL(tslu_factor_kernel, dim3((unsigned)((below + 255) / 256)), dim3(256), 0, s, A, ld,
-                             (int)i0, (int)ib, (int)n, w.info, 0);
-        hipLaunchKernelGGL(tslu_factor_kernel, dim3(1), dim3(64), 0, s, A, ld, (int)i0, (int)ib, (int)n, w.info, 1);
+        hipLaunchKernelGGL(tslu_below_kernel, dim3((unsigned)std::max<int64_t>(1, (below + 255) / 256)), dim3(256),
+                           0, s, A, ld, (int)i0, (int)ib, (int)n, w.lu_top);
       }
       MOM_HIP(hipGetLastError());
       tournament = true;
@@ -2822,6 +2835,7 @@ struct mom_system {
   double be_hist[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // backward error after each outer step
   int coop_gmax[2] = {-1, -1};
   bool coop_refused = false;  // a cooperative launch was refused: panels by one launch per column
+  int pivoting = 0;  // complex64 panels: 0 partial pivoting, 1 tournament (mom_system_set_pivoting)
 };
 
 namespace mom {
@@ -2830,8 +2844,9 @@ static int ensure_pivwork(mom_system *S, PivotWork *w) {
   const int64_t mb = (S->n + 255) / 256 + 1;
   const size_t rowbytes = 2 * mb * IB * sizeof(double2);
   const size_t planb = ((1 + 4 * NB) * sizeof(int) + 255) / 256 * 256;
-  const size_t bytes =
-      rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64 + planb + 2 * mb * IB * sizeof(int);
+  const size_t tselb = (2 * mb * IB * sizeof(int) + 255) / 256 * 256;
+  const size_t bytes = rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64 + planb + tselb +
+                       IB * IB * sizeof(float2);
   if (S->pivwork.bytes < bytes) {
     int rc = S->pivwork.alloc(bytes);
     if (rc) return rc;
@@ -2843,7 +2858,9 @@ static int ensure_pivwork(mom_system *S, PivotWork *w) {
   w->info = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)));
   w->plan = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64);
   w->tsel = (int *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64 + planb);
+  w->lu_top = (float2 *)(base + rowbytes + 2 * mb * (sizeof(double) + sizeof(int)) + 64 + planb + tselb);
   w->mb = (int)mb;
+  w->tournament = (S->pivoting == 1 || g_tournament) ? 1 : 0;
   MOM_HIP(hipMemsetAsync(w->info, 0, sizeof(int), S->ctx->stream));
   return 0;
 }
@@ -3291,6 +3308,13 @@ int mom_system_destroy(mom_system *sys) {
   (void)hipSetDevice(sys->ctx->device);
   (void)hipStreamSynchronize(sys->ctx->stream);
   delete sys;
+  return 0;
+}
+
+int mom_system_set_pivoting(mom_system *sys, int32_t mode) {
+  if (!sys) return fail(MOM_ERR_ARG, "NULL argument");
+  if (mode != 0 && mode != 1) return fail(MOM_ERR_ARG, "pivoting must be 0 (partial) or 1 (tournament)");
+  sys->pivoting = mode;
   return 0;
 }
 

@@ -244,6 +244,11 @@ class System:
         Av = np.asfortranarray(A, dtype=np.complex128).reshape(-1, order="F").view(np.float64)
         capi.check(capi.lib().mom_system_set_matrix(self.h, np.ascontiguousarray(Av), N2))
 
+    def set_pivoting(self, mode):
+        """Pivoting of the complex64 factorisation (mixed path): 0 partial as
+        LAPACK (default), 1 tournament (mom_system_set_pivoting)."""
+        capi.check(capi.lib().mom_system_set_pivoting(self.h, int(mode)))
+
     def factor(self):
         st = capi.lib().mom_system_factor(self.h)
         if st > 0:

@@ -233,7 +233,8 @@ def sh_power(ctx, problem, dmeshes, omega, ri_list, shri_list, nls, chi2, angles
 
 
 def sh_power_translation_gradient(ctx, problem, dmeshes, omega, ri_list, shri_list, nls, chi2, angles, bodies,
-                                  precision=capi.MOM_PREC_FP64, tol=1e-12, ntheta=37, nphi=72, quad=None):
+                                  precision=capi.MOM_PREC_FP64, tol=1e-12, ntheta=37, nphi=72, quad=None,
+                                  pivoting=0):
     """SH power radiated into the rcs_ext grid, P = sum_d W_d (|et|^2 +
     |ep|^2) at 2 omega in domain 1 (power_grid weights), and its derivative
     with respect to a parameter that translates bodies rigidly: `bodies` is
@@ -257,7 +258,8 @@ def sh_power_translation_gradient(ctx, problem, dmeshes, omega, ri_list, shri_li
     their body, so the parameter must move every nonlinear domain rigidly
     (exact up to the quadrature table's 3e-8 barycentric defect).  The
     near/far classification is held at that of the unperturbed meshes.
-    Returns dict(P, dP, terms, x1, x2, grid)."""
+    pivoting: of the complex64 factorisations in the mixed path
+    (System.set_pivoting).  Returns dict(P, dP, terms, x1, x2, grid)."""
     from .gpu import far_fields
 
     Q, w, n = _quad(quad)
@@ -268,6 +270,8 @@ def sh_power_translation_gradient(ctx, problem, dmeshes, omega, ri_list, shri_li
     th, ph, W = power_grid(ntheta, nphi)
     S1 = System(ctx, N, precision)
     S2 = System(ctx, N, precision)
+    S1.set_pivoting(pivoting)
+    S2.set_pivoting(pivoting)
     try:
         P, x1, x2, et, ep = _sh_forward(ctx, problem, dmeshes, omega, ri_list, shri_list, nls, chi2, ang, S1, S2, tol,
                                         th, ph, W, quad)

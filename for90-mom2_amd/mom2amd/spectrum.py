@@ -51,16 +51,18 @@ def plane_waves_deg(ndir=128, seed=20251203, psis=(0.0, 90.0)):
 
 
 def solve_spectrum(ctx, dmeshes, nedges_total, wls, ri_of, angles_deg, precision=capi.MOM_PREC_MIXED, tol=1e-12,
-                   maxit=30, keep=None):
+                   maxit=30, keep=None, pivoting=0):
     """For every wavelength wl in wls: fill with ri_of(wl) (one index per
     domain), factor, and solve all plane waves (angles in degrees, converted
     with the reference's degtorad) in one block in domain 1's medium.
     Returns (X, info): X[i] the 2N x nsrc solutions of wavelength i (kept
     only for i in `keep` when given), info per wavelength: fill/factor/solve
     seconds, refinement steps, backward error and whether the block solve
-    fell back to FP64 factors."""
+    fell back to FP64 factors.  pivoting: of the complex64 factorisation in
+    the mixed path (0 partial, 1 tournament; System.set_pivoting)."""
     ang = np.asarray(angles_deg, dtype=np.float64) * DEGTORAD
     S = System(ctx, nedges_total, precision)
+    S.set_pivoting(pivoting)
     X, info = {}, []
     try:
         for i, wl in enumerate(wls):

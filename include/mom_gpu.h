@@ -230,6 +230,16 @@ int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, dou
                         double ri_im, int64_t nsrc, const double *angles, int32_t nq,
                         const double *qw, const double *qnodes, double tol, int32_t maxit,
                         double *X, int64_t ldx, int32_t *iters, double *berr);
+/* Pivoting of the complex64 factorisation of a MOM_PREC_MIXED system (no
+ * counterpart in the reference): 0 = partial pivoting as LAPACK's ZGETRF
+ * (the default), 1 = tournament pivoting (communication-avoiding LU: the
+ * pivot rows of each 32-column block chosen by partial pivoting inside
+ * groups of 256 rows and a reduction tree over their choices, one launch
+ * per tree level instead of one exchange per column).  The pivots differ
+ * from LAPACK's; the complex64 factors only precondition the complex128
+ * refinement, so the solution is the same to the refinement's tolerance.
+ * The complex128 factorisation always pivots partially. */
+int mom_system_set_pivoting(mom_system *sys, int32_t mode);
 /* Event timing of the kernel groups of mom_system_factor (reported by
  * mom_system_stats): 0 off, 1 the depth-256 trailing updates only (cheap
  * enough for timed runs), 2 every group (adds host work per launch). */

@@ -76,9 +76,9 @@ int mom_test_vertex_fd(int32_t enable);
  * Process-wide; lets the tests run that branch on any matrix. */
 int mom_test_batch_fallback(int32_t force);
 
-/* Panels of the complex64 factorisation (mixed path): 0 = partial pivoting
- * as LAPACK (the default), 1 = tournament pivoting (communication-avoiding
- * LU; the complex128 path always uses partial pivoting).  Process-wide. */
+/* Panels of the complex64 factorisation (mixed path) of every system:
+ * 1 = tournament pivoting whatever the system chose (mom_system_set_pivoting),
+ * 0 = each system's own choice.  Process-wide. */
 int mom_test_tournament(int32_t enable);
 
 #ifdef __cplusplus

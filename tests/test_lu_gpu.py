@@ -213,9 +213,9 @@ def test_block_refinement_falls_back_to_fp64(gpu_ctx):
 
 @pytest.mark.parametrize("n2", [66, 1100, 3000])
 def test_tournament_panels(gpu_ctx, n2):
-    """The complex64 factorisation with tournament pivoting (the mixed
-    path's opt-in panel mode) against the same system factored with partial
-    pivoting (the default):
+    """The complex64 factorisation with tournament pivoting
+    (mom_system_set_pivoting 1) against the same system factored with
+    partial pivoting (the default):
     both first solutions have the backward error of a complex64 LU, and the
     refined solutions agree with LAPACK's to the same backward error.  Sizes
     cover a single partial inner block, several panels with a partial last
@@ -227,16 +227,15 @@ def test_tournament_panels(gpu_ctx, n2):
     x_ref = np.linalg.solve(A, b)
     out = {}
     for mode in (1, 0):
-        capi.check(capi.lib().mom_test_tournament(mode))
+        S = System(gpu_ctx, n2 // 2, MOM_PREC_MIXED)
         try:
-            S = System(gpu_ctx, n2 // 2, MOM_PREC_MIXED)
+            S.set_pivoting(mode)
             S.set_matrix(A)
             S.factor()
             x = S.solve(b, tol=1e-13, maxit=20)
             st = S.stats()
-            S.close()
         finally:
-            capi.check(capi.lib().mom_test_tournament(0))
+            S.close()
         hist = st["berr_history"]
         print("n=%d tournament=%d: backward errors %s" % (n2, mode, ["%.1e" % v for v in hist]), flush=True)
         assert hist[0] < 1e-4, hist
