@@ -1447,8 +1447,8 @@ __global__ __launch_bounds__(256) void panel_block_kernel(typename Cx<T>::type *
 // After the replay slot s holds the old content of slot idx[s]; the plan is
 // the list of (destination row, source row) pairs, -1 for unchanged slots:
 // plan[0] = 2 jb, plan[1 + e] = destination, plan[1 + 2 NB + e] = source.
-// The plan of interchanges k1..k2-1 by the 256 threads of one workgroup
-// (pv[NB], first[NB], idx[2 NB] in LDS).
+// The plan of interchanges k1..k2-1 by the threads of one workgroup (at
+// least NB of them; pv[NB], first[NB], idx[2 NB] in LDS).
 __device__ void laswp_plan_block(const int *ipiv, int k1, int k2, int n, int *plan, int *pv, int *first, int *idx) {
   const int tid = threadIdx.x, jb = k2 - k1;
   if (tid < jb) {
@@ -1457,7 +1457,8 @@ __device__ void laswp_plan_block(const int *ipiv, int k1, int k2, int n, int *pl
     const int p = ipiv[k1 + tid] - 1;
     pv[tid] = (p >= k1 + tid && p < n) ? p : k1 + tid;
   }
-  for (int q = tid; q < 2 * NB; q += 256) idx[q] = q;
+  const int nt = blockDim.x;
+  for (int q = tid; q < 2 * NB; q += nt) idx[q] = q;
   __syncthreads();
   if (tid < jb) {
     int f = tid;
@@ -1482,7 +1483,7 @@ __device__ void laswp_plan_block(const int *ipiv, int k1, int k2, int n, int *pl
     plan[0] = 2 * jb;
   }
   __syncthreads();
-  for (int e = tid; e < 2 * jb; e += 256) {
+  for (int e = tid; e < 2 * jb; e += nt) {
     int dst = -1, src = -1;
     const bool live = e < jb || (first[e - jb] == e - jb && pv[e - jb] >= k2);
     if (live) {
@@ -1709,29 +1710,27 @@ __global__ __launch_bounds__(256) void unit_lower_inverse_kernel(const float2 *_
 // ellipsoid its panels took longer than the cooperative partial-pivoting
 // panel (one wave runs each 32-step selection serially, ~85 us per level).
 //
-// Selection, one workgroup of 256 threads per group of 256 rows, one row
-// per thread in registers: level 0 takes 256 consecutive rows of the
-// block, a later level up to eight candidate sets of ib rows (their
+// Selection, one workgroup of 256 threads per group of kTsluRows = 256
+// rows, one row per thread in registers: level 0 takes 256 consecutive rows
+// of the block, a later level up to eight candidate sets of ib rows (their
 // original values read again from A).  Partial pivoting over the group --
 // pivot search by a workgroup argmax of |re| + |im| with ties to the lowest
 // row as IZAMAX, pivot row broadcast through LDS -- writes the ib rows
 // chosen, in pivot order, as the group's candidate set (-1 where the group
 // had fewer rows).  A NaN magnitude counts as 0, so such rows only win
-// columns without any other nonzero candidate.
-constexpr int kTsluRows = 256;                // rows per workgroup
-constexpr int kTsluSets = kTsluRows / IB;     // candidate sets merged per workgroup
+// columns without any other nonzero candidate.  (Groups of 1024 rows, two
+// per thread, make the tree one level shallower but each column step costs
+// more than 1.5 times as much: slower on the 15360-unknown ellipsoid.)
+constexpr int kTsluThreads = 256;
+constexpr int kTsluRpt = 1;                              // rows per thread
+constexpr int kTsluRows = kTsluThreads * kTsluRpt;       // rows per workgroup
+constexpr int kTsluSets = kTsluRows / IB;                // candidate sets merged per workgroup
+constexpr int kTsluWaves = kTsluThreads / 64;
 
 __device__ __forceinline__ float2 cdiv_f(float2 a, float2 b) {
   if (b.x == 0.f && b.y == 0.f) return a;
   const float den = b.x * b.x + b.y * b.y;
   return make_float2((a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den);
-}
-
-// Key of a candidate: magnitude bits (non-negative floats order as their
-// bits) above the complemented row, so the largest key is the largest
-// magnitude with the lowest row; 0 for no candidate.
-__device__ __forceinline__ unsigned long long tslu_key(float mag, int row) {
-  return ((unsigned long long)__float_as_uint(mag) << 32) | (unsigned long long)(0xffffffffu - (unsigned)row);
 }
 
 // LAPACK-style interchanges of the chosen rows sel[0..ib) into positions
@@ -1765,115 +1764,164 @@ __device__ void tslu_pivots_wave(const int *__restrict__ sel, int i0, int ib, in
   }
 }
 
-// Wave maximum of 64-bit keys through DPP (quad permutations, row
-// rotations, row broadcasts; lane 63 ends with the maximum of all lanes),
-// read back to every lane.
-template <int CTRL>
-__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long k) {
-  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)k, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(k >> 32), CTRL, 0xf, 0xf, false);
-  return ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo;
-}
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
-  unsigned long long t;
-  t = dpp_u64<0xb1>(k);  // quad_perm [1, 0, 3, 2]
-  k = t > k ? t : k;
-  t = dpp_u64<0x4e>(k);  // quad_perm [2, 3, 0, 1]
-  k = t > k ? t : k;
-  t = dpp_u64<0x124>(k);  // row_ror:4
-  k = t > k ? t : k;
-  t = dpp_u64<0x128>(k);  // row_ror:8
-  k = t > k ? t : k;
-  t = dpp_u64<0x142>(k);  // row_bcast:15
-  k = t > k ? t : k;
-  t = dpp_u64<0x143>(k);  // row_bcast:31
-  k = t > k ? t : k;
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)k, 63);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 63);
-  return ((unsigned long long)hi << 32) | lo;
+// Wave maximum / minimum of an int through DPP (quad permutations, row
+// rotations, row broadcasts; lane 63 ends with the result over all lanes),
+// read back to every lane; lanes without a valid DPP source keep their own
+// value.  Non-negative floats order as their bits do as ints, below which
+// every negative float lies, so the maximum of magnitudes (or -1 for none)
+// is taken on their bits.
+template <bool MAX>
+__device__ __forceinline__ int wave_reduce_i32(int x) {
+#define MOM_DPP_STEP(c)                                              \
+  {                                                                  \
+    const int t = __builtin_amdgcn_update_dpp(x, x, c, 0xf, 0xf, false); \
+    x = MAX ? max(x, t) : min(x, t);                                 \
+  }
+  MOM_DPP_STEP(0xb1)   // quad_perm [1, 0, 3, 2]
+  MOM_DPP_STEP(0x4e)   // quad_perm [2, 3, 0, 1]
+  MOM_DPP_STEP(0x124)  // row_ror:4
+  MOM_DPP_STEP(0x128)  // row_ror:8
+  MOM_DPP_STEP(0x142)  // row_bcast:15
+  MOM_DPP_STEP(0x143)  // row_bcast:31
+#undef MOM_DPP_STEP
+  return __builtin_amdgcn_readlane(x, 63);
 }
 
-// One barrier per column: each wave publishes its best key and that row
-// (from the lane holding it) before the barrier, every thread then takes
-// the best of the four; the buffers alternate by column parity, so a wave
-// that runs ahead writes the other parity while the rest still read.
-// With lu_top (the last level, one group) the winner of step k also stores
-// its row -- multipliers left of k, U entries from k on -- as row k of the
-// block's L11 U11, and a zero pivot is recorded in info (first one).
-__global__ __launch_bounds__(256) void tslu_select_kernel(const float2 *__restrict__ A, int64_t ld, int i0, int ib,
-                                                          int n, const int *__restrict__ in, int nsets, int ngroups,
-                                                          int *__restrict__ out, float2 *__restrict__ lu_top,
-                                                          int *__restrict__ info, int *__restrict__ ipiv,
-                                                          int *__restrict__ plan) {
-  __shared__ unsigned long long part[2][4];
-  __shared__ float2 prow[2][4][IB];
+// One barrier per column.  Each wave finds its largest magnitude by a DPP
+// float maximum and the lowest row holding it (the first lane of the ballot,
+// or a DPP row minimum when several lanes tie), publishes (magnitude, row)
+// and, from the lane holding it, that row before the barrier; after it every
+// thread takes the best of the kTsluWaves entries (larger magnitude, then
+// lower row).  The buffers alternate by column parity, so a wave that runs
+// ahead writes the other parity while the rest still read.  The multipliers
+// use one reciprocal of |pivot|^2 per column.  With lu_top (the last level,
+// one group) the winner of step k also stores its row -- multipliers left
+// of k, U entries from k on -- as row k of the block's L11 U11, and a zero
+// pivot is recorded in info (first one).
+__global__ __launch_bounds__(kTsluThreads) void tslu_select_kernel(const float2 *__restrict__ A, int64_t ld, int i0,
+                                                                   int ib, int n, const int *__restrict__ in,
+                                                                   int nsets, int ngroups, int *__restrict__ out,
+                                                                   float2 *__restrict__ lu_top,
+                                                                   int *__restrict__ info, int *__restrict__ ipiv,
+                                                                   int *__restrict__ plan) {
+  __shared__ float pmag[2][kTsluWaves];
+  __shared__ int prw[2][kTsluWaves];
+  __shared__ float2 prow[2][kTsluWaves][IB];
   __shared__ int pv[NB], first[NB], idx[2 * NB];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int grp = blockIdx.x;
-  int row = -1;
-  if (!in) {
-    const int r = i0 + kTsluRows * grp + tid;
-    if (r < n) row = r;
-  } else {
-    const int set = grp * kTsluSets + tid / IB;
-    if (set < nsets && tid % IB < ib) row = in[set * IB + tid % IB];
-  }
-  float2 v[IB];
+  int row[kTsluRpt];
+  bool act[kTsluRpt];
+  float2 v[kTsluRpt][IB];
 #pragma unroll
-  for (int s = 0; s < IB; s++)
-    v[s] = (row >= 0 && s < ib) ? A[(int64_t)(i0 + s) * ld + row] : make_float2(0.f, 0.f);
-  bool act = row >= 0, done = false;
-  // fully unrolled (static register indices); k < ib and done are uniform
+  for (int h = 0; h < kTsluRpt; h++) {
+    const int q = tid + kTsluThreads * h;  // row of the group
+    row[h] = -1;
+    if (!in) {
+      const int r = i0 + kTsluRows * grp + q;
+      if (r < n) row[h] = r;
+    } else {
+      const int set = grp * kTsluSets + q / IB;
+      if (set < nsets && q % IB < ib) row[h] = in[set * IB + q % IB];
+    }
+    act[h] = row[h] >= 0;
+#pragma unroll
+    for (int s = 0; s < IB; s++)
+      v[h][s] = (act[h] && s < ib) ? A[(int64_t)(i0 + s) * ld + row[h]] : make_float2(0.f, 0.f);
+  }
+  // Straight-line steps (fully unrolled, static register indices; k < ib is
+  // uniform): every row is updated whatever its state, with multiplier 0
+  // once it is chosen or if it holds no row.  Rows already chosen are never
+  // read again, so whatever such an update leaves in them is harmless.
 #pragma clang loop unroll(full)
   for (int k = 0; k < IB; k++) {
-    if (k >= ib || done) continue;
+    if (k >= ib) continue;
     const int par = k & 1;
-    float c = fabsf(v[k].x) + fabsf(v[k].y);
-    if (!(c == c)) c = 0.f;
-    const unsigned long long key = act ? tslu_key(c, row) : 0ull;
-    const unsigned long long wmax = wave_max_u64(key);
+    // this lane's candidate: magnitude (-1 without an active row) and row
+    float cm = -1.f;
+    int cr = 0x7fffffff, ch = 0;
+#pragma unroll
+    for (int h = 0; h < kTsluRpt; h++) {
+      float c = fabsf(v[h][k].x) + fabsf(v[h][k].y);
+      if (!(c == c)) c = 0.f;
+      if (act[h] && (c > cm || (c == cm && row[h] < cr))) {
+        cm = c;
+        cr = row[h];
+        ch = h;
+      }
+    }
+    const float wm = __int_as_float(wave_reduce_i32<true>(__float_as_int(cm)));
+    const unsigned long long tie = __ballot(cm >= 0.f && cm == wm);
+    int wr = 0x7fffffff;
+    if (tie) {
+      wr = __builtin_amdgcn_readlane(cr, __builtin_ffsll((long long)tie) - 1);
+      if (__builtin_popcountll(tie) > 1) wr = wave_reduce_i32<false>((tie >> lane) & 1ull ? cr : 0x7fffffff);
+    }
     // columns s >= ib hold zeros in every row and stay zero, so the loops
     // below need no bound on s other than the static k
-    if (wmax != 0ull && key == wmax) {
+    if (tie && cm == wm && cr == wr) {
 #pragma unroll
       for (int s = 0; s < IB; s++)
-        if (s >= k) prow[par][wv][s] = v[s];
-    }
-    if (lane == 0) part[par][wv] = wmax;
-    __syncthreads();
-    unsigned long long best = part[par][0];
-    int bw = 0;
+        if (s >= k) {
+          float2 x = v[0][s];
 #pragma unroll
-    for (int q = 1; q < 4; q++)
-      if (part[par][q] > best) {
-        best = part[par][q];
+          for (int h = 1; h < kTsluRpt; h++)
+            if (ch == h) x = v[h][s];
+          prow[par][wv][s] = x;
+        }
+    }
+    if (lane == 0) {
+      pmag[par][wv] = tie ? wm : -1.f;
+      prw[par][wv] = wr;
+    }
+    __syncthreads();
+    float bm = pmag[par][0];
+    int br = prw[par][0], bw = 0;
+#pragma unroll
+    for (int q = 1; q < kTsluWaves; q++) {
+      const float m = pmag[par][q];
+      const int r = prw[par][q];
+      if (m > bm || (m == bm && r < br)) {
+        bm = m;
+        br = r;
         bw = q;
       }
-    if (best == 0ull) {  // no rows left in this group (uniform)
-      if (tid == 0)
-        for (int q = k; q < ib; q++) out[grp * IB + q] = -1;
-      done = true;
-      continue;
     }
-    const int bi = (int)(0xffffffffu - (unsigned)(best & 0xffffffffull));
-    if (act && row == bi) {
-      out[grp * IB + k] = bi;
-      act = false;
-      if (lu_top) {
+    // bm < 0: no rows left in this group (uniform); every row then has
+    // multiplier 0 below
+    if (bm < 0.f && tid == 0) out[grp * IB + k] = -1;
 #pragma unroll
-        for (int s = 0; s < IB; s++) lu_top[k * IB + s] = v[s];
-        if (v[k].x == 0.f && v[k].y == 0.f) atomicCAS(info, 0, i0 + k + 1);
+    for (int h = 0; h < kTsluRpt; h++) {
+      if (bm >= 0.f && act[h] && row[h] == br) {
+        out[grp * IB + k] = br;
+        act[h] = false;
+        if (lu_top) {
+#pragma unroll
+          for (int s = 0; s < IB; s++) lu_top[k * IB + s] = v[h][s];
+          if (v[h][k].x == 0.f && v[h][k].y == 0.f) atomicCAS(info, 0, i0 + k + 1);
+        }
       }
     }
-    if (act) {
-      const float2 l = cdiv_f(v[k], prow[par][bw][k]);
-      v[k] = l;
+    const float2 p = prow[par][bw][k];
+    const float den = p.x * p.x + p.y * p.y;
+    const float inv = den != 0.f ? 1.f / den : 0.f;
+    float2 l[kTsluRpt];
 #pragma unroll
-      for (int s = 0; s < IB; s++) {
-        if (s <= k) continue;
-        const float2 u = prow[par][bw][s];
-        v[s].x -= l.x * u.x - l.y * u.y;
-        v[s].y -= l.x * u.y + l.y * u.x;
+    for (int h = 0; h < kTsluRpt; h++) {
+      // a zero pivot leaves the column as it is (as LAPACK's update skips it)
+      const float2 a = v[h][k];
+      const float2 q = den != 0.f ? make_float2((a.x * p.x + a.y * p.y) * inv, (a.y * p.x - a.x * p.y) * inv) : a;
+      l[h] = act[h] ? q : make_float2(0.f, 0.f);
+      v[h][k] = act[h] ? q : a;
+    }
+#pragma unroll
+    for (int s = 0; s < IB; s++) {
+      if (s <= k) continue;
+      const float2 u = prow[par][bw][s];
+#pragma unroll
+      for (int h = 0; h < kTsluRpt; h++) {
+        v[h][s].x -= l[h].x * u.x - l[h].y * u.y;
+        v[h][s].y -= l[h].x * u.y + l[h].y * u.x;
       }
     }
   }
@@ -1964,12 +2012,12 @@ static int factor_panel(const Lane &L, typename Cx<T>::type *A, int64_t ld, int6
       int64_t groups = (n - i0 + kTsluRows - 1) / kTsluRows;
       if constexpr (std::is_same<T, float>::value) {
         // the last level (one group) also factors the chosen rows into lu_top
-        hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)groups), dim3(256), 0, s, A, ld, (int)i0,
+        hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)groups), dim3(kTsluThreads), 0, s, A, ld, (int)i0,
                            (int)ib, (int)n, (const int *)nullptr, 0, (int)groups, cur,
                            groups == 1 ? w.lu_top : (float2 *)nullptr, w.info, ipiv, w.plan);
         while (groups > 1) {
           const int64_t gn = (groups + kTsluSets - 1) / kTsluSets;
-          hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)gn), dim3(256), 0, s, A, ld, (int)i0,
+          hipLaunchKernelGGL(tslu_select_kernel, dim3((unsigned)gn), dim3(kTsluThreads), 0, s, A, ld, (int)i0,
                              (int)ib, (int)n, (const int *)cur, (int)groups, (int)gn, nxt,
                              gn == 1 ? w.lu_top : (float2 *)nullptr, w.info, ipiv, w.plan);
           std::swap(cur, nxt);

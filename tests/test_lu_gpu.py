@@ -219,7 +219,7 @@ def test_tournament_panels(gpu_ctx, n2):
     both first solutions have the backward error of a complex64 LU, and the
     refined solutions agree with LAPACK's to the same backward error.  Sizes
     cover a single partial inner block, several panels with a partial last
-    one, and more than eight candidate sets (two tree levels)."""
+    one, and several groups of 1024 rows (two tree levels)."""
     A = rand_complex(n2, seed=40 + n2)
     A[n2 // 2:, :] *= 1e-5
     A[:, n2 // 2:] *= 3e3
