@@ -304,10 +304,12 @@ def bench_spectrum(args):
     eps_inf - wp^2/(w(w + i gamma)) (eps_inf 9.5, wp 9.0 eV, gamma 0.07 eV; a
     single pole), 256 plane waves per wavelength (psi 0 and 90 deg times 128
     directions uniform on the sphere, numpy seed 20251203).  One step = for
-    every wavelength the fill, the mixed-precision LU and the 256 solves on
-    the reused factors (block refinement to backward error 1e-12), with one
-    device-resident system reused across the wavelengths
-    (mom2amd.spectrum.solve_spectrum, the path the spectrum parity test runs)."""
+    every wavelength the fill, the LU and the 256 solves on the reused
+    factors (block refinement to backward error 1e-12), --lanes wavelengths
+    at a time, each lane a device-resident system reused across its
+    wavelengths (mom2amd.spectrum.solve_spectrum, the path the spectrum
+    parity test runs).  The all-FP64 path is timed on eight wavelengths
+    spread over the range, one at a time and in lanes."""
     from mom2amd import DEGTORAD, Context, DeviceMesh, cs_prtsrf, load_problem, meshgen, omega_of
     from mom2amd.capi import MOM_PREC_FP64, MOM_PREC_MIXED
     from mom2amd.spectrum import drude_index, plane_waves_deg, solve_spectrum, wavelengths
@@ -324,31 +326,36 @@ def bench_spectrum(args):
     wls = wavelengths(nwl)
     ang = plane_waves_deg(128)
     ri_of = lambda wl: [1.0 + 0j, drude_index(wl)]  # noqa: E731
-    log("ellipsoid s=%d: %d faces, N=%d unknowns, %d wavelengths x %d sources" % (s, prob.master.nfaces, 2 * N,
-                                                                                  nwl, ang.shape[0]))
+    log("ellipsoid s=%d: %d faces, N=%d unknowns, %d wavelengths x %d sources, %d lanes"
+        % (s, prob.master.nfaces, 2 * N, nwl, ang.shape[0], args.lanes))
     ctx = Context(0)
     dms = [DeviceMesh(ctx, dm) for dm in prob.domains]
+    piv = 1 if args.pivoting == "tournament" else 0
 
-    def sweep(prec, idx, keep=None):
-        X, info = solve_spectrum(ctx, dms, N, wls[list(idx)], ri_of, ang, precision=prec, keep=keep,
-                                 pivoting=1 if args.pivoting == "tournament" else 0)
+    def sweep(prec, idx, keep=None, lanes=args.lanes):
+        t0 = time.perf_counter()
+        X, info = solve_spectrum(ctx, dms, N, wls[list(idx)], ri_of, ang, precision=prec, keep=keep, pivoting=piv,
+                                 lanes=lanes)
         t = {k: float(sum(r[k] for r in info)) for k in ("fill", "factor", "solve")}
-        t["wall"] = t["fill"] + t["factor"] + t["solve"]
-        return t, X, max(r["iters"] for r in info), max(r["berr"] for r in info), sum(r["fell_back"] for r in info)
+        t["clock"] = time.perf_counter() - t0
+        t["iters"] = max(r["iters"] for r in info)
+        t["berr"] = max(r["berr"] for r in info)
+        t["fell_back"] = sum(r["fell_back"] for r in info)
+        t["fp64_direct"] = sum(r["precision"] == "fp64" for r in info) if prec == MOM_PREC_MIXED else 0
+        return t, X
 
     for i in range(args.warmup):
-        w = sweep(MOM_PREC_MIXED, range(min(nwl, 4)))[0]
-        log("warmup %d (4 wavelengths): %.2f s" % (i, w["wall"]))
+        w = sweep(MOM_PREC_MIXED, range(min(nwl, 2 * args.lanes)))[0]
+        log("warmup %d (%d wavelengths): %.2f s" % (i, min(nwl, 2 * args.lanes), w["clock"]))
     runs = []
     check = [0, nwl - 1]
     for i in range(args.steps):
-        t0 = time.perf_counter()
-        r, X, mi, mb, nfb = sweep(MOM_PREC_MIXED, range(nwl), keep=check)
-        r["clock"] = time.perf_counter() - t0
+        r, X = sweep(MOM_PREC_MIXED, range(nwl), keep=check)
         runs.append(r)
-        log("step %d: %.2f s for %d wavelengths (fill %.2f factor %.2f solve %.2f; refinement steps <= %d, "
-            "berr <= %.1e, %d wavelengths refactored in FP64)" % (i, r["clock"], nwl, r["fill"], r["factor"],
-                                                                  r["solve"], mi, mb, nfb))
+        log("step %d: %.2f s for %d wavelengths (lane sums: fill %.2f factor %.2f solve %.2f; refinement steps <= %d, "
+            "berr <= %.1e, %d fell back to FP64 factors, %d factored directly in FP64)"
+            % (i, r["clock"], nwl, r["fill"], r["factor"], r["solve"], r["iters"], r["berr"], r["fell_back"],
+               r["fp64_direct"]))
     mean = float(np.mean([r["clock"] for r in runs]))
     nrhs = ang.shape[0]
     out = {"metric": "spectrum s @N=%d: %d wavelengths x %d plane waves, fill+LU+solves (mixed path)"
@@ -359,23 +366,35 @@ def bench_spectrum(args):
            "config": {"workload": "ellipsoid 80/40/40 nm from icosphere s=%d, %d faces, N=%d, %d wavelengths "
                                   "400-900 nm, Drude eps_in, %d plane waves per wavelength"
                                   % (s, prob.master.nfaces, 2 * N, nwl, nrhs), "n_unknowns": 2 * N,
-                      "complex64_pivoting": args.pivoting},
-           "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fill", "factor", "solve")},
+                      "complex64_pivoting": args.pivoting, "lanes": args.lanes},
+           "lane_sums_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fill", "factor", "solve")},
+           "fell_back_to_fp64": runs[-1]["fell_back"], "factored_directly_in_fp64": runs[-1]["fp64_direct"],
            "per_wavelength_ms": round(mean / nwl * 1e3, 2)}
     if not args.no_fp64:
-        # FP64 path on the first and last wavelengths; cross-sections of 16 sources compared with the mixed solutions
-        t64, X64, _, _, _ = sweep(MOM_PREC_FP64, check)
+        # the all-FP64 path on eight wavelengths spread over the range (both ends included); the cross-sections
+        # of 16 sources at the two ends compared with the mixed solutions
+        # both one wavelength at a time (cooperative panels) and in lanes (panels one launch per column); the
+        # speed-up is stated against the faster
+        sub = sorted(set(int(round(v)) for v in np.linspace(0, nwl - 1, min(nwl, 8))))
+        t64_1, X64 = sweep(MOM_PREC_FP64, sub, keep=[0, len(sub) - 1], lanes=1)
+        t64_l = sweep(MOM_PREC_FP64, sub, lanes=args.lanes)[0] if args.lanes > 1 else t64_1
+        log("fp64 path: %.3f s per wavelength one at a time, %.3f s in %d lanes"
+            % (t64_1["clock"] / len(sub), t64_l["clock"] / len(sub), args.lanes))
+        t64 = t64_1 if t64_1["clock"] <= t64_l["clock"] else t64_l
+        out["fp64_path_per_wavelength_s_by_lanes"] = {"1": round(t64_1["clock"] / len(sub), 4),
+                                                      str(args.lanes): round(t64_l["clock"] / len(sub), 4)}
         err = 0.0
-        for j, i in enumerate(check):
+        for j, i in zip([0, len(sub) - 1], check):
             om = omega_of(wls[i])
             for k in range(0, nrhs, nrhs // 8):
                 a = cs_prtsrf(ctx, dms[0], N, om, 1.0, X[i][:, k], *(ang[k] * DEGTORAD))
                 b = cs_prtsrf(ctx, dms[0], N, om, 1.0, X64[j][:, k], *(ang[k] * DEGTORAD))
                 err = max(err, abs(a[0] / b[0] - 1), abs(a[1] / b[1] - 1))
-        out["fp64_path_per_wavelength_s"] = round(t64["wall"] / len(check), 4)
-        out["speedup_vs_fp64_path"] = round(t64["wall"] / len(check) / (mean / nwl), 3)
+        out["fp64_path_per_wavelength_s"] = round(t64["clock"] / len(sub), 4)
+        out["fp64_path_wavelengths"] = len(sub)
+        out["speedup_vs_fp64_path"] = round(t64["clock"] / len(sub) / (mean / nwl), 3)
         out["sigma_relerr_vs_fp64_path"] = err
-        log("fp64 path: %.2f s per wavelength; sigma rel err (16 checks) %.1e" % (t64["wall"] / len(check), err))
+        log("sigma rel err mixed vs fp64 (16 checks) %.1e" % err)
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -694,6 +713,7 @@ def main():
                          "complex128 PMCHWT fill of the default sphere alone (for counter runs)")
     ap.add_argument("--cell-n", type=int, default=50, help="octasphere divisions of the symmetric workload")
     ap.add_argument("--nwl", type=int, default=64, help="wavelengths of the spectrum workload")
+    ap.add_argument("--lanes", type=int, default=4, help="wavelengths solved at once in the spectrum workload")
     ap.add_argument("--pivoting", choices=("tournament", "partial"), default="tournament",
                     help="complex64 panels of the mixed path in the spectrum, sh and adjoint workloads (the library "
                          "default, and the pmchwt workload's, is partial pivoting)")

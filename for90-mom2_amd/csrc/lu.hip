@@ -37,10 +37,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -1706,9 +1708,13 @@ __global__ __launch_bounds__(256) void unit_lower_inverse_kernel(const float2 *_
 // LAPACK's but are as good in practice; the complex64 factors only
 // precondition the complex128 refinement, whose result does not depend on
 // them beyond rounding.  The complex128 path keeps LAPACK's partial pivoting.
-// Off by default (mom_test_tournament switches it on): on the 15360-unknown
-// ellipsoid its panels took longer than the cooperative partial-pivoting
-// panel (one wave runs each 32-step selection serially, ~85 us per level).
+// Off by default (a system chooses it with mom_system_set_pivoting, the
+// test hook mom_test_tournament forces it): LAPACK's pivots stay the
+// default.  On the 15360-unknown ellipsoid the factorisation takes 0.16 s
+// with it against 0.23 s with the cooperative partial-pivoting panel; one
+// level of 32 selection steps takes about 40 us (1.2 us per column step,
+// latency of one wave per SIMD).  Its launch sequence depends only on n, so
+// a system replays it from a captured HIP graph (factor_graph).
 //
 // Selection, one workgroup of 256 threads per group of kTsluRows = 256
 // rows, one row per thread in registers: level 0 takes 256 consecutive rows
@@ -2860,12 +2866,34 @@ static double now_s() {
 
 }  // namespace mom
 
+namespace mom {
+// The complex64 factorisation with tournament pivoting (getrf and the
+// diagonal-block inverses) as a captured HIP graph: its launches depend only
+// on n and on the buffers they touch, so a system captures it on its first
+// factorisation and replays it on later ones.  `key` holds everything the
+// captured launches baked in (buffer addresses, the look-ahead stream, n and
+// the process-wide switches); any change captures again.
+struct FactorGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  std::vector<uintptr_t> key;
+  void reset() {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    x = nullptr;
+    g = nullptr;
+    key.clear();
+  }
+  ~FactorGraph() { reset(); }
+};
+}  // namespace mom
+
 struct mom_system {
   mom_ctx *ctx = nullptr;
   int64_t N = 0;  // nedges_total
   int64_t n = 0;  // 2N
   int precision = MOM_PREC_FP64;       // precision of the current factors
-  int precision_req = MOM_PREC_FP64;   // precision asked for at creation (restored by every fill)
+  int precision_req = MOM_PREC_FP64;   // asked for at creation or by mom_system_set_precision (set by every fill)
   bool fell_back = false;              // the last block solve moved a mixed system to FP64 factors
   bool filled = false, factored = false;
   mom::DevBuf Z64, Z32, ipiv, Linv, Uinv, rscale, cscale, rowsum, part, pivwork;
@@ -2883,7 +2911,9 @@ struct mom_system {
   double be_hist[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // backward error after each outer step
   int coop_gmax[2] = {-1, -1};
   bool coop_refused = false;  // a cooperative launch was refused: panels by one launch per column
+  bool coop_off = false;      // mom_system_set_cooperative(0): panels by one launch per column
   int pivoting = 0;  // complex64 panels: 0 partial pivoting, 1 tournament (mom_system_set_pivoting)
+  mom::FactorGraph fgraph;  // the tournament factorisation, captured (see factor_graph)
 };
 
 namespace mom {
@@ -2931,7 +2961,7 @@ static int panel_coop(mom_system *S, PanelCoop *c) {
     MOM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)panel_block_kernel<T>, 256, 0));
     S->coop_gmax[pi] = coop ? std::min(kPanelMaxGroups, nb * prop.multiProcessorCount) : 0;
   }
-  c->gmax = g_panel_mode == 1 ? 0 : S->coop_gmax[pi];
+  c->gmax = (g_panel_mode == 1 || S->coop_off) ? 0 : S->coop_gmax[pi];
   c->refused = &S->coop_refused;
   if (c->gmax == 0) return 0;
   const size_t cand = sizeof(u64) * 2 * kPanelMaxGroups * kCandWords, diag = sizeof(u64) * 2 * kRowWords;
@@ -2955,6 +2985,64 @@ static int panel_coop(mom_system *S, PanelCoop *c) {
   return 0;
 }
 
+// Whether factorisations with tournament pivoting replay a captured graph
+// (process-wide; the test hook mom_test_factor_graph switches it), and how
+// many captures and replays there were.
+static int g_factor_graph = 1;
+static std::atomic<int64_t> g_graph_counts[2] = {0, 0};
+
+// Factorisations whose panels are cooperative launches hold this lock while
+// their launches are in flight: two such grids made by different host
+// threads (systems of different contexts) could each fill the device with
+// workgroups that wait for the rest of their own grid.  Launches of other
+// kinds from other threads may run beside one cooperative grid, whose
+// missing workgroups then start as soon as those finish.
+static std::mutex g_coop_mu;
+
+// getrf and the diagonal-block inverses of the complex64 factors with
+// tournament pivoting, from the system's captured graph.  Every buffer the
+// launches touch is sized beforehand (no allocation while capturing), and
+// the capture is thread-local, so other host threads may keep allocating.
+static int factor_graph(mom_system *S, float2 *A, const PivotWork &w, const PanelCoop &coop) {
+  mom_ctx *ctx = S->ctx;
+  hipStream_t s = ctx->stream;
+  const int64_t n = S->n;
+  const bool la = g_lookahead != 0 && n > NB;
+  Lane X;
+  if (la) MOM_TRY(aux_lane(ctx, &X));
+  const size_t bt = sizeof(float2) * NB * (size_t)((n + kDmaBN - 1) / kDmaBN * kDmaBN);
+  if (ctx->panel_t.bytes < bt) MOM_TRY(ctx->panel_t.alloc(bt));
+  if (la && ctx->panel_t2.bytes < bt) MOM_TRY(ctx->panel_t2.alloc(bt));
+  if (n > NB && ctx->ublk.bytes < sizeof(float2) * NB * NB) MOM_TRY(ctx->ublk.alloc(sizeof(float2) * NB * NB));
+  const std::vector<uintptr_t> key = {
+      (uintptr_t)A, (uintptr_t)S->ipiv.ptr, (uintptr_t)S->pivwork.ptr, (uintptr_t)S->Linv.ptr,
+      (uintptr_t)S->Uinv.ptr, (uintptr_t)ctx->panel_t.ptr, (uintptr_t)ctx->panel_t2.ptr, (uintptr_t)ctx->ublk.ptr,
+      (uintptr_t)ctx->aux, (uintptr_t)ctx->fork_ev, (uintptr_t)ctx->join_ev, (uintptr_t)n, (uintptr_t)g_lookahead,
+      (uintptr_t)g_u_subst};
+  FactorGraph &G = S->fgraph;
+  if (!G.x || G.key != key) {
+    G.reset();
+    MOM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = getrf<float>(ctx, A, n, n, S->ipiv.as<int>(), w, nullptr, coop);
+    if (!rc) rc = diag_inverses<float>(s, A, n, n, S->Linv.as<double2>(), S->Uinv.as<double2>());
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return rc ? rc : fail(MOM_ERR_HIP, std::string("hipStreamEndCapture(factor): ") + hipGetErrorString(e));
+    }
+    G.g = g;
+    MOM_HIP(hipGraphInstantiate(&G.x, G.g, nullptr, nullptr, 0));
+    G.key = key;
+    g_graph_counts[0]++;
+  } else {
+    g_graph_counts[1]++;
+  }
+  MOM_HIP(hipGraphLaunch(G.x, s));
+  return 0;
+}
+
 // Factor the n x n matrix held by the system (complex128 in place, or the
 // scaled complex64 copy), build the permutation and diagonal-block inverses.
 template <typename T>
@@ -2967,12 +3055,31 @@ static int factor_impl(mom_system *S, typename Cx<T>::type *A) {
   if (S->clock.enabled) S->clock.reset();
   PanelCoop coop;
   if ((rc = panel_coop<T>(S, &coop))) return rc;
-  if ((rc = getrf<T>(S->ctx, A, n, n, S->ipiv.as<int>(), w, &S->clock, coop))) return rc;
   const int64_t nblk = (n + NB - 1) / NB;
   const size_t inv_bytes = sizeof(double2) * nblk * NB * NB;  // complex128 in both precisions
   if (S->Linv.bytes < inv_bytes && ((rc = S->Linv.alloc(inv_bytes)) || (rc = S->Uinv.alloc(inv_bytes)))) return rc;
-  if ((rc = diag_inverses<T>(s, A, n, n, S->Linv.as<double2>(), S->Uinv.as<double2>())))
-    return rc;
+  bool graphed = false;
+  if constexpr (std::is_same<T, float>::value) {
+    // tournament panels make no cooperative launches and no host reads
+    // inside getrf; event timing and the E = I - inv(L11) variant keep the
+    // launches eager
+    if (w.tournament && g_factor_graph && !S->clock.enabled && g_u_subst != 0) {
+      if ((rc = factor_graph(S, A, w, coop))) return rc;
+      graphed = true;
+    }
+  }
+  std::unique_lock<std::mutex> coop_lock(g_coop_mu, std::defer_lock);
+  if (!graphed) {
+    if (coop.gmax > 0 && !w.tournament) coop_lock.lock();  // released once the pivots have been read back
+    if ((rc = getrf<T>(S->ctx, A, n, n, S->ipiv.as<int>(), w, &S->clock, coop))) {
+      if (coop_lock.owns_lock()) (void)hipStreamSynchronize(s);
+      return rc;
+    }
+    if ((rc = diag_inverses<T>(s, A, n, n, S->Linv.as<double2>(), S->Uinv.as<double2>()))) {
+      if (coop_lock.owns_lock()) (void)hipStreamSynchronize(s);
+      return rc;
+    }
+  }
   std::vector<int> ip(n);
   int info = 0;
   MOM_TRY(copy_d2h(S->ctx, ip.data(), S->ipiv.ptr, sizeof(int) * n));
@@ -3211,6 +3318,16 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
   MOM_TRY(minv(dB.as<double2>(), false));
   int it = 0;
   std::vector<int> keep(nrhs);
+  // Stalled columns are finished one by one by GMRES, unless that would
+  // cost more than factoring the resident complex128 matrix -- about 15
+  // GMRES steps per column, each one application of the factors and one
+  // product, against 8n^3/3 flops at about 60 TFLOP/s -- in which case the
+  // system moves to FP64 factors and solves every column directly.  Once
+  // the columns stalled so far settle that, the refinement stops at once.
+  const double nd = (double)n;
+  const double t_lu64 = 8.0 / 3.0 * nd * nd * nd / 6.0e13;
+  const double t_col = 15.0 * (24.0 * nd * nd / 4.0e12 + 1.0e-3);
+  auto fallback_settled = [&](int64_t nstall) { return g_force_fp64_fallback || (double)nstall * t_col > t_lu64; };
   for (;; it++) {
     S->n_matvec++;
     MOM_HIP(hipMemcpyAsync(dR.ptr, dB.ptr, blk, hipMemcpyDeviceToDevice, s));
@@ -3227,6 +3344,9 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
     }
     if (it < kBeHistory) S->be_hist[it] = bmax;
     if (bmax <= tol || it >= maxit || nkeep == 0) break;
+    int64_t nstall_now = 0;
+    for (int64_t j = 0; j < nrhs; j++) nstall_now += be[j] > tol && !keep[j];
+    if (nstall_now > 0 && fallback_settled(nstall_now)) break;
     MOM_TRY(copy_h2d(S->ctx, dkeep.ptr, keep.data(), sizeof(int) * nrhs));
     hipLaunchKernelGGL(mask_cols_kernel, g2, dim3(256), 0, s, dR.as<double2>(), n, n, dkeep.as<int>());
     MOM_TRY(minv(dR.as<double2>(), true));
@@ -3234,18 +3354,11 @@ static int mixed_solve_batch(mom_system *S, int64_t nrhs, double *B, int64_t ldb
   }
   MOM_TRY(copy_d2h_2d(S->ctx, B, ldb * sizeof(double2), dX.ptr, n * sizeof(double2), n * sizeof(double2), nrhs));
   // Columns that stalled (typically near a resonance, where the FP32 factors'
-  // backward error is large) are finished by GMRES one at a time.  When that
-  // would cost more than factoring the resident complex128 matrix -- about
-  // 15 GMRES steps per column, each one application of the factors and one
-  // product, against 8n^3/3 flops at about 60 TFLOP/s -- the system moves to
-  // FP64 factors instead and solves every column directly (until the next
-  // fill).
+  // backward error is large): GMRES one at a time, or FP64 factors for all
+  // (until the next fill), as fallback_settled decides.
   int64_t nstall = 0;
   for (int64_t j = 0; j < nrhs; j++) nstall += be[j] > tol;
-  const double nd = (double)n;
-  const double t_lu64 = 8.0 / 3.0 * nd * nd * nd / 6.0e13;
-  const double t_col = 15.0 * (24.0 * nd * nd / 4.0e12 + 1.0e-3);
-  if (nstall > 0 && (g_force_fp64_fallback || (double)nstall * t_col > t_lu64)) {
+  if (nstall > 0 && fallback_settled(nstall)) {
     // the resident complex128 matrix is factored in place: if that fails
     // part-way it holds neither Z nor its factors, so the system needs a
     // new fill before it can factor or solve again
@@ -3363,6 +3476,23 @@ int mom_system_set_pivoting(mom_system *sys, int32_t mode) {
   if (!sys) return fail(MOM_ERR_ARG, "NULL argument");
   if (mode != 0 && mode != 1) return fail(MOM_ERR_ARG, "pivoting must be 0 (partial) or 1 (tournament)");
   sys->pivoting = mode;
+  return 0;
+}
+
+int mom_system_set_precision(mom_system *sys, int32_t precision) {
+  if (!sys) return fail(MOM_ERR_ARG, "NULL argument");
+  if (precision != MOM_PREC_FP64 && precision != MOM_PREC_MIXED) return fail(MOM_ERR_ARG, "unknown precision");
+  if (precision == MOM_PREC_MIXED && !sys->Z32.ptr)
+    return fail(MOM_ERR_ARG, "a system created MOM_PREC_FP64 has no complex64 copy");
+  sys->precision_req = precision;
+  if (sys->filled && !sys->factored) sys->precision = precision;
+  return 0;
+}
+
+int mom_system_set_cooperative(mom_system *sys, int32_t allow) {
+  if (!sys) return fail(MOM_ERR_ARG, "NULL argument");
+  if (allow != 0 && allow != 1) return fail(MOM_ERR_ARG, "cooperative switch must be 0 or 1");
+  sys->coop_off = allow == 0;
   return 0;
 }
 
@@ -3747,6 +3877,16 @@ extern "C" int mom_test_tournament(int32_t enable) {
 
 extern "C" int mom_test_batch_fallback(int32_t force) {
   g_force_fp64_fallback = force ? 1 : 0;
+  return 0;
+}
+
+extern "C" int mom_test_factor_graph(int32_t enable, int64_t *counts) {
+  if (enable != 0 && enable != 1 && enable != -1) return fail(MOM_ERR_ARG, "graph switch must be -1, 0 or 1");
+  if (enable >= 0) g_factor_graph = enable;
+  if (counts) {
+    counts[0] = g_graph_counts[0];
+    counts[1] = g_graph_counts[1];
+  }
   return 0;
 }
 

@@ -31,12 +31,13 @@ EXPORTS = [
     "mom_system_fill_symm", "mom_srcvec_pw_symm", "mom_cs_prtsrf_symm", "mom_rcs_ext_symm",
     "mom_pmchwt_vertex_adjoint", "mom_cs_abs_vertex_explicit", "mom_ctx_fill_tuning", "mom_far_fields_noalloc",
     "mom_ctx_fill_profile", "mom_mesh_classify_like", "mom_far_fields_adjoint", "mom_nlsurf_coef_adjoint",
-    "mom_domain_block_apply", "mom_system_set_pivoting",
+    "mom_domain_block_apply", "mom_system_set_pivoting", "mom_system_set_precision",
+    "mom_system_set_cooperative",
 ]
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
                 "mom_test_unit_lower_inverse", "mom_test_u_solve", "mom_test_vertex_fd", "mom_test_batch_fallback",
-                "mom_test_tournament"]
+                "mom_test_tournament", "mom_test_factor_graph"]
 
 
 class MomError(RuntimeError):
@@ -96,6 +97,7 @@ def lib():
     L.mom_test_vertex_fd.argtypes = [_i32]
     L.mom_test_batch_fallback.argtypes = [_i32]
     L.mom_test_tournament.argtypes = [_i32]
+    L.mom_test_factor_graph.argtypes = [_i32, C.POINTER(_i64)]
     L.mom_ctx_fill_tuning.argtypes = [_vp, C.POINTER(_i32), _dp]
     L.mom_ctx_fill_profile.argtypes = [_vp, _i32, _dp]
     L.mom_mesh_classify_like.argtypes = [_vp, _vp]
@@ -103,6 +105,8 @@ def lib():
     L.mom_nlsurf_coef_adjoint.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _dp, _dp, _i32, _dp, _dp, _dp, _dp,
                                           _dp]
     L.mom_system_set_pivoting.argtypes = [_vp, _i32]
+    L.mom_system_set_precision.argtypes = [_vp, _i32]
+    L.mom_system_set_cooperative.argtypes = [_vp, _i32]
     L.mom_domain_block_apply.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _i32, _dp, _dp, _dp, _dp]
     L.mom_test_unit_lower_inverse.argtypes = [_vp, _vp, _i64, _vp]
     L.mom_cs_prtsrf.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _d, _d, _i32, _dp, _dp, C.POINTER(_d),

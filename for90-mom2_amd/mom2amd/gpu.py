@@ -47,6 +47,7 @@ class Context:
 
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self.device = device
         self._children = weakref.WeakSet()
         capi.check(capi.lib().mom_ctx_create(device, C.byref(self.h)))
         _live_contexts.add(self)
@@ -91,6 +92,8 @@ class DeviceMesh:
 
     def __init__(self, ctx: Context, dm):
         self.ctx = ctx
+        self.host = dm  # the host mesh, for copies on other contexts
+        self.frozen = False  # near/far classification taken from another mesh (classify_like)
         self.nedges = dm.nedges
         self.nfaces = dm.nfaces
         self.h = C.c_void_p()
@@ -111,6 +114,7 @@ class DeviceMesh:
         length (same face count), freezing the near/far split of a perturbed
         mesh to that of the unperturbed one (mom_mesh_classify_like)."""
         capi.check(capi.lib().mom_mesh_classify_like(self.h, other.h))
+        self.frozen = True
 
     def close(self):
         if self.h:
@@ -243,6 +247,16 @@ class System:
         N2 = self.n
         Av = np.asfortranarray(A, dtype=np.complex128).reshape(-1, order="F").view(np.float64)
         capi.check(capi.lib().mom_system_set_matrix(self.h, np.ascontiguousarray(Av), N2))
+
+    def set_precision(self, precision):
+        """Precision of the next factorisation of a system created mixed:
+        capi.MOM_PREC_FP64 or capi.MOM_PREC_MIXED (mom_system_set_precision)."""
+        capi.check(capi.lib().mom_system_set_precision(self.h, int(precision)))
+
+    def set_cooperative(self, allow):
+        """Partial-pivoting panels by cooperative launches (1, default) or
+        one ordinary launch per column (0; mom_system_set_cooperative)."""
+        capi.check(capi.lib().mom_system_set_cooperative(self.h, int(allow)))
 
     def set_pivoting(self, mode):
         """Pivoting of the complex64 factorisation (mixed path): 0 partial as

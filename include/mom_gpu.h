@@ -240,6 +240,22 @@ int mom_system_solve_pw(mom_system *sys, const mom_mesh *mesh, double omega, dou
  * refinement, so the solution is the same to the refinement's tolerance.
  * The complex128 factorisation always pivots partially. */
 int mom_system_set_pivoting(mom_system *sys, int32_t mode);
+/* Precision of the factorisations of a system created MOM_PREC_MIXED (no
+ * counterpart in the reference): MOM_PREC_FP64 factors the complex128
+ * matrix itself, MOM_PREC_MIXED returns to the complex64 factors with
+ * refinement.  Takes effect at the next fill, or at once on a system filled
+ * but not yet factored.  A batch of wavelengths uses it to factor directly
+ * in FP64 where the complex64 factors stop converging (interior permittivity
+ * near zero).  A system created MOM_PREC_FP64 has no complex64 copy and
+ * accepts only MOM_PREC_FP64. */
+int mom_system_set_precision(mom_system *sys, int32_t precision);
+/* Launch kind of the partial-pivoting panels (no counterpart in the
+ * reference): 1 (default) one cooperative launch per 32-column block where
+ * the device runs the whole grid at once, 0 one ordinary launch per column
+ * (the same pivots, about 15% slower at 30720 unknowns).  Systems driven
+ * from worker threads, several at once, use 0: cooperative launches made
+ * from such threads were followed by a segmentation fault at process exit. */
+int mom_system_set_cooperative(mom_system *sys, int32_t allow);
 /* Event timing of the kernel groups of mom_system_factor (reported by
  * mom_system_stats): 0 off, 1 the depth-256 trailing updates only (cheap
  * enough for timed runs), 2 every group (adds host work per launch). */

@@ -81,6 +81,13 @@ int mom_test_batch_fallback(int32_t force);
  * 0 = each system's own choice.  Process-wide. */
 int mom_test_tournament(int32_t enable);
 
+/* Complex64 factorisations with tournament pivoting replay a HIP graph
+ * captured on the system's first factorisation (captured again when a
+ * buffer it baked in moves): enable 1 (default) or 0 switches that
+ * process-wide, -1 leaves it; counts (if not NULL) receives the process's
+ * captures [0] and replays [1] so far. */
+int mom_test_factor_graph(int32_t enable, int64_t *counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -244,3 +244,83 @@ def test_tournament_panels(gpu_ctx, n2):
         assert np.abs(x - x_ref).max() / np.abs(x_ref).max() < 1e-5
         out[mode] = x
     assert np.abs(out[1] - out[0]).max() / np.abs(out[0]).max() < 1e-5
+
+
+def _tournament_solve(S, A, b):
+    S.set_matrix(A)
+    S.factor()
+    x = S.solve(b, tol=1e-13, maxit=20)
+    return x, S.stats()["berr_history"]
+
+
+def test_factor_graph_replays_match_eager(gpu_ctx):
+    """The tournament factorisation replayed from its captured HIP graph
+    gives bit for bit the solutions and refinement history of the same
+    launches made eagerly: one system captures on its first factorisation and
+    replays on the next ones, with eager work (uploads, solves, another
+    system's factorisation that may move the shared GEMM scratch) between the
+    replays; every replay is checked against the eager result of its
+    matrix."""
+    L = capi.lib()
+    cnt = np.zeros(2, dtype=np.int64)
+    mats = [rand_complex(1100, seed=60), rand_complex(1100, seed=61)]
+    for A in mats:
+        A[550:, :] *= 1e-4
+    b = rand_complex(1100, 2, seed=62)
+    eager = []
+    capi.check(L.mom_test_factor_graph(0, None))
+    try:
+        S = System(gpu_ctx, 550, MOM_PREC_MIXED)
+        S.set_pivoting(1)
+        for A in mats:
+            eager.append(_tournament_solve(S, A, b))
+        S.close()
+    finally:
+        capi.check(L.mom_test_factor_graph(1, None))
+    S = System(gpu_ctx, 550, MOM_PREC_MIXED)
+    S2 = System(gpu_ctx, 1500, MOM_PREC_MIXED)
+    try:
+        S.set_pivoting(1)
+        S2.set_pivoting(1)
+        capi.check(L.mom_test_factor_graph(-1, cnt.ctypes.data_as(capi.C.POINTER(capi._i64))))
+        c0 = cnt.copy()
+        for rep, i in enumerate([0, 1, 0, 1]):
+            if rep == 2:
+                # a larger system of the same context in between
+                _tournament_solve(S2, rand_complex(3000, seed=63), rand_complex(3000, 1, seed=64))
+            x, hist = _tournament_solve(S, mats[i], b)
+            assert np.array_equal(x, eager[i][0]), rep
+            assert hist == eager[i][1], rep
+            assert berr(mats[i], x[:, 0], b[:, 0]) < 1e-13
+        capi.check(L.mom_test_factor_graph(-1, cnt.ctypes.data_as(capi.C.POINTER(capi._i64))))
+        assert cnt[0] - c0[0] >= 2  # S, then S2 (S may capture again after S2 moved the scratch)
+        assert cnt[1] - c0[1] >= 2  # S replayed at least on its second factorisation and after
+    finally:
+        S.close()
+        S2.close()
+
+
+def test_set_precision(gpu_ctx):
+    """mom_system_set_precision: a mixed system factored in FP64 for one
+    matrix (the refinement then has nothing to do: no preconditioner
+    applications) and back to complex64 factors for the next; a system
+    created FP64 refuses the mixed path."""
+    A = rand_complex(600, seed=70)
+    b = rand_complex(600, 1, seed=71)
+    S = System(gpu_ctx, 300, MOM_PREC_MIXED)
+    try:
+        for prec, minv in ((MOM_PREC_FP64, False), (MOM_PREC_MIXED, True)):
+            S.set_precision(prec)
+            S.set_matrix(A)
+            S.factor()
+            x = S.solve(b, tol=1e-13)
+            assert berr(A, x[:, 0], b[:, 0]) < 1e-13
+            assert (S.stats()["n_minv"] > 0) == minv
+    finally:
+        S.close()
+    S = System(gpu_ctx, 300, MOM_PREC_FP64)
+    try:
+        with pytest.raises(MomError):
+            S.set_precision(MOM_PREC_MIXED)
+    finally:
+        S.close()

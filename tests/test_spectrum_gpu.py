@@ -7,7 +7,8 @@ with the Drude index, 8 plane waves solved together by its solve_systems).
 Tolerances: matrix entries 1e-10 of their block's largest entry, sources
 1e-12, solutions x and sigma_sca, sigma_abs, rcs 1e-8 (relative), for the
 mixed path (partial-pivoting panels, and tournament-pivoting panels), the
-FP64 path and the mixed path's FP64-factor branch."""
+FP64 path and the mixed path's FP64-factor branch, one wavelength at a time
+and two at a time (lanes)."""
 import os
 
 import numpy as np
@@ -60,19 +61,32 @@ def test_spectrum_setup_and_fill_match_reference(gpu_ctx, spectrum_case):
         assert qerr < 1e-12
 
 
-@pytest.mark.parametrize("mode", ["mixed", "mixed_tournament", "fp64", "mixed_fp64_fallback"])
+@pytest.mark.parametrize("mode", ["mixed", "mixed_tournament", "fp64", "mixed_fp64_fallback",
+                                  "mixed_tournament_2lanes", "fp64_2lanes", "mixed_fp64_fallback_2lanes"])
 def test_spectrum_solutions_match_reference(gpu_ctx, spectrum_case, mode):
+    """Every path of solve_spectrum against the reference's run.  The
+    *_fp64_fallback modes force the block refinement's FP64-factor branch:
+    the wavelength after a fallback is then factored directly in FP64 (the
+    driver's policy), the one after that tries complex64 again.  The 2lanes
+    modes solve the three wavelengths two at a time on two contexts."""
     g, prob = spectrum_case
     dms = [DeviceMesh(gpu_ctx, d) for d in prob.domains]
     N = prob.nedges
     ang = g["angles_deg"]
-    prec = MOM_PREC_FP64 if mode == "fp64" else MOM_PREC_MIXED
-    capi.check(capi.lib().mom_test_batch_fallback(1 if mode == "mixed_fp64_fallback" else 0))
+    prec = MOM_PREC_FP64 if mode.startswith("fp64") else MOM_PREC_MIXED
+    fallback = "fp64_fallback" in mode
+    capi.check(capi.lib().mom_test_batch_fallback(1 if fallback else 0))
     try:
         X, info = solve_spectrum(gpu_ctx, dms, N, g["wl"], lambda wl: [1.0 + 0j, drude_index(wl)], ang,
-                                 precision=prec, pivoting=1 if mode == "mixed_tournament" else 0)
+                                 precision=prec, pivoting=1 if "tournament" in mode else 0,
+                                 lanes=2 if mode.endswith("2lanes") else 1)
     finally:
         capi.check(capi.lib().mom_test_batch_fallback(0))
+    if mode == "mixed_fp64_fallback":
+        assert [r["fell_back"] for r in info] == [True, False, True]
+        assert [r["precision"] for r in info] == ["mixed", "fp64", "mixed"]
+    if mode.endswith("2lanes"):
+        assert sorted(set(r["lane"] for r in info)) == [0, 1]
     for i, wl in enumerate(g["wl"]):
         om = omega_of(wl)
         x_ref = g["x_%d" % i]
@@ -87,9 +101,9 @@ def test_spectrum_solutions_match_reference(gpu_ctx, spectrum_case, mode):
         assert xerr < 1e-8
         assert serr < 1e-8
         assert rerr < 1e-8
-        if mode == "mixed_fp64_fallback":
-            assert info[i]["fell_back"]
-        elif mode in ("mixed", "mixed_tournament"):
+        if fallback:
+            assert info[i]["fell_back"] or info[i]["precision"] == "fp64"
+        elif prec == MOM_PREC_MIXED:
             assert info[i]["fell_back"] or info[i]["berr"] <= 1e-12
     for d in dms:
         d.close()
