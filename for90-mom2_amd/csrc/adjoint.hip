@@ -371,18 +371,22 @@ static int vertex_common(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_tota
   MOM_TRY(make_rule(nq, qw, qnodes, &rule));
   int rc;
   if ((rc = dx.alloc(sizeof(double2) * 2 * nedges_total)) || (rc = dl.alloc(sizeof(double2) * 2 * nedges_total)) ||
-      (rc = dv.alloc(sizeof(double) * 3 * mesh->V)))
+      (rc = dv.alloc(sizeof(double) * 9 * mesh->F)))
     return rc;
   MOM_TRY(copy_h2d(ctx, dx.ptr, x, sizeof(double2) * 2 * nedges_total));
   MOM_TRY(copy_h2d(ctx, dl.ptr, lam, sizeof(double2) * 2 * nedges_total));
-  MOM_HIP(hipMemsetAsync(dv.ptr, 0, dv.bytes, ctx->stream));
   return 0;
 }
 
+// The faces' values (dv: [F][9], every face's own local nodes) summed into
+// the nodes in face order, so that the node gradient does not depend on
+// the order in which the device finished.
 static int vertex_finish(mom_ctx *ctx, const mom_mesh *mesh, DevBuf &dv, double *dvert) {
-  std::vector<double> h(3 * mesh->V);
+  std::vector<double> h(9 * mesh->F);
   MOM_TRY(copy_d2h(ctx, h.data(), dv.ptr, sizeof(double) * h.size()));
-  for (size_t i = 0; i < h.size(); i++) dvert[i] += h[i];
+  for (int64_t f = 0; f < mesh->F; f++)
+    for (int v = 0; v < 3; v++)
+      for (int c = 0; c < 3; c++) dvert[3 * mesh->h_fnodes[3 * f + v] + c] += h[9 * f + 3 * v + c];
   return 0;
 }
 

@@ -381,6 +381,7 @@ int mom_mesh_create(mom_ctx *ctx, int64_t nnodes, const double *p, int64_t nface
     }
   }
   m->h_eparent.assign(eparent, eparent + E);
+  m->h_fnodes.assign(fnodes, fnodes + 3 * F);
   m->h_fedges.assign(fedges, fedges + 3 * F);
   m->h_efaces.assign(efaces, efaces + 2 * E);
   m->h_eside = eside;

@@ -179,8 +179,6 @@ __global__ void quad_points_kernel(int F, int Q, const double *__restrict__ p,
 }
 
 constexpr int kWaves = 4;
-constexpr int kTestPerWave = 4;
-constexpr int kTestPerBlock = kWaves * kTestPerWave;
 
 // Far / smooth kernel: one thread per (test face, source face) pair.
 // QT > 0 fixes the rule size at compile time (full unrolling of the source
@@ -358,7 +356,6 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
   const bool valid = m < F;
   const int mm = valid ? m : 0;
   const double cmx = a.js[0] * a.cp[3 * mm], cmy = a.js[1] * a.cp[3 * mm + 1], cmz = a.js[2] * a.cp[3 * mm + 2];
-  const double cm3[3] = {cmx, cmy, cmz};
   const int64_t plane = (int64_t)pa.S * W;
 
   for (int t = 0; t < kSrcPerWave; t++) {
@@ -2057,10 +2054,16 @@ struct VPairArgs {
   double2 c1, c2, zsi, kk4, kk8;
   const double2 *x, *lam;  // [2N]
   int64_t N;
-  double *dvert;       // [V][3] accumulated
   int fd_only;         // differences for far pairs too (check of the closed form)
   const int *pairs;    // near pairs (test, source), sorted; the near launch takes those with test <= source
+  const int64_t *near_off;  // [F + 1]: the near pairs of test face m are near_off[m] .. near_off[m+1]
   int64_t npairs;
+  // Outputs, all by face: 9 values per face, Re d/d(coordinate c of the
+  // face's local node v) at 3 v + c; every sum has a fixed order.
+  double *part_t;      // [F][ntx][9]: pair tiles, test faces, summed over the tile's source faces
+  double *part_s;      // [F][nty][9]: pair tiles, source faces, summed over the tile's test faces
+  int ntx, nty;
+  double *near_out;    // [npairs][18]: near pair (m <= n): m's 9 values, then n's
 };
 
 __device__ __forceinline__ double2 cm(double2 a, double2 b) {
@@ -2332,12 +2335,16 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
     }
   const double2 c2T2 = cm(g.c2, T2);
   const double kr = g.kr, ki = g.ki, k2r = kr * kr - ki * ki, k2i = 2.0 * kr * ki;
-  // GT/GS: by node (barycentric points); VT/VS: by side (free vertices), mapped
-  // to nodes at the end so that every array index stays static
-  double2 GT[3][3], GS[3][3], VT[3][3], VS[3][3], dAt[3], dAs[3], SG = make_double2(0.0, 0.0);
+  // Only Re d Phi / d(node) is wanted, and every accumulator below is a
+  // real linear function of its terms, so only real parts are summed (the
+  // complex factors are multiplied out before).  GT/GS: by node
+  // (barycentric points); VT/VS: by side (free vertices), mapped to nodes at
+  // the end so that every array index stays static.
+  double GT[3][3], GS[3][3], VT[3][3], VS[3][3], dAt[3], dAs[3];
+  double2 SG = make_double2(0.0, 0.0);
   for (int k = 0; k < 3; k++) {
-    dAt[k] = dAs[k] = make_double2(0.0, 0.0);
-    for (int c = 0; c < 3; c++) GT[k][c] = GS[k][c] = VT[k][c] = VS[k][c] = make_double2(0.0, 0.0);
+    dAt[k] = dAs[k] = 0.0;
+    for (int c = 0; c < 3; c++) GT[k][c] = GS[k][c] = VT[k][c] = VS[k][c] = 0.0;
   }
   const int Q = g.rule.Q;
   for (int i = 0; i < Q; i++) {
@@ -2358,10 +2365,10 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
           Xb[q][c].y += b[p][q].y * ae;
         }
       }
-    double2 dRi[3], dE[3][3], dAti[3];
+    double dRi[3], dE[3][3], dAti[3];
     for (int k = 0; k < 3; k++) {
-      dRi[k] = dAti[k] = make_double2(0.0, 0.0);
-      for (int c = 0; c < 3; c++) dE[k][c] = make_double2(0.0, 0.0);
+      dRi[k] = dAti[k] = 0.0;
+      for (int c = 0; c < 3; c++) dE[k][c] = 0.0;
     }
     for (int j = 0; j < Q; j++) {
       double rs[3];
@@ -2410,144 +2417,112 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
       // T1 = sum_q alpha_q f_q . Xa_q ; T3 = sum_q alpha_q d . (f_q x Xb_q)
       double2 T1 = make_double2(0.0, 0.0), T3 = make_double2(0.0, 0.0), Dx[3];
       for (int c = 0; c < 3; c++) Dx[c] = make_double2(0.0, 0.0);
-      double2 dF[3][3];  // d Psi / d f_q
+      double dF[3][3];  // Re d Psi / d f_q
       const double2 Gc1 = cm(G, g.c1);
       for (int q = 0; q < 3; q++) {
-        double2 fxX[3], Xxd[3];
+        double sa = 0.0;  // Re d Psi / d alpha_q = Re f_q . (G c1 Xa_q + g Xb_q x d)
         for (int c = 0; c < 3; c++) {
           const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-          fxX[c] = make_double2(f[q][c1] * Xb[q][c2].x - f[q][c2] * Xb[q][c1].x,
-                                f[q][c1] * Xb[q][c2].y - f[q][c2] * Xb[q][c1].y);
-          Xxd[c] = make_double2(Xb[q][c1].x * d[c2] - Xb[q][c2].x * d[c1], Xb[q][c1].y * d[c2] - Xb[q][c2].y * d[c1]);
-        }
-        for (int c = 0; c < 3; c++) {
+          const double2 fxX = make_double2(f[q][c1] * Xb[q][c2].x - f[q][c2] * Xb[q][c1].x,
+                                           f[q][c1] * Xb[q][c2].y - f[q][c2] * Xb[q][c1].y);
+          const double2 Xxd = make_double2(Xb[q][c1].x * d[c2] - Xb[q][c2].x * d[c1],
+                                           Xb[q][c1].y * d[c2] - Xb[q][c2].y * d[c1]);
           T1.x += al_s[q] * f[q][c] * Xa[q][c].x;
           T1.y += al_s[q] * f[q][c] * Xa[q][c].y;
-          T3.x += al_s[q] * d[c] * fxX[c].x;
-          T3.y += al_s[q] * d[c] * fxX[c].y;
-          Dx[c].x += al_s[q] * fxX[c].x;
-          Dx[c].y += al_s[q] * fxX[c].y;
-          const double2 t1 = cm(Gc1, Xa[q][c]), t2 = cm(gg, Xxd[c]);
-          dF[q][c] = make_double2(al_s[q] * (t1.x + t2.x), al_s[q] * (t1.y + t2.y));
+          T3.x += al_s[q] * d[c] * fxX.x;
+          T3.y += al_s[q] * d[c] * fxX.y;
+          Dx[c].x += al_s[q] * fxX.x;
+          Dx[c].y += al_s[q] * fxX.y;
+          const double u = (Gc1.x * Xa[q][c].x - Gc1.y * Xa[q][c].y) + (gg.x * Xxd.x - gg.y * Xxd.y);
+          dF[q][c] = al_s[q] * u;
+          sa += f[q][c] * u;
         }
-        // d Psi / d alpha_q = f_q . (G c1 Xa_q + g Xb_q x d)
-        double2 sa = make_double2(0.0, 0.0);
-        for (int c = 0; c < 3; c++) {
-          const double2 t1 = cm(Gc1, Xa[q][c]), t2 = cm(gg, Xxd[c]);
-          sa.x += f[q][c] * (t1.x + t2.x);
-          sa.y += f[q][c] * (t1.y + t2.y);
-        }
-        dAs[q].x += ww * sa.x;
-        dAs[q].y += ww * sa.y;
+        dAs[q] += ww * sa;
       }
-      // d Psi / d R and the total d Psi / d d
+      // Re d Psi / d R and the total Re d Psi / d d
       const double2 c1T1 = cm(g.c1, T1);
-      const double2 dPR1 = cm(Gp, make_double2(c1T1.x + c2T2.x, c1T1.y + c2T2.y)), dPR2 = cm(gp, T3);
-      const double2 dPR = make_double2(dPR1.x + dPR2.x, dPR1.y + dPR2.y);
-      double2 DD[3];
-      for (int c = 0; c < 3; c++) {
-        const double2 ex = cm(gg, Dx[c]);
-        DD[c] = make_double2(ex.x + dPR.x * d[c] * iR, ex.y + dPR.y * d[c] * iR);
-      }
-      // d Psi / d e_p = alpha_p (G c1 Fa_p + g Fb_p), Fa_p = sum_q a_pq alpha_q f_q, Fb_p = sum_q b_pq alpha_q d x f_q
+      const double2 A1 = make_double2(c1T1.x + c2T2.x, c1T1.y + c2T2.y);
+      const double dPR = (Gp.x * A1.x - Gp.y * A1.y) + (gp.x * T3.x - gp.y * T3.y);
+      double DD[3];
+      for (int c = 0; c < 3; c++) DD[c] = (gg.x * Dx[c].x - gg.y * Dx[c].y) + dPR * d[c] * iR;
+      // Re d Psi / d e_p = alpha_p Re (G c1 Fa_p + g Fb_p), Fa_p = sum_q a_pq alpha_q f_q,
+      // Fb_p = sum_q b_pq alpha_q d x f_q
       for (int p = 0; p < 3; p++) {
-        double2 tot[3];
-        for (int c = 0; c < 3; c++) tot[c] = make_double2(0.0, 0.0);
+        double tot[3] = {0.0, 0.0, 0.0};
         for (int q = 0; q < 3; q++) {
           double dxf[3];
           cross3(d, f[q], dxf);
-          const double2 ga = cm(Gc1, a[p][q]), gb = cm(gg, b[p][q]);
-          for (int c = 0; c < 3; c++) {
-            tot[c].x += al_s[q] * (ga.x * f[q][c] + gb.x * dxf[c]);
-            tot[c].y += al_s[q] * (ga.y * f[q][c] + gb.y * dxf[c]);
-          }
+          const double ga = Gc1.x * a[p][q].x - Gc1.y * a[p][q].y, gb = gg.x * b[p][q].x - gg.y * b[p][q].y;
+          for (int c = 0; c < 3; c++) tot[c] += al_s[q] * (ga * f[q][c] + gb * dxf[c]);
         }
-        double2 sa = make_double2(0.0, 0.0);
+        double sa = 0.0;
         for (int c = 0; c < 3; c++) {
-          sa.x += e[p][c] * tot[c].x;
-          sa.y += e[p][c] * tot[c].y;
-          const double2 de = make_double2(al_t[p] * tot[c].x, al_t[p] * tot[c].y);
-          dE[p][c].x += ww * de.x;
-          dE[p][c].y += ww * de.y;
-          dRi[c].x += ww * de.x;
-          dRi[c].y += ww * de.y;
+          sa += e[p][c] * tot[c];
+          const double de = al_t[p] * tot[c];
+          dE[p][c] += ww * de;
+          dRi[c] += ww * de;
         }
-        dAti[p].x += ww * sa.x;
-        dAti[p].y += ww * sa.y;
+        dAti[p] += ww * sa;
       }
-      for (int c = 0; c < 3; c++) {
-        dRi[c].x += ww * DD[c].x;
-        dRi[c].y += ww * DD[c].y;
-      }
+      for (int c = 0; c < 3; c++) dRi[c] += ww * DD[c];
       // source side: r'_j = sum_k b_jk S_k, d depends on -r'_j; free vertices
       const double bj[3] = {g.rule.nodes[2 * Q + j], g.rule.nodes[j], g.rule.nodes[Q + j]};
       for (int c = 0; c < 3; c++) {
-        double2 sf = make_double2(-DD[c].x, -DD[c].y);
+        double sf = -DD[c];
         for (int q = 0; q < 3; q++) {
-          sf.x += dF[q][c].x;
-          sf.y += dF[q][c].y;
-          VS[q][c].x -= ww * dF[q][c].x;
-          VS[q][c].y -= ww * dF[q][c].y;
+          sf += dF[q][c];
+          VS[q][c] -= ww * dF[q][c];
         }
-        for (int k = 0; k < 3; k++) {
-          GS[k][c].x += ww * bj[k] * sf.x;
-          GS[k][c].y += ww * bj[k] * sf.y;
-        }
+        for (int k = 0; k < 3; k++) GS[k][c] += ww * bj[k] * sf;
       }
       SG.x += ww * G.x;
       SG.y += ww * G.y;
     }
     const double bi[3] = {g.rule.nodes[2 * Q + i], g.rule.nodes[i], g.rule.nodes[Q + i]};
     for (int c = 0; c < 3; c++) {
-      for (int k = 0; k < 3; k++) {
-        GT[k][c].x += bi[k] * dRi[c].x;
-        GT[k][c].y += bi[k] * dRi[c].y;
-      }
-      for (int p = 0; p < 3; p++) {
-        VT[p][c].x -= dE[p][c].x;
-        VT[p][c].y -= dE[p][c].y;
-      }
+      for (int k = 0; k < 3; k++) GT[k][c] += bi[k] * dRi[c];
+      for (int p = 0; p < 3; p++) VT[p][c] -= dE[p][c];
     }
-    for (int p = 0; p < 3; p++) {
-      dAt[p].x += dAti[p].x;
-      dAt[p].y += dAti[p].y;
-    }
+    for (int p = 0; p < 3; p++) dAt[p] += dAti[p];
   }
   // the c2 T2 term's edge-length dependence: d/d alpha_p of c2 SG sum a_pq 4 alpha_p alpha_q
   const double2 c2SG = cm(g.c2, SG);
   for (int p = 0; p < 3; p++)
     for (int q = 0; q < 3; q++) {
-      const double2 t = cm(c2SG, a[p][q]);
-      dAt[p].x += 4.0 * al_s[q] * t.x;
-      dAt[p].y += 4.0 * al_s[q] * t.y;
-      dAs[q].x += 4.0 * al_t[p] * t.x;
-      dAs[q].y += 4.0 * al_t[p] * t.y;
+      const double t = c2SG.x * a[p][q].x - c2SG.y * a[p][q].y;
+      dAt[p] += 4.0 * al_s[q] * t;
+      dAs[q] += 4.0 * al_t[p] * t;
     }
   // alpha_p = s_p L_p / 2 with L_p = |P_{p+1} - P_p|
   for (int p = 0; p < 3; p++)
     for (int c = 0; c < 3; c++) {
       const double ht = ft.sg[p] * 0.5 * ut[p][c], hs = fs.sg[p] * 0.5 * us[p][c];
-      GT[(p + 1) % 3][c].x += dAt[p].x * ht;
-      GT[p][c].x -= dAt[p].x * ht;
-      GS[(p + 1) % 3][c].x += dAs[p].x * hs;
-      GS[p][c].x -= dAs[p].x * hs;
+      GT[(p + 1) % 3][c] += dAt[p] * ht;
+      GT[p][c] -= dAt[p] * ht;
+      GS[(p + 1) % 3][c] += dAs[p] * hs;
+      GS[p][c] -= dAs[p] * hs;
     }
   for (int k = 0; k < 3; k++)
     for (int c = 0; c < 3; c++) {
-      double t = GT[k][c].x, u = GS[k][c].x;
+      double t = GT[k][c], u = GS[k][c];
       for (int p = 0; p < 3; p++) {
-        t += ft.lv[p] == k ? VT[p][c].x : 0.0;
-        u += fs.lv[p] == k ? VS[p][c].x : 0.0;
+        t += ft.lv[p] == k ? VT[p][c] : 0.0;
+        u += fs.lv[p] == k ? VS[p][c] : 0.0;
       }
       gt[k][c] = t;
       gs[k][c] = u;
     }
 }
 
-// Re d Phi / d(node coordinate) of the unordered face pair (m <= n) for its
-// distinct nodes, added to dvert (far pairs in closed form, near pairs by
-// differences).
-__device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near, bool analytic_only = false) {
+// Re d Phi / d(node coordinate) of the unordered face pair (m <= n) by
+// fourth-order central differences in the coordinates of its distinct
+// nodes, added to ot[3 v + c] (face m's local node v) and os (face n's; a
+// node of both counts in ot, and a self pair adds only to ot).  The slots
+// are this thread's own (LDS or global memory), so nothing stays in
+// registers across the differences.  analytic_only: the int.f90 analytic
+// parts of a near pair alone.
+__device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near, bool analytic_only, double *ot,
+                            double *os) {
   const bool self = (m == n);
   VFace fm, fn;
   vload_face(g, m, fm);
@@ -2586,48 +2561,82 @@ __device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near, bool an
         }
         val[st] = vpair_value(g, pm, pn, near, self, amn, bmn, anm, bnm, analytic_only).x;
       }
-      unsafeAtomicAdd(g.dvert + 3 * node + c, (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h));
+      const double dv = (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h);
+      int vm = -1, vn = -1;
+      for (int v = 0; v < 3; v++) {
+        if (g.fnodes[3 * m + v] == node) vm = v;
+        if (g.fnodes[3 * n + v] == node) vn = v;
+      }
+      if (vm >= 0)
+        ot[3 * vm + c] += dv;
+      else if (vn >= 0)
+        os[3 * vn + c] += dv;
     }
   }
 }
 
-// Grid launch over (m, n >= m).  FD = false: the far pairs in closed form (a
-// node shared by the two faces receives both faces' partial derivatives);
-// the near pairs have their own launch, so that their differences do not
-// hold whole waves of far pairs.  FD = true: every pair by differences (the
-// check of the closed form).
+// Pair tiles of kVTM test faces x kVTN source faces (one thread per pair,
+// the pairs m <= n).  FD = false: the far pairs in closed form; the near
+// pairs have their own launch, so that their differences do not hold whole
+// waves of far pairs.  FD = true: every pair by differences (the check of
+// the closed form).  Each tile sums its test faces' values over its source
+// faces and its source faces' values over its test faces, in a fixed order,
+// into part_t / part_s (no atomics: the gradient is the same bit for bit
+// from run to run).
+constexpr int kVTM = 8, kVTN = 16;
 template <bool FD>
 __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x, m = blockIdx.y;
-  if (n >= g.F || n < m) return;
-  const bool near = near_pair(g.cp + 3 * m, g.cp + 3 * n, g.thr2);
-  if (FD) {
-    vertex_pair(g, m, n, near);
-    return;
+  __shared__ double st[kVTM][kVTN][9], ss[kVTM][kVTN][9];
+  const int tn = threadIdx.x % kVTN, tm = threadIdx.x / kVTN;
+  const int n0 = blockIdx.x * kVTN, m0 = blockIdx.y * kVTM;
+  if (n0 + kVTN - 1 < m0) return;  // the whole tile below the diagonal (uniform)
+  const int m = m0 + tm, n = n0 + tn;
+  for (int k = 0; k < 9; k++) st[tm][tn][k] = ss[tm][tn][k] = 0.0;
+  if (m < g.F && n < g.F && n >= m) {
+    const bool near = near_pair(g.cp + 3 * m, g.cp + 3 * n, g.thr2);
+    if (FD) {
+      vertex_pair(g, m, n, near, false, &st[tm][tn][0], &ss[tm][tn][0]);
+    } else if (!near) {
+      VFace fm, fn;
+      vload_face(g, m, fm);
+      vload_face(g, n, fn);
+      double2 amn[3][3], bmn[3][3], anm[3][3], bnm[3][3];
+      vcoef(g, m, n, amn, bmn);
+      vcoef(g, n, m, anm, bnm);
+      for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) {
+          amn[p][q] = make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
+          bmn[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
+        }
+      double gt[3][3], gs[3][3];
+      vfar_grad<false>(g, fm, fn, amn, bmn, gt, gs);
+      for (int v = 0; v < 3; v++)
+        for (int c = 0; c < 3; c++) {
+          st[tm][tn][3 * v + c] = gt[v][c];
+          ss[tm][tn][3 * v + c] = gs[v][c];
+        }
+    }
   }
-  if (near) return;
-  VFace fm, fn;
-  vload_face(g, m, fm);
-  vload_face(g, n, fn);
-  double2 amn[3][3], bmn[3][3], anm[3][3], bnm[3][3];
-  vcoef(g, m, n, amn, bmn);
-  vcoef(g, n, m, anm, bnm);
-  for (int p = 0; p < 3; p++)
-    for (int q = 0; q < 3; q++) {
-      amn[p][q] = make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
-      bmn[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
-    }
-  double gt[3][3], gs[3][3];
-  vfar_grad<false>(g, fm, fn, amn, bmn, gt, gs);
-  for (int v = 0; v < 3; v++)
-    for (int c = 0; c < 3; c++) {
-      unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * m + v] + c, gt[v][c]);
-      unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * n + v] + c, gs[v][c]);
-    }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kVTM * 9; e += 128) {
+    const int r = e / 9, k = e % 9;
+    if (m0 + r >= g.F) continue;
+    double t = 0.0;
+    for (int j = 0; j < kVTN; j++) t += st[r][j][k];
+    g.part_t[((int64_t)(m0 + r) * g.ntx + blockIdx.x) * 9 + k] = t;
+  }
+  for (int e = threadIdx.x; e < kVTN * 9; e += 128) {
+    const int j = e / 9, k = e % 9;
+    if (n0 + j >= g.F) continue;
+    double t = 0.0;
+    for (int r = 0; r < kVTM; r++) t += ss[r][j][k];
+    g.part_s[((int64_t)(n0 + j) * g.nty + blockIdx.y) * 9 + k] = t;
+  }
 }
 
 // One thread per entry of the near list with test <= source: the smoothed
-// point loop in closed form, the int.f90 analytic parts by differences.
+// point loop in closed form, the int.f90 analytic parts by differences;
+// the pair's values go to its entry of near_out.
 __global__ __launch_bounds__(128) void vertex_near_kernel(VPairArgs g) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= g.npairs) return;
@@ -2652,13 +2661,58 @@ __global__ __launch_bounds__(128) void vertex_near_kernel(VPairArgs g) {
       }
     double gt[3][3], gs[3][3];
     vfar_grad<true>(g, fm, fn, amn, bmn, gt, gs);
+    if (self)  // both copies of the face are face m
+      for (int v = 0; v < 3; v++)
+        for (int c = 0; c < 3; c++) {
+          gt[v][c] += gs[v][c];
+          gs[v][c] = 0.0;
+        }
+    double *o = g.near_out + 18 * id;
     for (int v = 0; v < 3; v++)
       for (int c = 0; c < 3; c++) {
-        unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * m + v] + c, gt[v][c]);
-        unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * n + v] + c, gs[v][c]);
+        o[3 * v + c] = gt[v][c];
+        o[9 + 3 * v + c] = gs[v][c];
       }
   }
-  vertex_pair(g, m, n, true, true);
+  // the analytic parts, added to the entry
+  double *o = g.near_out + 18 * id;
+  vertex_pair(g, m, n, true, true, o, o + 9);
+}
+
+// Per face: its tile partial sums (in tile order) and its near pairs (in
+// near-list order; a pair listed as (f, n) with n < f has its values in the
+// entry (n, f), found by bisection in n's sorted row), into fg[F][9].
+__global__ __launch_bounds__(128) void vertex_face_sum_kernel(VPairArgs g, double *fg) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.F) return;
+  double acc[9];
+  for (int k = 0; k < 9; k++) acc[k] = 0.0;
+  for (int t = 0; t < g.ntx; t++)
+    for (int k = 0; k < 9; k++) acc[k] += g.part_t[((int64_t)f * g.ntx + t) * 9 + k];
+  for (int t = 0; t < g.nty; t++)
+    for (int k = 0; k < 9; k++) acc[k] += g.part_s[((int64_t)f * g.nty + t) * 9 + k];
+  if (g.near_out) {
+    for (int64_t id = g.near_off[f]; id < g.near_off[f + 1]; id++) {
+      const int n = g.pairs[2 * id + 1];
+      const double *o = nullptr;
+      if (n >= f) {
+        o = g.near_out + 18 * id;
+      } else {
+        int64_t lo = g.near_off[n], hi = g.near_off[n + 1] - 1;
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) / 2;
+          if (g.pairs[2 * mid + 1] < f)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        if (lo < g.near_off[n + 1] && g.pairs[2 * lo + 1] == f) o = g.near_out + 18 * lo + 9;
+      }
+      if (o)
+        for (int k = 0; k < 9; k++) acc[k] += o[k];
+    }
+  }
+  for (int k = 0; k < 9; k++) fg[(int64_t)f * 9 + k] = acc[k];
 }
 
 // Per face of the exterior domain: Psi = the face's share of sigma_abs
@@ -2677,7 +2731,7 @@ struct VFaceArgs {
   double cabs_c;  // -2/(eps0 c0 Re ri)
   const double2 *x, *lam;
   int64_t N;
-  double *dvert;
+  double *fg;  // [F][9]: Re d/d(coordinate c of the face's local node v) at 3 v + c
 };
 
 __device__ double vface_value(const VFaceArgs &g, int f, const VFace &F) {
@@ -2758,13 +2812,12 @@ __global__ __launch_bounds__(128) void vertex_face_kernel(VFaceArgs g) {
         Fs.P[v][c] += steps[st];
         val[st] = vface_value(g, f, Fs);
       }
-      unsafeAtomicAdd(g.dvert + 3 * g.fnodes[3 * f + v] + c,
-                      (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h));
+      g.fg[(int64_t)f * 9 + 3 * v + c] = (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h);
     }
 }
 
 int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
-                        const double2 *x, const double2 *lam, double *dvert, int fd_only) {
+                        const double2 *x, const double2 *lam, double *fg, int fd_only) {
   const cd I(0.0, 1.0);
   const cd k = ri * omega / kC0;
   const cd c1 = -(I * omega) * mu0();
@@ -2792,26 +2845,45 @@ int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
   g.x = x;
   g.lam = lam;
   g.N = N;
-  g.dvert = dvert;
   g.fd_only = fd_only;
   g.pairs = m->near.as<int>();
+  g.near_off = m->near_off.as<int64_t>();
   g.npairs = m->nnear;
+  const int64_t F = m->F;
+  g.ntx = (int)((F + kVTN - 1) / kVTN);
+  g.nty = (int)((F + kVTM - 1) / kVTM);
+  // tiles wholly below the diagonal write nothing: their slots stay zero
+  DevBuf pt, ps, no;
+  int rc;
+  if ((rc = pt.alloc(sizeof(double) * 9 * F * g.ntx)) || (rc = ps.alloc(sizeof(double) * 9 * F * g.nty)) ||
+      (!fd_only && m->nnear > 0 && (rc = no.alloc(sizeof(double) * 18 * m->nnear))))
+    return rc;
+  hipStream_t s = ctx->stream;
+  MOM_HIP(hipMemsetAsync(pt.ptr, 0, pt.bytes, s));
+  MOM_HIP(hipMemsetAsync(ps.ptr, 0, ps.bytes, s));
+  if (no.ptr) MOM_HIP(hipMemsetAsync(no.ptr, 0, no.bytes, s));
+  g.part_t = pt.as<double>();
+  g.part_s = ps.as<double>();
+  g.near_out = no.ptr ? no.as<double>() : nullptr;
+  const dim3 grid((unsigned)g.ntx, (unsigned)g.nty);
   if (fd_only)
-    hipLaunchKernelGGL(vertex_pair_kernel<true>, dim3((unsigned)((m->F + 127) / 128), (unsigned)m->F), dim3(128), 0,
-                       ctx->stream, g);
+    hipLaunchKernelGGL(vertex_pair_kernel<true>, grid, dim3(128), 0, s, g);
   else
-    hipLaunchKernelGGL(vertex_pair_kernel<false>, dim3((unsigned)((m->F + 127) / 128), (unsigned)m->F), dim3(128), 0,
-                       ctx->stream, g);
+    hipLaunchKernelGGL(vertex_pair_kernel<false>, grid, dim3(128), 0, s, g);
   MOM_HIP(hipGetLastError());
-  if (!fd_only && m->nnear > 0) {
-    hipLaunchKernelGGL(vertex_near_kernel, dim3((unsigned)((m->nnear + 127) / 128)), dim3(128), 0, ctx->stream, g);
+  if (g.near_out) {
+    hipLaunchKernelGGL(vertex_near_kernel, dim3((unsigned)((m->nnear + 127) / 128)), dim3(128), 0, s, g);
     MOM_HIP(hipGetLastError());
   }
+  hipLaunchKernelGGL(vertex_face_sum_kernel, dim3((unsigned)((F + 127) / 128)), dim3(128), 0, s, g, fg);
+  MOM_HIP(hipGetLastError());
+  // the partial sums must outlive the queued kernels
+  MOM_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
 int vertex_adjoint_faces(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
-                         const double2 *x, const double2 *lam, double th, double ph, double ps, double *dvert) {
+                         const double2 *x, const double2 *lam, double th, double ph, double ps, double *fg) {
   VFaceArgs g;
   g.F = (int)m->F;
   g.rule = rule;
@@ -2833,7 +2905,7 @@ int vertex_adjoint_faces(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, 
   g.x = x;
   g.lam = lam;
   g.N = N;
-  g.dvert = dvert;
+  g.fg = fg;
   hipLaunchKernelGGL(vertex_face_kernel, dim3((unsigned)((m->F + 127) / 128)), dim3(128), 0, ctx->stream, g);
   MOM_HIP(hipGetLastError());
   return 0;

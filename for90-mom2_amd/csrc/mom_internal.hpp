@@ -133,6 +133,7 @@ struct mom_mesh {
   std::vector<int> h_eparent;
   // host connectivity (face edges, edge faces, edge face-side codes)
   std::vector<int32_t> h_fedges, h_efaces, h_eside;
+  std::vector<int32_t> h_fnodes;  // [F][3] (node gradients are summed through it on the host)
 };
 
 namespace mom {
@@ -238,15 +239,17 @@ int far_fields_no_alloc(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
 int fill_system_symm(mom_ctx *ctx, int32_t ndom, const mom_mesh *const *doms, const double *ri, int64_t N,
                      const int32_t *bnd, const Group &G, double omega, int32_t nq, const double *qw,
                      const double *qnodes, double2 *const *Z, const int64_t *ld, const int64_t *order);
-// Vertex-coordinate adjoint (fill.hip), dvert[3V] (device) accumulated:
-// Re d/dP of lambda^T Z_d x for the domain's blocks (central differences of
-// each face pair's contraction, frozen near/far classification) ...
-// (fd_only: the far pairs by differences too, as a check of their closed form)
+// Vertex-coordinate adjoint (fill.hip), fg[F][9] (device) written per face,
+// Re d/d(coordinate c of the face's local node v) at 3 v + c:
+// Re d/dP of lambda^T Z_d x for the domain's blocks (far pairs in closed
+// form, near pairs' smoothed part in closed form and analytic part by
+// central differences, frozen near/far classification) ...
+// (fd_only: every pair by differences, as a check of the closed forms)
 int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
-                        const double2 *x, const double2 *lam, double *dvert, int fd_only = 0);
+                        const double2 *x, const double2 *lam, double *fg, int fd_only = 0);
 // ... and of sigma_abs|_x + Re(lambda^T q) for the exterior domain and one plane wave
 int vertex_adjoint_faces(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t N,
-                         const double2 *x, const double2 *lam, double th, double ph, double ps, double *dvert);
+                         const double2 *x, const double2 *lam, double th, double ph, double ps, double *fg);
 // Transpose of the far-field map (fill.hip): g[2 Nt] (host) = L^T v for
 // adjoint values v[8 ndir] of (et, ep, ht, hp) per direction (host arrays)
 int far_fields_adjoint(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri, int64_t Nt,

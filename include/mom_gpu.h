@@ -211,10 +211,13 @@ int mom_pmchwt_deps_apply(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_tot
  * nodes, host); mom_cs_abs_vertex_explicit adds d/dP (sigma_abs at fixed x +
  * Re lambda^T q) for the exterior mesh and one plane wave (cs.f90:14-104,
  * source.f90:136-247).  Each face pair's (or face's) contribution is rebuilt
- * from its nodes with the fill's formulas and differentiated by the
- * fourth-order central difference (h = 1e-3 of its mean edge length) in
- * every coordinate of its nodes; the near/far classification stays that of
- * the unperturbed mesh.
+ * from its nodes with the fill's formulas and differentiated: far pairs and
+ * the near pairs' smoothed point sums in closed form, the near pairs'
+ * int.f90 analytic parts and the face terms by the fourth-order central
+ * difference (h = 1e-3 of the mean edge length) in every coordinate of
+ * their nodes; the near/far classification stays that of the mesh (or the
+ * one frozen by mom_mesh_classify_like).  Every sum has a fixed order: the
+ * result is the same bit for bit from run to run.
  * x, lambda: [2N] complex (host).  The caller sums the domains' dvert through
  * their nodes' master indices.
  */

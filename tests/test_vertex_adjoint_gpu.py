@@ -217,3 +217,21 @@ def test_classify_like_keeps_or_freezes_the_near_list(gpu_ctx):
         d3.classify_like(dms[0])
     for d in dms + [d2, d3]:
         d.close()
+
+
+def test_vertex_gradient_is_bitwise_reproducible(gpu_ctx):
+    """Every sum of the node-coordinate adjoint has a fixed order (pair
+    tiles, near-list entries and faces, then the faces into the nodes on the
+    host): the gradient of the same solve computed twice is bitwise equal."""
+    nodes, faces = meshgen.icosphere(2, 50.0)
+    P, dms, S, x = _solve(gpu_ctx, nodes, faces, MOM_PREC_FP64)
+    try:
+        _, g1 = sigma_abs_vertex_gradient(gpu_ctx, S, x, dms, P, OMEGA, RI, ANG)
+        _, g2 = sigma_abs_vertex_gradient(gpu_ctx, S, x, dms, P, OMEGA, RI, ANG)
+    finally:
+        S.close()
+        for d in dms:
+            d.close()
+    g1, g2 = np.asarray(g1), np.asarray(g2)
+    assert np.abs(g1).max() > 0
+    assert np.array_equal(g1, g2)
