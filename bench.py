@@ -7,8 +7,9 @@ edges, N = 61440 unknowns J and M), exterior index 1, interior
 eps = -9.4+1.5i, vacuum wavelength 600 nm, x-polarised plane wave along +z.
 
 One step = the whole solve on the device: PMCHWT fill (complex128), FP32
-matrix-core LU of the equilibrated complex64 copy, plane-wave RHS and
-GMRES iterative refinement against the complex128 matrix to backward error
+matrix-core LU of the equilibrated complex64 copy (tournament pivoting;
+--pmchwt-pivoting partial for LAPACK's pivots), plane-wave RHS and GMRES
+iterative refinement against the complex128 matrix to backward error
 <= 1e-12.  `value` is the mean wall time of a step [s].  After the timed
 steps the all-FP64 GPU path (complex128 fill + LU + solve) is timed once on
 the same problem, and the cross-sections of both solutions are compared.
@@ -64,13 +65,14 @@ def build_case(subdiv):
                 ri=[1.0 + 0j, complex(np.sqrt(-9.4 + 1.5j))])
 
 
-def run_path(ctx, dms, case, prec, profile=0, tol=1e-12):
+def run_path(ctx, dms, case, prec, profile=0, tol=1e-12, pivoting=0):
     from mom2amd import DEGTORAD, System, cs_prtsrf, rcs
     from mom2amd import capi
 
     N = case["problem"].nedges
     ang = np.array([0.0, 90.0, 0.0]) * DEGTORAD
     S = System(ctx, N, prec)
+    S.set_pivoting(pivoting)
     if profile:
         capi.check(capi.lib().mom_system_profile(S.h, profile))
         ctx.fill_profile(1)  # events around the fill's pair and assembly launches
@@ -714,9 +716,11 @@ def main():
     ap.add_argument("--cell-n", type=int, default=50, help="octasphere divisions of the symmetric workload")
     ap.add_argument("--nwl", type=int, default=64, help="wavelengths of the spectrum workload")
     ap.add_argument("--lanes", type=int, default=4, help="wavelengths solved at once in the spectrum workload")
+    ap.add_argument("--pmchwt-pivoting", choices=("partial", "tournament"), default="tournament",
+                    help="complex64 panels of the pmchwt workload's mixed path (the FP64 path always pivots as LAPACK)")
     ap.add_argument("--pivoting", choices=("tournament", "partial"), default="tournament",
                     help="complex64 panels of the mixed path in the spectrum, sh and adjoint workloads (the library "
-                         "default, and the pmchwt workload's, is partial pivoting)")
+                         "default is partial pivoting)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -749,15 +753,16 @@ def main():
                                                                  2 * N, time.perf_counter() - t0))
     ctx = Context(0)
     dms = [DeviceMesh(ctx, d) for d in case["problem"].domains]
+    piv = 1 if args.pmchwt_pivoting == "tournament" else 0
     for i in range(args.warmup):
-        st, _ = run_path(ctx, dms, case, MOM_PREC_MIXED)
+        st, _ = run_path(ctx, dms, case, MOM_PREC_MIXED, pivoting=piv)
         log("warmup %d: %.2f s (fill %.2f factor %.2f solve %.2f)" % (i, st["wall_s"], st["fill_s"], st["factor_s"],
                                                                      st["solve_s"]))
     times, gemm_s, gemm_flops, gemm_launches, last = [], 0.0, 0.0, 0, None
     fprof = {}
     for i in range(args.steps):
         # timed steps: HIP events around the depth-256 trailing updates only
-        st, cs_mixed = run_path(ctx, dms, case, MOM_PREC_MIXED, profile=1)
+        st, cs_mixed = run_path(ctx, dms, case, MOM_PREC_MIXED, profile=1, pivoting=piv)
         times.append(st["wall_s"])
         gemm_s += st["gemm_s"]
         gemm_flops += st["gemm_flops"]
@@ -771,7 +776,7 @@ def main():
                st["n_matvec"], " ".join("%.1e" % v for v in st["berr_history"]), st["gemm_s"]))
     mixed_s = float(np.mean(times))
     # one more, untimed step with every kernel group timed, for the factor breakdown
-    bd, _ = run_path(ctx, dms, case, MOM_PREC_MIXED, profile=2)
+    bd, _ = run_path(ctx, dms, case, MOM_PREC_MIXED, profile=2, pivoting=piv)
     log("breakdown step (all groups timed, not in value): %.2f s; factor %.2f: gemm %.2f (narrow %.2f) panel %.2f "
         "trsm %.2f laswp %.2f" % (bd["wall_s"], bd["factor_s"], bd["gemm_s"], bd["gemm_narrow_s"], bd["panel_s"],
                                   bd["trsm_s"], bd["laswp_s"]))
@@ -799,7 +804,7 @@ def main():
         "config": {"workload": "PMCHWT sphere R=50nm icosphere s=%d (%d faces), N=%d unknowns, 600 nm, "
                                "eps_in=-9.4+1.5i, 1 plane wave; fill+LU+solve, GMRES-IR to backward err 1e-12"
                                % (args.subdiv, case["problem"].domains[0].nfaces, 2 * N),
-                   "n_unknowns": 2 * N},
+                   "n_unknowns": 2 * N, "complex64_pivoting": args.pmchwt_pivoting},
         "phases_s": {"fill": round(last["fill_s"], 4), "factor": round(last["factor_s"], 4),
                      "solve_ir": round(last["solve_s"], 4), "ir_iters": last["iters"], "ir_factor_solves": last["n_minv"],
                      "ir_fp64_products": last["n_matvec"], "berr": last["berr"]},
