@@ -156,11 +156,12 @@ int make_rule(int32_t nq, const double *qw, const double *qnodes, QuadRule *r) {
 
 // Near face pairs: centroid distance^2 < (3*avelen)^2 (sysmat.f90:233-248),
 // evaluated exactly as the reference does, found with a uniform grid.  With
-// js, the pairs of the test faces reflected by J = diag(js) and the source
-// faces: |J c_m - c_n|^2, which has the bits of the reference's
-// |c_m - J c_n|^2 since J only flips signs.
+// J, the pairs of the test faces and the source faces moved by J:
+// |c_m - J c_n|^2 in the reference's order of operations
+// (image_near_diff), the candidates taken around J^T c_m.
 void near_pairs(const std::vector<double> &cp, int64_t F, double avelen, std::vector<int32_t> &out,
-                const double *js) {
+                const double *J) {
+#pragma clang fp contract(off)
   const double thr = avelen * 3;
   const double thr2 = thr * thr;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -187,10 +188,10 @@ void near_pairs(const std::vector<double> &cp, int64_t F, double avelen, std::ve
     double tc[3];
     int64_t tcell[3];
     for (int c = 0; c < 3; c++) {
-      tc[c] = js ? js[c] * cp[3 * m + c] : cp[3 * m + c];
-      // a reflected centroid may lie outside the grid: cells beyond it are skipped below
+      tc[c] = J ? J[c] * cp[3 * m] + J[3 + c] * cp[3 * m + 1] + J[6 + c] * cp[3 * m + 2] : cp[3 * m + c];
+      // a moved centroid may lie outside the grid: cells beyond it are skipped below
       const double u = (tc[c] - lo[c]) / cell;
-      tcell[c] = js ? (u < -2.0 ? -3 : std::min(dim[c] + 1, (int64_t)std::floor(u))) : cell_of[3 * m + c];
+      tcell[c] = J ? (u < -2.0 ? -3 : std::min(dim[c] + 1, (int64_t)std::floor(u))) : cell_of[3 * m + c];
     }
     for (int64_t a = -1; a <= 1; a++)
       for (int64_t b = -1; b <= 1; b++)
@@ -200,8 +201,15 @@ void near_pairs(const std::vector<double> &cp, int64_t F, double avelen, std::ve
           auto it = grid.find(key(i, j, k));
           if (it == grid.end()) continue;
           for (int32_t n : it->second) {
-            double d0 = tc[0] - cp[3 * n], d1 = tc[1] - cp[3 * n + 1], d2 = tc[2] - cp[3 * n + 2];
-            double ds = d0 * d0 + d1 * d1 + d2 * d2;
+            double d[3];
+            if (J) {
+              image_near_diff(J, &cp[3 * m], &cp[3 * n], d);
+            } else {
+              d[0] = tc[0] - cp[3 * n];
+              d[1] = tc[1] - cp[3 * n + 1];
+              d[2] = tc[2] - cp[3 * n + 2];
+            }
+            double ds = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
             if (ds < thr2) nb.push_back(n);
           }
         }
@@ -540,7 +548,7 @@ int mom_cs_prtsrf_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total,
                        double ri_im, int32_t ngen, const int32_t *planes, const double *x, double theta, double phi,
                        double psi, int32_t nq, const double *qw, const double *qnodes, double *csca, double *cabs) {
   Group G;
-  MOM_TRY(mirror_group(ngen, planes, &G));
+  MOM_TRY(symm_group(ngen, planes, &G));
   return cs_impl(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, theta, phi, psi, nq, qw, qnodes, csca, cabs, G);
 }
 
@@ -551,7 +559,7 @@ int mom_srcvec_pw_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total,
   if (nsrc < 1 || nedges_total < 1 || ldq < 2 * nedges_total || mesh->max_parent >= nedges_total)
     return fail(MOM_ERR_ARG, "bad sizes");
   Group G;
-  MOM_TRY(mirror_group(ngen, planes, &G));
+  MOM_TRY(symm_group(ngen, planes, &G));
   QuadRule rule;
   int rc = make_rule(nq, qw, qnodes, &rule);
   if (rc) return rc;
@@ -671,7 +679,7 @@ int mom_rcs_ext_symm(mom_ctx *ctx, const mom_mesh *mesh, int64_t nedges_total, d
                      int32_t filter, const double *fdir, int32_t nq, const double *qw, const double *qnodes,
                      double *rcs) {
   Group G;
-  MOM_TRY(mirror_group(ngen, planes, &G));
+  MOM_TRY(symm_group(ngen, planes, &G));
   return rcs_impl(ctx, mesh, nedges_total, omega, ri_re, ri_im, x, ntheta, nphi, filter, fdir, nq, qw, qnodes, rcs,
                   G);
 }

@@ -57,7 +57,9 @@ struct FarArgs {
   double thr2;        // near threshold^2
   double2 c1, c2;     // D and H prefactors
   double js[3];       // test faces reflected by diag(js) (all 1: no image)
-  int image;          // js is a mirror image (the self-pair rule does not apply)
+  int image;          // an image (the self-pair rule does not apply)
+  int gen;            // a non-diagonal image: test faces moved by M, near rule |c_m - J c_n|
+  double M[9], J[9];  // M = J^T, row-major (gen)
   FillTarget out;
 };
 
@@ -146,19 +148,24 @@ __device__ __forceinline__ void store_reps(const FillTarget &o, int64_t gi, int6
   for (int r = 0; r < o.nrep; r++) {
     const int *mp = o.rep.map[r];
     const int a = mp[gi], b = mp[gj], am = mp[gi + N], bm = mp[gj + N];
-    const double e = o.rep.e[r], ed = e * o.detj;
+    const double2 e = o.rep.e[r], ed = make_double2(e.x * o.detj, e.y * o.detj);
     const int64_t ld = o.rep.ld[r];
     double2 *Z = o.rep.Z[r];
-    auto add = [&](int i, int j, double vr, double vi) {
+    // adds w (vr + i vi); a real field action (mirror groups) keeps the
+    // products of the real case
+    auto add = [&](int i, int j, double2 w, double vr, double vi) {
       if (i < 0 || j < 0) return;
       double2 *z = Z + i + (int64_t)j * ld;
       const double2 v = *z;
-      *z = make_double2(v.x + vr, v.y + vi);
+      if (w.y == 0.0)
+        *z = make_double2(v.x + w.x * vr, v.y + w.x * vi);
+      else
+        *z = make_double2(v.x + (w.x * vr - w.y * vi), v.y + (w.x * vi + w.y * vr));
     };
-    add(a, b, e * ar, e * ai);
-    add(am, bm, ed * mr, ed * mi);
-    add(am, b, ed * kr, ed * ki);
-    add(a, bm, -e * kr, -e * ki);
+    add(a, b, e, ar, ai);
+    add(am, bm, ed, mr, mi);
+    add(am, b, ed, kr, ki);
+    add(a, bm, e, -kr, -ki);
   }
 }
 
@@ -340,7 +347,16 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
     const int l = idx % 64, jc = idx / 64;
     const int m = m0 + l;
     double v = 0.0;
-    if (m < F) v = a.js[jc % 3] * (a.qp[(int64_t)m * 3 * Q + jc] - a.cp[3 * m + jc % 3]);
+    if (m < F) {
+      if (a.gen) {
+        const int c = jc % 3;
+        const double *q = a.qp + (int64_t)m * 3 * Q + (jc - c);
+        const double *cm = a.cp + 3 * m;
+        v = a.M[3 * c] * (q[0] - cm[0]) + a.M[3 * c + 1] * (q[1] - cm[1]) + a.M[3 * c + 2] * (q[2] - cm[2]);
+      } else {
+        v = a.js[jc % 3] * (a.qp[(int64_t)m * 3 * Q + jc] - a.cp[3 * m + jc % 3]);
+      }
+    }
     tpts[jc * 64 + l] = v;
   }
   for (int idx = threadIdx.x; idx < kSrcPerBlock * Q * 3; idx += blockDim.x) {
@@ -355,7 +371,17 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
   const int m = m0 + lane;
   const bool valid = m < F;
   const int mm = valid ? m : 0;
-  const double cmx = a.js[0] * a.cp[3 * mm], cmy = a.js[1] * a.cp[3 * mm + 1], cmz = a.js[2] * a.cp[3 * mm + 2];
+  double cmx, cmy, cmz;  // the (moved) test centroid
+  if (a.gen) {
+    const double *c = a.cp + 3 * mm;
+    cmx = a.M[0] * c[0] + a.M[1] * c[1] + a.M[2] * c[2];
+    cmy = a.M[3] * c[0] + a.M[4] * c[1] + a.M[5] * c[2];
+    cmz = a.M[6] * c[0] + a.M[7] * c[1] + a.M[8] * c[2];
+  } else {
+    cmx = a.js[0] * a.cp[3 * mm];
+    cmy = a.js[1] * a.cp[3 * mm + 1];
+    cmz = a.js[2] * a.cp[3 * mm + 2];
+  }
   const int64_t plane = (int64_t)pa.S * W;
 
   for (int t = 0; t < kSrcPerWave; t++) {
@@ -369,8 +395,17 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
     // acc[2..7] = S_rho' (source), acc[8..13] = S_rho (test),
     // acc[18..23] = M_rho', acc[24..29] = M_rho, acc[30..35] = -M_x.
     const double dx = cmx - cpn[0], dy = cmy - cpn[1], dz = cmz - cpn[2];
-    const double clm[3] = {a.js[0] * a.ccls[3 * mm], a.js[1] * a.ccls[3 * mm + 1], a.js[2] * a.ccls[3 * mm + 2]};
-    const bool near = near_pair(clm, a.ccls + 3 * n, a.thr2);
+    bool near;
+    if (a.gen) {
+      // the reference's |c_m - J c_n| with the host list's rounding
+      double d[3];
+      image_near_diff(a.J, a.ccls + 3 * mm, a.ccls + 3 * n, d);
+      const double zero[3] = {0.0, 0.0, 0.0};
+      near = near_pair(d, zero, a.thr2);
+    } else {
+      const double clm[3] = {a.js[0] * a.ccls[3 * mm], a.js[1] * a.ccls[3 * mm + 1], a.js[2] * a.ccls[3 * mm + 2]};
+      near = near_pair(clm, a.ccls + 3 * n, a.thr2);
+    }
     double acc[36];
     for (int k = 0; k < 36; k++) acc[k] = 0.0;
     if (near)
@@ -392,8 +427,17 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
       for (int p = 0; p < 3; p++) {
         const int sp = 3 * mm + p;
         const double coefm = a.side.coef[sp], divm = a.side.div[sp];
-        const double vm[3] = {a.js[0] * a.side.v[3 * sp] - cmx, a.js[1] * a.side.v[3 * sp + 1] - cmy,
-                              a.js[2] * a.side.v[3 * sp + 2] - cmz};
+        double vm[3];
+        if (a.gen) {
+          const double *v = a.side.v + 3 * sp;
+          vm[0] = (a.M[0] * v[0] + a.M[1] * v[1] + a.M[2] * v[2]) - cmx;
+          vm[1] = (a.M[3] * v[0] + a.M[4] * v[1] + a.M[5] * v[2]) - cmy;
+          vm[2] = (a.M[6] * v[0] + a.M[7] * v[1] + a.M[8] * v[2]) - cmz;
+        } else {
+          vm[0] = a.js[0] * a.side.v[3 * sp] - cmx;
+          vm[1] = a.js[1] * a.side.v[3 * sp + 1] - cmy;
+          vm[2] = a.js[2] * a.side.v[3 * sp + 2] - cmz;
+        }
         // D: Srr' - nu_n.Srho - nu_m.Srho' + (nu_m.nu_n) S0
         const double vmn = vm[0] * vn[0] + vm[1] * vn[1] + vm[2] * vn[2];
         double sr = acc[14] + vmn * acc[0], si = acc[15] + vmn * acc[1];
@@ -743,8 +787,22 @@ struct NearArgs {
   double2 c1, c2;
   double js[3];  // test faces reflected by diag(js)
   int image;
+  int gen;       // test points and free vertices moved by M = J^T (non-diagonal image)
+  double M[9];
   double2 *nb;   // per pair and (test side, source side): D+H (or the selected operator), K
 };
+
+// M x for the near rule's test points (MATMUL(TRANSPOSE(J), r) in the
+// reference's order, no contraction)
+__device__ __forceinline__ void mat3_apply(const double *M, const double *x, double *y) {
+#pragma clang fp contract(off)
+  for (int i = 0; i < 3; i++) {
+    double t = M[3 * i] * x[0];
+    t = t + M[3 * i + 1] * x[1];
+    t = t + M[3 * i + 2] * x[2];
+    y[i] = t;
+  }
+}
 
 __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -771,7 +829,10 @@ __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
   double coefm[3], vm[3][3];
   for (int p = 0; p < 3; p++) {
     coefm[p] = a.side.coef[3 * m + p];
-    for (int k = 0; k < 3; k++) vm[p][k] = a.js[k] * a.side.v[3 * (3 * m + p) + k];
+    if (a.gen)
+      mat3_apply(a.M, a.side.v + 3 * (3 * m + p), vm[p]);
+    else
+      for (int k = 0; k < 3; k++) vm[p][k] = a.js[k] * a.side.v[3 * (3 * m + p) + k];
   }
   double accD[3][3][2], accK[3][3][2], accH[3][2];
   for (int p = 0; p < 3; p++)
@@ -780,7 +841,14 @@ __global__ __launch_bounds__(128) void near_fill_kernel(NearArgs a) {
 
   for (int i = 0; i < a.Q; i++) {
     const double *rq = a.qp + ((int64_t)m * a.Q + i) * 3;
-    const double r[3] = {a.js[0] * rq[0], a.js[1] * rq[1], a.js[2] * rq[2]};
+    double r[3];
+    if (a.gen) {
+      mat3_apply(a.M, rq, r);
+    } else {
+      r[0] = a.js[0] * rq[0];
+      r[1] = a.js[1] * rq[1];
+      r[2] = a.js[2] * rq[2];
+    }
     PointInts P;
     d_point_ints(r, f, P);
     // intK31 / intK3m1 (int.f90:330-374), shared by the three edges
@@ -1060,8 +1128,17 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   a.thr2 = thr * thr;
   a.c1 = make_double2(c1.real(), c1.imag());
   a.c2 = make_double2(c2.real(), c2.imag());
-  for (int c = 0; c < 3; c++) a.js[c] = img ? img->js[c] : 1.0;
   a.image = img ? 1 : 0;
+  a.gen = (img && !img->diag) ? 1 : 0;
+  for (int c = 0; c < 9; c++) {
+    a.M[c] = img ? img->M[c] : (c % 4 == 0 ? 1.0 : 0.0);
+    a.J[c] = img ? img->M[3 * (c % 3) + c / 3] : (c % 4 == 0 ? 1.0 : 0.0);
+  }
+  for (int c = 0; c < 3; c++) a.js[c] = a.M[4 * c];
+  // the transposed blocks of an image belong to its inverse element: an
+  // image that is not its own inverse (a rotation by other than pi)
+  // integrates every ordered pair and has no transposed assembly
+  const bool all_pairs = img && !img->invol;
   a.out = t;
   // far/smooth pairs, slab by slab of source faces, through the face-side
   // planes (no atomics on the matrix)
@@ -1113,7 +1190,8 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   FillTimer timer(ctx);
   for (int k = 0; k < nslab; k++) {
     pa.n0 = k * S;
-    pa.mlo = pa.n0;  // pairs with both faces below n0 were integrated by earlier slabs
+    // pairs with both faces below n0 were integrated by earlier slabs
+    pa.mlo = all_pairs ? 0 : pa.n0;
     const int scur = (int)std::min<int64_t>(S, m->F - pa.n0);
     auto grid_of = [&](int spw) {
       return dim3((unsigned)((m->F - pa.mlo + 63) / 64), (unsigned)((scur + kWaves * spw - 1) / (kWaves * spw)));
@@ -1172,7 +1250,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
       // transposed blocks: test sides in the slab, source sides beyond it
       const int beyond = pa.n0 + scur;
       int nb = 0;
-      while (nb < (int)m->E && maxface[bydesc[nb]] >= beyond) nb++;
+      while (!all_pairs && nb < (int)m->E && maxface[bydesc[nb]] >= beyond) nb++;
       if (nb > 0) {
         hipLaunchKernelGGL(assemble_transposed_kernel, dim3((unsigned)((ne + kTR - 1) / kTR), (unsigned)((nb + kTC - 1) / kTC)),
                            dim3(256), 0, ctx->stream, ga, ddesc.as<int>(), ne, nb);
@@ -1192,7 +1270,7 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
   const int64_t *off = m->near_off.as<int64_t>();
   if (img) {
     std::vector<int32_t> np;
-    near_pairs(m->h_cls(), m->F, m->avelen, np, img->js);
+    near_pairs(m->h_cls(), m->F, m->avelen, np, a.J);
     nnear = (int64_t)np.size() / 2;
     if (nnear > 0) {
       std::vector<int64_t> ho;
@@ -1232,6 +1310,8 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     b.c2 = a.c2;
     for (int c = 0; c < 3; c++) b.js[c] = a.js[c];
     b.image = a.image;
+    b.gen = a.gen;
+    for (int c = 0; c < 9; c++) b.M[c] = a.M[c];
     b.nb = ctx->scratch.as<double2>();
     hipEvent_t ev = timer.begin();
     hipLaunchKernelGGL(near_fill_kernel, dim3((unsigned)((nnear + 127) / 128)), dim3(128), 0,
@@ -1289,8 +1369,18 @@ __device__ __forceinline__ void pw_eh(const PwSrc &S, const double *r, double kr
   }
 }
 
+// y = J x and y = J^T x for a row-major 3 x 3 J (MATMUL order)
+__device__ __forceinline__ void jac_mul(const double *J, const double *x, double *y) {
+  for (int i = 0; i < 3; i++) y[i] = J[3 * i] * x[0] + J[3 * i + 1] * x[1] + J[3 * i + 2] * x[2];
+}
+__device__ __forceinline__ void jac_tmul(const double *J, const double *x, double *y) {
+  for (int i = 0; i < 3; i++) y[i] = J[i] * x[0] + J[3 + i] * x[1] + J[6 + i] * x[2];
+}
+
 // w[(col 3F + 3 f + p) 2 + {0, 1}]: <f_p, E_r>, <f_p, H_r> of face side p of
-// face f for column col = source * G.n + representation
+// face f for column col = source * G.n + representation.  With a group
+// (src_field_frags, source.f90:250-315): E_r(r) = (1/n) sum_g ef[g][r] J_g
+// E(J_g^T r), H_r the same with det J_g.
 __global__ void srcvec_kernel(int F, int Q, QuadRule rule, const double *__restrict__ qp,
                               const double *__restrict__ area, FaceSide side, int nsrc,
                               const PwSrc *__restrict__ src, double kr, double ki, double2 hfac,
@@ -1310,15 +1400,23 @@ __global__ void srcvec_kernel(int F, int Q, QuadRule rule, const double *__restr
     } else {
       for (int c = 0; c < 3; c++) E[c][0] = E[c][1] = H[c][0] = H[c][1] = 0.0;
       for (int g = 0; g < G.n; g++) {
-        const double pt[3] = {G.js[g][0] * r[0], G.js[g][1] * r[1], G.js[g][2] * r[2]};
+        double pt[3];
+        jac_tmul(G.J[g], r, pt);
         double Eg[3][2], Hg[3][2];
         pw_eh(S, pt, kr, ki, hfac, Eg, Hg);
-        const double e = G.ef[g][rep], eh = e * G.detj[g];
+        const double2 e = G.ef[g][rep], eh = make_double2(e.x * G.detj[g], e.y * G.detj[g]);
+        double Er[3], Ei[3], Hr[3], Hi[3];
+        const double er[3] = {Eg[0][0], Eg[1][0], Eg[2][0]}, ei[3] = {Eg[0][1], Eg[1][1], Eg[2][1]};
+        const double hr[3] = {Hg[0][0], Hg[1][0], Hg[2][0]}, hi[3] = {Hg[0][1], Hg[1][1], Hg[2][1]};
+        jac_mul(G.J[g], er, Er);
+        jac_mul(G.J[g], ei, Ei);
+        jac_mul(G.J[g], hr, Hr);
+        jac_mul(G.J[g], hi, Hi);
         for (int c = 0; c < 3; c++) {
-          E[c][0] += G.js[g][c] * Eg[c][0] * e;
-          E[c][1] += G.js[g][c] * Eg[c][1] * e;
-          H[c][0] += G.js[g][c] * Hg[c][0] * eh;
-          H[c][1] += G.js[g][c] * Hg[c][1] * eh;
+          E[c][0] += Er[c] * e.x - Ei[c] * e.y;
+          E[c][1] += Ei[c] * e.x + Er[c] * e.y;
+          H[c][0] += Hr[c] * eh.x - Hi[c] * eh.y;
+          H[c][1] += Hi[c] * eh.x + Hr[c] * eh.y;
         }
       }
       for (int c = 0; c < 3; c++) {
@@ -1434,27 +1532,44 @@ __global__ void cs_kernel(int F, int Q, QuadRule rule, const double *__restrict_
   for (int t = 0; t < Q; t++) {
     const double *r0 = qp + ((int64_t)f * Q + t) * 3;
    for (int gi = 0; gi < G.n; gi++) {
-    const double *js = G.js[gi];
-    const double nor[3] = {js[0] * nor0[0], js[1] * nor0[1], js[2] * nor0[2]};
-    const double r[3] = {js[0] * r0[0], js[1] * r0[1], js[2] * r0[2]};
+    const double *Jg = G.J[gi];
+    const bool dg = Jg[1] == 0.0 && Jg[2] == 0.0 && Jg[3] == 0.0 && Jg[5] == 0.0 && Jg[6] == 0.0 && Jg[7] == 0.0;
+    double nor[3], r[3];
+    jac_mul(Jg, nor0, nor);
+    jac_mul(Jg, r0, r);
     double Et[3][2] = {{0, 0}, {0, 0}, {0, 0}}, Ht[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     for (int rep = 0; rep < G.n; rep++) {
-      const double e = G.ef[gi][rep], eh = e * G.detj[gi];
+      const double2 e = G.ef[gi][rep], eh = make_double2(e.x * G.detj[gi], e.y * G.detj[gi]);
       const double2 *xr = x + rep * ldx;
       for (int p = 0; p < 3; p++) {
         const double cf = side.coef[3 * f + p];
         const double *v = side.v + 3 * (3 * f + p);
         double fv[3] = {cf * (r0[0] - v[0]), cf * (r0[1] - v[1]), cf * (r0[2] - v[2])};
-        double fn[3] = {nor0[1] * fv[2] - nor0[2] * fv[1], nor0[2] * fv[0] - nor0[0] * fv[2],
-                        nor0[0] * fv[1] - nor0[1] * fv[0]};
-        for (int c = 0; c < 3; c++) fn[c] *= js[c];
+        double fn0[3] = {nor0[1] * fv[2] - nor0[2] * fv[1], nor0[2] * fv[0] - nor0[0] * fv[2],
+                         nor0[0] * fv[1] - nor0[1] * fv[0]};
+        double fn[3];
+        if (dg)
+          for (int c = 0; c < 3; c++) fn[c] = fn0[c] * Jg[4 * c];
+        else
+          jac_mul(Jg, fn0, fn);
         const int g = side.g[3 * f + p];
         const double2 xj = xr[g], xm = xr[g + Nt];
-        for (int c = 0; c < 3; c++) {
-          Ht[c][0] -= xj.x * fn[c] * eh;
-          Ht[c][1] -= xj.y * fn[c] * eh;
-          Et[c][0] += xm.x * fn[c] * e;
-          Et[c][1] += xm.y * fn[c] * e;
+        if (e.y == 0.0) {
+          for (int c = 0; c < 3; c++) {
+            Ht[c][0] -= xj.x * fn[c] * eh.x;
+            Ht[c][1] -= xj.y * fn[c] * eh.x;
+            Et[c][0] += xm.x * fn[c] * e.x;
+            Et[c][1] += xm.y * fn[c] * e.x;
+          }
+        } else {
+          const double2 je = make_double2(xj.x * eh.x - xj.y * eh.y, xj.x * eh.y + xj.y * eh.x);
+          const double2 me = make_double2(xm.x * e.x - xm.y * e.y, xm.x * e.y + xm.y * e.x);
+          for (int c = 0; c < 3; c++) {
+            Ht[c][0] -= je.x * fn[c];
+            Ht[c][1] -= je.y * fn[c];
+            Et[c][0] += me.x * fn[c];
+            Et[c][1] += me.y * fn[c];
+          }
         }
       }
     }
@@ -1539,31 +1654,52 @@ __global__ void farfield_sources_kernel(int F, int Q, QuadRule rule, const doubl
   const int gi = (int)(s / FQ);
   const int64_t s0 = s - gi * FQ;
   const int f = (int)(s0 / Q), t = (int)(s0 - (int64_t)f * Q);
-  const double *js = G.js[gi];
+  const double *Jg = G.J[gi];
+  const bool dg = Jg[1] == 0.0 && Jg[2] == 0.0 && Jg[3] == 0.0 && Jg[5] == 0.0 && Jg[6] == 0.0 && Jg[7] == 0.0;
   const double *r = qp + s0 * 3;
   double J[3][2] = {{0, 0}, {0, 0}, {0, 0}}, M[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   for (int rep = 0; rep < G.n; rep++) {
-    const double e = G.ef[gi][rep], em = e * G.detj[gi];
+    const double2 e = G.ef[gi][rep], em = make_double2(e.x * G.detj[gi], e.y * G.detj[gi]);
     const double2 *xr = x + rep * ldx;
     for (int p = 0; p < 3; p++) {
       const int g = side.g[3 * f + p];
       if (g < 0) continue;
       const double cf = side.coef[3 * f + p];
       const double *v = side.v + 3 * (3 * f + p);
-      const double fv[3] = {js[0] * (cf * (r[0] - v[0])), js[1] * (cf * (r[1] - v[1])),
-                            js[2] * (cf * (r[2] - v[2]))};
+      const double f0[3] = {cf * (r[0] - v[0]), cf * (r[1] - v[1]), cf * (r[2] - v[2])};
+      double fv[3];
+      if (dg)
+        for (int c = 0; c < 3; c++) fv[c] = Jg[4 * c] * f0[c];
+      else
+        jac_mul(Jg, f0, fv);
       const double2 xj = xr[g], xm = xr[g + Nt];
-      for (int c = 0; c < 3; c++) {
-        J[c][0] += xj.x * e * fv[c];
-        J[c][1] += xj.y * e * fv[c];
-        M[c][0] += xm.x * em * fv[c];
-        M[c][1] += xm.y * em * fv[c];
+      if (e.y == 0.0) {
+        for (int c = 0; c < 3; c++) {
+          J[c][0] += xj.x * e.x * fv[c];
+          J[c][1] += xj.y * e.x * fv[c];
+          M[c][0] += xm.x * em.x * fv[c];
+          M[c][1] += xm.y * em.x * fv[c];
+        }
+      } else {
+        const double2 je = make_double2(xj.x * e.x - xj.y * e.y, xj.x * e.y + xj.y * e.x);
+        const double2 me = make_double2(xm.x * em.x - xm.y * em.y, xm.x * em.y + xm.y * em.x);
+        for (int c = 0; c < 3; c++) {
+          J[c][0] += je.x * fv[c];
+          J[c][1] += je.y * fv[c];
+          M[c][0] += me.x * fv[c];
+          M[c][1] += me.y * fv[c];
+        }
       }
     }
   }
   const double wa = rule.w[t] * area[f];
+  double pr[3];
+  if (dg)
+    for (int c = 0; c < 3; c++) pr[c] = Jg[4 * c] * r[c];
+  else
+    jac_mul(Jg, r, pr);
   for (int c = 0; c < 3; c++) {
-    pts[3 * s + c] = js[c] * r[c];
+    pts[3 * s + c] = pr[c];
     cur[c * S + s] = make_double2(wa * J[c][0], wa * J[c][1]);
     cur[(3 + c) * S + s] = make_double2(wa * M[c][0], wa * M[c][1]);
   }

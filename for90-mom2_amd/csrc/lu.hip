@@ -3548,7 +3548,7 @@ int mom_system_fill_symm(int32_t nrep, mom_system *const *systems, int32_t ndom,
   if (!systems || !doms || !ri || !bnd) return fail(MOM_ERR_ARG, "NULL argument");
   if (nedges_total < 1 || ndom < 1) return fail(MOM_ERR_ARG, "bad sizes");
   Group G;
-  MOM_TRY(mirror_group(ngen, planes, &G));
+  MOM_TRY(symm_group(ngen, planes, &G));
   if (nrep != G.n) return fail(MOM_ERR_ARG, "one system per representation of the group is needed");
   double2 *Z[kMaxReps];
   int64_t ld[kMaxReps], order[kMaxReps];

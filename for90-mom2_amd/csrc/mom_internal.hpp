@@ -155,7 +155,7 @@ struct RepOut {
   double2 *Z[kMaxReps];
   int64_t ld[kMaxReps];
   const int *map[kMaxReps];
-  double e[kMaxReps];  // field action of the element in each representation
+  double2 e[kMaxReps];  // field action of the element in each representation (complex for rotations)
 };
 
 // Output of the fill kernels.
@@ -172,37 +172,61 @@ struct FillTarget {
   RepOut rep;
 };
 
-// A group of mirror symmetries (symmetry.f90:26-241 restricted to the
-// planes x = 0, y = 0, z = 0 and their products): per element its diagonal
-// Jacobian, determinant, identifier and generator bits, and the field
-// action ef[element][representation] (the conjugated representation).
+// A symmetry group (symmetry.f90:26-241): products of the mirrors of the
+// planes x = 0, y = 0, z = 0 and of rotations about the z axis.  Per
+// element its Jacobian J (row-major, J[3 i + j] = J_ij), determinant,
+// identifier and generator bits, and the field action
+// ef[element][representation] (the conjugated representation; complex for
+// rotations).
 struct Group {
   int n = 1;
-  double js[kMaxReps][3];
+  double J[kMaxReps][9];
   double detj[kMaxReps];
-  double ef[kMaxReps][kMaxReps];
+  double2 ef[kMaxReps][kMaxReps];
   int id[kMaxReps];
   int genbits[kMaxReps];
 };
-constexpr int kGidIdentity = 1, kGidMxp = 2, kGidMyp = 3, kGidMzp = 4;  // symmetry.f90:20-24
+constexpr int kGidIdentity = 1, kGidMxp = 2, kGidMyp = 3, kGidMzp = 4, kGidRz = 5;  // symmetry.f90:20-24
 constexpr int kBndNone = 1, kBndXplane = 2, kBndYplane = 3, kBndZplane = 4;  // mesh.f90:16-25
+// generator codes of the `symm` line: 1, 2, 3 = mxp, myp, mzp; kGenRz + n = rn
+constexpr int kGenRz = 100;
 Group identity_group();
-// read_symm's product of group_mp(plane) for planes[i] in 1..3 (x, y, z)
-int mirror_group(int ngen, const int32_t *planes, Group *g);
+// read_symm's product of the generators gens[i] (codes above)
+int symm_group(int ngen, const int32_t *gens, Group *g);
+// whether J is diagonal (a product of mirrors) / its own inverse
+bool jac_diagonal(const double *J);
+bool jac_involution(const double *J);
 
-// A mirror image J = diag(js) of the source faces (symmetry.f90:61-104):
-// the fill integrates the test faces reflected by J against the source
-// faces, which equals the reference's integral over the reflected source
-// (srcint.f90:155-224, 368-448) since J is its own inverse.
+// An image J of the source faces (symmetry.f90:61-156): the fill integrates
+// the test faces moved by M = J^T against the source faces, which equals
+// the reference's integral over the source moved by J (srcint.f90:155-224,
+// 368-448), J being orthogonal.  For a mirror (diagonal J, its own inverse)
+// each unordered face pair is integrated once as without an image; a
+// rotation's transposed blocks belong to its inverse element, so its
+// ordered pairs are integrated one by one.
 struct Image {
-  double js[3];
+  double M[9];  // J^T, row-major
   double detj;
+  bool diag;    // J (and M) diagonal: the mirror code path
+  bool invol;   // J = J^-1: the transposed blocks are this element's own
 };
 
 // api.hip: near face pairs (test, source) sorted, of the mesh with itself or,
-// with js, of the test faces reflected by diag(js) against the source faces
+// with J (row-major), of the test faces against the source faces moved by J:
+// |c_m - J c_n|^2 < (3 avelen)^2 in the reference's order of operations
 void near_pairs(const std::vector<double> &cp, int64_t F, double avelen, std::vector<int32_t> &out,
-                const double *js = nullptr);
+                const double *J = nullptr);
+// c - J c' for the near rule of an image (no contraction), shared by the host
+// list and the device kernels so that both classify every pair alike
+__host__ __device__ inline void image_near_diff(const double *J, const double *c, const double *cn, double *d) {
+#pragma clang fp contract(off)
+  for (int i = 0; i < 3; i++) {
+    double t = J[3 * i] * cn[0];
+    t = t + J[3 * i + 1] * cn[1];
+    t = t + J[3 * i + 2] * cn[2];
+    d[i] = c[i] - t;
+  }
+}
 // off[m] .. off[m+1]: the pairs of test face m in a list sorted by test face (F + 1 entries)
 void near_offsets(const std::vector<int32_t> &pairs, int64_t F, std::vector<int64_t> &off);
 

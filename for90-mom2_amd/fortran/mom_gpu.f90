@@ -385,6 +385,16 @@ MODULE mom_gpu
        INTEGER(C_INT) :: st
      END FUNCTION mom_symm_group
 
+     FUNCTION mom_symm_group_c(ngen, gens, nga, ef, detj, jmat, id) BIND(C, NAME='mom_symm_group_c') RESULT(st)
+       IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+       INTEGER(C_INT32_T), VALUE :: ngen
+       INTEGER(C_INT32_T), DIMENSION(*), INTENT(IN) :: gens
+       INTEGER(C_INT32_T) :: nga
+       REAL(C_DOUBLE), DIMENSION(*) :: ef, detj, jmat
+       INTEGER(C_INT32_T), DIMENSION(*) :: id
+       INTEGER(C_INT) :: st
+     END FUNCTION mom_symm_group_c
+
      FUNCTION mom_symm_edge_bc(nedges_total, bnd, ngen, planes, rep, id, dim) &
           BIND(C, NAME='mom_symm_edge_bc') RESULT(st)
        IMPORT :: C_INT, C_INT32_T, C_INT64_T

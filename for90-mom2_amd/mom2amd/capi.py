@@ -32,7 +32,7 @@ EXPORTS = [
     "mom_pmchwt_vertex_adjoint", "mom_cs_abs_vertex_explicit", "mom_ctx_fill_tuning", "mom_far_fields_noalloc",
     "mom_ctx_fill_profile", "mom_mesh_classify_like", "mom_far_fields_adjoint", "mom_nlsurf_coef_adjoint",
     "mom_domain_block_apply", "mom_system_set_pivoting", "mom_system_set_precision",
-    "mom_system_set_cooperative",
+    "mom_system_set_cooperative", "mom_symm_group_c",
 ]
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
@@ -107,6 +107,7 @@ def lib():
     L.mom_system_set_pivoting.argtypes = [_vp, _i32]
     L.mom_system_set_precision.argtypes = [_vp, _i32]
     L.mom_system_set_cooperative.argtypes = [_vp, _i32]
+    L.mom_symm_group_c.argtypes = [_i32, _ip, C.POINTER(_i32), _dp, _dp, _dp, _ip]
     L.mom_domain_block_apply.argtypes = [_vp, _vp, _i64, _d, _d, _d, _i32, _i32, _dp, _dp, _dp, _dp]
     L.mom_test_unit_lower_inverse.argtypes = [_vp, _vp, _i64, _vp]
     L.mom_cs_prtsrf.argtypes = [_vp, _vp, _i64, _d, _d, _d, _dp, _d, _d, _d, _i32, _dp, _dp, C.POINTER(_d),

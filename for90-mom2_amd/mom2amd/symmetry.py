@@ -1,5 +1,6 @@
-"""Mirror-symmetry decomposition: the `symm` command with the planes x = 0,
-y = 0, z = 0, solved one representation at a time on the device.
+"""Symmetry decomposition: the `symm` command with the mirror planes x = 0,
+y = 0, z = 0, the rotations about the z axis (rn) and their products, solved
+one representation at a time on the device.
 
 The reference builds the group (read_symm, interface.f90:160-210), fills one
 matrix per representation (sysmat_pmchwt, sysmat.f90:18-105), projects the
@@ -22,7 +23,10 @@ and far fields sum the images of every representation's currents
   solutions.
 
 An odd reduced order is padded by one decoupled unknown (unit diagonal,
-zero source), which keeps every device matrix of even order.
+zero source), which keeps every device matrix of even order.  Rotation
+groups (group_rz, symmetry.f90:105-156) have complex field actions and are
+for bodies that no rotation cut crosses, such as n copies of a particle
+around the z axis: the reference sets no edge coupling for rz1/rz2 cuts.
 """
 from __future__ import annotations
 
@@ -35,23 +39,37 @@ from .gpu import System, _domain_handles, _quad, _ri_array
 
 __all__ = ["PLANES", "group", "edge_bc", "SymmetricSystems", "srcvec_symm", "cs_prtsrf_symm", "rcs_symm"]
 
-# generators of the `symm` line (interface.f90:173-185)
+# generators of the `symm` line (interface.f90:173-188): the mirror planes,
+# and "rN" for group_rz(N) (code 100 + N)
 PLANES = {"mxp": 1, "myp": 2, "mzp": 3}
 
 
+def _code(g):
+    if isinstance(g, str):
+        if g in PLANES:
+            return PLANES[g]
+        if g[:1] == "r" and g[1:].isdigit():
+            return 100 + int(g[1:])
+        raise ValueError("unknown symmetry generator %r" % g)
+    return int(g)
+
+
 def _planes(gens):
-    return np.ascontiguousarray([PLANES[g] if isinstance(g, str) else int(g) for g in gens], dtype=np.int32)
+    return np.ascontiguousarray([_code(g) for g in gens], dtype=np.int32)
 
 
 def group(gens):
-    """The group of the mirror generators gens (e.g. ["mxp", "myp"]):
-    dict(n, ef[n,n] (element, representation), detj[n], jdiag[n,3], id[n])."""
+    """The group of the generators gens (e.g. ["mxp", "myp"] or ["r3",
+    "mzp"]): dict(n, ef[n,n] complex (element, representation), detj[n],
+    J[n,3,3], jdiag[n,3] (the diagonal of J), id[n])."""
     p = _planes(gens)
     n = C.c_int32()
-    ef, detj, jd, gid = np.zeros(64), np.zeros(8), np.zeros(24), np.zeros(8, np.int32)
-    capi.check(capi.lib().mom_symm_group(len(p), p, C.byref(n), ef, detj, jd, gid))
+    ef, detj, jm, gid = np.zeros(128), np.zeros(8), np.zeros(72), np.zeros(8, np.int32)
+    capi.check(capi.lib().mom_symm_group_c(len(p), p, C.byref(n), ef, detj, jm, gid))
     n = n.value
-    return dict(n=n, ef=ef[:n * n].reshape(n, n), detj=detj[:n], jdiag=jd[:3 * n].reshape(n, 3), id=gid[:n])
+    J = jm[:9 * n].reshape(n, 3, 3)
+    return dict(n=n, ef=ef[:2 * n * n].view(np.complex128).reshape(n, n), detj=detj[:n], J=J,
+                jdiag=np.array([np.diag(j) for j in J]), id=gid[:n])
 
 
 def edge_bc(bnd, gens, rep):
@@ -68,7 +86,7 @@ def srcvec_symm(ctx, dmesh, nedges_total, omega, ri, gens, angles, quad=None):
     """q[2N, nga, nsrc]: the plane waves projected on each representation."""
     Q, w, n = _quad(quad)
     p = _planes(gens)
-    nga = 1 << len(p)
+    nga = group(gens)["n"]
     ang = np.ascontiguousarray(np.atleast_2d(np.asarray(angles, dtype=np.float64))).ravel()
     nsrc = ang.size // 3
     N2 = 2 * nedges_total
@@ -104,8 +122,8 @@ def rcs_symm(ctx, dmesh, nedges_total, omega, ri, gens, x, ntheta, nphi, quad=No
 
 
 class SymmetricSystems:
-    """One device system per representation of the mirror group `gens` for
-    the problem whose master edges have boundary classes `bnd`."""
+    """One device system per representation of the group `gens` for the
+    problem whose master edges have boundary classes `bnd`."""
 
     def __init__(self, ctx, nedges_total, bnd, gens, precision=capi.MOM_PREC_MIXED):
         self.ctx = ctx

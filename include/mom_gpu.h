@@ -447,12 +447,17 @@ int mom_problem_topology(const mom_problem *prob, int32_t dom, int32_t *enodes, 
 int mom_problem_mesh_create(mom_ctx *ctx, const mom_problem *prob, int32_t dom, mom_mesh **mesh);
 
 /* ---------------------------------------------------------------------------
- * Mirror symmetry (the `symm` and `sbnd` commands with the planes x = 0,
- * y = 0, z = 0).  The mesh holds one symmetry cell; its edges on a mirror
- * plane are boundary edges (one face), classified by the line elements that
- * the mesh file puts on them.  planes[ngen] lists the generators in the
- * order of the `symm` line: 1 = mxp, 2 = myp, 3 = mzp.  Rotation groups and
- * periodic boundaries are not supported (MOM_ERR_ARG).
+ * Symmetry (the `symm` and `sbnd` commands): the mirrors of the planes
+ * x = 0, y = 0, z = 0, the rotations about the z axis (rn) and their
+ * products, groups of up to 8 elements.  The mesh holds one symmetry cell;
+ * its edges on a mirror plane are boundary edges (one face), classified by
+ * the line elements that the mesh file puts on them.  planes[ngen] (the
+ * generators, up to 4) lists them in the order of the `symm` line: 1 = mxp,
+ * 2 = myp, 3 = mzp, 100 + n = rn (2 <= n <= 8, group_rz, symmetry.f90:
+ * 105-156).  A rotation group is for bodies that no rotation cut crosses (n
+ * copies of a particle around the z axis): the boundary classes of rotation
+ * cuts (rz1, rz2) and of periodic cells are not supported (MOM_ERR_ARG), as
+ * the reference sets no edge coupling for them (mesh.f90:195-258).
  */
 /* Line elements (0-based master node pairs, ids) for classification, for a
  * problem built from arrays; mom_problem_load_gmsh keeps the file's lines. */
@@ -465,13 +470,22 @@ int mom_problem_set_lines(mom_problem *prob, int64_t nlines, const int32_t *node
 int mom_problem_classify_edges(mom_problem *prob, int32_t id, int32_t bnd, int64_t *nclassified);
 /* Boundary class of every edge of domain dom (-1: master), 1 = none. */
 int mom_problem_edge_bnd(const mom_problem *prob, int32_t dom, int32_t *bnd);
-/* read_symm (interface.f90:160-210): the group of the mirror generators,
- * built by group_mp and product_group (symmetry.f90:61-104, 173-241).
- * nga = 2^ngen elements; ef[g*nga + r] field action of element g in
- * representation r; detj[g]; jdiag[3g..3g+2] the diagonal Jacobian; id[g]
- * (1 identity, 2/3/4 the pure mirrors, 0 products).  Outputs may be NULL. */
+/* read_symm (interface.f90:160-210): the group of the generators, built by
+ * group_mp, group_rz and product_group (symmetry.f90:61-156, 173-241), for
+ * a group whose Jacobians are all diagonal (the mirror planes): nga
+ * elements; ef[g*nga + r] field action of element g in representation r;
+ * detj[g]; jdiag[3g..3g+2] the diagonal Jacobian; id[g] (1 identity, 2/3/4
+ * the pure mirrors, 0 products).  Outputs may be NULL. */
 int mom_symm_group(int32_t ngen, const int32_t *planes, int32_t *nga, double *ef, double *detj, double *jdiag,
                    int32_t *id);
+/* The same for any group of the generators above (MOM_ERR_ARG from
+ * mom_symm_group for a group with rotations): ef[2 (g*nga + r) + {0, 1}]
+ * the complex field action (conj(exp(i 2 pi k l / n)) for group_rz),
+ * jmat[9g..9g+8] the Jacobian row-major (J[3i + j] = J_ij), id[g] (5 the
+ * last rotation of group_rz, as the reference marks it).  Outputs may be
+ * NULL. */
+int mom_symm_group_c(int32_t ngen, const int32_t *gens, int32_t *nga, double *ef, double *detj, double *jmat,
+                     int32_t *id);
 /* edge_bc (bc.f90:28-118) of representation rep (0-based) for the master
  * edges' boundary classes bnd[N]: id[2N] = -1 for the unknowns the
  * representation removes, 0 otherwise; dim = the kept count, the order of

@@ -1,4 +1,5 @@
-! ref_symm.f90 -- golden-vector driver for a mirror-symmetric problem,
+! ref_symm.f90 -- golden-vector driver for a symmetric problem (mirror planes
+! and rotations about z),
 ! linked against the reference's own modules (built by oracle/build_ref.sh).
 ! TEST INFRASTRUCTURE ONLY.
 !
@@ -15,7 +16,7 @@
 !   post.bin      csca, cabs (cs_prtsrf) and rcs(ntheta,nphi)
 !
 ! Input on stdin, one item per line:
-!   mesh file / scale / "ngen name1 name2 .." (mxp, myp, mzp) /
+!   mesh file / scale / "ngen name1 name2 .." (mxp, myp, mzp, rN) /
 !   nbnd / per boundary: "id name" (xplane, yplane, zplane) / ndom /
 !   per domain: "nsurf id1 id2 ..." / wavelength [m] / per domain: "re im"
 !   refractive index / "theta phi psi" [deg] / "ntheta nphi" / output dir
@@ -33,7 +34,7 @@ PROGRAM ref_symm
   CHARACTER(LEN=256) :: mshname
   CHARACTER(LEN=8) :: names(3), bname
   REAL(KIND=dp) :: scale, wl, omega, th, ph, ps, csca, cabs, re, im
-  INTEGER :: ndom, d, nsurf, i, ids(16), nth, nph, N, u, ngen, nbnd, bid, bcls, nga
+  INTEGER :: ndom, d, nsurf, i, ids(16), nth, nph, N, u, ngen, nbnd, bid, bcls, nga, nr
   TYPE(srcdata) :: src
   TYPE(group_action), DIMENSION(:), ALLOCATABLE :: ga
   COMPLEX(KIND=dp), DIMENSION(:,:,:), ALLOCATABLE :: A, x
@@ -66,6 +67,10 @@ PROGRAM ref_symm
      IF(TRIM(names(i))=='mxp') CALL group_mp(1, ga)
      IF(TRIM(names(i))=='myp') CALL group_mp(2, ga)
      IF(TRIM(names(i))=='mzp') CALL group_mp(3, ga)
+     IF(names(i)(1:1)=='r') THEN
+        READ(names(i)(2:3),*) nr
+        CALL group_rz(nr, ga)
+     END IF
      CALL product_group(b%ga, ga)
      DO d=1,SIZE(ga)
         DEALLOCATE(ga(d)%ef)

@@ -146,8 +146,8 @@ REF_SYMM_BIN = os.path.join(ROOT, "oracle", "_ref", "mom_ref_symm")
 
 
 def run_symm(mshfile, scale, gens, sbnd, domains, wl, ri, angles_deg, ntheta=5, nphi=7, outdir=None, threads=None):
-    """Run the reference's mirror-symmetric set-up and linear solve
-    (oracle/_ref/mom_ref_symm): gens e.g. ["mxp", "myp", "mzp"], sbnd a list
+    """Run the reference's symmetric set-up and linear solve
+    (oracle/_ref/mom_ref_symm): gens e.g. ["mxp", "myp", "mzp"] or ["r3"], sbnd a list
     of (line id, "xplane"|"yplane"|"zplane").  Returns the meshes with their
     edge classes, A[2N,2N,nga] before reduction, q[2N,nga], the expanded
     solutions x[2N,nga], csca, cabs and rcs."""
@@ -165,7 +165,9 @@ def run_symm(mshfile, scale, gens, sbnd, domains, wl, ri, angles_deg, ntheta=5, 
     for d in range(len(domains) + 1):
         res["mesh"][d]["bnd"] = np.fromfile(os.path.join(outdir, "bnd_%d.bin" % d), np.int32)
     N2 = 2 * res["mesh"][0]["enodes"].shape[0]
-    nga = 1 << len(gens)
+    nga = 1
+    for g in gens:
+        nga *= int(g[1:]) if g.startswith("r") else 2
     res["A"] = np.fromfile(os.path.join(outdir, "A.bin"), np.complex128).reshape(N2, N2, nga, order="F")
     res["q"] = np.fromfile(os.path.join(outdir, "q.bin"), np.complex128).reshape(N2, nga, order="F")
     res["x"] = np.fromfile(os.path.join(outdir, "x.bin"), np.complex128).reshape(N2, nga, order="F")

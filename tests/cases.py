@@ -42,13 +42,14 @@ def blockwise_relerr(A, B, nblk=2):
     return err
 
 
-def sphere_cell(msh_dir, n, reflect=(), radius=50.0):
-    """Mirror-symmetry cell of the octasphere (meshgen.sphere_cell) with its
-    boundary edges classified (`sbnd`), at the sphere case's wavelength and
-    permittivity."""
+def sphere_cell(msh_dir, n, reflect=(), radius=50.0, shift=0.0):
+    """Symmetry cell of the octasphere (meshgen.sphere_cell, moved by `shift`
+    along x) with its boundary edges classified (`sbnd`), at the sphere
+    case's wavelength and permittivity."""
     nodes, faces, lines, lids = meshgen.sphere_cell(n, radius, reflect)
+    nodes = nodes + np.array([shift, 0.0, 0.0])
     tag = "".join(str(a) for a in reflect)
-    path = os.path.join(str(msh_dir), "cell%d_%s_r%g.msh" % (n, tag, radius))
+    path = os.path.join(str(msh_dir), "cell%d_%s_r%g_x%g.msh" % (n, tag, radius, shift))
     if not os.path.exists(path):
         meshgen.write_gmsh(path, nodes, faces, 15, lines, lids)
     sbnd = [(i, c) for c, i in meshgen.OCTANT_LINE_IDS.items() if i in set(int(v) for v in lids)]

@@ -27,7 +27,8 @@ Fixtures (all complex128 / float64 at full precision):
                  all solved together by the reference's own solve_systems:
                  per wavelength q, x, csca, cabs, the rcs 37x72 of source 1,
                  sampled A entries and block norms
-Usage: python tests/golden/make_golden.py [symm|spectrum|ops]   (only those fixtures)
+  trimer_r3.npz, ring_r4_mzp.npz  rotation groups (main_rot)
+Usage: python tests/golden/make_golden.py [symm|rot|spectrum|ops]   (only those fixtures)
 """
 import os
 import sys
@@ -93,17 +94,20 @@ def case(name, mshfile, domains, wl, ri, angles, full_A, nsample=3000, seed=1, s
     print(name, "N=%d" % N, "csca=%.6e cabs=%.6e" % (P["csca"], P["cabs"]))
 
 
-def symm_case(name, n, reflect, gens, angles, with_A):
-    """Mirror-symmetric sphere cell (meshgen.sphere_cell) solved by the
-    reference per representation (oracle/_ref/mom_ref_symm)."""
+def symm_case(name, n, reflect, gens, angles, with_A, radius=50.0, shift=0.0):
+    """Symmetry cell (meshgen.sphere_cell of radius `radius`, moved by
+    `shift` along x) solved by the reference per representation
+    (oracle/_ref/mom_ref_symm)."""
     tmp = os.path.join(HERE, "_msh")
-    nodes, faces, lines, lids = meshgen.sphere_cell(n, 50.0, reflect)
+    nodes, faces, lines, lids = meshgen.sphere_cell(n, radius, reflect)
+    nodes = nodes + np.array([shift, 0.0, 0.0])
     fn = os.path.join(tmp, name + ".msh")
     meshgen.write_gmsh(fn, nodes, faces, 15, lines, lids)
     sbnd = [(i, c) for c, i in meshgen.OCTANT_LINE_IDS.items() if i in set(int(v) for v in lids)]
     ri = [1.0 + 0j, np.sqrt(-9.4 + 1.5j)]
     R = refrun.run_symm(fn, 1e-9, gens, sbnd, [[15], [-15]], 600e-9, ri, angles, ntheta=37, nphi=72)
     data = dict(n=np.array(n), reflect=np.array(list(reflect) + [-1] * (3 - len(reflect))),
+                radius=np.array(radius), shift=np.array(shift),
                 gens=np.array(gens), wl=np.array(600e-9), ri=np.array(ri, dtype=np.complex128),
                 angles_deg=np.array(angles, float), q=R["q"], x=R["x"], csca=np.array(R["csca"]),
                 cabs=np.array(R["cabs"]), rcs=R["rcs"])
@@ -131,6 +135,23 @@ def main_symm():
     os.makedirs(os.path.join(HERE, "_msh"), exist_ok=True)
     symm_case("octant_n4", 4, (), ["mxp", "myp", "mzp"], (40.0, 25.0, 30.0), True)
     symm_case("quarter_n4", 4, (2,), ["myp", "mxp"], (40.0, 25.0, 30.0), False)
+    main_rot()
+
+
+def main_rot():
+    """Fixtures of the rotation groups (group_rz; bodies that no rotation cut
+    crosses):
+      trimer_r3.npz  the n=2 octasphere of radius 20 nm centred at (35, 0, 0)
+                     nm, group r3 (three copies around z, 3 complex
+                     representations), incidence (40, 25, 30) deg: A before
+                     reduction, q, the expanded solutions x, csca, cabs, rcs
+                     37x72, mesh arrays and edge classes
+      ring_r4_mzp.npz the upper half (z >= 0) of the same sphere, group r4 mzp
+                     (four spheres around z from eight half cells, 8
+                     representations): the same outputs"""
+    os.makedirs(os.path.join(HERE, "_msh"), exist_ok=True)
+    symm_case("trimer_r3", 2, (0, 1, 2), ["r3"], (40.0, 25.0, 30.0), True, radius=20.0, shift=35.0)
+    symm_case("ring_r4_mzp", 2, (0, 1), ["r4", "mzp"], (40.0, 25.0, 30.0), True, radius=20.0, shift=35.0)
 
 
 def main_ops():
@@ -214,6 +235,8 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["symm"]:
         main_symm()
+    elif sys.argv[1:] == ["rot"]:
+        main_rot()
     elif sys.argv[1:] == ["spectrum"]:
         main_spectrum()
     elif sys.argv[1:] == ["ops"]:
