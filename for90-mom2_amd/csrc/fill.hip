@@ -658,23 +658,73 @@ __global__ __launch_bounds__(256) void assemble_transposed_kernel(AssembleArgs g
 // ---------------------------------------------------------------------------
 // Analytic singular integrals (int.f90, Hanninen 2006), FP64, evaluated with
 // FMA contraction off so that exact-zero branch tests and rounding follow the
-// reference's arithmetic.
-struct NearFace {
-  double p[3][3];  // nodes
-  double n[3];     // normal
-  double m[3][3];  // in-plane edge normals m(:,e)
-  double s[3][3];  // edge tangents s(:,e)
+// reference's arithmetic.  They are templates on the scalar type: double for
+// the fill, D1 (a value with its derivative in one coordinate, forward mode)
+// for the exact derivatives of the node-coordinate adjoint.  Branches test
+// the value (dval), so both instantiations take the same branch.
+struct D1 {
+  double v, d;
+  D1() = default;
+  __device__ D1(double x) : v(x), d(0.0) {}
+  __device__ D1(double x, double y) : v(x), d(y) {}
 };
+__device__ inline D1 operator+(D1 a, D1 b) { return D1(a.v + b.v, a.d + b.d); }
+__device__ inline D1 operator-(D1 a, D1 b) { return D1(a.v - b.v, a.d - b.d); }
+__device__ inline D1 operator-(D1 a) { return D1(-a.v, -a.d); }
+__device__ inline D1 operator*(D1 a, D1 b) { return D1(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ inline D1 operator*(double a, D1 b) { return D1(a * b.v, a * b.d); }
+__device__ inline D1 operator*(D1 a, double b) { return D1(a.v * b, a.d * b); }
+__device__ inline D1 operator/(D1 a, D1 b) {
+  const double q = a.v / b.v;
+  return D1(q, (a.d - q * b.d) / b.v);
+}
+__device__ inline D1 operator/(D1 a, double b) { return D1(a.v / b, a.d / b); }
+__device__ inline D1 operator/(double a, D1 b) {
+  const double q = a / b.v;
+  return D1(q, -q * b.d / b.v);
+}
+__device__ inline D1 operator+(D1 a, double b) { return D1(a.v + b, a.d); }
+__device__ inline D1 operator+(double a, D1 b) { return D1(a + b.v, b.d); }
+__device__ inline D1 operator-(D1 a, double b) { return D1(a.v - b, a.d); }
+__device__ inline D1 operator-(double a, D1 b) { return D1(a - b.v, -b.d); }
+__device__ inline D1 &operator+=(D1 &a, D1 b) { return a = a + b; }
+__device__ inline double dval(double x) { return x; }
+__device__ inline double dval(D1 x) { return x.v; }
+__device__ inline double tsqrt(double x) { return sqrt(x); }
+__device__ inline D1 tsqrt(D1 x) {
+  const double r = sqrt(x.v);
+  return D1(r, 0.5 * x.d / r);
+}
+__device__ inline double tlog(double x) { return log(x); }
+__device__ inline D1 tlog(D1 x) { return D1(log(x.v), x.d / x.v); }
+__device__ inline double tfabs(double x) { return fabs(x); }
+__device__ inline D1 tfabs(D1 x) { return x.v < 0.0 ? -x : x; }
+__device__ inline double tatan2(double y, double x) { return atan2(y, x); }
+__device__ inline D1 tatan2(D1 y, D1 x) {
+  return D1(atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / (x.v * x.v + y.v * y.v));
+}
 
-__device__ inline double dot3(const double *a, const double *b) {
+template <class T>
+struct NearFaceT {
+  T p[3][3];  // nodes
+  T n[3];     // normal
+  T m[3][3];  // in-plane edge normals m(:,e)
+  T s[3][3];  // edge tangents s(:,e)
+};
+using NearFace = NearFaceT<double>;
+
+template <class T>
+__device__ inline T dot3(const T *a, const T *b) {
 #pragma clang fp contract(off)
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
-__device__ inline double norm3(const double *a) {
+template <class T>
+__device__ inline T norm3(const T *a) {
 #pragma clang fp contract(off)
-  return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  return tsqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
 }
-__device__ inline void cross3(const double *a, const double *b, double *r) {
+template <class T>
+__device__ inline void cross3(const T *a, const T *b, T *r) {
 #pragma clang fp contract(off)
   r[0] = a[1] * b[2] - a[2] * b[1];
   r[1] = a[2] * b[0] - a[0] * b[2];
@@ -682,38 +732,39 @@ __device__ inline void cross3(const double *a, const double *b, double *r) {
 }
 
 // int.f90:15-53 intLm1 for side e (nodes e, e+1)
-__device__ double d_intLm1(const double *r, const NearFace &f, int e) {
+template <class T>
+__device__ T d_intLm1(const T *r, const NearFaceT<T> &f, int e) {
 #pragma clang fp contract(off)
-  const double *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
-  double d2[3] = {r[0] - p2[0], r[1] - p2[1], r[2] - p2[2]};
-  double d1[3] = {r[0] - p1[0], r[1] - p1[1], r[2] - p1[2]};
-  double Rp = norm3(d2), Rm = norm3(d1);
-  double q2[3] = {p2[0] - r[0], p2[1] - r[1], p2[2] - r[2]};
-  double q1[3] = {p1[0] - r[0], p1[1] - r[1], p1[2] - r[2]};
-  double sp = dot3(q2, s), sm = dot3(q1, s);
-  double div1 = Rp - sp, div2 = Rm + sm;
-  if (fabs(div1) > fabs(div2)) return log((Rm - sm) / div1);
-  return log((Rp + sp) / div2);
+  const T *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
+  T d2[3] = {r[0] - p2[0], r[1] - p2[1], r[2] - p2[2]};
+  T d1[3] = {r[0] - p1[0], r[1] - p1[1], r[2] - p1[2]};
+  T Rp = norm3(d2), Rm = norm3(d1);
+  T q2[3] = {p2[0] - r[0], p2[1] - r[1], p2[2] - r[2]};
+  T q1[3] = {p1[0] - r[0], p1[1] - r[1], p1[2] - r[2]};
+  T sp = dot3(q2, s), sm = dot3(q1, s);
+  T div1 = Rp - sp, div2 = Rm + sm;
+  if (fabs(dval(div1)) > fabs(dval(div2))) return tlog((Rm - sm) / div1);
+  return tlog((Rp + sp) / div2);
 }
 
 // int.f90:55-135 intL1 and intL3 of side e (they share R0^2, s+-, R+-)
-__device__ void d_intL13(const double *r, const NearFace &f, int e, double lm1, double *L1,
-                         double *L3) {
+template <class T>
+__device__ void d_intL13(const T *r, const NearFaceT<T> &f, int e, T lm1, T *L1, T *L3) {
 #pragma clang fp contract(off)
-  const double *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
-  double p[3], rp[3];
+  const T *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
+  T p[3], rp[3];
   for (int k = 0; k < 3; k++) p[k] = 0.5 * (p1[k] + p2[k]);
   for (int k = 0; k < 3; k++) rp[k] = r[k] - p[k];
-  double h = dot3(f.n, rp), t = dot3(f.m[e], rp);
-  double R02 = t * t + h * h;
-  double d2[3] = {r[0] - p2[0], r[1] - p2[1], r[2] - p2[2]};
-  double d1[3] = {r[0] - p1[0], r[1] - p1[1], r[2] - p1[2]};
-  double Rp = norm3(d2), Rm = norm3(d1);
-  double q2[3] = {p2[0] - r[0], p2[1] - r[1], p2[2] - r[2]};
-  double q1[3] = {p1[0] - r[0], p1[1] - r[1], p1[2] - r[2]};
-  double sp = dot3(q2, s), sm = dot3(q1, s);
-  double l1;
-  if (R02 == 0)
+  T h = dot3(f.n, rp), t = dot3(f.m[e], rp);
+  T R02 = t * t + h * h;
+  T d2[3] = {r[0] - p2[0], r[1] - p2[1], r[2] - p2[2]};
+  T d1[3] = {r[0] - p1[0], r[1] - p1[1], r[2] - p1[2]};
+  T Rp = norm3(d2), Rm = norm3(d1);
+  T q2[3] = {p2[0] - r[0], p2[1] - r[1], p2[2] - r[2]};
+  T q1[3] = {p1[0] - r[0], p1[1] - r[1], p1[2] - r[2]};
+  T sp = dot3(q2, s), sm = dot3(q1, s);
+  T l1;
+  if (dval(R02) == 0)
     l1 = 0.5 * (sp * Rp - sm * Rm);
   else
     l1 = 0.5 * R02 * lm1 + 0.5 * (sp * Rp - sm * Rm);
@@ -722,38 +773,42 @@ __device__ void d_intL13(const double *r, const NearFace &f, int e, double lm1, 
 }
 
 // int.f90:137-162 intSm3h (signed solid angle)
-__device__ double d_intSm3h(const double *r, const NearFace &f) {
+template <class T>
+__device__ T d_intSm3h(const T *r, const NearFaceT<T> &f) {
 #pragma clang fp contract(off)
-  double a[3][3];
+  T a[3][3];
   for (int v = 0; v < 3; v++) {
-    double d[3] = {f.p[v][0] - r[0], f.p[v][1] - r[1], f.p[v][2] - r[2]};
-    double l = norm3(d);
+    T d[3] = {f.p[v][0] - r[0], f.p[v][1] - r[1], f.p[v][2] - r[2]};
+    T l = norm3(d);
     for (int k = 0; k < 3; k++) a[v][k] = d[k] / l;
   }
-  double x = 1.0 + dot3(a[0], a[1]) + dot3(a[0], a[2]) + dot3(a[1], a[2]);
-  double c[3];
+  T x = 1.0 + dot3(a[0], a[1]) + dot3(a[0], a[2]) + dot3(a[1], a[2]);
+  T c[3];
   cross3(a[1], a[2], c);
-  double y = fabs(dot3(a[0], c));
-  double om = 2.0 * atan2(y, x);
-  double rp1[3] = {r[0] - f.p[0][0], r[1] - f.p[0][1], r[2] - f.p[0][2]};
-  return dot3(f.n, rp1) > 0.0 ? fabs(om) : -fabs(om);
+  T y = tfabs(dot3(a[0], c));
+  T om = 2.0 * tatan2(y, x);
+  T rp1[3] = {r[0] - f.p[0][0], r[1] - f.p[0][1], r[2] - f.p[0][2]};
+  return dval(dot3(f.n, rp1)) > 0.0 ? tfabs(om) : -tfabs(om);
 }
 
-struct PointInts {
-  double h;          // n.(r - p1)
-  double lm1[3], l1[3], l3[3];
-  double sm3h, sm1, s1;
+template <class T>
+struct PointIntsT {
+  T h;          // n.(r - p1)
+  T lm1[3], l1[3], l3[3];
+  T sm3h, sm1, s1;
 };
+using PointInts = PointIntsT<double>;
 
-__device__ void d_point_ints(const double *r, const NearFace &f, PointInts &o) {
+template <class T>
+__device__ void d_point_ints(const T *r, const NearFaceT<T> &f, PointIntsT<T> &o) {
 #pragma clang fp contract(off)
-  double t[3];
+  T t[3];
   for (int e = 0; e < 3; e++) {
-    double d[3] = {r[0] - f.p[e][0], r[1] - f.p[e][1], r[2] - f.p[e][2]};
+    T d[3] = {r[0] - f.p[e][0], r[1] - f.p[e][1], r[2] - f.p[e][2]};
     t[e] = dot3(f.m[e], d);
   }
-  double d0[3] = {r[0] - f.p[0][0], r[1] - f.p[0][1], r[2] - f.p[0][2]};
-  const double h = dot3(f.n, d0);
+  T d0[3] = {r[0] - f.p[0][0], r[1] - f.p[0][1], r[2] - f.p[0][2]};
+  const T h = dot3(f.n, d0);
   o.h = h;
   for (int e = 0; e < 3; e++) {
     o.lm1[e] = d_intLm1(r, f, e);
@@ -761,10 +816,10 @@ __device__ void d_point_ints(const double *r, const NearFace &f, PointInts &o) {
   }
   o.sm3h = d_intSm3h(r, f);
   // int.f90:181-226 intSm1
-  if (h == 0) {
-    double e1 = t[0] == 0 ? 0.0 : o.lm1[0];
-    double e2 = t[1] == 0 ? 0.0 : o.lm1[1];
-    double e3 = t[2] == 0 ? 0.0 : o.lm1[2];
+  if (dval(h) == 0) {
+    T e1 = dval(t[0]) == 0 ? T(0.0) : o.lm1[0];
+    T e2 = dval(t[1]) == 0 ? T(0.0) : o.lm1[1];
+    T e3 = dval(t[2]) == 0 ? T(0.0) : o.lm1[2];
     o.sm1 = -(t[0] * e1 + t[1] * e2 + t[2] * e3);
   } else {
     o.sm1 = -h * o.sm3h - (t[0] * o.lm1[0] + t[1] * o.lm1[1] + t[2] * o.lm1[2]);
@@ -2113,27 +2168,33 @@ int far_fields_no_alloc(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
 // - lambda_J x_M at the pair's edges), rebuilt from the node positions with
 // the fill's formulas (point pairs with G or the smoothed kernels, the
 // analytic near parts, the self-pair rule), and differentiate it in each
-// coordinate of each distinct node of the pair by the fourth-order central
-// difference [8(f(h) - f(-h)) - (f(2h) - f(-2h))] / 12h with h = 1e-3 of the
-// pair's mean edge length (truncation near 1e-12, rounding near 1e-10
-// relative).  The near/far classification is frozen at the unperturbed
-// mesh, as the reference's is piecewise constant in the nodes.
-struct VFace {
-  double P[3][3];   // nodes
+// coordinate of each distinct node of the pair: the point-pair sums by
+// reverse mode written out by hand (vfar_grad), the analytic near parts by
+// forward mode (D1) through the int.f90 closed forms.  The check of both,
+// mom_test_vertex_fd, takes the fourth-order central difference
+// [8(f(h) - f(-h)) - (f(2h) - f(-2h))] / 12h with h = 1e-3 of the pair's
+// mean edge length (truncation near 1e-12, rounding near 1e-10 relative).
+// The near/far classification is frozen at the unperturbed mesh, as the
+// reference's is piecewise constant in the nodes.
+template <class T>
+struct VFaceT {
+  T P[3][3];        // nodes
   double sg[3];     // sign of each side's RWG function
   int lv[3];        // local index of each side's free vertex
 };
+using VFace = VFaceT<double>;
 
 // node l's coordinate k by selects (a dynamic index would move the face to scratch)
-__device__ __forceinline__ double vnode(const VFace &f, int l, int k) {
+template <class T>
+__device__ __forceinline__ T vnode(const VFaceT<T> &f, int l, int k) {
   return l == 0 ? f.P[0][k] : (l == 1 ? f.P[1][k] : f.P[2][k]);
 }
 
 // One face's RWG data from its nodes (compute_basis_data, mesh.f90:1265-1322;
 // rwgf.f90:34-78): area, coef = s L/(2A), div = s L/A, free vertex.
-__device__ __forceinline__ void vface_rwg(const VFace &f, double &A, double coef[3], double divv[3],
-                                          double v[3][3]) {
-  double a[3], b[3], c[3];
+template <class T>
+__device__ __forceinline__ void vface_rwg(const VFaceT<T> &f, T &A, T coef[3], T divv[3], T v[3][3]) {
+  T a[3], b[3], c[3];
   for (int k = 0; k < 3; k++) {
     a[k] = f.P[2][k] - f.P[0][k];
     b[k] = f.P[2][k] - f.P[1][k];
@@ -2141,23 +2202,25 @@ __device__ __forceinline__ void vface_rwg(const VFace &f, double &A, double coef
   cross3(a, b, c);
   A = 0.5 * norm3(c);
   for (int p = 0; p < 3; p++) {
-    double e[3];
+    T e[3];
     for (int k = 0; k < 3; k++) e[k] = f.P[(p + 1) % 3][k] - f.P[p][k];
-    const double L = norm3(e);
+    const T L = norm3(e);
     coef[p] = f.sg[p] * (L / (2.0 * A));
     divv[p] = f.sg[p] * L / A;
     for (int k = 0; k < 3; k++) v[p][k] = vnode(f, f.lv[p], k);
   }
 }
 
-__device__ __forceinline__ void vface_point(const VFace &f, const QuadRule &rule, int t, double r[3]) {
+template <class T>
+__device__ __forceinline__ void vface_point(const VFaceT<T> &f, const QuadRule &rule, int t, T r[3]) {
   const int Q = rule.Q;
   for (int k = 0; k < 3; k++)
     r[k] = rule.nodes[2 * Q + t] * f.P[0][k] + rule.nodes[t] * f.P[1][k] + rule.nodes[Q + t] * f.P[2][k];
 }
 
-__device__ void vface_near(const VFace &f, NearFace &o) {
-  double a[3], b[3], c[3];
+template <class T>
+__device__ void vface_near(const VFaceT<T> &f, NearFaceT<T> &o) {
+  T a[3], b[3], c[3];
   for (int v = 0; v < 3; v++)
     for (int k = 0; k < 3; k++) o.p[v][k] = f.P[v][k];
   for (int k = 0; k < 3; k++) {
@@ -2165,15 +2228,15 @@ __device__ void vface_near(const VFace &f, NearFace &o) {
     b[k] = f.P[2][k] - f.P[0][k];
   }
   cross3(a, b, c);
-  const double nc = norm3(c);
+  const T nc = norm3(c);
   for (int k = 0; k < 3; k++) o.n[k] = c[k] / nc;
   for (int e = 0; e < 3; e++) {
-    double d[3];
+    T d[3];
     for (int k = 0; k < 3; k++) d[k] = f.P[(e + 1) % 3][k] - f.P[e][k];
-    const double l = norm3(d);
+    const T l = norm3(d);
     for (int k = 0; k < 3; k++) o.s[e][k] = d[k] / l;
     cross3(o.s[e], o.n, c);
-    const double lm = norm3(c);
+    const T lm = norm3(c);
     for (int k = 0; k < 3; k++) o.m[e][k] = c[k] / lm;
   }
 }
@@ -2208,62 +2271,73 @@ __device__ __forceinline__ double2 cm(double2 a, double2 b) {
 
 // Sum over the analytic near parts of test face ft against source face fs
 // (near_fill_kernel's terms) contracted with a[3][3], b[3][3]: per test
-// side p and source side q, a (c1 D + c2 H) + b K.
-__device__ double2 vnear_part(const VPairArgs &g, const VFace &ft, const VFace &fs, bool self, const double2 a[3][3],
-                              const double2 b[3][3]) {
-  NearFace f;
+// side p and source side q, a (c1 D + c2 H) + b K.  T = double for the
+// values, D1 for their derivative in one node coordinate.
+template <class T>
+struct CT {
+  T x, y;
+};
+template <class T>
+__device__ __forceinline__ CT<T> cmt(double2 a, CT<T> b) {
+  return CT<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+template <class T>
+__device__ CT<T> vnear_part_t(const VPairArgs &g, const VFaceT<T> &ft, const VFaceT<T> &fs, bool self,
+                              const double2 a[3][3], const double2 b[3][3]) {
+  NearFaceT<T> f;
   vface_near(fs, f);
-  double At, ct[3], dt[3], vt[3][3], As, cs[3], ds[3], vs[3][3];
+  T At, ct[3], dt[3], vt[3][3], As, cs[3], ds[3], vs[3][3];
   vface_rwg(ft, At, ct, dt, vt);
   vface_rwg(fs, As, cs, ds, vs);
-  double2 acc = make_double2(0.0, 0.0);
+  CT<T> acc{T(0.0), T(0.0)};
   for (int i = 0; i < g.rule.Q; i++) {
-    double r[3];
+    T r[3];
     vface_point(ft, g.rule, i, r);
-    PointInts P;
+    PointIntsT<T> P;
     d_point_ints(r, f, P);
-    double k31[3], k3m1[3], rho[3];
+    T k31[3], k3m1[3], rho[3];
     for (int c = 0; c < 3; c++) {
-      double sm = f.m[0][c] * P.l1[0] + f.m[1][c] * P.l1[1] + f.m[2][c] * P.l1[2];
-      k31[c] = (P.h == 0) ? sm : sm - P.h * f.n[c] * P.sm1;
+      T sm = f.m[0][c] * P.l1[0] + f.m[1][c] * P.l1[1] + f.m[2][c] * P.l1[2];
+      k31[c] = (dval(P.h) == 0) ? sm : sm - P.h * f.n[c] * P.sm1;
       k3m1[c] = f.m[0][c] * P.lm1[0] + f.m[1][c] * P.lm1[1] + f.m[2][c] * P.lm1[2] + f.n[c] * P.sm3h;
       rho[c] = r[c] - P.h * f.n[c];
     }
-    double2 s = make_double2(0.0, 0.0);
+    CT<T> s{T(0.0), T(0.0)};
     for (int q = 0; q < 3; q++) {
-      const double h1 = ds[q] * P.sm1, h0 = ds[q] * P.s1;
-      const double2 aH = make_double2(g.kk4.x * h1 - g.kk8.x * h0, -(g.kk8.y * h0));
-      double2 aD[3], aK[3];
+      const T h1 = ds[q] * P.sm1, h0 = ds[q] * P.s1;
+      const CT<T> aH{g.kk4.x * h1 - g.kk8.x * h0, -(g.kk8.y * h0)};
+      CT<T> aD[3], aK[3];
       for (int c = 0; c < 3; c++) {
-        const double k2m1 = cs[q] * (f.m[0][c] * P.l1[0] + f.m[1][c] * P.l1[1] + f.m[2][c] * P.l1[2] +
-                                     (rho[c] - vs[q][c]) * P.sm1);
-        const double k21 = cs[q] * (1.0 / 3.0 * (f.m[0][c] * P.l3[0] + f.m[1][c] * P.l3[1] + f.m[2][c] * P.l3[2]) +
-                                    (rho[c] - vs[q][c]) * P.s1);
-        aD[c] = make_double2(g.kk4.x * k2m1 - g.kk8.x * k21, -(g.kk8.y * k21));
-        aK[c] = make_double2(0.0, 0.0);
+        const T k2m1 = cs[q] * (f.m[0][c] * P.l1[0] + f.m[1][c] * P.l1[1] + f.m[2][c] * P.l1[2] +
+                                (rho[c] - vs[q][c]) * P.sm1);
+        const T k21 = cs[q] * (1.0 / 3.0 * (f.m[0][c] * P.l3[0] + f.m[1][c] * P.l3[1] + f.m[2][c] * P.l3[2]) +
+                               (rho[c] - vs[q][c]) * P.s1);
+        aD[c] = CT<T>{g.kk4.x * k2m1 - g.kk8.x * k21, -(g.kk8.y * k21)};
+        aK[c] = CT<T>{T(0.0), T(0.0)};
       }
       if (!self) {
-        double d[3] = {r[0] - vs[q][0], r[1] - vs[q][1], r[2] - vs[q][2]}, c1v[3], c0v[3];
+        T d[3] = {r[0] - vs[q][0], r[1] - vs[q][1], r[2] - vs[q][2]}, c1v[3], c0v[3];
         cross3(d, k3m1, c1v);
         cross3(d, k31, c0v);
         for (int c = 0; c < 3; c++) {
-          const double k4m1 = -cs[q] * c1v[c], k41 = -cs[q] * c0v[c];
-          aK[c] = make_double2(g.kk4.x * k4m1 - g.kk8.x * k41, -(g.kk8.y * k41));
+          const T k4m1 = -cs[q] * c1v[c], k41 = -cs[q] * c0v[c];
+          aK[c] = CT<T>{g.kk4.x * k4m1 - g.kk8.x * k41, -(g.kk8.y * k41)};
         }
       }
       for (int p = 0; p < 3; p++) {
-        double fm[3];
+        T fm[3];
         for (int c = 0; c < 3; c++) fm[c] = ct[p] * (r[c] - vt[p][c]);
-        double2 dd = make_double2(0.0, 0.0), kk = make_double2(0.0, 0.0);
+        CT<T> dd{T(0.0), T(0.0)}, kk{T(0.0), T(0.0)};
         for (int c = 0; c < 3; c++) {
           dd.x += fm[c] * aD[c].x;
           dd.y += fm[c] * aD[c].y;
           kk.x += fm[c] * aK[c].x;
           kk.y += fm[c] * aK[c].y;
         }
-        const double2 Dv = cm(g.c1, dd);
-        const double2 Hv = cm(g.c2, make_double2(dt[p] * aH.x, dt[p] * aH.y));
-        const double2 t1 = cm(a[p][q], make_double2(Dv.x + Hv.x, Dv.y + Hv.y)), t2 = cm(b[p][q], kk);
+        const CT<T> Dv = cmt(g.c1, dd);
+        const CT<T> Hv = cmt(g.c2, CT<T>{dt[p] * aH.x, dt[p] * aH.y});
+        const CT<T> t1 = cmt(a[p][q], CT<T>{Dv.x + Hv.x, Dv.y + Hv.y}), t2 = cmt(b[p][q], kk);
         s.x += t1.x + t2.x;
         s.y += t1.y + t2.y;
       }
@@ -2272,6 +2346,12 @@ __device__ double2 vnear_part(const VPairArgs &g, const VFace &ft, const VFace &
     acc.y += g.rule.w[i] * s.y * At;
   }
   return acc;
+}
+
+__device__ double2 vnear_part(const VPairArgs &g, const VFace &ft, const VFace &fs, bool self, const double2 a[3][3],
+                              const double2 b[3][3]) {
+  const CT<double> v = vnear_part_t(g, ft, fs, self, a, b);
+  return make_double2(v.x, v.y);
 }
 
 // The point-pair sums of the unordered pair (ft, fs) with the coefficients of
@@ -2404,7 +2484,7 @@ __device__ void vcoef(const VPairArgs &g, int ft, int fs, double2 a[3][3], doubl
 
 __device__ double2 vpair_value(const VPairArgs &g, const VFace &fm, const VFace &fn, bool near, bool self,
                                const double2 amn[3][3], const double2 bmn[3][3], const double2 anm[3][3],
-                               const double2 bnm[3][3], bool analytic_only = false) {
+                               const double2 bnm[3][3]) {
   double2 ac[3][3], bc[3][3];
   for (int p = 0; p < 3; p++)
     for (int q = 0; q < 3; q++) {
@@ -2412,7 +2492,7 @@ __device__ double2 vpair_value(const VPairArgs &g, const VFace &fm, const VFace 
       ac[p][q] = self ? amn[p][q] : make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
       bc[p][q] = self ? bmn[p][q] : make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
     }
-  double2 v = analytic_only ? make_double2(0.0, 0.0) : vfar_part(g, fm, fn, near, self, ac, bc);
+  double2 v = vfar_part(g, fm, fn, near, self, ac, bc);
   if (near) {
     const double2 n1 = vnear_part(g, fm, fn, self, amn, bmn);
     v.x += n1.x;
@@ -2655,10 +2735,9 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
 // nodes, added to ot[3 v + c] (face m's local node v) and os (face n's; a
 // node of both counts in ot, and a self pair adds only to ot).  The slots
 // are this thread's own (LDS or global memory), so nothing stays in
-// registers across the differences.  analytic_only: the int.f90 analytic
-// parts of a near pair alone.
-__device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near, bool analytic_only, double *ot,
-                            double *os) {
+// registers across the differences.  This is the check of the closed forms
+// (mom_test_vertex_fd).
+__device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near, double *ot, double *os) {
   const bool self = (m == n);
   VFace fm, fn;
   vload_face(g, m, fm);
@@ -2695,7 +2774,7 @@ __device__ void vertex_pair(const VPairArgs &g, int m, int n, bool near, bool an
           if (g.fnodes[3 * m + v] == node) pm.P[v][c] += steps[st];
           if (g.fnodes[3 * n + v] == node) pn.P[v][c] += steps[st];
         }
-        val[st] = vpair_value(g, pm, pn, near, self, amn, bmn, anm, bnm, analytic_only).x;
+        val[st] = vpair_value(g, pm, pn, near, self, amn, bmn, anm, bnm).x;
       }
       const double dv = (8.0 * (val[0] - val[1]) - (val[2] - val[3])) / (12.0 * h);
       int vm = -1, vn = -1;
@@ -2731,7 +2810,7 @@ __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
   if (m < g.F && n < g.F && n >= m) {
     const bool near = near_pair(g.cp + 3 * m, g.cp + 3 * n, g.thr2);
     if (FD) {
-      vertex_pair(g, m, n, near, false, &st[tm][tn][0], &ss[tm][tn][0]);
+      vertex_pair(g, m, n, near, &st[tm][tn][0], &ss[tm][tn][0]);
     } else if (!near) {
       VFace fm, fn;
       vload_face(g, m, fm);
@@ -2771,8 +2850,8 @@ __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
 }
 
 // One thread per entry of the near list with test <= source: the smoothed
-// point loop in closed form, the int.f90 analytic parts by differences;
-// the pair's values go to its entry of near_out.
+// point loop in closed form; the pair's values go to its entry of near_out
+// (vertex_near_analytic_kernel adds the int.f90 analytic parts after).
 __global__ __launch_bounds__(128) void vertex_near_kernel(VPairArgs g) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (id >= g.npairs) return;
@@ -2810,9 +2889,62 @@ __global__ __launch_bounds__(128) void vertex_near_kernel(VPairArgs g) {
         o[9 + 3 * v + c] = gs[v][c];
       }
   }
-  // the analytic parts, added to the entry
+}
+
+// The int.f90 analytic parts of the near pairs, differentiated exactly: one
+// thread per (entry m <= n of the near list, distinct node k of the pair,
+// coordinate c), forward mode (D1) through both faces' geometry, the closed
+// forms and the contraction with the coefficients.  The derivative goes to
+// the thread's own slot of the entry (face m's local node if the node is
+// m's, else face n's), added to what vertex_near_kernel wrote there.
+__global__ __launch_bounds__(256) void vertex_near_analytic_kernel(VPairArgs g) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t id = t / 18;
+  if (id >= g.npairs) return;
+  const int sub = (int)(t - 18 * id), k = sub / 3, c = sub - 3 * k;
+  const int m = g.pairs[2 * id], n = g.pairs[2 * id + 1];
+  if (n < m) return;
+  const bool self = (m == n);
+  // distinct nodes: face m's three, then face n's others in order
+  const int m0 = g.fnodes[3 * m], m1 = g.fnodes[3 * m + 1], m2 = g.fnodes[3 * m + 2];
+  int node = k == 0 ? m0 : (k == 1 ? m1 : (k == 2 ? m2 : -1));
+  for (int v = 0, cnt = 3; v < 3; v++) {
+    const int nv = g.fnodes[3 * n + v];
+    if (nv == m0 || nv == m1 || nv == m2) continue;
+    if (cnt == k) node = nv;
+    cnt++;
+  }
+  if (node < 0) return;
+  VFace fm0, fn0;
+  vload_face(g, m, fm0);
+  vload_face(g, n, fn0);
+  VFaceT<D1> fm, fn;
+  int vm = -1, vn = -1;
+  for (int v = 0; v < 3; v++) {
+    const bool om = g.fnodes[3 * m + v] == node, on = g.fnodes[3 * n + v] == node;
+    if (om) vm = v;
+    if (on) vn = v;
+    for (int j = 0; j < 3; j++) {
+      fm.P[v][j] = D1(fm0.P[v][j], (om && j == c) ? 1.0 : 0.0);
+      fn.P[v][j] = D1(fn0.P[v][j], (on && j == c) ? 1.0 : 0.0);
+    }
+    fm.sg[v] = fm0.sg[v];
+    fm.lv[v] = fm0.lv[v];
+    fn.sg[v] = fn0.sg[v];
+    fn.lv[v] = fn0.lv[v];
+  }
+  double2 a[3][3], b[3][3];
+  vcoef(g, m, n, a, b);
+  double dv = vnear_part_t(g, fm, fn, self, a, b).x.d;
+  if (!self) {
+    vcoef(g, n, m, a, b);
+    dv += vnear_part_t(g, fn, fm, false, a, b).x.d;
+  }
   double *o = g.near_out + 18 * id;
-  vertex_pair(g, m, n, true, true, o, o + 9);
+  if (vm >= 0)
+    o[3 * vm + c] += dv;
+  else
+    o[9 + 3 * vn + c] += dv;
 }
 
 // Per face: its tile partial sums (in tile order) and its near pairs (in
@@ -3009,6 +3141,9 @@ int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
   MOM_HIP(hipGetLastError());
   if (g.near_out) {
     hipLaunchKernelGGL(vertex_near_kernel, dim3((unsigned)((m->nnear + 127) / 128)), dim3(128), 0, s, g);
+    MOM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(vertex_near_analytic_kernel, dim3((unsigned)((18 * m->nnear + 255) / 256)), dim3(256), 0, s,
+                       g);
     MOM_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(vertex_face_sum_kernel, dim3((unsigned)((F + 127) / 128)), dim3(128), 0, s, g, fg);

@@ -66,7 +66,8 @@ int mom_test_unit_lower_inverse(mom_ctx *ctx, const double *L, int64_t ldl, doub
 
 /* Node-coordinate adjoint: 1 = every face pair by fourth-order differences
  * (the closed forms of the far pairs and of the near pairs' smoothed point
- * loop are checked against it), 0 = the default. */
+ * loop, and the forward-mode derivatives of the near pairs' analytic parts,
+ * are checked against it), 0 = the default. */
 int mom_test_vertex_fd(int32_t enable);
 
 /* Block refinement of many right-hand sides in the mixed path: 1 = take the

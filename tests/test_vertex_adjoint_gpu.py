@@ -110,8 +110,9 @@ def test_vertex_gradient_matches_finite_differences(gpu_ctx):
 
 
 def test_far_pair_closed_form_matches_differences(gpu_ctx):
-    """The closed-form derivatives (far pairs, and the near pairs' smoothed
-    point loop) against fourth-order differences of the same pair sums
+    """The closed-form derivatives (far pairs, the near pairs' smoothed point
+    loop and, by forward mode, their int.f90 analytic parts) against
+    fourth-order differences of the same pair sums
     (mom_test_vertex_fd), per domain and node coordinate: <= 1e-8 of the
     largest component."""
     import ctypes as C  # noqa: F401
