@@ -2507,85 +2507,119 @@ __device__ double2 vpair_value(const VPairArgs &g, const VFace &fm, const VFace 
 }
 
 // Far pairs: the derivative of the point-pair sum in closed form (reverse
-// mode by hand).  With e_p = r_i - v_p, f_q = r'_j - v_q, d = r_i - r'_j,
-// alpha = s L/2 and beta = s L (the areas cancel against the RWG
-// coefficients), each point pair adds
+// mode by hand).  With e_p = r - v_p, f_q = r' - v_q, d = r - r' (r, r' the
+// test and source points, v the free vertices), alpha = s L/2 and beta =
+// s L (the areas cancel against the RWG coefficients), each point pair adds
 //   G (c1 sum_pq a_pq alpha_p alpha_q e_p.f_q + c2 sum_pq a_pq beta_p beta_q)
-//   + g sum_pq b_pq alpha_p alpha_q d.(f_q x e_p),
-// with G' = G (ik - 1/R) and g' = G (k^2/R - 3/R^3 + 3ik/R^2); the gradients
-// with respect to e, f, d, R and alpha are carried to the barycentric
-// points, the free vertices and the edge lengths.  gt/gs: Re d/d(node) of
-// the test and source faces' nodes (local order).
+//   + g sum_pq b_pq alpha_p alpha_q d.(f_q x e_p).
+// With A_pq = a_pq alpha_p alpha_q and B_pq = b_pq alpha_p alpha_q the sums
+// are polynomials in r and r' with per-pair coefficients,
+//   sum A e_p.f_q = S r.r' - r.Vs - Vt.r' + W,
+//   sum B d.(f_q x e_p) = Y.(r x r') + d.X,
+// S = sum A, Vt = sum A v_p, Vs = sum A v_q, W = sum A v_p.v_q,
+// Y = sum B (v_q - v_p), X = sum B (v_q x v_p).  The point loop therefore
+// carries only the gradients in r and r' (to the barycentric points) and 14
+// complex moments of G and g, which give the derivatives in the
+// coefficients; those go to the edge lengths and free vertices per pair.
+// G' = G (ik - 1/R) and g' = G (k^2/R - 3/R^3 + 3ik/R^2).  gt/gs: Re
+// d/d(node) of the test and source faces' nodes (local order).
 // NEAR: the smoothed kernels of near pairs, Gs = [(e^{ikR} - 1)/R + k^2 R/2]/4pi
 // and gs = [e^{ikR}(1/R^3 - ik/R^2) - 1/R^3 - k^2/(2R)]/4pi, with
 // Gs' = [ik e^{ikR}/R - (e^{ikR} - 1)/R^2 + k^2/2]/4pi and
 // gs' = [ik e^{ikR}(1/R^3 - ik/R^2) + e^{ikR}(2ik/R^3 - 3/R^4) + 3/R^4 + k^2/(2R^2)]/4pi;
 // coincident points (R = 0, self pairs) contribute Gs(0) = ik/4pi and no
 // derivative.  A self pair passes b = 0 (its K block is zero).
+__device__ __forceinline__ double re_mul(double2 a, double2 b) { return a.x * b.x - a.y * b.y; }
+
+// The coefficients of the unordered pair (m, n) with both orders combined
+// (a[p][q] for side p of m and side q of n); a self pair takes its own a and
+// b = 0 (its K block is zero).
+__device__ void vcoef_pair(const VPairArgs &g, int m, int n, double2 a[3][3], double2 b[3][3]) {
+  vcoef(g, m, n, a, b);
+  if (m == n) {
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) b[p][q] = make_double2(0.0, 0.0);
+    return;
+  }
+  double2 an[3][3], bn[3][3];
+  vcoef(g, n, m, an, bn);
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) {
+      a[p][q] = make_double2(a[p][q].x + an[q][p].x, a[p][q].y + an[q][p].y);
+      b[p][q] = make_double2(b[p][q].x + bn[q][p].x, b[p][q].y + bn[q][p].y);
+    }
+}
+
 template <bool NEAR>
-__device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, const double2 a[3][3],
-                          const double2 b[3][3], double gt[3][3], double gs[3][3]) {
-  double al_t[3], al_s[3], ut[3][3], us[3][3], vt[3][3], vs[3][3];
+__device__ void vfar_grad(const VPairArgs &g, int m, int n, const VFace &ft, const VFace &fs, double gt[3][3],
+                          double gs[3][3]) {
+  double al_t[3], al_s[3];
   for (int p = 0; p < 3; p++) {
     double et[3], es[3];
     for (int k = 0; k < 3; k++) {
       et[k] = ft.P[(p + 1) % 3][k] - ft.P[p][k];
       es[k] = fs.P[(p + 1) % 3][k] - fs.P[p][k];
-      vt[p][k] = vnode(ft, ft.lv[p], k);
-      vs[p][k] = vnode(fs, fs.lv[p], k);
     }
-    const double Lt = norm3(et), Ls = norm3(es);
-    al_t[p] = ft.sg[p] * Lt / 2.0;
-    al_s[p] = fs.sg[p] * Ls / 2.0;
-    for (int k = 0; k < 3; k++) {
-      ut[p][k] = et[k] / Lt;
-      us[p][k] = es[k] / Ls;
-    }
+    al_t[p] = ft.sg[p] * norm3(et) / 2.0;
+    al_s[p] = fs.sg[p] * norm3(es) / 2.0;
   }
-  double2 T2 = make_double2(0.0, 0.0);
-  for (int p = 0; p < 3; p++)
-    for (int q = 0; q < 3; q++) {
-      const double bb = 4.0 * al_t[p] * al_s[q];
-      T2.x += a[p][q].x * bb;
-      T2.y += a[p][q].y * bb;
+  // the pair's coefficients, with c1 and c2 folded in where the loop uses them
+  double2 c1S, c1Vs[3], c1Vt[3], K0, Y[3], X[3];
+  {
+    double2 a[3][3], b[3][3];
+    vcoef_pair(g, m, n, a, b);
+    double2 S = make_double2(0.0, 0.0), W = S, Vs[3], Vt[3];
+    for (int c = 0; c < 3; c++) Vs[c] = Vt[c] = Y[c] = X[c] = S;
+    for (int p = 0; p < 3; p++)
+      for (int q = 0; q < 3; q++) {
+        const double aa = al_t[p] * al_s[q];
+        const double2 At = make_double2(a[p][q].x * aa, a[p][q].y * aa);
+        const double2 Bt = make_double2(b[p][q].x * aa, b[p][q].y * aa);
+        double vp[3], vq[3];
+        for (int c = 0; c < 3; c++) {
+          vp[c] = vnode(ft, ft.lv[p], c);
+          vq[c] = vnode(fs, fs.lv[q], c);
+        }
+        double x[3];
+        cross3(vq, vp, x);
+        const double vv = vp[0] * vq[0] + vp[1] * vq[1] + vp[2] * vq[2];
+        S.x += At.x;
+        S.y += At.y;
+        W.x += At.x * vv;
+        W.y += At.y * vv;
+        for (int c = 0; c < 3; c++) {
+          Vt[c].x += At.x * vp[c];
+          Vt[c].y += At.y * vp[c];
+          Vs[c].x += At.x * vq[c];
+          Vs[c].y += At.y * vq[c];
+          Y[c].x += Bt.x * (vq[c] - vp[c]);
+          Y[c].y += Bt.y * (vq[c] - vp[c]);
+          X[c].x += Bt.x * x[c];
+          X[c].y += Bt.y * x[c];
+        }
+      }
+    c1S = cm(g.c1, S);
+    for (int c = 0; c < 3; c++) {
+      c1Vs[c] = cm(g.c1, Vs[c]);
+      c1Vt[c] = cm(g.c1, Vt[c]);
     }
-  const double2 c2T2 = cm(g.c2, T2);
+    const double2 c1W = cm(g.c1, W), c2S = cm(g.c2, S);
+    K0 = make_double2(c1W.x + 4.0 * c2S.x, c1W.y + 4.0 * c2S.y);
+  }
   const double kr = g.kr, ki = g.ki, k2r = kr * kr - ki * ki, k2i = 2.0 * kr * ki;
-  // Only Re d Phi / d(node) is wanted, and every accumulator below is a
-  // real linear function of its terms, so only real parts are summed (the
-  // complex factors are multiplied out before).  GT/GS: by node
-  // (barycentric points); VT/VS: by side (free vertices), mapped to nodes at
-  // the end so that every array index stays static.
-  double GT[3][3], GS[3][3], VT[3][3], VS[3][3], dAt[3], dAs[3];
-  double2 SG = make_double2(0.0, 0.0);
-  for (int k = 0; k < 3; k++) {
-    dAt[k] = dAs[k] = 0.0;
-    for (int c = 0; c < 3; c++) GT[k][c] = GS[k][c] = VT[k][c] = VS[k][c] = 0.0;
-  }
+  // GT/GS: Re d/d(node) through the points; moments of G and g
+  double GT[3][3], GS[3][3];
+  for (int k = 0; k < 3; k++)
+    for (int c = 0; c < 3; c++) GT[k][c] = GS[k][c] = 0.0;
+  double2 MG = make_double2(0.0, 0.0), Mrr = MG, MGr[3], MGs[3], Mgx[3], Mgd[3];
+  for (int c = 0; c < 3; c++) MGr[c] = MGs[c] = Mgx[c] = Mgd[c] = MG;
   const int Q = g.rule.Q;
   for (int i = 0; i < Q; i++) {
     double r[3];
     vface_point(ft, g.rule, i, r);
-    double e[3][3];
-    for (int p = 0; p < 3; p++)
-      for (int c = 0; c < 3; c++) e[p][c] = r[c] - vt[p][c];
-    double2 Xa[3][3], Xb[3][3];  // sum_p {a,b}_pq alpha_p e_p
-    for (int q = 0; q < 3; q++)
-      for (int c = 0; c < 3; c++) {
-        Xa[q][c] = Xb[q][c] = make_double2(0.0, 0.0);
-        for (int p = 0; p < 3; p++) {
-          const double ae = al_t[p] * e[p][c];
-          Xa[q][c].x += a[p][q].x * ae;
-          Xa[q][c].y += a[p][q].y * ae;
-          Xb[q][c].x += b[p][q].x * ae;
-          Xb[q][c].y += b[p][q].y * ae;
-        }
-      }
-    double dRi[3], dE[3][3], dAti[3];
-    for (int k = 0; k < 3; k++) {
-      dRi[k] = dAti[k] = 0.0;
-      for (int c = 0; c < 3; c++) dE[k][c] = 0.0;
-    }
+    double dRi[3] = {0.0, 0.0, 0.0};
+    double2 sG = make_double2(0.0, 0.0), sg = sG, sGr[3], sgr[3];
+    for (int c = 0; c < 3; c++) sGr[c] = sgr[c] = sG;
     for (int j = 0; j < Q; j++) {
       double rs[3];
       vface_point(fs, g.rule, j, rs);
@@ -2626,98 +2660,141 @@ __device__ void vfar_grad(const VPairArgs &g, const VFace &ft, const VFace &fs, 
         gp = make_double2((ikEh.x + Ehp.x + 3.0 * iR4 + 0.5 * k2r * iR2) * kOofpi,
                           (ikEh.y + Ehp.y + 0.5 * k2i * iR2) * kOofpi);
       }
-      const double ww = g.rule.w[i] * g.rule.w[j];
-      double f[3][3];
-      for (int q = 0; q < 3; q++)
-        for (int c = 0; c < 3; c++) f[q][c] = rs[c] - vs[q][c];
-      // T1 = sum_q alpha_q f_q . Xa_q ; T3 = sum_q alpha_q d . (f_q x Xb_q)
-      double2 T1 = make_double2(0.0, 0.0), T3 = make_double2(0.0, 0.0), Dx[3];
-      for (int c = 0; c < 3; c++) Dx[c] = make_double2(0.0, 0.0);
-      double dF[3][3];  // Re d Psi / d f_q
-      const double2 Gc1 = cm(G, g.c1);
-      for (int q = 0; q < 3; q++) {
-        double sa = 0.0;  // Re d Psi / d alpha_q = Re f_q . (G c1 Xa_q + g Xb_q x d)
-        for (int c = 0; c < 3; c++) {
-          const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-          const double2 fxX = make_double2(f[q][c1] * Xb[q][c2].x - f[q][c2] * Xb[q][c1].x,
-                                           f[q][c1] * Xb[q][c2].y - f[q][c2] * Xb[q][c1].y);
-          const double2 Xxd = make_double2(Xb[q][c1].x * d[c2] - Xb[q][c2].x * d[c1],
-                                           Xb[q][c1].y * d[c2] - Xb[q][c2].y * d[c1]);
-          T1.x += al_s[q] * f[q][c] * Xa[q][c].x;
-          T1.y += al_s[q] * f[q][c] * Xa[q][c].y;
-          T3.x += al_s[q] * d[c] * fxX.x;
-          T3.y += al_s[q] * d[c] * fxX.y;
-          Dx[c].x += al_s[q] * fxX.x;
-          Dx[c].y += al_s[q] * fxX.y;
-          const double u = (Gc1.x * Xa[q][c].x - Gc1.y * Xa[q][c].y) + (gg.x * Xxd.x - gg.y * Xxd.y);
-          dF[q][c] = al_s[q] * u;
-          sa += f[q][c] * u;
-        }
-        dAs[q] += ww * sa;
-      }
-      // Re d Psi / d R and the total Re d Psi / d d
-      const double2 c1T1 = cm(g.c1, T1);
-      const double2 A1 = make_double2(c1T1.x + c2T2.x, c1T1.y + c2T2.y);
-      const double dPR = (Gp.x * A1.x - Gp.y * A1.y) + (gp.x * T3.x - gp.y * T3.y);
-      double DD[3];
-      for (int c = 0; c < 3; c++) DD[c] = (gg.x * Dx[c].x - gg.y * Dx[c].y) + dPR * d[c] * iR;
-      // Re d Psi / d e_p = alpha_p Re (G c1 Fa_p + g Fb_p), Fa_p = sum_q a_pq alpha_q f_q,
-      // Fb_p = sum_q b_pq alpha_q d x f_q
-      for (int p = 0; p < 3; p++) {
-        double tot[3] = {0.0, 0.0, 0.0};
-        for (int q = 0; q < 3; q++) {
-          double dxf[3];
-          cross3(d, f[q], dxf);
-          const double ga = Gc1.x * a[p][q].x - Gc1.y * a[p][q].y, gb = gg.x * b[p][q].x - gg.y * b[p][q].y;
-          for (int c = 0; c < 3; c++) tot[c] += al_s[q] * (ga * f[q][c] + gb * dxf[c]);
-        }
-        double sa = 0.0;
-        for (int c = 0; c < 3; c++) {
-          sa += e[p][c] * tot[c];
-          const double de = al_t[p] * tot[c];
-          dE[p][c] += ww * de;
-          dRi[c] += ww * de;
-        }
-        dAti[p] += ww * sa;
-      }
-      for (int c = 0; c < 3; c++) dRi[c] += ww * DD[c];
-      // source side: r'_j = sum_k b_jk S_k, d depends on -r'_j; free vertices
-      const double bj[3] = {g.rule.nodes[2 * Q + j], g.rule.nodes[j], g.rule.nodes[Q + j]};
+      // P1 = c1 (S r.r' - r.Vs - Vt.r' + W) + 4 c2 S, P2 = Y.(r x r') + d.X
+      double rxs[3];
+      cross3(r, rs, rxs);
+      const double rr = r[0] * rs[0] + r[1] * rs[1] + r[2] * rs[2];
+      double2 P1 = make_double2(c1S.x * rr + K0.x, c1S.y * rr + K0.y), P2 = make_double2(0.0, 0.0);
       for (int c = 0; c < 3; c++) {
-        double sf = -DD[c];
-        for (int q = 0; q < 3; q++) {
-          sf += dF[q][c];
-          VS[q][c] -= ww * dF[q][c];
-        }
-        for (int k = 0; k < 3; k++) GS[k][c] += ww * bj[k] * sf;
+        P1.x -= r[c] * c1Vs[c].x + rs[c] * c1Vt[c].x;
+        P1.y -= r[c] * c1Vs[c].y + rs[c] * c1Vt[c].y;
+        P2.x += Y[c].x * rxs[c] + d[c] * X[c].x;
+        P2.y += Y[c].y * rxs[c] + d[c] * X[c].y;
       }
-      SG.x += ww * G.x;
-      SG.y += ww * G.y;
+      // Re dPsi/dR / R, and the parts of Re dPsi/dr, dPsi/dr'
+      const double hR = (re_mul(Gp, P1) + re_mul(gp, P2)) * iR;
+      const double GSr = re_mul(G, c1S);
+      double gY[3], Dr[3], Ds[3];
+      for (int c = 0; c < 3; c++) gY[c] = re_mul(gg, Y[c]);
+      double yxr[3], sxy[3];
+      cross3(rs, gY, sxy);
+      cross3(gY, r, yxr);
+      const double ww = g.rule.w[i] * g.rule.w[j], wj = g.rule.w[j];
+      for (int c = 0; c < 3; c++) {
+        const double gX = re_mul(gg, X[c]);
+        Dr[c] = hR * d[c] + GSr * rs[c] - re_mul(G, c1Vs[c]) + sxy[c] + gX;
+        Ds[c] = -hR * d[c] + GSr * r[c] - re_mul(G, c1Vt[c]) + yxr[c] - gX;
+        dRi[c] += wj * Dr[c];
+      }
+      const double bj[3] = {g.rule.nodes[2 * Q + j], g.rule.nodes[j], g.rule.nodes[Q + j]};
+      for (int k = 0; k < 3; k++) {
+        const double wb = ww * bj[k];
+        for (int c = 0; c < 3; c++) GS[k][c] += wb * Ds[c];
+      }
+      sG.x += wj * G.x;
+      sG.y += wj * G.y;
+      sg.x += wj * gg.x;
+      sg.y += wj * gg.y;
+      for (int c = 0; c < 3; c++) {
+        sGr[c].x += wj * G.x * rs[c];
+        sGr[c].y += wj * G.y * rs[c];
+        sgr[c].x += wj * gg.x * rs[c];
+        sgr[c].y += wj * gg.y * rs[c];
+      }
     }
+    const double wi = g.rule.w[i];
     const double bi[3] = {g.rule.nodes[2 * Q + i], g.rule.nodes[i], g.rule.nodes[Q + i]};
+    for (int k = 0; k < 3; k++)
+      for (int c = 0; c < 3; c++) GT[k][c] += wi * bi[k] * dRi[c];
+    MG.x += wi * sG.x;
+    MG.y += wi * sG.y;
     for (int c = 0; c < 3; c++) {
-      for (int k = 0; k < 3; k++) GT[k][c] += bi[k] * dRi[c];
-      for (int p = 0; p < 3; p++) VT[p][c] -= dE[p][c];
+      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      MGr[c].x += wi * r[c] * sG.x;
+      MGr[c].y += wi * r[c] * sG.y;
+      MGs[c].x += wi * sGr[c].x;
+      MGs[c].y += wi * sGr[c].y;
+      Mrr.x += wi * r[c] * sGr[c].x;
+      Mrr.y += wi * r[c] * sGr[c].y;
+      Mgx[c].x += wi * (r[c1] * sgr[c2].x - r[c2] * sgr[c1].x);
+      Mgx[c].y += wi * (r[c1] * sgr[c2].y - r[c2] * sgr[c1].y);
+      Mgd[c].x += wi * (r[c] * sg.x - sgr[c].x);
+      Mgd[c].y += wi * (r[c] * sg.y - sgr[c].y);
     }
-    for (int p = 0; p < 3; p++) dAt[p] += dAti[p];
   }
-  // the c2 T2 term's edge-length dependence: d/d alpha_p of c2 SG sum a_pq 4 alpha_p alpha_q
-  const double2 c2SG = cm(g.c2, SG);
+  // derivatives in the coefficients: dS = c1 Mrr + 4 c2 MG, dVs = -c1 MGr,
+  // dVt = -c1 MGs, dW = c1 MG, dY = Mgx, dX = Mgd
+  double2 DS, DVs[3], DVt[3], DW;
+  {
+    const double2 a1 = cm(g.c1, Mrr), a2 = cm(g.c2, MG);
+    DS = make_double2(a1.x + 4.0 * a2.x, a1.y + 4.0 * a2.y);
+    DW = cm(g.c1, MG);
+    for (int c = 0; c < 3; c++) {
+      const double2 t1 = cm(g.c1, MGr[c]), t2 = cm(g.c1, MGs[c]);
+      DVs[c] = make_double2(-t1.x, -t1.y);
+      DVt[c] = make_double2(-t2.x, -t2.y);
+    }
+  }
+  // ... and through them in the edge lengths (dAt, dAs) and free vertices;
+  // the a, b are read again rather than held through the point loop (the
+  // empty asm keeps the compiler from reusing the first reads)
+  asm volatile("" ::: "memory");
+  double2 a[3][3], b[3][3];
+  vcoef_pair(g, m, n, a, b);
+  double dAt[3] = {0.0, 0.0, 0.0}, dAs[3] = {0.0, 0.0, 0.0}, VT[3][3], VS[3][3];
+  for (int k = 0; k < 3; k++)
+    for (int c = 0; c < 3; c++) VT[k][c] = VS[k][c] = 0.0;
   for (int p = 0; p < 3; p++)
     for (int q = 0; q < 3; q++) {
-      const double t = c2SG.x * a[p][q].x - c2SG.y * a[p][q].y;
-      dAt[p] += 4.0 * al_s[q] * t;
-      dAs[q] += 4.0 * al_t[p] * t;
+      double vp[3], vq[3];
+      for (int c = 0; c < 3; c++) {
+        vp[c] = vnode(ft, ft.lv[p], c);
+        vq[c] = vnode(fs, fs.lv[q], c);
+      }
+      double x[3];
+      cross3(vq, vp, x);
+      const double vv = vp[0] * vq[0] + vp[1] * vq[1] + vp[2] * vq[2];
+      // Om = a (DS + DVt.v_p + DVs.v_q + DW v_p.v_q) + b (DY.(v_q - v_p) + DX.(v_q x v_p))
+      double2 oa = make_double2(DS.x + DW.x * vv, DS.y + DW.y * vv), ob = make_double2(0.0, 0.0);
+      for (int c = 0; c < 3; c++) {
+        oa.x += DVt[c].x * vp[c] + DVs[c].x * vq[c];
+        oa.y += DVt[c].y * vp[c] + DVs[c].y * vq[c];
+        ob.x += Mgx[c].x * (vq[c] - vp[c]) + Mgd[c].x * x[c];
+        ob.y += Mgx[c].y * (vq[c] - vp[c]) + Mgd[c].y * x[c];
+      }
+      const double om = re_mul(a[p][q], oa) + re_mul(b[p][q], ob);
+      dAt[p] += al_s[q] * om;
+      dAs[q] += al_t[p] * om;
+      // d/dv_p: a (DVt + DW v_q) + b (-DY + DX x v_q); d/dv_q: a (DVs + DW v_p) + b (DY + v_p x DX)
+      const double aa = al_t[p] * al_s[q];
+      for (int c = 0; c < 3; c++) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        const double2 xq = make_double2(Mgd[c1].x * vq[c2] - Mgd[c2].x * vq[c1], Mgd[c1].y * vq[c2] - Mgd[c2].y * vq[c1]);
+        const double2 px = make_double2(vp[c1] * Mgd[c2].x - vp[c2] * Mgd[c1].x, vp[c1] * Mgd[c2].y - vp[c2] * Mgd[c1].y);
+        const double tp = re_mul(a[p][q], make_double2(DVt[c].x + DW.x * vq[c], DVt[c].y + DW.y * vq[c])) +
+                          re_mul(b[p][q], make_double2(xq.x - Mgx[c].x, xq.y - Mgx[c].y));
+        const double tq = re_mul(a[p][q], make_double2(DVs[c].x + DW.x * vp[c], DVs[c].y + DW.y * vp[c])) +
+                          re_mul(b[p][q], make_double2(Mgx[c].x + px.x, Mgx[c].y + px.y));
+        VT[p][c] += aa * tp;
+        VS[q][c] += aa * tq;
+      }
     }
   // alpha_p = s_p L_p / 2 with L_p = |P_{p+1} - P_p|
-  for (int p = 0; p < 3; p++)
+  for (int p = 0; p < 3; p++) {
+    double et[3], es[3];
+    for (int k = 0; k < 3; k++) {
+      et[k] = ft.P[(p + 1) % 3][k] - ft.P[p][k];
+      es[k] = fs.P[(p + 1) % 3][k] - fs.P[p][k];
+    }
+    const double lt = norm3(et), ls = norm3(es);
     for (int c = 0; c < 3; c++) {
-      const double ht = ft.sg[p] * 0.5 * ut[p][c], hs = fs.sg[p] * 0.5 * us[p][c];
+      const double ht = ft.sg[p] * 0.5 * et[c] / lt, hs = fs.sg[p] * 0.5 * es[c] / ls;
       GT[(p + 1) % 3][c] += dAt[p] * ht;
       GT[p][c] -= dAt[p] * ht;
       GS[(p + 1) % 3][c] += dAs[p] * hs;
       GS[p][c] -= dAs[p] * hs;
     }
+  }
   for (int k = 0; k < 3; k++)
     for (int c = 0; c < 3; c++) {
       double t = GT[k][c], u = GS[k][c];
@@ -2815,16 +2892,8 @@ __global__ __launch_bounds__(128) void vertex_pair_kernel(VPairArgs g) {
       VFace fm, fn;
       vload_face(g, m, fm);
       vload_face(g, n, fn);
-      double2 amn[3][3], bmn[3][3], anm[3][3], bnm[3][3];
-      vcoef(g, m, n, amn, bmn);
-      vcoef(g, n, m, anm, bnm);
-      for (int p = 0; p < 3; p++)
-        for (int q = 0; q < 3; q++) {
-          amn[p][q] = make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
-          bmn[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
-        }
       double gt[3][3], gs[3][3];
-      vfar_grad<false>(g, fm, fn, amn, bmn, gt, gs);
+      vfar_grad<false>(g, m, n, fm, fn, gt, gs);
       for (int v = 0; v < 3; v++)
         for (int c = 0; c < 3; c++) {
           st[tm][tn][3 * v + c] = gt[v][c];
@@ -2862,20 +2931,8 @@ __global__ __launch_bounds__(128) void vertex_near_kernel(VPairArgs g) {
     VFace fm, fn;
     vload_face(g, m, fm);
     vload_face(g, n, fn);
-    double2 amn[3][3], bmn[3][3], anm[3][3], bnm[3][3];
-    vcoef(g, m, n, amn, bmn);
-    vcoef(g, n, m, anm, bnm);
-    for (int p = 0; p < 3; p++)
-      for (int q = 0; q < 3; q++) {
-        if (self) {
-          bmn[p][q] = make_double2(0.0, 0.0);
-        } else {
-          amn[p][q] = make_double2(amn[p][q].x + anm[q][p].x, amn[p][q].y + anm[q][p].y);
-          bmn[p][q] = make_double2(bmn[p][q].x + bnm[q][p].x, bmn[p][q].y + bnm[q][p].y);
-        }
-      }
     double gt[3][3], gs[3][3];
-    vfar_grad<true>(g, fm, fn, amn, bmn, gt, gs);
+    vfar_grad<true>(g, m, n, fm, fn, gt, gs);
     if (self)  // both copies of the face are face m
       for (int v = 0; v < 3; v++)
         for (int c = 0; c < 3; c++) {
