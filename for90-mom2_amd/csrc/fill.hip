@@ -76,28 +76,9 @@ struct FarArgs {
 // [-1000, 710]: exp underflows to 0 below and overflows to inf above.
 constexpr double kCisMaxArg = 524288.0;  // 2^19
 
-__device__ __forceinline__ void cis_exp(double a, double b, double &er, double &ei) {
-  a = fmin(fmax(a, -1000.0), 710.0);
-  // exp(a) = 2^n exp(r)
-  const double nf = rint(a * 0x1.71547652b82fep+0);  // a / ln2
-  double r = fma(-nf, 0x1.62e42fee00000p-1, a);
-  r = fma(-nf, 0x1.a39ef35793c76p-33, r);
-  double p = 0x1.6124613a86d09p-33;                   // 1/13!
-  p = fma(p, r, 0x1.1eed8eff8d898p-29);               // 1/12!
-  p = fma(p, r, 0x1.ae64567f544e4p-26);               // 1/11!
-  p = fma(p, r, 0x1.27e4fb7789f5cp-22);               // 1/10!
-  p = fma(p, r, 0x1.71de3a556c734p-19);               // 1/9!
-  p = fma(p, r, 0x1.a01a01a01a01ap-16);               // 1/8!
-  p = fma(p, r, 0x1.a01a01a01a01ap-13);               // 1/7!
-  p = fma(p, r, 0x1.6c16c16c16c17p-10);               // 1/6!
-  p = fma(p, r, 0x1.1111111111111p-7);                // 1/5!
-  p = fma(p, r, 0x1.5555555555555p-5);                // 1/4!
-  p = fma(p, r, 0x1.5555555555555p-3);                // 1/3!
-  p = fma(p, r, 0.5);
-  p = fma(p, r, 1.0);
-  p = fma(p, r, 1.0);
-  const double ex = ldexp(p, (int)nf);
-  // sin, cos of b = q pi/2 + t
+// cos b and sin b (the rotation part of cis_exp); cis_exp(a, b) with a = 0
+// gives exactly these values, since its exponential is then 1 exactly.
+__device__ __forceinline__ void cis(double b, double &c, double &s) {
   const double qf = rint(b * 0x1.45f306dc9c883p-1);  // b * 2/pi
   double t = fma(-qf, 0x1.921fb54400000p+0, b);
   t = fma(-qf, 0x1.0b4611a600000p-34, t);
@@ -122,8 +103,35 @@ __device__ __forceinline__ void cis_exp(double a, double b, double &er, double &
   const double cs = fma(cp, t2, 1.0);
   const int q = (int)qf & 3;
   const double s1 = (q & 1) ? cs : sn, c1 = (q & 1) ? sn : cs;
-  ei = ex * ((q & 2) ? -s1 : s1);
-  er = ex * (((q + 1) & 2) ? -c1 : c1);
+  s = (q & 2) ? -s1 : s1;
+  c = ((q + 1) & 2) ? -c1 : c1;
+}
+
+__device__ __forceinline__ void cis_exp(double a, double b, double &er, double &ei) {
+  a = fmin(fmax(a, -1000.0), 710.0);
+  // exp(a) = 2^n exp(r)
+  const double nf = rint(a * 0x1.71547652b82fep+0);  // a / ln2
+  double r = fma(-nf, 0x1.62e42fee00000p-1, a);
+  r = fma(-nf, 0x1.a39ef35793c76p-33, r);
+  double p = 0x1.6124613a86d09p-33;                   // 1/13!
+  p = fma(p, r, 0x1.1eed8eff8d898p-29);               // 1/12!
+  p = fma(p, r, 0x1.ae64567f544e4p-26);               // 1/11!
+  p = fma(p, r, 0x1.27e4fb7789f5cp-22);               // 1/10!
+  p = fma(p, r, 0x1.71de3a556c734p-19);               // 1/9!
+  p = fma(p, r, 0x1.a01a01a01a01ap-16);               // 1/8!
+  p = fma(p, r, 0x1.a01a01a01a01ap-13);               // 1/7!
+  p = fma(p, r, 0x1.6c16c16c16c17p-10);               // 1/6!
+  p = fma(p, r, 0x1.1111111111111p-7);                // 1/5!
+  p = fma(p, r, 0x1.5555555555555p-5);                // 1/4!
+  p = fma(p, r, 0x1.5555555555555p-3);                // 1/3!
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  const double ex = ldexp(p, (int)nf);
+  double c, sn;
+  cis(b, c, sn);
+  ei = ex * sn;
+  er = ex * c;
 }
 
 // near_faces (sysmat.f90:233-260) without FMA contraction, so the
@@ -194,7 +202,9 @@ constexpr int kWaves = 4;
 // DK: the k-derivatives of the kernels instead (derivative of the fill with
 // respect to the wavenumber): dG/dk = i R G, dg/dk = k G; smoothed parts
 // dGs/dk = (i e^{ikR} + k R)/4pi, dgs/dk = k (e^{ikR} - 1)/(4pi R).
-template <bool SMOOTH, int QT, bool DK = false>
+// LL: a lossless medium (Im k = 0), where e^{ikR} is cos kR + i sin kR with
+// the same bits as cis_exp gives it.
+template <bool SMOOTH, int QT, bool DK = false, bool LL = false>
 __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *__restrict__ tpts,
                                                 const double *__restrict__ spts, int lane,
                                                 double dx, double dy, double dz, double acc[36]) {
@@ -220,7 +230,10 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
         const double iR = rsqrt(R2);
         const double R = R2 * iR;
         double Er, Ei;
-        cis_exp(-ki * R, kr * R, Er, Ei);
+        if constexpr (LL)
+          cis(kr * R, Er, Ei);
+        else
+          cis_exp(-ki * R, kr * R, Er, Ei);
         if constexpr (DK) {
           // dG/dk = i e^{ikR}/4pi, dg/dk = k G
           const double e = iR * kOofpi;
@@ -248,7 +261,10 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
         const double iR = rsqrt(R2);
         const double R = R2 * iR;
         double Er, Ei;
-        cis_exp(-ki * R, kr * R, Er, Ei);
+        if constexpr (LL)
+          cis(kr * R, Er, Ei);
+        else
+          cis_exp(-ki * R, kr * R, Er, Ei);
         Gr = (-Ei + kr * R) * kOofpi;
         Gi = (Er + ki * R) * kOofpi;
         const double f = iR * kOofpi;
@@ -258,7 +274,10 @@ __device__ __forceinline__ void pair_point_loop(const FarArgs &a, const double *
         const double iR = rsqrt(R2);
         const double R = R2 * iR;
         double Er, Ei;
-        cis_exp(-ki * R, kr * R, Er, Ei);
+        if constexpr (LL)
+          cis(kr * R, Er, Ei);
+        else
+          cis_exp(-ki * R, kr * R, Er, Ei);
         // Gs = [(e^{ikR} - 1)/R + k^2 R/2] / 4pi
         Gr = ((Er - 1.0) * iR + 0.5 * k2r * R) * kOofpi;
         Gi = (Ei * iR + 0.5 * k2i * R) * kOofpi;
@@ -332,7 +351,7 @@ struct PlaneArgs {
 // SPW source faces per wave (wave-uniform); the fill's autotuner picks SPW
 // among 2, 4 and 8 per context.  The per-pair arithmetic does not depend on
 // SPW, so every choice gives the same bits.
-template <int QT, bool DK = false, int SPW = 4>
+template <int QT, bool DK = false, int SPW = 4, bool LL = false>
 __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
   constexpr int kSrcPerWave = SPW, kSrcPerBlock = kWaves * SPW;
   const FarArgs &a = pa.f;
@@ -409,9 +428,9 @@ __global__ __launch_bounds__(256, 2) void pair_plane_kernel(PlaneArgs pa) {
     double acc[36];
     for (int k = 0; k < 36; k++) acc[k] = 0.0;
     if (near)
-      pair_point_loop<true, QT, DK>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
+      pair_point_loop<true, QT, DK, LL>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
     else
-      pair_point_loop<false, QT, DK>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
+      pair_point_loop<false, QT, DK, LL>(a, spts[tl], tpts, lane, -dx, -dy, -dz, acc);
     if (!valid) continue;
 
     const double AA = a.area[mm] * a.area[n];
@@ -1154,6 +1173,9 @@ struct FillTimer {
   ~FillTimer() { harvest(); }
 };
 
+// test hook: the lossless variant of the pair kernel (mom_test_fill_lossless)
+static int g_fill_lossless = 1;
+
 int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double omega, cd ri,
                 const FillTarget &t, const Image *img) {
   int rc = compute_quad_points(ctx, m, rule);
@@ -1251,13 +1273,25 @@ int fill_domain(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, double om
     auto grid_of = [&](int spw) {
       return dim3((unsigned)((m->F - pa.mlo + 63) / 64), (unsigned)((scur + kWaves * spw - 1) / (kWaves * spw)));
     };
+    // a lossless medium (the vacuum outside) skips the exponential of e^{ikR}
+    const bool ll = a.ki == 0.0 && g_fill_lossless;
     auto launch13 = [&](int spw) {
-      if (spw == 2)
-        hipLaunchKernelGGL((pair_plane_kernel<13, false, 2>), grid_of(2), dim3(256), 0, ctx->stream, pa);
-      else if (spw == 8)
-        hipLaunchKernelGGL((pair_plane_kernel<13, false, 8>), grid_of(8), dim3(256), 0, ctx->stream, pa);
-      else
-        hipLaunchKernelGGL((pair_plane_kernel<13, false, 4>), grid_of(4), dim3(256), 0, ctx->stream, pa);
+      if (spw == 2) {
+        if (ll)
+          hipLaunchKernelGGL((pair_plane_kernel<13, false, 2, true>), grid_of(2), dim3(256), 0, ctx->stream, pa);
+        else
+          hipLaunchKernelGGL((pair_plane_kernel<13, false, 2>), grid_of(2), dim3(256), 0, ctx->stream, pa);
+      } else if (spw == 8) {
+        if (ll)
+          hipLaunchKernelGGL((pair_plane_kernel<13, false, 8, true>), grid_of(8), dim3(256), 0, ctx->stream, pa);
+        else
+          hipLaunchKernelGGL((pair_plane_kernel<13, false, 8>), grid_of(8), dim3(256), 0, ctx->stream, pa);
+      } else {
+        if (ll)
+          hipLaunchKernelGGL((pair_plane_kernel<13, false, 4, true>), grid_of(4), dim3(256), 0, ctx->stream, pa);
+        else
+          hipLaunchKernelGGL((pair_plane_kernel<13, false, 4>), grid_of(4), dim3(256), 0, ctx->stream, pa);
+      }
     };
     if (t.dk) {
       hipLaunchKernelGGL((pair_plane_kernel<0, true>), grid_of(4), dim3(256), 0, ctx->stream, pa);
@@ -3240,3 +3274,9 @@ int vertex_adjoint_faces(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, 
 }
 
 }  // namespace mom
+
+extern "C" int mom_test_fill_lossless(int32_t enable) {
+  if (enable != 0 && enable != 1) return mom::fail(MOM_ERR_ARG, "lossless switch must be 0 or 1");
+  mom::g_fill_lossless = enable;
+  return 0;
+}

@@ -89,6 +89,11 @@ int mom_test_tournament(int32_t enable);
  * captures [0] and replays [1] so far. */
 int mom_test_factor_graph(int32_t enable, int64_t *counts);
 
+/* The fill's pair kernel in a lossless medium (Im k = 0) skips the
+ * exponential of e^{ikR}, which is exactly 1 there: 1 (default) takes that
+ * variant, 0 the general one (the same bits, checked by a test). */
+int mom_test_fill_lossless(int32_t enable);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,3 +47,32 @@ def test_fill_sources_and_solve_are_bitwise_reproducible(gpu_ctx, msh_dir, name)
         assert same, lab
     for d in dms:
         d.close()
+
+
+def test_lossless_pair_kernel_gives_the_same_bits(gpu_ctx, msh_dir):
+    """The exterior domain (vacuum, Im k = 0) fills with the pair kernel's
+    lossless variant, which skips the exponential of e^{ikR} (exactly 1
+    there): the matrix is bitwise the one of the general kernel."""
+    from mom2amd import capi
+    case = sphere(msh_dir, 3)
+    dms = [DeviceMesh(gpu_ctx, d) for d in case["problem"].domains]
+    N = case["problem"].nedges
+    L = capi.lib()
+    Z = []
+    try:
+        for enable in (1, 0):
+            capi.check(L.mom_test_fill_lossless(enable))
+            S = System(gpu_ctx, N, MOM_PREC_MIXED)
+            try:
+                S.fill(dms, case["ri"], case["omega"])
+                Z.append(S.get_matrix())
+            finally:
+                S.close()
+    finally:
+        capi.check(L.mom_test_fill_lossless(1))
+        for d in dms:
+            d.close()
+    assert case["ri"][0].imag == 0.0
+    same = np.array_equal(Z[0], Z[1])
+    print("lossless variant vs general kernel: bitwise equal %s" % same, flush=True)
+    assert same
