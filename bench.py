@@ -536,35 +536,46 @@ def bench_adjoint(args):
 
 
 def bench_symmetric(args):
-    """Mirror symmetry: the sphere (R = 50 nm, eps_in = -9.4+1.5i, 600 nm)
-    meshed as an octasphere of 8 n^2 faces, n = 50 (2E = 60000 unknowns for
-    the closed surface), solved on its octant cell with the group mxp myp
-    mzp: 8 representations of about 7500 unknowns each.  One step = the
-    fill of all representations (each image integrated once), their LU
-    factorisations, the plane-wave sources projected on the representations,
-    the solves and cs_prtsrf over the images, in the mixed path.  The closed
-    octasphere solved whole (mixed path, once, not in value) gives the
-    comparison time and the sigma agreement."""
+    """Symmetry decomposition, --group octant (default): the sphere (R = 50
+    nm, eps_in = -9.4+1.5i, 600 nm) meshed as an octasphere of 8 n^2 faces,
+    n = 50 (2E = 60000 unknowns for the closed surface), solved on its
+    octant cell with the group mxp myp mzp: 8 representations of about 7500
+    unknowns each.  --group ring: four such spheres (n = 25, 5000 faces
+    each) centred 120 nm from the z axis at 0, 90, 180 and 270 degrees,
+    solved on the upper half of one (2500 faces) with the group r4 mzp: 8
+    representations with complex characters, 60000 unknowns for the whole
+    ring.  One step = the fill of all representations (each image
+    integrated once), their LU factorisations, the plane-wave sources
+    projected on the representations, the solves and cs_prtsrf over the
+    images, in the mixed path.  The whole structure solved without symmetry
+    (mixed path, once, not in value; each sphere of the ring its own
+    interior domain) gives the comparison time and the sigma agreement."""
     from mom2amd import (Context, DeviceMesh, DEGTORAD, SymmetricSystems, System, build_problem, cs_prtsrf,
-                         cs_prtsrf_symm, load_problem, meshgen, omega_of, srcvec_symm)
+                         cs_prtsrf_symm, load_problem, meshgen, omega_of, srcvec_symm, symm_group)
     from mom2amd.capi import MOM_PREC_MIXED
 
-    n = args.cell_n
-    nodes, faces, lines, lids = meshgen.sphere_cell(n, 50.0, ())
+    ring = args.group == "ring"
+    n = args.cell_n if args.cell_n > 0 else (25 if ring else 50)
+    radius, shift = 50.0, (120.0 if ring else 0.0)
+    nodes, faces, lines, lids = meshgen.sphere_cell(n, radius, (0, 1) if ring else ())
+    nodes = nodes + np.array([shift, 0.0, 0.0])
     d = tempfile.mkdtemp(prefix="mom_bench_")
-    path = os.path.join(d, "octant%d.msh" % n)
+    path = os.path.join(d, "cell%d.msh" % n)
     meshgen.write_gmsh(path, nodes, faces, 15, lines, lids)
-    sbnd = [(meshgen.OCTANT_LINE_IDS[c], c) for c in ("xplane", "yplane", "zplane")]
+    planes = ("zplane",) if ring else ("xplane", "yplane", "zplane")
+    sbnd = [(meshgen.OCTANT_LINE_IDS[c], c) for c in planes]
     prob = load_problem(path, 1e-9, [[15], [-15]], sbnd=sbnd)
     N = prob.nedges
     omega = omega_of(600e-9)
     ri = [1.0 + 0j, np.sqrt(-9.4 + 1.5j)]
     ang = np.array([40.0, 25.0, 30.0]) * DEGTORAD
-    gens = ["mxp", "myp", "mzp"]
+    gens = ["r4", "mzp"] if ring else ["mxp", "myp", "mzp"]
+    nwhole = (4 if ring else 1) * 24 * n * n
     ctx = Context(0)
     dms = [DeviceMesh(ctx, dm) for dm in prob.domains]
-    log("octant cell n=%d: %d faces, %d edges (%d on the planes); closed sphere %d faces, %d unknowns"
-        % (n, prob.master.nfaces, N, int((prob.master.bnd > 1).sum()), 8 * n * n, 24 * n * n))
+    log("%s cell n=%d: %d faces, %d edges (%d on the planes), group %s; whole structure %d faces, %d unknowns"
+        % ("ring (upper half sphere)" if ring else "octant", n, prob.master.nfaces, N,
+           int((prob.master.bnd > 1).sum()), " ".join(gens), nwhole // 3, nwhole))
 
     def step():
         t0 = time.perf_counter()
@@ -593,24 +604,44 @@ def bench_symmetric(args):
         log("step %d: %.3f s (fill %.3f, factor %.3f, sources+solve+cs %.3f); csca %.9e cabs %.9e, berr %.1e"
             % (i, r["wall"], r["fill"], r["factor"], r["solve"], r["cs"][0], r["cs"][1], r["berr"]))
     mean = float(np.mean([r["wall"] for r in runs]))
-    out = {"metric": "fill+LU+solve s, octant of a %d-unknown sphere by mirror symmetry (mixed path)" % (24 * n * n),
+    if ring:
+        metric = "fill+LU+solve s, ring of four spheres (%d unknowns) by r4 x mzp symmetry (mixed path)" % nwhole
+        wl = ("four spheres R=50nm (octaspheres n=%d, %d faces each) centred 120 nm from the z axis at 0/90/180/270 "
+              "deg, upper half cell %d faces, group r4 mzp (8 representations of %s unknowns), 600 nm, "
+              "eps_in=-9.4+1.5i" % (n, 8 * n * n, prob.master.nfaces, runs[-1]["dims"]))
+    else:
+        metric = "fill+LU+solve s, octant of a %d-unknown sphere by mirror symmetry (mixed path)" % nwhole
+        wl = ("sphere R=50nm as octasphere n=%d (%d faces), octant cell %d faces, group mxp myp mzp (8 "
+              "representations of %s unknowns), 600 nm, eps_in=-9.4+1.5i"
+              % (n, 8 * n * n, prob.master.nfaces, runs[-1]["dims"]))
+    out = {"metric": metric,
            "value": round(mean, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(mean * 1e3, 2), "higher_is_better": False, "scaling": "strong",
            "vs_baseline": None, "dtype": "fp32-mfma LU + fp64 fill/residual", "data": "synthetic octasphere",
-           "config": {"workload": "sphere R=50nm as octasphere n=%d (%d faces), octant cell %d faces, group mxp myp "
-                                  "mzp (8 representations of %s unknowns), 600 nm, eps_in=-9.4+1.5i"
-                                  % (n, 8 * n * n, prob.master.nfaces, runs[-1]["dims"]),
-                      "n_unknowns_closed": 24 * n * n},
+           "config": {"workload": wl, "group": " ".join(gens), "n_unknowns_closed": nwhole},
            "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fill", "factor", "solve")},
            "csca_m2": runs[-1]["cs"][0], "cabs_m2": runs[-1]["cs"][1]}
     if not args.no_fp64:
-        fn, ff = meshgen.octasphere(n, 50.0)
-        P = build_problem(fn, ff, np.full(ff.shape[0], 15), 1e-9, [[15], [-15]])
+        if ring:
+            # the four spheres, each its own interior domain (the decomposition takes the rotation images in the
+            # exterior domain alone)
+            sn, sf, _, _ = meshgen.sphere_cell(n, radius, (0, 1, 2))
+            sn = sn + np.array([shift, 0.0, 0.0])
+            Js = symm_group(["r4"])["J"]
+            fn = np.concatenate([sn @ J.T for J in Js])
+            ff = np.concatenate([sf + k * sn.shape[0] for k in range(4)])
+            fid = np.repeat(15 + np.arange(4), sf.shape[0])
+            P = build_problem(fn, ff, fid, 1e-9, [[15, 16, 17, 18], [-15], [-16], [-17], [-18]])
+            riw = [ri[0]] + [ri[1]] * 4
+        else:
+            fn, ff = meshgen.octasphere(n, 50.0)
+            P = build_problem(fn, ff, np.full(ff.shape[0], 15), 1e-9, [[15], [-15]])
+            riw = ri
         fdms = [DeviceMesh(ctx, dm) for dm in P.domains]
         t0 = time.perf_counter()
         S = System(ctx, P.nedges, MOM_PREC_MIXED)
         try:
-            S.fill(fdms, ri, omega)
+            S.fill(fdms, riw, omega)
             S.factor()
             xf = S.solve_pw(fdms[0], omega, ri[0], ang)[:, 0]
             csf = cs_prtsrf(ctx, fdms[0], P.nedges, omega, ri[0], xf, *ang)
@@ -622,8 +653,9 @@ def bench_symmetric(args):
         cs = runs[-1]["cs"]
         out["closed_sphere"] = {"seconds": round(tf, 4), "n_unknowns": 2 * P.nedges, "speedup": round(tf / mean, 2),
                                 "csca_relerr": abs(cs[0] / csf[0] - 1), "cabs_relerr": abs(cs[1] / csf[1] - 1)}
-        log("closed octasphere (N=%d, mixed path): %.3f s, %.1fx the symmetric solve; sigma rel diff %.1e / %.1e"
-            % (2 * P.nedges, tf, tf / mean, out["closed_sphere"]["csca_relerr"], out["closed_sphere"]["cabs_relerr"]))
+        log("whole %s (N=%d, mixed path): %.3f s, %.1fx the symmetric solve; sigma rel diff %.1e / %.1e"
+            % ("ring" if ring else "octasphere", 2 * P.nedges, tf, tf / mean, out["closed_sphere"]["csca_relerr"],
+               out["closed_sphere"]["cabs_relerr"]))
     ctx.close()
     print(json.dumps(out), flush=True)
 
@@ -713,7 +745,11 @@ def main():
                          "permittivity gradient on a perturbed sphere (sh, spectrum and adjoint use s=4 unless "
                          "--subdiv is given); symmetric: the octant of an octasphere by mirror symmetry; fill: the "
                          "complex128 PMCHWT fill of the default sphere alone (for counter runs)")
-    ap.add_argument("--cell-n", type=int, default=50, help="octasphere divisions of the symmetric workload")
+    ap.add_argument("--cell-n", type=int, default=0,
+                    help="octasphere divisions of the symmetric workload (default 50 for octant, 25 for ring)")
+    ap.add_argument("--group", choices=("octant", "ring"), default="octant",
+                    help="symmetric workload: the octant of one sphere (mxp myp mzp) or a ring of four spheres "
+                         "(r4 mzp)")
     ap.add_argument("--nwl", type=int, default=64, help="wavelengths of the spectrum workload")
     ap.add_argument("--lanes", type=int, default=4, help="wavelengths solved at once in the spectrum workload")
     ap.add_argument("--pmchwt-pivoting", choices=("partial", "tournament"), default="tournament",
