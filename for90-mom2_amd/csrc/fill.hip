@@ -732,8 +732,20 @@ struct NearFaceT {
 };
 using NearFace = NearFaceT<double>;
 
-template <class T>
-__device__ inline T dot3(const T *a, const T *b) {
+// the type of a value computed from a TA and a TB (D1 if either is)
+template <class TA, class TB>
+struct Promote {
+  using type = D1;
+};
+template <>
+struct Promote<double, double> {
+  using type = double;
+};
+template <class TA, class TB>
+using PromoteT = typename Promote<TA, TB>::type;
+
+template <class TA, class TB>
+__device__ inline PromoteT<TA, TB> dot3(const TA *a, const TB *b) {
 #pragma clang fp contract(off)
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
@@ -742,19 +754,23 @@ __device__ inline T norm3(const T *a) {
 #pragma clang fp contract(off)
   return tsqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
 }
-template <class T>
-__device__ inline void cross3(const T *a, const T *b, T *r) {
+template <class TA, class TB, class TR>
+__device__ inline void cross3(const TA *a, const TB *b, TR *r) {
 #pragma clang fp contract(off)
   r[0] = a[1] * b[2] - a[2] * b[1];
   r[1] = a[2] * b[0] - a[0] * b[2];
   r[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// The closed forms take the point r (TP) and the face (TF) separately, so
+// that a derivative in one of them leaves the other in plain doubles.
+
 // int.f90:15-53 intLm1 for side e (nodes e, e+1)
-template <class T>
-__device__ T d_intLm1(const T *r, const NearFaceT<T> &f, int e) {
+template <class TP, class TF>
+__device__ PromoteT<TP, TF> d_intLm1(const TP *r, const NearFaceT<TF> &f, int e) {
 #pragma clang fp contract(off)
-  const T *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
+  using T = PromoteT<TP, TF>;
+  const TF *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
   T d2[3] = {r[0] - p2[0], r[1] - p2[1], r[2] - p2[2]};
   T d1[3] = {r[0] - p1[0], r[1] - p1[1], r[2] - p1[2]};
   T Rp = norm3(d2), Rm = norm3(d1);
@@ -767,11 +783,12 @@ __device__ T d_intLm1(const T *r, const NearFaceT<T> &f, int e) {
 }
 
 // int.f90:55-135 intL1 and intL3 of side e (they share R0^2, s+-, R+-)
-template <class T>
-__device__ void d_intL13(const T *r, const NearFaceT<T> &f, int e, T lm1, T *L1, T *L3) {
+template <class TP, class TF, class T>
+__device__ void d_intL13(const TP *r, const NearFaceT<TF> &f, int e, T lm1, T *L1, T *L3) {
 #pragma clang fp contract(off)
-  const T *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
-  T p[3], rp[3];
+  const TF *p1 = f.p[e], *p2 = f.p[(e + 1) % 3], *s = f.s[e];
+  TF p[3];
+  T rp[3];
   for (int k = 0; k < 3; k++) p[k] = 0.5 * (p1[k] + p2[k]);
   for (int k = 0; k < 3; k++) rp[k] = r[k] - p[k];
   T h = dot3(f.n, rp), t = dot3(f.m[e], rp);
@@ -792,9 +809,10 @@ __device__ void d_intL13(const T *r, const NearFaceT<T> &f, int e, T lm1, T *L1,
 }
 
 // int.f90:137-162 intSm3h (signed solid angle)
-template <class T>
-__device__ T d_intSm3h(const T *r, const NearFaceT<T> &f) {
+template <class TP, class TF>
+__device__ PromoteT<TP, TF> d_intSm3h(const TP *r, const NearFaceT<TF> &f) {
 #pragma clang fp contract(off)
+  using T = PromoteT<TP, TF>;
   T a[3][3];
   for (int v = 0; v < 3; v++) {
     T d[3] = {f.p[v][0] - r[0], f.p[v][1] - r[1], f.p[v][2] - r[2]};
@@ -818,9 +836,10 @@ struct PointIntsT {
 };
 using PointInts = PointIntsT<double>;
 
-template <class T>
-__device__ void d_point_ints(const T *r, const NearFaceT<T> &f, PointIntsT<T> &o) {
+template <class TP, class TF>
+__device__ void d_point_ints(const TP *r, const NearFaceT<TF> &f, PointIntsT<PromoteT<TP, TF>> &o) {
 #pragma clang fp contract(off)
+  using T = PromoteT<TP, TF>;
   T t[3];
   for (int e = 0; e < 3; e++) {
     T d[3] = {r[0] - f.p[e][0], r[1] - f.p[e][1], r[2] - f.p[e][2]};
@@ -2305,8 +2324,9 @@ __device__ __forceinline__ double2 cm(double2 a, double2 b) {
 
 // Sum over the analytic near parts of test face ft against source face fs
 // (near_fill_kernel's terms) contracted with a[3][3], b[3][3]: per test
-// side p and source side q, a (c1 D + c2 H) + b K.  T = double for the
-// values, D1 for their derivative in one node coordinate.
+// side p and source side q, a (c1 D + c2 H) + b K.  TT and TS, the scalar
+// types of the test and the source face: double for the values, D1 in the
+// face whose node coordinate the derivative is taken in.
 template <class T>
 struct CT {
   T x, y;
@@ -2316,17 +2336,20 @@ __device__ __forceinline__ CT<T> cmt(double2 a, CT<T> b) {
   return CT<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
-template <class T>
-__device__ CT<T> vnear_part_t(const VPairArgs &g, const VFaceT<T> &ft, const VFaceT<T> &fs, bool self,
-                              const double2 a[3][3], const double2 b[3][3]) {
-  NearFaceT<T> f;
+template <class TT, class TS>
+__device__ CT<PromoteT<TT, TS>> vnear_part_t(const VPairArgs &g, const VFaceT<TT> &ft, const VFaceT<TS> &fs,
+                                             bool self, const double2 a[3][3], const double2 b[3][3]) {
+  using T = PromoteT<TT, TS>;
+  NearFaceT<TS> f;
   vface_near(fs, f);
-  T At, ct[3], dt[3], vt[3][3], As, cs[3], ds[3], vs[3][3];
+  TT At, ct[3], dt[3], vt[3][3];
+  TS As, cs[3], ds[3], vs[3][3];
   vface_rwg(ft, At, ct, dt, vt);
   vface_rwg(fs, As, cs, ds, vs);
   CT<T> acc{T(0.0), T(0.0)};
+#pragma unroll 1
   for (int i = 0; i < g.rule.Q; i++) {
-    T r[3];
+    TT r[3];
     vface_point(ft, g.rule, i, r);
     PointIntsT<T> P;
     d_point_ints(r, f, P);
@@ -2360,7 +2383,7 @@ __device__ CT<T> vnear_part_t(const VPairArgs &g, const VFaceT<T> &ft, const VFa
         }
       }
       for (int p = 0; p < 3; p++) {
-        T fm[3];
+        TT fm[3];
         for (int c = 0; c < 3; c++) fm[c] = ct[p] * (r[c] - vt[p][c]);
         CT<T> dd{T(0.0), T(0.0)}, kk{T(0.0), T(0.0)};
         for (int c = 0; c < 3; c++) {
@@ -2982,60 +3005,72 @@ __global__ __launch_bounds__(128) void vertex_near_kernel(VPairArgs g) {
   }
 }
 
-// The int.f90 analytic parts of the near pairs, differentiated exactly: one
-// thread per (entry m <= n of the near list, distinct node k of the pair,
-// coordinate c), forward mode (D1) through both faces' geometry, the closed
-// forms and the contraction with the coefficients.  The derivative goes to
-// the thread's own slot of the entry (face m's local node if the node is
-// m's, else face n's), added to what vertex_near_kernel wrote there.
-__global__ __launch_bounds__(256) void vertex_near_analytic_kernel(VPairArgs g) {
+// The int.f90 analytic parts of the near pairs, differentiated exactly:
+// forward mode (D1) through the faces' geometry, the closed forms and the
+// contraction with the coefficients, one thread per (entry m <= n of the
+// near list, local vertex v of a face, coordinate c).  Three launches, each
+// with one code path for all its threads: ROLE 0 the nodes of face m that
+// face n does not hold (face n stays in plain doubles), ROLE 1 those of
+// face n that face m does not hold, ROLE 2 the nodes of face m that face n
+// holds too (every node of a self pair; both faces in D1).  The derivative
+// goes to the thread's own slot of the entry (face m's local node v for
+// ROLE 0 and 2, face n's for ROLE 1), added to what vertex_near_kernel
+// wrote there.
+__device__ __forceinline__ void vseed(const VFace &f0, int node, int c, const int *fnodes, VFaceT<D1> &f) {
+  for (int v = 0; v < 3; v++) {
+    const bool on = fnodes[v] == node;
+    for (int j = 0; j < 3; j++) f.P[v][j] = D1(f0.P[v][j], (on && j == c) ? 1.0 : 0.0);
+    f.sg[v] = f0.sg[v];
+    f.lv[v] = f0.lv[v];
+  }
+}
+
+template <int ROLE>
+__global__ __launch_bounds__(128) void vertex_near_analytic_kernel(VPairArgs g) {
+  // the coefficients live in LDS (each thread its own), not in registers
+  __shared__ double2 sab[128][2][3][3];
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t id = t / 18;
+  const int64_t id = t / 9;
   if (id >= g.npairs) return;
-  const int sub = (int)(t - 18 * id), k = sub / 3, c = sub - 3 * k;
+  const int sub = (int)(t - 9 * id), v = sub / 3, c = sub - 3 * v;
   const int m = g.pairs[2 * id], n = g.pairs[2 * id + 1];
   if (n < m) return;
-  const bool self = (m == n);
-  // distinct nodes: face m's three, then face n's others in order
-  const int m0 = g.fnodes[3 * m], m1 = g.fnodes[3 * m + 1], m2 = g.fnodes[3 * m + 2];
-  int node = k == 0 ? m0 : (k == 1 ? m1 : (k == 2 ? m2 : -1));
-  for (int v = 0, cnt = 3; v < 3; v++) {
-    const int nv = g.fnodes[3 * n + v];
-    if (nv == m0 || nv == m1 || nv == m2) continue;
-    if (cnt == k) node = nv;
-    cnt++;
-  }
-  if (node < 0) return;
+  const int own = ROLE == 1 ? n : m, other = ROLE == 1 ? m : n;
+  const int node = g.fnodes[3 * own + v];
+  const bool shared = g.fnodes[3 * other] == node || g.fnodes[3 * other + 1] == node || g.fnodes[3 * other + 2] == node;
+  if (shared != (ROLE == 2)) return;
   VFace fm0, fn0;
   vload_face(g, m, fm0);
   vload_face(g, n, fn0);
-  VFaceT<D1> fm, fn;
-  int vm = -1, vn = -1;
-  for (int v = 0; v < 3; v++) {
-    const bool om = g.fnodes[3 * m + v] == node, on = g.fnodes[3 * n + v] == node;
-    if (om) vm = v;
-    if (on) vn = v;
-    for (int j = 0; j < 3; j++) {
-      fm.P[v][j] = D1(fm0.P[v][j], (om && j == c) ? 1.0 : 0.0);
-      fn.P[v][j] = D1(fn0.P[v][j], (on && j == c) ? 1.0 : 0.0);
+  double2(*a)[3] = sab[threadIdx.x][0], (*b)[3] = sab[threadIdx.x][1];
+  double dv;
+  if (ROLE == 2) {
+    const bool self = (m == n);
+    VFaceT<D1> fm, fn;
+    vseed(fm0, node, c, g.fnodes + 3 * m, fm);
+    vseed(fn0, node, c, g.fnodes + 3 * n, fn);
+    vcoef(g, m, n, a, b);
+    dv = vnear_part_t(g, fm, fn, self, a, b).x.d;
+    if (!self) {
+      vcoef(g, n, m, a, b);
+      dv += vnear_part_t(g, fn, fm, false, a, b).x.d;
     }
-    fm.sg[v] = fm0.sg[v];
-    fm.lv[v] = fm0.lv[v];
-    fn.sg[v] = fn0.sg[v];
-    fn.lv[v] = fn0.lv[v];
-  }
-  double2 a[3][3], b[3][3];
-  vcoef(g, m, n, a, b);
-  double dv = vnear_part_t(g, fm, fn, self, a, b).x.d;
-  if (!self) {
+  } else if (ROLE == 0) {
+    VFaceT<D1> fm;
+    vseed(fm0, node, c, g.fnodes + 3 * m, fm);
+    vcoef(g, m, n, a, b);
+    dv = vnear_part_t(g, fm, fn0, false, a, b).x.d;
     vcoef(g, n, m, a, b);
-    dv += vnear_part_t(g, fn, fm, false, a, b).x.d;
+    dv += vnear_part_t(g, fn0, fm, false, a, b).x.d;
+  } else {
+    VFaceT<D1> fn;
+    vseed(fn0, node, c, g.fnodes + 3 * n, fn);
+    vcoef(g, m, n, a, b);
+    dv = vnear_part_t(g, fm0, fn, false, a, b).x.d;
+    vcoef(g, n, m, a, b);
+    dv += vnear_part_t(g, fn, fm0, false, a, b).x.d;
   }
-  double *o = g.near_out + 18 * id;
-  if (vm >= 0)
-    o[3 * vm + c] += dv;
-  else
-    o[9 + 3 * vn + c] += dv;
+  g.near_out[18 * id + (ROLE == 1 ? 9 : 0) + 3 * v + c] += dv;
 }
 
 // Per face: its tile partial sums (in tile order) and its near pairs (in
@@ -3233,8 +3268,10 @@ int vertex_adjoint_fill(mom_ctx *ctx, const mom_mesh *m, const QuadRule &rule, d
   if (g.near_out) {
     hipLaunchKernelGGL(vertex_near_kernel, dim3((unsigned)((m->nnear + 127) / 128)), dim3(128), 0, s, g);
     MOM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(vertex_near_analytic_kernel, dim3((unsigned)((18 * m->nnear + 255) / 256)), dim3(256), 0, s,
-                       g);
+    const dim3 ga((unsigned)((9 * m->nnear + 127) / 128));
+    hipLaunchKernelGGL(vertex_near_analytic_kernel<0>, ga, dim3(128), 0, s, g);
+    hipLaunchKernelGGL(vertex_near_analytic_kernel<1>, ga, dim3(128), 0, s, g);
+    hipLaunchKernelGGL(vertex_near_analytic_kernel<2>, ga, dim3(128), 0, s, g);
     MOM_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(vertex_face_sum_kernel, dim3((unsigned)((F + 127) / 128)), dim3(128), 0, s, g, fg);
