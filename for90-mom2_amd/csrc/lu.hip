@@ -1810,8 +1810,9 @@ __global__ __launch_bounds__(kTsluThreads) void tslu_select_kernel(const float2 
                                                                    float2 *__restrict__ lu_top,
                                                                    int *__restrict__ info, int *__restrict__ ipiv,
                                                                    int *__restrict__ plan) {
-  __shared__ float pmag[2][kTsluWaves];
-  __shared__ int prw[2][kTsluWaves];
+  static_assert(kTsluWaves == 4, "the best of the waves' candidates is read as one float4 and one int4");
+  __shared__ __align__(16) float pmag[2][kTsluWaves];
+  __shared__ __align__(16) int prw[2][kTsluWaves];
   __shared__ float2 prow[2][kTsluWaves][IB];
   __shared__ int pv[NB], first[NB], idx[2 * NB];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1881,18 +1882,20 @@ __global__ __launch_bounds__(kTsluThreads) void tslu_select_kernel(const float2 
       prw[par][wv] = wr;
     }
     __syncthreads();
-    float bm = pmag[par][0];
-    int br = prw[par][0], bw = 0;
-#pragma unroll
-    for (int q = 1; q < kTsluWaves; q++) {
-      const float m = pmag[par][q];
-      const int r = prw[par][q];
-      if (m > bm || (m == bm && r < br)) {
-        bm = m;
-        br = r;
-        bw = q;
-      }
-    }
+    // the four waves' entries in two LDS reads, the best by selects
+    const float4 pm4 = *reinterpret_cast<const float4 *>(&pmag[par][0]);
+    const int4 pr4 = *reinterpret_cast<const int4 *>(&prw[par][0]);
+    float bm = pm4.x;
+    int br = pr4.x, bw = 0;
+    auto take = [&](float m, int r, int q) {
+      const bool better = m > bm || (m == bm && r < br);
+      bm = better ? m : bm;
+      br = better ? r : br;
+      bw = better ? q : bw;
+    };
+    take(pm4.y, pr4.y, 1);
+    take(pm4.z, pr4.z, 2);
+    take(pm4.w, pr4.w, 3);
     // bm < 0: no rows left in this group (uniform); every row then has
     // multiplier 0 below
     if (bm < 0.f && tid == 0) out[grp * IB + k] = -1;
@@ -1910,7 +1913,9 @@ __global__ __launch_bounds__(kTsluThreads) void tslu_select_kernel(const float2 
     }
     const float2 p = prow[par][bw][k];
     const float den = p.x * p.x + p.y * p.y;
-    const float inv = den != 0.f ? 1.f / den : 0.f;
+    // the hardware reciprocal (1 ulp): the factors only precondition the
+    // complex128 refinement
+    const float inv = den != 0.f ? __builtin_amdgcn_rcpf(den) : 0.f;
     float2 l[kTsluRpt];
 #pragma unroll
     for (int h = 0; h < kTsluRpt; h++) {
@@ -1920,14 +1925,16 @@ __global__ __launch_bounds__(kTsluThreads) void tslu_select_kernel(const float2 
       l[h] = act[h] ? q : make_float2(0.f, 0.f);
       v[h][k] = act[h] ? q : a;
     }
+    // v -= l u by four FMAs per entry
 #pragma unroll
     for (int s = 0; s < IB; s++) {
       if (s <= k) continue;
       const float2 u = prow[par][bw][s];
 #pragma unroll
       for (int h = 0; h < kTsluRpt; h++) {
-        v[h][s].x -= l[h].x * u.x - l[h].y * u.y;
-        v[h][s].y -= l[h].x * u.y + l[h].y * u.x;
+        const float nlx = -l[h].x, nly = -l[h].y;
+        v[h][s].x = fmaf(l[h].y, u.y, fmaf(nlx, u.x, v[h][s].x));
+        v[h][s].y = fmaf(nly, u.x, fmaf(nlx, u.y, v[h][s].y));
       }
     }
   }
