@@ -165,6 +165,16 @@ int symm_group(int ngen, const int32_t *gens, Group *out) {
     if (g.n * f.n > kMaxReps) return fail(MOM_ERR_ARG, "the group has more than 8 elements");
     g = product(g, f);
   }
+  // The decomposition uses one-dimensional representations, so the group
+  // must be abelian: a rotation about z by other than pi does not commute
+  // with the mirrors of x = 0 and y = 0 (with them it generates a dihedral
+  // group), only with the mirror of z = 0.
+  for (int i = 0; i < ngen; i++)
+    if (gens[i] >= kGenRz + 3)
+      for (int j = 0; j < ngen; j++)
+        if (gens[j] == 1 || gens[j] == 2)
+          return fail(MOM_ERR_ARG, "a rotation about z by 2 pi / n, n > 2, combines only with mzp (with mxp or myp "
+                                   "the group is not abelian)");
   *out = g;
   return 0;
 }

@@ -454,7 +454,8 @@ int mom_problem_mesh_create(mom_ctx *ctx, const mom_problem *prob, int32_t dom, 
  * the line elements that the mesh file puts on them.  planes[ngen] (the
  * generators, up to 4) lists them in the order of the `symm` line: 1 = mxp,
  * 2 = myp, 3 = mzp, 100 + n = rn (2 <= n <= 8, group_rz, symmetry.f90:
- * 105-156).  A rotation group is for bodies that no rotation cut crosses (n
+ * 105-156); rn with n > 2 combines only with mzp (with mxp or myp the group
+ * is not abelian, MOM_ERR_ARG).  A rotation group is for bodies that no rotation cut crosses (n
  * copies of a particle around the z axis): the boundary classes of rotation
  * cuts (rz1, rz2) and of periodic cells are not supported (MOM_ERR_ARG), as
  * the reference sets no edge coupling for them (mesh.f90:195-258).

@@ -50,9 +50,11 @@ def test_group_rejects_bad_generators():
         symm_group(["mxp", "mxp"])
     with pytest.raises(MomError):
         symm_group([4])
-    for gens in (["r1"], ["r9"], ["r2", "r3"], ["r8", "mzp"], ["r3", "mxp", "myp"]):
+    for gens in (["r1"], ["r9"], ["r2", "r3"], ["r8", "mzp"], ["r3", "mxp", "myp"], ["r4", "mxp"], ["myp", "r3"]):
         with pytest.raises(MomError):
             symm_group(gens)
+    # r2 (rotation by pi, diagonal) commutes with every mirror
+    assert symm_group(["r2", "mxp"])["n"] == 4
 
 
 def _rz(a):
