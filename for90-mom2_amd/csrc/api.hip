@@ -263,6 +263,13 @@ int mom_ctx_fill_tuning(const mom_ctx *ctx, int32_t *spw, double *ms) {
   return 0;
 }
 
+int mom_test_fill_spw(mom_ctx *ctx, int32_t spw) {
+  if (!ctx) return fail(MOM_ERR_ARG, "NULL argument");
+  if (spw != 0 && spw != 2 && spw != 4 && spw != 8) return fail(MOM_ERR_ARG, "sources per wave must be 0, 2, 4 or 8");
+  ctx->fill_spw = spw;
+  return 0;
+}
+
 int mom_ctx_fill_profile(mom_ctx *ctx, int32_t enable, double *out) {
   if (!ctx) return fail(MOM_ERR_ARG, "NULL argument");
   if (out)

@@ -2309,14 +2309,24 @@ __device__ __forceinline__ void cmac(typename Cx<TC>::type &acc, typename Cx<TS>
   acc.y += mr * v.y + mi * v.x;
 }
 
-// y[r] = sum_c M[r, c] x[c] for the NB x NB diagonal-block inverse M; the
-// result overwrites x (in tmp) and is kept in xb for the panel update.
-// Grid (NB/8, nrhs): 8 rows per workgroup (128-byte row segments of M per
-// column), 32 column slices per row, so that 32 workgroups share the 1 MB
-// read of M.
+// b[c0 + i, j] = xb[i, j] for i < w: a solved block back into its rows
+template <typename CV>
+__global__ void copy_block_kernel(const CV *__restrict__ xb, CV *b, int64_t ldb, int64_t c0, int w) {
+  const int64_t rhs = blockIdx.y;
+  for (int i = threadIdx.x; i < w; i += blockDim.x) b[rhs * ldb + c0 + i] = xb[rhs * NB + i];
+}
+
+// y[r] = sum_c M[r, c] x[c] for the NB x NB diagonal-block inverse M, into
+// xb.  Grid (NB/8, nrhs): 8 rows per workgroup (128-byte row segments of M
+// per column), 32 column slices per row, so that 32 workgroups share the
+// 1 MB read of M.  Every workgroup reads the whole block x, so the result
+// must not go back into x in this launch (a workgroup that starts late
+// would read rows another one has already replaced): the next launch
+// (panel_gemv_sub_kernel / _h_kernel, or copy_block_kernel after the last
+// block) copies xb into the block's rows.
 template <typename TS, typename TC, bool H = false>
 __global__ __launch_bounds__(256) void block_apply_kernel(const double2 *__restrict__ M,
-                                                           typename Cx<TC>::type *x, int64_t ldx,
+                                                           const typename Cx<TC>::type *__restrict__ x, int64_t ldx,
                                                            typename Cx<TC>::type *__restrict__ xb, int w) {
   using CV = typename Cx<TC>::type;
   __shared__ CV xs[NB];
@@ -2344,7 +2354,6 @@ __global__ __launch_bounds__(256) void block_apply_kernel(const double2 *__restr
       s.x += part[k][rl].x;
       s.y += part[k][rl].y;
     }
-    x[rhs * ldx + r] = s;
     xb[rhs * NB + r] = s;
   }
 }
@@ -2360,7 +2369,12 @@ __global__ __launch_bounds__(256) void panel_gemv_sub_kernel(const typename Cx<T
   __shared__ CV xs[NB];
   __shared__ CV part[4][64];
   const int64_t rhs = blockIdx.y;
-  for (int i = threadIdx.x; i < w; i += 256) xs[i] = xb[rhs * NB + i];
+  for (int i = threadIdx.x; i < w; i += 256) {
+    const CV v = xb[rhs * NB + i];
+    xs[i] = v;
+    // the solved block into its own rows [c0, c0 + w), outside [r0, r1)
+    if (blockIdx.x == 0) b[rhs * ldb + c0 + i] = v;
+  }
   __syncthreads();
   const int rl = threadIdx.x & 63, sl = threadIdx.x >> 6;
   const int64_t r = r0 + blockIdx.x * 64 + rl;
@@ -2414,6 +2428,8 @@ static int getrs(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, int
     if (r0 + w < n)
       hipLaunchKernelGGL((panel_gemv_sub_kernel<TS, TC>), dim3((unsigned)((n - r0 - w + 63) / 64), (unsigned)nrhs),
                          dim3(256), 0, s, LU, ld, r0 + w, n, r0, w, xb, tmp, n);
+    else
+      hipLaunchKernelGGL(copy_block_kernel<CV>, dim3(1, (unsigned)nrhs), dim3(256), 0, s, xb, tmp, n, r0, w);
   }
   for (int64_t b = nblk - 1; b >= 0; b--) {
     const int64_t r0 = b * NB;
@@ -2423,6 +2439,8 @@ static int getrs(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, int
     if (r0 > 0)
       hipLaunchKernelGGL((panel_gemv_sub_kernel<TS, TC>), dim3((unsigned)((r0 + 63) / 64), (unsigned)nrhs), dim3(256),
                          0, s, LU, ld, (int64_t)0, r0, r0, w, xb, tmp, n);
+    else
+      hipLaunchKernelGGL(copy_block_kernel<CV>, dim3(1, (unsigned)nrhs), dim3(256), 0, s, xb, tmp, n, r0, w);
   }
   MOM_HIP(hipGetLastError());
   MOM_HIP(hipMemcpy2DAsync(B, ldb * sizeof(CV), tmp, n * sizeof(CV), n * sizeof(CV), nrhs, hipMemcpyDeviceToDevice,
@@ -2440,6 +2458,9 @@ __global__ __launch_bounds__(256) void panel_gemv_sub_h_kernel(const typename Cx
                                                                 typename Cx<TC>::type *b, int64_t ldb) {
   using CV = typename Cx<TC>::type;
   const int64_t rhs = blockIdx.y;
+  // the solved block into its own rows [c0, c0 + w), outside [r0, r1)
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < w; i += 256) b[rhs * ldb + c0 + i] = xb[rhs * NB + i];
   const int lane = threadIdx.x & 63;
   const int64_t r = r0 + blockIdx.x * 4 + (threadIdx.x >> 6);
   if (r >= r1) return;
@@ -2493,6 +2514,8 @@ static int getrs_h(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, i
     if (r0 + w < n)
       hipLaunchKernelGGL((panel_gemv_sub_h_kernel<TS, TC>), dim3((unsigned)((n - r0 - w + 3) / 4), (unsigned)nrhs),
                          dim3(256), 0, s, LU, ld, r0 + w, n, r0, w, xb, tmp, n);
+    else
+      hipLaunchKernelGGL(copy_block_kernel<CV>, dim3(1, (unsigned)nrhs), dim3(256), 0, s, xb, tmp, n, r0, w);
   }
   for (int64_t b = nblk - 1; b >= 0; b--) {
     const int64_t r0 = b * NB;
@@ -2502,6 +2525,8 @@ static int getrs_h(hipStream_t s, const typename Cx<TS>::type *LU, int64_t ld, i
     if (r0 > 0)
       hipLaunchKernelGGL((panel_gemv_sub_h_kernel<TS, TC>), dim3((unsigned)((r0 + 3) / 4), (unsigned)nrhs), dim3(256),
                          0, s, LU, ld, (int64_t)0, r0, r0, w, xb, tmp, n);
+    else
+      hipLaunchKernelGGL(copy_block_kernel<CV>, dim3(1, (unsigned)nrhs), dim3(256), 0, s, xb, tmp, n, r0, w);
   }
   hipLaunchKernelGGL(scatter_rows_kernel<CV>, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, tmp, n, B,
                      ldb, perm, n, nrhs);

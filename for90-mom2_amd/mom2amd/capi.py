@@ -37,7 +37,8 @@ EXPORTS = [
 # test hooks declared in include/mom_gpu_testing.h
 TEST_EXPORTS = ["mom_test_mfma_layout", "mom_test_gemm_sub", "mom_test_panel_mode", "mom_test_lookahead",
                 "mom_test_unit_lower_inverse", "mom_test_u_solve", "mom_test_vertex_fd", "mom_test_batch_fallback",
-                "mom_test_tournament", "mom_test_factor_graph", "mom_test_fill_lossless"]
+                "mom_test_tournament", "mom_test_factor_graph", "mom_test_fill_lossless",
+                "mom_test_fill_spw"]
 
 
 class MomError(RuntimeError):
@@ -99,6 +100,7 @@ def lib():
     L.mom_test_tournament.argtypes = [_i32]
     L.mom_test_factor_graph.argtypes = [_i32, C.POINTER(_i64)]
     L.mom_test_fill_lossless.argtypes = [_i32]
+    L.mom_test_fill_spw.argtypes = [C.c_void_p, _i32]
     L.mom_ctx_fill_tuning.argtypes = [_vp, C.POINTER(_i32), _dp]
     L.mom_ctx_fill_profile.argtypes = [_vp, _i32, _dp]
     L.mom_mesh_classify_like.argtypes = [_vp, _vp]

@@ -94,6 +94,11 @@ int mom_test_factor_graph(int32_t enable, int64_t *counts);
  * variant, 0 the general one (the same bits, checked by a test). */
 int mom_test_fill_lossless(int32_t enable);
 
+/* The fill's pair-kernel variant of a context: 2, 4 or 8 source faces per
+ * wave, or 0 to let the next fill's autotuner choose again.  Every variant
+ * gives the same bits (checked by a test). */
+int mom_test_fill_spw(mom_ctx *ctx, int32_t spw);
+
 #ifdef __cplusplus
 }
 #endif
