@@ -2128,8 +2128,13 @@ static int g_u_subst = 1;
 // U12 row block on the main stream; then the trailing update splits in two.
 // The next block column is updated on the look-ahead stream, which goes on
 // to factor panel k+1 there, while the main stream updates the remaining
-// columns; the main stream waits for panel k+1 before step k+1.  The two
-// streams touch disjoint columns between fork and join.
+// columns; the main stream waits for panel k+1 before step k+1.  The
+// look-ahead stream also forms panel k+1's interchange plan and applies it
+// to the columns left of block k (the main stream's update reads block k's
+// columns and writes those right of block k+1), so that the main stream's
+// chain before its next update holds only the interchanges of block k's
+// columns and of the right part.  The two streams touch disjoint columns
+// between fork and join.
 template <typename T>
 static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, int *ipiv, PivotWork w,
                  PhaseClock *st, PanelCoop coop = PanelCoop()) {
@@ -2161,22 +2166,40 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
     }
     return 0;
   };
-  if (la) MOM_TRY(panel(M, 0, std::min<int64_t>(NB, n)));
-  for (int64_t j0 = 0; j0 < n; j0 += NB) {
+  auto laswp = [&](hipStream_t ls, int64_t c0, int64_t c1, int64_t c2, int64_t c3) -> int {
+    const int64_t cols = (c1 - c0) + (c3 - c2);
+    if (cols <= 0) return 0;
+    const int64_t groups = (cols + kLaswpCols - 1) / kLaswpCols;
+    hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, ls, A, ld,
+                       w.plan, (int)c0, (int)c1, (int)c2, (int)c3);
+    MOM_HIP(hipGetLastError());
+    return 0;
+  };
+  if (la) {
+    const int64_t jb0 = std::min<int64_t>(NB, n);
+    MOM_TRY(panel(M, 0, jb0));
+    hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, 0, (int)jb0, (int)n, w.plan);
+    MOM_HIP(hipGetLastError());
+  }
+  for (int64_t j0 = 0, prev = 0; j0 < n; prev = std::min<int64_t>(NB, n - j0), j0 += NB) {
     const int64_t jb = std::min<int64_t>(NB, n - j0);
     if (!la) MOM_TRY(panel(M, j0, jb));
     // row interchanges of this panel on the columns left and right of it
-    const int64_t left = j0, right = n - (j0 + jb);
-    if (left + right > 0) {
+    // (with look-ahead the plan was formed, and the columns left of the
+    // previous block interchanged, on the look-ahead stream)
+    {
       Timed tl(st, s, kPhLaswp);
-      hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)j0, (int)(j0 + jb), (int)n, w.plan);
-      MOM_HIP(hipGetLastError());
-      const int64_t groups = (left + right + kLaswpCols - 1) / kLaswpCols;
-      hipLaunchKernelGGL(laswp_apply_kernel<T>, dim3((unsigned)std::min<int64_t>(groups, 4096)), dim3(256), 0, s, A,
-                         ld, w.plan, 0, (int)j0, (int)(j0 + jb), (int)n);
-      MOM_HIP(hipGetLastError());
+      if (la) {
+        MOM_TRY(laswp(s, j0 - prev, j0, j0 + jb, n));
+      } else if (j0 + (n - (j0 + jb)) > 0) {
+        hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, s, ipiv, (int)j0, (int)(j0 + jb), (int)n,
+                           w.plan);
+        MOM_HIP(hipGetLastError());
+        MOM_TRY(laswp(s, 0, j0, j0 + jb, n));
+      }
       MOM_TRY(tl.done());
     }
+    const int64_t right = n - (j0 + jb);
     if (right <= 0) continue;
     if (inv_u && jb == NB) {
       typename Cx<T>::type *A12 = A + (j0 + jb) * ld + j0;
@@ -2197,6 +2220,11 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
     MOM_HIP(hipStreamWaitEvent(X.s, ctx->fork_ev, 0));
     MOM_TRY(gemm_sub<T>(X, L21, ld, U12, ld, A22, ld, rows, nxt, jb, st));
     MOM_TRY(panel(X, r0, nxt));
+    // the next panel's plan, and its interchanges on the columns left of
+    // this block (the main stream reads this block's columns until the join)
+    hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, X.s, ipiv, (int)r0, (int)(r0 + nxt), (int)n, w.plan);
+    MOM_HIP(hipGetLastError());
+    MOM_TRY(laswp(X.s, 0, j0, j0, j0));
     MOM_HIP(hipEventRecord(ctx->join_ev, X.s));
     MOM_TRY(gemm_sub<T>(M, L21, ld, U12 + nxt * ld, ld, A22 + nxt * ld, ld, rows, right - nxt, jb, st));
     MOM_HIP(hipStreamWaitEvent(s, ctx->join_ev, 0));
