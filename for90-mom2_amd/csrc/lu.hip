@@ -2221,12 +2221,20 @@ static int getrf(mom_ctx *ctx, typename Cx<T>::type *A, int64_t ld, int64_t n, i
     MOM_TRY(gemm_sub<T>(X, L21, ld, U12, ld, A22, ld, rows, nxt, jb, st));
     MOM_TRY(panel(X, r0, nxt));
     // the next panel's plan, and its interchanges on the columns left of
-    // this block (the main stream reads this block's columns until the join)
+    // this block (the main stream reads this block's columns until the join).
+    // The join precedes those interchanges: the main stream's next step
+    // touches only the columns from this block on, so it waits for the
+    // panel and the plan alone.
     hipLaunchKernelGGL(laswp_plan_kernel, dim3(1), dim3(256), 0, X.s, ipiv, (int)r0, (int)(r0 + nxt), (int)n, w.plan);
     MOM_HIP(hipGetLastError());
-    MOM_TRY(laswp(X.s, 0, j0, j0, j0));
     MOM_HIP(hipEventRecord(ctx->join_ev, X.s));
+    MOM_TRY(laswp(X.s, 0, j0, j0, j0));
     MOM_TRY(gemm_sub<T>(M, L21, ld, U12 + nxt * ld, ld, A22 + nxt * ld, ld, rows, right - nxt, jb, st));
+    MOM_HIP(hipStreamWaitEvent(s, ctx->join_ev, 0));
+  }
+  if (la) {
+    // the last interchanges on the left columns
+    MOM_HIP(hipEventRecord(ctx->join_ev, X.s));
     MOM_HIP(hipStreamWaitEvent(s, ctx->join_ev, 0));
   }
   return 0;
