@@ -45,7 +45,7 @@ PEAK_HBM = 8000.0  # GB/s
 # profiles/pair_plane_kernel_N61440_fp64_counters.md.
 FILL_FLOPS_PER_POINT_PAIR = 60
 FILL_ISSUED_FLOPS_PER_POINT_PAIR = 159.8
-ASSEMBLY_TRAFFIC_S5_BYTES = 7.704e11
+ASSEMBLY_TRAFFIC_S5_BYTES = 6.586e11
 
 
 def log(msg):
