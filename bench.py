@@ -572,6 +572,7 @@ def bench_symmetric(args):
     ang = np.array([40.0, 25.0, 30.0]) * DEGTORAD
     gens = ["r4", "mzp"] if ring else ["mxp", "myp", "mzp"]
     nwhole = (4 if ring else 1) * 24 * n * n
+    piv = 1 if args.pivoting == "tournament" else 0
     ctx = Context(0)
     dms = [DeviceMesh(ctx, dm) for dm in prob.domains]
     log("%s cell n=%d: %d faces, %d edges (%d on the planes), group %s; whole structure %d faces, %d unknowns"
@@ -582,6 +583,8 @@ def bench_symmetric(args):
         t0 = time.perf_counter()
         SS = SymmetricSystems(ctx, N, prob.master.bnd, gens, MOM_PREC_MIXED)
         try:
+            for sy in SS.systems:
+                sy.set_pivoting(piv)
             SS.fill(dms, ri, omega)
             t1 = time.perf_counter()
             SS.factor()
@@ -619,7 +622,8 @@ def bench_symmetric(args):
            "value": round(mean, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(mean * 1e3, 2), "higher_is_better": False, "scaling": "strong",
            "vs_baseline": None, "dtype": "fp32-mfma LU + fp64 fill/residual", "data": "synthetic octasphere",
-           "config": {"workload": wl, "group": " ".join(gens), "n_unknowns_closed": nwhole},
+           "config": {"workload": wl, "group": " ".join(gens), "n_unknowns_closed": nwhole,
+                      "complex64_pivoting": args.pivoting},
            "phases_s": {k: round(float(np.mean([r[k] for r in runs])), 4) for k in ("fill", "factor", "solve")},
            "csca_m2": runs[-1]["cs"][0], "cabs_m2": runs[-1]["cs"][1]}
     if not args.no_fp64:
@@ -642,6 +646,7 @@ def bench_symmetric(args):
         t0 = time.perf_counter()
         S = System(ctx, P.nedges, MOM_PREC_MIXED)
         try:
+            S.set_pivoting(piv)
             S.fill(fdms, riw, omega)
             S.factor()
             xf = S.solve_pw(fdms[0], omega, ri[0], ang)[:, 0]
